@@ -1,0 +1,168 @@
+#!/usr/bin/env python3
+"""Throughput of the fused stochastic-LLG step on one MI355X, with its HBM roofline and a CPU baseline.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload bcc64|tri256|sc16] [--precision fp32|fp64]
+
+A step is one stochastic Heun step (noise + field + predictor, then field + corrector) over the whole
+synthetic lattice, inputs resident in HBM.  Default workload: bcc 64^3 conventional cells (524 288
+spins), two shells with per-bond random 3x3 tensors, random per-site SIA, T = 300 K, FP32 -- the
+HBM-roofline case.  Prints one JSON line.
+
+roofline.achieved = algorithmic bytes per step / measured step time, with the algorithmic bytes of a
+Heun step: 6w (read e, write e) + 6w if the SIA is per site + 9w nnb/2 if bond tensors are per bond
+(each unordered bond stored once), w = 4 (FP32) or 8 (FP64) bytes.  The step time comes from HIP
+events recorded on the stream the kernels run on.
+cpu_baseline: the C++/OpenMP restatement of magex's fused CPU Heun (oracle/cpu_llg.cpp), FP64, on a
+bounded sample of the same workload, on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "spin-steps/sec + % HBM roofline (FP32 vs FP64); batched speedup; grad cost/forward"
+HBM_PEAK_GBS = 8000.0
+
+
+def alg_bytes(system, precision):
+    """Compulsory HBM bytes of one Heun step per spin."""
+    w = 4 if precision == "fp32" else 8
+    b = 6 * w
+    A = system.A.reshape(system.ncell, system.uc.nsites, 9)
+    if not (A == A[:1]).all():
+        b += 6 * w
+    if system.bond_tensors is not None:
+        nnb = len(system.stencil) / system.uc.nsites
+        b += 9 * w * nnb / 2
+    return b
+
+
+def workload(name):
+    from magex_amd import configs
+    if name == "bcc64":
+        s, p = configs.bcc64()
+        desc = ("bcc 64^3 conventional (524288 spins), J1=20 J2=10 meV + per-bond random sym/antisym tensors "
+                "(sigma 1%), random SIA, mu=2.2, alpha=0.1, T=300 K, dt=0.1 fs")
+    elif name == "tri256":
+        s, p = configs.tri256()
+        desc = "triangular 256^2 (65536 spins), J=(5,-0.5,0.2) meV, DMI 1.5, K=0.3, B=2 T, alpha=0.05, T=20 K, dt=0.5 fs"
+    elif name == "sc16":
+        s, p = configs.sc16()
+        desc = "sc 16^3 (4096 spins), J=1, K=0.1 meV, B=0.5 T, alpha=0.1, T=0, dt=0.1 fs"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return s, p, desc
+
+
+def gpu_rate(system, p, precision, steps, warmup):
+    import torch
+    from magex_amd.llg import LLGSolver
+    sol = LLGSolver(system, p["alpha"], p["dt"], p["temperature"], p["seed"], precision=precision)
+    e = torch.as_tensor(system.init_spins("random", seed=0), device="cuda",
+                        dtype=torch.float32 if precision == "fp32" else torch.float64).contiguous()
+    sol.step_device(e, warmup)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record()
+    sol.step_device(e, steps)
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = start.elapsed_time(stop)
+    assert torch.isfinite(e).all(), "non-finite spins"
+    return dict(wall_s=wall, event_ms=ev_ms, step_ms=ev_ms / steps)
+
+
+def cpu_baseline(system, p, seconds):
+    """magex's fused Heun restated in C++/OpenMP, FP64, timed on a bounded sample of the workload."""
+    import numpy as np
+    from oracle import cpu
+    src, dst, J = [], [], []
+    for k in range(len(system.stencil)):
+        a, b, blk = system._entry_blocks(k)
+        ok = b >= 0
+        src.append(a[ok])
+        dst.append(b[ok])
+        J.append(np.asarray(blk)[ok])
+    blocks = cpu.per_bond_blocks(system.N, np.concatenate(src), np.concatenate(dst), np.concatenate(J))
+    h = cpu.CpuHeun(system, p["alpha"], p["dt"], p["temperature"], seed=1, blocks=blocks)
+    e = system.init_spins("random", seed=0)
+    h.run(e, 1)
+    t = time.perf_counter()
+    h.run(e, 1)
+    one = time.perf_counter() - t
+    n = max(1, int(seconds / 3 / max(one, 1e-9)))
+    best = float("inf")
+    for _ in range(3):
+        t = time.perf_counter()
+        h.run(e, n)
+        best = min(best, (time.perf_counter() - t) / n)
+    threads = cpu.max_threads()
+    return dict(value=system.N / best, unit="spin-steps/s", cores=threads, kind="port",
+                sample=f"best of 3 x {n} Heun steps of the full {system.N}-spin lattice, FP64, {threads} OpenMP "
+                       f"threads (magex's fused two-pass Heun, per-bond 3x3 blocks, ziggurat noise)")
+
+
+def traffic_from_profiles(workload_name, precision):
+    """HBM bytes per step measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE), if committed."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload_name}_{precision}.txt")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_step:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="bcc64")
+    ap.add_argument("--precision", default="fp32", choices=("fp32", "fp64"))
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (the multi-GPU sharding of this "
+              "path is a design, not a measurement)", file=sys.stderr)
+        sys.exit(2)
+    system, p, desc = workload(a.workload)
+    print(f"workload {a.workload}: {system.N} spins; precision {a.precision}", file=sys.stderr, flush=True)
+    r = gpu_rate(system, p, a.precision, a.steps, a.warmup)
+    other = "fp64" if a.precision == "fp32" else "fp32"
+    r2 = gpu_rate(system, p, other, max(10, a.steps // 5), max(2, a.warmup // 5))
+    N = system.N
+    value = N * a.steps / r["wall_s"]
+    B = alg_bytes(system, a.precision)
+    achieved = B * N / (r["step_ms"] * 1e-3) / 1e9
+    traffic = traffic_from_profiles(a.workload, a.precision)
+    out = {
+        "metric": METRIC, "value": value, "unit": "spin-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": r["wall_s"] * 1e3 / a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if a.precision == "fp32" else "f64",
+        "data": "synthetic (seeded random couplings and spins)",
+        "config": {"workload": f"{a.workload}: {desc}", "spins": N, "precision": a.precision,
+                   "alg_bytes_per_spin_step": B},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "Heun step = k_heun pass 1 + pass 2 (event time per step)"},
+        other: {"spin_steps_per_s": N / (r2["step_ms"] * 1e-3), "ms_per_step": r2["step_ms"],
+                "roofline_frac": alg_bytes(system, other) * N / (r2["step_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+    }
+    if not a.no_cpu_baseline:
+        print("cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(system, p, a.cpu_seconds)
+        out["vs_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,127 @@
+/* mxllg.h: C ABI of the MI355X (gfx950) atomistic LLG hot path.
+ *
+ * One library, libmxllg.so, replaces the kernels magex calls on this path:
+ *   - the numba CPU kernels _fused1 / _fused2 / _field / _gauss3 (magex/fast.py:189-223, 336-390) that
+ *     LLGSolver._fast_step / _fast_field drive (magex/llg.py:153-214);
+ *   - the numba.cuda kernels _k_pred / _k_corr and their host class CudaLLG
+ *     (magex/cuda_backend.py:144-218, 338-377);
+ *   - the NumPy field / energy / observables of SpinSystem (magex/spinmodel.py:454-484, 517-524)
+ *     and the Berg-Luscher charge of magex/texture.py:30-63.
+ *
+ * Conventions (magex/spinmodel.py:3-11, magex/llg.py:1-9): energies in meV, fields in T, time in ps,
+ * E = sum_i e_i.A_i.e_i - sum_{i!=j} e_i.J_ij.e_j - mu_B sum_i mu_i e_i.B, every ordered pair counted.
+ *
+ * Memory: every spin array argument is a DEVICE pointer to N x 3 values (x, y, z per site) in the
+ * model's precision (double for MX_FP64, float for MX_FP32), sites in magex's order
+ * (site = cell_index * nsub + sublattice, cell_index = (c0 * L1 + c1) * L2 + c2; spinmodel.py:197-206).
+ * Model set-up calls take HOST arrays of double and copy them.  The model owns its device buffers;
+ * spin buffers belong to the caller.  All work is enqueued on the caller's stream (a hipStream_t,
+ * passed as void*; NULL = default stream) and nothing synchronises unless stated.
+ *
+ * Errors: every int-returning call returns MX_OK (0) or an MX_E* code; mx_last_error() gives the
+ * message of the last failing call on the calling thread.  A model is not thread-safe: one host
+ * thread drives one model at a time (as magex's solver objects, magex/llg.py:96).
+ */
+#ifndef MXLLG_H
+#define MXLLG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { MX_OK = 0, MX_EINVAL = 1, MX_ENOMEM = 2, MX_EHIP = 3, MX_EUNSUPPORTED = 4 };
+enum { MX_FP64 = 0, MX_FP32 = 1 };
+enum { MX_FIELD_BEFF = 0, MX_FIELD_GRADIENT = 1, MX_FIELD_EXCHANGE = 2 };
+enum { MX_HEUN = 0, MX_DEPONDT = 1 };
+enum { MX_OBS_E = 0, MX_OBS_M = 1, MX_OBS_ORDER = 4, MX_OBS_TORQUE = 7, MX_OBS_LEN = 8 };
+
+typedef struct mx_model mx_model;
+
+/* Library identity. */
+const char* mx_last_error(void);
+int mx_version(void);
+
+/* Model of a periodic (or open) supercell of a unit cell with nsub magnetic sites.
+ * Replaces the array set-up of fast.FastModel (magex/fast.py:630-672) and cuda_backend._Device
+ * (magex/cuda_backend.py:307-335); the 3N x 3N CSR of SpinSystem (spinmodel.py:211-241) is never built:
+ * couplings are a stencil over cells.  L: supercell (cells per axis); pbc: periodic flags. */
+int mx_model_create(int precision, int nsub, const int32_t L[3], const int32_t pbc[3], mx_model** out);
+void mx_model_destroy(mx_model* m);
+int mx_model_info(const mx_model* m, int64_t* n_sites, int32_t* precision, int32_t* nsub);
+
+/* Exchange stencil: nent entries of 6 int32 (s, t, R0, R1, R2, type): the coupling J = table[type] acts
+ * from site (c, s) on site (c + R, t) (spinmodel.py:214-241).  table: ntypes x width doubles, width 1
+ * (J I), 4 (J I + DMI vector D, J^A_ab = eps_abc D_c; fast.py:104-111) or 9 (row-major 3x3).
+ * Entries whose target is the source itself on every cell are rejected: fold them into the on-site
+ * tensor first (spinmodel.py:224-229). */
+int mx_model_set_stencil(mx_model* m, int32_t nent, const int32_t* ent, int32_t ntypes, int32_t width,
+                         const double* table);
+
+/* Per-bond 3x3 tensors on the stencil (no translation symmetry; the per-bond storage of
+ * fast.py:612-614).  Every stencil entry must have a partner (t, s, -R) so that each unordered bond is
+ * stored once and J_ji = J_ij^T.  The stored ("forward") entries are those with s < t, or s == t and
+ * (R0, R1, R2) lexicographically positive, in stencil order; J: nfwd x ncell x 9 doubles, J[f, c] =
+ * row-major tensor of forward entry f from cell c.  Replaces the stencil's type table. */
+int mx_model_set_bond_tensors(mx_model* m, const double* J);
+
+/* On-site tensor A (gradient 2 A e, energy e.A.e; spinmodel.py:462,479).  per_site=0: nsub x 9,
+ * per_site=1: N x 9 (site order as above). */
+int mx_model_set_onsite(mx_model* m, int per_site, const double* A);
+
+/* Cubic E = Kc (a1^2 a2^2 + a2^2 a3^2 + a3^2 a1^2), a = C e, and higher uniaxial
+ * E = -K2 (e.u)^4 - K3 (e.u)^6 (magex/terms.py:35-75, in-kernel as fast.py:47-63).  Per-site arrays
+ * of N doubles; pass NULL to switch a term off. */
+int mx_model_set_cubic(mx_model* m, const double* Kc, const double C[9]);
+int mx_model_set_uniaxial_higher(mx_model* m, const double* K2, const double* K3, const double u[3]);
+
+/* Moments mu (mu_B) and signs of the reference state: per_site=0: nsub values, 1: N values. */
+int mx_model_set_moments(mx_model* m, int per_site, const double* mu, const double* signs);
+
+/* Pinned sites (N bytes, nonzero = fixed; NULL = none). */
+int mx_model_set_pinned(mx_model* m, const uint8_t* pinned);
+
+/* External field B (T), held by value: no upload per step (cuda_backend.py:331-335 uploads it). */
+int mx_model_set_field(mx_model* m, const double B[3]);
+
+/* Dynamics of LLGSolver (llg.py:49-122): dt (ps); damping alpha and temperature T (K) either uniform
+ * (per_site=0: one value) or per site (N values); sigma_i = sqrt(2 alpha_i k_B T_i / (gamma mu_i mu_B dt)).
+ * precession 0/1; integrator MX_HEUN (MX_DEPONDT is reserved). */
+int mx_model_set_dynamics(mx_model* m, double dt, int alpha_per_site, const double* alpha, int T_per_site,
+                          const double* T, int precession, int integrator);
+
+/* nsteps stochastic Heun steps on e (device, N x 3, in place); replaces one _fused1 + _fused2 pair per
+ * step (fast.py:336-390, llg.py:216-227).  Thermal noise: standard normals from a counter-based
+ * generator keyed by (seed, step0 + n, site) -- or, if noise != NULL, read from noise (device,
+ * nsteps x N x 3, same precision) -- times sigma_i, held over both Heun stages. */
+int mx_llg_run(mx_model* m, void* e, int64_t nsteps, uint64_t seed, uint64_t step0, const void* noise,
+               void* stream);
+
+/* The standard normals mx_llg_run draws for (seed, step): out (device, N x 3). */
+int mx_noise(mx_model* m, void* out, uint64_t seed, uint64_t step, void* stream);
+
+/* Field of e: kind MX_FIELD_BEFF (T, SpinSystem.effective_field), MX_FIELD_GRADIENT (meV, dE/de,
+ * SpinSystem.gradient) or MX_FIELD_EXCHANGE (meV, h = sum_j J_ij e_j, SpinSystem.exchange_field);
+ * spinmodel.py:454-473.  out: device N x 3. */
+int mx_field(mx_model* m, const void* e, void* out, int kind, void* stream);
+
+/* Observables of e into out (device, MX_OBS_LEN doubles): [E (meV, SpinSystem.energy), sum e (3),
+ * sum s_i e_i (3, order parameter numerator), max_i |e_i x B_eff,i| (T, LLGSolver.max_torque)];
+ * spinmodel.py:475-484,517-524, llg.py:250-252. */
+int mx_observables(mx_model* m, const void* e, double* out, void* stream);
+
+/* de/dt = -gamma/(1+alpha^2) [e x B + alpha e x (e x B)] for given fields (llg.py:21-28): e, B, out
+ * device N x 3 in the model precision; alpha uniform. */
+int mx_rhs(mx_model* m, const void* e, const void* B, double alpha, int precession, void* out, void* stream);
+
+/* Berg-Luscher topological charge (texture.py:30-63) over ntri triangles tri (device, ntri x 3 int32
+ * site indices, each counter-clockwise in the layer plane); Q = hand / (4 pi) sum 2 atan2(...).
+ * out: device double. */
+int mx_topo_charge(mx_model* m, const void* e, const int32_t* tri, int64_t ntri, double hand, double* out,
+                   void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,563 @@
+// Host side of the C ABI declared in include/mxllg.h: model set-up, device buffers, argument checks and
+// the step loop.  No allocation happens inside mx_llg_run / mx_field / mx_observables: every buffer
+// is made when the model is configured.
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <new>
+#include <stdexcept>
+#include <tuple>
+
+#include "mxllg_internal.h"
+
+namespace mx {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+static void fail(int code, const std::string& msg) { throw Error{code, msg}; }
+
+static void hip_ok(hipError_t e, const char* what) {
+  if (e != hipSuccess) fail(e == hipErrorOutOfMemory ? MX_ENOMEM : MX_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename F>
+static int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const Error& e) {
+    g_err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return MX_ENOMEM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return MX_EHIP;
+  }
+}
+
+static size_t rsize(const Model& m) { return m.prec == MX_FP64 ? sizeof(double) : sizeof(float); }
+
+static void dfree(void*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+// Device copy of n host doubles, converted to the model precision.  perm: reorder a magex-order
+// per-site array of n = N * comps values (site-major) into internal component-major order.
+static void upload(const Model& m, void*& dst, const double* src, size_t n) {
+  dfree(dst);
+  hip_ok(hipMalloc(&dst, n * rsize(m)), "hipMalloc");
+  if (m.prec == MX_FP64) {
+    hip_ok(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+  } else {
+    std::vector<float> tmp(src, src + n);
+    hip_ok(hipMemcpy(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+  }
+}
+
+// magex order site r = c * ns + s -> internal order q = s * ncell + c
+static inline int internal_index(const Model& m, int r) {
+  const int c = r / m.ns, s = r - c * m.ns;
+  return s * m.ncell + c;
+}
+
+// per-site magex-order array (N x comps, site-major) -> internal order, component-major (comps x N)
+static std::vector<double> to_internal_soa(const Model& m, const double* src, int comps) {
+  std::vector<double> out(size_t(m.N) * comps);
+  for (int r = 0; r < m.N; ++r) {
+    const int q = internal_index(m, r);
+    for (int k = 0; k < comps; ++k) out[size_t(k) * m.N + q] = src[size_t(r) * comps + k];
+  }
+  return out;
+}
+
+static void ensure_scratch(Model& m) {
+  if (!m.ep_d) hip_ok(hipMalloc(&m.ep_d, size_t(m.N) * 3 * rsize(m)), "hipMalloc ep");
+  if (!m.k1_d) hip_ok(hipMalloc(&m.k1_d, size_t(m.N) * 3 * rsize(m)), "hipMalloc k1");
+  if (m.ns > 1 && !m.eint_d) hip_ok(hipMalloc(&m.eint_d, size_t(m.N) * 3 * rsize(m)), "hipMalloc e");
+  if (!m.part_d) {
+    const int nblk = m.ns * ((m.ncell + BLOCK - 1) / BLOCK);
+    m.npart = std::max(nblk, (2 * m.N + BLOCK - 1) / BLOCK + 1);
+    hip_ok(hipMalloc(reinterpret_cast<void**>(&m.part_d), size_t(m.npart) * MX_OBS_LEN * sizeof(double)),
+           "hipMalloc partials");
+  }
+}
+
+static void use_device(const Model& m) { hip_ok(hipSetDevice(m.device), "hipSetDevice"); }
+
+static void need_model(const mx_model* m) {
+  if (!m) fail(MX_EINVAL, "null model");
+}
+
+static void recompute_sigma(Model& m);
+
+}  // namespace mx
+
+struct mx_model : mx::Model {};
+
+using namespace mx;
+
+extern "C" {
+
+const char* mx_last_error(void) { return g_err.c_str(); }
+
+int mx_version(void) { return 1; }
+
+int mx_model_create(int precision, int nsub, const int32_t L[3], const int32_t pbc[3], mx_model** out) {
+  return guarded([&]() -> int {
+    if (!out || !L || !pbc) fail(MX_EINVAL, "null argument");
+    *out = nullptr;
+    if (precision != MX_FP64 && precision != MX_FP32) fail(MX_EINVAL, "precision must be MX_FP64 or MX_FP32");
+    if (nsub < 1 || nsub > MAXSUB) fail(MX_EUNSUPPORTED, "1 to " + std::to_string(MAXSUB) + " sites per cell");
+    int64_t n = nsub;
+    for (int a = 0; a < 3; ++a) {
+      if (L[a] < 1 || L[a] > 32767) fail(MX_EINVAL, "supercell extents must lie in [1, 32767]");
+      n *= L[a];
+    }
+    if (n > (int64_t(1) << 30)) fail(MX_EUNSUPPORTED, "more than 2^30 sites");
+    auto* m = new mx_model();
+    m->prec = precision;
+    hip_ok(hipGetDevice(&m->device), "hipGetDevice");
+    m->ns = nsub;
+    for (int a = 0; a < 3; ++a) {
+      m->L[a] = L[a];
+      m->pbc[a] = pbc[a] ? 1 : 0;
+    }
+    m->ncell = L[0] * L[1] * L[2];
+    m->N = int(n);
+    m->sptr.assign(nsub + 1, 0);
+    m->Asub.assign(9 * nsub, 0.0);
+    m->mu_sub.assign(nsub, 2.0);
+    m->sign_sub.assign(nsub, 1.0);
+    m->alpha_h.assign(1, 0.1);
+    m->T_h.assign(1, 0.0);
+    m->table.assign(1, 0.0);
+    m->width = 1;
+    m->ntypes = 1;
+    *out = m;
+    return MX_OK;
+  });
+}
+
+void mx_model_destroy(mx_model* m) {
+  if (!m) return;
+  (void)hipSetDevice(m->device);
+  for (void** p : {&m->bond_d, &m->A_d, &m->Kc_d, &m->K2_d, &m->K3_d, &m->inv_mu_d, &m->sign_d, &m->pinned_d,
+                   &m->alpha_d, &m->sigma_d, &m->ep_d, &m->k1_d, &m->eint_d, &m->out_d})
+    dfree(*p);
+  if (m->part_d) (void)hipFree(m->part_d);
+  delete m;
+}
+
+int mx_model_info(const mx_model* m, int64_t* n_sites, int32_t* precision, int32_t* nsub) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (n_sites) *n_sites = m->N;
+    if (precision) *precision = m->prec;
+    if (nsub) *nsub = m->ns;
+    return MX_OK;
+  });
+}
+
+int mx_model_set_stencil(mx_model* m, int32_t nent, const int32_t* ent, int32_t ntypes, int32_t width,
+                         const double* table) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (nent < 0 || (nent > 0 && !ent)) fail(MX_EINVAL, "bad stencil");
+    if (nent > MAXENT) fail(MX_EUNSUPPORTED, "more than " + std::to_string(MAXENT) + " stencil entries");
+    if (width != 1 && width != 4 && width != 9) fail(MX_EINVAL, "coupling width must be 1, 4 or 9");
+    if (ntypes < 1 || !table) fail(MX_EINVAL, "empty coupling table");
+    if (int64_t(ntypes) * width > MAXTAB)
+      fail(MX_EUNSUPPORTED, "coupling table larger than " + std::to_string(MAXTAB) + " values");
+    // sort entries by source sublattice (stable), validate, reduce offsets
+    std::vector<int> order;
+    for (int s = 0; s < m->ns; ++s)
+      for (int k = 0; k < nent; ++k)
+        if (ent[6 * k] == s) order.push_back(k);
+    if (int(order.size()) != nent) fail(MX_EINVAL, "stencil source sublattice out of range");
+    m->ent.clear();
+    m->sptr.assign(m->ns + 1, 0);
+    for (int k : order) {
+      const int32_t* e = ent + 6 * k;
+      const int s = e[0], t = e[1], typ = e[5];
+      if (t < 0 || t >= m->ns) fail(MX_EINVAL, "stencil target sublattice out of range");
+      if (typ < 0 || typ >= ntypes) fail(MX_EINVAL, "stencil coupling type out of range");
+      int d[3];
+      bool self = (s == t);
+      for (int a = 0; a < 3; ++a) {
+        const int R = e[2 + a];
+        if (m->pbc[a]) {
+          d[a] = ((R % m->L[a]) + m->L[a]) % m->L[a];
+          self = self && d[a] == 0;
+        } else {
+          if (R < -32767 || R > 32767) fail(MX_EINVAL, "stencil offset out of range");
+          d[a] = R;  // |R| >= L on an open axis never finds a target: the kernel's bounds check drops it
+          self = self && R == 0;
+        }
+      }
+      if (self) fail(MX_EINVAL, "stencil entry couples a site to itself: fold it into the on-site tensor");
+      m->ent.push_back(Ent{typ, t, (d[0] & 0xffff) | (d[1] << 16), d[2]});
+      m->sptr[s + 1]++;
+    }
+    for (int s = 0; s < m->ns; ++s) m->sptr[s + 1] += m->sptr[s];
+    m->ent_raw.assign(ent, ent + 6 * size_t(nent));
+    m->ent_order = order;
+    m->nent = nent;
+    m->ntypes = ntypes;
+    m->width = width;
+    m->table.assign(table, table + size_t(ntypes) * width);
+    m->coup = width == 1 ? COUP_TABLE1 : width == 4 ? COUP_TABLE4 : COUP_TABLE9;
+    dfree(m->bond_d);
+    m->nfwd = 0;
+    return MX_OK;
+  });
+}
+
+int mx_model_set_bond_tensors(mx_model* m, const double* J) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (!J) fail(MX_EINVAL, "null tensors");
+    if (m->nent == 0) fail(MX_EINVAL, "set the stencil first");
+    use_device(*m);
+    // forward flag and partner of every entry (in the caller's order)
+    const int n = m->nent;
+    auto raw = [&](int k, int q) { return m->ent_raw[6 * size_t(k) + q]; };
+    std::vector<int> fwd_slot(n, -1), partner(n, -1);
+    int nfwd = 0;
+    for (int k = 0; k < n; ++k) {
+      const int s = raw(k, 0), t = raw(k, 1);
+      const std::tuple<int, int, int> R{raw(k, 2), raw(k, 3), raw(k, 4)};
+      const bool f = s < t || (s == t && R > std::tuple<int, int, int>{0, 0, 0});
+      if (f) fwd_slot[k] = nfwd++;
+      for (int q = 0; q < n; ++q)
+        if (raw(q, 0) == t && raw(q, 1) == s && raw(q, 2) == -raw(k, 2) && raw(q, 3) == -raw(k, 3) &&
+            raw(q, 4) == -raw(k, 4))
+          partner[k] = q;
+      if (partner[k] < 0) fail(MX_EINVAL, "stencil entry without a reverse partner: per-bond storage needs J_ji");
+    }
+    // reorder into component-major [f][q][cell]
+    std::vector<double> buf(size_t(nfwd) * 9 * m->ncell);
+    for (int k = 0; k < n; ++k) {
+      if (fwd_slot[k] < 0) continue;
+      const int f = fwd_slot[k];
+      for (int c = 0; c < m->ncell; ++c)
+        for (int q = 0; q < 9; ++q)
+          buf[(size_t(f) * 9 + q) * m->ncell + c] = J[(size_t(f) * m->ncell + c) * 9 + q];
+    }
+    // entries (internal order) point at a forward slot: their own, or the partner's (back)
+    for (size_t pos = 0; pos < m->ent.size(); ++pos) {
+      const int k = m->ent_order[pos];
+      if (fwd_slot[k] >= 0) {
+        m->ent[pos].slot = fwd_slot[k];
+        m->ent[pos].tb &= 0xffff;
+      } else {
+        m->ent[pos].slot = fwd_slot[partner[k]];
+        m->ent[pos].tb = (m->ent[pos].tb & 0xffff) | (1 << 16);
+      }
+    }
+    upload(*m, m->bond_d, buf.data(), buf.size());
+    m->nfwd = nfwd;
+    m->coup = COUP_BOND;
+    return MX_OK;
+  });
+}
+
+int mx_model_set_onsite(mx_model* m, int per_site, const double* A) {
+  return guarded([&]() -> int {
+    need_model(m);
+    use_device(*m);
+    dfree(m->A_d);
+    m->amode = ONSITE_SUB;
+    m->has_A = 0;
+    std::fill(m->Asub.begin(), m->Asub.end(), 0.0);
+    if (!A) return MX_OK;
+    if (!per_site) {
+      for (int k = 0; k < 9 * m->ns; ++k) {
+        m->Asub[k] = A[k];
+        if (A[k] != 0.0) m->has_A = 1;
+      }
+      return MX_OK;
+    }
+    bool sym = true, any = false;
+    for (int r = 0; r < m->N && sym; ++r) {
+      const double* a = A + 9 * size_t(r);
+      sym = a[1] == a[3] && a[2] == a[6] && a[5] == a[7];
+    }
+    for (size_t k = 0; k < size_t(m->N) * 9 && !any; ++k) any = A[k] != 0.0;
+    if (!any) return MX_OK;
+    m->has_A = 1;
+    if (sym) {
+      std::vector<double> six(size_t(m->N) * 6);
+      for (int r = 0; r < m->N; ++r) {
+        const double* a = A + 9 * size_t(r);
+        const double v[6] = {a[0], a[4], a[8], a[1], a[2], a[5]};
+        for (int k = 0; k < 6; ++k) six[size_t(r) * 6 + k] = v[k];
+      }
+      const std::vector<double> soa = to_internal_soa(*m, six.data(), 6);
+      upload(*m, m->A_d, soa.data(), soa.size());
+      m->amode = ONSITE_SITE6;
+    } else {
+      const std::vector<double> soa = to_internal_soa(*m, A, 9);
+      upload(*m, m->A_d, soa.data(), soa.size());
+      m->amode = ONSITE_SITE9;
+    }
+    return MX_OK;
+  });
+}
+
+int mx_model_set_cubic(mx_model* m, const double* Kc, const double C[9]) {
+  return guarded([&]() -> int {
+    need_model(m);
+    use_device(*m);
+    dfree(m->Kc_d);
+    m->has_c = 0;
+    if (!Kc) return MX_OK;
+    if (!C) fail(MX_EINVAL, "cubic axes missing");
+    const std::vector<double> v = to_internal_soa(*m, Kc, 1);
+    upload(*m, m->Kc_d, v.data(), v.size());
+    for (int q = 0; q < 9; ++q) m->C[q] = C[q];
+    m->has_c = 1;
+    return MX_OK;
+  });
+}
+
+int mx_model_set_uniaxial_higher(mx_model* m, const double* K2, const double* K3, const double u[3]) {
+  return guarded([&]() -> int {
+    need_model(m);
+    use_device(*m);
+    dfree(m->K2_d);
+    dfree(m->K3_d);
+    m->has_u = 0;
+    if (!K2) return MX_OK;
+    if (!u) fail(MX_EINVAL, "uniaxial axis missing");
+    const std::vector<double> zeros(m->N, 0.0);
+    const std::vector<double> a = to_internal_soa(*m, K2, 1);
+    const std::vector<double> b = to_internal_soa(*m, K3 ? K3 : zeros.data(), 1);
+    upload(*m, m->K2_d, a.data(), a.size());
+    upload(*m, m->K3_d, b.data(), b.size());
+    for (int q = 0; q < 3; ++q) m->u[q] = u[q];
+    m->has_u = 1;
+    return MX_OK;
+  });
+}
+
+int mx_model_set_moments(mx_model* m, int per_site, const double* mu, const double* signs) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (!mu) fail(MX_EINVAL, "null moments");
+    use_device(*m);
+    dfree(m->inv_mu_d);
+    dfree(m->sign_d);
+    m->mu_site.clear();
+    m->sign_site_h.clear();
+    if (!per_site) {
+      for (int s = 0; s < m->ns; ++s) {
+        if (!(mu[s] > 0)) fail(MX_EINVAL, "moments must be positive");
+        m->mu_sub[s] = mu[s];
+        m->sign_sub[s] = signs ? signs[s] : 1.0;
+      }
+    } else {
+      m->mu_site.assign(mu, mu + m->N);
+      std::vector<double> inv(m->N);
+      for (int r = 0; r < m->N; ++r) {
+        if (!(mu[r] > 0)) fail(MX_EINVAL, "moments must be positive");
+        inv[r] = 1.0 / (mu[r] * MU_B);
+      }
+      const std::vector<double> a = to_internal_soa(*m, inv.data(), 1);
+      upload(*m, m->inv_mu_d, a.data(), a.size());
+      if (signs) {
+        m->sign_site_h.assign(signs, signs + m->N);
+        const std::vector<double> b = to_internal_soa(*m, signs, 1);
+        upload(*m, m->sign_d, b.data(), b.size());
+      }
+    }
+    if (m->dyn_set) recompute_sigma(*m);
+    return MX_OK;
+  });
+}
+
+int mx_model_set_pinned(mx_model* m, const uint8_t* pinned) {
+  return guarded([&]() -> int {
+    need_model(m);
+    use_device(*m);
+    dfree(m->pinned_d);
+    if (!pinned) return MX_OK;
+    std::vector<uint8_t> v(m->N);
+    bool any = false;
+    for (int r = 0; r < m->N; ++r) {
+      v[internal_index(*m, r)] = pinned[r] ? 1 : 0;
+      any = any || pinned[r];
+    }
+    if (!any) return MX_OK;
+    hip_ok(hipMalloc(&m->pinned_d, m->N), "hipMalloc pinned");
+    hip_ok(hipMemcpy(m->pinned_d, v.data(), m->N, hipMemcpyHostToDevice), "hipMemcpy pinned");
+    return MX_OK;
+  });
+}
+
+int mx_model_set_field(mx_model* m, const double B[3]) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (!B) fail(MX_EINVAL, "null field");
+    for (int a = 0; a < 3; ++a) m->B[a] = B[a];
+    return MX_OK;
+  });
+}
+
+namespace mx {
+// sigma_i = sqrt(2 alpha_i k_B T_i / (gamma mu_i mu_B dt)) (magex/llg.py:111, fast path llg.py:181-182)
+static void recompute_sigma(Model& m) {
+  const bool site_alpha = m.alpha_h.size() > 1, site_T = m.T_h.size() > 1, site_mu = !m.mu_site.empty();
+  dfree(m.alpha_d);
+  dfree(m.sigma_d);
+  m.noisy = 0;
+  for (double t : m.T_h) m.noisy = m.noisy || t > 0;
+  if (site_T) m.noisy = 1;  // a site-temperature map always takes the noisy path (llg.py:175)
+  m.alpha_sub_v.assign(m.ns, 0.0);
+  m.sigma_sub_v.assign(m.ns, 0.0);
+  if (!site_alpha && !site_T && !site_mu) {
+    for (int s = 0; s < m.ns; ++s) {
+      m.alpha_sub_v[s] = m.alpha_h[0];
+      m.sigma_sub_v[s] = std::sqrt(2.0 * m.alpha_h[0] * K_B * m.T_h[0] / (GAMMA * m.mu_sub[s] * MU_B * m.dt));
+    }
+    return;
+  }
+  std::vector<double> al(m.N), sg(m.N);
+  for (int r = 0; r < m.N; ++r) {
+    const double a = site_alpha ? m.alpha_h[r] : m.alpha_h[0];
+    const double t = site_T ? m.T_h[r] : m.T_h[0];
+    const double mu = site_mu ? m.mu_site[r] : m.mu_sub[r % m.ns];
+    al[r] = a;
+    sg[r] = std::sqrt(2.0 * a * K_B * t / (GAMMA * mu * MU_B * m.dt));
+  }
+  const std::vector<double> a2 = to_internal_soa(m, al.data(), 1);
+  const std::vector<double> s2 = to_internal_soa(m, sg.data(), 1);
+  upload(m, m.alpha_d, a2.data(), a2.size());
+  upload(m, m.sigma_d, s2.data(), s2.size());
+}
+}  // namespace mx
+
+int mx_model_set_dynamics(mx_model* m, double dt, int alpha_per_site, const double* alpha, int T_per_site,
+                          const double* T, int precession, int integrator) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (!alpha || !T) fail(MX_EINVAL, "null damping or temperature");
+    if (!(dt > 0)) fail(MX_EINVAL, "dt must be positive");
+    if (integrator != MX_HEUN) fail(MX_EUNSUPPORTED, "only the Heun integrator is implemented");
+    use_device(*m);
+    m->dt = dt;
+    m->alpha_h.assign(alpha, alpha + (alpha_per_site ? m->N : 1));
+    m->T_h.assign(T, T + (T_per_site ? m->N : 1));
+    for (double t : m->T_h)
+      if (t < 0 || std::isnan(t)) fail(MX_EINVAL, "temperatures must be >= 0");
+    m->precession = precession ? 1 : 0;
+    m->integrator = integrator;
+    recompute_sigma(*m);
+    m->dyn_set = 1;
+    ensure_scratch(*m);
+    return MX_OK;
+  });
+}
+
+int mx_llg_run(mx_model* m, void* e, int64_t nsteps, uint64_t seed, uint64_t step0, const void* noise,
+               void* stream) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (!e) fail(MX_EINVAL, "null spins");
+    if (nsteps < 0) fail(MX_EINVAL, "negative step count");
+    if (!m->dyn_set) fail(MX_EINVAL, "set the dynamics first");
+    use_device(*m);
+    ensure_scratch(*m);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    void* E = e;
+    if (m->ns > 1) {
+      int rc = launch_permute(*m, e, m->eint_d, 1, s);
+      if (rc) return rc;
+      E = m->eint_d;
+    }
+    const size_t stride = size_t(m->N) * 3 * rsize(*m);
+    for (int64_t n = 0; n < nsteps; ++n) {
+      const void* nz = noise ? static_cast<const char*>(noise) + size_t(n) * stride : nullptr;
+      int rc = launch_heun_step(*m, E, seed, step0 + uint64_t(n), nz, s);
+      if (rc) return rc;
+    }
+    if (m->ns > 1) return launch_permute(*m, m->eint_d, e, 0, s);
+    return MX_OK;
+  });
+}
+
+int mx_noise(mx_model* m, void* out, uint64_t seed, uint64_t step, void* stream) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (!out) fail(MX_EINVAL, "null output");
+    use_device(*m);
+    return launch_noise(*m, out, seed, step, static_cast<hipStream_t>(stream));
+  });
+}
+
+static const void* internal_spins(mx_model* m, const void* e, hipStream_t s, int& rc) {
+  rc = MX_OK;
+  if (m->ns == 1) return e;
+  rc = launch_permute(*m, e, m->eint_d, 1, s);
+  return m->eint_d;
+}
+
+int mx_field(mx_model* m, const void* e, void* out, int kind, void* stream) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (!e || !out) fail(MX_EINVAL, "null buffer");
+    if (kind < MX_FIELD_BEFF || kind > MX_FIELD_EXCHANGE) fail(MX_EINVAL, "bad field kind");
+    use_device(*m);
+    ensure_scratch(*m);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc;
+    const void* E = internal_spins(m, e, s, rc);
+    if (rc) return rc;
+    return launch_field(*m, E, out, kind, s);
+  });
+}
+
+int mx_observables(mx_model* m, const void* e, double* out, void* stream) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (!e || !out) fail(MX_EINVAL, "null buffer");
+    use_device(*m);
+    ensure_scratch(*m);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc;
+    const void* E = internal_spins(m, e, s, rc);
+    if (rc) return rc;
+    return launch_observables(*m, E, out, s);
+  });
+}
+
+int mx_rhs(mx_model* m, const void* e, const void* B, double alpha, int precession, void* out, void* stream) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (!e || !B || !out) fail(MX_EINVAL, "null buffer");
+    use_device(*m);
+    return launch_rhs(*m, e, B, alpha, precession ? 1 : 0, out, static_cast<hipStream_t>(stream));
+  });
+}
+
+int mx_topo_charge(mx_model* m, const void* e, const int32_t* tri, int64_t ntri, double hand, double* out,
+                   void* stream) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (!e || !out || (ntri > 0 && !tri)) fail(MX_EINVAL, "null buffer");
+    if (ntri < 0) fail(MX_EINVAL, "negative triangle count");
+    use_device(*m);
+    ensure_scratch(*m);
+    return launch_topo(*m, e, tri, ntri, hand, out, static_cast<hipStream_t>(stream));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+// Internal declarations shared by the host API (mxllg_api.hip) and the kernels (mxllg_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "mxllg.h"
+
+namespace mx {
+
+constexpr double MU_B = 5.7883818060e-2;  // meV / T
+constexpr double K_B = 8.617333262e-2;    // meV / K
+constexpr double GAMMA = 0.1760859630;    // rad / (ps T)
+
+constexpr int MAXSUB = 8;   // sublattices per cell
+constexpr int MAXENT = 48;  // stencil entries over all sublattices
+constexpr int MAXTAB = 160; // coupling-table values (ntypes x width) held in the kernel arguments
+constexpr int BLOCK = 256;  // threads per workgroup (4 wavefronts of 64)
+
+enum CoupMode { COUP_TABLE1 = 0, COUP_TABLE4 = 1, COUP_TABLE9 = 2, COUP_BOND = 3 };
+enum OnsiteMode { ONSITE_SUB = 0, ONSITE_SITE6 = 1, ONSITE_SITE9 = 2 };
+
+// One stencil entry, packed into 16 bytes so the whole stencil rides in the kernel arguments
+// (constant, read with scalar loads because every lane of a workgroup shares the sublattice):
+//   x = coupling slot (type-table row, or forward-bond slot), y = target sublattice | back << 16,
+//   z = (d0 & 0xffff) | d1 << 16, w = d2; offsets d are reduced into [0, L) on periodic axes.
+struct Ent {
+  int32_t slot, tb, d01, d2;
+};
+
+// Kernel parameters, passed by value (< 3 KB).  Sites live in sublattice-major order inside the
+// library (internal index = s * ncell + cell), so a workgroup never mixes sublattices.
+template <typename R>
+struct KP {
+  int L0, L1, L2, ncell, ns, N, bps, nblk;  // bps: workgroups per sublattice
+  int pbc0, pbc1, pbc2;
+  int sptr[MAXSUB + 1];
+  Ent ent[MAXENT];
+  R tab[MAXTAB];   // COUP_TABLE*: ntypes x width, read with scalar loads
+  const R* bond;   // COUP_BOND: nfwd x 9 x ncell (component-major per forward slot)
+  int amode, has_A;
+  const R* A;  // ONSITE_SITE6: 6 x N (xx, yy, zz, xy, xz, yz); ONSITE_SITE9: 9 x N (row-major)
+  R Asub[MAXSUB][9];
+  int has_c, has_u;
+  const R* Kc;
+  const R* K2;
+  const R* K3;
+  R C[9];
+  R u[3];
+  const R* inv_mu;  // per-site 1 / (mu mu_B), or null -> inv_mu_sub
+  R inv_mu_sub[MAXSUB];
+  const R* alpha;  // per-site damping, or null -> alpha_sub
+  R alpha_sub[MAXSUB];
+  const R* sigma;  // per-site thermal amplitude (T), or null -> sigma_sub
+  R sigma_sub[MAXSUB];
+  R sign_sub[MAXSUB];
+  const R* sign_site;  // per-site reference signs, or null
+  const uint8_t* pinned;
+  R B[3];
+  R dt;
+  int prec, noisy;
+  uint32_t seed_lo, seed_hi;
+  const R* noise;  // injected standard normals of the current step (N x 3, magex order), or null
+};
+
+// Host-side model.
+struct Model {
+  int prec;  // MX_FP64 / MX_FP32
+  int device;
+  int ns, L[3], pbc[3], ncell, N;
+  // stencil
+  int nent = 0;
+  std::vector<int32_t> ent_raw;  // nent x 6 as given
+  std::vector<int> sptr;
+  std::vector<Ent> ent;          // internal order (sorted by source sublattice)
+  std::vector<int> ent_order;    // ent[pos] came from ent_raw entry ent_order[pos]
+  int coup = COUP_TABLE1;
+  int ntypes = 0, width = 0;
+  std::vector<double> table;
+  void* bond_d = nullptr;
+  int nfwd = 0;
+  // on-site
+  int amode = ONSITE_SUB, has_A = 0;
+  std::vector<double> Asub;  // ns x 9
+  void* A_d = nullptr;
+  int has_c = 0, has_u = 0;
+  void *Kc_d = nullptr, *K2_d = nullptr, *K3_d = nullptr;
+  double C[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, u[3] = {0, 0, 1};
+  // moments and dynamics
+  std::vector<double> mu_sub, sign_sub;
+  std::vector<double> mu_site, sign_site_h;  // empty unless per-site
+  void* inv_mu_d = nullptr;
+  void* sign_d = nullptr;
+  void* pinned_d = nullptr;
+  double B[3] = {0, 0, 0};
+  double dt = 1e-3;
+  int precession = 1, integrator = MX_HEUN;
+  std::vector<double> alpha_h, T_h;  // size 1 (uniform) or N (per site), magex order
+  void* alpha_d = nullptr;
+  void* sigma_d = nullptr;
+  std::vector<double> alpha_sub_v, sigma_sub_v;
+  int noisy = 0;
+  int dyn_set = 0;
+  // scratch
+  void *ep_d = nullptr, *k1_d = nullptr, *eint_d = nullptr, *out_d = nullptr;
+  double* part_d = nullptr;
+  int npart = 0;
+};
+
+// Kernel launchers (mxllg_kernels.hip).  All take internal-order device buffers unless noted.
+template <typename R>
+void fill_kp(const Model& m, KP<R>& p);
+int launch_heun_step(const Model& m, void* e, uint64_t seed, uint64_t step, const void* noise_step, hipStream_t s);
+int launch_field(const Model& m, const void* e_int, void* out_ref, int kind, hipStream_t s);
+int launch_observables(const Model& m, const void* e_int, double* out, hipStream_t s);
+int launch_noise(const Model& m, void* out_ref, uint64_t seed, uint64_t step, hipStream_t s);
+int launch_permute(const Model& m, const void* src, void* dst, int to_internal, hipStream_t s);
+int launch_rhs(const Model& m, const void* e, const void* B, double alpha, int prec, void* out, hipStream_t s);
+int launch_topo(const Model& m, const void* e_ref, const int32_t* tri, int64_t ntri, double hand, double* out,
+                hipStream_t s);
+
+void set_error(const std::string& msg);
+
+}  // namespace mx

@@ -1,0 +1,205 @@
+"""Atomistic stochastic Landau-Lifshitz-Gilbert dynamics on the GPU (drop-in for magex.llg).
+
+    de_i/dt = -gamma/(1+alpha^2) [ e_i x B_i + alpha e_i x (e_i x B_i) ]
+
+B_i = B_eff,i + b_i with the Brown thermal field <b b> = 2 alpha k_B T / (gamma mu_i mu_B dt) per
+component and step, held over both stages of the stochastic Heun scheme, with |e_i| = 1 restored after
+each stage (magex/llg.py:1-9, 216-227).  Units: ps, T, meV, K.
+
+Every step runs in the HIP kernels of libmxllg.so (two fused passes: noise + field + predictor, then
+field + corrector).  The noise is not numpy's stream: it is a counter-based generator keyed by (seed,
+step, site), so a run is bitwise reproducible and resumable from (spins, step count).
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from .constants import GAMMA, K_B, MU_B  # noqa: F401
+from .spinmodel import normalize  # noqa: F401
+
+INTEGRATORS = ("heun", "rk4", "depondt", "sib", "euler")
+DEVICE_INTEGRATORS = ("heun",)
+
+
+def llg_rhs(e, B, alpha, precession=True, precision="fp64"):
+    """de/dt (1/ps) for unit spins e (N,3) in fields B (N,3) (magex/llg.py:21-28), on the device.
+
+    alpha: scalar.  numpy inputs give numpy output; device tensors stay on the device.
+    """
+    from .device import DeviceModel, as_device, like_input
+    from .models import make_model
+    from .spinmodel import SpinSystem, load_results
+    if np.ndim(alpha):
+        raise NotImplementedError("llg_rhs on the device takes a scalar alpha")
+    n = int(np.shape(e)[0])
+    key = (n, precision)
+    dm = _RHS_MODELS.get(key)
+    if dm is None:
+        sys_ = SpinSystem(load_results(make_model("chain", J=[0.0])), (n, 1, 1), (False, False, False))
+        dm = _RHS_MODELS[key] = DeviceModel(sys_, precision)
+    ed, Bd = as_device(e, dm.dtype), as_device(B, dm.dtype)
+    return like_input(dm.rhs(ed, Bd, float(alpha), precession), e)
+
+
+_RHS_MODELS = {}
+
+
+class LLGSolver:
+    """Stochastic Heun LLG of a SpinSystem on the GPU (magex/llg.py:48-283).
+
+    Same constructor as magex's LLGSolver plus ``precision`` ("fp64", magex's arithmetic, or "fp32").
+    backend: "auto", "hip" or "numba" (all mean the device kernels); "numpy" raises: this package
+    has no CPU path.  Only the Heun integrator with the classical thermostat is on the device path
+    (ValueError for an unknown integrator, NotImplementedError for the others, STT / SOT, the quantum
+    thermostat, field_func and extra_field).
+    """
+
+    def __init__(self, system, alpha=0.1, dt=1e-3, temperature=0.0, seed=None, precession=True,
+                 thermostat="classical", integrator="heun", stt=None, sot=None, backend="auto", precision="fp64"):
+        self.sys = system
+        a = getattr(system, "alpha_site", None)
+        self.alpha = np.asarray(a if (a is not None and np.isscalar(alpha)) else alpha, float)
+        if self.alpha.ndim and a is not None and np.isscalar(alpha):
+            self.alpha = np.where(np.isnan(self.alpha), float(alpha), self.alpha)
+        self.dt = float(dt)
+        self.T = float(temperature)
+        self.rng = np.random.default_rng(seed)
+        self.seed = seed
+        self.precession = bool(precession)
+        self.thermostat = thermostat
+        if integrator not in INTEGRATORS:
+            raise ValueError(f"integrator must be one of {INTEGRATORS}")
+        if integrator not in DEVICE_INTEGRATORS:
+            raise NotImplementedError(f"integrator '{integrator}' is not on the device path (heun is)")
+        if thermostat != "classical":
+            raise NotImplementedError("only the classical (white-noise) thermostat is on the device path")
+        if stt or sot:
+            raise NotImplementedError("spin-transfer / spin-orbit torques are not on the device path")
+        if backend not in ("auto", "hip", "numba"):
+            if backend == "numpy":
+                raise RuntimeError("magex_amd has no NumPy path: use backend='auto' (HIP kernels)")
+            raise ValueError(f"unknown backend '{backend}'")
+        self.integrator = integrator
+        self.precision = precision
+        self.time = 0.0
+        self.field_func = None
+        self.extra_field = None
+        self.stt = self.sot = None
+        # the noise key: the seed itself, or 64 bits drawn from fresh entropy
+        self.noise_seed = int(seed) if seed is not None else int(self.rng.integers(0, 2**63))
+        self.step_count = 0
+        self.fast = self.dm = system.device_model(precision)
+
+    # ------------------------------------------------------------------ device plumbing
+    def _sync(self):
+        if self.field_func is not None or self.extra_field is not None:
+            raise NotImplementedError("field_func / extra_field are not on the device path")
+        self.dm = self.sys.device_model(self.precision)
+        self.fast = self.dm
+        from .structure import site_temperature
+        T = site_temperature(self.sys, self.T)
+        self.dm.set_dynamics(self.dt, self.alpha, T, self.precession)
+        return self.dm
+
+    def _to_device(self, e):
+        from .device import as_device
+        return as_device(e, self.dm.dtype)
+
+    @property
+    def sigma(self):
+        """Thermal-field amplitude per site, sqrt(2 alpha k_B T / (gamma mu mu_B dt)) (llg.py:111)."""
+        from .structure import site_temperature
+        T = site_temperature(self.sys, self.T)
+        s = np.sqrt(2.0 * self.alpha * K_B * np.asarray(T, float) / (GAMMA * self.sys.mu * MU_B * self.dt))
+        return np.broadcast_to(s, (self.sys.N,))
+
+    def thermal_field(self):
+        """The thermal field b (N,3) of the next step (T), as the kernels will draw it."""
+        if self.T <= 0 and getattr(self.sys, "site_T", None) is None:
+            return 0.0
+        dm = self._sync()
+        b = self.sigma[:, None] * dm.noise(self.noise_seed, self.step_count).cpu().numpy()
+        if self.sys.pinned is not None:
+            b[self.sys.pinned] = 0.0
+        return b
+
+    def step_device(self, e_dev, nsteps=1, noise=None):
+        """Advance a device tensor in place by nsteps (the CudaLLG.step_device shape,
+        magex/cuda_backend.py:362-369).  noise: injected (nsteps, N, 3) standard normals or None."""
+        dm = self._sync()
+        dm.run(e_dev, nsteps, self.noise_seed, self.step_count, noise)
+        self.step_count += int(nsteps)
+        self.time += nsteps * self.dt
+        return e_dev
+
+    def step(self, e):
+        """One integration step; returns new spins (numpy in -> numpy out, the input is not mutated)."""
+        from .device import like_input
+        self._sync()
+        ed = self._to_device(e)
+        if ed is e:
+            ed = ed.clone()
+        self.step_device(ed, 1)
+        return like_input(ed, e)
+
+    def max_torque(self, e):
+        """max_i |e_i x B_eff,i| (T) (magex/llg.py:250-252)."""
+        dm = self._sync()
+        return float(dm.observables(self._to_device(e))[7])
+
+    def run(self, e, steps, output_every=100, snapshots=0, callback=None, torque_tol=None, log=None):
+        """Integrate steps steps (magex/llg.py:254-283); returns (spins, table, snapshots).
+
+        table columns: t (ps), E/N (meV), <ex>, <ey>, <ez>, order parameter, max torque (T).
+        Between output points the steps run back to back on the device without host round trips.
+        """
+        from .device import like_input
+        dm = self._sync()
+        ed = self._to_device(e)
+        if ed is e:
+            ed = ed.clone()
+        N = self.sys.N
+        rows, snaps = [], []
+        snap_every = max(1, steps // snapshots) if snapshots else 0
+        t0 = time.time()
+        marks = set(range(0, steps + 1, max(1, output_every))) | {steps}
+        if snap_every:
+            marks |= set(range(0, steps + 1, snap_every))
+        if callback:
+            marks = set(range(steps + 1))
+        order = sorted(marks)
+        n = 0
+        for m in order:
+            if m > n:
+                self.step_device(ed, m - n)
+                n = m
+            if n % output_every == 0 or n == steps:
+                o = dm.observables(ed)
+                mvec = o[1:4] / N
+                row = [self.time, o[0] / N, *mvec, float(np.linalg.norm(o[4:7]) / N), float(o[7])]
+                rows.append(row)
+                if log and (n % (output_every * 10) == 0 or n == steps):
+                    log(f"  step {n:8d}  t = {self.time:9.3f} ps  E/N = {row[1]: .6f} meV  "
+                        f"m = ({mvec[0]: .3f},{mvec[1]: .3f},{mvec[2]: .3f})  |torque|max = {row[6]:.2e} T")
+                if torque_tol is not None and self.T == 0 and row[6] < torque_tol and n > 0:
+                    if log:
+                        log(f"  converged: max torque {row[6]:.2e} T < {torque_tol:g} T after {n} steps")
+                    break
+            if snap_every and n % snap_every == 0:
+                snaps.append((self.time, ed.cpu().numpy().astype(float)))
+            if callback:
+                callback(n, like_input(ed, e))
+        if log:
+            log(f"  LLG wall time {time.time() - t0:.1f} s")
+        return like_input(ed, e), np.array(rows), snaps
+
+
+def save_spins(path, system, e):
+    """spins_*.npz + .csv writer (magex/llg.py:292-296)."""
+    e = e.cpu().numpy() if hasattr(e, "cpu") else np.asarray(e)
+    np.savez_compressed(path, spins=e, positions=system.positions, mu=system.mu, sublattice=system.sublattice,
+                        cell=system.uc.cell, supercell=np.array(system.L))
+    np.savetxt(path.replace(".npz", ".csv"), np.column_stack([system.positions, e]), delimiter=",", fmt="%.6f",
+               header="x_A,y_A,z_A,ex,ey,ez", comments="")

@@ -1,0 +1,228 @@
+"""Parity of the HIP path (libmxllg.so through magex_amd) with magex's outputs and the CPU oracle.
+
+Bars: FP64 within 1e-10 relative of magex's CPU Heun (the golden trajectories, and the oracle on the
+same injected noise); the noise generator bit-compatible with its Philox restatement up to float32
+Box-Muller round-off; FP32 within a stated per-spin angle error of FP64 on identical noise.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, golden
+from oracle import magex_ref as R
+from oracle import philox
+from magex_amd import models as M
+from magex_amd import spinmodel as S
+from magex_amd.llg import LLGSolver
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(L, field=(0, 0, 0), pbc=(True, True, True), **kw):
+    d = M.make_model(**kw)
+    return (S.SpinSystem(S.load_results(d), L, pbc, field), R.SpinSystem(R.load_results(d), L, pbc, field))
+
+
+def _rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300))
+
+
+def test_c1_fp64_1000_steps_matches_magex(gpu):
+    """Config 1 (sc 16^3, J=1, K=0.1, B_z=0.5, alpha=0.1, dt=0.1 fs, T=0, 1000 steps) vs magex's Heun."""
+    print("case: c1 sc16 fp64 1000 steps", flush=True)
+    g = golden("c1_sc16.npz")
+    s, _ = _pair((16, 16, 16), (0, 0, 0.5), lattice="sc", J=[1.0], K=0.1, moment=2.0)
+    assert _rel(s.effective_field(g["e0"]), g["B0"]) < 1e-13
+    assert abs(s.energy(g["e0"]) - g["E0"]) <= 1e-12 * abs(g["E0"])
+    sol = LLGSolver(s, alpha=0.1, dt=1e-4, seed=0)
+    e = sol.step(g["e0"])
+    assert _rel(e, g["e1"]) < 1e-13
+    e, tab, _ = sol.run(e, 9, output_every=9)
+    assert _rel(e, g["e10"]) < 1e-12
+    e, tab, _ = sol.run(e, 990, output_every=330)
+    assert _rel(e, g["e1000"]) < 1e-10
+    assert abs(tab[-1, 1] * s.N - g["E1000"]) < 1e-10 * abs(g["E1000"])
+
+
+def test_tri_field_energy_and_injected_noise_match_magex(gpu):
+    """Triangular 3 shells + DMI (width-4 table): field, gradient, exchange, energy, 10 noisy steps."""
+    print("case: tri12 golden", flush=True)
+    g = golden("tri12.npz")
+    s, o = _pair((12, 12, 1), (0, 0, 2.0), lattice="triangular", J=[5.0, -0.5, 0.2], D=1.5, K=0.3)
+    e0 = g["e0"]
+    assert _rel(s.exchange_field(e0), g["H0"]) < 1e-13
+    assert _rel(s.gradient(e0), g["G0"]) < 1e-13
+    assert _rel(s.effective_field(e0), g["B0"]) < 1e-13
+    assert abs(s.energy(e0) - g["E0"]) < 1e-12 * abs(g["E0"])
+    sol = LLGSolver(s, alpha=0.05, dt=5e-4, temperature=20.0, seed=1)
+    ed = gpu.as_tensor(e0, device="cuda")
+    sol.step_device(ed, 10, noise=gpu.as_tensor(g["noise"], device="cuda"))
+    e = ed.cpu().numpy()
+    assert _rel(e, g["e10"]) < 1e-12
+    assert abs(s.order_parameter(e) - g["order10"]) < 1e-12
+
+
+def test_square_dmi_matches_magex(gpu):
+    print("case: square8 golden", flush=True)
+    g = golden("square8.npz")
+    s, _ = _pair((8, 8, 1), lattice="square", J=[5.0, -0.5], D=1.5, K=0.3)
+    assert _rel(s.effective_field(g["e0"]), g["B0"]) < 1e-13
+    e, _, _ = LLGSolver(s, alpha=0.3, dt=1e-3).run(g["e0"], 50, output_every=50)
+    assert _rel(e, g["e50"]) < 1e-11
+
+
+def _bcc_perbond():
+    import scipy.sparse as sp
+    g = golden("bcc_perbond.npz")
+    d = json.load(open(os.path.join(GOLDEN, "bcc_conventional_model.json")))
+    s = S.SpinSystem(S.load_results(d), (4, 4, 4))
+    src, dst, blocks = g["src"], g["dst"], g["blocks"]
+    rows = (3 * src[:, None, None] + np.arange(3)[None, :, None]).repeat(3, 2).ravel()
+    cols = (3 * dst[:, None, None] + np.arange(3)[None, None, :]).repeat(3, 1).ravel()
+    s.Jsp = sp.csr_matrix((blocks.ravel(), (rows, cols)), shape=(3 * s.N, 3 * s.N))
+    s.A = g["A"]
+    return s, g
+
+
+def test_bcc_perbond_matches_magex(gpu):
+    """Two sublattices, per-bond random tensors stored once per bond, per-site symmetric SIA, T=300 K."""
+    print("case: bcc per-bond golden", flush=True)
+    s, g = _bcc_perbond()
+    assert _rel(s.effective_field(g["e0"]), g["B0"]) < 1e-13
+    assert abs(s.energy(g["e0"]) - g["E0"]) < 1e-12 * abs(g["E0"])
+    sol = LLGSolver(s, alpha=0.1, dt=1e-4, temperature=300.0, seed=3)
+    ed = gpu.as_tensor(g["e0"], device="cuda")
+    sol.step_device(ed, 10, noise=gpu.as_tensor(g["noise"], device="cuda"))
+    assert _rel(ed.cpu().numpy(), g["e10"]) < 1e-12
+
+
+def test_device_noise_is_the_philox_stream(gpu):
+    print("case: noise vs philox", flush=True)
+    s, _ = _pair((64, 64, 1), lattice="square", J=[1.0])
+    dm = s.device_model("fp64")
+    for seed, step in ((0, 0), (12345, 7), (2**40 + 3, 2**33 + 5)):
+        dev = dm.noise(seed, step).cpu().numpy()
+        ref = philox.normals(seed, step, s.N)
+        assert np.abs(dev - ref).max() < 2e-5
+    x = np.concatenate([dm.noise(9, k).cpu().numpy().ravel() for k in range(40)])
+    assert abs(x.mean()) < 0.005 and abs(x.var() - 1) < 0.006
+    assert abs((x**4).mean() / x.var() ** 2 - 3) < 0.03
+    f32 = s.device_model("fp32").noise(12345, 7).cpu().numpy()
+    assert np.array_equal(f32.astype(np.float64), dm.noise(12345, 7).cpu().numpy())
+
+
+def test_thermal_run_matches_oracle_on_device_noise(gpu):
+    """Internal generator at T > 0: 40 steps on a 32^2 triangular DMI lattice equal the oracle fed the
+    same normals (read back with mx_noise)."""
+    print("case: tri32 thermal vs oracle", flush=True)
+    s, o = _pair((32, 32, 1), (0, 0, 2.0), lattice="triangular", J=[5.0, -0.5, 0.2], D=1.5, K=0.3)
+    e0 = s.init_spins("random", seed=0)
+    sol = LLGSolver(s, alpha=0.05, dt=5e-4, temperature=20.0, seed=77)
+    e, _, _ = sol.run(e0, 40, output_every=40)
+    dm = s.device_model("fp64")
+    normals = [dm.noise(77, n).cpu().numpy() for n in range(40)]
+    ref = R.run_heun(o, e0.copy(), 40, 0.05, 5e-4, T=20.0, normals=lambda n: normals[n])
+    assert _rel(e, ref) < 1e-11
+
+
+@pytest.mark.parametrize("kw,L,pbc", [
+    (dict(lattice="honeycomb", J=[1.0, 0.1], D=0.1, K=0.05), (6, 5, 1), (True, True, True)),
+    (dict(lattice="kagome", J=[1.0], K=0.05, K_axis=(1, 0, 1)), (5, 4, 1), (True, True, True)),
+    (dict(lattice="sc", J=[-1.0], order="afm"), (3, 4, 3), (True, True, True)),
+    (dict(lattice="triangular", J=[1.0, 0.2], D=0.3), (7, 6, 1), (False, True, False)),
+    (dict(lattice="sc", J=[1.0, 0.3], K=0.1), (2, 3, 4), (True, True, False)),
+    (dict(lattice="bcc_conventional", J=[20.0, 10.0], moment=2.2), (3, 4, 5), (True, True, True)),
+    (dict(lattice="sc", J=[1.0], Jani=[[0.1, 0.02, 0], [0.02, -0.05, 0], [0, 0, -0.05]], D=0.1), (4, 4, 4),
+     (True, True, True)),
+])
+def test_lattices_match_oracle(gpu, kw, L, pbc):
+    """Several sublattices, open axes, self-image folding, width-9 tables: field, observables, 30 steps."""
+    print(f"case: lattice {kw['lattice']} {L} {pbc}", flush=True)
+    s, o = _pair(L, (0.1, -0.2, 0.4), pbc, **kw)
+    e0 = s.init_spins("random", seed=4)
+    assert _rel(s.effective_field(e0), o.effective_field(e0)) < 1e-13
+    assert abs(s.energy(e0) - o.energy(e0)) < 1e-12 * max(abs(o.energy(e0)), 1.0)
+    assert abs(s.order_parameter(e0) - o.order_parameter(e0)) < 1e-13
+    sol = LLGSolver(s, alpha=0.2, dt=1e-3)
+    mt = sol.max_torque(e0)
+    ref_t = np.max(np.linalg.norm(np.cross(e0, o.effective_field(e0)), axis=1))
+    assert abs(mt - ref_t) < 1e-12 * ref_t
+    e, _, _ = sol.run(e0, 30, output_every=10)
+    assert _rel(e, R.run_heun(o, e0.copy(), 30, 0.2, 1e-3)) < 1e-11
+
+
+def test_pinned_and_site_dependent_inputs(gpu):
+    """Pinned spins stay fixed; per-site alpha, moments and temperatures enter sigma as magex's."""
+    print("case: pinned + per-site", flush=True)
+    s, o = _pair((6, 6, 2), (0, 0, 1.0), lattice="sc", J=[1.0], K=0.05)
+    rng = np.random.default_rng(5)
+    pinned = rng.random(s.N) < 0.2
+    mu = 1.5 + rng.random(s.N)
+    alpha = 0.05 + 0.2 * rng.random(s.N)
+    Tsite = 50 * rng.random(s.N)
+    for sys_ in (s, o):
+        sys_.pinned = pinned
+        sys_.mu = mu.copy()
+        sys_.site_T = Tsite
+    s.touch()
+    e0 = s.init_spins("random", seed=6)
+    sol = LLGSolver(s, alpha=alpha, dt=5e-4, temperature=10.0, seed=11)
+    e, _, _ = sol.run(e0, 20, output_every=20)
+    dm = s.device_model("fp64")
+    normals = [dm.noise(11, n).cpu().numpy() for n in range(20)]
+    ref = R.run_heun(o, e0.copy(), 20, alpha, 5e-4, T=10.0, normals=lambda n: normals[n])
+    assert _rel(e, ref) < 1e-11
+    assert np.array_equal(e[pinned], e0[pinned])
+    assert _rel(s.gradient(e0), o.gradient(e0)) < 1e-13
+
+
+def test_fp32_angle_error_against_fp64(gpu):
+    """FP32 vs FP64 on identical noise (tri 64^2, 3 shells + DMI, T = 20 K, 500 steps): mean per-spin
+    angle below 1e-5 rad, max below 1e-3 rad."""
+    print("case: fp32 vs fp64 angle", flush=True)
+    s, _ = _pair((64, 64, 1), (0, 0, 2.0), lattice="triangular", J=[5.0, -0.5, 0.2], D=1.5, K=0.3)
+    e0 = s.init_spins("random", seed=0)
+    a = LLGSolver(s, alpha=0.05, dt=5e-4, temperature=20.0, seed=3).run(e0, 500, output_every=500)[0]
+    b = LLGSolver(s, alpha=0.05, dt=5e-4, temperature=20.0, seed=3, precision="fp32").run(
+        e0.astype(np.float32), 500, output_every=500)[0].astype(np.float64)
+    b = R.normalize(b)
+    ang = np.arctan2(np.linalg.norm(np.cross(a, b), axis=1), np.einsum("ij,ij->i", a, b))
+    print(f"fp32 angle error: mean {ang.mean():.3e} max {ang.max():.3e}")
+    assert ang.mean() < 1e-5 and ang.max() < 1e-3
+
+
+def test_topological_charge_matches_magex(gpu):
+    print("case: topological charge", flush=True)
+    from magex_amd.texture import topological_charge
+    g = golden("charge.npz")
+    s, _ = _pair((30, 30, 1), lattice="triangular", J=[1.0])
+    for k in ("tri_sky", "tri_rand"):
+        assert abs(topological_charge(s, g[k + "_e"]) - g[k + "_Q"]) < 1e-9
+    s2, _ = _pair((20, 20, 1), lattice="square", J=[1.0])
+    assert abs(topological_charge(s2, g["sq_sky_e"]) - g["sq_sky_Q"]) < 1e-9
+    assert abs(topological_charge(s, np.tile([0, 0, 1.0], (s.N, 1)))) < 1e-12
+
+
+def test_larmor_on_device(gpu):
+    """Single free spin: Larmor phase and damped polar angle against the closed form."""
+    print("case: larmor", flush=True)
+    s, _ = _pair((1, 1, 1), (0, 0, 2.0), (False, False, False), lattice="sc", J=[0.0])
+    th0, dt, n = 1.2, 1e-4, 5000
+    for a in (0.0, 0.3):
+        e = np.array([[np.sin(th0), 0, np.cos(th0)]])
+        e, _, _ = LLGSolver(s, alpha=a, dt=dt).run(e, n, output_every=n)
+        w = R.GAMMA * 2.0 / (1 + a * a)
+        th = 2 * np.arctan(np.tan(th0 / 2) * np.exp(-a * w * n * dt))
+        assert abs(np.arccos(e[0, 2]) - th) < 1e-7
+        assert abs(np.arctan2(e[0, 1], e[0, 0]) - w * n * dt) < 1e-7
+
+
+def test_llg_rhs_on_device(gpu):
+    from magex_amd.llg import llg_rhs
+    rng = np.random.default_rng(1)
+    e = R.normalize(rng.normal(size=(100, 3)))
+    B = rng.normal(size=(100, 3))
+    for prec in (True, False):
+        assert _rel(llg_rhs(e, B, 0.3, prec), R.llg_rhs(e, B, 0.3, prec)) < 1e-14

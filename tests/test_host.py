@@ -1,0 +1,149 @@
+"""Host-side logic of magex_amd (no GPU): model generator, results reader, stencil assembly, per-bond
+storage, coupling-table compaction, triangulation, and the C ABI library's exports."""
+import ctypes
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conftest import GOLDEN, ROOT, golden
+from oracle import magex_ref as R
+from magex_amd import models as M
+from magex_amd import spinmodel as S
+
+
+def test_make_model_matches_magex():
+    d = json.load(open(os.path.join(GOLDEN, "make_model_cases.json")))
+    for name, kw in d["kwargs"].items():
+        mine, ref = M.make_model(**kw), d["models"][name]
+        assert [(p["i"], p["j"], p["R"], p["nu"]) for p in mine["pairs"]] == \
+               [(q["i"], q["j"], q["R"], q["nu"]) for q in ref["pairs"]], name
+        assert np.allclose([p["J"] for p in mine["pairs"]], [q["J"] for q in ref["pairs"]], rtol=0, atol=1e-15)
+        for k in ("cell", "positions", "moments", "A", "mesh", "refs"):
+            assert np.allclose(mine[k], ref[k], rtol=0, atol=1e-15), (name, k)
+
+
+def test_load_results_matches_magex(tmp_path):
+    g = golden("load_results_tri.npz")
+    p = str(tmp_path / "m.json")
+    M.write_model(p, lattice="triangular", J=[5.0, -0.5, 0.2], D=1.5, K=0.3)
+    uc = S.load_results(p)
+    assert np.array_equal([c[0] for c in uc.couplings], g["i"])
+    assert np.array_equal([c[2] for c in uc.couplings], g["R"])
+    assert np.allclose([c[3] for c in uc.couplings], g["J"], rtol=0, atol=1e-15)
+    for k in ("A", "moments", "signs", "ref_dirs"):
+        assert np.allclose(getattr(uc, k), g[k])
+    assert len(S.load_results(p, rcut=3.15, jtol=0.3).couplings) == int(g["n_filtered"])
+
+
+CASES = [
+    (dict(lattice="sc", J=[1.0], K=0.1), (4, 4, 4), (True,) * 3),
+    (dict(lattice="sc", J=[1.0, 0.3]), (2, 3, 1), (True,) * 3),            # self images fold into A
+    (dict(lattice="triangular", J=[5.0, -0.5, 0.2], D=1.5, K=0.3), (6, 5, 1), (True,) * 3),
+    (dict(lattice="triangular", J=[1.0, 0.2], D=0.3), (5, 4, 1), (False, True, False)),  # open axis
+    (dict(lattice="honeycomb", J=[1.0, 0.1], D=0.1), (4, 3, 1), (True,) * 3),
+    (dict(lattice="kagome", J=[1.0], K=0.05, K_axis=(1, 0, 1)), (3, 3, 1), (True,) * 3),
+    (dict(lattice="sc", J=[-1.0], order="afm"), (3, 3, 3), (True,) * 3),
+    (dict(lattice="bcc_conventional", J=[20.0, 10.0], moment=2.2), (3, 3, 3), (True,) * 3),
+    (dict(lattice="sc", J=[1.0], Jani=[[0.1, 0.02, 0], [0.02, -0.05, 0], [0, 0, -0.05]]), (3, 3, 3), (True,) * 3),
+]
+
+
+@pytest.mark.parametrize("kw,L,pbc", CASES)
+def test_stencil_assembly_equals_csr(kw, L, pbc):
+    """Summing the stencil over cells gives magex's 3N x 3N matrix and on-site tensors exactly."""
+    d = M.make_model(**kw)
+    s = S.SpinSystem(S.load_results(d), L, pbc)
+    o = R.SpinSystem(R.load_results(d), L, pbc)
+    assert abs(s.Jsp - o.Jsp).max() < 1e-14
+    assert np.abs(s.A - o.A).max() < 1e-15
+    assert np.array_equal(s.positions, o.positions) and np.array_equal(s.mu, o.mu)
+
+
+def test_bond_tensors_roundtrip_through_csr():
+    """Golden per-bond bcc: assigning the CSR stores each bond once; reading it back is exact."""
+    g = golden("bcc_perbond.npz")
+    d = json.load(open(os.path.join(GOLDEN, "bcc_conventional_model.json")))
+    s = S.SpinSystem(S.load_results(d), (4, 4, 4))
+    src, dst, blocks = g["src"], g["dst"], g["blocks"]
+    rows = (3 * src[:, None, None] + np.arange(3)[None, :, None]).repeat(3, 2).ravel()
+    cols = (3 * dst[:, None, None] + np.arange(3)[None, None, :]).repeat(3, 1).ravel()
+    Jsp = sp.csr_matrix((blocks.ravel(), (rows, cols)), shape=(3 * s.N, 3 * s.N))
+    s.Jsp = Jsp
+    assert s.bond_tensors.shape == (len(s.forward_entries()), s.ncell, 3, 3)
+    s._Jsp = None
+    assert abs(s.Jsp - Jsp).max() == 0.0
+
+
+def test_coupling_table_compaction():
+    from magex_amd.device import coupling_table
+    iso = [np.eye(3) * 2.0, np.eye(3) * 2.0, np.eye(3) * -1.0]
+    t, ty = coupling_table(iso)
+    assert t.shape == (2, 1) and list(ty) == [0, 0, 1]
+    dm = [np.eye(3) + M.skew([0.1, 0.2, 0.3]), np.eye(3)]
+    t, _ = coupling_table(dm)
+    assert t.shape == (2, 4) and np.allclose(t[0], [1.0, 0.1, 0.2, 0.3])
+    t, _ = coupling_table([np.diag([1.0, 2.0, 3.0])])
+    assert t.shape == (1, 9)
+
+
+@pytest.mark.parametrize("lat,L", [("triangular", (8, 7, 1)), ("square", (6, 6, 1)), ("honeycomb", (4, 4, 1))])
+def test_triangulation_matches_oracle_charge(lat, L):
+    """The cached CCW triangle list reproduces the oracle's Berg-Luscher sum for random spins."""
+    from magex_amd.texture import triangles
+    d = M.make_model(lattice=lat, J=[1.0])
+    s = S.SpinSystem(S.load_results(d), L)
+    e = R.normalize(np.random.default_rng(2).normal(size=(s.N, 3)))
+    tri = triangles(s)
+    s1, s2, s3 = e[tri[:, 0]], e[tri[:, 1]], e[tri[:, 2]]
+    num = np.einsum("ij,ij->i", s1, np.cross(s2, s3))
+    den = 1 + np.einsum("ij,ij->i", s1, s2) + np.einsum("ij,ij->i", s2, s3) + np.einsum("ij,ij->i", s3, s1)
+    Q = (2 * np.arctan2(num, den)).sum() / (4 * np.pi)
+    assert abs(Q - R.topological_charge(s.positions, s.uc.cell, s.L, s.pbc, e)) < 1e-10
+
+
+# ------------------------------------------------------------------ the C ABI library
+def _declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "mxllg.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(mx_\w+)\(", txt, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from magex_amd import _lib
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    names = _declared_symbols()
+    assert len(names) >= 18
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(_lib.SIGNATURES), "ctypes signatures and header disagree"
+
+
+def test_library_argument_errors_without_gpu():
+    """Argument checks that fire before any device call map to ValueError / NotImplementedError."""
+    from magex_amd import _lib
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    Lv = np.array([4, 4, 4], np.int32)
+    pb = np.array([1, 1, 1], np.int32)
+    with pytest.raises(ValueError, match="precision"):
+        _lib.check(L.mx_model_create(7, 1, _lib.hptr(Lv), _lib.hptr(pb), ctypes.byref(h)))
+    with pytest.raises(NotImplementedError, match="sites per cell"):
+        _lib.check(L.mx_model_create(0, 99, _lib.hptr(Lv), _lib.hptr(pb), ctypes.byref(h)))
+    with pytest.raises(ValueError, match="null model"):
+        _lib.check(L.mx_model_set_field(None, _lib.hptr(np.zeros(3))))
+
+
+def test_product_has_no_cpu_path():
+    """magex_amd never imports the oracle and refuses the NumPy backend."""
+    for dirpath, _, files in os.walk(os.path.join(ROOT, "magex_amd")):
+        for f in files:
+            if f.endswith(".py"):
+                txt = open(os.path.join(dirpath, f)).read()
+                assert "oracle" not in re.sub(r"#.*|\"\"\"[\s\S]*?\"\"\"", "", txt), f
+    from magex_amd.llg import LLGSolver
+    s = S.SpinSystem(S.load_results(M.make_model("sc", J=[1.0])), (2, 2, 2))
+    with pytest.raises((RuntimeError, ValueError)):
+        LLGSolver(s, backend="numpy")

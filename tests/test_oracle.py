@@ -1,0 +1,210 @@
+"""Pins the CPU oracle (oracle/) before anything is compared with it.
+
+Against: magex's own outputs (tests/golden, written by tests/golden/make_golden.py from magex's NumPy
+path), the Random123 known-answer vectors of Philox4x32-10, and closed forms (Larmor precession,
+the damped single-spin solution, finite-difference gradients, energy conservation at alpha = 0).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conftest import GOLDEN, golden
+from oracle import magex_ref as R
+from oracle import philox
+
+
+def _sys(L, field=(0, 0, 0), pbc=(True, True, True), **kw):
+    return R.SpinSystem(R.load_results(R.make_model(**kw)), L, pbc, field)
+
+
+# ------------------------------------------------------------------ magex's own vectors
+def test_make_model_matches_magex():
+    d = json.load(open(os.path.join(GOLDEN, "make_model_cases.json")))
+    for name, kw in d["kwargs"].items():
+        mine, ref = R.make_model(**kw), d["models"][name]
+        assert len(mine["pairs"]) == len(ref["pairs"]), name
+        for p, q in zip(mine["pairs"], ref["pairs"]):
+            assert (p["i"], p["j"], p["R"], p["nu"]) == (q["i"], q["j"], q["R"], q["nu"]), name
+            assert np.allclose(p["J"], q["J"], rtol=0, atol=1e-15), name
+        for k in ("cell", "positions", "moments", "A"):
+            assert np.allclose(mine[k], ref[k], rtol=0, atol=1e-15), (name, k)
+
+
+def test_load_results_matches_magex():
+    g = golden("load_results_tri.npz")
+    uc = R.load_results(R.make_model(lattice="triangular", J=[5.0, -0.5, 0.2], D=1.5, K=0.3))
+    assert np.array_equal([c[0] for c in uc.couplings], g["i"])
+    assert np.array_equal([c[1] for c in uc.couplings], g["j"])
+    assert np.array_equal([c[2] for c in uc.couplings], g["R"])
+    assert np.allclose([c[3] for c in uc.couplings], g["J"], rtol=0, atol=1e-15)
+    assert np.allclose(uc.A, g["A"], rtol=0, atol=1e-15)
+    uc2 = R.load_results(R.make_model(lattice="triangular", J=[5.0, -0.5, 0.2], D=1.5, K=0.3), rcut=3.15, jtol=0.3)
+    assert len(uc2.couplings) == int(g["n_filtered"])
+
+
+def test_c1_trajectory_matches_magex():
+    """sc 16^3, T = 0: field, energy and the Heun trajectory after 1, 10 steps (1000 in the next test)."""
+    g = golden("c1_sc16.npz")
+    s = _sys((16, 16, 16), (0, 0, 0.5), lattice="sc", J=[1.0], K=0.1, moment=2.0)
+    e0 = s.init_random(0)
+    assert np.array_equal(e0, g["e0"])
+    assert np.abs(s.effective_field(e0) - g["B0"]).max() < 1e-13
+    assert abs(s.energy(e0) - g["E0"]) < 1e-10 * abs(g["E0"])
+    e = R.heun_step(s, e0, 0.1, 1e-4)
+    assert np.abs(e - g["e1"]).max() < 1e-14
+    e = R.run_heun(s, e, 9, 0.1, 1e-4)
+    assert np.abs(e - g["e10"]).max() < 1e-14
+
+
+def test_c1_1000_steps_c_baseline_matches_magex():
+    from oracle import cpu
+    g = golden("c1_sc16.npz")
+    s = _sys((16, 16, 16), (0, 0, 0.5), lattice="sc", J=[1.0], K=0.1, moment=2.0)
+    e = g["e0"].copy()
+    cpu.CpuHeun(s, 0.1, 1e-4).run(e, 1000)
+    assert np.abs(e - g["e1000"]).max() < 1e-12
+
+
+def test_tri_noise_injected_matches_magex():
+    g = golden("tri12.npz")
+    s = _sys((12, 12, 1), (0, 0, 2.0), lattice="triangular", J=[5.0, -0.5, 0.2], D=1.5, K=0.3)
+    e0 = g["e0"]
+    assert np.abs(s.exchange_field(e0) - g["H0"]).max() < 1e-13
+    assert np.abs(s.gradient(e0) - g["G0"]).max() < 1e-12
+    assert abs(s.energy(e0) - g["E0"]) < 1e-11
+    e = R.run_heun(s, e0.copy(), 10, 0.05, 5e-4, T=20.0, normals=lambda n: g["noise"][n])
+    assert np.abs(e - g["e10"]).max() < 1e-13
+    assert abs(s.order_parameter(e) - g["order10"]) < 1e-13
+
+
+def test_square_matches_magex():
+    g = golden("square8.npz")
+    s = _sys((8, 8, 1), lattice="square", J=[5.0, -0.5], D=1.5, K=0.3)
+    assert np.abs(s.effective_field(g["e0"]) - g["B0"]).max() < 1e-12
+    e = R.run_heun(s, g["e0"].copy(), 50, 0.3, 1e-3)
+    assert np.abs(e - g["e50"]).max() < 1e-13
+
+
+def bcc_perbond_system():
+    """The golden per-bond bcc system rebuilt in the oracle: 2-site cubic cell 4^3, per-bond tensors
+    from the fixture, per-site SIA."""
+    g = golden("bcc_perbond.npz")
+    d = json.load(open(os.path.join(GOLDEN, "bcc_conventional_model.json")))
+    s = R.SpinSystem(R.load_results(d), (4, 4, 4))
+    src, dst, blocks = g["src"], g["dst"], g["blocks"]
+    rows = (3 * src[:, None, None] + np.arange(3)[None, :, None]).repeat(3, 2).ravel()
+    cols = (3 * dst[:, None, None] + np.arange(3)[None, None, :]).repeat(3, 1).ravel()
+    s.Jsp = sp.csr_matrix((blocks.ravel(), (rows, cols)), shape=s.Jsp.shape)
+    s.A = g["A"]
+    return s, g
+
+
+def test_bcc_perbond_matches_magex():
+    s, g = bcc_perbond_system()
+    assert np.abs(s.effective_field(g["e0"]) - g["B0"]).max() < 1e-12
+    assert abs(s.energy(g["e0"]) - g["E0"]) < 1e-10
+    e = R.run_heun(s, g["e0"].copy(), 10, 0.1, 1e-4, T=300.0, normals=lambda n: g["noise"][n])
+    assert np.abs(e - g["e10"]).max() < 1e-13
+
+
+def test_charge_matches_magex():
+    g = golden("charge.npz")
+    uc = R.load_results(R.make_model(lattice="triangular", J=[1.0]))
+    s = R.SpinSystem(uc, (30, 30, 1))
+    for k in ("tri_sky", "tri_rand"):
+        Q = R.topological_charge(s.positions, uc.cell, s.L, s.pbc, g[k + "_e"])
+        assert abs(Q - g[k + "_Q"]) < 1e-9
+    uc2 = R.load_results(R.make_model(lattice="square", J=[1.0]))
+    s2 = R.SpinSystem(uc2, (20, 20, 1))
+    assert abs(R.topological_charge(s2.positions, uc2.cell, s2.L, s2.pbc, g["sq_sky_e"]) - g["sq_sky_Q"]) < 1e-9
+
+
+# ------------------------------------------------------------------ closed forms
+def test_larmor_and_damped_spin_closed_form():
+    """Single spin in B z: phi(t) = gamma B t / (1 + a^2) (counter-clockwise about B, as de/dt = -g e x B),
+    tan(theta/2) = tan(theta0/2) exp(-a gamma B t / (1 + a^2))."""
+    s = _sys((1, 1, 1), (0, 0, 2.0), (False, False, False), lattice="sc", J=[0.0])
+    for a in (0.0, 0.3):
+        th0 = 1.2
+        e = np.array([[np.sin(th0), 0, np.cos(th0)]])
+        dt, n = 1e-4, 5000
+        e = R.run_heun(s, e, n, a, dt)
+        t = n * dt
+        w = R.GAMMA * 2.0 / (1 + a * a)
+        th = 2 * np.arctan(np.tan(th0 / 2) * np.exp(-a * w * t))
+        assert abs(np.arccos(e[0, 2]) - th) < 1e-7
+        assert abs(np.arctan2(e[0, 1], e[0, 0]) - w * t) < 1e-7
+
+
+def test_gradient_matches_finite_difference():
+    s = _sys((5, 5, 1), (0.1, 0.2, 0.5), lattice="triangular", J=[1.0, 0.2], D=0.3, K=0.1)
+    e = s.init_random(1)
+    g = s.gradient(e)
+    h = 1e-6
+    for i, a in [(0, 0), (3, 2), (7, 1), (24, 0)]:
+        ep, em = e.copy(), e.copy()
+        ep[i, a] += h
+        em[i, a] -= h
+        assert abs((s.energy(ep) - s.energy(em)) / (2 * h) - g[i, a]) < 1e-7
+
+
+def test_energy_conserved_without_damping():
+    s = _sys((4, 4, 4), lattice="sc", J=[1.0], K=0.05)
+    e = s.init_random(3)
+    E0 = s.energy(e)
+    e = R.run_heun(s, e, 500, 0.0, 2e-4)
+    assert abs(s.energy(e) - E0) / abs(E0) < 1e-4
+
+
+# ------------------------------------------------------------------ generators
+def test_philox_known_answers():
+    """Random123 kat_vectors for philox4x32-10."""
+    cases = [((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+             ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+             ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+              (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1))]
+    for c, k, out in cases:
+        got = philox.philox4x32_10(np.array(c, np.uint32), np.array(k, np.uint32))
+        assert tuple(int(x) for x in got) == out
+
+
+def test_philox_normals_moments():
+    x = philox.normals(7, 3, 400000).ravel()
+    assert abs(x.mean()) < 0.005 and abs(x.var() - 1) < 0.006
+    assert abs((x**4).mean() / x.var() ** 2 - 3) < 0.03
+    assert abs((np.abs(x) > 3).mean() / 0.0026998 - 1) < 0.08
+
+
+def test_c_ziggurat_moments():
+    """The C restatement of magex's ziggurat passes magex's own moment test (test_fast.py:39-46)."""
+    from oracle import cpu
+    x = cpu.gauss3(300000, 7).ravel()
+    assert abs(x.mean()) < 0.006 and abs(x.var() - 1) < 0.006
+    assert abs((x**4).mean() / x.var() ** 2 - 3) < 0.03
+    assert abs((np.abs(x) > 3).mean() / 0.0026998 - 1) < 0.08
+
+
+def test_c_baseline_noise_injected_equals_numpy_oracle():
+    from oracle import cpu
+    s, g = bcc_perbond_system()
+    e = g["e0"].copy()
+    cpu.CpuHeun(s, 0.1, 1e-4, T=300.0).run(e, 10, noise=g["noise"])
+    assert np.abs(e - g["e10"]).max() < 1e-13
+
+
+@pytest.mark.parametrize("planted", ["dmi_sign", "self_fold", "noise_scale"])
+def test_planted_mistakes_are_caught(planted):
+    """A plausible oracle mistake must fail the golden comparisons above."""
+    g = golden("tri12.npz")
+    kw = dict(lattice="triangular", J=[5.0, -0.5, 0.2], D=1.5, K=0.3)
+    if planted == "dmi_sign":
+        kw["D"] = -1.5
+    s = _sys((12, 12, 1), (0, 0, 2.0), **kw)
+    T = 20.0 * (4.0 if planted == "noise_scale" else 1.0)
+    if planted == "self_fold":
+        s.A = s.A * 0.0
+    e = R.run_heun(s, g["e0"].copy(), 10, 0.05, 5e-4, T=T, normals=lambda n: g["noise"][n])
+    assert np.abs(e - g["e10"]).max() > 1e-6
