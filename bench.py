@@ -129,16 +129,26 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=("fp32", "fp64"))
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a plain 256 MiB device copy first (the reference kernel for rocprof FETCH/WRITE_SIZE)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (the multi-GPU sharding of this "
               "path is a design, not a measurement)", file=sys.stderr)
         sys.exit(2)
+    if a.copy_calibration:
+        import torch
+        x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
     system, p, desc = workload(a.workload)
     print(f"workload {a.workload}: {system.N} spins; precision {a.precision}", file=sys.stderr, flush=True)
     r = gpu_rate(system, p, a.precision, a.steps, a.warmup)
     other = "fp64" if a.precision == "fp32" else "fp32"
-    r2 = gpu_rate(system, p, other, max(10, a.steps // 5), max(2, a.warmup // 5))
+    r2 = None if a.no_fp64 else gpu_rate(system, p, other, max(10, a.steps // 5), max(2, a.warmup // 5))
     N = system.N
     value = N * a.steps / r["wall_s"]
     B = alg_bytes(system, a.precision)
@@ -154,9 +164,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "Heun step = k_heun pass 1 + pass 2 (event time per step)"},
-        other: {"spin_steps_per_s": N / (r2["step_ms"] * 1e-3), "ms_per_step": r2["step_ms"],
-                "roofline_frac": alg_bytes(system, other) * N / (r2["step_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
+    if r2 is not None:
+        out[other] = {"spin_steps_per_s": N / (r2["step_ms"] * 1e-3), "ms_per_step": r2["step_ms"],
+                      "roofline_frac": alg_bytes(system, other) * N / (r2["step_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
     if not a.no_cpu_baseline:
         print("cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(system, p, a.cpu_seconds)

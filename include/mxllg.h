@@ -98,6 +98,12 @@ int mx_model_set_dynamics(mx_model* m, double dt, int alpha_per_site, const doub
 int mx_llg_run(mx_model* m, void* e, int64_t nsteps, uint64_t seed, uint64_t step0, const void* noise,
                void* stream);
 
+/* Steps per captured step loop (hipGraph replayed by mx_llg_run when no noise is injected and at least
+ * two loops fit; default 32, 0 = always launch step by step).  Results are identical either way: the
+ * captured kernels read the step number from a device counter, so the noise sequence is the same.
+ * No counterpart in magex (its step loop is a Python for loop, llg.py:263-280). */
+int mx_model_set_graph_steps(mx_model* m, int32_t steps);
+
 /* The standard normals mx_llg_run draws for (seed, step): out (device, N x 3). */
 int mx_noise(mx_model* m, void* out, uint64_t seed, uint64_t step, void* stream);
 

@@ -14,6 +14,8 @@ namespace mx {
 
 static thread_local std::string g_err;
 
+static const char* mx_last_error_text() { return g_err.c_str(); }
+
 void set_error(const std::string& msg) { g_err = msg; }
 
 struct Error {
@@ -93,11 +95,63 @@ static void ensure_scratch(Model& m) {
 
 static void use_device(const Model& m) { hip_ok(hipSetDevice(m.device), "hipSetDevice"); }
 
+// Bring the device constant block up to date (synchronous copy; only after configuration changes).
+static void sync_static(Model& m) {
+  if (!m.static_dirty) return;
+  const int rc = upload_static(m);
+  if (rc) fail(rc, "model constants: " + std::string(mx_last_error_text()));
+}
+
 static void need_model(const mx_model* m) {
   if (!m) fail(MX_EINVAL, "null model");
 }
 
 static void recompute_sigma(Model& m);
+
+static void drop_graph(Model& m) {
+  if (m.gexec) (void)hipGraphExecDestroy(m.gexec);
+  m.gexec = nullptr;
+  m.gkey.clear();
+}
+
+template <typename T>
+static void put(std::vector<char>& k, const T& v) {
+  const char* p = reinterpret_cast<const char*>(&v);
+  k.insert(k.end(), p, p + sizeof(T));
+}
+
+// Capture (once per configuration) graph_steps Heun steps plus the counter advance on the model's
+// own stream; everything a kernel argument bakes in is part of the key.
+static void ensure_graph(Model& m, void* E, uint64_t seed) {
+  std::vector<char> key;
+  for (const void* p : {E, m.ep_d, m.k1_d, m.static_d, m.bond_d, m.A_d, m.Kc_d, m.K2_d, m.K3_d, m.inv_mu_d,
+                        m.alpha_d, m.sigma_d, m.sign_d, m.pinned_d})
+    put(key, p);
+  put(key, seed);
+  for (double b : m.B) put(key, b);
+  put(key, m.dt);
+  for (int v : {m.precession, m.noisy, m.coup, m.pbc[0], m.pbc[1], m.pbc[2], m.graph_steps, m.prec}) put(key, v);
+  if (m.gexec && key == m.gkey) return;
+  drop_graph(m);
+  if (!m.counter_d) hip_ok(hipMalloc(reinterpret_cast<void**>(&m.counter_d), sizeof(uint64_t)), "hipMalloc counter");
+  if (!m.cap_stream) hip_ok(hipStreamCreateWithFlags(&m.cap_stream, hipStreamNonBlocking), "hipStreamCreate");
+  hip_ok(hipStreamBeginCapture(m.cap_stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  int rc = MX_OK;
+  for (int n = 0; n < m.graph_steps && rc == MX_OK; ++n)
+    rc = launch_heun_step(m, E, seed, uint64_t(n), nullptr, m.cap_stream, m.counter_d);
+  if (rc == MX_OK) rc = launch_counter(m.counter_d, uint64_t(m.graph_steps), 1, m.cap_stream);
+  hipGraph_t g = nullptr;
+  const hipError_t end = hipStreamEndCapture(m.cap_stream, &g);
+  if (rc != MX_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    fail(rc, "capturing the step loop: " + g_err);
+  }
+  hip_ok(end, "hipStreamEndCapture");
+  const hipError_t inst = hipGraphInstantiate(&m.gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  hip_ok(inst, "hipGraphInstantiate");
+  m.gkey = key;
+}
 
 }  // namespace mx
 
@@ -151,9 +205,12 @@ void mx_model_destroy(mx_model* m) {
   if (!m) return;
   (void)hipSetDevice(m->device);
   for (void** p : {&m->bond_d, &m->A_d, &m->Kc_d, &m->K2_d, &m->K3_d, &m->inv_mu_d, &m->sign_d, &m->pinned_d,
-                   &m->alpha_d, &m->sigma_d, &m->ep_d, &m->k1_d, &m->eint_d, &m->out_d})
+                   &m->alpha_d, &m->sigma_d, &m->ep_d, &m->k1_d, &m->eint_d, &m->static_d})
     dfree(*p);
   if (m->part_d) (void)hipFree(m->part_d);
+  drop_graph(*m);
+  if (m->counter_d) (void)hipFree(m->counter_d);
+  if (m->cap_stream) (void)hipStreamDestroy(m->cap_stream);
   delete m;
 }
 
@@ -217,6 +274,7 @@ int mx_model_set_stencil(mx_model* m, int32_t nent, const int32_t* ent, int32_t 
     m->coup = width == 1 ? COUP_TABLE1 : width == 4 ? COUP_TABLE4 : COUP_TABLE9;
     dfree(m->bond_d);
     m->nfwd = 0;
+    m->static_dirty = 1;
     return MX_OK;
   });
 }
@@ -243,15 +301,7 @@ int mx_model_set_bond_tensors(mx_model* m, const double* J) {
           partner[k] = q;
       if (partner[k] < 0) fail(MX_EINVAL, "stencil entry without a reverse partner: per-bond storage needs J_ji");
     }
-    // reorder into component-major [f][q][cell]
-    std::vector<double> buf(size_t(nfwd) * 9 * m->ncell);
-    for (int k = 0; k < n; ++k) {
-      if (fwd_slot[k] < 0) continue;
-      const int f = fwd_slot[k];
-      for (int c = 0; c < m->ncell; ++c)
-        for (int q = 0; q < 9; ++q)
-          buf[(size_t(f) * 9 + q) * m->ncell + c] = J[(size_t(f) * m->ncell + c) * 9 + q];
-    }
+    // J arrives as [f][cell][9], the device layout: one contiguous row-major block per bond
     // entries (internal order) point at a forward slot: their own, or the partner's (back)
     for (size_t pos = 0; pos < m->ent.size(); ++pos) {
       const int k = m->ent_order[pos];
@@ -263,9 +313,10 @@ int mx_model_set_bond_tensors(mx_model* m, const double* J) {
         m->ent[pos].tb = (m->ent[pos].tb & 0xffff) | (1 << 16);
       }
     }
-    upload(*m, m->bond_d, buf.data(), buf.size());
+    upload(*m, m->bond_d, J, size_t(nfwd) * m->ncell * 9);
     m->nfwd = nfwd;
     m->coup = COUP_BOND;
+    m->static_dirty = 1;
     return MX_OK;
   });
 }
@@ -278,13 +329,17 @@ int mx_model_set_onsite(mx_model* m, int per_site, const double* A) {
     m->amode = ONSITE_SUB;
     m->has_A = 0;
     std::fill(m->Asub.begin(), m->Asub.end(), 0.0);
-    if (!A) return MX_OK;
+    if (!A) {
+      m->static_dirty = 1;
+      return MX_OK;
+    }
     if (!per_site) {
       for (int k = 0; k < 9 * m->ns; ++k) {
         m->Asub[k] = A[k];
         if (A[k] != 0.0) m->has_A = 1;
       }
-      return MX_OK;
+      m->static_dirty = 1;
+    return MX_OK;
     }
     bool sym = true, any = false;
     for (int r = 0; r < m->N && sym; ++r) {
@@ -292,7 +347,10 @@ int mx_model_set_onsite(mx_model* m, int per_site, const double* A) {
       sym = a[1] == a[3] && a[2] == a[6] && a[5] == a[7];
     }
     for (size_t k = 0; k < size_t(m->N) * 9 && !any; ++k) any = A[k] != 0.0;
-    if (!any) return MX_OK;
+    if (!any) {
+      m->static_dirty = 1;
+      return MX_OK;
+    }
     m->has_A = 1;
     if (sym) {
       std::vector<double> six(size_t(m->N) * 6);
@@ -309,6 +367,7 @@ int mx_model_set_onsite(mx_model* m, int per_site, const double* A) {
       upload(*m, m->A_d, soa.data(), soa.size());
       m->amode = ONSITE_SITE9;
     }
+    m->static_dirty = 1;
     return MX_OK;
   });
 }
@@ -319,12 +378,16 @@ int mx_model_set_cubic(mx_model* m, const double* Kc, const double C[9]) {
     use_device(*m);
     dfree(m->Kc_d);
     m->has_c = 0;
-    if (!Kc) return MX_OK;
+    if (!Kc) {
+      m->static_dirty = 1;
+      return MX_OK;
+    }
     if (!C) fail(MX_EINVAL, "cubic axes missing");
     const std::vector<double> v = to_internal_soa(*m, Kc, 1);
     upload(*m, m->Kc_d, v.data(), v.size());
     for (int q = 0; q < 9; ++q) m->C[q] = C[q];
     m->has_c = 1;
+    m->static_dirty = 1;
     return MX_OK;
   });
 }
@@ -336,7 +399,10 @@ int mx_model_set_uniaxial_higher(mx_model* m, const double* K2, const double* K3
     dfree(m->K2_d);
     dfree(m->K3_d);
     m->has_u = 0;
-    if (!K2) return MX_OK;
+    if (!K2) {
+      m->static_dirty = 1;
+      return MX_OK;
+    }
     if (!u) fail(MX_EINVAL, "uniaxial axis missing");
     const std::vector<double> zeros(m->N, 0.0);
     const std::vector<double> a = to_internal_soa(*m, K2, 1);
@@ -345,6 +411,7 @@ int mx_model_set_uniaxial_higher(mx_model* m, const double* K2, const double* K3
     upload(*m, m->K3_d, b.data(), b.size());
     for (int q = 0; q < 3; ++q) m->u[q] = u[q];
     m->has_u = 1;
+    m->static_dirty = 1;
     return MX_OK;
   });
 }
@@ -380,6 +447,7 @@ int mx_model_set_moments(mx_model* m, int per_site, const double* mu, const doub
       }
     }
     if (m->dyn_set) recompute_sigma(*m);
+    m->static_dirty = 1;
     return MX_OK;
   });
 }
@@ -389,16 +457,23 @@ int mx_model_set_pinned(mx_model* m, const uint8_t* pinned) {
     need_model(m);
     use_device(*m);
     dfree(m->pinned_d);
-    if (!pinned) return MX_OK;
+    if (!pinned) {
+      m->static_dirty = 1;
+      return MX_OK;
+    }
     std::vector<uint8_t> v(m->N);
     bool any = false;
     for (int r = 0; r < m->N; ++r) {
       v[internal_index(*m, r)] = pinned[r] ? 1 : 0;
       any = any || pinned[r];
     }
-    if (!any) return MX_OK;
+    if (!any) {
+      m->static_dirty = 1;
+      return MX_OK;
+    }
     hip_ok(hipMalloc(&m->pinned_d, m->N), "hipMalloc pinned");
     hip_ok(hipMemcpy(m->pinned_d, v.data(), m->N, hipMemcpyHostToDevice), "hipMemcpy pinned");
+    m->static_dirty = 1;
     return MX_OK;
   });
 }
@@ -463,6 +538,20 @@ int mx_model_set_dynamics(mx_model* m, double dt, int alpha_per_site, const doub
     recompute_sigma(*m);
     m->dyn_set = 1;
     ensure_scratch(*m);
+    m->static_dirty = 1;
+    return MX_OK;
+  });
+}
+
+int mx_model_set_graph_steps(mx_model* m, int32_t steps) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (steps < 0 || steps > 4096) fail(MX_EINVAL, "graph_steps must lie in [0, 4096]");
+    if (steps != m->graph_steps) {
+      use_device(*m);
+      drop_graph(*m);
+    }
+    m->graph_steps = steps;
     return MX_OK;
   });
 }
@@ -476,6 +565,7 @@ int mx_llg_run(mx_model* m, void* e, int64_t nsteps, uint64_t seed, uint64_t ste
     if (!m->dyn_set) fail(MX_EINVAL, "set the dynamics first");
     use_device(*m);
     ensure_scratch(*m);
+    sync_static(*m);
     hipStream_t s = static_cast<hipStream_t>(stream);
     void* E = e;
     if (m->ns > 1) {
@@ -484,7 +574,16 @@ int mx_llg_run(mx_model* m, void* e, int64_t nsteps, uint64_t seed, uint64_t ste
       E = m->eint_d;
     }
     const size_t stride = size_t(m->N) * 3 * rsize(*m);
-    for (int64_t n = 0; n < nsteps; ++n) {
+    int64_t n = 0;
+    const int G = m->graph_steps;
+    if (!noise && G > 0 && nsteps >= 2 * int64_t(G)) {
+      // replay a captured loop of G steps; the kernels take the step number from a device counter
+      ensure_graph(*m, E, seed);
+      int rc = launch_counter(m->counter_d, step0, 0, s);
+      if (rc) return rc;
+      for (; n + G <= nsteps; n += G) hip_ok(hipGraphLaunch(m->gexec, s), "hipGraphLaunch");
+    }
+    for (; n < nsteps; ++n) {
       const void* nz = noise ? static_cast<const char*>(noise) + size_t(n) * stride : nullptr;
       int rc = launch_heun_step(*m, E, seed, step0 + uint64_t(n), nz, s);
       if (rc) return rc;
@@ -517,6 +616,7 @@ int mx_field(mx_model* m, const void* e, void* out, int kind, void* stream) {
     if (kind < MX_FIELD_BEFF || kind > MX_FIELD_EXCHANGE) fail(MX_EINVAL, "bad field kind");
     use_device(*m);
     ensure_scratch(*m);
+    sync_static(*m);
     hipStream_t s = static_cast<hipStream_t>(stream);
     int rc;
     const void* E = internal_spins(m, e, s, rc);
@@ -531,6 +631,7 @@ int mx_observables(mx_model* m, const void* e, double* out, void* stream) {
     if (!e || !out) fail(MX_EINVAL, "null buffer");
     use_device(*m);
     ensure_scratch(*m);
+    sync_static(*m);
     hipStream_t s = static_cast<hipStream_t>(stream);
     int rc;
     const void* E = internal_spins(m, e, s, rc);

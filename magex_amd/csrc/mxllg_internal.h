@@ -15,55 +15,55 @@ constexpr double MU_B = 5.7883818060e-2;  // meV / T
 constexpr double K_B = 8.617333262e-2;    // meV / K
 constexpr double GAMMA = 0.1760859630;    // rad / (ps T)
 
-constexpr int MAXSUB = 8;   // sublattices per cell
-constexpr int MAXENT = 48;  // stencil entries over all sublattices
-constexpr int MAXTAB = 160; // coupling-table values (ntypes x width) held in the kernel arguments
-constexpr int BLOCK = 256;  // threads per workgroup (4 wavefronts of 64)
+constexpr int MAXSUB = 8;    // sublattices per cell
+constexpr int MAXENT = 64;   // stencil entries over all sublattices
+constexpr int MAXTAB = 256;  // coupling-table values (ntypes x width)
+constexpr int BLOCK = 256;   // threads per workgroup (4 wavefronts of 64)
 
 enum CoupMode { COUP_TABLE1 = 0, COUP_TABLE4 = 1, COUP_TABLE9 = 2, COUP_BOND = 3 };
 enum OnsiteMode { ONSITE_SUB = 0, ONSITE_SITE6 = 1, ONSITE_SITE9 = 2 };
 
-// One stencil entry, packed into 16 bytes so the whole stencil rides in the kernel arguments
-// (constant, read with scalar loads because every lane of a workgroup shares the sublattice):
-//   x = coupling slot (type-table row, or forward-bond slot), y = target sublattice | back << 16,
-//   z = (d0 & 0xffff) | d1 << 16, w = d2; offsets d are reduced into [0, L) on periodic axes.
+// One stencil entry (16 bytes): slot = coupling slot (type-table row, or forward-bond slot),
+// tb = target sublattice | back << 16, d01 = (d0 & 0xffff) | d1 << 16, d2; offsets d are reduced into
+// [0, L) on periodic axes.
 struct Ent {
   int32_t slot, tb, d01, d2;
 };
 
-// Kernel parameters, passed by value (< 3 KB).  Sites live in sublattice-major order inside the
-// library (internal index = s * ncell + cell), so a workgroup never mixes sublattices.
+// Model constants, resident in device memory (one block per model, uploaded when the model is
+// configured).  Kernels read it through a const __restrict__ pointer with wavefront-uniform indices,
+// so it comes in through the scalar cache; the kernel arguments themselves stay small.
 template <typename R>
-struct KP {
+struct Static {
   int L0, L1, L2, ncell, ns, N, bps, nblk;  // bps: workgroups per sublattice
-  int pbc0, pbc1, pbc2;
+  int pbc0, pbc1, pbc2, allpbc;
+  int amode, has_A, has_c, has_u;
   int sptr[MAXSUB + 1];
   Ent ent[MAXENT];
-  R tab[MAXTAB];   // COUP_TABLE*: ntypes x width, read with scalar loads
-  const R* bond;   // COUP_BOND: nfwd x 9 x ncell (component-major per forward slot)
-  int amode, has_A;
-  const R* A;  // ONSITE_SITE6: 6 x N (xx, yy, zz, xy, xz, yz); ONSITE_SITE9: 9 x N (row-major)
+  R tab[MAXTAB];  // COUP_TABLE*: ntypes x width
   R Asub[MAXSUB][9];
-  int has_c, has_u;
+  R inv_mu_sub[MAXSUB], alpha_sub[MAXSUB], sigma_sub[MAXSUB], sign_sub[MAXSUB];
+  R C[9], u[3];
+};
+
+// Per-launch values and the model's per-site arrays, passed by value: pointers that arrive as kernel
+// arguments are known to be global memory, so their loads compile to global_load (not flat_load).
+template <typename R>
+struct Dyn {
+  R B0, B1, B2, dt;
+  int prec, noisy;
+  uint32_t seed_lo, seed_hi;
+  const R* bond;  // COUP_BOND: [nfwd][ncell][9], row-major tensor per bond (36 B FP32 / 72 B FP64)
+  const R* A;     // ONSITE_SITE6: 6 x N (xx, yy, zz, xy, xz, yz); ONSITE_SITE9: 9 x N (row-major)
   const R* Kc;
   const R* K2;
   const R* K3;
-  R C[9];
-  R u[3];
   const R* inv_mu;  // per-site 1 / (mu mu_B), or null -> inv_mu_sub
-  R inv_mu_sub[MAXSUB];
-  const R* alpha;  // per-site damping, or null -> alpha_sub
-  R alpha_sub[MAXSUB];
-  const R* sigma;  // per-site thermal amplitude (T), or null -> sigma_sub
-  R sigma_sub[MAXSUB];
-  R sign_sub[MAXSUB];
-  const R* sign_site;  // per-site reference signs, or null
+  const R* alpha;   // per-site damping, or null -> alpha_sub
+  const R* sigma;   // per-site thermal amplitude (T), or null -> sigma_sub
+  const R* sign_site;
   const uint8_t* pinned;
-  R B[3];
-  R dt;
-  int prec, noisy;
-  uint32_t seed_lo, seed_hi;
-  const R* noise;  // injected standard normals of the current step (N x 3, magex order), or null
+  const uint64_t* step_ptr;  // captured step loops: device step counter added to the step argument
 };
 
 // Host-side model.
@@ -75,8 +75,8 @@ struct Model {
   int nent = 0;
   std::vector<int32_t> ent_raw;  // nent x 6 as given
   std::vector<int> sptr;
-  std::vector<Ent> ent;          // internal order (sorted by source sublattice)
-  std::vector<int> ent_order;    // ent[pos] came from ent_raw entry ent_order[pos]
+  std::vector<Ent> ent;        // internal order (sorted by source sublattice)
+  std::vector<int> ent_order;  // ent[pos] came from ent_raw entry ent_order[pos]
   int coup = COUP_TABLE1;
   int ntypes = 0, width = 0;
   std::vector<double> table;
@@ -104,16 +104,26 @@ struct Model {
   std::vector<double> alpha_sub_v, sigma_sub_v;
   int noisy = 0;
   int dyn_set = 0;
+  // device constant block (Static<R>) and its dirty flag
+  void* static_d = nullptr;
+  int static_dirty = 1;
   // scratch
-  void *ep_d = nullptr, *k1_d = nullptr, *eint_d = nullptr, *out_d = nullptr;
+  void *ep_d = nullptr, *k1_d = nullptr, *eint_d = nullptr;
   double* part_d = nullptr;
   int npart = 0;
+  // captured step loop (hipGraph of graph_steps Heun steps + counter advance), keyed by what it bakes in
+  int graph_steps = 32;
+  uint64_t* counter_d = nullptr;
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<char> gkey;
 };
 
 // Kernel launchers (mxllg_kernels.hip).  All take internal-order device buffers unless noted.
-template <typename R>
-void fill_kp(const Model& m, KP<R>& p);
-int launch_heun_step(const Model& m, void* e, uint64_t seed, uint64_t step, const void* noise_step, hipStream_t s);
+int upload_static(Model& m);
+int launch_heun_step(const Model& m, void* e, uint64_t seed, uint64_t step, const void* noise_step, hipStream_t s,
+                     const uint64_t* step_ptr = nullptr);
+int launch_counter(uint64_t* c, uint64_t value, int add, hipStream_t s);
 int launch_field(const Model& m, const void* e_int, void* out_ref, int kind, hipStream_t s);
 int launch_observables(const Model& m, const void* e_int, double* out, hipStream_t s);
 int launch_noise(const Model& m, void* out_ref, uint64_t seed, uint64_t step, hipStream_t s);

@@ -5,13 +5,17 @@
 // fetched byte used), sites in sublattice-major order s * ncell + cell with cells row-major (c2
 // fastest).  Neighbour indices are never stored: a site's neighbours are cell offsets from the stencil
 // in the kernel arguments, wrapped on periodic axes.  Couplings are either a small type table (kernel
-// arguments, scalar loads) or per-bond tensors stored once per unordered bond, component-major
-// (bond[(f * 9 + q) * ncell + cell]) so that a wavefront reads 64 consecutive floats per component.
+// constant block, scalar loads) or per-bond tensors stored once per unordered bond as contiguous
+// row-major 3x3 blocks (bond[(f * ncell + cell) * 9 + q]: a wavefront reads 64 consecutive 36-B
+// blocks with two dwordx4 loads and one dword load per lane).  Model constants sit in a device
+// block (Static<R>) read with wavefront-uniform indices, so the kernel arguments stay a few words.
 //
 // Workgroups: 256 threads, one site each; a workgroup never mixes sublattices, so the stencil loop is
 // wavefront-uniform.  The block -> tile map is XCD-aware: the dispatcher deals workgroups round-robin
 // to the 8 XCDs, so tile = xcd * (G / 8) + (block / 8) gives every XCD one contiguous slab of the
 // lattice and the neighbour re-reads of that slab hit its own L2.
+#include <cstring>
+
 #include "mxllg_internal.h"
 
 namespace mx {
@@ -27,16 +31,21 @@ struct Site {
 };
 
 template <typename R>
-__device__ __forceinline__ bool site_of(const KP<R>& p, Site& st) {
+__device__ __forceinline__ bool site_of(const Static<R>* __restrict__ S, Site& st) {
   const int b = xcd_tile(blockIdx.x, gridDim.x);
-  st.s = b / p.bps;
-  st.c = (b - st.s * p.bps) * BLOCK + threadIdx.x;
-  if (st.c >= p.ncell) return false;
-  st.c2 = st.c % p.L2;
-  const int t = st.c / p.L2;
-  st.c1 = t % p.L1;
-  st.c0 = t / p.L1;
-  st.i = st.s * p.ncell + st.c;
+  const int ncell = S->ncell, L1 = S->L1, L2 = S->L2;
+  // sublattices interleave tile by tile, so one XCD works on every sublattice of its slab of cells and
+  // the partner reads of per-bond tensors stored by another sublattice hit the same L2
+  const int ns = S->ns;
+  const int tile = b / ns;
+  st.s = b - tile * ns;
+  st.c = tile * BLOCK + threadIdx.x;
+  if (st.c >= ncell) return false;
+  st.c2 = st.c % L2;
+  const int t = st.c / L2;
+  st.c1 = t % L1;
+  st.c0 = t / L1;
+  st.i = st.s * ncell + st.c;
   return true;
 }
 
@@ -83,38 +92,54 @@ __device__ __forceinline__ R rnorm3(R x, R y, R z) {
   return R(1) / sqrt(x * x + y * y + z * z);
 }
 
-// h_i = sum_j J_ij e_j over the stencil of site st (magex/fast.py:86-120 semantics).
-template <typename R, int CM>
-__device__ __forceinline__ void exchange(const KP<R>& p, const R* __restrict__ e, const Site& st, R& hx, R& hy,
-                                         R& hz) {
+// h_i = sum_j J_ij e_j over the stencil of site st (magex/fast.py:86-120 semantics).  PBC: every axis
+// periodic (no bounds test); otherwise a neighbour past an open face contributes zero, read through a
+// clamped (valid) index so the loop stays branch-free and unrollable.
+template <typename R, int CM, bool PBC>
+__device__ __forceinline__ void exchange(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
+                                         const Site& st, R& hx, R& hy, R& hz) {
   hx = hy = hz = R(0);
-  const int kb = p.sptr[st.s], ke = p.sptr[st.s + 1];
+  const int kb = S->sptr[st.s], ke = S->sptr[st.s + 1];
+  const int L0 = S->L0, L1 = S->L1, L2 = S->L2, ncell = S->ncell;
+  const bool p0 = S->pbc0, p1 = S->pbc1, p2 = S->pbc2;
+#pragma unroll 2
   for (int k = kb; k < ke; ++k) {
-    const Ent en = p.ent[k];
+    const Ent en = S->ent[k];
     int n0 = st.c0 + int(int16_t(en.d01 & 0xffff));
     int n1 = st.c1 + (en.d01 >> 16);
     int n2 = st.c2 + en.d2;
     bool ok = true;
-    if (p.pbc0) n0 -= (n0 >= p.L0) ? p.L0 : 0; else ok = ok && unsigned(n0) < unsigned(p.L0);
-    if (p.pbc1) n1 -= (n1 >= p.L1) ? p.L1 : 0; else ok = ok && unsigned(n1) < unsigned(p.L1);
-    if (p.pbc2) n2 -= (n2 >= p.L2) ? p.L2 : 0; else ok = ok && unsigned(n2) < unsigned(p.L2);
-    if (!ok) continue;
-    const int cn = (n0 * p.L1 + n1) * p.L2 + n2;
-    const int j = (en.tb & 0xffff) * p.ncell + cn;
-    const R ex = e[3 * j], ey = e[3 * j + 1], ez = e[3 * j + 2];
+    if constexpr (PBC) {
+      n0 = n0 >= L0 ? n0 - L0 : n0;
+      n1 = n1 >= L1 ? n1 - L1 : n1;
+      n2 = n2 >= L2 ? n2 - L2 : n2;
+    } else {
+      n0 = (p0 && n0 >= L0) ? n0 - L0 : n0;
+      n1 = (p1 && n1 >= L1) ? n1 - L1 : n1;
+      n2 = (p2 && n2 >= L2) ? n2 - L2 : n2;
+      ok = unsigned(n0) < unsigned(L0) && unsigned(n1) < unsigned(L1) && unsigned(n2) < unsigned(L2);
+    }
+    const int cn = ok ? (n0 * L1 + n1) * L2 + n2 : st.c;
+    const int j = (en.tb & 0xffff) * ncell + cn;
+    R ex = e[3 * j], ey = e[3 * j + 1], ez = e[3 * j + 2];
+    if constexpr (!PBC) {
+      ex = ok ? ex : R(0);
+      ey = ok ? ey : R(0);
+      ez = ok ? ez : R(0);
+    }
     if constexpr (CM == COUP_TABLE1) {
-      const R J = p.tab[en.slot];
+      const R J = S->tab[en.slot];
       hx += J * ex;
       hy += J * ey;
       hz += J * ez;
     } else if constexpr (CM == COUP_TABLE4) {
       // J I + antisymmetric part: (J^A e_j) = e_j x D
-      const R* T = p.tab + 4 * en.slot;
+      const R* T = S->tab + 4 * en.slot;
       hx += T[0] * ex + (ey * T[3] - ez * T[2]);
       hy += T[0] * ey + (ez * T[1] - ex * T[3]);
       hz += T[0] * ez + (ex * T[2] - ey * T[1]);
     } else if constexpr (CM == COUP_TABLE9) {
-      const R* T = p.tab + 9 * en.slot;
+      const R* T = S->tab + 9 * en.slot;
       hx += T[0] * ex + T[1] * ey + T[2] * ez;
       hy += T[3] * ex + T[4] * ey + T[5] * ez;
       hz += T[6] * ex + T[7] * ey + T[8] * ez;
@@ -122,10 +147,9 @@ __device__ __forceinline__ void exchange(const KP<R>& p, const R* __restrict__ e
       // per-bond tensor stored once per unordered bond: forward entries read their own cell's copy,
       // backward entries the partner's copy at the neighbour cell, transposed (J_ji = J_ij^T)
       const bool back = (en.tb >> 16) != 0;
-      const R* Jb = p.bond + size_t(en.slot) * 9 * p.ncell + (back ? cn : st.c);
-      const int n = p.ncell;
-      const R J0 = Jb[0], J1 = Jb[n], J2 = Jb[2 * n], J3 = Jb[3 * n], J4 = Jb[4 * n], J5 = Jb[5 * n],
-              J6 = Jb[6 * n], J7 = Jb[7 * n], J8 = Jb[8 * n];
+      const R* Jb = d.bond + (size_t(en.slot) * ncell + (back ? cn : st.c)) * 9;
+      const R J0 = Jb[0], J1 = Jb[1], J2 = Jb[2], J3 = Jb[3], J4 = Jb[4], J5 = Jb[5], J6 = Jb[6], J7 = Jb[7],
+              J8 = Jb[8];
       if (!back) {
         hx += J0 * ex + J1 * ey + J2 * ez;
         hy += J3 * ex + J4 * ey + J5 * ez;
@@ -141,83 +165,90 @@ __device__ __forceinline__ void exchange(const KP<R>& p, const R* __restrict__ e
 
 // On-site gradient dE/de (magex/fast.py:39-64): 2 A e + cubic + higher uniaxial.
 template <typename R>
-__device__ __forceinline__ void onsite_grad(const KP<R>& p, const Site& st, R x, R y, R z, R& gx, R& gy, R& gz) {
+__device__ __forceinline__ void onsite_grad(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st, R x, R y,
+                                            R z, R& gx, R& gy, R& gz) {
   gx = gy = gz = R(0);
-  if (p.amode == ONSITE_SUB) {
-    if (p.has_A) {
-      const R* a = p.Asub[st.s];
+  const int amode = S->amode;
+  if (amode == ONSITE_SUB) {
+    if (S->has_A) {
+      const R* a = S->Asub[st.s];
       gx = R(2) * (a[0] * x + a[1] * y + a[2] * z);
       gy = R(2) * (a[3] * x + a[4] * y + a[5] * z);
       gz = R(2) * (a[6] * x + a[7] * y + a[8] * z);
     }
-  } else if (p.amode == ONSITE_SITE6) {
-    const int N = p.N, i = st.i;
-    const R axx = p.A[i], ayy = p.A[N + i], azz = p.A[2 * N + i];
-    const R axy = p.A[3 * N + i], axz = p.A[4 * N + i], ayz = p.A[5 * N + i];
+  } else if (amode == ONSITE_SITE6) {
+    const int N = S->N, i = st.i;
+    const R* A = d.A;
+    const R axx = A[i], ayy = A[N + i], azz = A[2 * N + i];
+    const R axy = A[3 * N + i], axz = A[4 * N + i], ayz = A[5 * N + i];
     gx = R(2) * (axx * x + axy * y + axz * z);
     gy = R(2) * (axy * x + ayy * y + ayz * z);
     gz = R(2) * (axz * x + ayz * y + azz * z);
   } else {
-    const int N = p.N, i = st.i;
-    const R* a = p.A + i;
+    const int N = S->N;
+    const R* a = d.A + st.i;
     gx = R(2) * (a[0] * x + a[N] * y + a[2 * N] * z);
     gy = R(2) * (a[3 * N] * x + a[4 * N] * y + a[5 * N] * z);
     gz = R(2) * (a[6 * N] * x + a[7 * N] * y + a[8 * N] * z);
   }
-  if (p.has_c) {
-    const R* C = p.C;
+  if (S->has_c) {
+    const R* C = S->C;
     const R a1 = C[0] * x + C[1] * y + C[2] * z;
     const R a2 = C[3] * x + C[4] * y + C[5] * z;
     const R a3 = C[6] * x + C[7] * y + C[8] * z;
-    const R q1 = a1 * a1, q2 = a2 * a2, q3 = a3 * a3, kc = p.Kc[st.i];
+    const R q1 = a1 * a1, q2 = a2 * a2, q3 = a3 * a3, kc = d.Kc[st.i];
     const R d1 = R(2) * a1 * (q2 + q3) * kc, d2 = R(2) * a2 * (q1 + q3) * kc, d3 = R(2) * a3 * (q1 + q2) * kc;
     gx += d1 * C[0] + d2 * C[3] + d3 * C[6];
     gy += d1 * C[1] + d2 * C[4] + d3 * C[7];
     gz += d1 * C[2] + d2 * C[5] + d3 * C[8];
   }
-  if (p.has_u) {
-    const R c = x * p.u[0] + y * p.u[1] + z * p.u[2];
+  if (S->has_u) {
+    const R* u = S->u;
+    const R c = x * u[0] + y * u[1] + z * u[2];
     const R c2 = c * c;
-    const R f = R(-4) * p.K2[st.i] * c2 * c - R(6) * p.K3[st.i] * c2 * c2 * c;
-    gx += f * p.u[0];
-    gy += f * p.u[1];
-    gz += f * p.u[2];
+    const R f = R(-4) * d.K2[st.i] * c2 * c - R(6) * d.K3[st.i] * c2 * c2 * c;
+    gx += f * u[0];
+    gy += f * u[1];
+    gz += f * u[2];
   }
 }
 
 // On-site energy (magex/fast.py:66-84).
 template <typename R>
-__device__ __forceinline__ R onsite_energy(const KP<R>& p, const Site& st, R x, R y, R z) {
+__device__ __forceinline__ R onsite_energy(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st, R x, R y,
+                                          R z) {
   R E = R(0);
-  if (p.amode == ONSITE_SUB) {
-    if (p.has_A) {
-      const R* a = p.Asub[st.s];
+  const int amode = S->amode;
+  if (amode == ONSITE_SUB) {
+    if (S->has_A) {
+      const R* a = S->Asub[st.s];
       E = x * (a[0] * x + a[1] * y + a[2] * z) + y * (a[3] * x + a[4] * y + a[5] * z) +
           z * (a[6] * x + a[7] * y + a[8] * z);
     }
-  } else if (p.amode == ONSITE_SITE6) {
-    const int N = p.N, i = st.i;
-    const R axx = p.A[i], ayy = p.A[N + i], azz = p.A[2 * N + i];
-    const R axy = p.A[3 * N + i], axz = p.A[4 * N + i], ayz = p.A[5 * N + i];
+  } else if (amode == ONSITE_SITE6) {
+    const int N = S->N, i = st.i;
+    const R* A = d.A;
+    const R axx = A[i], ayy = A[N + i], azz = A[2 * N + i];
+    const R axy = A[3 * N + i], axz = A[4 * N + i], ayz = A[5 * N + i];
     E = axx * x * x + ayy * y * y + azz * z * z + R(2) * (axy * x * y + axz * x * z + ayz * y * z);
   } else {
-    const int N = p.N;
-    const R* a = p.A + st.i;
+    const int N = S->N;
+    const R* a = d.A + st.i;
     E = x * (a[0] * x + a[N] * y + a[2 * N] * z) + y * (a[3 * N] * x + a[4 * N] * y + a[5 * N] * z) +
         z * (a[6 * N] * x + a[7 * N] * y + a[8 * N] * z);
   }
-  if (p.has_c) {
-    const R* C = p.C;
+  if (S->has_c) {
+    const R* C = S->C;
     const R a1 = C[0] * x + C[1] * y + C[2] * z;
     const R a2 = C[3] * x + C[4] * y + C[5] * z;
     const R a3 = C[6] * x + C[7] * y + C[8] * z;
     const R q1 = a1 * a1, q2 = a2 * a2, q3 = a3 * a3;
-    E += p.Kc[st.i] * (q1 * q2 + q2 * q3 + q3 * q1);
+    E += d.Kc[st.i] * (q1 * q2 + q2 * q3 + q3 * q1);
   }
-  if (p.has_u) {
-    const R c = x * p.u[0] + y * p.u[1] + z * p.u[2];
+  if (S->has_u) {
+    const R c = x * S->u[0] + y * S->u[1] + z * S->u[2];
     const R c4 = c * c * c * c;
-    E += -p.K2[st.i] * c4 - p.K3[st.i] * c4 * c * c;
+    E += -d.K2[st.i] * c4 - d.K3[st.i] * c4 * c * c;
   }
   return E;
 }
@@ -240,16 +271,17 @@ __device__ __forceinline__ void llg_rhs(R ex, R ey, R ez, R bx, R by, R bz, R a,
 }
 
 template <typename R>
-__device__ __forceinline__ void thermal(const KP<R>& p, const Site& st, uint64_t step, R& bx, R& by, R& bz) {
-  const R sg = p.sigma ? p.sigma[st.i] : p.sigma_sub[st.s];
-  const int iref = st.c * p.ns + st.s;
-  if (p.noise) {
-    bx = sg * p.noise[3 * iref];
-    by = sg * p.noise[3 * iref + 1];
-    bz = sg * p.noise[3 * iref + 2];
+__device__ __forceinline__ void thermal(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st,
+                                        uint64_t step, const R* __restrict__ noise, R& bx, R& by, R& bz) {
+  const R sg = d.sigma ? d.sigma[st.i] : S->sigma_sub[st.s];
+  const int iref = st.c * S->ns + st.s;
+  if (noise) {
+    bx = sg * noise[3 * iref];
+    by = sg * noise[3 * iref + 1];
+    bz = sg * noise[3 * iref + 2];
   } else {
     float n0, n1, n2;
-    gauss3(p.seed_lo, p.seed_hi, step, uint32_t(iref), n0, n1, n2);
+    gauss3(d.seed_lo, d.seed_hi, step, uint32_t(iref), n0, n1, n2);
     bx = sg * R(n0);
     by = sg * R(n1);
     bz = sg * R(n2);
@@ -257,20 +289,45 @@ __device__ __forceinline__ void thermal(const KP<R>& p, const Site& st, uint64_t
 }
 
 template <typename R>
-__device__ __forceinline__ R inv_mu_of(const KP<R>& p, const Site& st) {
-  return p.inv_mu ? p.inv_mu[st.i] : p.inv_mu_sub[st.s];
+__device__ __forceinline__ R inv_mu_of(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st) {
+  return d.inv_mu ? d.inv_mu[st.i] : S->inv_mu_sub[st.s];
+}
+
+// B_i = (2 h_i - g_i) / (mu_i mu_B) + B (+ thermal) at spin (x, y, z) (magex/fast.py:309-317).
+template <typename R, int CM, bool PBC>
+__device__ __forceinline__ void local_field(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
+                                            const Site& st, R x, R y, R z, uint64_t step,
+                                            const R* __restrict__ noise, R& bx, R& by, R& bz) {
+  R hx, hy, hz, gx, gy, gz;
+  exchange<R, CM, PBC>(S, d, e, st, hx, hy, hz);
+  onsite_grad(S, d, st, x, y, z, gx, gy, gz);
+  const R im = inv_mu_of(S, d, st);
+  bx = (R(2) * hx - gx) * im + d.B0;
+  by = (R(2) * hy - gy) * im + d.B1;
+  bz = (R(2) * hz - gz) * im + d.B2;
+  if (d.noisy) {
+    R nx, ny, nz;
+    thermal(S, d, st, step, noise, nx, ny, nz);
+    bx += nx;
+    by += ny;
+    bz += nz;
+  }
 }
 
 // ------------------------------------------------------------------------------------------ Heun
 // PASS 1 (magex/fast.py:336-368): noise + field(e) + predictor ep = n(e + dt k1), keeps k1.
 // PASS 2 (magex/fast.py:370-390): field(ep) + corrector e = n(e + dt/2 (k1 + f(ep))), in place.
-template <typename R, int CM, int PASS>
-__global__ __launch_bounds__(BLOCK) void k_heun(const KP<R> p, R* __restrict__ e, R* __restrict__ ep,
-                                                R* __restrict__ k1, uint64_t step) {
+// The noise of a step is regenerated in pass 2 from the same counter rather than stored.
+template <typename R, int CM, bool PBC, int PASS>
+__global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S, const Dyn<R> d, R* __restrict__ e,
+                                                R* __restrict__ ep, R* __restrict__ k1, uint64_t step,
+                                                const R* __restrict__ noise) {
   Site st;
-  if (!site_of(p, st)) return;
+  if (!site_of(S, st)) return;
+  if (d.step_ptr) step += *d.step_ptr;  // graph replays: step = device counter + offset of this node
   const int i = st.i;
-  const bool pin = p.pinned && p.pinned[i];
+  const bool pin = d.pinned && d.pinned[i];
+  const R a = d.alpha ? d.alpha[i] : S->alpha_sub[st.s];
   if (PASS == 1) {
     const R x = e[3 * i], y = e[3 * i + 1], z = e[3 * i + 2];
     if (pin) {
@@ -280,25 +337,13 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const KP<R> p, R* __restrict__ e
       k1[3 * i] = k1[3 * i + 1] = k1[3 * i + 2] = R(0);
       return;
     }
-    R hx, hy, hz, gx, gy, gz;
-    exchange<R, CM>(p, e, st, hx, hy, hz);
-    onsite_grad(p, st, x, y, z, gx, gy, gz);
-    const R im = inv_mu_of(p, st);
-    R bx = (R(2) * hx - gx) * im + p.B[0], by = (R(2) * hy - gy) * im + p.B[1], bz = (R(2) * hz - gz) * im + p.B[2];
-    if (p.noisy) {
-      R nx, ny, nz;
-      thermal(p, st, step, nx, ny, nz);
-      bx += nx;
-      by += ny;
-      bz += nz;
-    }
-    const R a = p.alpha ? p.alpha[i] : p.alpha_sub[st.s];
-    R fx, fy, fz;
-    llg_rhs(x, y, z, bx, by, bz, a, p.prec, fx, fy, fz);
+    R bx, by, bz, fx, fy, fz;
+    local_field<R, CM, PBC>(S, d, e, st, x, y, z, step, noise, bx, by, bz);
+    llg_rhs(x, y, z, bx, by, bz, a, d.prec, fx, fy, fz);
     k1[3 * i] = fx;
     k1[3 * i + 1] = fy;
     k1[3 * i + 2] = fz;
-    const R px = x + p.dt * fx, py = y + p.dt * fy, pz = z + p.dt * fz;
+    const R px = x + d.dt * fx, py = y + d.dt * fy, pz = z + d.dt * fz;
     const R q = rnorm3(px, py, pz);
     ep[3 * i] = px * q;
     ep[3 * i + 1] = py * q;
@@ -306,22 +351,10 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const KP<R> p, R* __restrict__ e
   } else {
     if (pin) return;
     const R x = ep[3 * i], y = ep[3 * i + 1], z = ep[3 * i + 2];
-    R hx, hy, hz, gx, gy, gz;
-    exchange<R, CM>(p, ep, st, hx, hy, hz);
-    onsite_grad(p, st, x, y, z, gx, gy, gz);
-    const R im = inv_mu_of(p, st);
-    R bx = (R(2) * hx - gx) * im + p.B[0], by = (R(2) * hy - gy) * im + p.B[1], bz = (R(2) * hz - gz) * im + p.B[2];
-    if (p.noisy) {
-      R nx, ny, nz;
-      thermal(p, st, step, nx, ny, nz);
-      bx += nx;
-      by += ny;
-      bz += nz;
-    }
-    const R a = p.alpha ? p.alpha[i] : p.alpha_sub[st.s];
-    R fx, fy, fz;
-    llg_rhs(x, y, z, bx, by, bz, a, p.prec, fx, fy, fz);
-    const R hdt = R(0.5) * p.dt;
+    R bx, by, bz, fx, fy, fz;
+    local_field<R, CM, PBC>(S, d, ep, st, x, y, z, step, noise, bx, by, bz);
+    llg_rhs(x, y, z, bx, by, bz, a, d.prec, fx, fy, fz);
+    const R hdt = R(0.5) * d.dt;
     const R cx = e[3 * i] + hdt * (k1[3 * i] + fx);
     const R cy = e[3 * i + 1] + hdt * (k1[3 * i + 1] + fy);
     const R cz = e[3 * i + 2] + hdt * (k1[3 * i + 2] + fz);
@@ -334,36 +367,36 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const KP<R> p, R* __restrict__ e
 
 // ------------------------------------------------------------------------------------------ field
 // kind: MX_FIELD_BEFF (T), MX_FIELD_GRADIENT (meV), MX_FIELD_EXCHANGE (meV); output in magex order.
-template <typename R, int CM>
-__global__ __launch_bounds__(BLOCK) void k_field(const KP<R> p, const R* __restrict__ e, R* __restrict__ out,
-                                                 int kind) {
+template <typename R, int CM, bool PBC>
+__global__ __launch_bounds__(BLOCK) void k_field(const Static<R>* __restrict__ S, const Dyn<R> d,
+                                                 const R* __restrict__ e, R* __restrict__ out, int kind) {
   Site st;
-  if (!site_of(p, st)) return;
-  const int i = st.i, o = 3 * (st.c * p.ns + st.s);
+  if (!site_of(S, st)) return;
+  const int i = st.i, o = 3 * (st.c * S->ns + st.s);
   R hx, hy, hz;
-  exchange<R, CM>(p, e, st, hx, hy, hz);
+  exchange<R, CM, PBC>(S, d, e, st, hx, hy, hz);
   if (kind == MX_FIELD_EXCHANGE) {
     out[o] = hx;
     out[o + 1] = hy;
     out[o + 2] = hz;
     return;
   }
-  if (p.pinned && p.pinned[i]) {
+  if (d.pinned && d.pinned[i]) {
     out[o] = out[o + 1] = out[o + 2] = R(0);
     return;
   }
   R gx, gy, gz;
-  onsite_grad(p, st, e[3 * i], e[3 * i + 1], e[3 * i + 2], gx, gy, gz);
-  const R im = inv_mu_of(p, st);
+  onsite_grad(S, d, st, e[3 * i], e[3 * i + 1], e[3 * i + 2], gx, gy, gz);
+  const R im = inv_mu_of(S, d, st);
   if (kind == MX_FIELD_BEFF) {
-    out[o] = (R(2) * hx - gx) * im + p.B[0];
-    out[o + 1] = (R(2) * hy - gy) * im + p.B[1];
-    out[o + 2] = (R(2) * hz - gz) * im + p.B[2];
+    out[o] = (R(2) * hx - gx) * im + d.B0;
+    out[o + 1] = (R(2) * hy - gy) * im + d.B1;
+    out[o + 2] = (R(2) * hz - gz) * im + d.B2;
   } else {
     const R mm = R(1) / im;  // mu mu_B
-    out[o] = gx - R(2) * hx - mm * p.B[0];
-    out[o + 1] = gy - R(2) * hy - mm * p.B[1];
-    out[o + 2] = gz - R(2) * hz - mm * p.B[2];
+    out[o] = gx - R(2) * hx - mm * d.B0;
+    out[o + 1] = gy - R(2) * hy - mm * d.B1;
+    out[o + 2] = gz - R(2) * hz - mm * d.B2;
   }
 }
 
@@ -401,29 +434,29 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], int nmax_from, dou
 
 // Per-block partials of [E, sum e (3), sum s e (3), max |e x B_eff|] (spinmodel.py:475-484, 517-524;
 // llg.py:250-252).
-template <typename R, int CM>
-__global__ __launch_bounds__(BLOCK) void k_obs(const KP<R> p, const R* __restrict__ e, double* __restrict__ part) {
+template <typename R, int CM, bool PBC>
+__global__ __launch_bounds__(BLOCK) void k_obs(const Static<R>* __restrict__ S, const Dyn<R> d,
+                                               const R* __restrict__ e, double* __restrict__ part) {
   double v[MX_OBS_LEN] = {0, 0, 0, 0, 0, 0, 0, 0};
   Site st;
-  if (site_of(p, st)) {
+  if (site_of(S, st)) {
     const int i = st.i;
     const R x = e[3 * i], y = e[3 * i + 1], z = e[3 * i + 2];
     R hx, hy, hz, gx, gy, gz;
-    exchange<R, CM>(p, e, st, hx, hy, hz);
-    onsite_grad(p, st, x, y, z, gx, gy, gz);
-    const R im = inv_mu_of(p, st);
-    const R Eon = onsite_energy(p, st, x, y, z);
-    v[0] = double(Eon) - double(x * hx + y * hy + z * hz) - double(x * p.B[0] + y * p.B[1] + z * p.B[2]) / double(im);
+    exchange<R, CM, PBC>(S, d, e, st, hx, hy, hz);
+    onsite_grad(S, d, st, x, y, z, gx, gy, gz);
+    const R im = inv_mu_of(S, d, st);
+    const R Eon = onsite_energy(S, d, st, x, y, z);
+    v[0] = double(Eon) - double(x * hx + y * hy + z * hz) - double(x * d.B0 + y * d.B1 + z * d.B2) / double(im);
     v[1] = x;
     v[2] = y;
     v[3] = z;
-    const R sg = p.sign_site ? p.sign_site[i] : p.sign_sub[st.s];
+    const R sg = d.sign_site ? d.sign_site[i] : S->sign_sub[st.s];
     v[4] = sg * x;
     v[5] = sg * y;
     v[6] = sg * z;
-    if (!(p.pinned && p.pinned[i])) {
-      const R bx = (R(2) * hx - gx) * im + p.B[0], by = (R(2) * hy - gy) * im + p.B[1],
-              bz = (R(2) * hz - gz) * im + p.B[2];
+    if (!(d.pinned && d.pinned[i])) {
+      const R bx = (R(2) * hx - gx) * im + d.B0, by = (R(2) * hy - gy) * im + d.B1, bz = (R(2) * hz - gz) * im + d.B2;
       const double tx = y * bz - z * by, ty = z * bx - x * bz, tz = x * by - y * bx;
       v[7] = sqrt(tx * tx + ty * ty + tz * tz);
     }
@@ -450,14 +483,20 @@ __global__ __launch_bounds__(BLOCK) void k_final(const double* __restrict__ part
 
 // ------------------------------------------------------------------------------------------ misc
 template <typename R>
-__global__ __launch_bounds__(BLOCK) void k_noise(const KP<R> p, R* __restrict__ out, uint64_t step) {
+__global__ __launch_bounds__(BLOCK) void k_noise(int N, uint32_t seed_lo, uint32_t seed_hi, uint64_t step,
+                                                 R* __restrict__ out) {
   const int r = blockIdx.x * BLOCK + threadIdx.x;
-  if (r >= p.N) return;
+  if (r >= N) return;
   float n0, n1, n2;
-  gauss3(p.seed_lo, p.seed_hi, step, uint32_t(r), n0, n1, n2);
+  gauss3(seed_lo, seed_hi, step, uint32_t(r), n0, n1, n2);
   out[3 * r] = R(n0);
   out[3 * r + 1] = R(n1);
   out[3 * r + 2] = R(n2);
+}
+
+// Step counter of captured step loops: set before the first replay, advanced by each replay.
+__global__ void k_counter(uint64_t* __restrict__ c, uint64_t value, int add) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) c[0] = add ? c[0] + value : value;
 }
 
 // magex order (c * ns + s) <-> internal order (s * ncell + c)
@@ -515,97 +554,147 @@ static int check_launch(const char* what) {
 }
 
 template <typename R>
-void fill_kp(const Model& m, KP<R>& p) {
-  p.L0 = m.L[0];
-  p.L1 = m.L[1];
-  p.L2 = m.L[2];
-  p.ncell = m.ncell;
-  p.ns = m.ns;
-  p.N = m.N;
-  p.bps = (m.ncell + BLOCK - 1) / BLOCK;
-  p.nblk = p.bps * m.ns;
-  p.pbc0 = m.pbc[0];
-  p.pbc1 = m.pbc[1];
-  p.pbc2 = m.pbc[2];
-  for (int s = 0; s <= MAXSUB; ++s) p.sptr[s] = s <= m.ns ? m.sptr[s] : m.sptr[m.ns];
-  for (int k = 0; k < MAXENT; ++k) p.ent[k] = k < m.nent ? m.ent[k] : Ent{0, 0, 0, 0};
-  for (int k = 0; k < MAXTAB; ++k) p.tab[k] = k < int(m.table.size()) ? R(m.table[k]) : R(0);
-  p.bond = static_cast<const R*>(m.bond_d);
-  p.amode = m.amode;
-  p.has_A = m.has_A;
-  p.A = static_cast<const R*>(m.A_d);
-  for (int s = 0; s < MAXSUB; ++s)
-    for (int q = 0; q < 9; ++q) p.Asub[s][q] = s < m.ns ? R(m.Asub[9 * s + q]) : R(0);
-  p.has_c = m.has_c;
-  p.has_u = m.has_u;
-  p.Kc = static_cast<const R*>(m.Kc_d);
-  p.K2 = static_cast<const R*>(m.K2_d);
-  p.K3 = static_cast<const R*>(m.K3_d);
-  for (int q = 0; q < 9; ++q) p.C[q] = R(m.C[q]);
-  for (int q = 0; q < 3; ++q) p.u[q] = R(m.u[q]);
-  p.inv_mu = static_cast<const R*>(m.inv_mu_d);
-  p.alpha = static_cast<const R*>(m.alpha_d);
-  p.sigma = static_cast<const R*>(m.sigma_d);
-  p.sign_site = static_cast<const R*>(m.sign_d);
-  for (int s = 0; s < MAXSUB; ++s) {
-    const bool ok = s < m.ns;
-    p.inv_mu_sub[s] = ok ? R(1.0 / (m.mu_sub[s] * MU_B)) : R(0);
-    p.sign_sub[s] = ok ? R(m.sign_sub[s]) : R(0);
-    p.alpha_sub[s] = ok && !m.alpha_sub_v.empty() ? R(m.alpha_sub_v[s]) : R(0);
-    p.sigma_sub[s] = ok && !m.sigma_sub_v.empty() ? R(m.sigma_sub_v[s]) : R(0);
+static void fill_static(const Model& m, Static<R>& S) {
+  std::memset(&S, 0, sizeof(S));
+  S.L0 = m.L[0];
+  S.L1 = m.L[1];
+  S.L2 = m.L[2];
+  S.ncell = m.ncell;
+  S.ns = m.ns;
+  S.N = m.N;
+  S.bps = (m.ncell + BLOCK - 1) / BLOCK;
+  S.nblk = S.bps * m.ns;
+  S.pbc0 = m.pbc[0];
+  S.pbc1 = m.pbc[1];
+  S.pbc2 = m.pbc[2];
+  S.allpbc = m.pbc[0] && m.pbc[1] && m.pbc[2];
+  S.amode = m.amode;
+  S.has_A = m.has_A;
+  S.has_c = m.has_c;
+  S.has_u = m.has_u;
+  for (int s = 0; s <= MAXSUB; ++s) S.sptr[s] = s <= m.ns ? m.sptr[s] : m.sptr[m.ns];
+  for (int k = 0; k < m.nent; ++k) S.ent[k] = m.ent[k];
+  for (size_t k = 0; k < m.table.size() && k < size_t(MAXTAB); ++k) S.tab[k] = R(m.table[k]);
+  for (int s = 0; s < m.ns; ++s) {
+    for (int q = 0; q < 9; ++q) S.Asub[s][q] = R(m.Asub[9 * s + q]);
+    S.inv_mu_sub[s] = R(1.0 / (m.mu_sub[s] * MU_B));
+    S.sign_sub[s] = R(m.sign_sub[s]);
+    S.alpha_sub[s] = m.alpha_sub_v.empty() ? R(0) : R(m.alpha_sub_v[s]);
+    S.sigma_sub[s] = m.sigma_sub_v.empty() ? R(0) : R(m.sigma_sub_v[s]);
   }
-  p.pinned = static_cast<const uint8_t*>(m.pinned_d);
-  for (int q = 0; q < 3; ++q) p.B[q] = R(m.B[q]);
-  p.dt = R(m.dt);
-  p.prec = m.precession;
-  p.noisy = m.noisy;
-  p.seed_lo = p.seed_hi = 0;
-  p.noise = nullptr;
-}
-
-static_assert(sizeof(KP<double>) <= 4000, "kernel arguments must stay below 4 KB");
-
-template <typename R, int CM>
-static int heun_cm(const KP<R>& p, R* e, R* ep, R* k1, uint64_t step, hipStream_t s) {
-  hipLaunchKernelGGL((k_heun<R, CM, 1>), dim3(p.nblk), dim3(BLOCK), 0, s, p, e, ep, k1, step);
-  hipLaunchKernelGGL((k_heun<R, CM, 2>), dim3(p.nblk), dim3(BLOCK), 0, s, p, e, ep, k1, step);
-  return check_launch("heun step");
+  for (int q = 0; q < 9; ++q) S.C[q] = R(m.C[q]);
+  for (int q = 0; q < 3; ++q) S.u[q] = R(m.u[q]);
 }
 
 template <typename R>
-static int heun_t(const Model& m, void* e, uint64_t seed, uint64_t step, const void* noise, hipStream_t s) {
-  KP<R> p;
-  fill_kp(m, p);
-  p.seed_lo = uint32_t(seed);
-  p.seed_hi = uint32_t(seed >> 32);
-  p.noise = static_cast<const R*>(noise);
+static int upload_static_t(Model& m) {
+  Static<R> S;
+  fill_static(m, S);
+  if (!m.static_d) {
+    const hipError_t err = hipMalloc(&m.static_d, sizeof(Static<R>));
+    if (err != hipSuccess) {
+      set_error(std::string("hipMalloc model constants: ") + hipGetErrorString(err));
+      return MX_ENOMEM;
+    }
+  }
+  const hipError_t err = hipMemcpy(m.static_d, &S, sizeof(S), hipMemcpyHostToDevice);
+  if (err != hipSuccess) {
+    set_error(std::string("upload model constants: ") + hipGetErrorString(err));
+    return MX_EHIP;
+  }
+  m.static_dirty = 0;
+  return MX_OK;
+}
+
+int upload_static(Model& m) { return m.prec == MX_FP64 ? upload_static_t<double>(m) : upload_static_t<float>(m); }
+
+template <typename R>
+static Dyn<R> make_dyn(const Model& m, uint64_t seed) {
+  Dyn<R> d;
+  d.B0 = R(m.B[0]);
+  d.B1 = R(m.B[1]);
+  d.B2 = R(m.B[2]);
+  d.dt = R(m.dt);
+  d.prec = m.precession;
+  d.noisy = m.noisy;
+  d.seed_lo = uint32_t(seed);
+  d.seed_hi = uint32_t(seed >> 32);
+  d.bond = static_cast<const R*>(m.bond_d);
+  d.A = static_cast<const R*>(m.A_d);
+  d.Kc = static_cast<const R*>(m.Kc_d);
+  d.K2 = static_cast<const R*>(m.K2_d);
+  d.K3 = static_cast<const R*>(m.K3_d);
+  d.inv_mu = static_cast<const R*>(m.inv_mu_d);
+  d.alpha = static_cast<const R*>(m.alpha_d);
+  d.sigma = static_cast<const R*>(m.sigma_d);
+  d.sign_site = static_cast<const R*>(m.sign_d);
+  d.pinned = static_cast<const uint8_t*>(m.pinned_d);
+  d.step_ptr = nullptr;
+  return d;
+}
+
+static inline int nblk_of(const Model& m) { return m.ns * ((m.ncell + BLOCK - 1) / BLOCK); }
+
+// Dispatch a kernel template over (coupling mode, all-periodic) at run time.
+#define MX_DISPATCH(m, CALL)                                                        \
+  do {                                                                              \
+    const bool allp = (m).pbc[0] && (m).pbc[1] && (m).pbc[2];                       \
+    switch ((m).coup) {                                                             \
+      case COUP_TABLE1:                                                             \
+        if (allp) { CALL(COUP_TABLE1, true); } else { CALL(COUP_TABLE1, false); }   \
+        break;                                                                      \
+      case COUP_TABLE4:                                                             \
+        if (allp) { CALL(COUP_TABLE4, true); } else { CALL(COUP_TABLE4, false); }   \
+        break;                                                                      \
+      case COUP_TABLE9:                                                             \
+        if (allp) { CALL(COUP_TABLE9, true); } else { CALL(COUP_TABLE9, false); }   \
+        break;                                                                      \
+      default:                                                                      \
+        if (allp) { CALL(COUP_BOND, true); } else { CALL(COUP_BOND, false); }       \
+        break;                                                                      \
+    }                                                                               \
+  } while (0)
+
+template <typename R>
+static int heun_t(const Model& m, void* e, uint64_t seed, uint64_t step, const void* noise, hipStream_t s,
+                  const uint64_t* step_ptr) {
+  const Static<R>* S = static_cast<const Static<R>*>(m.static_d);
+  Dyn<R> d = make_dyn<R>(m, seed);
+  d.step_ptr = step_ptr;
   R* E = static_cast<R*>(e);
   R* ep = static_cast<R*>(m.ep_d);
   R* k1 = static_cast<R*>(m.k1_d);
-  switch (m.coup) {
-    case COUP_TABLE1: return heun_cm<R, COUP_TABLE1>(p, E, ep, k1, step, s);
-    case COUP_TABLE4: return heun_cm<R, COUP_TABLE4>(p, E, ep, k1, step, s);
-    case COUP_TABLE9: return heun_cm<R, COUP_TABLE9>(p, E, ep, k1, step, s);
-    default: return heun_cm<R, COUP_BOND>(p, E, ep, k1, step, s);
-  }
+  const R* nz = static_cast<const R*>(noise);
+  const dim3 g(nblk_of(m)), b(BLOCK);
+#define MX_HEUN_CALL(CM, PBC)                                                             \
+  hipLaunchKernelGGL((k_heun<R, CM, PBC, 1>), g, b, 0, s, S, d, E, ep, k1, step, nz); \
+  hipLaunchKernelGGL((k_heun<R, CM, PBC, 2>), g, b, 0, s, S, d, E, ep, k1, step, nz)
+  MX_DISPATCH(m, MX_HEUN_CALL);
+#undef MX_HEUN_CALL
+  return check_launch("heun step");
 }
 
-int launch_heun_step(const Model& m, void* e, uint64_t seed, uint64_t step, const void* noise, hipStream_t s) {
-  return m.prec == MX_FP64 ? heun_t<double>(m, e, seed, step, noise, s) : heun_t<float>(m, e, seed, step, noise, s);
+int launch_heun_step(const Model& m, void* e, uint64_t seed, uint64_t step, const void* noise, hipStream_t s,
+                     const uint64_t* step_ptr) {
+  return m.prec == MX_FP64 ? heun_t<double>(m, e, seed, step, noise, s, step_ptr)
+                           : heun_t<float>(m, e, seed, step, noise, s, step_ptr);
+}
+
+int launch_counter(uint64_t* c, uint64_t value, int add, hipStream_t s) {
+  hipLaunchKernelGGL(k_counter, dim3(1), dim3(64), 0, s, c, value, add);
+  return check_launch("step counter");
 }
 
 template <typename R>
 static int field_t(const Model& m, const void* e, void* out, int kind, hipStream_t s) {
-  KP<R> p;
-  fill_kp(m, p);
+  const Static<R>* S = static_cast<const Static<R>*>(m.static_d);
+  const Dyn<R> d = make_dyn<R>(m, 0);
   const R* E = static_cast<const R*>(e);
   R* O = static_cast<R*>(out);
-  switch (m.coup) {
-    case COUP_TABLE1: hipLaunchKernelGGL((k_field<R, COUP_TABLE1>), dim3(p.nblk), dim3(BLOCK), 0, s, p, E, O, kind); break;
-    case COUP_TABLE4: hipLaunchKernelGGL((k_field<R, COUP_TABLE4>), dim3(p.nblk), dim3(BLOCK), 0, s, p, E, O, kind); break;
-    case COUP_TABLE9: hipLaunchKernelGGL((k_field<R, COUP_TABLE9>), dim3(p.nblk), dim3(BLOCK), 0, s, p, E, O, kind); break;
-    default: hipLaunchKernelGGL((k_field<R, COUP_BOND>), dim3(p.nblk), dim3(BLOCK), 0, s, p, E, O, kind); break;
-  }
+  const dim3 g(nblk_of(m)), b(BLOCK);
+#define MX_FIELD_CALL(CM, PBC) hipLaunchKernelGGL((k_field<R, CM, PBC>), g, b, 0, s, S, d, E, O, kind)
+  MX_DISPATCH(m, MX_FIELD_CALL);
+#undef MX_FIELD_CALL
   return check_launch("field");
 }
 
@@ -615,16 +704,15 @@ int launch_field(const Model& m, const void* e, void* out, int kind, hipStream_t
 
 template <typename R>
 static int obs_t(const Model& m, const void* e, double* out, hipStream_t s) {
-  KP<R> p;
-  fill_kp(m, p);
+  const Static<R>* S = static_cast<const Static<R>*>(m.static_d);
+  const Dyn<R> d = make_dyn<R>(m, 0);
   const R* E = static_cast<const R*>(e);
-  switch (m.coup) {
-    case COUP_TABLE1: hipLaunchKernelGGL((k_obs<R, COUP_TABLE1>), dim3(p.nblk), dim3(BLOCK), 0, s, p, E, m.part_d); break;
-    case COUP_TABLE4: hipLaunchKernelGGL((k_obs<R, COUP_TABLE4>), dim3(p.nblk), dim3(BLOCK), 0, s, p, E, m.part_d); break;
-    case COUP_TABLE9: hipLaunchKernelGGL((k_obs<R, COUP_TABLE9>), dim3(p.nblk), dim3(BLOCK), 0, s, p, E, m.part_d); break;
-    default: hipLaunchKernelGGL((k_obs<R, COUP_BOND>), dim3(p.nblk), dim3(BLOCK), 0, s, p, E, m.part_d); break;
-  }
-  hipLaunchKernelGGL((k_final<MX_OBS_LEN>), dim3(1), dim3(BLOCK), 0, s, m.part_d, p.nblk, 7, 1.0, out);
+  const int nb = nblk_of(m);
+  const dim3 g(nb), b(BLOCK);
+#define MX_OBS_CALL(CM, PBC) hipLaunchKernelGGL((k_obs<R, CM, PBC>), g, b, 0, s, S, d, E, m.part_d)
+  MX_DISPATCH(m, MX_OBS_CALL);
+#undef MX_OBS_CALL
+  hipLaunchKernelGGL((k_final<MX_OBS_LEN>), dim3(1), dim3(BLOCK), 0, s, m.part_d, nb, 7, 1.0, out);
   return check_launch("observables");
 }
 
@@ -632,18 +720,15 @@ int launch_observables(const Model& m, const void* e, double* out, hipStream_t s
   return m.prec == MX_FP64 ? obs_t<double>(m, e, out, s) : obs_t<float>(m, e, out, s);
 }
 
-template <typename R>
-static int noise_t(const Model& m, void* out, uint64_t seed, uint64_t step, hipStream_t s) {
-  KP<R> p;
-  fill_kp(m, p);
-  p.seed_lo = uint32_t(seed);
-  p.seed_hi = uint32_t(seed >> 32);
-  hipLaunchKernelGGL((k_noise<R>), dim3((m.N + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, p, static_cast<R*>(out), step);
-  return check_launch("noise");
-}
-
 int launch_noise(const Model& m, void* out, uint64_t seed, uint64_t step, hipStream_t s) {
-  return m.prec == MX_FP64 ? noise_t<double>(m, out, seed, step, s) : noise_t<float>(m, out, seed, step, s);
+  const dim3 g((m.N + BLOCK - 1) / BLOCK);
+  if (m.prec == MX_FP64)
+    hipLaunchKernelGGL((k_noise<double>), g, dim3(BLOCK), 0, s, m.N, uint32_t(seed), uint32_t(seed >> 32), step,
+                       static_cast<double*>(out));
+  else
+    hipLaunchKernelGGL((k_noise<float>), g, dim3(BLOCK), 0, s, m.N, uint32_t(seed), uint32_t(seed >> 32), step,
+                       static_cast<float*>(out));
+  return check_launch("noise");
 }
 
 int launch_permute(const Model& m, const void* src, void* dst, int to_internal, hipStream_t s) {
@@ -684,8 +769,5 @@ int launch_topo(const Model& m, const void* e, const int32_t* tri, int64_t ntri,
   hipLaunchKernelGGL((k_final<1>), dim3(1), dim3(BLOCK), 0, s, m.part_d, nblk, 1, hand / (4.0 * M_PI), out);
   return check_launch("topological charge");
 }
-
-template void fill_kp<double>(const Model&, KP<double>&);
-template void fill_kp<float>(const Model&, KP<float>&);
 
 }  // namespace mx

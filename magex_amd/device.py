@@ -175,6 +175,10 @@ class DeviceModel:
                                              int(bool(precession)), _lib.MX_HEUN))
         self._dyn = key
 
+    def set_graph_steps(self, n):
+        """Steps per captured hipGraph step loop (0: launch every step directly)."""
+        check(self.lib.mx_model_set_graph_steps(self.h, int(n)))
+
     # ------------------------------------------------------------------ kernels
     def _check_spins(self, e):
         if e.shape != (self.N, 3) or e.dtype != self.dtype or not e.is_contiguous() or e.device.type != "cuda":
