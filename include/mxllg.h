@@ -99,7 +99,7 @@ int mx_llg_run(mx_model* m, void* e, int64_t nsteps, uint64_t seed, uint64_t ste
                void* stream);
 
 /* Steps per captured step loop (hipGraph replayed by mx_llg_run when no noise is injected and at least
- * two loops fit; default 32, 0 = always launch step by step).  Results are identical either way: the
+ * two loops fit; default 0 = always launch step by step).  Results are identical either way: the
  * captured kernels read the step number from a device counter, so the noise sequence is the same.
  * No counterpart in magex (its step loop is a Python for loop, llg.py:263-280). */
 int mx_model_set_graph_steps(mx_model* m, int32_t steps);

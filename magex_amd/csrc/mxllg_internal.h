@@ -112,7 +112,7 @@ struct Model {
   double* part_d = nullptr;
   int npart = 0;
   // captured step loop (hipGraph of graph_steps Heun steps + counter advance), keyed by what it bakes in
-  int graph_steps = 32;
+  int graph_steps = 0;
   uint64_t* counter_d = nullptr;
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t gexec = nullptr;

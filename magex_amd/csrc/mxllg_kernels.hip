@@ -92,72 +92,101 @@ __device__ __forceinline__ R rnorm3(R x, R y, R z) {
   return R(1) / sqrt(x * x + y * y + z * z);
 }
 
-// h_i = sum_j J_ij e_j over the stencil of site st (magex/fast.py:86-120 semantics).  PBC: every axis
-// periodic (no bounds test); otherwise a neighbour past an open face contributes zero, read through a
-// clamped (valid) index so the loop stays branch-free and unrollable.
+// Neighbour cell of st along stencil entry en (reduced offsets: one conditional wrap per periodic axis).
+// PBC: every axis periodic.  Otherwise ok = false past an open face and the returned cell is the
+// site's own (a valid address; its contribution is zeroed by the caller).
+template <typename R, bool PBC>
+__device__ __forceinline__ int neighbour_cell(const Static<R>* __restrict__ S, const Site& st, const Ent& en,
+                                              bool& ok) {
+  const int L0 = S->L0, L1 = S->L1, L2 = S->L2;
+  int n0 = st.c0 + int(int16_t(en.d01 & 0xffff));
+  int n1 = st.c1 + (en.d01 >> 16);
+  int n2 = st.c2 + en.d2;
+  if constexpr (PBC) {
+    n0 = n0 >= L0 ? n0 - L0 : n0;
+    n1 = n1 >= L1 ? n1 - L1 : n1;
+    n2 = n2 >= L2 ? n2 - L2 : n2;
+    ok = true;
+  } else {
+    n0 = (S->pbc0 && n0 >= L0) ? n0 - L0 : n0;
+    n1 = (S->pbc1 && n1 >= L1) ? n1 - L1 : n1;
+    n2 = (S->pbc2 && n2 >= L2) ? n2 - L2 : n2;
+    ok = unsigned(n0) < unsigned(L0) && unsigned(n1) < unsigned(L1) && unsigned(n2) < unsigned(L2);
+  }
+  return ok ? (n0 * L1 + n1) * L2 + n2 : st.c;
+}
+
+// Entries per batch of the stencil loop: every load of a batch is issued before the first is used, so
+// a wavefront keeps CH gathers (and, per bond, CH tensors) in flight instead of one.
+template <typename R, int CM>
+struct Chunk {
+  static constexpr int value = CM == COUP_BOND ? (sizeof(R) == 8 ? 3 : 5) : 8;
+};
+
+// h_i = sum_j J_ij e_j over the stencil of site st (magex/fast.py:86-120 semantics), accumulated in
+// stencil order.  The loop body has no branch: entries past the end of a batch re-read the last entry
+// and contribute zero, so the compiler issues a batch's loads back to back with one wait.
 template <typename R, int CM, bool PBC>
 __device__ __forceinline__ void exchange(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
                                          const Site& st, R& hx, R& hy, R& hz) {
+  constexpr int CH = Chunk<R, CM>::value;
+  constexpr int NJ = CM == COUP_BOND ? 9 : 1;
   hx = hy = hz = R(0);
   const int kb = S->sptr[st.s], ke = S->sptr[st.s + 1];
-  const int L0 = S->L0, L1 = S->L1, L2 = S->L2, ncell = S->ncell;
-  const bool p0 = S->pbc0, p1 = S->pbc1, p2 = S->pbc2;
-#pragma unroll 2
-  for (int k = kb; k < ke; ++k) {
-    const Ent en = S->ent[k];
-    int n0 = st.c0 + int(int16_t(en.d01 & 0xffff));
-    int n1 = st.c1 + (en.d01 >> 16);
-    int n2 = st.c2 + en.d2;
-    bool ok = true;
-    if constexpr (PBC) {
-      n0 = n0 >= L0 ? n0 - L0 : n0;
-      n1 = n1 >= L1 ? n1 - L1 : n1;
-      n2 = n2 >= L2 ? n2 - L2 : n2;
-    } else {
-      n0 = (p0 && n0 >= L0) ? n0 - L0 : n0;
-      n1 = (p1 && n1 >= L1) ? n1 - L1 : n1;
-      n2 = (p2 && n2 >= L2) ? n2 - L2 : n2;
-      ok = unsigned(n0) < unsigned(L0) && unsigned(n1) < unsigned(L1) && unsigned(n2) < unsigned(L2);
+  const int ncell = S->ncell;
+  for (int k0 = kb; k0 < ke; k0 += CH) {
+    R ex[CH], ey[CH], ez[CH], J[CH][NJ];
+    int slot[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const bool live = k0 + u < ke;
+      const Ent en = S->ent[live ? k0 + u : ke - 1];
+      slot[u] = en.slot;
+      bool ok;
+      const int cn = neighbour_cell<R, PBC>(S, st, en, ok);
+      ok = ok && live;
+      const int j = (en.tb & 0xffff) * ncell + cn;
+      ex[u] = e[3 * j];
+      ey[u] = e[3 * j + 1];
+      ez[u] = e[3 * j + 2];
+      ex[u] = ok ? ex[u] : R(0);
+      ey[u] = ok ? ey[u] : R(0);
+      ez[u] = ok ? ez[u] : R(0);
+      if constexpr (CM == COUP_BOND) {
+        // per-bond tensor stored once per unordered bond: a forward entry reads its own cell's copy,
+        // a backward entry its partner's copy at the neighbour cell, transposed (J_ji = J_ij^T)
+        const bool back = (en.tb >> 16) != 0;
+        const R* Jb = d.bond + (size_t(en.slot) * ncell + (back ? cn : st.c)) * 9;
+        R t[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) t[q] = Jb[q];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) J[u][q] = back ? t[(q % 3) * 3 + q / 3] : t[q];
+      }
     }
-    const int cn = ok ? (n0 * L1 + n1) * L2 + n2 : st.c;
-    const int j = (en.tb & 0xffff) * ncell + cn;
-    R ex = e[3 * j], ey = e[3 * j + 1], ez = e[3 * j + 2];
-    if constexpr (!PBC) {
-      ex = ok ? ex : R(0);
-      ey = ok ? ey : R(0);
-      ez = ok ? ez : R(0);
-    }
-    if constexpr (CM == COUP_TABLE1) {
-      const R J = S->tab[en.slot];
-      hx += J * ex;
-      hy += J * ey;
-      hz += J * ez;
-    } else if constexpr (CM == COUP_TABLE4) {
-      // J I + antisymmetric part: (J^A e_j) = e_j x D
-      const R* T = S->tab + 4 * en.slot;
-      hx += T[0] * ex + (ey * T[3] - ez * T[2]);
-      hy += T[0] * ey + (ez * T[1] - ex * T[3]);
-      hz += T[0] * ez + (ex * T[2] - ey * T[1]);
-    } else if constexpr (CM == COUP_TABLE9) {
-      const R* T = S->tab + 9 * en.slot;
-      hx += T[0] * ex + T[1] * ey + T[2] * ez;
-      hy += T[3] * ex + T[4] * ey + T[5] * ez;
-      hz += T[6] * ex + T[7] * ey + T[8] * ez;
-    } else {
-      // per-bond tensor stored once per unordered bond: forward entries read their own cell's copy,
-      // backward entries the partner's copy at the neighbour cell, transposed (J_ji = J_ij^T)
-      const bool back = (en.tb >> 16) != 0;
-      const R* Jb = d.bond + (size_t(en.slot) * ncell + (back ? cn : st.c)) * 9;
-      const R J0 = Jb[0], J1 = Jb[1], J2 = Jb[2], J3 = Jb[3], J4 = Jb[4], J5 = Jb[5], J6 = Jb[6], J7 = Jb[7],
-              J8 = Jb[8];
-      if (!back) {
-        hx += J0 * ex + J1 * ey + J2 * ez;
-        hy += J3 * ex + J4 * ey + J5 * ez;
-        hz += J6 * ex + J7 * ey + J8 * ez;
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const R x = ex[u], y = ey[u], z = ez[u];
+      if constexpr (CM == COUP_TABLE1) {
+        const R Jv = S->tab[slot[u]];
+        hx += Jv * x;
+        hy += Jv * y;
+        hz += Jv * z;
+      } else if constexpr (CM == COUP_TABLE4) {
+        // J I + antisymmetric part: (J^A e_j) = e_j x D
+        const R* T = S->tab + 4 * slot[u];
+        hx += T[0] * x + (y * T[3] - z * T[2]);
+        hy += T[0] * y + (z * T[1] - x * T[3]);
+        hz += T[0] * z + (x * T[2] - y * T[1]);
+      } else if constexpr (CM == COUP_TABLE9) {
+        const R* T = S->tab + 9 * slot[u];
+        hx += T[0] * x + T[1] * y + T[2] * z;
+        hy += T[3] * x + T[4] * y + T[5] * z;
+        hz += T[6] * x + T[7] * y + T[8] * z;
       } else {
-        hx += J0 * ex + J3 * ey + J6 * ez;
-        hy += J1 * ex + J4 * ey + J7 * ez;
-        hz += J2 * ex + J5 * ey + J8 * ez;
+        hx += J[u][0] * x + J[u][1] * y + J[u][2] * z;
+        hy += J[u][3] * x + J[u][4] * y + J[u][5] * z;
+        hz += J[u][6] * x + J[u][7] * y + J[u][8] * z;
       }
     }
   }
