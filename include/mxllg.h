@@ -51,6 +51,15 @@ int mx_model_create(int precision, int nsub, const int32_t L[3], const int32_t p
 void mx_model_destroy(mx_model* m);
 int mx_model_info(const mx_model* m, int64_t* n_sites, int32_t* precision, int32_t* nsub);
 
+/* Cell layout inside the library (the ABI's arrays are always in magex's site order): tiled = 1
+ * groups the cells of a fully periodic supercell into 3D tiles of 256 cells with power-of-two sides
+ * when the supercell allows it (each side divisible, at least 1024 cells); tiled = 0 (the default)
+ * keeps them row-major.  Results are identical; on MI355X the row-major order measured faster.
+ * Only callable before any per-site data is set.  mx_model_layout reports the tile sides (0 when
+ * row-major).  No counterpart in magex (its arrays are always row-major, spinmodel.py:197-206). */
+int mx_model_set_layout(mx_model* m, int tiled);
+int mx_model_layout(const mx_model* m, int32_t tile[3]);
+
 /* Exchange stencil: nent entries of 6 int32 (s, t, R0, R1, R2, type): the coupling J = table[type] acts
  * from site (c, s) on site (c + R, t) (spinmodel.py:214-241).  table: ntypes x width doubles, width 1
  * (J I), 4 (J I + DMI vector D, J^A_ab = eps_abc D_c; fast.py:104-111) or 9 (row-major 3x3).

@@ -30,6 +30,8 @@ SIGNATURES = {
     "mx_model_create": (_I, [_I, _I, _P, _P, _P]),
     "mx_model_destroy": (None, [_P]),
     "mx_model_info": (_I, [_P, _P, _P, _P]),
+    "mx_model_set_layout": (_I, [_P, _I]),
+    "mx_model_layout": (_I, [_P, _P]),
     "mx_model_set_stencil": (_I, [_P, ctypes.c_int32, _P, ctypes.c_int32, ctypes.c_int32, _P]),
     "mx_model_set_bond_tensors": (_I, [_P, _P]),
     "mx_model_set_onsite": (_I, [_P, _I, _P]),

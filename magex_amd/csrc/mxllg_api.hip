@@ -65,10 +65,61 @@ static void upload(const Model& m, void*& dst, const double* src, size_t n) {
   }
 }
 
-// magex order site r = c * ns + s -> internal order q = s * ncell + c
+// Internal cell of row-major cell c: itself, or its tile-major index in the TILED layout.
+static inline int internal_cell(const Model& m, int c) {
+  if (m.geo != GEO_TILED) return c;
+  const int c2 = c % m.L[2], t = c / m.L[2], c1 = t % m.L[1], c0 = t / m.L[1];
+  const int b0 = m.tb[0], b1 = m.tb[1], b2 = m.tb[2];
+  const int tile = ((c0 >> b0) * (m.L[1] >> b1) + (c1 >> b1)) * (m.L[2] >> b2) + (c2 >> b2);
+  const int l = ((c0 & ((1 << b0) - 1)) << (b1 + b2)) | ((c1 & ((1 << b1) - 1)) << b2) | (c2 & ((1 << b2) - 1));
+  return tile * BLOCK + l;
+}
+
+// magex order site r = c * ns + s -> internal order q = s * ncell + internal_cell(c)
 static inline int internal_index(const Model& m, int r) {
   const int c = r / m.ns, s = r - c * m.ns;
-  return s * m.ncell + c;
+  return s * m.ncell + internal_cell(m, c);
+}
+
+// Tile sides 2^tb[a] with tb0 + tb1 + tb2 = log2(BLOCK), each side dividing L; bits are dealt one at a
+// time to the axis with the fewest so far (ties to the faster axis), so tiles come out near-cubic.
+static bool choose_tiles(const Model& m, int tb[3]) {
+  if (!(m.pbc[0] && m.pbc[1] && m.pbc[2]) || m.ncell < 4 * BLOCK) return false;
+  int tz[3];
+  for (int a = 0; a < 3; ++a) {
+    tz[a] = 0;
+    while (tz[a] < 8 && (m.L[a] % (2 << tz[a])) == 0) ++tz[a];
+    tb[a] = 0;
+  }
+  int bits = 0;
+  while ((1 << bits) < BLOCK) ++bits;
+  for (int k = 0; k < bits; ++k) {
+    int best = -1;
+    for (int a = 2; a >= 0; --a)
+      if (tb[a] < tz[a] && (best < 0 || tb[a] < tb[best])) best = a;
+    if (best < 0) return false;
+    ++tb[best];
+  }
+  return true;
+}
+
+// Fix the cell layout and upload perm[q] = magex site of internal site q.
+static void set_layout(Model& m, int want_tiled) {
+  int tb[3];
+  const bool allp = m.pbc[0] && m.pbc[1] && m.pbc[2];
+  if (want_tiled && choose_tiles(m, tb)) {
+    m.geo = GEO_TILED;
+    for (int a = 0; a < 3; ++a) m.tb[a] = tb[a];
+  } else {
+    m.geo = allp ? GEO_PERIODIC : GEO_OPEN;
+    m.tb[0] = m.tb[1] = m.tb[2] = 0;
+  }
+  std::vector<int32_t> perm(m.N);
+  for (int r = 0; r < m.N; ++r) perm[internal_index(m, r)] = r;
+  if (!m.perm_d) hip_ok(hipMalloc(reinterpret_cast<void**>(&m.perm_d), sizeof(int32_t) * size_t(m.N)), "hipMalloc perm");
+  hip_ok(hipMemcpy(m.perm_d, perm.data(), sizeof(int32_t) * size_t(m.N), hipMemcpyHostToDevice), "hipMemcpy perm");
+  m.want_tiled = want_tiled;
+  m.static_dirty = 1;
 }
 
 // per-site magex-order array (N x comps, site-major) -> internal order, component-major (comps x N)
@@ -84,7 +135,7 @@ static std::vector<double> to_internal_soa(const Model& m, const double* src, in
 static void ensure_scratch(Model& m) {
   if (!m.ep_d) hip_ok(hipMalloc(&m.ep_d, size_t(m.N) * 3 * rsize(m)), "hipMalloc ep");
   if (!m.k1_d) hip_ok(hipMalloc(&m.k1_d, size_t(m.N) * 3 * rsize(m)), "hipMalloc k1");
-  if (m.ns > 1 && !m.eint_d) hip_ok(hipMalloc(&m.eint_d, size_t(m.N) * 3 * rsize(m)), "hipMalloc e");
+  if (needs_permute(m) && !m.eint_d) hip_ok(hipMalloc(&m.eint_d, size_t(m.N) * 3 * rsize(m)), "hipMalloc e");
   if (!m.part_d) {
     const int nblk = m.ns * ((m.ncell + BLOCK - 1) / BLOCK);
     m.npart = std::max(nblk, (2 * m.N + BLOCK - 1) / BLOCK + 1);
@@ -196,6 +247,12 @@ int mx_model_create(int precision, int nsub, const int32_t L[3], const int32_t p
     m->table.assign(1, 0.0);
     m->width = 1;
     m->ntypes = 1;
+    try {
+      set_layout(*m, 0);
+    } catch (...) {
+      mx_model_destroy(m);
+      throw;
+    }
     *out = m;
     return MX_OK;
   });
@@ -207,6 +264,7 @@ void mx_model_destroy(mx_model* m) {
   for (void** p : {&m->bond_d, &m->A_d, &m->Kc_d, &m->K2_d, &m->K3_d, &m->inv_mu_d, &m->sign_d, &m->pinned_d,
                    &m->alpha_d, &m->sigma_d, &m->ep_d, &m->k1_d, &m->eint_d, &m->static_d})
     dfree(*p);
+  if (m->perm_d) (void)hipFree(m->perm_d);
   if (m->part_d) (void)hipFree(m->part_d);
   drop_graph(*m);
   if (m->counter_d) (void)hipFree(m->counter_d);
@@ -234,11 +292,30 @@ int mx_model_set_stencil(mx_model* m, int32_t nent, const int32_t* ent, int32_t 
     if (ntypes < 1 || !table) fail(MX_EINVAL, "empty coupling table");
     if (int64_t(ntypes) * width > MAXTAB)
       fail(MX_EUNSUPPORTED, "coupling table larger than " + std::to_string(MAXTAB) + " values");
-    // sort entries by source sublattice (stable), validate, reduce offsets
+    // group entries by source sublattice; inside a group, bonds to other sublattices come first, ordered
+    // by their pair (the entry and its reverse partner share a key), so the two workgroups that read a
+    // per-bond tensor from either end reach it at the same point of their loops and the second read
+    // finds it in L2; same-sublattice bonds follow in the caller's order
+    std::vector<int> pair_key(nent);
+    for (int k = 0; k < nent; ++k) {
+      pair_key[k] = k;
+      for (int q = 0; q < nent; ++q)
+        if (ent[6 * q] == ent[6 * k + 1] && ent[6 * q + 1] == ent[6 * k] && ent[6 * q + 2] == -ent[6 * k + 2] &&
+            ent[6 * q + 3] == -ent[6 * k + 3] && ent[6 * q + 4] == -ent[6 * k + 4])
+          pair_key[k] = std::min(pair_key[k], q);
+    }
     std::vector<int> order;
-    for (int s = 0; s < m->ns; ++s)
+    for (int s = 0; s < m->ns; ++s) {
+      std::vector<int> grp;
       for (int k = 0; k < nent; ++k)
-        if (ent[6 * k] == s) order.push_back(k);
+        if (ent[6 * k] == s) grp.push_back(k);
+      std::stable_sort(grp.begin(), grp.end(), [&](int a, int b) {
+        const bool sa = ent[6 * a + 1] == s, sb = ent[6 * b + 1] == s;
+        if (sa != sb) return !sa;
+        return sa ? a < b : pair_key[a] < pair_key[b];
+      });
+      order.insert(order.end(), grp.begin(), grp.end());
+    }
     if (int(order.size()) != nent) fail(MX_EINVAL, "stencil source sublattice out of range");
     m->ent.clear();
     m->sptr.assign(m->ns + 1, 0);
@@ -282,6 +359,7 @@ int mx_model_set_stencil(mx_model* m, int32_t nent, const int32_t* ent, int32_t 
 int mx_model_set_bond_tensors(mx_model* m, const double* J) {
   return guarded([&]() -> int {
     need_model(m);
+    m->layout_frozen = 1;
     if (!J) fail(MX_EINVAL, "null tensors");
     if (m->nent == 0) fail(MX_EINVAL, "set the stencil first");
     use_device(*m);
@@ -301,7 +379,6 @@ int mx_model_set_bond_tensors(mx_model* m, const double* J) {
           partner[k] = q;
       if (partner[k] < 0) fail(MX_EINVAL, "stencil entry without a reverse partner: per-bond storage needs J_ji");
     }
-    // J arrives as [f][cell][9], the device layout: one contiguous row-major block per bond
     // entries (internal order) point at a forward slot: their own, or the partner's (back)
     for (size_t pos = 0; pos < m->ent.size(); ++pos) {
       const int k = m->ent_order[pos];
@@ -313,7 +390,15 @@ int mx_model_set_bond_tensors(mx_model* m, const double* J) {
         m->ent[pos].tb = (m->ent[pos].tb & 0xffff) | (1 << 16);
       }
     }
-    upload(*m, m->bond_d, J, size_t(nfwd) * m->ncell * 9);
+    // J arrives as [f][magex cell][9]; the device holds [f][internal cell][9]
+    std::vector<double> buf(size_t(nfwd) * m->ncell * 9);
+    for (int f = 0; f < nfwd; ++f)
+      for (int c = 0; c < m->ncell; ++c) {
+        const double* src = J + (size_t(f) * m->ncell + c) * 9;
+        double* dst = buf.data() + (size_t(f) * m->ncell + internal_cell(*m, c)) * 9;
+        for (int q = 0; q < 9; ++q) dst[q] = src[q];
+      }
+    upload(*m, m->bond_d, buf.data(), buf.size());
     m->nfwd = nfwd;
     m->coup = COUP_BOND;
     m->static_dirty = 1;
@@ -324,6 +409,7 @@ int mx_model_set_bond_tensors(mx_model* m, const double* J) {
 int mx_model_set_onsite(mx_model* m, int per_site, const double* A) {
   return guarded([&]() -> int {
     need_model(m);
+    m->layout_frozen = 1;
     use_device(*m);
     dfree(m->A_d);
     m->amode = ONSITE_SUB;
@@ -375,6 +461,7 @@ int mx_model_set_onsite(mx_model* m, int per_site, const double* A) {
 int mx_model_set_cubic(mx_model* m, const double* Kc, const double C[9]) {
   return guarded([&]() -> int {
     need_model(m);
+    m->layout_frozen = 1;
     use_device(*m);
     dfree(m->Kc_d);
     m->has_c = 0;
@@ -395,6 +482,7 @@ int mx_model_set_cubic(mx_model* m, const double* Kc, const double C[9]) {
 int mx_model_set_uniaxial_higher(mx_model* m, const double* K2, const double* K3, const double u[3]) {
   return guarded([&]() -> int {
     need_model(m);
+    m->layout_frozen = 1;
     use_device(*m);
     dfree(m->K2_d);
     dfree(m->K3_d);
@@ -419,6 +507,7 @@ int mx_model_set_uniaxial_higher(mx_model* m, const double* K2, const double* K3
 int mx_model_set_moments(mx_model* m, int per_site, const double* mu, const double* signs) {
   return guarded([&]() -> int {
     need_model(m);
+    m->layout_frozen = 1;
     if (!mu) fail(MX_EINVAL, "null moments");
     use_device(*m);
     dfree(m->inv_mu_d);
@@ -455,6 +544,7 @@ int mx_model_set_moments(mx_model* m, int per_site, const double* mu, const doub
 int mx_model_set_pinned(mx_model* m, const uint8_t* pinned) {
   return guarded([&]() -> int {
     need_model(m);
+    m->layout_frozen = 1;
     use_device(*m);
     dfree(m->pinned_d);
     if (!pinned) {
@@ -524,6 +614,7 @@ int mx_model_set_dynamics(mx_model* m, double dt, int alpha_per_site, const doub
                           const double* T, int precession, int integrator) {
   return guarded([&]() -> int {
     need_model(m);
+    m->layout_frozen = 1;
     if (!alpha || !T) fail(MX_EINVAL, "null damping or temperature");
     if (!(dt > 0)) fail(MX_EINVAL, "dt must be positive");
     if (integrator != MX_HEUN) fail(MX_EUNSUPPORTED, "only the Heun integrator is implemented");
@@ -539,6 +630,24 @@ int mx_model_set_dynamics(mx_model* m, double dt, int alpha_per_site, const doub
     m->dyn_set = 1;
     ensure_scratch(*m);
     m->static_dirty = 1;
+    return MX_OK;
+  });
+}
+
+int mx_model_set_layout(mx_model* m, int tiled) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (m->layout_frozen) fail(MX_EINVAL, "the cell layout is fixed once site data has been set");
+    use_device(*m);
+    set_layout(*m, tiled ? 1 : 0);
+    return MX_OK;
+  });
+}
+
+int mx_model_layout(const mx_model* m, int32_t tile[3]) {
+  return guarded([&]() -> int {
+    need_model(m);
+    for (int a = 0; a < 3; ++a) tile[a] = m->geo == GEO_TILED ? (1 << m->tb[a]) : 0;
     return MX_OK;
   });
 }
@@ -568,7 +677,7 @@ int mx_llg_run(mx_model* m, void* e, int64_t nsteps, uint64_t seed, uint64_t ste
     sync_static(*m);
     hipStream_t s = static_cast<hipStream_t>(stream);
     void* E = e;
-    if (m->ns > 1) {
+    if (needs_permute(*m)) {
       int rc = launch_permute(*m, e, m->eint_d, 1, s);
       if (rc) return rc;
       E = m->eint_d;
@@ -588,7 +697,7 @@ int mx_llg_run(mx_model* m, void* e, int64_t nsteps, uint64_t seed, uint64_t ste
       int rc = launch_heun_step(*m, E, seed, step0 + uint64_t(n), nz, s);
       if (rc) return rc;
     }
-    if (m->ns > 1) return launch_permute(*m, m->eint_d, e, 0, s);
+    if (needs_permute(*m)) return launch_permute(*m, m->eint_d, e, 0, s);
     return MX_OK;
   });
 }
@@ -604,7 +713,7 @@ int mx_noise(mx_model* m, void* out, uint64_t seed, uint64_t step, void* stream)
 
 static const void* internal_spins(mx_model* m, const void* e, hipStream_t s, int& rc) {
   rc = MX_OK;
-  if (m->ns == 1) return e;
+  if (!needs_permute(*m)) return e;
   rc = launch_permute(*m, e, m->eint_d, 1, s);
   return m->eint_d;
 }

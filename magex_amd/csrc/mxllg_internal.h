@@ -21,6 +21,9 @@ constexpr int MAXTAB = 256;  // coupling-table values (ntypes x width)
 constexpr int BLOCK = 256;   // threads per workgroup (4 wavefronts of 64)
 
 enum CoupMode { COUP_TABLE1 = 0, COUP_TABLE4 = 1, COUP_TABLE9 = 2, COUP_BOND = 3 };
+// Cell layouts: OPEN (row-major, some axis open), PERIODIC (row-major, all axes periodic), TILED (all
+// axes periodic, cells grouped in 3D tiles of BLOCK cells, power-of-two sides, one tile per workgroup).
+enum Geometry { GEO_OPEN = 0, GEO_PERIODIC = 1, GEO_TILED = 2 };
 enum OnsiteMode { ONSITE_SUB = 0, ONSITE_SITE6 = 1, ONSITE_SITE9 = 2 };
 
 // One stencil entry (16 bytes): slot = coupling slot (type-table row, or forward-bond slot),
@@ -37,6 +40,7 @@ template <typename R>
 struct Static {
   int L0, L1, L2, ncell, ns, N, bps, nblk;  // bps: workgroups per sublattice
   int pbc0, pbc1, pbc2, allpbc;
+  int tb0, tb1, tb2, nt1, nt2;  // TILED: log2 of the tile sides; tiles along axes 1 and 2
   int amode, has_A, has_c, has_u;
   int sptr[MAXSUB + 1];
   Ent ent[MAXENT];
@@ -71,6 +75,9 @@ struct Model {
   int prec;  // MX_FP64 / MX_FP32
   int device;
   int ns, L[3], pbc[3], ncell, N;
+  // cell layout: geo (Geometry), tile sides 2^tb; perm_d[q] = magex index of internal site q
+  int geo = GEO_PERIODIC, tb[3] = {0, 0, 0}, want_tiled = 1, layout_frozen = 0;
+  int32_t* perm_d = nullptr;
   // stencil
   int nent = 0;
   std::vector<int32_t> ent_raw;  // nent x 6 as given
@@ -128,6 +135,7 @@ int launch_field(const Model& m, const void* e_int, void* out_ref, int kind, hip
 int launch_observables(const Model& m, const void* e_int, double* out, hipStream_t s);
 int launch_noise(const Model& m, void* out_ref, uint64_t seed, uint64_t step, hipStream_t s);
 int launch_permute(const Model& m, const void* src, void* dst, int to_internal, hipStream_t s);
+bool needs_permute(const Model& m);
 int launch_rhs(const Model& m, const void* e, const void* B, double alpha, int prec, void* out, hipStream_t s);
 int launch_topo(const Model& m, const void* e_ref, const int32_t* tri, int64_t ntri, double hand, double* out,
                 hipStream_t s);

@@ -2,8 +2,11 @@
 // corrector, field, observables, counter-based thermal noise, topological charge.
 //
 // Data layout (HBM): spins are packed x,y,z per site (12 B FP32 / 24 B FP64: one dwordx3 load, every
-// fetched byte used), sites in sublattice-major order s * ncell + cell with cells row-major (c2
-// fastest).  Neighbour indices are never stored: a site's neighbours are cell offsets from the stencil
+// fetched byte used), sites in sublattice-major order s * ncell + cell.  Cells are row-major (c2
+// fastest) or, on fully periodic supercells whose sides allow it, grouped into 3D tiles of 256 cells
+// with power-of-two sides (4 x 8 x 8 on a 3D lattice, 16 x 16 in 2D), tile-major: one workgroup owns
+// one tile of one sublattice, so both ends of almost every bond are read by the same or adjacent
+// workgroups within a few microseconds.  Neighbour indices are never stored: a site's neighbours are cell offsets from the stencil
 // in the kernel arguments, wrapped on periodic axes.  Couplings are either a small type table (kernel
 // constant block, scalar loads) or per-bond tensors stored once per unordered bond as contiguous
 // row-major 3x3 blocks (bond[(f * ncell + cell) * 9 + q]: a wavefront reads 64 consecutive 36-B
@@ -27,10 +30,10 @@ __device__ __forceinline__ int xcd_tile(int b, int G) {
 }
 
 struct Site {
-  int s, c, c0, c1, c2, i;
+  int s, c, c0, c1, c2, i, crm;  // sublattice, internal cell, coordinates, internal site, magex cell
 };
 
-template <typename R>
+template <typename R, int GEO>
 __device__ __forceinline__ bool site_of(const Static<R>* __restrict__ S, Site& st) {
   const int b = xcd_tile(blockIdx.x, gridDim.x);
   const int ncell = S->ncell, L1 = S->L1, L2 = S->L2;
@@ -39,12 +42,24 @@ __device__ __forceinline__ bool site_of(const Static<R>* __restrict__ S, Site& s
   const int ns = S->ns;
   const int tile = b / ns;
   st.s = b - tile * ns;
-  st.c = tile * BLOCK + threadIdx.x;
-  if (st.c >= ncell) return false;
-  st.c2 = st.c % L2;
-  const int t = st.c / L2;
-  st.c1 = t % L1;
-  st.c0 = t / L1;
+  if constexpr (GEO == GEO_TILED) {
+    // tile coordinates are wavefront-uniform (scalar); the lane's place in the tile is shifts and masks
+    const int b0 = S->tb0, b1 = S->tb1, b2 = S->tb2;
+    const int t2 = tile % S->nt2, tq = tile / S->nt2, t1 = tq % S->nt1, t0 = tq / S->nt1;
+    const int l = threadIdx.x;
+    st.c0 = (t0 << b0) + (l >> (b1 + b2));
+    st.c1 = (t1 << b1) + ((l >> b2) & ((1 << b1) - 1));
+    st.c2 = (t2 << b2) + (l & ((1 << b2) - 1));
+    st.c = tile * BLOCK + l;
+  } else {
+    st.c = tile * BLOCK + threadIdx.x;
+    if (st.c >= ncell) return false;
+    st.c2 = st.c % L2;
+    const int t = st.c / L2;
+    st.c1 = t % L1;
+    st.c0 = t / L1;
+  }
+  st.crm = (st.c0 * L1 + st.c1) * L2 + st.c2;
   st.i = st.s * ncell + st.c;
   return true;
 }
@@ -92,28 +107,36 @@ __device__ __forceinline__ R rnorm3(R x, R y, R z) {
   return R(1) / sqrt(x * x + y * y + z * z);
 }
 
-// Neighbour cell of st along stencil entry en (reduced offsets: one conditional wrap per periodic axis).
-// PBC: every axis periodic.  Otherwise ok = false past an open face and the returned cell is the
-// site's own (a valid address; its contribution is zeroed by the caller).
-template <typename R, bool PBC>
+// Neighbour cell (internal index) of st along stencil entry en; offsets are reduced, so one conditional
+// wrap per periodic axis.  GEO_OPEN: ok = false past an open face, and the returned cell is the site's
+// own (a valid address; the caller zeroes its contribution).
+template <typename R, int GEO>
 __device__ __forceinline__ int neighbour_cell(const Static<R>* __restrict__ S, const Site& st, const Ent& en,
                                               bool& ok) {
   const int L0 = S->L0, L1 = S->L1, L2 = S->L2;
   int n0 = st.c0 + int(int16_t(en.d01 & 0xffff));
   int n1 = st.c1 + (en.d01 >> 16);
   int n2 = st.c2 + en.d2;
-  if constexpr (PBC) {
-    n0 = n0 >= L0 ? n0 - L0 : n0;
-    n1 = n1 >= L1 ? n1 - L1 : n1;
-    n2 = n2 >= L2 ? n2 - L2 : n2;
-    ok = true;
-  } else {
+  if constexpr (GEO == GEO_OPEN) {
     n0 = (S->pbc0 && n0 >= L0) ? n0 - L0 : n0;
     n1 = (S->pbc1 && n1 >= L1) ? n1 - L1 : n1;
     n2 = (S->pbc2 && n2 >= L2) ? n2 - L2 : n2;
     ok = unsigned(n0) < unsigned(L0) && unsigned(n1) < unsigned(L1) && unsigned(n2) < unsigned(L2);
+    return ok ? (n0 * L1 + n1) * L2 + n2 : st.c;
+  } else {
+    n0 = n0 >= L0 ? n0 - L0 : n0;
+    n1 = n1 >= L1 ? n1 - L1 : n1;
+    n2 = n2 >= L2 ? n2 - L2 : n2;
+    ok = true;
+    if constexpr (GEO == GEO_TILED) {
+      const int b0 = S->tb0, b1 = S->tb1, b2 = S->tb2;
+      const int tile = ((n0 >> b0) * S->nt1 + (n1 >> b1)) * S->nt2 + (n2 >> b2);
+      const int l = ((n0 & ((1 << b0) - 1)) << (b1 + b2)) | ((n1 & ((1 << b1) - 1)) << b2) | (n2 & ((1 << b2) - 1));
+      return tile * BLOCK + l;
+    } else {
+      return (n0 * L1 + n1) * L2 + n2;
+    }
   }
-  return ok ? (n0 * L1 + n1) * L2 + n2 : st.c;
 }
 
 // Entries per batch of the stencil loop: every load of a batch is issued before the first is used, so
@@ -126,7 +149,7 @@ struct Chunk {
 // h_i = sum_j J_ij e_j over the stencil of site st (magex/fast.py:86-120 semantics), accumulated in
 // stencil order.  The loop body has no branch: entries past the end of a batch re-read the last entry
 // and contribute zero, so the compiler issues a batch's loads back to back with one wait.
-template <typename R, int CM, bool PBC>
+template <typename R, int CM, int GEO>
 __device__ __forceinline__ void exchange(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
                                          const Site& st, R& hx, R& hy, R& hz) {
   constexpr int CH = Chunk<R, CM>::value;
@@ -143,7 +166,7 @@ __device__ __forceinline__ void exchange(const Static<R>* __restrict__ S, const 
       const Ent en = S->ent[live ? k0 + u : ke - 1];
       slot[u] = en.slot;
       bool ok;
-      const int cn = neighbour_cell<R, PBC>(S, st, en, ok);
+      const int cn = neighbour_cell<R, GEO>(S, st, en, ok);
       ok = ok && live;
       const int j = (en.tb & 0xffff) * ncell + cn;
       ex[u] = e[3 * j];
@@ -303,7 +326,7 @@ template <typename R>
 __device__ __forceinline__ void thermal(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st,
                                         uint64_t step, const R* __restrict__ noise, R& bx, R& by, R& bz) {
   const R sg = d.sigma ? d.sigma[st.i] : S->sigma_sub[st.s];
-  const int iref = st.c * S->ns + st.s;
+  const int iref = st.crm * S->ns + st.s;
   if (noise) {
     bx = sg * noise[3 * iref];
     by = sg * noise[3 * iref + 1];
@@ -323,12 +346,12 @@ __device__ __forceinline__ R inv_mu_of(const Static<R>* __restrict__ S, const Dy
 }
 
 // B_i = (2 h_i - g_i) / (mu_i mu_B) + B (+ thermal) at spin (x, y, z) (magex/fast.py:309-317).
-template <typename R, int CM, bool PBC>
+template <typename R, int CM, int GEO>
 __device__ __forceinline__ void local_field(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
                                             const Site& st, R x, R y, R z, uint64_t step,
                                             const R* __restrict__ noise, R& bx, R& by, R& bz) {
   R hx, hy, hz, gx, gy, gz;
-  exchange<R, CM, PBC>(S, d, e, st, hx, hy, hz);
+  exchange<R, CM, GEO>(S, d, e, st, hx, hy, hz);
   onsite_grad(S, d, st, x, y, z, gx, gy, gz);
   const R im = inv_mu_of(S, d, st);
   bx = (R(2) * hx - gx) * im + d.B0;
@@ -347,12 +370,12 @@ __device__ __forceinline__ void local_field(const Static<R>* __restrict__ S, con
 // PASS 1 (magex/fast.py:336-368): noise + field(e) + predictor ep = n(e + dt k1), keeps k1.
 // PASS 2 (magex/fast.py:370-390): field(ep) + corrector e = n(e + dt/2 (k1 + f(ep))), in place.
 // The noise of a step is regenerated in pass 2 from the same counter rather than stored.
-template <typename R, int CM, bool PBC, int PASS>
+template <typename R, int CM, int GEO, int PASS>
 __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S, const Dyn<R> d, R* __restrict__ e,
                                                 R* __restrict__ ep, R* __restrict__ k1, uint64_t step,
                                                 const R* __restrict__ noise) {
   Site st;
-  if (!site_of(S, st)) return;
+  if (!site_of<R, GEO>(S, st)) return;
   if (d.step_ptr) step += *d.step_ptr;  // graph replays: step = device counter + offset of this node
   const int i = st.i;
   const bool pin = d.pinned && d.pinned[i];
@@ -367,7 +390,7 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S,
       return;
     }
     R bx, by, bz, fx, fy, fz;
-    local_field<R, CM, PBC>(S, d, e, st, x, y, z, step, noise, bx, by, bz);
+    local_field<R, CM, GEO>(S, d, e, st, x, y, z, step, noise, bx, by, bz);
     llg_rhs(x, y, z, bx, by, bz, a, d.prec, fx, fy, fz);
     k1[3 * i] = fx;
     k1[3 * i + 1] = fy;
@@ -381,7 +404,7 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S,
     if (pin) return;
     const R x = ep[3 * i], y = ep[3 * i + 1], z = ep[3 * i + 2];
     R bx, by, bz, fx, fy, fz;
-    local_field<R, CM, PBC>(S, d, ep, st, x, y, z, step, noise, bx, by, bz);
+    local_field<R, CM, GEO>(S, d, ep, st, x, y, z, step, noise, bx, by, bz);
     llg_rhs(x, y, z, bx, by, bz, a, d.prec, fx, fy, fz);
     const R hdt = R(0.5) * d.dt;
     const R cx = e[3 * i] + hdt * (k1[3 * i] + fx);
@@ -396,14 +419,14 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S,
 
 // ------------------------------------------------------------------------------------------ field
 // kind: MX_FIELD_BEFF (T), MX_FIELD_GRADIENT (meV), MX_FIELD_EXCHANGE (meV); output in magex order.
-template <typename R, int CM, bool PBC>
+template <typename R, int CM, int GEO>
 __global__ __launch_bounds__(BLOCK) void k_field(const Static<R>* __restrict__ S, const Dyn<R> d,
                                                  const R* __restrict__ e, R* __restrict__ out, int kind) {
   Site st;
-  if (!site_of(S, st)) return;
-  const int i = st.i, o = 3 * (st.c * S->ns + st.s);
+  if (!site_of<R, GEO>(S, st)) return;
+  const int i = st.i, o = 3 * (st.crm * S->ns + st.s);
   R hx, hy, hz;
-  exchange<R, CM, PBC>(S, d, e, st, hx, hy, hz);
+  exchange<R, CM, GEO>(S, d, e, st, hx, hy, hz);
   if (kind == MX_FIELD_EXCHANGE) {
     out[o] = hx;
     out[o + 1] = hy;
@@ -463,16 +486,16 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], int nmax_from, dou
 
 // Per-block partials of [E, sum e (3), sum s e (3), max |e x B_eff|] (spinmodel.py:475-484, 517-524;
 // llg.py:250-252).
-template <typename R, int CM, bool PBC>
+template <typename R, int CM, int GEO>
 __global__ __launch_bounds__(BLOCK) void k_obs(const Static<R>* __restrict__ S, const Dyn<R> d,
                                                const R* __restrict__ e, double* __restrict__ part) {
   double v[MX_OBS_LEN] = {0, 0, 0, 0, 0, 0, 0, 0};
   Site st;
-  if (site_of(S, st)) {
+  if (site_of<R, GEO>(S, st)) {
     const int i = st.i;
     const R x = e[3 * i], y = e[3 * i + 1], z = e[3 * i + 2];
     R hx, hy, hz, gx, gy, gz;
-    exchange<R, CM, PBC>(S, d, e, st, hx, hy, hz);
+    exchange<R, CM, GEO>(S, d, e, st, hx, hy, hz);
     onsite_grad(S, d, st, x, y, z, gx, gy, gz);
     const R im = inv_mu_of(S, d, st);
     const R Eon = onsite_energy(S, d, st, x, y, z);
@@ -528,13 +551,13 @@ __global__ void k_counter(uint64_t* __restrict__ c, uint64_t value, int add) {
   if (threadIdx.x == 0 && blockIdx.x == 0) c[0] = add ? c[0] + value : value;
 }
 
-// magex order (c * ns + s) <-> internal order (s * ncell + c)
+// magex order <-> internal order through perm[q] = magex site of internal site q
 template <typename R>
-__global__ __launch_bounds__(BLOCK) void k_permute(const R* __restrict__ src, R* __restrict__ dst, int ncell, int ns,
-                                                   int to_internal) {
-  const int r = blockIdx.x * BLOCK + threadIdx.x;
-  if (r >= ncell * ns) return;
-  const int c = r / ns, s = r - c * ns, q = s * ncell + c;
+__global__ __launch_bounds__(BLOCK) void k_permute(const R* __restrict__ src, R* __restrict__ dst,
+                                                   const int32_t* __restrict__ perm, int n, int to_internal) {
+  const int q = blockIdx.x * BLOCK + threadIdx.x;
+  if (q >= n) return;
+  const int r = perm[q];
   const int a = to_internal ? r : q, b = to_internal ? q : r;
   dst[3 * b] = src[3 * a];
   dst[3 * b + 1] = src[3 * a + 1];
@@ -597,6 +620,11 @@ static void fill_static(const Model& m, Static<R>& S) {
   S.pbc1 = m.pbc[1];
   S.pbc2 = m.pbc[2];
   S.allpbc = m.pbc[0] && m.pbc[1] && m.pbc[2];
+  S.tb0 = m.tb[0];
+  S.tb1 = m.tb[1];
+  S.tb2 = m.tb[2];
+  S.nt1 = m.L[1] >> m.tb[1];
+  S.nt2 = m.L[2] >> m.tb[2];
   S.amode = m.amode;
   S.has_A = m.has_A;
   S.has_c = m.has_c;
@@ -664,24 +692,21 @@ static Dyn<R> make_dyn(const Model& m, uint64_t seed) {
 
 static inline int nblk_of(const Model& m) { return m.ns * ((m.ncell + BLOCK - 1) / BLOCK); }
 
-// Dispatch a kernel template over (coupling mode, all-periodic) at run time.
-#define MX_DISPATCH(m, CALL)                                                        \
-  do {                                                                              \
-    const bool allp = (m).pbc[0] && (m).pbc[1] && (m).pbc[2];                       \
-    switch ((m).coup) {                                                             \
-      case COUP_TABLE1:                                                             \
-        if (allp) { CALL(COUP_TABLE1, true); } else { CALL(COUP_TABLE1, false); }   \
-        break;                                                                      \
-      case COUP_TABLE4:                                                             \
-        if (allp) { CALL(COUP_TABLE4, true); } else { CALL(COUP_TABLE4, false); }   \
-        break;                                                                      \
-      case COUP_TABLE9:                                                             \
-        if (allp) { CALL(COUP_TABLE9, true); } else { CALL(COUP_TABLE9, false); }   \
-        break;                                                                      \
-      default:                                                                      \
-        if (allp) { CALL(COUP_BOND, true); } else { CALL(COUP_BOND, false); }       \
-        break;                                                                      \
-    }                                                                               \
+// Dispatch a kernel template over (coupling mode, geometry) at run time.
+#define MX_GEO(m, CALL, CM)                                       \
+  do {                                                            \
+    if ((m).geo == GEO_TILED) { CALL(CM, GEO_TILED); }            \
+    else if ((m).geo == GEO_PERIODIC) { CALL(CM, GEO_PERIODIC); } \
+    else { CALL(CM, GEO_OPEN); }                                  \
+  } while (0)
+#define MX_DISPATCH(m, CALL)                                      \
+  do {                                                            \
+    switch ((m).coup) {                                           \
+      case COUP_TABLE1: MX_GEO(m, CALL, COUP_TABLE1); break;      \
+      case COUP_TABLE4: MX_GEO(m, CALL, COUP_TABLE4); break;      \
+      case COUP_TABLE9: MX_GEO(m, CALL, COUP_TABLE9); break;      \
+      default: MX_GEO(m, CALL, COUP_BOND); break;                 \
+    }                                                             \
   } while (0)
 
 template <typename R>
@@ -695,9 +720,9 @@ static int heun_t(const Model& m, void* e, uint64_t seed, uint64_t step, const v
   R* k1 = static_cast<R*>(m.k1_d);
   const R* nz = static_cast<const R*>(noise);
   const dim3 g(nblk_of(m)), b(BLOCK);
-#define MX_HEUN_CALL(CM, PBC)                                                             \
-  hipLaunchKernelGGL((k_heun<R, CM, PBC, 1>), g, b, 0, s, S, d, E, ep, k1, step, nz); \
-  hipLaunchKernelGGL((k_heun<R, CM, PBC, 2>), g, b, 0, s, S, d, E, ep, k1, step, nz)
+#define MX_HEUN_CALL(CM, GEO)                                                             \
+  hipLaunchKernelGGL((k_heun<R, CM, GEO, 1>), g, b, 0, s, S, d, E, ep, k1, step, nz); \
+  hipLaunchKernelGGL((k_heun<R, CM, GEO, 2>), g, b, 0, s, S, d, E, ep, k1, step, nz)
   MX_DISPATCH(m, MX_HEUN_CALL);
 #undef MX_HEUN_CALL
   return check_launch("heun step");
@@ -721,7 +746,7 @@ static int field_t(const Model& m, const void* e, void* out, int kind, hipStream
   const R* E = static_cast<const R*>(e);
   R* O = static_cast<R*>(out);
   const dim3 g(nblk_of(m)), b(BLOCK);
-#define MX_FIELD_CALL(CM, PBC) hipLaunchKernelGGL((k_field<R, CM, PBC>), g, b, 0, s, S, d, E, O, kind)
+#define MX_FIELD_CALL(CM, GEO) hipLaunchKernelGGL((k_field<R, CM, GEO>), g, b, 0, s, S, d, E, O, kind)
   MX_DISPATCH(m, MX_FIELD_CALL);
 #undef MX_FIELD_CALL
   return check_launch("field");
@@ -738,7 +763,7 @@ static int obs_t(const Model& m, const void* e, double* out, hipStream_t s) {
   const R* E = static_cast<const R*>(e);
   const int nb = nblk_of(m);
   const dim3 g(nb), b(BLOCK);
-#define MX_OBS_CALL(CM, PBC) hipLaunchKernelGGL((k_obs<R, CM, PBC>), g, b, 0, s, S, d, E, m.part_d)
+#define MX_OBS_CALL(CM, GEO) hipLaunchKernelGGL((k_obs<R, CM, GEO>), g, b, 0, s, S, d, E, m.part_d)
   MX_DISPATCH(m, MX_OBS_CALL);
 #undef MX_OBS_CALL
   hipLaunchKernelGGL((k_final<MX_OBS_LEN>), dim3(1), dim3(BLOCK), 0, s, m.part_d, nb, 7, 1.0, out);
@@ -760,14 +785,16 @@ int launch_noise(const Model& m, void* out, uint64_t seed, uint64_t step, hipStr
   return check_launch("noise");
 }
 
+bool needs_permute(const Model& m) { return m.ns > 1 || m.geo == GEO_TILED; }
+
 int launch_permute(const Model& m, const void* src, void* dst, int to_internal, hipStream_t s) {
   const dim3 g((m.N + BLOCK - 1) / BLOCK);
   if (m.prec == MX_FP64)
     hipLaunchKernelGGL((k_permute<double>), g, dim3(BLOCK), 0, s, static_cast<const double*>(src),
-                       static_cast<double*>(dst), m.ncell, m.ns, to_internal);
+                       static_cast<double*>(dst), m.perm_d, m.N, to_internal);
   else
     hipLaunchKernelGGL((k_permute<float>), g, dim3(BLOCK), 0, s, static_cast<const float*>(src),
-                       static_cast<float*>(dst), m.ncell, m.ns, to_internal);
+                       static_cast<float*>(dst), m.perm_d, m.N, to_internal);
   return check_launch("permute");
 }
 

@@ -9,6 +9,7 @@ library's.
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
@@ -72,13 +73,17 @@ def coupling_table(tensors):
     return np.ascontiguousarray(table, float), np.asarray(typ, np.int32)
 
 
-MAXTAB = 160   # coupling-table values the kernels hold in their arguments (mxllg_internal.h)
+MAXTAB = 256   # coupling-table values the kernels hold in their constant block (mxllg_internal.h)
+
+# Cell layout inside the library: "tiled" (3D tiles of 256 cells where the supercell allows it) or
+# "rowmajor".  Results are identical; only the memory access pattern changes.
+DEFAULT_LAYOUT = os.environ.get("MAGEX_AMD_LAYOUT", "rowmajor")
 
 
 class DeviceModel:
     """libmxllg model of a SpinSystem in precision "fp64" or "fp32"."""
 
-    def __init__(self, system, precision="fp64"):
+    def __init__(self, system, precision="fp64", tiled=None):
         from .terms import CubicAnisotropy, UniaxialHigher
         torch = _torch()
         if precision not in ("fp64", "fp32"):
@@ -96,6 +101,10 @@ class DeviceModel:
                                 hptr(np.array(system.L, np.int32)), hptr(np.array(system.pbc, np.int32)),
                                 ctypes.byref(h)))
         self.h = h
+        if tiled is None:
+            tiled = DEFAULT_LAYOUT == "tiled"
+        if not tiled:
+            check(L.mx_model_set_layout(h, 0))
         # exchange stencil + coupling table (or per-bond tensors)
         st = system.stencil
         table, typ = coupling_table([J for *_, J in st])
@@ -174,6 +183,12 @@ class DeviceModel:
         check(self.lib.mx_model_set_dynamics(self.h, float(dt), int(a.size > 1), hptr(a), int(t.size > 1), hptr(t),
                                              int(bool(precession)), _lib.MX_HEUN))
         self._dyn = key
+
+    def layout(self):
+        """Tile sides of the library's cell layout ((0, 0, 0): row-major)."""
+        t = np.zeros(3, np.int32)
+        check(self.lib.mx_model_layout(self.h, hptr(t)))
+        return tuple(int(x) for x in t)
 
     def set_graph_steps(self, n):
         """Steps per captured hipGraph step loop (0: launch every step directly)."""
