@@ -111,6 +111,43 @@ def cpu_baseline(system, p, seconds):
                        f"threads (magex's fused two-pass Heun, per-bond 3x3 blocks, ziggurat noise)")
 
 
+def batch_rates(steps, per_replica_steps):
+    """The batched form on 96 triangular replicas with L uniform in {32, 48, ..., 192} (seeded), NN J=1,
+    K=0.1 meV, B=0.5 T, a geometric 5-300 K ladder, FP32: one launch pair per step for all replicas
+    (autotuned shape) against the same replicas run as separate plain simulations."""
+    import numpy as np
+    import torch
+    from magex_amd import configs
+    from magex_amd.batch import ReplicaBatch, per_replica_rate
+    systems, p = configs.batch96()
+    T, seeds = p["temperatures"], p["seeds"]
+    alpha = np.full(len(systems), p["alpha"])
+    spins = [s.init_spins("random", seed=seeds[k]) for k, s in enumerate(systems)]
+    batch = ReplicaBatch(systems, alpha, p["dt"], T, seeds, "fp32")
+    e = batch.pack(spins)
+    shape, _, _ = batch.autotune(e, steps=8)
+    batch.run(e, 5)
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    w0 = time.perf_counter()
+    t0.record()
+    batch.run(e, steps)
+    t1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    total = int(batch.offsets[-1])
+    rate = total * steps / wall
+    base, _ = per_replica_rate(systems, alpha, p["dt"], T, seeds, spins, per_replica_steps)
+    m = batch.magnetization(e)
+    return {"workload": "96 triangular replicas, L in {32..192} (seeded), J=1 K=0.1 meV B=0.5 T, T 5-300 K, "
+                        "alpha=0.1, dt=0.1 fs, FP32", "spins": total, "steps": steps,
+            "spin_steps_per_s": rate, "ms_per_step": 1e3 * wall / steps,
+            "event_ms_per_step": t0.elapsed_time(t1) / steps, "launch": {"block": shape[0],
+            "sites_per_thread": shape[1], "graph_steps": shape[2]},
+            "per_replica_spin_steps_per_s": base, "per_replica_steps": per_replica_steps,
+            "speedup_vs_per_replica": rate / base, "mean_abs_m_range": [float(m.min()), float(m.max())]}
+
+
 def traffic_from_profiles(workload_name, precision):
     """HBM bytes per step measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE), if committed."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload_name}_{precision}.txt")
@@ -130,6 +167,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=300)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 256 MiB device copy first (the reference kernel for rocprof FETCH/WRITE_SIZE)")
     a = ap.parse_args()
@@ -168,6 +207,9 @@ def main():
     if r2 is not None:
         out[other] = {"spin_steps_per_s": N / (r2["step_ms"] * 1e-3), "ms_per_step": r2["step_ms"],
                       "roofline_frac": alg_bytes(system, other) * N / (r2["step_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    if not a.no_batch:
+        print("batched replicas ...", file=sys.stderr, flush=True)
+        out["batched"] = batch_rates(a.batch_steps, max(20, a.batch_steps // 10))
     if not a.no_cpu_baseline:
         print("cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(system, p, a.cpu_seconds)

@@ -136,6 +136,33 @@ int mx_rhs(mx_model* m, const void* e, const void* B, double alpha, int precessi
 int mx_topo_charge(mx_model* m, const void* e, const int32_t* tri, int64_t ntri, double hand, double* out,
                    void* stream);
 
+/* ---------------------------------------------------------------------------------------------
+ * Batched form: nrep independent replicas of one unit-cell model, each on its own periodic supercell
+ * and with its own damping, temperature and noise seed, advanced together (one launch per Heun pass
+ * for the whole batch).  No counterpart in magex: its closest analogue is a Python loop over replicas
+ * (advanced_mc.parallel_tempering, advanced_mc.py:38-94; GNEB.forces, gneb.py:171-187).
+ * Replica r's spins are N_r x 3 values at site offset off[r] of one packed device array, each
+ * replica in magex's site order; replica r's noise is the stream mx_llg_run draws with seed seeds[r]
+ * on its own supercell, so a batched run equals nrep separate runs bit for bit. */
+typedef struct mx_batch mx_batch;
+
+/* Replicas of proto's unit cell (stencil and coupling table, per-sublattice on-site tensor, moments,
+ * field, precision) on supercells L (nrep x 3 int32, all axes periodic, each side longer than the
+ * stencil's reach).  Per-bond tensors and per-site arrays of proto are not carried over. */
+int mx_batch_create(const mx_model* proto, int32_t nrep, const int32_t* L, mx_batch** out);
+void mx_batch_destroy(mx_batch* b);
+/* off: nrep + 1 site offsets (off[nrep] = total sites). */
+int mx_batch_offsets(const mx_batch* b, int64_t* off);
+/* dt (ps); alpha and T (K): nrep values each; precession 0/1. */
+int mx_batch_set_dynamics(mx_batch* b, double dt, const double* alpha, const double* T, int precession);
+/* Launch shape: threads per workgroup (64, 128, 256 or 512), sites per thread (1 to 8), and steps per
+ * captured step loop (0 = direct launches).  Results do not depend on the shape. */
+int mx_batch_set_launch(mx_batch* b, int32_t block, int32_t sites_per_thread, int32_t graph_steps);
+/* nsteps Heun steps of every replica on the packed spins e (device, in place); seeds: nrep keys. */
+int mx_batch_run(mx_batch* b, void* e, int64_t nsteps, const uint64_t* seeds, uint64_t step0, void* stream);
+/* Per-replica observables (device, nrep x MX_OBS_LEN doubles, layout as mx_observables). */
+int mx_batch_observables(mx_batch* b, const void* e, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -48,6 +48,13 @@ SIGNATURES = {
     "mx_observables": (_I, [_P, _P, _P, _P]),
     "mx_rhs": (_I, [_P, _P, _P, _D, _I, _P, _P]),
     "mx_topo_charge": (_I, [_P, _P, _P, _I64, _D, _P, _P]),
+    "mx_batch_create": (_I, [_P, ctypes.c_int32, _P, _P]),
+    "mx_batch_destroy": (None, [_P]),
+    "mx_batch_offsets": (_I, [_P, _P]),
+    "mx_batch_set_dynamics": (_I, [_P, _D, _P, _P, _I]),
+    "mx_batch_set_launch": (_I, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "mx_batch_run": (_I, [_P, _P, _I64, _P, _U64, _P]),
+    "mx_batch_observables": (_I, [_P, _P, _P, _P]),
 }
 
 _lib = None

@@ -206,8 +206,6 @@ static void ensure_graph(Model& m, void* E, uint64_t seed) {
 
 }  // namespace mx
 
-struct mx_model : mx::Model {};
-
 using namespace mx;
 
 extern "C" {

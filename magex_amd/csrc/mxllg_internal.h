@@ -143,3 +143,5 @@ int launch_topo(const Model& m, const void* e_ref, const int32_t* tri, int64_t n
 void set_error(const std::string& msg);
 
 }  // namespace mx
+
+struct mx_model : mx::Model {};

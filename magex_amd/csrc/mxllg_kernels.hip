@@ -19,16 +19,11 @@
 // lattice and the neighbour re-reads of that slab hit its own L2.
 #include <cstring>
 
-#include "mxllg_internal.h"
+#include "mxllg_device.h"
 
 namespace mx {
 
 // ------------------------------------------------------------------------------------------ helpers
-__device__ __forceinline__ int xcd_tile(int b, int G) {
-  const int x = b & 7, q = b >> 3, per = G >> 3, rem = G & 7;
-  return x * per + min(x, rem) + q;
-}
-
 struct Site {
   int s, c, c0, c1, c2, i, crm;  // sublattice, internal cell, coordinates, internal site, magex cell
 };
@@ -62,49 +57,6 @@ __device__ __forceinline__ bool site_of(const Static<R>* __restrict__ S, Site& s
   st.crm = (st.c0 * L1 + st.c1) * L2 + st.c2;
   st.i = st.s * ncell + st.c;
   return true;
-}
-
-// Philox4x32-10 (Salmon et al., SC'11): counter (c0..c3), key (k0, k1).
-__device__ __forceinline__ void philox10(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3, uint32_t k0,
-                                         uint32_t k1) {
-#pragma unroll
-  for (int r = 0; r < 10; ++r) {
-    const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-    const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
-    const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
-    c0 = n0;
-    c1 = lo1;
-    c2 = n2;
-    c3 = lo0;
-    k0 += 0x9E3779B9u;
-    k1 += 0xBB67AE85u;
-  }
-}
-
-// Three standard normals for (seed, step, site): 23-bit uniforms u = (2 (w >> 9) + 1) 2^-24 in (0, 1),
-// exact in float; Box-Muller with the hardware log2 / sin / cos (v_sin_f32 and v_cos_f32 take their
-// argument in revolutions, so cos(2 pi u) is one instruction).  Computed in float in both precisions,
-// so FP32 and FP64 runs see identical noise.
-__device__ __forceinline__ void gauss3(uint32_t k0, uint32_t k1, uint64_t step, uint32_t site, float& n0, float& n1,
-                                       float& n2) {
-  uint32_t c0 = site, c1 = uint32_t(step), c2 = uint32_t(step >> 32), c3 = 0u;
-  philox10(c0, c1, c2, c3, k0, k1);
-  const float s24 = 5.9604644775390625e-8f;  // 2^-24
-  const float u0 = float(2u * (c0 >> 9) + 1u) * s24;
-  const float u1 = float(2u * (c1 >> 9) + 1u) * s24;
-  const float u2 = float(2u * (c2 >> 9) + 1u) * s24;
-  const float u3 = float(2u * (c3 >> 9) + 1u) * s24;
-  const float m2ln2 = -1.3862943611198906f;  // -2 ln 2: -2 ln u = -2 ln2 log2 u
-  const float r0 = __builtin_sqrtf(fmaxf(m2ln2 * __builtin_amdgcn_logf(u0), 0.0f));
-  const float r2 = __builtin_sqrtf(fmaxf(m2ln2 * __builtin_amdgcn_logf(u2), 0.0f));
-  n0 = r0 * __builtin_amdgcn_cosf(u1);
-  n1 = r0 * __builtin_amdgcn_sinf(u1);
-  n2 = r2 * __builtin_amdgcn_cosf(u3);
-}
-
-template <typename R>
-__device__ __forceinline__ R rnorm3(R x, R y, R z) {
-  return R(1) / sqrt(x * x + y * y + z * z);
 }
 
 // Neighbour cell (internal index) of st along stencil entry en; offsets are reduced, so one conditional
@@ -305,23 +257,6 @@ __device__ __forceinline__ R onsite_energy(const Static<R>* __restrict__ S, cons
   return E;
 }
 
-// de/dt = -gamma/(1+a^2) [e x B + a e x (e x B)] (magex/llg.py:21-28, fast.py:225-236).
-template <typename R>
-__device__ __forceinline__ void llg_rhs(R ex, R ey, R ez, R bx, R by, R bz, R a, int prec, R& fx, R& fy, R& fz) {
-  const R cx = ey * bz - ez * by, cy = ez * bx - ex * bz, cz = ex * by - ey * bx;
-  const R dx = ey * cz - ez * cy, dy = ez * cx - ex * cz, dz = ex * cy - ey * cx;
-  const R pre = R(-GAMMA) / (R(1) + a * a);
-  if (prec) {
-    fx = pre * (cx + a * dx);
-    fy = pre * (cy + a * dy);
-    fz = pre * (cz + a * dz);
-  } else {
-    fx = pre * a * dx;
-    fy = pre * a * dy;
-    fz = pre * a * dz;
-  }
-}
-
 template <typename R>
 __device__ __forceinline__ void thermal(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st,
                                         uint64_t step, const R* __restrict__ noise, R& bx, R& by, R& bz) {
@@ -453,37 +388,6 @@ __global__ __launch_bounds__(BLOCK) void k_field(const Static<R>* __restrict__ S
 }
 
 // ------------------------------------------------------------------------------------------ reductions
-// Wavefront (64-lane) sum / max by xor shuffles, then across the 4 wavefronts of the block in LDS,
-// in a fixed order (bitwise reproducible).
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
-}
-
-template <int NV>
-__device__ __forceinline__ void block_reduce(double (&v)[NV], int nmax_from, double* __restrict__ dst) {
-  __shared__ double red[BLOCK / 64][NV];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < NV; ++k) v[k] = (k >= nmax_from) ? wave_max(v[k]) : wave_sum(v[k]);
-  if (lane == 0)
-#pragma unroll
-    for (int k = 0; k < NV; ++k) red[w][k] = v[k];
-  __syncthreads();
-  if (threadIdx.x < NV) {
-    const int k = threadIdx.x;
-    double a = red[0][k];
-    for (int q = 1; q < BLOCK / 64; ++q) a = (k >= nmax_from) ? fmax(a, red[q][k]) : a + red[q][k];
-    dst[k] = a;
-  }
-}
-
 // Per-block partials of [E, sum e (3), sum s e (3), max |e x B_eff|] (spinmodel.py:475-484, 517-524;
 // llg.py:250-252).
 template <typename R, int CM, int GEO>
@@ -514,23 +418,6 @@ __global__ __launch_bounds__(BLOCK) void k_obs(const Static<R>* __restrict__ S, 
     }
   }
   block_reduce<MX_OBS_LEN>(v, 7, part + size_t(blockIdx.x) * MX_OBS_LEN);
-}
-
-// Second stage: one workgroup folds the per-block partials in a fixed order, then scales them.
-template <int NV>
-__global__ __launch_bounds__(BLOCK) void k_final(const double* __restrict__ part, int nblk, int nmax_from,
-                                                 double scale, double* __restrict__ out) {
-  __shared__ double res[NV];
-  double v[NV];
-#pragma unroll
-  for (int k = 0; k < NV; ++k) v[k] = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += BLOCK)
-#pragma unroll
-    for (int k = 0; k < NV; ++k)
-      v[k] = (k >= nmax_from) ? fmax(v[k], part[size_t(b) * NV + k]) : v[k] + part[size_t(b) * NV + k];
-  block_reduce<NV>(v, nmax_from, res);
-  __syncthreads();
-  if (threadIdx.x < NV) out[threadIdx.x] = res[threadIdx.x] * scale;
 }
 
 // ------------------------------------------------------------------------------------------ misc

@@ -103,8 +103,7 @@ class DeviceModel:
         self.h = h
         if tiled is None:
             tiled = DEFAULT_LAYOUT == "tiled"
-        if not tiled:
-            check(L.mx_model_set_layout(h, 0))
+        check(L.mx_model_set_layout(h, int(bool(tiled))))
         # exchange stencil + coupling table (or per-bond tensors)
         st = system.stencil
         table, typ = coupling_table([J for *_, J in st])
