@@ -25,6 +25,7 @@ struct Rep {
   int off;    // first site of the replica in the packed arrays
   int tile0;  // first logical workgroup of the replica
   int ntile;  // workgroups of the replica
+  float rL1, rL2;  // 1 / L1, 1 / L2 for the exact float division of cell indices (< 2^24)
   int pad;
   double alpha;
   double sig[MAXSUB];  // sigma of sublattice s: sqrt(2 alpha k_B T / (gamma mu_s mu_B dt))
@@ -56,14 +57,20 @@ struct BSite {
   int s, c, c0, c1, c2, i, iref;
 };
 
-// h_i over the stencil with raw offsets (|R| < L on every axis), accumulated in stencil order; the
-// batch of CH gathers is branch-free (entries past the end re-read the last one and count zero).
-template <typename R, int CM>
-__device__ __forceinline__ void bexchange(const BStatic<R>* __restrict__ S, const Rep& rp, const R* __restrict__ e,
-                                          const BSite& st, R& hx, R& hy, R& hz) {
+// h_i over the stencil with raw offsets (|R| < L on every axis), accumulated in stencil order; each
+// batch of CH gathers is branch-free (entries past the end re-read the last one and count zero), and
+// mid() runs between the first batch's loads and their use.
+template <typename R, int CM, typename Mid>
+__device__ __forceinline__ void bexchange(const BStatic<R>* __restrict__ S, const Rep* __restrict__ rp,
+                                          const R* __restrict__ e, const BSite& st, R& hx, R& hy, R& hz, Mid&& mid) {
   constexpr int CH = 8;
   hx = hy = hz = R(0);
   const int kb = S->sptr[st.s], ke = S->sptr[st.s + 1], ns = S->ns;
+  const int L0 = rp->L0, L1 = rp->L1, L2 = rp->L2, off = rp->off;
+  if (kb >= ke) {
+    mid();
+    return;
+  }
   for (int k0 = kb; k0 < ke; k0 += CH) {
     R ex[CH], ey[CH], ez[CH];
     int slot[CH];
@@ -73,17 +80,18 @@ __device__ __forceinline__ void bexchange(const BStatic<R>* __restrict__ S, cons
       const BEnt en = S->ent[live ? k0 + u : ke - 1];
       slot[u] = en.slot;
       int n0 = st.c0 + int(int16_t(en.r01 & 0xffff)), n1 = st.c1 + (en.r01 >> 16), n2 = st.c2 + en.r2;
-      n0 += n0 < 0 ? rp.L0 : 0;
-      n0 -= n0 >= rp.L0 ? rp.L0 : 0;
-      n1 += n1 < 0 ? rp.L1 : 0;
-      n1 -= n1 >= rp.L1 ? rp.L1 : 0;
-      n2 += n2 < 0 ? rp.L2 : 0;
-      n2 -= n2 >= rp.L2 ? rp.L2 : 0;
-      const int j = rp.off + ((n0 * rp.L1 + n1) * rp.L2 + n2) * ns + en.t;
+      n0 += n0 < 0 ? L0 : 0;
+      n0 -= n0 >= L0 ? L0 : 0;
+      n1 += n1 < 0 ? L1 : 0;
+      n1 -= n1 >= L1 ? L1 : 0;
+      n2 += n2 < 0 ? L2 : 0;
+      n2 -= n2 >= L2 ? L2 : 0;
+      const int j = off + ((n0 * L1 + n1) * L2 + n2) * ns + en.t;
       ex[u] = live ? e[3 * j] : R(0);
       ey[u] = live ? e[3 * j + 1] : R(0);
       ez[u] = live ? e[3 * j + 2] : R(0);
     }
+    if (k0 == kb) mid();
 #pragma unroll
     for (int u = 0; u < CH; ++u) {
       const R x = ex[u], y = ey[u], z = ez[u];
@@ -107,6 +115,12 @@ __device__ __forceinline__ void bexchange(const BStatic<R>* __restrict__ S, cons
   }
 }
 
+template <typename R, int CM>
+__device__ __forceinline__ void bexchange(const BStatic<R>* __restrict__ S, const Rep* __restrict__ rp,
+                                          const R* __restrict__ e, const BSite& st, R& hx, R& hy, R& hz) {
+  bexchange<R, CM>(S, rp, e, st, hx, hy, hz, [] {});
+}
+
 template <typename R>
 __device__ __forceinline__ void bonsite(const BStatic<R>* __restrict__ S, int s, R x, R y, R z, R& gx, R& gy, R& gz) {
   gx = gy = gz = R(0);
@@ -120,19 +134,20 @@ __device__ __forceinline__ void bonsite(const BStatic<R>* __restrict__ S, int s,
 
 // The workgroup's replica and its place in it; st gets the site of (thread, u).
 template <typename R, int BS>
-__device__ __forceinline__ bool bsite(const BStatic<R>* __restrict__ S, const Rep& rp, int lt, int spt, int u,
-                                      BSite& st) {
+__device__ __forceinline__ bool bsite(const BStatic<R>* __restrict__ S, const Rep* __restrict__ rp, int lt, int spt,
+                                      int u, BSite& st) {
   const int ns = S->ns;
   const int tq = lt / ns;
   st.s = lt - tq * ns;
   st.c = (tq * spt + u) * BS + int(threadIdx.x);
-  if (st.c >= rp.ncell) return false;
-  st.c2 = st.c % rp.L2;
-  const int t = st.c / rp.L2;
-  st.c1 = t % rp.L1;
-  st.c0 = t / rp.L1;
+  if (st.c >= rp->ncell) return false;
+  const int L1 = rp->L1, L2 = rp->L2;
+  const int t = fdiv(st.c, L2, rp->rL2);
+  st.c2 = st.c - t * L2;
+  st.c0 = fdiv(t, L1, rp->rL1);
+  st.c1 = t - st.c0 * L1;
   st.iref = st.c * ns + st.s;
-  st.i = rp.off + st.iref;
+  st.i = rp->off + st.iref;
   return true;
 }
 
@@ -142,29 +157,31 @@ __global__ __launch_bounds__(BS) void k_bheun(const BStatic<R>* __restrict__ S, 
                                               R* __restrict__ e, R* __restrict__ ep, R* __restrict__ k1,
                                               uint64_t step, int spt, const uint64_t* __restrict__ step_ptr) {
   const int b = xcd_tile(blockIdx.x, gridDim.x);
-  const Rep rp = reps[tile_rep[b]];
+  const Rep* __restrict__ rp = reps + tile_rep[b];  // descriptor read through the pointer: scalar loads
   if (step_ptr) step += *step_ptr;
-  const int lt = b - rp.tile0;
-  const R a = R(rp.alpha);
+  const int lt = b - rp->tile0;
+  const R a = R(rp->alpha);
   for (int u = 0; u < spt; ++u) {
     BSite st;
     if (!bsite<R, BS>(S, rp, lt, spt, u, st)) return;
     const int i = st.i;
     const R* src = PASS == 1 ? e : ep;
     const R x = src[3 * i], y = src[3 * i + 1], z = src[3 * i + 2];
-    R hx, hy, hz, gx, gy, gz;
-    bexchange<R, CM>(S, rp, src, st, hx, hy, hz);
+    R hx, hy, hz, gx, gy, gz, nx = R(0), ny = R(0), nz = R(0);
+    bexchange<R, CM>(S, rp, src, st, hx, hy, hz, [&] {
+      if (d.noisy) {
+        float n0, n1, n2;
+        gauss3(rp->seed_lo, rp->seed_hi, step, uint32_t(st.iref), n0, n1, n2);
+        const R sg = R(rp->sig[st.s]);
+        nx = sg * R(n0);
+        ny = sg * R(n1);
+        nz = sg * R(n2);
+      }
+    });
     bonsite(S, st.s, x, y, z, gx, gy, gz);
     const R im = S->inv_mu_sub[st.s];
-    R bx = (R(2) * hx - gx) * im + d.B0, by = (R(2) * hy - gy) * im + d.B1, bz = (R(2) * hz - gz) * im + d.B2;
-    if (d.noisy) {
-      float n0, n1, n2;
-      gauss3(rp.seed_lo, rp.seed_hi, step, uint32_t(st.iref), n0, n1, n2);
-      const R sg = R(rp.sig[st.s]);
-      bx += sg * R(n0);
-      by += sg * R(n1);
-      bz += sg * R(n2);
-    }
+    const R bx = (R(2) * hx - gx) * im + d.B0 + nx, by = (R(2) * hy - gy) * im + d.B1 + ny,
+            bz = (R(2) * hz - gz) * im + d.B2 + nz;
     R fx, fy, fz;
     llg_rhs(x, y, z, bx, by, bz, a, d.prec, fx, fy, fz);
     if (PASS == 1) {
@@ -196,8 +213,8 @@ __global__ __launch_bounds__(BS) void k_bobs(const BStatic<R>* __restrict__ S, c
                                              const int32_t* __restrict__ tile_rep, const BDyn<R> d,
                                              const R* __restrict__ e, int spt, double* __restrict__ part) {
   const int b = xcd_tile(blockIdx.x, gridDim.x);
-  const Rep rp = reps[tile_rep[b]];
-  const int lt = b - rp.tile0;
+  const Rep* __restrict__ rp = reps + tile_rep[b];
+  const int lt = b - rp->tile0;
   double v[MX_OBS_LEN] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int u = 0; u < spt; ++u) {
     BSite st;
@@ -228,12 +245,13 @@ __global__ __launch_bounds__(BS) void k_bobs(const BStatic<R>* __restrict__ S, c
 __global__ __launch_bounds__(BLOCK) void k_bfinal(const Rep* __restrict__ reps, const double* __restrict__ part,
                                                   double* __restrict__ out) {
   __shared__ double res[MX_OBS_LEN];
-  const Rep rp = reps[blockIdx.x];
+  const Rep* __restrict__ rp = reps + blockIdx.x;
+  const int ntile = rp->ntile, tile0 = rp->tile0;
   double v[MX_OBS_LEN] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int t = threadIdx.x; t < rp.ntile; t += BLOCK)
+  for (int t = threadIdx.x; t < ntile; t += BLOCK)
 #pragma unroll
     for (int k = 0; k < MX_OBS_LEN; ++k) {
-      const double x = part[size_t(rp.tile0 + t) * MX_OBS_LEN + k];
+      const double x = part[size_t(tile0 + t) * MX_OBS_LEN + k];
       v[k] = k == 7 ? fmax(v[k], x) : v[k] + x;
     }
   block_reduce<MX_OBS_LEN>(v, 7, res);
@@ -471,6 +489,9 @@ int mx_batch_create(const mx_model* proto, int32_t nrep, const int32_t* L, mx_ba
           if (L[3 * r + a] < 1 || L[3 * r + a] <= reach[a]) bfail(MX_EINVAL, "replica supercell shorter than the stencil reach");
         const int64_t nc = int64_t(p.L0) * p.L1 * p.L2;
         if (off + nc * b->ns > (int64_t(1) << 30)) bfail(MX_EUNSUPPORTED, "more than 2^30 sites in a batch");
+        if (nc >= (int64_t(1) << 24)) bfail(MX_EUNSUPPORTED, "a replica of 2^24 cells or more");
+        p.rL1 = 1.0f / float(p.L1);
+        p.rL2 = 1.0f / float(p.L2);
         p.ncell = int(nc);
         p.off = int(off);
         off += nc * b->ns;

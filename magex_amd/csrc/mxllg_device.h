@@ -50,6 +50,15 @@ __device__ __forceinline__ void gauss3(uint32_t k0, uint32_t k1, uint64_t step, 
   n2 = r2 * __builtin_amdgcn_cosf(u3);
 }
 
+// floor(n / d) for 0 <= n < 2^24 and d >= 1, with rcp = 1/d rounded to float: the float product is
+// within one unit of the quotient, and the two integer corrections make it exact.
+__device__ __forceinline__ int fdiv(int n, int d, float rcp) {
+  int q = int(float(n) * rcp);
+  q -= q * d > n ? 1 : 0;
+  q += (q + 1) * d <= n ? 1 : 0;
+  return q;
+}
+
 template <typename R>
 __device__ __forceinline__ R rnorm3(R x, R y, R z) {
   return R(1) / sqrt(x * x + y * y + z * z);

@@ -41,6 +41,8 @@ struct Static {
   int L0, L1, L2, ncell, ns, N, bps, nblk;  // bps: workgroups per sublattice
   int pbc0, pbc1, pbc2, allpbc;
   int tb0, tb1, tb2, nt1, nt2;  // TILED: log2 of the tile sides; tiles along axes 1 and 2
+  int fdiv;                     // ncell < 2^24: divide cell indices by L1, L2 through rL1, rL2
+  float rL1, rL2;
   int amode, has_A, has_c, has_u;
   int sptr[MAXSUB + 1];
   Ent ent[MAXENT];

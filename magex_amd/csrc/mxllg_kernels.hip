@@ -49,10 +49,17 @@ __device__ __forceinline__ bool site_of(const Static<R>* __restrict__ S, Site& s
   } else {
     st.c = tile * BLOCK + threadIdx.x;
     if (st.c >= ncell) return false;
-    st.c2 = st.c % L2;
-    const int t = st.c / L2;
-    st.c1 = t % L1;
-    st.c0 = t / L1;
+    if (S->fdiv) {
+      const int t = fdiv(st.c, L2, S->rL2);
+      st.c2 = st.c - t * L2;
+      st.c0 = fdiv(t, L1, S->rL1);
+      st.c1 = t - st.c0 * L1;
+    } else {
+      st.c2 = st.c % L2;
+      const int t = st.c / L2;
+      st.c1 = t % L1;
+      st.c0 = t / L1;
+    }
   }
   st.crm = (st.c0 * L1 + st.c1) * L2 + st.c2;
   st.i = st.s * ncell + st.c;
@@ -98,20 +105,19 @@ struct Chunk {
   static constexpr int value = CM == COUP_BOND ? (sizeof(R) == 8 ? 3 : 5) : 8;
 };
 
-// h_i = sum_j J_ij e_j over the stencil of site st (magex/fast.py:86-120 semantics), accumulated in
-// stencil order.  The loop body has no branch: entries past the end of a batch re-read the last entry
-// and contribute zero, so the compiler issues a batch's loads back to back with one wait.
+// One batch of CH stencil entries: gather() issues every load of the batch without a branch (entries
+// past the end re-read the last entry and contribute zero), accumulate() adds the batch to h in stencil
+// order.  Splitting the two lets ALU work (the thermal noise) run while the first batch is in flight.
 template <typename R, int CM, int GEO>
-__device__ __forceinline__ void exchange(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
-                                         const Site& st, R& hx, R& hy, R& hz) {
-  constexpr int CH = Chunk<R, CM>::value;
-  constexpr int NJ = CM == COUP_BOND ? 9 : 1;
-  hx = hy = hz = R(0);
-  const int kb = S->sptr[st.s], ke = S->sptr[st.s + 1];
-  const int ncell = S->ncell;
-  for (int k0 = kb; k0 < ke; k0 += CH) {
-    R ex[CH], ey[CH], ez[CH], J[CH][NJ];
-    int slot[CH];
+struct Batch {
+  static constexpr int CH = Chunk<R, CM>::value;
+  static constexpr int NJ = CM == COUP_BOND ? 9 : 1;
+  R ex[CH], ey[CH], ez[CH], J[CH][NJ];
+  int slot[CH];
+
+  __device__ __forceinline__ void gather(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
+                                         const Site& st, int k0, int ke) {
+    const int ncell = S->ncell;
 #pragma unroll
     for (int u = 0; u < CH; ++u) {
       const bool live = k0 + u < ke;
@@ -139,6 +145,9 @@ __device__ __forceinline__ void exchange(const Static<R>* __restrict__ S, const 
         for (int q = 0; q < 9; ++q) J[u][q] = back ? t[(q % 3) * 3 + q / 3] : t[q];
       }
     }
+  }
+
+  __device__ __forceinline__ void accumulate(const Static<R>* __restrict__ S, R& hx, R& hy, R& hz) const {
 #pragma unroll
     for (int u = 0; u < CH; ++u) {
       const R x = ex[u], y = ey[u], z = ez[u];
@@ -165,6 +174,33 @@ __device__ __forceinline__ void exchange(const Static<R>* __restrict__ S, const 
       }
     }
   }
+};
+
+// h_i = sum_j J_ij e_j over the stencil of site st (magex/fast.py:86-120 semantics), accumulated in
+// stencil order; mid() runs after the first batch's loads are issued and before they are used.
+template <typename R, int CM, int GEO, typename Mid>
+__device__ __forceinline__ void exchange(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
+                                         const Site& st, R& hx, R& hy, R& hz, Mid&& mid) {
+  hx = hy = hz = R(0);
+  const int kb = S->sptr[st.s], ke = S->sptr[st.s + 1];
+  if (kb >= ke) {
+    mid();
+    return;
+  }
+  Batch<R, CM, GEO> bt;
+  bt.gather(S, d, e, st, kb, ke);
+  mid();
+  bt.accumulate(S, hx, hy, hz);
+  for (int k0 = kb + Batch<R, CM, GEO>::CH; k0 < ke; k0 += Batch<R, CM, GEO>::CH) {
+    bt.gather(S, d, e, st, k0, ke);
+    bt.accumulate(S, hx, hy, hz);
+  }
+}
+
+template <typename R, int CM, int GEO>
+__device__ __forceinline__ void exchange(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
+                                         const Site& st, R& hx, R& hy, R& hz) {
+  exchange<R, CM, GEO>(S, d, e, st, hx, hy, hz, [] {});
 }
 
 // On-site gradient dE/de (magex/fast.py:39-64): 2 A e + cubic + higher uniaxial.
@@ -280,25 +316,21 @@ __device__ __forceinline__ R inv_mu_of(const Static<R>* __restrict__ S, const Dy
   return d.inv_mu ? d.inv_mu[st.i] : S->inv_mu_sub[st.s];
 }
 
-// B_i = (2 h_i - g_i) / (mu_i mu_B) + B (+ thermal) at spin (x, y, z) (magex/fast.py:309-317).
+// B_i = (2 h_i - g_i) / (mu_i mu_B) + B (+ thermal) at spin (x, y, z) (magex/fast.py:309-317).  The
+// generator's normals are computed while the first neighbour batch is in flight.
 template <typename R, int CM, int GEO>
 __device__ __forceinline__ void local_field(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
                                             const Site& st, R x, R y, R z, uint64_t step,
                                             const R* __restrict__ noise, R& bx, R& by, R& bz) {
-  R hx, hy, hz, gx, gy, gz;
-  exchange<R, CM, GEO>(S, d, e, st, hx, hy, hz);
+  R hx, hy, hz, gx, gy, gz, nx = R(0), ny = R(0), nz = R(0);
+  exchange<R, CM, GEO>(S, d, e, st, hx, hy, hz, [&] {
+    if (d.noisy) thermal(S, d, st, step, noise, nx, ny, nz);
+  });
   onsite_grad(S, d, st, x, y, z, gx, gy, gz);
   const R im = inv_mu_of(S, d, st);
-  bx = (R(2) * hx - gx) * im + d.B0;
-  by = (R(2) * hy - gy) * im + d.B1;
-  bz = (R(2) * hz - gz) * im + d.B2;
-  if (d.noisy) {
-    R nx, ny, nz;
-    thermal(S, d, st, step, noise, nx, ny, nz);
-    bx += nx;
-    by += ny;
-    bz += nz;
-  }
+  bx = (R(2) * hx - gx) * im + d.B0 + nx;
+  by = (R(2) * hy - gy) * im + d.B1 + ny;
+  bz = (R(2) * hz - gz) * im + d.B2 + nz;
 }
 
 // ------------------------------------------------------------------------------------------ Heun
@@ -512,6 +544,9 @@ static void fill_static(const Model& m, Static<R>& S) {
   S.tb2 = m.tb[2];
   S.nt1 = m.L[1] >> m.tb[1];
   S.nt2 = m.L[2] >> m.tb[2];
+  S.fdiv = m.ncell < (1 << 24);
+  S.rL1 = 1.0f / float(m.L[1]);
+  S.rL2 = 1.0f / float(m.L[2]);
   S.amode = m.amode;
   S.has_A = m.has_A;
   S.has_c = m.has_c;
