@@ -163,6 +163,32 @@ int mx_batch_run(mx_batch* b, void* e, int64_t nsteps, const uint64_t* seeds, ui
 /* Per-replica observables (device, nrep x MX_OBS_LEN doubles, layout as mx_observables). */
 int mx_batch_observables(mx_batch* b, const void* e, double* out, void* stream);
 
+/* ---------------------------------------------------------------------------------------------
+ * Differentiable form: reverse-mode adjoint of a deterministic-noise Heun trajectory with respect to
+ * the initial spins and to P <= 8 parameters theta of a linearly parameterized Hamiltonian,
+ * J_k = sum_p theta_p U[p][k] on every stencil entry k and A_s = sum_p theta_p Aun[p][s] on every
+ * sublattice (shells, DMI and anisotropy constants).  No counterpart in magex, whose solvers are not
+ * differentiable (magex/llg.py:87-120 is the forward it differentiates).
+ * The adjoint takes over the model's couplings and on-site tensors (the model keeps its stencil,
+ * moments, field and dynamics); per-bond tensors, per-site on-site tensors, cubic / higher uniaxial
+ * terms and pinned sites are not supported.  Spins and gradients are in magex site order. */
+typedef struct mx_adjoint mx_adjoint;
+
+/* unit_bonds: nparam x nent x 9 (row-major 3x3 per stencil entry, the order given to
+ * mx_model_set_stencil; an entry and its reverse partner must carry transposed tensors);
+ * unit_onsite: nparam x nsub x 9.  checkpoint_every: steps between stored spin states. */
+int mx_adjoint_create(mx_model* m, int32_t nparam, const double* unit_bonds, const double* unit_onsite,
+                      int32_t checkpoint_every, mx_adjoint** out);
+void mx_adjoint_destroy(mx_adjoint* a);
+/* theta: nparam values (host); sets the model's couplings and on-site tensors. */
+int mx_adjoint_set_params(mx_adjoint* a, const double* theta);
+/* nsteps Heun steps of e (device, N x 3, in place) as mx_llg_run with the same seed and step0,
+ * recording checkpoints for mx_adjoint_backward. */
+int mx_adjoint_forward(mx_adjoint* a, void* e, int64_t nsteps, uint64_t seed, uint64_t step0, void* stream);
+/* Given grad_e = dL/de_final (device, N x 3), writes dL/de_0 into grad_e0 (device, N x 3, may be null)
+ * and dL/dtheta into grad_theta (device, nparam doubles), for the last mx_adjoint_forward. */
+int mx_adjoint_backward(mx_adjoint* a, const void* grad_e, void* grad_e0, double* grad_theta, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

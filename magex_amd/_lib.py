@@ -55,6 +55,11 @@ SIGNATURES = {
     "mx_batch_set_launch": (_I, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "mx_batch_run": (_I, [_P, _P, _I64, _P, _U64, _P]),
     "mx_batch_observables": (_I, [_P, _P, _P, _P]),
+    "mx_adjoint_create": (_I, [_P, ctypes.c_int32, _P, _P, ctypes.c_int32, _P]),
+    "mx_adjoint_destroy": (None, [_P]),
+    "mx_adjoint_set_params": (_I, [_P, _P]),
+    "mx_adjoint_forward": (_I, [_P, _P, ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint64, _P]),
+    "mx_adjoint_backward": (_I, [_P, _P, _P, _P, _P]),
 }
 
 _lib = None

@@ -1,0 +1,194 @@
+"""Differentiable LLG: gradients of a Heun trajectory with respect to the initial spins and to the
+constants of the Hamiltonian, through the library's hand-written adjoint.
+
+No counterpart in magex: its solvers (magex/llg.py:87-120) only run forward.  Here a Hamiltonian whose
+couplings and on-site tensors are linear in a few constants theta (exchange shells, DMI, anisotropy)
+is written J_k(theta) = sum_p theta_p U_p[k] per stencil entry and A_s(theta) = sum_p theta_p Aun_p[s];
+the units come from a builder ``build(theta) -> SpinSystem`` (for example ``configs.square128``) by
+differencing, and are checked to reproduce the builder's couplings.
+
+    sol = DifferentiableLLG(lambda th: configs.square128(*th)[0], theta0, alpha=0.3, dt=1e-3)
+    theta = torch.tensor(theta0, requires_grad=True)
+    e = sol.trajectory(theta, e0, 500)           # autograd-aware
+    loss = ((e - target) ** 2).mean(); loss.backward()
+
+The backward replays each checkpoint interval (default 50 steps) and runs the adjoint kernels on the
+device; see mxllg_adjoint.hip for the equations.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import check, hptr
+from .device import DeviceModel, _stream, _torch
+
+MAXP = 8
+
+
+def linear_units(build, theta_ref, tol=1e-12):
+    """Unit tensors (U (P, nent, 3, 3), Aun (P, ns, 3, 3)) and the system at theta_ref.
+
+    U_p = J(theta_ref + e_p) - J(theta_ref) entry by entry; a theta-independent remainder
+    J(theta_ref) - sum_p theta_ref_p U_p, if any, is returned as an extra fixed unit (offset, or None).
+    Raises ValueError when the builder's stencil changes with theta or its couplings are not linear."""
+    theta_ref = np.asarray(theta_ref, float)
+    P = len(theta_ref)
+    base = build(theta_ref)
+    key = [(s, t, tuple(R)) for s, t, R, _ in base.stencil]
+    ns = base.uc.nsites
+
+    def arrays(sys_):
+        if [(s, t, tuple(R)) for s, t, R, _ in sys_.stencil] != key:
+            raise ValueError("the builder's stencil changes with the parameters (keep every shell nonzero at the "
+                             "reference point)")
+        if sys_.bond_tensors is not None:
+            raise NotImplementedError("per-bond tensors are not parameterized")
+        A = np.asarray(sys_.A, float).reshape(sys_.ncell, ns, 3, 3)
+        if not np.all(A == A[:1]):
+            raise NotImplementedError("per-site on-site tensors are not parameterized")
+        return np.array([J for *_, J in sys_.stencil], float).reshape(-1, 3, 3), A[0]
+
+    J0, A0 = arrays(base)
+    U = np.empty((P,) + J0.shape)
+    Aun = np.empty((P,) + A0.shape)
+    for p in range(P):
+        th = theta_ref.copy()
+        th[p] += 1.0
+        J1, A1 = arrays(build(th))
+        U[p], Aun[p] = J1 - J0, A1 - A0
+    offJ = J0 - np.tensordot(theta_ref, U, 1)
+    offA = A0 - np.tensordot(theta_ref, Aun, 1)
+    scale = max(np.abs(J0).max(initial=0.0), np.abs(A0).max(initial=0.0), 1e-300)
+    offset = None
+    if np.abs(offJ).max(initial=0.0) > tol * scale or np.abs(offA).max(initial=0.0) > tol * scale:
+        offset = (offJ, offA)
+    # linearity: a second point must be reproduced by the units
+    th = theta_ref * 0.5 + 0.25
+    J2, A2 = arrays(build(th))
+    pJ = np.tensordot(th, U, 1) + (offset[0] if offset else 0.0)
+    pA = np.tensordot(th, Aun, 1) + (offset[1] if offset else 0.0)
+    if np.abs(pJ - J2).max(initial=0.0) > 1e-10 * scale or np.abs(pA - A2).max(initial=0.0) > 1e-10 * scale:
+        raise ValueError("the builder's couplings are not linear in the parameters")
+    return U, Aun, offset, base
+
+
+class DifferentiableLLG:
+    """Deterministic-noise Heun trajectories of a parameterized SpinSystem with adjoint gradients.
+
+    build(theta) -> SpinSystem defines the Hamiltonian (its field, moments and stencil are taken at
+    theta0; couplings and on-site tensors must be linear in theta).  alpha, dt (ps), temperature (K) and
+    seed as LLGSolver; the thermal field is a fixed realization (the library's Philox stream), so the
+    gradient is that of one sample path.  precision "fp64" or "fp32"."""
+
+    def __init__(self, build, theta0, alpha, dt, temperature=0.0, seed=0, precision="fp64", checkpoint_every=50,
+                 precession=True):
+        _torch()
+        U, Aun, offset, system = linear_units(build, theta0)
+        self.P = len(theta0)
+        if offset is not None:
+            U = np.concatenate([U, offset[0][None]])
+            Aun = np.concatenate([Aun, offset[1][None]])
+        if len(U) > MAXP:
+            raise ValueError(f"at most {MAXP} parameters (including a fixed offset)")
+        if system.terms or system.pinned is not None:
+            raise NotImplementedError("extra on-site terms and pinned sites are not differentiated")
+        self.offset = offset is not None
+        self.system = system
+        self.precision = precision
+        self.dtype = torch.float64 if precision == "fp64" else torch.float32
+        self.N = system.N
+        self.seed = int(seed)
+        self.dm = DeviceModel(system, precision)
+        self.dm.set_dynamics(dt, alpha, temperature, precession)
+        self.checkpoint_every = int(checkpoint_every)
+        L = self.dm.lib
+        h = ctypes.c_void_p()
+        self._U = np.ascontiguousarray(U.reshape(len(U), -1, 9))
+        self._Aun = np.ascontiguousarray(Aun.reshape(len(Aun), -1, 9))
+        check(L.mx_adjoint_create(self.dm.h, len(U), hptr(self._U), hptr(self._Aun), self.checkpoint_every,
+                                  ctypes.byref(h)))
+        self.h = h
+        self._theta = None
+        self._token = 0          # id of the forward the adjoint record belongs to
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            self.dm.lib.mx_adjoint_destroy(h)
+            self.h = None
+
+    def set_params(self, theta):
+        th = np.asarray(theta, float).reshape(-1)
+        if th.size != self.P:
+            raise ValueError(f"expected {self.P} parameters")
+        full = np.append(th, 1.0) if self.offset else th
+        key = full.tobytes()
+        if key != self._theta:
+            check(self.dm.lib.mx_adjoint_set_params(self.h, hptr(np.ascontiguousarray(full))))
+            self._theta = key
+
+    def _spins(self, e):
+        if not isinstance(e, torch.Tensor):
+            e = torch.as_tensor(np.asarray(e, float))
+        e = e.detach().to(device="cuda", dtype=self.dtype).contiguous()
+        if e.shape != (self.N, 3):
+            raise ValueError(f"spins must be ({self.N}, 3)")
+        return e.clone()
+
+    def forward(self, theta, e0, nsteps, step0=0):
+        """Spins after nsteps steps from e0 (new device tensor); records checkpoints for backward()."""
+        self.set_params(theta)
+        e = self._spins(e0)
+        check(self.dm.lib.mx_adjoint_forward(self.h, ctypes.c_void_p(e.data_ptr()), int(nsteps), self.seed,
+                                             int(step0), _stream()))
+        self._token += 1
+        return e
+
+    def backward(self, grad_e, want_e0=True):
+        """(dL/dtheta (P,) float64 device tensor, dL/de0 or None) for the last forward()."""
+        g = grad_e.detach().to(device="cuda", dtype=self.dtype).contiguous()
+        gt = torch.empty(MAXP, dtype=torch.float64, device="cuda")
+        g0 = torch.empty_like(g) if want_e0 else None
+        check(self.dm.lib.mx_adjoint_backward(self.h, ctypes.c_void_p(g.data_ptr()),
+                                              ctypes.c_void_p(g0.data_ptr() if g0 is not None else 0),
+                                              ctypes.c_void_p(gt.data_ptr()), _stream()))
+        return gt[:self.P], g0
+
+    def trajectory(self, theta, e0, nsteps, step0=0):
+        """Differentiable e(nsteps) as a function of theta (torch tensor, P) and e0 (N, 3)."""
+        if not isinstance(theta, torch.Tensor):
+            theta = torch.as_tensor(np.asarray(theta, float))
+        if not isinstance(e0, torch.Tensor):
+            e0 = torch.as_tensor(np.asarray(e0, float))
+        return _HeunTrajectory.apply(theta, e0, self, int(nsteps), int(step0))
+
+
+class _HeunTrajectory(torch.autograd.Function):
+    """e_final = Heun^nsteps(e0; theta): forward through mx_adjoint_forward, backward through
+    mx_adjoint_backward (re-running the forward first if another forward has run since)."""
+
+    @staticmethod
+    def forward(ctx, theta, e0, sol, nsteps, step0):
+        th = theta.detach().cpu().double().numpy()
+        e = sol.forward(th, e0, nsteps, step0)
+        ctx.sol, ctx.nsteps, ctx.step0, ctx.theta = sol, nsteps, step0, th
+        ctx.token = sol._token
+        ctx.e0 = e0.detach().clone()
+        ctx.meta = (theta.dtype, theta.device, e0.dtype, e0.device)
+        return e.to(dtype=e0.dtype, device=e0.device)
+
+    @staticmethod
+    def backward(ctx, g):
+        sol = ctx.sol
+        if sol._token != ctx.token:
+            sol.forward(ctx.theta, ctx.e0, ctx.nsteps, ctx.step0)
+            ctx.token = sol._token
+        gt, g0 = sol.backward(g, want_e0=ctx.needs_input_grad[1])
+        tdt, tdev, edt, edev = ctx.meta
+        gt = gt.to(dtype=tdt, device=tdev)
+        if g0 is not None:
+            g0 = g0.to(dtype=edt, device=edev)
+        return gt, g0, None, None, None

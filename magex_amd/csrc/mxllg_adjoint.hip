@@ -1,0 +1,567 @@
+// Differentiable form: reverse-mode adjoint of the stochastic Heun trajectory with respect to the
+// spins and to the parameters of a linearly parameterized Hamiltonian.  No counterpart in magex.
+//
+// Model: couplings J_k = sum_p theta_p U[p][k] on the stencil entries k, on-site A_s = sum_p theta_p
+// Aun[p][s]; B_i = 2 (h_i - A_i e_i) / (mu_i mu_B) + B_ext + b_i with h_i = sum_k J_k e_j(k).
+// One step:  k1 = f(e, B(e)),  ep = N(e + dt k1),  k2 = f(ep, B(ep)),  e' = N(e + dt/2 (k1 + k2)),
+// f(e, B) = pre [prec e x B + alpha e x (e x B)], pre = -gamma / (1 + alpha^2), N(u) = u / |u|.
+// Given lambda = dL/de', one adjoint step returns dL/de and adds dL/dtheta:
+//   normalisation:  dL/du = (lambda - y (y . lambda)) / |u|,  y = N(u)
+//   f:              with a = pre l and w = prec a + alpha (a x e):  dL/de = alpha (c x a) + B x w,
+//                   dL/dB = w x e,  c = e x B
+//   field:          nu_i = 2 dL/dB_i / (mu_i mu_B);  dL/de_j += sum_i J_ij^T nu_i - A_j^T nu_j, and since
+//                   J_ji = J_ij^T the first term is the forward gather applied to nu
+//   parameters:     dL/dtheta_p += sum_i nu_i . (sum_k U[p][k] x_j(k) - Aun[p] x_i)  (x = e or ep).
+// The forward run keeps the spins every C steps; the backward replays each segment from its
+// checkpoint (saving e, ep, k1, B(e), B(ep) per step, noise regenerated from the same counters) and
+// runs three kernels per step: one local, two gathers.
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "mxllg_field.h"
+
+namespace mx {
+
+constexpr int MAXP = 8;  // parameters
+
+template <typename R>
+struct AStatic {
+  int P;
+  R U[MAXP][MAXENT][9];    // unit bond tensors per parameter, in the library's stencil order
+  R Aun[MAXP][MAXSUB][9];  // unit on-site tensors per parameter
+};
+
+// Vector-Jacobian product of f(e, B) with cotangent l (see the header comment).
+template <typename R>
+__device__ __forceinline__ void rhs_vjp(R ex, R ey, R ez, R bx, R by, R bz, R alpha, int prec, R lx, R ly, R lz,
+                                        R& gex, R& gey, R& gez, R& gbx, R& gby, R& gbz) {
+  const R pre = R(-GAMMA) / (R(1) + alpha * alpha);
+  const R ax = pre * lx, ay = pre * ly, az = pre * lz;
+  const R cx = ey * bz - ez * by, cy = ez * bx - ex * bz, cz = ex * by - ey * bx;
+  const R p = prec ? R(1) : R(0);
+  const R wx = p * ax + alpha * (ay * ez - az * ey);
+  const R wy = p * ay + alpha * (az * ex - ax * ez);
+  const R wz = p * az + alpha * (ax * ey - ay * ex);
+  gex = alpha * (cy * az - cz * ay) + (by * wz - bz * wy);
+  gey = alpha * (cz * ax - cx * az) + (bz * wx - bx * wz);
+  gez = alpha * (cx * ay - cy * ax) + (bx * wy - by * wx);
+  gbx = wy * ez - wz * ey;
+  gby = wz * ex - wx * ez;
+  gbz = wx * ey - wy * ex;
+}
+
+// (v - y (y . v)) / n
+template <typename R>
+__device__ __forceinline__ void unproject(R yx, R yy, R yz, R n, R& vx, R& vy, R& vz) {
+  const R d = yx * vx + yy * vy + yz * vz;
+  vx = (vx - yx * d) / n;
+  vy = (vy - yy * d) / n;
+  vz = (vz - yz * d) / n;
+}
+
+template <typename R>
+__device__ __forceinline__ void onsite_T(const Static<R>* __restrict__ S, int s, R x, R y, R z, R& ox, R& oy, R& oz) {
+  // A^T v for the sublattice's on-site tensor
+  const R* a = S->Asub[s];
+  ox = a[0] * x + a[3] * y + a[6] * z;
+  oy = a[1] * x + a[4] * y + a[7] * z;
+  oz = a[2] * x + a[5] * y + a[8] * z;
+}
+
+// Adjoint gather at site st: H = sum_k J_k nu_j (J from the model's width-9 table), and for every
+// parameter t_p = sum_k U[p][k] x_j; then G_p = nu_i . (t_p - Aun[p] x_i) is accumulated into gp.
+template <typename R, int GEO>
+__device__ __forceinline__ void adj_gather(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
+                                           const R* __restrict__ nu, const R* __restrict__ x, const Site& st,
+                                           R nix, R niy, R niz, R xix, R xiy, R xiz, R& hx, R& hy, R& hz,
+                                           double (&gp)[MAXP]) {
+  const int P = A->P, ncell = S->ncell;
+  R t[MAXP][3];
+#pragma unroll
+  for (int p = 0; p < MAXP; ++p) t[p][0] = t[p][1] = t[p][2] = R(0);
+  hx = hy = hz = R(0);
+  const int kb = S->sptr[st.s], ke = S->sptr[st.s + 1];
+  for (int k = kb; k < ke; ++k) {
+    const Ent en = S->ent[k];
+    bool ok;
+    const int cn = neighbour_cell<R, GEO>(S, st, en, ok);
+    const int j = (en.tb & 0xffff) * ncell + cn;
+    const R w = ok ? R(1) : R(0);
+    const R vx = w * nu[3 * j], vy = w * nu[3 * j + 1], vz = w * nu[3 * j + 2];
+    const R qx = w * x[3 * j], qy = w * x[3 * j + 1], qz = w * x[3 * j + 2];
+    const R* T = S->tab + 9 * en.slot;
+    hx += T[0] * vx + T[1] * vy + T[2] * vz;
+    hy += T[3] * vx + T[4] * vy + T[5] * vz;
+    hz += T[6] * vx + T[7] * vy + T[8] * vz;
+#pragma unroll
+    for (int p = 0; p < MAXP; ++p) {
+      if (p < P) {
+        const R* U = A->U[p][k];
+        t[p][0] += U[0] * qx + U[1] * qy + U[2] * qz;
+        t[p][1] += U[3] * qx + U[4] * qy + U[5] * qz;
+        t[p][2] += U[6] * qx + U[7] * qy + U[8] * qz;
+      }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < MAXP; ++p) {
+    if (p < P) {
+      const R* a = A->Aun[p][st.s];
+      const R ux = t[p][0] - (a[0] * xix + a[1] * xiy + a[2] * xiz);
+      const R uy = t[p][1] - (a[3] * xix + a[4] * xiy + a[5] * xiz);
+      const R uz = t[p][2] - (a[6] * xix + a[7] * xiy + a[8] * xiz);
+      gp[p] += double(nix * ux + niy * uy + niz * uz);
+    }
+  }
+}
+
+// Step-local part (corrector): from lambda = dL/de', the saved e, ep, k1 and B(ep): lambda_u, the
+// cotangent of ep through k2 (gep) and nu2.
+template <typename R, int GEO>
+__global__ __launch_bounds__(BLOCK) void k_adj_a(const Static<R>* __restrict__ S, const Dyn<R> d,
+                                                 const R* __restrict__ lam, const R* __restrict__ e,
+                                                 const R* __restrict__ ep, const R* __restrict__ k1,
+                                                 const R* __restrict__ B2, R* __restrict__ lamu, R* __restrict__ gep,
+                                                 R* __restrict__ nu2) {
+  Site st;
+  if (!site_of<R, GEO>(S, st)) return;
+  const int i = st.i;
+  const R a = d.alpha ? d.alpha[i] : S->alpha_sub[st.s];
+  const R px = ep[3 * i], py = ep[3 * i + 1], pz = ep[3 * i + 2];
+  const R bx = B2[3 * i], by = B2[3 * i + 1], bz = B2[3 * i + 2];
+  R fx, fy, fz;
+  llg_rhs(px, py, pz, bx, by, bz, a, d.prec, fx, fy, fz);
+  const R hdt = R(0.5) * d.dt;
+  const R ux = e[3 * i] + hdt * (k1[3 * i] + fx), uy = e[3 * i + 1] + hdt * (k1[3 * i + 1] + fy),
+          uz = e[3 * i + 2] + hdt * (k1[3 * i + 2] + fz);
+  const R n = sqrt(ux * ux + uy * uy + uz * uz);
+  R lx = lam[3 * i], ly = lam[3 * i + 1], lz = lam[3 * i + 2];
+  unproject(ux / n, uy / n, uz / n, n, lx, ly, lz);
+  lamu[3 * i] = lx;
+  lamu[3 * i + 1] = ly;
+  lamu[3 * i + 2] = lz;
+  R gx, gy, gz, hx, hy, hz;
+  rhs_vjp(px, py, pz, bx, by, bz, a, d.prec, hdt * lx, hdt * ly, hdt * lz, gx, gy, gz, hx, hy, hz);
+  gep[3 * i] = gx;
+  gep[3 * i + 1] = gy;
+  gep[3 * i + 2] = gz;
+  const R im2 = R(2) * (d.inv_mu ? d.inv_mu[i] : S->inv_mu_sub[st.s]);
+  nu2[3 * i] = im2 * hx;
+  nu2[3 * i + 1] = im2 * hy;
+  nu2[3 * i + 2] = im2 * hz;
+}
+
+// Gather nu2 (dL/dep) and the predictor's adjoint: writes nu1 and the partial dL/de (part).
+template <typename R, int GEO>
+__global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
+                                                 const Dyn<R> d, const R* __restrict__ e, const R* __restrict__ ep,
+                                                 const R* __restrict__ B1, const R* __restrict__ lamu,
+                                                 const R* __restrict__ gep, const R* __restrict__ nu2,
+                                                 R* __restrict__ nu1, R* __restrict__ part, double* __restrict__ gpart) {
+  double gp[MAXP];
+#pragma unroll
+  for (int p = 0; p < MAXP; ++p) gp[p] = 0.0;
+  Site st;
+  if (site_of<R, GEO>(S, st)) {
+    const int i = st.i;
+    const R a = d.alpha ? d.alpha[i] : S->alpha_sub[st.s];
+    const R px = ep[3 * i], py = ep[3 * i + 1], pz = ep[3 * i + 2];
+    const R vx2 = nu2[3 * i], vy2 = nu2[3 * i + 1], vz2 = nu2[3 * i + 2];
+    R hx, hy, hz, ox, oy, oz;
+    adj_gather<R, GEO>(S, A, nu2, ep, st, vx2, vy2, vz2, px, py, pz, hx, hy, hz, gp);
+    onsite_T(S, st.s, vx2, vy2, vz2, ox, oy, oz);
+    R lx = gep[3 * i] + hx - ox, ly = gep[3 * i + 1] + hy - oy, lz = gep[3 * i + 2] + hz - oz;  // dL/dep
+    const R ex = e[3 * i], ey = e[3 * i + 1], ez = e[3 * i + 2];
+    const R bx = B1[3 * i], by = B1[3 * i + 1], bz = B1[3 * i + 2];
+    R fx, fy, fz;
+    llg_rhs(ex, ey, ez, bx, by, bz, a, d.prec, fx, fy, fz);
+    const R vx = ex + d.dt * fx, vy = ey + d.dt * fy, vz = ez + d.dt * fz;
+    const R n = sqrt(vx * vx + vy * vy + vz * vz);
+    unproject(vx / n, vy / n, vz / n, n, lx, ly, lz);  // dL/dv, v = e + dt k1
+    const R ux = lamu[3 * i], uy = lamu[3 * i + 1], uz = lamu[3 * i + 2];
+    const R hdt = R(0.5) * d.dt;
+    const R kx = d.dt * lx + hdt * ux, ky = d.dt * ly + hdt * uy, kz = d.dt * lz + hdt * uz;  // dL/dk1
+    R gx, gy, gz, qx, qy, qz;
+    rhs_vjp(ex, ey, ez, bx, by, bz, a, d.prec, kx, ky, kz, gx, gy, gz, qx, qy, qz);
+    const R im2 = R(2) * (d.inv_mu ? d.inv_mu[i] : S->inv_mu_sub[st.s]);
+    nu1[3 * i] = im2 * qx;
+    nu1[3 * i + 1] = im2 * qy;
+    nu1[3 * i + 2] = im2 * qz;
+    part[3 * i] = ux + lx + gx;
+    part[3 * i + 1] = uy + ly + gy;
+    part[3 * i + 2] = uz + lz + gz;
+  }
+  double red[MAXP];
+#pragma unroll
+  for (int p = 0; p < MAXP; ++p) red[p] = gp[p];
+  __shared__ double res[MAXP];
+  block_reduce<MAXP>(red, MAXP, res);
+  __syncthreads();
+  if (threadIdx.x < MAXP) gpart[size_t(blockIdx.x) * MAXP + threadIdx.x] += res[threadIdx.x];
+}
+
+// Gather nu1: dL/de of the step's input, and the parameter terms at e.
+template <typename R, int GEO>
+__global__ __launch_bounds__(BLOCK) void k_adj_c(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
+                                                 const R* __restrict__ e, const R* __restrict__ nu1,
+                                                 const R* __restrict__ part, R* __restrict__ lam,
+                                                 double* __restrict__ gpart) {
+  double gp[MAXP];
+#pragma unroll
+  for (int p = 0; p < MAXP; ++p) gp[p] = 0.0;
+  Site st;
+  if (site_of<R, GEO>(S, st)) {
+    const int i = st.i;
+    const R vx = nu1[3 * i], vy = nu1[3 * i + 1], vz = nu1[3 * i + 2];
+    R hx, hy, hz, ox, oy, oz;
+    adj_gather<R, GEO>(S, A, nu1, e, st, vx, vy, vz, e[3 * i], e[3 * i + 1], e[3 * i + 2], hx, hy, hz, gp);
+    onsite_T(S, st.s, vx, vy, vz, ox, oy, oz);
+    lam[3 * i] = part[3 * i] + hx - ox;
+    lam[3 * i + 1] = part[3 * i + 1] + hy - oy;
+    lam[3 * i + 2] = part[3 * i + 2] + hz - oz;
+  }
+  double red[MAXP];
+#pragma unroll
+  for (int p = 0; p < MAXP; ++p) red[p] = gp[p];
+  __shared__ double res[MAXP];
+  block_reduce<MAXP>(red, MAXP, res);
+  __syncthreads();
+  if (threadIdx.x < MAXP) gpart[size_t(blockIdx.x) * MAXP + threadIdx.x] += res[threadIdx.x];
+}
+
+template <typename T>
+__global__ void k_zero(T* __restrict__ p, size_t n) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = T(0);
+}
+
+template <typename T>
+__global__ void k_copy(const T* __restrict__ a, T* __restrict__ b, size_t n) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+}  // namespace mx
+
+// ============================================================================================ host
+using namespace mx;
+
+struct mx_adjoint {
+  mx_model* m = nullptr;
+  int P = 0, C = 50;
+  std::vector<double> U, Aun;  // P x nent x 9 (library order), P x ns x 9
+  std::vector<double> theta;
+  void* astatic_d = nullptr;
+  // forward record
+  bool recorded = false;
+  int64_t nsteps = 0;
+  uint64_t seed = 0, step0 = 0;
+  void* ckpt_d = nullptr;  // (nckpt) x N x 3
+  int64_t ckpt_cap = 0;
+  // segment and adjoint scratch
+  void *seg_d = nullptr;  // 5 x C x N x 3: e, ep, k1, B1, B2
+  void *lam_d = nullptr, *lamu_d = nullptr, *gep_d = nullptr, *nu1_d = nullptr, *nu2_d = nullptr, *part_d = nullptr;
+  void* ework_d = nullptr;
+  double* gpart_d = nullptr;  // per-workgroup parameter-gradient partials, nblk x MAXP
+  double* gout_d = nullptr;   // their fold, MAXP
+  int nblk = 0;
+};
+
+namespace {
+
+struct AErr {
+  int code;
+  std::string msg;
+};
+[[noreturn]] void afail(int code, const std::string& msg) { throw AErr{code, msg}; }
+void aok(hipError_t e, const char* what) {
+  if (e != hipSuccess) afail(e == hipErrorOutOfMemory ? MX_ENOMEM : MX_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+void aok_rc(int rc, const char* what) {
+  if (rc != MX_OK) afail(rc, std::string(what) + ": " + mx_last_error());
+}
+template <typename F>
+int aguard(F&& f) {
+  try {
+    return f();
+  } catch (const AErr& e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error("host allocation failed");
+    return MX_ENOMEM;
+  }
+}
+void afree(void*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+template <typename R>
+void upload_astatic(mx_adjoint& a) {
+  const Model& m = *a.m;
+  AStatic<R> A;
+  std::memset(&A, 0, sizeof(A));
+  A.P = a.P;
+  for (int p = 0; p < a.P; ++p) {
+    for (int k = 0; k < m.nent; ++k)
+      for (int q = 0; q < 9; ++q) A.U[p][k][q] = R(a.U[(size_t(p) * m.nent + k) * 9 + q]);
+    for (int s = 0; s < m.ns; ++s)
+      for (int q = 0; q < 9; ++q) A.Aun[p][s][q] = R(a.Aun[(size_t(p) * m.ns + s) * 9 + q]);
+  }
+  if (!a.astatic_d) aok(hipMalloc(&a.astatic_d, sizeof(A)), "hipMalloc adjoint constants");
+  aok(hipMemcpy(a.astatic_d, &A, sizeof(A), hipMemcpyHostToDevice), "hipMemcpy adjoint constants");
+}
+
+// The model's couplings become sum_p theta_p U[p] (one width-9 table row per entry) and its on-site
+// tensors sum_p theta_p Aun[p].
+void apply_theta(mx_adjoint& a) {
+  Model& m = *a.m;
+  m.table.assign(size_t(m.nent) * 9, 0.0);
+  for (int k = 0; k < m.nent; ++k) {
+    m.ent[k].slot = k;
+    for (int p = 0; p < a.P; ++p)
+      for (int q = 0; q < 9; ++q) m.table[9 * size_t(k) + q] += a.theta[p] * a.U[(size_t(p) * m.nent + k) * 9 + q];
+  }
+  m.ntypes = m.nent;
+  m.width = 9;
+  m.coup = COUP_TABLE9;
+  m.amode = ONSITE_SUB;
+  m.has_A = 0;
+  std::fill(m.Asub.begin(), m.Asub.end(), 0.0);
+  for (int s = 0; s < m.ns; ++s)
+    for (int p = 0; p < a.P; ++p)
+      for (int q = 0; q < 9; ++q) {
+        m.Asub[9 * s + q] += a.theta[p] * a.Aun[(size_t(p) * m.ns + s) * 9 + q];
+        if (m.Asub[9 * s + q] != 0.0) m.has_A = 1;
+      }
+  m.static_dirty = 1;
+}
+
+size_t nbytes3(const Model& m) { return size_t(m.N) * 3 * prec_size(m); }
+
+void ensure_buffers(mx_adjoint& a) {
+  const Model& m = *a.m;
+  const size_t b = nbytes3(m);
+  if (!a.seg_d) aok(hipMalloc(&a.seg_d, 5 * size_t(a.C) * b), "hipMalloc segment");
+  for (void** p : {&a.lam_d, &a.lamu_d, &a.gep_d, &a.nu1_d, &a.nu2_d, &a.part_d, &a.ework_d})
+    if (!*p) aok(hipMalloc(p, b), "hipMalloc adjoint state");
+  a.nblk = m.ns * ((m.ncell + BLOCK - 1) / BLOCK);
+  if (!a.gpart_d) aok(hipMalloc(reinterpret_cast<void**>(&a.gpart_d), sizeof(double) * MAXP * a.nblk), "hipMalloc gpart");
+  if (!a.gout_d) aok(hipMalloc(reinterpret_cast<void**>(&a.gout_d), sizeof(double) * MAXP), "hipMalloc gout");
+}
+
+template <typename R>
+void copy_dev(const void* src, void* dst, size_t nelem, hipStream_t s) {
+  hipLaunchKernelGGL((k_copy<R>), dim3((nelem + 255) / 256), dim3(256), 0, s, static_cast<const R*>(src),
+                     static_cast<R*>(dst), nelem);
+}
+
+template <typename R>
+void run_adjoint_kernels(mx_adjoint& a, const void* e, const void* ep, const void* k1, const void* B1, const void* B2,
+                         hipStream_t s) {
+  const Model& m = *a.m;
+  const Static<R>* S = static_cast<const Static<R>*>(m.static_d);
+  const AStatic<R>* A = static_cast<const AStatic<R>*>(a.astatic_d);
+  Dyn<R> d{};
+  d.dt = R(m.dt);
+  d.prec = m.precession;
+  d.alpha = static_cast<const R*>(m.alpha_d);
+  d.inv_mu = static_cast<const R*>(m.inv_mu_d);
+  const R* E = static_cast<const R*>(e);
+  const R* EP = static_cast<const R*>(ep);
+  const R* K1 = static_cast<const R*>(k1);
+  const R* b1 = static_cast<const R*>(B1);
+  const R* b2 = static_cast<const R*>(B2);
+  R* lam = static_cast<R*>(a.lam_d);
+  R* lamu = static_cast<R*>(a.lamu_d);
+  R* gep = static_cast<R*>(a.gep_d);
+  R* nu1 = static_cast<R*>(a.nu1_d);
+  R* nu2 = static_cast<R*>(a.nu2_d);
+  R* part = static_cast<R*>(a.part_d);
+  const dim3 g(a.nblk), b(BLOCK);
+  if (m.geo == GEO_OPEN) {
+    hipLaunchKernelGGL((k_adj_a<R, GEO_OPEN>), g, b, 0, s, S, d, lam, E, EP, K1, b2, lamu, gep, nu2);
+    hipLaunchKernelGGL((k_adj_b<R, GEO_OPEN>), g, b, 0, s, S, A, d, E, EP, b1, lamu, gep, nu2, nu1, part, a.gpart_d);
+    hipLaunchKernelGGL((k_adj_c<R, GEO_OPEN>), g, b, 0, s, S, A, E, nu1, part, lam, a.gpart_d);
+  } else if (m.geo == GEO_PERIODIC) {
+    hipLaunchKernelGGL((k_adj_a<R, GEO_PERIODIC>), g, b, 0, s, S, d, lam, E, EP, K1, b2, lamu, gep, nu2);
+    hipLaunchKernelGGL((k_adj_b<R, GEO_PERIODIC>), g, b, 0, s, S, A, d, E, EP, b1, lamu, gep, nu2, nu1, part,
+                       a.gpart_d);
+    hipLaunchKernelGGL((k_adj_c<R, GEO_PERIODIC>), g, b, 0, s, S, A, E, nu1, part, lam, a.gpart_d);
+  } else {
+    hipLaunchKernelGGL((k_adj_a<R, GEO_TILED>), g, b, 0, s, S, d, lam, E, EP, K1, b2, lamu, gep, nu2);
+    hipLaunchKernelGGL((k_adj_b<R, GEO_TILED>), g, b, 0, s, S, A, d, E, EP, b1, lamu, gep, nu2, nu1, part, a.gpart_d);
+    hipLaunchKernelGGL((k_adj_c<R, GEO_TILED>), g, b, 0, s, S, A, E, nu1, part, lam, a.gpart_d);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int mx_adjoint_create(mx_model* m, int32_t nparam, const double* unit_bonds, const double* unit_onsite,
+                      int32_t checkpoint_every, mx_adjoint** out) {
+  return aguard([&]() -> int {
+    if (!m || !out || !unit_bonds || !unit_onsite) afail(MX_EINVAL, "null argument");
+    *out = nullptr;
+    if (nparam < 1 || nparam > MAXP) afail(MX_EUNSUPPORTED, "1 to 8 parameters");
+    if (checkpoint_every < 1) afail(MX_EINVAL, "checkpoint interval must be positive");
+    if (m->nent < 1) afail(MX_EINVAL, "set the stencil first");
+    if (m->pinned_d || m->has_c || m->has_u || m->amode != ONSITE_SUB || m->coup == COUP_BOND)
+      afail(MX_EUNSUPPORTED, "the adjoint covers stencil couplings and per-sublattice on-site tensors");
+    if (m->nent * 9 > MAXTAB) afail(MX_EUNSUPPORTED, "stencil too large for a per-entry coupling table");
+    // unit tensors arrive in the caller's stencil order; reorder into the library's, and check that
+    // every entry's partner carries the transposed tensors (the adjoint gather relies on J_ji = J_ij^T)
+    const int n = m->nent;
+    std::vector<int> pos_of(n);
+    for (int pos = 0; pos < n; ++pos) pos_of[m->ent_order[pos]] = pos;
+    auto raw = [&](int k, int q) { return m->ent_raw[6 * size_t(k) + q]; };
+    for (int k = 0; k < n; ++k) {
+      int partner = -1;
+      for (int q = 0; q < n; ++q)
+        if (raw(q, 0) == raw(k, 1) && raw(q, 1) == raw(k, 0) && raw(q, 2) == -raw(k, 2) && raw(q, 3) == -raw(k, 3) &&
+            raw(q, 4) == -raw(k, 4))
+          partner = q;
+      if (partner < 0) afail(MX_EINVAL, "stencil entry without a reverse partner");
+      for (int p = 0; p < nparam; ++p)
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) {
+            const double x = unit_bonds[(size_t(p) * n + k) * 9 + 3 * r + c];
+            const double y = unit_bonds[(size_t(p) * n + partner) * 9 + 3 * c + r];
+            if (std::fabs(x - y) > 1e-12 * (std::fabs(x) + 1.0))
+              afail(MX_EINVAL, "unit couplings of a bond and its reverse must be transposes");
+          }
+    }
+    auto* a = new mx_adjoint();
+    try {
+      a->m = m;
+      a->P = nparam;
+      a->C = checkpoint_every;
+      a->U.assign(size_t(nparam) * n * 9, 0.0);
+      for (int p = 0; p < nparam; ++p)
+        for (int k = 0; k < n; ++k)
+          for (int q = 0; q < 9; ++q)
+            a->U[(size_t(p) * n + pos_of[k]) * 9 + q] = unit_bonds[(size_t(p) * n + k) * 9 + q];
+      a->Aun.assign(unit_onsite, unit_onsite + size_t(nparam) * m->ns * 9);
+      a->theta.assign(nparam, 0.0);
+      aok(hipSetDevice(m->device), "hipSetDevice");
+      if (m->prec == MX_FP64) upload_astatic<double>(*a);
+      else upload_astatic<float>(*a);
+      apply_theta(*a);
+    } catch (...) {
+      mx_adjoint_destroy(a);
+      throw;
+    }
+    *out = a;
+    return MX_OK;
+  });
+}
+
+void mx_adjoint_destroy(mx_adjoint* a) {
+  if (!a) return;
+  if (a->m) (void)hipSetDevice(a->m->device);
+  for (void** p : {&a->astatic_d, &a->ckpt_d, &a->seg_d, &a->lam_d, &a->lamu_d, &a->gep_d, &a->nu1_d, &a->nu2_d,
+                   &a->part_d, &a->ework_d, reinterpret_cast<void**>(&a->gpart_d),
+                   reinterpret_cast<void**>(&a->gout_d)})
+    afree(*p);
+  delete a;
+}
+
+int mx_adjoint_set_params(mx_adjoint* a, const double* theta) {
+  return aguard([&]() -> int {
+    if (!a || !theta) afail(MX_EINVAL, "null argument");
+    a->theta.assign(theta, theta + a->P);
+    apply_theta(*a);
+    a->recorded = false;  // a recorded forward belongs to the previous parameters
+    return MX_OK;
+  });
+}
+
+int mx_adjoint_forward(mx_adjoint* a, void* e, int64_t nsteps, uint64_t seed, uint64_t step0, void* stream) {
+  return aguard([&]() -> int {
+    if (!a || !e) afail(MX_EINVAL, "null argument");
+    if (nsteps < 0) afail(MX_EINVAL, "negative step count");
+    Model& m = *a->m;
+    if (!m.dyn_set) afail(MX_EINVAL, "set the dynamics first");
+    aok(hipSetDevice(m.device), "hipSetDevice");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int64_t nck = nsteps / a->C + 1;
+    const size_t b = nbytes3(m);
+    if (nck > a->ckpt_cap) {
+      afree(a->ckpt_d);
+      aok(hipMalloc(&a->ckpt_d, size_t(nck) * b), "hipMalloc checkpoints");
+      a->ckpt_cap = nck;
+    }
+    ensure_buffers(*a);
+    a->recorded = false;
+    a->nsteps = nsteps;
+    a->seed = seed;
+    a->step0 = step0;
+    // checkpoint k holds the (magex-order) spins after k * C steps
+    char* ck = static_cast<char*>(a->ckpt_d);
+    const size_t nel = size_t(m.N) * 3;
+    for (int64_t k = 0; k * a->C <= nsteps; ++k) {
+      if (m.prec == MX_FP64) copy_dev<double>(e, ck + size_t(k) * b, nel, s);
+      else copy_dev<float>(e, ck + size_t(k) * b, nel, s);
+      const int64_t n = std::min<int64_t>(a->C, nsteps - k * a->C);
+      if (n > 0) aok_rc(mx_llg_run(a->m, e, n, seed, step0 + uint64_t(k * a->C), nullptr, stream), "forward");
+    }
+    a->recorded = true;
+    return MX_OK;
+  });
+}
+
+int mx_adjoint_backward(mx_adjoint* a, const void* grad_e, void* grad_e0, double* grad_theta, void* stream) {
+  return aguard([&]() -> int {
+    if (!a || !grad_e || !grad_theta) afail(MX_EINVAL, "null argument");
+    Model& m = *a->m;
+    if (!a->recorded) afail(MX_EINVAL, "no forward recorded for the current parameters: run mx_adjoint_forward");
+    aok(hipSetDevice(m.device), "hipSetDevice");
+    aok_rc(sync_model_static(m), "model constants");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t b = nbytes3(m), nel = size_t(m.N) * 3;
+    const bool fp64 = m.prec == MX_FP64;
+    // the adjoint works in the library's internal site order; permute at entry and exit
+    if (needs_permute(m)) aok_rc(launch_permute(m, grad_e, a->lam_d, 1, s), "permute");
+    else if (fp64) copy_dev<double>(grad_e, a->lam_d, nel, s);
+    else copy_dev<float>(grad_e, a->lam_d, nel, s);
+    hipLaunchKernelGGL((k_zero<double>), dim3((MAXP * a->nblk + 255) / 256), dim3(256), 0, s, a->gpart_d,
+                       size_t(MAXP) * a->nblk);
+    char* seg = static_cast<char*>(a->seg_d);
+    char* ck = static_cast<char*>(a->ckpt_d);
+    const int C = a->C;
+    auto slot = [&](int which, int n) { return seg + (size_t(which) * C + n) * b; };
+    for (int64_t k = (a->nsteps - 1) / C; k >= 0 && a->nsteps > 0; --k) {
+      const int n_seg = int(std::min<int64_t>(C, a->nsteps - k * C));
+      // replay the segment from its checkpoint (internal order), saving e, ep, k1, B(e), B(ep)
+      void* ew = a->ework_d;
+      if (needs_permute(m)) aok_rc(launch_permute(m, ck + size_t(k) * b, ew, 1, s), "permute");
+      else if (fp64) copy_dev<double>(ck + size_t(k) * b, ew, nel, s);
+      else copy_dev<float>(ck + size_t(k) * b, ew, nel, s);
+      for (int n = 0; n < n_seg; ++n) {
+        const uint64_t step = a->step0 + uint64_t(k * C + n);
+        aok_rc(launch_heun_pass(m, 1, ew, slot(1, n), slot(2, n), a->seed, step, slot(0, n), slot(3, n), s), "replay");
+        aok_rc(launch_heun_pass(m, 2, ew, slot(1, n), slot(2, n), a->seed, step, nullptr, slot(4, n), s), "replay");
+      }
+      for (int n = n_seg - 1; n >= 0; --n) {
+        if (fp64) run_adjoint_kernels<double>(*a, slot(0, n), slot(1, n), slot(2, n), slot(3, n), slot(4, n), s);
+        else run_adjoint_kernels<float>(*a, slot(0, n), slot(1, n), slot(2, n), slot(3, n), slot(4, n), s);
+      }
+    }
+    hipLaunchKernelGGL((k_final<MAXP>), dim3(1), dim3(BLOCK), 0, s, a->gpart_d, a->nblk, MAXP, 1.0, a->gout_d);
+    hipLaunchKernelGGL((k_copy<double>), dim3(1), dim3(64), 0, s, a->gout_d, grad_theta, size_t(a->P));
+    if (grad_e0) {
+      if (needs_permute(m)) aok_rc(launch_permute(m, a->lam_d, grad_e0, 0, s), "permute");
+      else if (fp64) copy_dev<double>(a->lam_d, grad_e0, nel, s);
+      else copy_dev<float>(a->lam_d, grad_e0, nel, s);
+    }
+    const hipError_t err = hipGetLastError();
+    if (err != hipSuccess) afail(MX_EHIP, std::string("adjoint: ") + hipGetErrorString(err));
+    return MX_OK;
+  });
+}
+
+}  // extern "C"

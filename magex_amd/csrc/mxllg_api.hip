@@ -146,6 +146,9 @@ static void ensure_scratch(Model& m) {
 
 static void use_device(const Model& m) { hip_ok(hipSetDevice(m.device), "hipSetDevice"); }
 
+int sync_model_static(Model& m) { return m.static_dirty ? upload_static(m) : MX_OK; }
+size_t prec_size(const Model& m) { return rsize(m); }
+
 // Bring the device constant block up to date (synchronous copy; only after configuration changes).
 static void sync_static(Model& m) {
   if (!m.static_dirty) return;

@@ -70,6 +70,8 @@ struct Dyn {
   const R* sign_site;
   const uint8_t* pinned;
   const uint64_t* step_ptr;  // captured step loops: device step counter added to the step argument
+  R* save_e;                 // adjoint replays: pass 1 stores e here, both passes store B_eff in save_B
+  R* save_B;
 };
 
 // Host-side model.
@@ -133,6 +135,11 @@ int upload_static(Model& m);
 int launch_heun_step(const Model& m, void* e, uint64_t seed, uint64_t step, const void* noise_step, hipStream_t s,
                      const uint64_t* step_ptr = nullptr);
 int launch_counter(uint64_t* c, uint64_t value, int add, hipStream_t s);
+// One Heun pass (1 or 2) with optional stores of e and B_eff (magex order is not used: internal order).
+int launch_heun_pass(const Model& m, int pass, void* e, void* ep, void* k1, uint64_t seed, uint64_t step,
+                     void* save_e, void* save_B, hipStream_t s);
+int sync_model_static(Model& m);
+size_t prec_size(const Model& m);
 int launch_field(const Model& m, const void* e_int, void* out_ref, int kind, hipStream_t s);
 int launch_observables(const Model& m, const void* e_int, double* out, hipStream_t s);
 int launch_noise(const Model& m, void* out_ref, uint64_t seed, uint64_t step, hipStream_t s);

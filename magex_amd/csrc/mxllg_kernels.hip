@@ -19,319 +19,9 @@
 // lattice and the neighbour re-reads of that slab hit its own L2.
 #include <cstring>
 
-#include "mxllg_device.h"
+#include "mxllg_field.h"
 
 namespace mx {
-
-// ------------------------------------------------------------------------------------------ helpers
-struct Site {
-  int s, c, c0, c1, c2, i, crm;  // sublattice, internal cell, coordinates, internal site, magex cell
-};
-
-template <typename R, int GEO>
-__device__ __forceinline__ bool site_of(const Static<R>* __restrict__ S, Site& st) {
-  const int b = xcd_tile(blockIdx.x, gridDim.x);
-  const int ncell = S->ncell, L1 = S->L1, L2 = S->L2;
-  // sublattices interleave tile by tile, so one XCD works on every sublattice of its slab of cells and
-  // the partner reads of per-bond tensors stored by another sublattice hit the same L2
-  const int ns = S->ns;
-  const int tile = b / ns;
-  st.s = b - tile * ns;
-  if constexpr (GEO == GEO_TILED) {
-    // tile coordinates are wavefront-uniform (scalar); the lane's place in the tile is shifts and masks
-    const int b0 = S->tb0, b1 = S->tb1, b2 = S->tb2;
-    const int t2 = tile % S->nt2, tq = tile / S->nt2, t1 = tq % S->nt1, t0 = tq / S->nt1;
-    const int l = threadIdx.x;
-    st.c0 = (t0 << b0) + (l >> (b1 + b2));
-    st.c1 = (t1 << b1) + ((l >> b2) & ((1 << b1) - 1));
-    st.c2 = (t2 << b2) + (l & ((1 << b2) - 1));
-    st.c = tile * BLOCK + l;
-  } else {
-    st.c = tile * BLOCK + threadIdx.x;
-    if (st.c >= ncell) return false;
-    if (S->fdiv) {
-      const int t = fdiv(st.c, L2, S->rL2);
-      st.c2 = st.c - t * L2;
-      st.c0 = fdiv(t, L1, S->rL1);
-      st.c1 = t - st.c0 * L1;
-    } else {
-      st.c2 = st.c % L2;
-      const int t = st.c / L2;
-      st.c1 = t % L1;
-      st.c0 = t / L1;
-    }
-  }
-  st.crm = (st.c0 * L1 + st.c1) * L2 + st.c2;
-  st.i = st.s * ncell + st.c;
-  return true;
-}
-
-// Neighbour cell (internal index) of st along stencil entry en; offsets are reduced, so one conditional
-// wrap per periodic axis.  GEO_OPEN: ok = false past an open face, and the returned cell is the site's
-// own (a valid address; the caller zeroes its contribution).
-template <typename R, int GEO>
-__device__ __forceinline__ int neighbour_cell(const Static<R>* __restrict__ S, const Site& st, const Ent& en,
-                                              bool& ok) {
-  const int L0 = S->L0, L1 = S->L1, L2 = S->L2;
-  int n0 = st.c0 + int(int16_t(en.d01 & 0xffff));
-  int n1 = st.c1 + (en.d01 >> 16);
-  int n2 = st.c2 + en.d2;
-  if constexpr (GEO == GEO_OPEN) {
-    n0 = (S->pbc0 && n0 >= L0) ? n0 - L0 : n0;
-    n1 = (S->pbc1 && n1 >= L1) ? n1 - L1 : n1;
-    n2 = (S->pbc2 && n2 >= L2) ? n2 - L2 : n2;
-    ok = unsigned(n0) < unsigned(L0) && unsigned(n1) < unsigned(L1) && unsigned(n2) < unsigned(L2);
-    return ok ? (n0 * L1 + n1) * L2 + n2 : st.c;
-  } else {
-    n0 = n0 >= L0 ? n0 - L0 : n0;
-    n1 = n1 >= L1 ? n1 - L1 : n1;
-    n2 = n2 >= L2 ? n2 - L2 : n2;
-    ok = true;
-    if constexpr (GEO == GEO_TILED) {
-      const int b0 = S->tb0, b1 = S->tb1, b2 = S->tb2;
-      const int tile = ((n0 >> b0) * S->nt1 + (n1 >> b1)) * S->nt2 + (n2 >> b2);
-      const int l = ((n0 & ((1 << b0) - 1)) << (b1 + b2)) | ((n1 & ((1 << b1) - 1)) << b2) | (n2 & ((1 << b2) - 1));
-      return tile * BLOCK + l;
-    } else {
-      return (n0 * L1 + n1) * L2 + n2;
-    }
-  }
-}
-
-// Entries per batch of the stencil loop: every load of a batch is issued before the first is used, so
-// a wavefront keeps CH gathers (and, per bond, CH tensors) in flight instead of one.
-template <typename R, int CM>
-struct Chunk {
-  static constexpr int value = CM == COUP_BOND ? (sizeof(R) == 8 ? 3 : 5) : 8;
-};
-
-// One batch of CH stencil entries: gather() issues every load of the batch without a branch (entries
-// past the end re-read the last entry and contribute zero), accumulate() adds the batch to h in stencil
-// order.  Splitting the two lets ALU work (the thermal noise) run while the first batch is in flight.
-template <typename R, int CM, int GEO>
-struct Batch {
-  static constexpr int CH = Chunk<R, CM>::value;
-  static constexpr int NJ = CM == COUP_BOND ? 9 : 1;
-  R ex[CH], ey[CH], ez[CH], J[CH][NJ];
-  int slot[CH];
-
-  __device__ __forceinline__ void gather(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
-                                         const Site& st, int k0, int ke) {
-    const int ncell = S->ncell;
-#pragma unroll
-    for (int u = 0; u < CH; ++u) {
-      const bool live = k0 + u < ke;
-      const Ent en = S->ent[live ? k0 + u : ke - 1];
-      slot[u] = en.slot;
-      bool ok;
-      const int cn = neighbour_cell<R, GEO>(S, st, en, ok);
-      ok = ok && live;
-      const int j = (en.tb & 0xffff) * ncell + cn;
-      ex[u] = e[3 * j];
-      ey[u] = e[3 * j + 1];
-      ez[u] = e[3 * j + 2];
-      ex[u] = ok ? ex[u] : R(0);
-      ey[u] = ok ? ey[u] : R(0);
-      ez[u] = ok ? ez[u] : R(0);
-      if constexpr (CM == COUP_BOND) {
-        // per-bond tensor stored once per unordered bond: a forward entry reads its own cell's copy,
-        // a backward entry its partner's copy at the neighbour cell, transposed (J_ji = J_ij^T)
-        const bool back = (en.tb >> 16) != 0;
-        const R* Jb = d.bond + (size_t(en.slot) * ncell + (back ? cn : st.c)) * 9;
-        R t[9];
-#pragma unroll
-        for (int q = 0; q < 9; ++q) t[q] = Jb[q];
-#pragma unroll
-        for (int q = 0; q < 9; ++q) J[u][q] = back ? t[(q % 3) * 3 + q / 3] : t[q];
-      }
-    }
-  }
-
-  __device__ __forceinline__ void accumulate(const Static<R>* __restrict__ S, R& hx, R& hy, R& hz) const {
-#pragma unroll
-    for (int u = 0; u < CH; ++u) {
-      const R x = ex[u], y = ey[u], z = ez[u];
-      if constexpr (CM == COUP_TABLE1) {
-        const R Jv = S->tab[slot[u]];
-        hx += Jv * x;
-        hy += Jv * y;
-        hz += Jv * z;
-      } else if constexpr (CM == COUP_TABLE4) {
-        // J I + antisymmetric part: (J^A e_j) = e_j x D
-        const R* T = S->tab + 4 * slot[u];
-        hx += T[0] * x + (y * T[3] - z * T[2]);
-        hy += T[0] * y + (z * T[1] - x * T[3]);
-        hz += T[0] * z + (x * T[2] - y * T[1]);
-      } else if constexpr (CM == COUP_TABLE9) {
-        const R* T = S->tab + 9 * slot[u];
-        hx += T[0] * x + T[1] * y + T[2] * z;
-        hy += T[3] * x + T[4] * y + T[5] * z;
-        hz += T[6] * x + T[7] * y + T[8] * z;
-      } else {
-        hx += J[u][0] * x + J[u][1] * y + J[u][2] * z;
-        hy += J[u][3] * x + J[u][4] * y + J[u][5] * z;
-        hz += J[u][6] * x + J[u][7] * y + J[u][8] * z;
-      }
-    }
-  }
-};
-
-// h_i = sum_j J_ij e_j over the stencil of site st (magex/fast.py:86-120 semantics), accumulated in
-// stencil order; mid() runs after the first batch's loads are issued and before they are used.
-template <typename R, int CM, int GEO, typename Mid>
-__device__ __forceinline__ void exchange(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
-                                         const Site& st, R& hx, R& hy, R& hz, Mid&& mid) {
-  hx = hy = hz = R(0);
-  const int kb = S->sptr[st.s], ke = S->sptr[st.s + 1];
-  if (kb >= ke) {
-    mid();
-    return;
-  }
-  Batch<R, CM, GEO> bt;
-  bt.gather(S, d, e, st, kb, ke);
-  mid();
-  bt.accumulate(S, hx, hy, hz);
-  for (int k0 = kb + Batch<R, CM, GEO>::CH; k0 < ke; k0 += Batch<R, CM, GEO>::CH) {
-    bt.gather(S, d, e, st, k0, ke);
-    bt.accumulate(S, hx, hy, hz);
-  }
-}
-
-template <typename R, int CM, int GEO>
-__device__ __forceinline__ void exchange(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
-                                         const Site& st, R& hx, R& hy, R& hz) {
-  exchange<R, CM, GEO>(S, d, e, st, hx, hy, hz, [] {});
-}
-
-// On-site gradient dE/de (magex/fast.py:39-64): 2 A e + cubic + higher uniaxial.
-template <typename R>
-__device__ __forceinline__ void onsite_grad(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st, R x, R y,
-                                            R z, R& gx, R& gy, R& gz) {
-  gx = gy = gz = R(0);
-  const int amode = S->amode;
-  if (amode == ONSITE_SUB) {
-    if (S->has_A) {
-      const R* a = S->Asub[st.s];
-      gx = R(2) * (a[0] * x + a[1] * y + a[2] * z);
-      gy = R(2) * (a[3] * x + a[4] * y + a[5] * z);
-      gz = R(2) * (a[6] * x + a[7] * y + a[8] * z);
-    }
-  } else if (amode == ONSITE_SITE6) {
-    const int N = S->N, i = st.i;
-    const R* A = d.A;
-    const R axx = A[i], ayy = A[N + i], azz = A[2 * N + i];
-    const R axy = A[3 * N + i], axz = A[4 * N + i], ayz = A[5 * N + i];
-    gx = R(2) * (axx * x + axy * y + axz * z);
-    gy = R(2) * (axy * x + ayy * y + ayz * z);
-    gz = R(2) * (axz * x + ayz * y + azz * z);
-  } else {
-    const int N = S->N;
-    const R* a = d.A + st.i;
-    gx = R(2) * (a[0] * x + a[N] * y + a[2 * N] * z);
-    gy = R(2) * (a[3 * N] * x + a[4 * N] * y + a[5 * N] * z);
-    gz = R(2) * (a[6 * N] * x + a[7 * N] * y + a[8 * N] * z);
-  }
-  if (S->has_c) {
-    const R* C = S->C;
-    const R a1 = C[0] * x + C[1] * y + C[2] * z;
-    const R a2 = C[3] * x + C[4] * y + C[5] * z;
-    const R a3 = C[6] * x + C[7] * y + C[8] * z;
-    const R q1 = a1 * a1, q2 = a2 * a2, q3 = a3 * a3, kc = d.Kc[st.i];
-    const R d1 = R(2) * a1 * (q2 + q3) * kc, d2 = R(2) * a2 * (q1 + q3) * kc, d3 = R(2) * a3 * (q1 + q2) * kc;
-    gx += d1 * C[0] + d2 * C[3] + d3 * C[6];
-    gy += d1 * C[1] + d2 * C[4] + d3 * C[7];
-    gz += d1 * C[2] + d2 * C[5] + d3 * C[8];
-  }
-  if (S->has_u) {
-    const R* u = S->u;
-    const R c = x * u[0] + y * u[1] + z * u[2];
-    const R c2 = c * c;
-    const R f = R(-4) * d.K2[st.i] * c2 * c - R(6) * d.K3[st.i] * c2 * c2 * c;
-    gx += f * u[0];
-    gy += f * u[1];
-    gz += f * u[2];
-  }
-}
-
-// On-site energy (magex/fast.py:66-84).
-template <typename R>
-__device__ __forceinline__ R onsite_energy(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st, R x, R y,
-                                          R z) {
-  R E = R(0);
-  const int amode = S->amode;
-  if (amode == ONSITE_SUB) {
-    if (S->has_A) {
-      const R* a = S->Asub[st.s];
-      E = x * (a[0] * x + a[1] * y + a[2] * z) + y * (a[3] * x + a[4] * y + a[5] * z) +
-          z * (a[6] * x + a[7] * y + a[8] * z);
-    }
-  } else if (amode == ONSITE_SITE6) {
-    const int N = S->N, i = st.i;
-    const R* A = d.A;
-    const R axx = A[i], ayy = A[N + i], azz = A[2 * N + i];
-    const R axy = A[3 * N + i], axz = A[4 * N + i], ayz = A[5 * N + i];
-    E = axx * x * x + ayy * y * y + azz * z * z + R(2) * (axy * x * y + axz * x * z + ayz * y * z);
-  } else {
-    const int N = S->N;
-    const R* a = d.A + st.i;
-    E = x * (a[0] * x + a[N] * y + a[2 * N] * z) + y * (a[3 * N] * x + a[4 * N] * y + a[5 * N] * z) +
-        z * (a[6 * N] * x + a[7 * N] * y + a[8 * N] * z);
-  }
-  if (S->has_c) {
-    const R* C = S->C;
-    const R a1 = C[0] * x + C[1] * y + C[2] * z;
-    const R a2 = C[3] * x + C[4] * y + C[5] * z;
-    const R a3 = C[6] * x + C[7] * y + C[8] * z;
-    const R q1 = a1 * a1, q2 = a2 * a2, q3 = a3 * a3;
-    E += d.Kc[st.i] * (q1 * q2 + q2 * q3 + q3 * q1);
-  }
-  if (S->has_u) {
-    const R c = x * S->u[0] + y * S->u[1] + z * S->u[2];
-    const R c4 = c * c * c * c;
-    E += -d.K2[st.i] * c4 - d.K3[st.i] * c4 * c * c;
-  }
-  return E;
-}
-
-template <typename R>
-__device__ __forceinline__ void thermal(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st,
-                                        uint64_t step, const R* __restrict__ noise, R& bx, R& by, R& bz) {
-  const R sg = d.sigma ? d.sigma[st.i] : S->sigma_sub[st.s];
-  const int iref = st.crm * S->ns + st.s;
-  if (noise) {
-    bx = sg * noise[3 * iref];
-    by = sg * noise[3 * iref + 1];
-    bz = sg * noise[3 * iref + 2];
-  } else {
-    float n0, n1, n2;
-    gauss3(d.seed_lo, d.seed_hi, step, uint32_t(iref), n0, n1, n2);
-    bx = sg * R(n0);
-    by = sg * R(n1);
-    bz = sg * R(n2);
-  }
-}
-
-template <typename R>
-__device__ __forceinline__ R inv_mu_of(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st) {
-  return d.inv_mu ? d.inv_mu[st.i] : S->inv_mu_sub[st.s];
-}
-
-// B_i = (2 h_i - g_i) / (mu_i mu_B) + B (+ thermal) at spin (x, y, z) (magex/fast.py:309-317).  The
-// generator's normals are computed while the first neighbour batch is in flight.
-template <typename R, int CM, int GEO>
-__device__ __forceinline__ void local_field(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
-                                            const Site& st, R x, R y, R z, uint64_t step,
-                                            const R* __restrict__ noise, R& bx, R& by, R& bz) {
-  R hx, hy, hz, gx, gy, gz, nx = R(0), ny = R(0), nz = R(0);
-  exchange<R, CM, GEO>(S, d, e, st, hx, hy, hz, [&] {
-    if (d.noisy) thermal(S, d, st, step, noise, nx, ny, nz);
-  });
-  onsite_grad(S, d, st, x, y, z, gx, gy, gz);
-  const R im = inv_mu_of(S, d, st);
-  bx = (R(2) * hx - gx) * im + d.B0 + nx;
-  by = (R(2) * hy - gy) * im + d.B1 + ny;
-  bz = (R(2) * hz - gz) * im + d.B2 + nz;
-}
 
 // ------------------------------------------------------------------------------------------ Heun
 // PASS 1 (magex/fast.py:336-368): noise + field(e) + predictor ep = n(e + dt k1), keeps k1.
@@ -358,6 +48,14 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S,
     }
     R bx, by, bz, fx, fy, fz;
     local_field<R, CM, GEO>(S, d, e, st, x, y, z, step, noise, bx, by, bz);
+    if (d.save_e) {
+      d.save_e[3 * i] = x;
+      d.save_e[3 * i + 1] = y;
+      d.save_e[3 * i + 2] = z;
+      d.save_B[3 * i] = bx;
+      d.save_B[3 * i + 1] = by;
+      d.save_B[3 * i + 2] = bz;
+    }
     llg_rhs(x, y, z, bx, by, bz, a, d.prec, fx, fy, fz);
     k1[3 * i] = fx;
     k1[3 * i + 1] = fy;
@@ -372,6 +70,11 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S,
     const R x = ep[3 * i], y = ep[3 * i + 1], z = ep[3 * i + 2];
     R bx, by, bz, fx, fy, fz;
     local_field<R, CM, GEO>(S, d, ep, st, x, y, z, step, noise, bx, by, bz);
+    if (d.save_B) {
+      d.save_B[3 * i] = bx;
+      d.save_B[3 * i + 1] = by;
+      d.save_B[3 * i + 2] = bz;
+    }
     llg_rhs(x, y, z, bx, by, bz, a, d.prec, fx, fy, fz);
     const R hdt = R(0.5) * d.dt;
     const R cx = e[3 * i] + hdt * (k1[3 * i] + fx);
@@ -609,6 +312,8 @@ static Dyn<R> make_dyn(const Model& m, uint64_t seed) {
   d.sign_site = static_cast<const R*>(m.sign_d);
   d.pinned = static_cast<const uint8_t*>(m.pinned_d);
   d.step_ptr = nullptr;
+  d.save_e = nullptr;
+  d.save_B = nullptr;
   return d;
 }
 
@@ -654,6 +359,32 @@ int launch_heun_step(const Model& m, void* e, uint64_t seed, uint64_t step, cons
                      const uint64_t* step_ptr) {
   return m.prec == MX_FP64 ? heun_t<double>(m, e, seed, step, noise, s, step_ptr)
                            : heun_t<float>(m, e, seed, step, noise, s, step_ptr);
+}
+
+template <typename R>
+static int pass_t(const Model& m, int pass, void* e, void* ep, void* k1, uint64_t seed, uint64_t step, void* save_e,
+                  void* save_B, hipStream_t s) {
+  const Static<R>* S = static_cast<const Static<R>*>(m.static_d);
+  Dyn<R> d = make_dyn<R>(m, seed);
+  d.save_e = static_cast<R*>(save_e);
+  d.save_B = static_cast<R*>(save_B);
+  R* E = static_cast<R*>(e);
+  R* EP = static_cast<R*>(ep);
+  R* K1 = static_cast<R*>(k1);
+  const R* nz = nullptr;
+  const dim3 g(nblk_of(m)), b(BLOCK);
+#define MX_PASS_CALL(CM, GEO)                                                                 \
+  if (pass == 1) hipLaunchKernelGGL((k_heun<R, CM, GEO, 1>), g, b, 0, s, S, d, E, EP, K1, step, nz); \
+  else hipLaunchKernelGGL((k_heun<R, CM, GEO, 2>), g, b, 0, s, S, d, E, EP, K1, step, nz)
+  MX_DISPATCH(m, MX_PASS_CALL);
+#undef MX_PASS_CALL
+  return check_launch("heun pass");
+}
+
+int launch_heun_pass(const Model& m, int pass, void* e, void* ep, void* k1, uint64_t seed, uint64_t step,
+                     void* save_e, void* save_B, hipStream_t s) {
+  return m.prec == MX_FP64 ? pass_t<double>(m, pass, e, ep, k1, seed, step, save_e, save_B, s)
+                           : pass_t<float>(m, pass, e, ep, k1, seed, step, save_e, save_B, s);
 }
 
 int launch_counter(uint64_t* c, uint64_t value, int add, hipStream_t s) {

@@ -1,0 +1,81 @@
+"""FP64 torch.autograd restatement of magex's Heun trajectory, for gradients (test infrastructure only).
+
+TEST INFRASTRUCTURE ONLY.  Nothing in ``magex_amd`` imports this module; only ``tests/`` uses it, as
+the checker of the library's adjoint.  magex has no gradients; this module differentiates its forward
+by automatic differentiation: the trajectory of magex_ref.run_heun (magex/llg.py:216-227, the field of
+magex/spinmodel.py:454-473) written in torch FP64 on the CPU with the couplings linear in parameters
+theta, Jsp(theta) = sum_p theta_p Jsp_p (+ a fixed remainder), A(theta) likewise.
+
+Pinning (tests/test_oracle.py): the forward equals magex_ref.run_heun at the same theta to round-off
+(magex_ref is pinned to magex's golden trajectories); the gradient is torch's reverse mode of that
+forward, checked against central finite differences of the same forward.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import magex_ref as R
+
+
+def _coo(M):
+    M = M.tocoo()
+    idx = torch.as_tensor(np.vstack([M.row, M.col]), dtype=torch.int64)
+    return torch.sparse_coo_tensor(idx, torch.as_tensor(M.data, dtype=torch.float64), M.shape).coalesce()
+
+
+class TorchHeun:
+    """build(theta) -> magex_ref.SpinSystem, linear in theta; units by differences at theta_ref."""
+
+    def __init__(self, build, theta_ref, alpha, dt, precession=True):
+        theta_ref = np.asarray(theta_ref, float)
+        base = build(theta_ref)
+        self.N = base.N
+        Jp, Ap = [], []
+        for p in range(len(theta_ref)):
+            th = theta_ref.copy()
+            th[p] += 1.0
+            s = build(th)
+            Jp.append(s.Jsp - base.Jsp)
+            Ap.append(s.A - base.A)
+        J0 = base.Jsp - sum(t * M for t, M in zip(theta_ref, Jp))
+        A0 = base.A - sum(t * M for t, M in zip(theta_ref, Ap))
+        self.Jp = [_coo(M) for M in Jp] + [_coo(J0)]
+        self.Ap = torch.as_tensor(np.stack(Ap + [A0]))
+        self.mu = torch.as_tensor(base.mu)
+        self.B = torch.as_tensor(base.B)
+        self.alpha, self.dt, self.precession = float(alpha), float(dt), precession
+        self.base = base
+
+    def effective_field(self, th, e):
+        """B_eff (spinmodel.py:458-473) with theta th (torch, P) and e (N, 3)."""
+        thf = torch.cat([th, th.new_ones(1)])
+        x = e.reshape(-1, 1)
+        h = sum(thf[p] * torch.sparse.mm(self.Jp[p], x) for p in range(len(self.Jp))).reshape(-1, 3)
+        A = torch.einsum("p,piab->iab", thf, self.Ap)
+        g = -2.0 * h - R.MU_B * self.mu[:, None] * self.B[None, :] + 2.0 * torch.einsum("iab,ib->ia", A, e)
+        return -g / (R.MU_B * self.mu[:, None])
+
+    def rhs(self, e, B):
+        a = self.alpha
+        pre = -R.GAMMA / (1.0 + a * a)
+        exB = torch.cross(e, B, dim=1)
+        out = a * torch.cross(e, exB, dim=1)
+        if self.precession:
+            out = out + exB
+        return pre * out
+
+    @staticmethod
+    def normalize(u):
+        return u / torch.linalg.norm(u, dim=1, keepdim=True)
+
+    def run(self, th, e, steps, thermal=None):
+        """steps Heun steps; thermal(n) -> (N, 3) thermal field of step n (torch FP64) or None."""
+        dt = self.dt
+        for n in range(steps):
+            b = 0.0 if thermal is None else thermal(n)
+            k1 = self.rhs(e, self.effective_field(th, e) + b)
+            ep = self.normalize(e + dt * k1)
+            k2 = self.rhs(ep, self.effective_field(th, ep) + b)
+            e = self.normalize(e + 0.5 * dt * (k1 + k2))
+        return e

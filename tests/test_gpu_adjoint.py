@@ -1,0 +1,195 @@
+"""Parity of the differentiable form: the library's adjoint against the FP64 torch.autograd restatement
+of the same trajectory (oracle/torch_heun.py) and against central finite differences of the library's
+own forward.
+
+Tolerances: FP64 gradients match autograd to 1e-9 relative (round-off of two different evaluation
+orders through a few hundred steps) and central differences with h = 1e-4 meV to 1e-6 relative; the
+FP32 gradient of the square 128^2 workload stays within 2e-3 of the FP64 one (its trajectory carries
+FP32 round-off through 500 steps).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import magex_ref as R
+from oracle.torch_heun import TorchHeun
+
+pytestmark = pytest.mark.gpu
+
+THETA0 = np.array([4.0, -0.4, 1.0, 0.2])        # J1, J2, D, K (meV)
+THETA_T = np.array([5.0, -0.5, 1.5, 0.3])
+
+
+def _builders(L):
+    from magex_amd import configs
+    dev = lambda th: configs.square128(*th, L=L)[0]                                     # noqa: E731
+    ref = lambda th: R.SpinSystem(R.load_results(R.make_model("square", J=[th[0], th[1]], D=th[2], K=th[3])),
+                                  (L, L, 1))                                           # noqa: E731
+    return dev, ref
+
+
+def _solver(L, precision="fp64", C=50, T=0.0, seed=0, alpha=0.3, dt=1e-3):
+    from magex_amd.adjoint import DifferentiableLLG
+    dev, _ = _builders(L)
+    return DifferentiableLLG(dev, THETA0, alpha=alpha, dt=dt, temperature=T, seed=seed, precision=precision,
+                             checkpoint_every=C)
+
+
+def _mse_grad(e, tgt):
+    return 2.0 * (e - tgt) / e.numel()
+
+
+def _grads(sol, theta, e0, tgt, steps):
+    e = sol.forward(theta, e0, steps)
+    gt, g0 = sol.backward(_mse_grad(e, tgt))
+    torch.cuda.synchronize()
+    return e, gt.cpu().numpy(), g0.double().cpu().numpy()
+
+
+def _relerr(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300))
+
+
+def test_adjoint_matches_fp64_autograd(gpu):
+    print("case: adjoint vs autograd, square 16^2, 120 steps, C=50", flush=True)
+    L, steps = 16, 120
+    sol = _solver(L, C=50)
+    _, ref_build = _builders(L)
+    o = ref_build(THETA0)
+    e0 = o.init_random(0)
+    tgt = torch.as_tensor(o.init_random(1))
+    e, gt, g0 = _grads(sol, THETA0, e0, tgt.cuda(), steps)
+    Th = TorchHeun(ref_build, THETA0, alpha=0.3, dt=1e-3)
+    t = torch.tensor(THETA0, requires_grad=True)
+    x0 = torch.tensor(e0, requires_grad=True)
+    eT = Th.run(t, x0, steps)
+    ((eT - tgt) ** 2).mean().backward()
+    assert np.abs(e.cpu().numpy() - eT.detach().numpy()).max() < 1e-12
+    assert _relerr(gt, t.grad.numpy()) < 1e-9, (gt, t.grad)
+    assert _relerr(g0, x0.grad.numpy()) < 1e-9
+
+
+def test_adjoint_thermal_matches_autograd(gpu):
+    """T > 0: the gradient of one sample path, the thermal field read back from the device generator."""
+    print("case: adjoint thermal, square 12^2, 40 steps, C=16", flush=True)
+    L, steps, T, seed = 12, 40, 50.0, 7
+    sol = _solver(L, C=16, T=T, seed=seed)
+    _, ref_build = _builders(L)
+    o = ref_build(THETA0)
+    e0 = o.init_random(2)
+    tgt = torch.as_tensor(o.init_random(3))
+    e, gt, g0 = _grads(sol, THETA0, e0, tgt.cuda(), steps)
+    sg = R.sigma(o, 0.3, T, 1e-3)
+    fields = [torch.as_tensor(sg[:, None] * sol.dm.noise(seed, n).double().cpu().numpy()) for n in range(steps)]
+    Th = TorchHeun(ref_build, THETA0, alpha=0.3, dt=1e-3)
+    t = torch.tensor(THETA0, requires_grad=True)
+    x0 = torch.tensor(e0, requires_grad=True)
+    eT = Th.run(t, x0, steps, thermal=lambda n: fields[n])
+    ((eT - tgt) ** 2).mean().backward()
+    assert np.abs(e.cpu().numpy() - eT.detach().numpy()).max() < 1e-12
+    assert _relerr(gt, t.grad.numpy()) < 1e-9, (gt, t.grad)
+    assert _relerr(g0, x0.grad.numpy()) < 1e-9
+
+
+def test_adjoint_checkpoint_interval_is_exact(gpu):
+    """Replays from checkpoints recompute the forward bit for bit, so the interval does not change the
+    gradient at all (segments of 50, 7 and one segment of the whole run)."""
+    print("case: adjoint checkpoint intervals, square 20^2, 100 steps", flush=True)
+    L, steps = 20, 100
+    _, ref_build = _builders(L)
+    o = ref_build(THETA0)
+    e0 = o.init_random(4)
+    tgt = torch.as_tensor(o.init_random(5)).cuda()
+    out = [_grads(_solver(L, C=C), THETA0, e0, tgt, steps) for C in (50, 7, 1000)]
+    for e, gt, g0 in out[1:]:
+        assert torch.equal(e, out[0][0])
+        assert np.array_equal(gt, out[0][1])
+        assert np.array_equal(g0, out[0][2])
+
+
+def test_adjoint_forward_is_the_plain_trajectory(gpu):
+    """mx_adjoint_forward at theta is the plain solver on the builder's system at theta."""
+    print("case: adjoint forward vs LLGSolver, square 32^2", flush=True)
+    from magex_amd.llg import LLGSolver
+    L = 32
+    dev, _ = _builders(L)
+    sol = _solver(L)
+    s = dev(THETA_T)
+    e0 = s.init_spins("random", seed=6)
+    e = sol.forward(THETA_T, e0, 60).cpu().numpy()
+    ref = LLGSolver(s, 0.3, 1e-3, 0.0, 0, precision="fp64")
+    x = torch.as_tensor(e0, device="cuda")
+    ref.step_device(x, 60)
+    assert np.abs(e - x.cpu().numpy()).max() < 1e-12
+
+
+def test_adjoint_square128_matches_central_differences(gpu):
+    """The differentiation workload: square 128^2, 500 damped steps, checkpoints every 50, MSE to the
+    texture of the target constants from the same start; FP64 gradient vs central differences
+    (h = 1e-4 meV) of the library's forward, to 1e-6 relative per parameter."""
+    print("case: adjoint square128 vs central differences (FP64)", flush=True)
+    from magex_amd import configs
+    L, steps, h = 128, 500, 1e-4
+    sol = _solver(L, C=50)
+    s0 = configs.square128(L=L)[0]
+    e0 = s0.init_spins("random", seed=0)
+    tgt = sol.forward(THETA_T, e0, steps)
+
+    def loss(th):
+        e = sol.forward(th, e0, steps)
+        return float(((e - tgt) ** 2).mean())
+
+    e, gt, g0 = _grads(sol, THETA0, e0, tgt, steps)
+    for p in range(4):
+        d = np.zeros(4)
+        d[p] = h
+        fd = (loss(THETA0 + d) - loss(THETA0 - d)) / (2 * h)
+        print(f"  theta[{p}]: adjoint {gt[p]:.12e}  central difference {fd:.12e}", flush=True)
+        assert abs(gt[p] - fd) <= 1e-6 * abs(fd), (p, gt[p], fd)
+    # two components of the initial spins (e0 is not renormalized by the solver's first step)
+    for site, comp in ((100, 0), (9000, 2)):
+        def loss_e(dx):
+            x = torch.as_tensor(e0, device="cuda").clone()
+            x[site, comp] += dx
+            e = sol.forward(THETA0, x, steps)
+            return float(((e - tgt) ** 2).mean())
+        fd = (loss_e(h) - loss_e(-h)) / (2 * h)
+        assert abs(g0[site, comp] - fd) <= 1e-6 * abs(fd) + 1e-14, (site, comp, g0[site, comp], fd)
+
+
+def test_adjoint_fp32_tracks_fp64(gpu):
+    print("case: adjoint square128 FP32 vs FP64", flush=True)
+    from magex_amd import configs
+    L, steps = 128, 500
+    e0 = configs.square128(L=L)[0].init_spins("random", seed=0)
+    res = {}
+    for prec in ("fp64", "fp32"):
+        sol = _solver(L, precision=prec, C=50)
+        tgt = sol.forward(THETA_T, e0, steps)
+        e = sol.forward(THETA0, e0, steps)
+        gt, _ = sol.backward(_mse_grad(e, tgt))
+        res[prec] = gt.cpu().numpy()
+    print(f"  fp64 {res['fp64']}\n  fp32 {res['fp32']}", flush=True)
+    assert _relerr(res["fp32"], res["fp64"]) < 2e-3
+
+
+def test_autograd_function(gpu):
+    """trajectory() as a torch.autograd.Function: FP32 leaf parameters, loss.backward(), and a backward
+    that follows another forward (which re-runs its own forward first)."""
+    print("case: autograd Function, square 16^2", flush=True)
+    L, steps = 16, 80
+    sol = _solver(L, C=20)
+    _, ref_build = _builders(L)
+    o = ref_build(THETA0)
+    e0 = torch.as_tensor(o.init_random(8), device="cuda")
+    tgt = torch.as_tensor(o.init_random(9), device="cuda")
+    theta = torch.tensor(THETA0, dtype=torch.float32, requires_grad=True)
+    x0 = e0.clone().requires_grad_(True)
+    e = sol.trajectory(theta, x0, steps)
+    loss = ((e - tgt) ** 2).mean()
+    sol.forward(THETA_T, e0, steps)             # an unrelated forward in between
+    loss.backward()
+    _, gt, g0 = _grads(sol, theta.detach().double().numpy(), e0, tgt, steps)
+    assert theta.grad.dtype == torch.float32
+    assert _relerr(theta.grad.double().numpy(), gt) < 1e-6
+    assert _relerr(x0.grad.cpu().numpy(), g0) < 1e-12

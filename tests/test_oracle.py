@@ -208,3 +208,71 @@ def test_planted_mistakes_are_caught(planted):
         s.A = s.A * 0.0
     e = R.run_heun(s, g["e0"].copy(), 10, 0.05, 5e-4, T=T, normals=lambda n: g["noise"][n])
     assert np.abs(e - g["e10"]).max() > 1e-6
+
+
+# ------------------------------------------------------------------ FP64 autograd restatement
+def _square_builder(L):
+    return lambda th: R.SpinSystem(R.load_results(R.make_model("square", J=[th[0], th[1]], D=th[2], K=th[3])),
+                                   (L, L, 1))
+
+
+def test_torch_heun_forward_matches_oracle():
+    """The autograd restatement's trajectory is magex_ref.run_heun's (T = 0 and with a thermal field)."""
+    import torch
+    from oracle.torch_heun import TorchHeun
+    th = np.array([4.0, -0.4, 1.0, 0.2])
+    build = _square_builder(6)
+    T = TorchHeun(build, th, alpha=0.3, dt=1e-3)
+    o = build(th)
+    e0 = o.init_random(5)
+    ref = R.run_heun(o, e0.copy(), 25, 0.3, 1e-3)
+    got = T.run(torch.as_tensor(th), torch.as_tensor(e0), 25).numpy()
+    assert np.abs(got - ref).max() < 1e-13
+    # a different theta than the reference point of the units
+    th2 = np.array([5.0, -0.5, 1.5, 0.3])
+    ref2 = R.run_heun(build(th2), e0.copy(), 25, 0.3, 1e-3)
+    got2 = T.run(torch.as_tensor(th2), torch.as_tensor(e0), 25).numpy()
+    assert np.abs(got2 - ref2).max() < 1e-13
+    # thermal field: magex_ref draws sigma * normals; the restatement takes the field itself
+    rng = np.random.default_rng(1)
+    nz = [rng.normal(size=(o.N, 3)) for _ in range(10)]
+    sg = R.sigma(o, 0.3, 30.0, 1e-3)
+    ref3 = R.run_heun(o, e0.copy(), 10, 0.3, 1e-3, T=30.0, normals=lambda n: nz[n])
+    got3 = T.run(torch.as_tensor(th), torch.as_tensor(e0), 10,
+                 thermal=lambda n: torch.as_tensor(sg[:, None] * nz[n])).numpy()
+    assert np.abs(got3 - ref3).max() < 1e-13
+
+
+def test_torch_heun_gradient_matches_central_differences():
+    import torch
+    from oracle.torch_heun import TorchHeun
+    th = np.array([4.0, -0.4, 1.0, 0.2])
+    T = TorchHeun(_square_builder(5), th, alpha=0.3, dt=1e-3)
+    e0 = T.base.init_random(2)
+    tgt = torch.as_tensor(T.base.init_random(3))
+
+    def loss(t, e):
+        return ((T.run(t, e, 20) - tgt) ** 2).mean()
+
+    t = torch.tensor(th, requires_grad=True)
+    e = torch.tensor(e0, requires_grad=True)
+    loss(t, e).backward()
+    h = 1e-4
+    with torch.no_grad():
+        for p in range(4):
+            d = torch.zeros(4, dtype=torch.float64)
+            d[p] = h
+            fd = (loss(t + d, e) - loss(t - d, e)) / (2 * h)
+            assert abs(fd - t.grad[p]) <= 1e-7 * abs(fd) + 1e-12, p
+        # one spin component
+        d = torch.zeros_like(e)
+        d[3, 1] = h
+        fd = (loss(t, e + d) - loss(t, e - d)) / (2 * h)
+        assert abs(fd - e.grad[3, 1]) <= 1e-7 * abs(fd) + 1e-12
+    # a planted mistake: a sign error in the DMI unit must show in the gradient check above
+    Tbad = TorchHeun(_square_builder(5), th, alpha=0.3, dt=1e-3)
+    Tbad.Jp[2] = -Tbad.Jp[2]
+    with torch.no_grad():
+        a = ((Tbad.run(torch.as_tensor(th), torch.as_tensor(e0), 20) - tgt) ** 2).mean()
+        b = loss(torch.as_tensor(th), torch.as_tensor(e0))
+    assert abs(a - b) > 1e-9
