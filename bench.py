@@ -148,6 +148,46 @@ def batch_rates(steps, per_replica_steps):
             "speedup_vs_per_replica": rate / base, "mean_abs_m_range": [float(m.min()), float(m.max())]}
 
 
+def adjoint_rates(steps=500, checkpoint_every=50, reps=3):
+    """The differentiable form on square 128^2 (16 384 spins): J1, J2, interfacial D, K as parameters at
+    (4, -0.4, 1.0, 0.2) meV, 500 damped Heun steps (alpha = 0.3, dt = 1 fs, T = 0) from a seeded random
+    start, MSE to the texture of (5, -0.5, 1.5, 0.3) meV, checkpoints every 50 steps, FP32.  forward =
+    the recorded forward (mx_adjoint_forward), backward = segment replays + adjoint sweep
+    (mx_adjoint_backward); both timed with HIP events on the stream they run on, best of reps."""
+    import numpy as np
+    import torch
+    from magex_amd import configs
+    from magex_amd.adjoint import DifferentiableLLG
+    theta0, theta_t = np.array([4.0, -0.4, 1.0, 0.2]), np.array([5.0, -0.5, 1.5, 0.3])
+    sol = DifferentiableLLG(lambda th: configs.square128(*th)[0], theta0, alpha=0.3, dt=1e-3, precision="fp32",
+                            checkpoint_every=checkpoint_every)
+    e0 = sol.system.init_spins("random", seed=0)
+    tgt = sol.forward(theta_t, e0, steps)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    fwd, bwd = [], []
+    for r in range(reps + 1):
+        torch.cuda.synchronize()
+        ev[0].record()
+        e = sol.forward(theta0, e0, steps)
+        ev[1].record()
+        g = 2.0 * (e - tgt) / e.numel()
+        torch.cuda.synchronize()
+        ev[1].synchronize()
+        t_f = ev[0].elapsed_time(ev[1])
+        ev[1].record()
+        gt, _ = sol.backward(g)
+        ev[2].record()
+        torch.cuda.synchronize()
+        if r:                                   # the first round warms up
+            fwd.append(t_f)
+            bwd.append(ev[1].elapsed_time(ev[2]))
+    f, b = min(fwd), min(bwd)
+    return {"workload": "square 128^2 (16384 spins), J1 J2 D K learnable, 500 steps alpha=0.3 dt=1 fs T=0, "
+                        "checkpoints every 50, MSE loss, FP32",
+            "forward_ms": f, "backward_ms": b, "grad_over_forward": b / f, "forward_plus_grad_over_forward": (f + b) / f,
+            "grad_theta": [float(x) for x in gt.cpu().numpy()]}
+
+
 def traffic_from_profiles(workload_name, precision):
     """HBM bytes per step measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE), if committed."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload_name}_{precision}.txt")
@@ -169,6 +209,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=300)
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 256 MiB device copy first (the reference kernel for rocprof FETCH/WRITE_SIZE)")
     a = ap.parse_args()
@@ -210,6 +251,9 @@ def main():
     if not a.no_batch:
         print("batched replicas ...", file=sys.stderr, flush=True)
         out["batched"] = batch_rates(a.batch_steps, max(20, a.batch_steps // 10))
+    if not a.no_adjoint:
+        print("adjoint ...", file=sys.stderr, flush=True)
+        out["adjoint"] = adjoint_rates()
     if not a.no_cpu_baseline:
         print("cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(system, p, a.cpu_seconds)

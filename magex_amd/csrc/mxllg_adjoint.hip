@@ -29,8 +29,8 @@ constexpr int MAXP = 8;  // parameters
 template <typename R>
 struct AStatic {
   int P;
-  R U[MAXP][MAXENT][9];    // unit bond tensors per parameter, in the library's stencil order
-  R Aun[MAXP][MAXSUB][9];  // unit on-site tensors per parameter
+  R U[MAXENT][MAXP][9];    // unit bond tensors per stencil entry (library order) and parameter
+  R Aun[MAXSUB][MAXP][9];  // unit on-site tensors per sublattice and parameter
 };
 
 // Vector-Jacobian product of f(e, B) with cotangent l (see the header comment).
@@ -71,73 +71,95 @@ __device__ __forceinline__ void onsite_T(const Static<R>* __restrict__ S, int s,
 }
 
 // Adjoint gather at site st: H = sum_k J_k nu_j (J from the model's width-9 table), and for every
-// parameter t_p = sum_k U[p][k] x_j; then G_p = nu_i . (t_p - Aun[p] x_i) is accumulated into gp.
-template <typename R, int GEO>
+// parameter t_p = sum_k U[k][p] x_j; then G_p = nu_i . (t_p - Aun[s][p] x_i) is accumulated into gp.
+// NP (4 or 8) is the parameter count rounded up; padded parameters have zero units.
+// Entries go in batches of CH whose loads are all issued before the first is used (entries past the
+// end re-read the last one with weight zero), so a wavefront keeps 2 CH gathers in flight.
+template <typename R, int GEO, int NP>
 __device__ __forceinline__ void adj_gather(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
                                            const R* __restrict__ nu, const R* __restrict__ x, const Site& st,
                                            R nix, R niy, R niz, R xix, R xiy, R xiz, R& hx, R& hy, R& hz,
-                                           double (&gp)[MAXP]) {
-  const int P = A->P, ncell = S->ncell;
-  R t[MAXP][3];
+                                           double (&gp)[NP]) {
+  constexpr int CH = 8;
+  const int ncell = S->ncell;
+  R t[NP][3];
 #pragma unroll
-  for (int p = 0; p < MAXP; ++p) t[p][0] = t[p][1] = t[p][2] = R(0);
+  for (int p = 0; p < NP; ++p) t[p][0] = t[p][1] = t[p][2] = R(0);
   hx = hy = hz = R(0);
   const int kb = S->sptr[st.s], ke = S->sptr[st.s + 1];
-  for (int k = kb; k < ke; ++k) {
-    const Ent en = S->ent[k];
-    bool ok;
-    const int cn = neighbour_cell<R, GEO>(S, st, en, ok);
-    const int j = (en.tb & 0xffff) * ncell + cn;
-    const R w = ok ? R(1) : R(0);
-    const R vx = w * nu[3 * j], vy = w * nu[3 * j + 1], vz = w * nu[3 * j + 2];
-    const R qx = w * x[3 * j], qy = w * x[3 * j + 1], qz = w * x[3 * j + 2];
-    const R* T = S->tab + 9 * en.slot;
-    hx += T[0] * vx + T[1] * vy + T[2] * vz;
-    hy += T[3] * vx + T[4] * vy + T[5] * vz;
-    hz += T[6] * vx + T[7] * vy + T[8] * vz;
+  for (int k0 = kb; k0 < ke; k0 += CH) {
+    R vx[CH], vy[CH], vz[CH], qx[CH], qy[CH], qz[CH];
+    int kk[CH];
 #pragma unroll
-    for (int p = 0; p < MAXP; ++p) {
-      if (p < P) {
-        const R* U = A->U[p][k];
-        t[p][0] += U[0] * qx + U[1] * qy + U[2] * qz;
-        t[p][1] += U[3] * qx + U[4] * qy + U[5] * qz;
-        t[p][2] += U[6] * qx + U[7] * qy + U[8] * qz;
+    for (int u = 0; u < CH; ++u) {
+      const bool live = k0 + u < ke;
+      kk[u] = live ? k0 + u : ke - 1;
+      const Ent en = S->ent[kk[u]];
+      bool ok;
+      const int cn = neighbour_cell<R, GEO>(S, st, en, ok);
+      ok = ok && live;
+      const int j = (en.tb & 0xffff) * ncell + cn;
+      vx[u] = nu[3 * j];
+      vy[u] = nu[3 * j + 1];
+      vz[u] = nu[3 * j + 2];
+      qx[u] = x[3 * j];
+      qy[u] = x[3 * j + 1];
+      qz[u] = x[3 * j + 2];
+      const R w = ok ? R(1) : R(0);
+      vx[u] *= w;
+      vy[u] *= w;
+      vz[u] *= w;
+      qx[u] *= w;
+      qy[u] *= w;
+      qz[u] *= w;
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const R* T = S->tab + 9 * S->ent[kk[u]].slot;
+      hx += T[0] * vx[u] + T[1] * vy[u] + T[2] * vz[u];
+      hy += T[3] * vx[u] + T[4] * vy[u] + T[5] * vz[u];
+      hz += T[6] * vx[u] + T[7] * vy[u] + T[8] * vz[u];
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        const R* U = A->U[kk[u]][p];
+        t[p][0] += U[0] * qx[u] + U[1] * qy[u] + U[2] * qz[u];
+        t[p][1] += U[3] * qx[u] + U[4] * qy[u] + U[5] * qz[u];
+        t[p][2] += U[6] * qx[u] + U[7] * qy[u] + U[8] * qz[u];
       }
     }
   }
 #pragma unroll
-  for (int p = 0; p < MAXP; ++p) {
-    if (p < P) {
-      const R* a = A->Aun[p][st.s];
-      const R ux = t[p][0] - (a[0] * xix + a[1] * xiy + a[2] * xiz);
-      const R uy = t[p][1] - (a[3] * xix + a[4] * xiy + a[5] * xiz);
-      const R uz = t[p][2] - (a[6] * xix + a[7] * xiy + a[8] * xiz);
-      gp[p] += double(nix * ux + niy * uy + niz * uz);
-    }
+  for (int p = 0; p < NP; ++p) {
+    const R* a = A->Aun[st.s][p];
+    const R ux = t[p][0] - (a[0] * xix + a[1] * xiy + a[2] * xiz);
+    const R uy = t[p][1] - (a[3] * xix + a[4] * xiy + a[5] * xiz);
+    const R uz = t[p][2] - (a[6] * xix + a[7] * xiy + a[8] * xiz);
+    gp[p] += double(nix * ux + niy * uy + niz * uz);
   }
 }
 
-// Step-local part (corrector): from lambda = dL/de', the saved e, ep, k1 and B(ep): lambda_u, the
+// Saved state of one step: e, ep, k1, B(e), B(ep), each N x 3 in internal site order.
+template <typename R>
+struct StepRec {
+  const R *e, *ep, *k1, *B1, *B2;
+};
+
+// Step-local part (corrector) at site i: from lambda = dL/de' and the step's record: lambda_u, the
 // cotangent of ep through k2 (gep) and nu2.
-template <typename R, int GEO>
-__global__ __launch_bounds__(BLOCK) void k_adj_a(const Static<R>* __restrict__ S, const Dyn<R> d,
-                                                 const R* __restrict__ lam, const R* __restrict__ e,
-                                                 const R* __restrict__ ep, const R* __restrict__ k1,
-                                                 const R* __restrict__ B2, R* __restrict__ lamu, R* __restrict__ gep,
-                                                 R* __restrict__ nu2) {
-  Site st;
-  if (!site_of<R, GEO>(S, st)) return;
+template <typename R>
+__device__ __forceinline__ void adj_local(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st,
+                                          const StepRec<R>& r, R lx, R ly, R lz, R* __restrict__ lamu,
+                                          R* __restrict__ gep, R* __restrict__ nu2) {
   const int i = st.i;
   const R a = d.alpha ? d.alpha[i] : S->alpha_sub[st.s];
-  const R px = ep[3 * i], py = ep[3 * i + 1], pz = ep[3 * i + 2];
-  const R bx = B2[3 * i], by = B2[3 * i + 1], bz = B2[3 * i + 2];
+  const R px = r.ep[3 * i], py = r.ep[3 * i + 1], pz = r.ep[3 * i + 2];
+  const R bx = r.B2[3 * i], by = r.B2[3 * i + 1], bz = r.B2[3 * i + 2];
   R fx, fy, fz;
   llg_rhs(px, py, pz, bx, by, bz, a, d.prec, fx, fy, fz);
   const R hdt = R(0.5) * d.dt;
-  const R ux = e[3 * i] + hdt * (k1[3 * i] + fx), uy = e[3 * i + 1] + hdt * (k1[3 * i + 1] + fy),
-          uz = e[3 * i + 2] + hdt * (k1[3 * i + 2] + fz);
+  const R ux = r.e[3 * i] + hdt * (r.k1[3 * i] + fx), uy = r.e[3 * i + 1] + hdt * (r.k1[3 * i + 1] + fy),
+          uz = r.e[3 * i + 2] + hdt * (r.k1[3 * i + 2] + fz);
   const R n = sqrt(ux * ux + uy * uy + uz * uz);
-  R lx = lam[3 * i], ly = lam[3 * i + 1], lz = lam[3 * i + 2];
   unproject(ux / n, uy / n, uz / n, n, lx, ly, lz);
   lamu[3 * i] = lx;
   lamu[3 * i + 1] = ly;
@@ -153,28 +175,53 @@ __global__ __launch_bounds__(BLOCK) void k_adj_a(const Static<R>* __restrict__ S
   nu2[3 * i + 2] = im2 * hz;
 }
 
-// Gather nu2 (dL/dep) and the predictor's adjoint: writes nu1 and the partial dL/de (part).
 template <typename R, int GEO>
+__global__ __launch_bounds__(BLOCK) void k_adj_a(const Static<R>* __restrict__ S, const Dyn<R> d,
+                                                 const R* __restrict__ lam, const StepRec<R> r, R* __restrict__ lamu,
+                                                 R* __restrict__ gep, R* __restrict__ nu2) {
+  Site st;
+  if (!site_of<R, GEO>(S, st)) return;
+  const int i = st.i;
+  adj_local(S, d, st, r, lam[3 * i], lam[3 * i + 1], lam[3 * i + 2], lamu, gep, nu2);
+}
+
+// Adds the workgroup's sums of gp to its row of gpart (stride MAXP); old is that row's entry for this
+// thread, loaded at kernel start so the read is off the critical path.
+template <int NP>
+__device__ __forceinline__ double partial_row(const double* __restrict__ gpart) {
+  return threadIdx.x < NP ? gpart[size_t(blockIdx.x) * MAXP + threadIdx.x] : 0.0;
+}
+
+template <int NP>
+__device__ __forceinline__ void store_partials(double (&gp)[NP], double old, double* __restrict__ gpart) {
+  __shared__ double res[NP];
+  block_reduce<NP>(gp, NP, res);
+  __syncthreads();
+  if (threadIdx.x < NP) gpart[size_t(blockIdx.x) * MAXP + threadIdx.x] = old + res[threadIdx.x];
+}
+
+// Gather nu2 (dL/dep) and the predictor's adjoint: writes nu1 and the partial dL/de (part).
+template <typename R, int GEO, int NP>
 __global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
-                                                 const Dyn<R> d, const R* __restrict__ e, const R* __restrict__ ep,
-                                                 const R* __restrict__ B1, const R* __restrict__ lamu,
+                                                 const Dyn<R> d, const StepRec<R> r, const R* __restrict__ lamu,
                                                  const R* __restrict__ gep, const R* __restrict__ nu2,
                                                  R* __restrict__ nu1, R* __restrict__ part, double* __restrict__ gpart) {
-  double gp[MAXP];
+  const double old = partial_row<NP>(gpart);
+  double gp[NP];
 #pragma unroll
-  for (int p = 0; p < MAXP; ++p) gp[p] = 0.0;
+  for (int p = 0; p < NP; ++p) gp[p] = 0.0;
   Site st;
   if (site_of<R, GEO>(S, st)) {
     const int i = st.i;
     const R a = d.alpha ? d.alpha[i] : S->alpha_sub[st.s];
-    const R px = ep[3 * i], py = ep[3 * i + 1], pz = ep[3 * i + 2];
+    const R px = r.ep[3 * i], py = r.ep[3 * i + 1], pz = r.ep[3 * i + 2];
     const R vx2 = nu2[3 * i], vy2 = nu2[3 * i + 1], vz2 = nu2[3 * i + 2];
     R hx, hy, hz, ox, oy, oz;
-    adj_gather<R, GEO>(S, A, nu2, ep, st, vx2, vy2, vz2, px, py, pz, hx, hy, hz, gp);
+    adj_gather<R, GEO, NP>(S, A, nu2, r.ep, st, vx2, vy2, vz2, px, py, pz, hx, hy, hz, gp);
     onsite_T(S, st.s, vx2, vy2, vz2, ox, oy, oz);
     R lx = gep[3 * i] + hx - ox, ly = gep[3 * i + 1] + hy - oy, lz = gep[3 * i + 2] + hz - oz;  // dL/dep
-    const R ex = e[3 * i], ey = e[3 * i + 1], ez = e[3 * i + 2];
-    const R bx = B1[3 * i], by = B1[3 * i + 1], bz = B1[3 * i + 2];
+    const R ex = r.e[3 * i], ey = r.e[3 * i + 1], ez = r.e[3 * i + 2];
+    const R bx = r.B1[3 * i], by = r.B1[3 * i + 1], bz = r.B1[3 * i + 2];
     R fx, fy, fz;
     llg_rhs(ex, ey, ez, bx, by, bz, a, d.prec, fx, fy, fz);
     const R vx = ex + d.dt * fx, vy = ey + d.dt * fy, vz = ez + d.dt * fz;
@@ -193,42 +240,39 @@ __global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S
     part[3 * i + 1] = uy + ly + gy;
     part[3 * i + 2] = uz + lz + gz;
   }
-  double red[MAXP];
-#pragma unroll
-  for (int p = 0; p < MAXP; ++p) red[p] = gp[p];
-  __shared__ double res[MAXP];
-  block_reduce<MAXP>(red, MAXP, res);
-  __syncthreads();
-  if (threadIdx.x < MAXP) gpart[size_t(blockIdx.x) * MAXP + threadIdx.x] += res[threadIdx.x];
+  store_partials<NP>(gp, old, gpart);
 }
 
-// Gather nu1: dL/de of the step's input, and the parameter terms at e.
-template <typename R, int GEO>
+// Gather nu1: lambda = dL/de of the step's input, and the parameter terms at e.  With FUSE the
+// previous step's local part (adj_local on its record prev) follows at once from lambda, which then
+// never goes through memory; otherwise lambda is stored for a later k_adj_a.
+template <typename R, int GEO, int NP, bool FUSE>
 __global__ __launch_bounds__(BLOCK) void k_adj_c(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
-                                                 const R* __restrict__ e, const R* __restrict__ nu1,
+                                                 const Dyn<R> d, const R* __restrict__ e, const R* __restrict__ nu1,
                                                  const R* __restrict__ part, R* __restrict__ lam,
-                                                 double* __restrict__ gpart) {
-  double gp[MAXP];
+                                                 const StepRec<R> prev, R* __restrict__ lamu, R* __restrict__ gep,
+                                                 R* __restrict__ nu2, double* __restrict__ gpart) {
+  const double old = partial_row<NP>(gpart);
+  double gp[NP];
 #pragma unroll
-  for (int p = 0; p < MAXP; ++p) gp[p] = 0.0;
+  for (int p = 0; p < NP; ++p) gp[p] = 0.0;
   Site st;
   if (site_of<R, GEO>(S, st)) {
     const int i = st.i;
     const R vx = nu1[3 * i], vy = nu1[3 * i + 1], vz = nu1[3 * i + 2];
     R hx, hy, hz, ox, oy, oz;
-    adj_gather<R, GEO>(S, A, nu1, e, st, vx, vy, vz, e[3 * i], e[3 * i + 1], e[3 * i + 2], hx, hy, hz, gp);
+    adj_gather<R, GEO, NP>(S, A, nu1, e, st, vx, vy, vz, e[3 * i], e[3 * i + 1], e[3 * i + 2], hx, hy, hz, gp);
     onsite_T(S, st.s, vx, vy, vz, ox, oy, oz);
-    lam[3 * i] = part[3 * i] + hx - ox;
-    lam[3 * i + 1] = part[3 * i + 1] + hy - oy;
-    lam[3 * i + 2] = part[3 * i + 2] + hz - oz;
+    const R lx = part[3 * i] + hx - ox, ly = part[3 * i + 1] + hy - oy, lz = part[3 * i + 2] + hz - oz;
+    if constexpr (FUSE) {
+      adj_local(S, d, st, prev, lx, ly, lz, lamu, gep, nu2);
+    } else {
+      lam[3 * i] = lx;
+      lam[3 * i + 1] = ly;
+      lam[3 * i + 2] = lz;
+    }
   }
-  double red[MAXP];
-#pragma unroll
-  for (int p = 0; p < MAXP; ++p) red[p] = gp[p];
-  __shared__ double res[MAXP];
-  block_reduce<MAXP>(red, MAXP, res);
-  __syncthreads();
-  if (threadIdx.x < MAXP) gpart[size_t(blockIdx.x) * MAXP + threadIdx.x] += res[threadIdx.x];
+  store_partials<NP>(gp, old, gpart);
 }
 
 template <typename T>
@@ -261,7 +305,9 @@ struct mx_adjoint {
   void* ckpt_d = nullptr;  // (nckpt) x N x 3
   int64_t ckpt_cap = 0;
   // segment and adjoint scratch
-  void *seg_d = nullptr;  // 5 x C x N x 3: e, ep, k1, B1, B2
+  void* seg_d = nullptr;  // records of one segment, see ensure_segment
+  int seg_cap = 0;        // steps the records hold
+  int64_t seg_k = -1;     // segment whose records seg_d holds (-1: none)
   void *lam_d = nullptr, *lamu_d = nullptr, *gep_d = nullptr, *nu1_d = nullptr, *nu2_d = nullptr, *part_d = nullptr;
   void* ework_d = nullptr;
   double* gpart_d = nullptr;  // per-workgroup parameter-gradient partials, nblk x MAXP
@@ -307,9 +353,9 @@ void upload_astatic(mx_adjoint& a) {
   A.P = a.P;
   for (int p = 0; p < a.P; ++p) {
     for (int k = 0; k < m.nent; ++k)
-      for (int q = 0; q < 9; ++q) A.U[p][k][q] = R(a.U[(size_t(p) * m.nent + k) * 9 + q]);
+      for (int q = 0; q < 9; ++q) A.U[k][p][q] = R(a.U[(size_t(p) * m.nent + k) * 9 + q]);
     for (int s = 0; s < m.ns; ++s)
-      for (int q = 0; q < 9; ++q) A.Aun[p][s][q] = R(a.Aun[(size_t(p) * m.ns + s) * 9 + q]);
+      for (int q = 0; q < 9; ++q) A.Aun[s][p][q] = R(a.Aun[(size_t(p) * m.ns + s) * 9 + q]);
   }
   if (!a.astatic_d) aok(hipMalloc(&a.astatic_d, sizeof(A)), "hipMalloc adjoint constants");
   aok(hipMemcpy(a.astatic_d, &A, sizeof(A), hipMemcpyHostToDevice), "hipMemcpy adjoint constants");
@@ -342,10 +388,64 @@ void apply_theta(mx_adjoint& a) {
 
 size_t nbytes3(const Model& m) { return size_t(m.N) * 3 * prec_size(m); }
 
+template <typename R>
+void copy_dev(const void* src, void* dst, size_t nelem, hipStream_t s) {
+  hipLaunchKernelGGL((k_copy<R>), dim3((nelem + 255) / 256), dim3(256), 0, s, static_cast<const R*>(src),
+                     static_cast<R*>(dst), nelem);
+}
+
+// Segment records for n steps: 5 arrays (e, ep, k1, B(e), B(ep)) of n x N x 3, record (which, step)
+// at (which * seg_cap + step) * N * 3 values.
+void ensure_segment(mx_adjoint& a, int n) {
+  if (n <= a.seg_cap) return;
+  afree(a.seg_d);
+  a.seg_cap = 0;
+  a.seg_k = -1;
+  aok(hipMalloc(&a.seg_d, 5 * size_t(n) * nbytes3(*a.m)), "hipMalloc segment records");
+  a.seg_cap = n;
+}
+
+char* seg_slot(mx_adjoint& a, int which, int n) {
+  return static_cast<char*>(a.seg_d) + (size_t(which) * a.seg_cap + n) * nbytes3(*a.m);
+}
+
+void to_internal(mx_adjoint& a, const void* src, void* dst, hipStream_t s) {
+  const Model& m = *a.m;
+  const size_t nel = size_t(m.N) * 3;
+  if (needs_permute(m)) aok_rc(launch_permute(m, src, dst, 1, s), "permute");
+  else if (m.prec == MX_FP64) copy_dev<double>(src, dst, nel, s);
+  else copy_dev<float>(src, dst, nel, s);
+}
+
+void to_magex(mx_adjoint& a, const void* src, void* dst, hipStream_t s) {
+  const Model& m = *a.m;
+  const size_t nel = size_t(m.N) * 3;
+  if (needs_permute(m)) aok_rc(launch_permute(m, src, dst, 0, s), "permute");
+  else if (m.prec == MX_FP64) copy_dev<double>(src, dst, nel, s);
+  else copy_dev<float>(src, dst, nel, s);
+}
+
+// Runs segment k (n_seg steps) from the magex-order spins src with the single-pass kernels, saving
+// every step's record; the final spins stay in ework_d (internal order).
+void record_segment(mx_adjoint& a, const void* src, int64_t k, int n_seg, hipStream_t s) {
+  Model& m = *a.m;
+  void* ew = a.ework_d;
+  to_internal(a, src, ew, s);
+  for (int n = 0; n < n_seg; ++n) {
+    const uint64_t step = a.step0 + uint64_t(k * a.C + n);
+    aok_rc(launch_heun_pass(m, 1, ew, seg_slot(a, 1, n), seg_slot(a, 2, n), a.seed, step, seg_slot(a, 0, n),
+                            seg_slot(a, 3, n), s),
+           "segment pass 1");
+    aok_rc(launch_heun_pass(m, 2, ew, seg_slot(a, 1, n), seg_slot(a, 2, n), a.seed, step, nullptr,
+                            seg_slot(a, 4, n), s),
+           "segment pass 2");
+  }
+  a.seg_k = k;
+}
+
 void ensure_buffers(mx_adjoint& a) {
   const Model& m = *a.m;
   const size_t b = nbytes3(m);
-  if (!a.seg_d) aok(hipMalloc(&a.seg_d, 5 * size_t(a.C) * b), "hipMalloc segment");
   for (void** p : {&a.lam_d, &a.lamu_d, &a.gep_d, &a.nu1_d, &a.nu2_d, &a.part_d, &a.ework_d})
     if (!*p) aok(hipMalloc(p, b), "hipMalloc adjoint state");
   a.nblk = m.ns * ((m.ncell + BLOCK - 1) / BLOCK);
@@ -353,15 +453,12 @@ void ensure_buffers(mx_adjoint& a) {
   if (!a.gout_d) aok(hipMalloc(reinterpret_cast<void**>(&a.gout_d), sizeof(double) * MAXP), "hipMalloc gout");
 }
 
-template <typename R>
-void copy_dev(const void* src, void* dst, size_t nelem, hipStream_t s) {
-  hipLaunchKernelGGL((k_copy<R>), dim3((nelem + 255) / 256), dim3(256), 0, s, static_cast<const R*>(src),
-                     static_cast<R*>(dst), nelem);
-}
 
-template <typename R>
-void run_adjoint_kernels(mx_adjoint& a, const void* e, const void* ep, const void* k1, const void* B1, const void* B2,
-                         hipStream_t s) {
+// Adjoint sweep over one replayed segment of n_seg steps (records rec(n), newest last): the local
+// part of the segment's last step from lam, then per step (newest first) k_adj_b and k_adj_c, the
+// latter fused with the next-older step's local part.  Leaves dL/de of the segment's first input in lam.
+template <typename R, int GEO, int NP>
+void sweep_t(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
   const Model& m = *a.m;
   const Static<R>* S = static_cast<const Static<R>*>(m.static_d);
   const AStatic<R>* A = static_cast<const AStatic<R>*>(a.astatic_d);
@@ -370,11 +467,6 @@ void run_adjoint_kernels(mx_adjoint& a, const void* e, const void* ep, const voi
   d.prec = m.precession;
   d.alpha = static_cast<const R*>(m.alpha_d);
   d.inv_mu = static_cast<const R*>(m.inv_mu_d);
-  const R* E = static_cast<const R*>(e);
-  const R* EP = static_cast<const R*>(ep);
-  const R* K1 = static_cast<const R*>(k1);
-  const R* b1 = static_cast<const R*>(B1);
-  const R* b2 = static_cast<const R*>(B2);
   R* lam = static_cast<R*>(a.lam_d);
   R* lamu = static_cast<R*>(a.lamu_d);
   R* gep = static_cast<R*>(a.gep_d);
@@ -382,19 +474,34 @@ void run_adjoint_kernels(mx_adjoint& a, const void* e, const void* ep, const voi
   R* nu2 = static_cast<R*>(a.nu2_d);
   R* part = static_cast<R*>(a.part_d);
   const dim3 g(a.nblk), b(BLOCK);
-  if (m.geo == GEO_OPEN) {
-    hipLaunchKernelGGL((k_adj_a<R, GEO_OPEN>), g, b, 0, s, S, d, lam, E, EP, K1, b2, lamu, gep, nu2);
-    hipLaunchKernelGGL((k_adj_b<R, GEO_OPEN>), g, b, 0, s, S, A, d, E, EP, b1, lamu, gep, nu2, nu1, part, a.gpart_d);
-    hipLaunchKernelGGL((k_adj_c<R, GEO_OPEN>), g, b, 0, s, S, A, E, nu1, part, lam, a.gpart_d);
-  } else if (m.geo == GEO_PERIODIC) {
-    hipLaunchKernelGGL((k_adj_a<R, GEO_PERIODIC>), g, b, 0, s, S, d, lam, E, EP, K1, b2, lamu, gep, nu2);
-    hipLaunchKernelGGL((k_adj_b<R, GEO_PERIODIC>), g, b, 0, s, S, A, d, E, EP, b1, lamu, gep, nu2, nu1, part,
-                       a.gpart_d);
-    hipLaunchKernelGGL((k_adj_c<R, GEO_PERIODIC>), g, b, 0, s, S, A, E, nu1, part, lam, a.gpart_d);
+  hipLaunchKernelGGL((k_adj_a<R, GEO>), g, b, 0, s, S, d, lam, rec[n_seg - 1], lamu, gep, nu2);
+  for (int n = n_seg - 1; n >= 0; --n) {
+    hipLaunchKernelGGL((k_adj_b<R, GEO, NP>), g, b, 0, s, S, A, d, rec[n], lamu, gep, nu2, nu1, part, a.gpart_d);
+    if (n > 0)
+      hipLaunchKernelGGL((k_adj_c<R, GEO, NP, true>), g, b, 0, s, S, A, d, rec[n].e, nu1, part, lam, rec[n - 1], lamu,
+                         gep, nu2, a.gpart_d);
+    else
+      hipLaunchKernelGGL((k_adj_c<R, GEO, NP, false>), g, b, 0, s, S, A, d, rec[n].e, nu1, part, lam, rec[n], lamu, gep,
+                         nu2, a.gpart_d);
+  }
+}
+
+template <typename R>
+void sweep(mx_adjoint& a, char* seg, size_t b, int C, int n_seg, hipStream_t s) {
+  std::vector<StepRec<R>> rec(n_seg);
+  for (int n = 0; n < n_seg; ++n) {
+    auto at = [&](int which) { return reinterpret_cast<const R*>(seg + (size_t(which) * C + n) * b); };
+    rec[n] = StepRec<R>{at(0), at(1), at(2), at(3), at(4)};
+  }
+  const int geo = a.m->geo;
+  if (a.P <= 4) {
+    if (geo == GEO_OPEN) sweep_t<R, GEO_OPEN, 4>(a, rec.data(), n_seg, s);
+    else if (geo == GEO_PERIODIC) sweep_t<R, GEO_PERIODIC, 4>(a, rec.data(), n_seg, s);
+    else sweep_t<R, GEO_TILED, 4>(a, rec.data(), n_seg, s);
   } else {
-    hipLaunchKernelGGL((k_adj_a<R, GEO_TILED>), g, b, 0, s, S, d, lam, E, EP, K1, b2, lamu, gep, nu2);
-    hipLaunchKernelGGL((k_adj_b<R, GEO_TILED>), g, b, 0, s, S, A, d, E, EP, b1, lamu, gep, nu2, nu1, part, a.gpart_d);
-    hipLaunchKernelGGL((k_adj_c<R, GEO_TILED>), g, b, 0, s, S, A, E, nu1, part, lam, a.gpart_d);
+    if (geo == GEO_OPEN) sweep_t<R, GEO_OPEN, 8>(a, rec.data(), n_seg, s);
+    else if (geo == GEO_PERIODIC) sweep_t<R, GEO_PERIODIC, 8>(a, rec.data(), n_seg, s);
+    else sweep_t<R, GEO_TILED, 8>(a, rec.data(), n_seg, s);
   }
 }
 
@@ -503,12 +610,26 @@ int mx_adjoint_forward(mx_adjoint* a, void* e, int64_t nsteps, uint64_t seed, ui
     // checkpoint k holds the (magex-order) spins after k * C steps
     char* ck = static_cast<char*>(a->ckpt_d);
     const size_t nel = size_t(m.N) * 3;
+    // the last segment runs with the single-pass kernels and keeps its records, so the backward starts
+    // without a replay
+    const int64_t k_last = nsteps > 0 ? (nsteps - 1) / a->C : -1;
+    ensure_segment(*a, int(std::min<int64_t>(a->C, std::max<int64_t>(nsteps, 1))));
+    a->seg_k = -1;
     for (int64_t k = 0; k * a->C <= nsteps; ++k) {
       if (m.prec == MX_FP64) copy_dev<double>(e, ck + size_t(k) * b, nel, s);
       else copy_dev<float>(e, ck + size_t(k) * b, nel, s);
       const int64_t n = std::min<int64_t>(a->C, nsteps - k * a->C);
-      if (n > 0) aok_rc(mx_llg_run(a->m, e, n, seed, step0 + uint64_t(k * a->C), nullptr, stream), "forward");
+      if (n <= 0) continue;
+      if (k == k_last) {
+        aok_rc(sync_model_static(m), "model constants");
+        record_segment(*a, e, k, int(n), s);
+        to_magex(*a, a->ework_d, e, s);
+      } else {
+        aok_rc(mx_llg_run(a->m, e, n, seed, step0 + uint64_t(k * a->C), nullptr, stream), "forward");
+      }
     }
+    const hipError_t err = hipGetLastError();
+    if (err != hipSuccess) afail(MX_EHIP, std::string("adjoint forward: ") + hipGetErrorString(err));
     a->recorded = true;
     return MX_OK;
   });
@@ -525,39 +646,21 @@ int mx_adjoint_backward(mx_adjoint* a, const void* grad_e, void* grad_e0, double
     const size_t b = nbytes3(m), nel = size_t(m.N) * 3;
     const bool fp64 = m.prec == MX_FP64;
     // the adjoint works in the library's internal site order; permute at entry and exit
-    if (needs_permute(m)) aok_rc(launch_permute(m, grad_e, a->lam_d, 1, s), "permute");
-    else if (fp64) copy_dev<double>(grad_e, a->lam_d, nel, s);
-    else copy_dev<float>(grad_e, a->lam_d, nel, s);
+    to_internal(*a, grad_e, a->lam_d, s);
     hipLaunchKernelGGL((k_zero<double>), dim3((MAXP * a->nblk + 255) / 256), dim3(256), 0, s, a->gpart_d,
                        size_t(MAXP) * a->nblk);
-    char* seg = static_cast<char*>(a->seg_d);
     char* ck = static_cast<char*>(a->ckpt_d);
     const int C = a->C;
-    auto slot = [&](int which, int n) { return seg + (size_t(which) * C + n) * b; };
     for (int64_t k = (a->nsteps - 1) / C; k >= 0 && a->nsteps > 0; --k) {
       const int n_seg = int(std::min<int64_t>(C, a->nsteps - k * C));
-      // replay the segment from its checkpoint (internal order), saving e, ep, k1, B(e), B(ep)
-      void* ew = a->ework_d;
-      if (needs_permute(m)) aok_rc(launch_permute(m, ck + size_t(k) * b, ew, 1, s), "permute");
-      else if (fp64) copy_dev<double>(ck + size_t(k) * b, ew, nel, s);
-      else copy_dev<float>(ck + size_t(k) * b, ew, nel, s);
-      for (int n = 0; n < n_seg; ++n) {
-        const uint64_t step = a->step0 + uint64_t(k * C + n);
-        aok_rc(launch_heun_pass(m, 1, ew, slot(1, n), slot(2, n), a->seed, step, slot(0, n), slot(3, n), s), "replay");
-        aok_rc(launch_heun_pass(m, 2, ew, slot(1, n), slot(2, n), a->seed, step, nullptr, slot(4, n), s), "replay");
-      }
-      for (int n = n_seg - 1; n >= 0; --n) {
-        if (fp64) run_adjoint_kernels<double>(*a, slot(0, n), slot(1, n), slot(2, n), slot(3, n), slot(4, n), s);
-        else run_adjoint_kernels<float>(*a, slot(0, n), slot(1, n), slot(2, n), slot(3, n), slot(4, n), s);
-      }
+      // replay the segment from its checkpoint unless its records are still held
+      if (a->seg_k != k) record_segment(*a, ck + size_t(k) * b, k, n_seg, s);
+      if (fp64) sweep<double>(*a, static_cast<char*>(a->seg_d), b, a->seg_cap, n_seg, s);
+      else sweep<float>(*a, static_cast<char*>(a->seg_d), b, a->seg_cap, n_seg, s);
     }
     hipLaunchKernelGGL((k_final<MAXP>), dim3(1), dim3(BLOCK), 0, s, a->gpart_d, a->nblk, MAXP, 1.0, a->gout_d);
     hipLaunchKernelGGL((k_copy<double>), dim3(1), dim3(64), 0, s, a->gout_d, grad_theta, size_t(a->P));
-    if (grad_e0) {
-      if (needs_permute(m)) aok_rc(launch_permute(m, a->lam_d, grad_e0, 0, s), "permute");
-      else if (fp64) copy_dev<double>(a->lam_d, grad_e0, nel, s);
-      else copy_dev<float>(a->lam_d, grad_e0, nel, s);
-    }
+    if (grad_e0) to_magex(*a, a->lam_d, grad_e0, s);
     const hipError_t err = hipGetLastError();
     if (err != hipSuccess) afail(MX_EHIP, std::string("adjoint: ") + hipGetErrorString(err));
     return MX_OK;

@@ -100,11 +100,22 @@ def test_adjoint_checkpoint_interval_is_exact(gpu):
     o = ref_build(THETA0)
     e0 = o.init_random(4)
     tgt = torch.as_tensor(o.init_random(5)).cuda()
-    out = [_grads(_solver(L, C=C), THETA0, e0, tgt, steps) for C in (50, 7, 1000)]
+    sols = [_solver(L, C=C) for C in (50, 7, 1000)]
+    out = [_grads(sol, THETA0, e0, tgt, steps) for sol in sols]
     for e, gt, g0 in out[1:]:
         assert torch.equal(e, out[0][0])
         assert np.array_equal(gt, out[0][1])
         assert np.array_equal(g0, out[0][2])
+    # a second backward of the same forward replays what the first one consumed
+    for sol in sols[:2]:
+        gt, g0 = sol.backward(_mse_grad(out[0][0], tgt))
+        assert np.array_equal(gt.cpu().numpy(), out[0][1])
+        assert np.array_equal(g0.double().cpu().numpy(), out[0][2])
+    # zero steps: the gradient of the loss itself
+    e = sols[0].forward(THETA0, e0, 0)
+    gt, g0 = sols[0].backward(_mse_grad(e, tgt))
+    assert np.array_equal(gt.cpu().numpy(), np.zeros(4))
+    assert torch.equal(g0, _mse_grad(e, tgt))
 
 
 def test_adjoint_forward_is_the_plain_trajectory(gpu):
