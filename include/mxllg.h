@@ -96,12 +96,13 @@ int mx_model_set_field(mx_model* m, const double B[3]);
 
 /* Dynamics of LLGSolver (llg.py:49-122): dt (ps); damping alpha and temperature T (K) either uniform
  * (per_site=0: one value) or per site (N values); sigma_i = sqrt(2 alpha_i k_B T_i / (gamma mu_i mu_B dt)).
- * precession 0/1; integrator MX_HEUN (MX_DEPONDT is reserved). */
+ * precession 0/1; integrator MX_HEUN (stochastic Heun) or MX_DEPONDT (Heun on rotation vectors,
+ * magex/fast.py:271-306 and the depondt branches of _fused1/_fused2, 336-390). */
 int mx_model_set_dynamics(mx_model* m, double dt, int alpha_per_site, const double* alpha, int T_per_site,
                           const double* T, int precession, int integrator);
 
-/* nsteps stochastic Heun steps on e (device, N x 3, in place); replaces one _fused1 + _fused2 pair per
- * step (fast.py:336-390, llg.py:216-227).  Thermal noise: standard normals from a counter-based
+/* nsteps steps of the model's integrator (stochastic Heun or Depondt) on e (device, N x 3, in place);
+ * replaces one _fused1 + _fused2 pair per step (fast.py:336-390, llg.py:216-240).  Thermal noise: standard normals from a counter-based
  * generator keyed by (seed, step0 + n, site) -- or, if noise != NULL, read from noise (device,
  * nsteps x N x 3, same precision) -- times sigma_i, held over both Heun stages. */
 int mx_llg_run(mx_model* m, void* e, int64_t nsteps, uint64_t seed, uint64_t step0, const void* noise,

@@ -4,7 +4,10 @@
 
 hipcc cross-compiles without a GPU.  Flags: -O3, C++17, gfx950 only; packed FP32 instructions are
 disabled (-target-feature -packed-fp32-ops) because some of their forms have given wrong results on
-this hardware generation.
+this hardware generation.  -ffp-contract=on fuses a * b + c into one FMA only inside a single source
+expression, so every kernel that spells a formula the same way rounds it the same way: the layouts,
+the batched and the plain kernels and the captured and direct step loops agree bit for bit (the
+default cross-statement contraction decides per kernel, from the surrounding code).
 """
 from __future__ import annotations
 
@@ -28,7 +31,7 @@ def hipcc():
 
 def command(out=TARGET):
     return [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-ffp-contract=on",
             "-Wall", "-Wno-unused-function", "-I", INCLUDE, "-I", HERE,
             *[os.path.join(HERE, s) for s in SOURCES], "-o", out]
 

@@ -13,7 +13,7 @@
 //                   J_ji = J_ij^T the first term is the forward gather applied to nu
 //   parameters:     dL/dtheta_p += sum_i nu_i . (sum_k U[p][k] x_j(k) - Aun[p] x_i)  (x = e or ep).
 // The forward run keeps the spins every C steps; the backward replays each segment from its
-// checkpoint (saving e, ep, k1, B(e), B(ep) per step, noise regenerated from the same counters) and
+// checkpoint (saving e, ep, w = e + dt/2 k1, B(e), B(ep) per step, noise regenerated from the same counters) and
 // runs three kernels per step: one local, two gathers.
 #include <cmath>
 #include <cstring>
@@ -138,10 +138,10 @@ __device__ __forceinline__ void adj_gather(const Static<R>* __restrict__ S, cons
   }
 }
 
-// Saved state of one step: e, ep, k1, B(e), B(ep), each N x 3 in internal site order.
+// Saved state of one step: e, ep, w = e + dt/2 k1, B(e), B(ep), each N x 3 in internal site order.
 template <typename R>
 struct StepRec {
-  const R *e, *ep, *k1, *B1, *B2;
+  const R *e, *ep, *w, *B1, *B2;
 };
 
 // Step-local part (corrector) at site i: from lambda = dL/de' and the step's record: lambda_u, the
@@ -157,8 +157,8 @@ __device__ __forceinline__ void adj_local(const Static<R>* __restrict__ S, const
   R fx, fy, fz;
   llg_rhs(px, py, pz, bx, by, bz, a, d.prec, fx, fy, fz);
   const R hdt = R(0.5) * d.dt;
-  const R ux = r.e[3 * i] + hdt * (r.k1[3 * i] + fx), uy = r.e[3 * i + 1] + hdt * (r.k1[3 * i + 1] + fy),
-          uz = r.e[3 * i + 2] + hdt * (r.k1[3 * i + 2] + fz);
+  // the record's k1 slot holds w = e + dt/2 k1 (the forward's pass-1 output), so u = w + dt/2 k2
+  const R ux = r.w[3 * i] + hdt * fx, uy = r.w[3 * i + 1] + hdt * fy, uz = r.w[3 * i + 2] + hdt * fz;
   const R n = sqrt(ux * ux + uy * uy + uz * uz);
   unproject(ux / n, uy / n, uz / n, n, lx, ly, lz);
   lamu[3 * i] = lx;
@@ -394,7 +394,7 @@ void copy_dev(const void* src, void* dst, size_t nelem, hipStream_t s) {
                      static_cast<R*>(dst), nelem);
 }
 
-// Segment records for n steps: 5 arrays (e, ep, k1, B(e), B(ep)) of n x N x 3, record (which, step)
+// Segment records for n steps: 5 arrays (e, ep, w, B(e), B(ep)) of n x N x 3, record (which, step)
 // at (which * seg_cap + step) * N * 3 values.
 void ensure_segment(mx_adjoint& a, int n) {
   if (n <= a.seg_cap) return;
@@ -593,6 +593,7 @@ int mx_adjoint_forward(mx_adjoint* a, void* e, int64_t nsteps, uint64_t seed, ui
     if (nsteps < 0) afail(MX_EINVAL, "negative step count");
     Model& m = *a->m;
     if (!m.dyn_set) afail(MX_EINVAL, "set the dynamics first");
+    if (m.integrator != MX_HEUN) afail(MX_EUNSUPPORTED, "the adjoint differentiates the Heun integrator");
     aok(hipSetDevice(m.device), "hipSetDevice");
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int64_t nck = nsteps / a->C + 1;

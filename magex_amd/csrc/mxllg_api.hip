@@ -184,7 +184,8 @@ static void ensure_graph(Model& m, void* E, uint64_t seed) {
   put(key, seed);
   for (double b : m.B) put(key, b);
   put(key, m.dt);
-  for (int v : {m.precession, m.noisy, m.coup, m.pbc[0], m.pbc[1], m.pbc[2], m.graph_steps, m.prec}) put(key, v);
+  for (int v : {m.precession, m.noisy, m.integrator, m.coup, m.pbc[0], m.pbc[1], m.pbc[2], m.graph_steps, m.prec})
+    put(key, v);
   if (m.gexec && key == m.gkey) return;
   drop_graph(m);
   if (!m.counter_d) hip_ok(hipMalloc(reinterpret_cast<void**>(&m.counter_d), sizeof(uint64_t)), "hipMalloc counter");
@@ -618,7 +619,7 @@ int mx_model_set_dynamics(mx_model* m, double dt, int alpha_per_site, const doub
     m->layout_frozen = 1;
     if (!alpha || !T) fail(MX_EINVAL, "null damping or temperature");
     if (!(dt > 0)) fail(MX_EINVAL, "dt must be positive");
-    if (integrator != MX_HEUN) fail(MX_EUNSUPPORTED, "only the Heun integrator is implemented");
+    if (integrator != MX_HEUN && integrator != MX_DEPONDT) fail(MX_EINVAL, "integrator must be MX_HEUN or MX_DEPONDT");
     use_device(*m);
     m->dt = dt;
     m->alpha_h.assign(alpha, alpha + (alpha_per_site ? m->N : 1));

@@ -184,20 +184,21 @@ __global__ __launch_bounds__(BS) void k_bheun(const BStatic<R>* __restrict__ S, 
             bz = (R(2) * hz - gz) * im + d.B2 + nz;
     R fx, fy, fz;
     llg_rhs(x, y, z, bx, by, bz, a, d.prec, fx, fy, fz);
+    const R hdt = R(0.5) * d.dt;
     if (PASS == 1) {
-      k1[3 * i] = fx;
-      k1[3 * i + 1] = fy;
-      k1[3 * i + 2] = fz;
+      // k1 keeps w = e + dt/2 k1, so pass 2 does not read e again
+      k1[3 * i] = x + hdt * fx;
+      k1[3 * i + 1] = y + hdt * fy;
+      k1[3 * i + 2] = z + hdt * fz;
       const R px = x + d.dt * fx, py = y + d.dt * fy, pz = z + d.dt * fz;
       const R q = rnorm3(px, py, pz);
       ep[3 * i] = px * q;
       ep[3 * i + 1] = py * q;
       ep[3 * i + 2] = pz * q;
     } else {
-      const R hdt = R(0.5) * d.dt;
-      const R cx = e[3 * i] + hdt * (k1[3 * i] + fx);
-      const R cy = e[3 * i + 1] + hdt * (k1[3 * i + 1] + fy);
-      const R cz = e[3 * i + 2] + hdt * (k1[3 * i + 2] + fz);
+      const R cx = k1[3 * i] + hdt * fx;
+      const R cy = k1[3 * i + 1] + hdt * fy;
+      const R cz = k1[3 * i + 2] + hdt * fz;
       const R q = rnorm3(cx, cy, cz);
       e[3 * i] = cx * q;
       e[3 * i + 1] = cy * q;

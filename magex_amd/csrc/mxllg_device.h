@@ -81,6 +81,35 @@ __device__ __forceinline__ void llg_rhs(R ex, R ey, R ez, R bx, R by, R bz, R a,
   }
 }
 
+// Rodrigues rotation of (x, y, z) by the rotation vector (wx, wy, wz) (angle |w|); the identity below
+// |w| = 1e-14 (magex/fast.py:271-282, llg.py:31-36).  1 - cos is formed as 2 sin^2(th / 2), which
+// keeps its relative accuracy at the small angles of a time step (it cancels in FP32 otherwise).
+template <typename R>
+__device__ __forceinline__ void rotate(R& x, R& y, R& z, R wx, R wy, R wz) {
+  const R th = sqrt(wx * wx + wy * wy + wz * wz);
+  if (!(th >= R(1e-14))) return;
+  const R kx = wx / th, ky = wy / th, kz = wz / th;
+  const R s = sin(th), h = sin(R(0.5) * th);
+  const R omc = R(2) * h * h, c = R(1) - omc;
+  const R kd = kx * x + ky * y + kz * z;
+  const R cx = ky * z - kz * y, cy = kz * x - kx * z, cz = kx * y - ky * x;
+  const R x2 = x * c + cx * s + kx * kd * omc;
+  const R y2 = y * c + cy * s + ky * kd * omc;
+  const R z2 = z * c + cz * s + kz * kd * omc;
+  x = x2;
+  y = y2;
+  z = z2;
+}
+
+// Depondt's rotation vector: de/dt = w x e with w = e x f for the tangent right-hand side f
+// (magex/fast.py:284-288).
+template <typename R>
+__device__ __forceinline__ void omega(R ex, R ey, R ez, R fx, R fy, R fz, R& wx, R& wy, R& wz) {
+  wx = ey * fz - ez * fy;
+  wy = ez * fx - ex * fz;
+  wz = ex * fy - ey * fx;
+}
+
 // Wavefront (64-lane) sum / max by xor shuffles, then across the 4 wavefronts of the block in LDS,
 // in a fixed order (bitwise reproducible).
 __device__ __forceinline__ double wave_sum(double v) {

@@ -58,6 +58,7 @@ template <typename R>
 struct Dyn {
   R B0, B1, B2, dt;
   int prec, noisy;
+  int integ;  // MX_HEUN or MX_DEPONDT
   uint32_t seed_lo, seed_hi;
   const R* bond;  // COUP_BOND: [nfwd][ncell][9], row-major tensor per bond (36 B FP32 / 72 B FP64)
   const R* A;     // ONSITE_SITE6: 6 x N (xx, yy, zz, xy, xz, yz); ONSITE_SITE9: 9 x N (row-major)
@@ -119,7 +120,7 @@ struct Model {
   void* static_d = nullptr;
   int static_dirty = 1;
   // scratch
-  void *ep_d = nullptr, *k1_d = nullptr, *eint_d = nullptr;
+  void *ep_d = nullptr, *k1_d = nullptr, *eint_d = nullptr;  // predictor ep; k1_d holds w = e + dt/2 k1
   double* part_d = nullptr;
   int npart = 0;
   // captured step loop (hipGraph of graph_steps Heun steps + counter advance), keyed by what it bakes in

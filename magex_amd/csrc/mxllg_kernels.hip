@@ -24,9 +24,13 @@
 namespace mx {
 
 // ------------------------------------------------------------------------------------------ Heun
-// PASS 1 (magex/fast.py:336-368): noise + field(e) + predictor ep = n(e + dt k1), keeps k1.
-// PASS 2 (magex/fast.py:370-390): field(ep) + corrector e = n(e + dt/2 (k1 + f(ep))), in place.
-// The noise of a step is regenerated in pass 2 from the same counter rather than stored.
+// PASS 1 (magex/fast.py:336-368): noise + field(e) + predictor ep = n(e + dt k1); keeps
+//         w = e + dt/2 k1 (the corrector's part known after pass 1) in the buffer k1.
+// PASS 2 (magex/fast.py:370-390): field(ep) + corrector e = n(w + dt/2 f(ep)), in place; e itself is
+//         not read again.  The noise of a step is regenerated in pass 2 from the same counter.
+// Depondt (d.integ == MX_DEPONDT, the same passes' depondt branches): pass 1 keeps the rotation vector
+//         w1 = e x f(e) in k1 and rotates ep = R(dt w1) e; pass 2 rotates e = n(R(dt/2 (w1 + w2)) e) with
+//         w2 = ep x f(ep).
 template <typename R, int CM, int GEO, int PASS>
 __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S, const Dyn<R> d, R* __restrict__ e,
                                                 R* __restrict__ ep, R* __restrict__ k1, uint64_t step,
@@ -43,7 +47,9 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S,
       ep[3 * i] = x;
       ep[3 * i + 1] = y;
       ep[3 * i + 2] = z;
-      k1[3 * i] = k1[3 * i + 1] = k1[3 * i + 2] = R(0);
+      k1[3 * i] = x;
+      k1[3 * i + 1] = y;
+      k1[3 * i + 2] = z;
       return;
     }
     R bx, by, bz, fx, fy, fz;
@@ -57,9 +63,22 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S,
       d.save_B[3 * i + 2] = bz;
     }
     llg_rhs(x, y, z, bx, by, bz, a, d.prec, fx, fy, fz);
-    k1[3 * i] = fx;
-    k1[3 * i + 1] = fy;
-    k1[3 * i + 2] = fz;
+    if (d.integ == MX_DEPONDT) {
+      R wx, wy, wz, px = x, py = y, pz = z;
+      omega(x, y, z, fx, fy, fz, wx, wy, wz);
+      k1[3 * i] = wx;
+      k1[3 * i + 1] = wy;
+      k1[3 * i + 2] = wz;
+      rotate(px, py, pz, d.dt * wx, d.dt * wy, d.dt * wz);
+      ep[3 * i] = px;
+      ep[3 * i + 1] = py;
+      ep[3 * i + 2] = pz;
+      return;
+    }
+    const R hdt = R(0.5) * d.dt;
+    k1[3 * i] = x + hdt * fx;
+    k1[3 * i + 1] = y + hdt * fy;
+    k1[3 * i + 2] = z + hdt * fz;
     const R px = x + d.dt * fx, py = y + d.dt * fy, pz = z + d.dt * fz;
     const R q = rnorm3(px, py, pz);
     ep[3 * i] = px * q;
@@ -77,9 +96,19 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S,
     }
     llg_rhs(x, y, z, bx, by, bz, a, d.prec, fx, fy, fz);
     const R hdt = R(0.5) * d.dt;
-    const R cx = e[3 * i] + hdt * (k1[3 * i] + fx);
-    const R cy = e[3 * i + 1] + hdt * (k1[3 * i + 1] + fy);
-    const R cz = e[3 * i + 2] + hdt * (k1[3 * i + 2] + fz);
+    R cx, cy, cz;
+    if (d.integ == MX_DEPONDT) {
+      R wx, wy, wz;
+      omega(x, y, z, fx, fy, fz, wx, wy, wz);
+      cx = e[3 * i];
+      cy = e[3 * i + 1];
+      cz = e[3 * i + 2];
+      rotate(cx, cy, cz, hdt * (k1[3 * i] + wx), hdt * (k1[3 * i + 1] + wy), hdt * (k1[3 * i + 2] + wz));
+    } else {
+      cx = k1[3 * i] + hdt * fx;
+      cy = k1[3 * i + 1] + hdt * fy;
+      cz = k1[3 * i + 2] + hdt * fz;
+    }
     const R q = rnorm3(cx, cy, cz);
     e[3 * i] = cx * q;
     e[3 * i + 1] = cy * q;
@@ -299,6 +328,7 @@ static Dyn<R> make_dyn(const Model& m, uint64_t seed) {
   d.dt = R(m.dt);
   d.prec = m.precession;
   d.noisy = m.noisy;
+  d.integ = m.integrator;
   d.seed_lo = uint32_t(seed);
   d.seed_hi = uint32_t(seed >> 32);
   d.bond = static_cast<const R*>(m.bond_d);

@@ -172,15 +172,18 @@ class DeviceModel:
     def set_field(self, B):
         check(self.lib.mx_model_set_field(self.h, hptr(np.ascontiguousarray(np.asarray(B, float).reshape(3)))))
 
-    def set_dynamics(self, dt, alpha, T, precession=True):
-        """alpha, T: scalars or per-site arrays (N,)."""
+    def set_dynamics(self, dt, alpha, T, precession=True, integrator="heun"):
+        """alpha, T: scalars or per-site arrays (N,); integrator "heun" or "depondt"."""
         a = np.ascontiguousarray(np.atleast_1d(np.asarray(alpha, float)))
         t = np.ascontiguousarray(np.atleast_1d(np.asarray(T, float)))
-        key = (float(dt), a.tobytes(), t.tobytes(), bool(precession))
+        codes = {"heun": _lib.MX_HEUN, "depondt": _lib.MX_DEPONDT}
+        if integrator not in codes:
+            raise ValueError(f"integrator must be one of {tuple(codes)}")
+        key = (float(dt), a.tobytes(), t.tobytes(), bool(precession), integrator)
         if key == self._dyn:
             return
         check(self.lib.mx_model_set_dynamics(self.h, float(dt), int(a.size > 1), hptr(a), int(t.size > 1), hptr(t),
-                                             int(bool(precession)), _lib.MX_HEUN))
+                                             int(bool(precession)), codes[integrator]))
         self._dyn = key
 
     def layout(self):

@@ -20,7 +20,7 @@ from .constants import GAMMA, K_B, MU_B  # noqa: F401
 from .spinmodel import normalize  # noqa: F401
 
 INTEGRATORS = ("heun", "rk4", "depondt", "sib", "euler")
-DEVICE_INTEGRATORS = ("heun",)
+DEVICE_INTEGRATORS = ("heun", "depondt")
 
 
 def llg_rhs(e, B, alpha, precession=True, precision="fp64"):
@@ -72,7 +72,7 @@ class LLGSolver:
         if integrator not in INTEGRATORS:
             raise ValueError(f"integrator must be one of {INTEGRATORS}")
         if integrator not in DEVICE_INTEGRATORS:
-            raise NotImplementedError(f"integrator '{integrator}' is not on the device path (heun is)")
+            raise NotImplementedError(f"integrator '{integrator}' is not on the device path (heun and depondt are)")
         if thermostat != "classical":
             raise NotImplementedError("only the classical (white-noise) thermostat is on the device path")
         if stt or sot:
@@ -100,7 +100,7 @@ class LLGSolver:
         self.fast = self.dm
         from .structure import site_temperature
         T = site_temperature(self.sys, self.T)
-        self.dm.set_dynamics(self.dt, self.alpha, T, self.precession)
+        self.dm.set_dynamics(self.dt, self.alpha, T, self.precession, self.integrator)
         return self.dm
 
     def _to_device(self, e):

@@ -9,6 +9,9 @@ Pinning (what each part is checked against, see tests/test_oracle.py):
   exchange_field / gradient / energy               -> golden vectors + central finite differences
   heun_step                                        -> golden trajectories (T=0 and injected noise),
                                                       Larmor closed form, damped-spin closed form
+  rotate / depondt_step                            -> Rodrigues closed form (rotation about an axis),
+                                                      second-order convergence to the exact LLG flow
+                                                      (a fine-step integration), agreement with Heun
   topological_charge                               -> golden charges + integer-Q textures
   neighbour_blocks                                 -> exact equality of its mat-vec with Jsp @ e
 """
@@ -283,8 +286,27 @@ def heun_step(system, e, alpha, dt, b=0.0, precession=True):
     return normalize(e + 0.5 * dt * (k1 + k2))
 
 
-def run_heun(system, e, steps, alpha, dt, T=0.0, normals=None, precession=True):
-    """steps Heun steps; normals(n) -> (N,3) standard normals of step n (injected noise)."""
+def rotate(e, w):                                            # llg.py:31-36
+    """Rodrigues rotation of unit vectors e (N,3) by rotation vectors w (N,3) (angle |w|)."""
+    th = np.linalg.norm(w, axis=1, keepdims=True)
+    k = np.where(th > 1e-14, w / np.maximum(th, 1e-300), 0.0)
+    c, s_ = np.cos(th), np.sin(th)
+    return e * c + np.cross(k, e) * s_ + k * np.einsum("ia,ia->i", k, e)[:, None] * (1 - c)
+
+
+def depondt_step(system, e, alpha, dt, b=0.0, precession=True):
+    """One Depondt step (Heun on rotation vectors), thermal field b held over both stages
+    (llg.py:236-240; fast.py:284-306)."""
+    a = alpha[:, None] if np.ndim(alpha) else alpha
+    w1 = np.cross(e, llg_rhs(e, system.effective_field(e) + b, a, precession))
+    ep = rotate(e, dt * w1)
+    w2 = np.cross(ep, llg_rhs(ep, system.effective_field(ep) + b, a, precession))
+    return normalize(rotate(e, 0.5 * dt * (w1 + w2)))
+
+
+def run_heun(system, e, steps, alpha, dt, T=0.0, normals=None, precession=True, integrator="heun"):
+    """steps Heun (or Depondt) steps; normals(n) -> (N,3) standard normals of step n (injected noise)."""
+    step = {"heun": heun_step, "depondt": depondt_step}[integrator]
     sg = sigma(system, alpha, T, dt) if (T > 0 or system.site_T is not None) else None
     for n in range(steps):
         b = 0.0
@@ -292,7 +314,7 @@ def run_heun(system, e, steps, alpha, dt, T=0.0, normals=None, precession=True):
             b = sg[:, None] * normals(n)
             if system.pinned is not None:
                 b[system.pinned] = 0.0
-        e = heun_step(system, e, alpha, dt, b, precession)
+        e = step(system, e, alpha, dt, b, precession)
     return e
 
 

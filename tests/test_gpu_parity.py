@@ -284,3 +284,62 @@ def test_llg_rhs_on_device(gpu):
     B = rng.normal(size=(100, 3))
     for prec in (True, False):
         assert _rel(llg_rhs(e, B, 0.3, prec), R.llg_rhs(e, B, 0.3, prec)) < 1e-14
+
+
+@pytest.mark.parametrize("precession", [True, False])
+def test_depondt_matches_oracle(gpu, precession):
+    """LLGSolver(integrator="depondt") against the oracle's Depondt step: T = 0 on a per-bond bcc
+    lattice, and the internal generator at T > 0 on a triangular DMI lattice (device normals)."""
+    print(f"case: depondt precession={precession}", flush=True)
+    s, g = _bcc_perbond()
+    d = json.load(open(os.path.join(GOLDEN, "bcc_conventional_model.json")))
+    o = R.SpinSystem(R.load_results(d), (4, 4, 4))
+    import scipy.sparse as sp
+    rows = (3 * g["src"][:, None, None] + np.arange(3)[None, :, None]).repeat(3, 2).ravel()
+    cols = (3 * g["dst"][:, None, None] + np.arange(3)[None, None, :]).repeat(3, 1).ravel()
+    o.Jsp = sp.csr_matrix((g["blocks"].ravel(), (rows, cols)), shape=(3 * o.N, 3 * o.N))
+    o.A = g["A"]
+    s.set_field((0.2, 0, 0.5))
+    o.set_field((0.2, 0, 0.5))
+    e0 = g["e0"]
+    sol = LLGSolver(s, alpha=0.2, dt=1e-4, integrator="depondt", precession=precession)
+    e, _, _ = sol.run(e0, 60, output_every=60)
+    ref = R.run_heun(o, e0.copy(), 60, 0.2, 1e-4, precession=precession, integrator="depondt")
+    assert _rel(e, ref) < 1e-11
+    s, o = _pair((24, 20, 1), (0, 0, 2.0), lattice="triangular", J=[5.0, -0.5, 0.2], D=1.5, K=0.3)
+    e0 = s.init_spins("random", seed=2)
+    sol = LLGSolver(s, alpha=0.05, dt=5e-4, temperature=20.0, seed=5, integrator="depondt", precession=precession)
+    e, _, _ = sol.run(e0, 40, output_every=40)
+    dm = s.device_model("fp64")
+    normals = [dm.noise(5, n).cpu().numpy() for n in range(40)]
+    ref = R.run_heun(o, e0.copy(), 40, 0.05, 5e-4, T=20.0, normals=lambda n: normals[n], precession=precession,
+                     integrator="depondt")
+    assert _rel(e, ref) < 1e-11
+    # FP32 Depondt carries no more round-off than FP32 Heun on the same noise: its mean angle from
+    # FP64 Depondt is within twice the FP32 Heun angle from FP64 Heun (plus 1e-7 rad)
+    def angle(integrator):
+        kw = dict(alpha=0.05, dt=5e-4, temperature=20.0, seed=5, integrator=integrator, precession=precession)
+        e64, _, _ = LLGSolver(s, **kw).run(e0, 40, output_every=40)
+        e32, _, _ = LLGSolver(s, precision="fp32", **kw).run(e0, 40, output_every=40)
+        return np.arccos(np.clip(np.sum(np.asarray(e32, float) * e64, 1), -1, 1)).mean()
+
+    a_dep, a_heun = angle("depondt"), angle("heun")
+    print(f"  FP32 mean angle: depondt {a_dep:.3e}  heun {a_heun:.3e}", flush=True)
+    assert a_dep < 2 * a_heun + 1e-7
+
+
+@pytest.mark.parametrize("integrator", ["heun", "depondt"])
+def test_device_class_matches_solver(gpu, integrator):
+    """The magex GPU-class shape (CudaLLG(s, 0.2, 1e-3, 0.0, 1, it).run(e0, 20), as magex's
+    tests/cuda_sim_check.py drives it) gives LLGSolver's trajectory, and its callback sees the spins."""
+    print(f"case: CudaLLG {integrator}", flush=True)
+    from magex_amd.hip_backend import CudaLLG
+    s, o = _pair((6, 6, 1), (0, 0, 1.0), lattice="square", J=[1.0, 0.2], D=0.3, K=0.05)
+    e0 = s.init_spins("random", seed=4)
+    seen = []
+    eg = CudaLLG(s, 0.2, 1e-3, 0.0, 1, integrator).run(e0, 20, callback=lambda n, e: seen.append((n, e)), every=8)
+    ref = R.run_heun(o, e0.copy(), 20, 0.2, 1e-3, integrator=integrator)
+    assert _rel(eg, ref) < 1e-12
+    assert [n for n, _ in seen] == [8, 16]
+    e8, _, _ = LLGSolver(s, alpha=0.2, dt=1e-3, seed=1, integrator=integrator).run(e0, 8, output_every=8)
+    assert np.array_equal(seen[0][1], e8)

@@ -276,3 +276,53 @@ def test_torch_heun_gradient_matches_central_differences():
         a = ((Tbad.run(torch.as_tensor(th), torch.as_tensor(e0), 20) - tgt) ** 2).mean()
         b = loss(torch.as_tensor(th), torch.as_tensor(e0))
     assert abs(a - b) > 1e-9
+
+
+# ------------------------------------------------------------------ Depondt
+def test_rotate_is_rodrigues():
+    """oracle rotate against scipy's rotation-vector routine and a quarter turn about z."""
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(0)
+    e = R.normalize(rng.normal(size=(50, 3)))
+    w = rng.normal(size=(50, 3)) * 0.7
+    w[3] = 0.0
+    ref = np.array([Rotation.from_rotvec(w[i]).apply(e[i]) for i in range(50)])
+    assert np.abs(R.rotate(e, w) - ref).max() < 1e-14
+    q = R.rotate(np.array([[1.0, 0, 0]]), np.array([[0, 0, np.pi / 2]]))
+    assert np.allclose(q, [[0, 1, 0]], atol=1e-15)
+
+
+def _flow_error(step_fn, s, e0, dt, t_end):
+    """Max deviation from the exact LLG flow (DOP853, tolerance 1e-13) after t_end ps."""
+    from scipy.integrate import solve_ivp
+
+    def f(t, y):
+        e = y.reshape(-1, 3)
+        return R.llg_rhs(e, s.effective_field(e), 0.1).ravel()
+
+    exact = solve_ivp(f, (0, t_end), e0.ravel(), method="DOP853", rtol=1e-13, atol=1e-13).y[:, -1].reshape(-1, 3)
+    e = e0.copy()
+    for _ in range(int(round(t_end / dt))):
+        e = step_fn(s, e, 0.1, dt)
+    return np.abs(e - exact).max()
+
+
+def test_depondt_is_second_order():
+    """Halving dt divides the global error by 4 (second order); a corrector that forgets the second
+    rotation vector is only first order and fails the same check."""
+    s = _sys((3, 3, 3), field=(0.3, 0, 1.0), lattice="sc", J=[1.0], K=0.1, D=0.0)
+    e0 = s.init_random(1)
+    errs = [_flow_error(R.depondt_step, s, e0, dt, 0.2) for dt in (8e-3, 4e-3, 2e-3)]
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    assert 3.5 < r1 < 4.5 and 3.5 < r2 < 4.5, errs
+
+    def first_order(sys_, e, alpha, dt):
+        w1 = np.cross(e, R.llg_rhs(e, sys_.effective_field(e), alpha))
+        return R.normalize(R.rotate(e, dt * w1))
+
+    bad = [_flow_error(first_order, s, e0, dt, 0.2) for dt in (8e-3, 4e-3)]
+    assert bad[0] / bad[1] < 2.5
+    # Depondt and Heun agree to O(dt^2)
+    h = R.run_heun(s, e0.copy(), 50, 0.1, 2e-3)
+    d = R.run_heun(s, e0.copy(), 50, 0.1, 2e-3, integrator="depondt")
+    assert np.abs(h - d).max() < 1e-4
