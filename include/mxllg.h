@@ -108,6 +108,18 @@ int mx_model_set_dynamics(mx_model* m, double dt, int alpha_per_site, const doub
 int mx_llg_run(mx_model* m, void* e, int64_t nsteps, uint64_t seed, uint64_t step0, const void* noise,
                void* stream);
 
+/* nsteps steps as mx_llg_run with internal noise, under a time-dependent uniform field: B (host,
+ * nsteps x 2 x 3 doubles) gives, for step n, the field of the predictor pass (B[6n..6n+2]) and of the
+ * corrector pass (B[6n+3..6n+5]) in place of the model's field -- magex evaluates field_func at t and
+ * t + dt (magex/llg.py:129-130, 204-211, 216-227). */
+int mx_llg_run_driven(mx_model* m, void* e, int64_t nsteps, const double* B, uint64_t seed, uint64_t step0,
+                      void* stream);
+
+/* Spin-orbit torque as effective fields on the dynamics (not on mx_field / mx_observables, as in
+ * magex): B += B_DL (e x p) + B_FL p, p a unit vector (magex/llg.py:83-86, 126-128).  B_DL = B_FL = 0
+ * switches it off. */
+int mx_model_set_sot(mx_model* m, const double p[3], double B_DL, double B_FL);
+
 /* Steps per captured step loop (hipGraph replayed by mx_llg_run when no noise is injected and at least
  * two loops fit; default 0 = always launch step by step).  Results are identical either way: the
  * captured kernels read the step number from a device counter, so the noise sequence is the same.

@@ -594,6 +594,7 @@ int mx_adjoint_forward(mx_adjoint* a, void* e, int64_t nsteps, uint64_t seed, ui
     Model& m = *a->m;
     if (!m.dyn_set) afail(MX_EINVAL, "set the dynamics first");
     if (m.integrator != MX_HEUN) afail(MX_EUNSUPPORTED, "the adjoint differentiates the Heun integrator");
+    if (m.sot_dl != 0.0 || m.sot_fl != 0.0) afail(MX_EUNSUPPORTED, "spin-orbit torque fields are not differentiated");
     aok(hipSetDevice(m.device), "hipSetDevice");
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int64_t nck = nsteps / a->C + 1;

@@ -183,6 +183,7 @@ static void ensure_graph(Model& m, void* E, uint64_t seed) {
     put(key, p);
   put(key, seed);
   for (double b : m.B) put(key, b);
+  for (double v : {m.sot_p[0], m.sot_p[1], m.sot_p[2], m.sot_dl, m.sot_fl}) put(key, v);
   put(key, m.dt);
   for (int v : {m.precession, m.noisy, m.integrator, m.coup, m.pbc[0], m.pbc[1], m.pbc[2], m.graph_steps, m.prec})
     put(key, v);
@@ -700,6 +701,45 @@ int mx_llg_run(mx_model* m, void* e, int64_t nsteps, uint64_t seed, uint64_t ste
       if (rc) return rc;
     }
     if (needs_permute(*m)) return launch_permute(*m, m->eint_d, e, 0, s);
+    return MX_OK;
+  });
+}
+
+int mx_llg_run_driven(mx_model* m, void* e, int64_t nsteps, const double* B, uint64_t seed, uint64_t step0,
+                      void* stream) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (!e || !B) fail(MX_EINVAL, "null spins or fields");
+    if (nsteps < 0) fail(MX_EINVAL, "negative step count");
+    if (!m->dyn_set) fail(MX_EINVAL, "set the dynamics first");
+    use_device(*m);
+    ensure_scratch(*m);
+    sync_static(*m);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    void* E = e;
+    if (needs_permute(*m)) {
+      int rc = launch_permute(*m, e, m->eint_d, 1, s);
+      if (rc) return rc;
+      E = m->eint_d;
+    }
+    for (int64_t n = 0; n < nsteps; ++n) {
+      const uint64_t step = step0 + uint64_t(n);
+      int rc = launch_heun_pass(*m, 1, E, m->ep_d, m->k1_d, seed, step, nullptr, nullptr, s, B + 6 * n);
+      if (rc == MX_OK) rc = launch_heun_pass(*m, 2, E, m->ep_d, m->k1_d, seed, step, nullptr, nullptr, s, B + 6 * n + 3);
+      if (rc) return rc;
+    }
+    if (needs_permute(*m)) return launch_permute(*m, m->eint_d, e, 0, s);
+    return MX_OK;
+  });
+}
+
+int mx_model_set_sot(mx_model* m, const double p[3], double B_DL, double B_FL) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (!p) fail(MX_EINVAL, "null polarisation");
+    for (int a = 0; a < 3; ++a) m->sot_p[a] = p[a];
+    m->sot_dl = B_DL;
+    m->sot_fl = B_FL;
     return MX_OK;
   });
 }

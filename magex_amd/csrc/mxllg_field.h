@@ -313,6 +313,12 @@ __device__ __forceinline__ void local_field(const Static<R>* __restrict__ S, con
   bx = (R(2) * hx - gx) * im + d.B0 + nx;
   by = (R(2) * hy - gy) * im + d.B1 + ny;
   bz = (R(2) * hz - gz) * im + d.B2 + nz;
+  if (d.sot) {
+    // spin-orbit (Slonczewski-like) torque as fields: B_DL (e x p) + B_FL p (magex/llg.py:126-128)
+    bx += d.bdl * (y * d.sp2 - z * d.sp1) + d.bfl * d.sp0;
+    by += d.bdl * (z * d.sp0 - x * d.sp2) + d.bfl * d.sp1;
+    bz += d.bdl * (x * d.sp1 - y * d.sp0) + d.bfl * d.sp2;
+  }
 }
 
 }  // namespace mx

@@ -59,6 +59,8 @@ struct Dyn {
   R B0, B1, B2, dt;
   int prec, noisy;
   int integ;  // MX_HEUN or MX_DEPONDT
+  int sot;    // spin-orbit torque fields on: B += bdl (e x p) + bfl p
+  R sp0, sp1, sp2, bdl, bfl;
   uint32_t seed_lo, seed_hi;
   const R* bond;  // COUP_BOND: [nfwd][ncell][9], row-major tensor per bond (36 B FP32 / 72 B FP64)
   const R* A;     // ONSITE_SITE6: 6 x N (xx, yy, zz, xy, xz, yz); ONSITE_SITE9: 9 x N (row-major)
@@ -108,6 +110,7 @@ struct Model {
   void* sign_d = nullptr;
   void* pinned_d = nullptr;
   double B[3] = {0, 0, 0};
+  double sot_p[3] = {0, 1, 0}, sot_dl = 0, sot_fl = 0;  // spin-orbit torque fields (off when both are 0)
   double dt = 1e-3;
   int precession = 1, integrator = MX_HEUN;
   std::vector<double> alpha_h, T_h;  // size 1 (uniform) or N (per site), magex order
@@ -137,8 +140,9 @@ int launch_heun_step(const Model& m, void* e, uint64_t seed, uint64_t step, cons
                      const uint64_t* step_ptr = nullptr);
 int launch_counter(uint64_t* c, uint64_t value, int add, hipStream_t s);
 // One Heun pass (1 or 2) with optional stores of e and B_eff (magex order is not used: internal order).
+// B_ext (3 doubles) replaces the model's uniform field for this pass when non-null.
 int launch_heun_pass(const Model& m, int pass, void* e, void* ep, void* k1, uint64_t seed, uint64_t step,
-                     void* save_e, void* save_B, hipStream_t s);
+                     void* save_e, void* save_B, hipStream_t s, const double* B_ext = nullptr);
 int sync_model_static(Model& m);
 size_t prec_size(const Model& m);
 int launch_field(const Model& m, const void* e_int, void* out_ref, int kind, hipStream_t s);

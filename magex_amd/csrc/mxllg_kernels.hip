@@ -329,6 +329,12 @@ static Dyn<R> make_dyn(const Model& m, uint64_t seed) {
   d.prec = m.precession;
   d.noisy = m.noisy;
   d.integ = m.integrator;
+  d.sot = (m.sot_dl != 0.0 || m.sot_fl != 0.0) ? 1 : 0;
+  d.sp0 = R(m.sot_p[0]);
+  d.sp1 = R(m.sot_p[1]);
+  d.sp2 = R(m.sot_p[2]);
+  d.bdl = R(m.sot_dl);
+  d.bfl = R(m.sot_fl);
   d.seed_lo = uint32_t(seed);
   d.seed_hi = uint32_t(seed >> 32);
   d.bond = static_cast<const R*>(m.bond_d);
@@ -393,9 +399,14 @@ int launch_heun_step(const Model& m, void* e, uint64_t seed, uint64_t step, cons
 
 template <typename R>
 static int pass_t(const Model& m, int pass, void* e, void* ep, void* k1, uint64_t seed, uint64_t step, void* save_e,
-                  void* save_B, hipStream_t s) {
+                  void* save_B, hipStream_t s, const double* B_ext) {
   const Static<R>* S = static_cast<const Static<R>*>(m.static_d);
   Dyn<R> d = make_dyn<R>(m, seed);
+  if (B_ext) {
+    d.B0 = R(B_ext[0]);
+    d.B1 = R(B_ext[1]);
+    d.B2 = R(B_ext[2]);
+  }
   d.save_e = static_cast<R*>(save_e);
   d.save_B = static_cast<R*>(save_B);
   R* E = static_cast<R*>(e);
@@ -412,9 +423,9 @@ static int pass_t(const Model& m, int pass, void* e, void* ep, void* k1, uint64_
 }
 
 int launch_heun_pass(const Model& m, int pass, void* e, void* ep, void* k1, uint64_t seed, uint64_t step,
-                     void* save_e, void* save_B, hipStream_t s) {
-  return m.prec == MX_FP64 ? pass_t<double>(m, pass, e, ep, k1, seed, step, save_e, save_B, s)
-                           : pass_t<float>(m, pass, e, ep, k1, seed, step, save_e, save_B, s);
+                     void* save_e, void* save_B, hipStream_t s, const double* B_ext) {
+  return m.prec == MX_FP64 ? pass_t<double>(m, pass, e, ep, k1, seed, step, save_e, save_B, s, B_ext)
+                           : pass_t<float>(m, pass, e, ep, k1, seed, step, save_e, save_B, s, B_ext);
 }
 
 int launch_counter(uint64_t* c, uint64_t value, int add, hipStream_t s) {

@@ -212,6 +212,21 @@ class DeviceModel:
                                   None if noise is None else ctypes.c_void_p(noise.data_ptr()), _stream()))
         return e
 
+    def run_driven(self, e, B, seed, step0=0):
+        """len(B) steps on e with the uniform field B[n, 0] in the predictor and B[n, 1] in the corrector
+        of step n (B: (nsteps, 2, 3) host array, in place of the model's field)."""
+        self._check_spins(e)
+        B = np.ascontiguousarray(np.asarray(B, float).reshape(-1, 2, 3))
+        seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        check(self.lib.mx_llg_run_driven(self.h, ctypes.c_void_p(e.data_ptr()), len(B), hptr(B), seed, int(step0),
+                                         _stream()))
+        return e
+
+    def set_sot(self, p, B_DL, B_FL):
+        """Spin-orbit torque fields B_DL (e x p) + B_FL p on the dynamics (p a unit vector)."""
+        check(self.lib.mx_model_set_sot(self.h, hptr(np.ascontiguousarray(np.asarray(p, float).reshape(3))),
+                                        float(B_DL), float(B_FL)))
+
     def field(self, e, kind=_lib.MX_FIELD_BEFF):
         self._check_spins(e)
         out = _torch().empty_like(e)

@@ -51,9 +51,11 @@ class LLGSolver:
 
     Same constructor as magex's LLGSolver plus ``precision`` ("fp64", magex's arithmetic, or "fp32").
     backend: "auto", "hip" or "numba" (all mean the device kernels); "numpy" raises: this package
-    has no CPU path.  Only the Heun integrator with the classical thermostat is on the device path
-    (ValueError for an unknown integrator, NotImplementedError for the others, STT / SOT, the quantum
-    thermostat, field_func and extra_field).
+    has no CPU path.  On the device path: the Heun and Depondt integrators, the classical thermostat,
+    spin-orbit torque fields (sot) and a uniform time-dependent ``field_func(t) -> (3,)``, evaluated
+    at t in the predictor and t + dt in the corrector as magex does.  ValueError for an unknown
+    integrator; NotImplementedError for rk4 / sib / euler, STT, the quantum thermostat, a
+    site-dependent field_func and extra_field.
     """
 
     def __init__(self, system, alpha=0.1, dt=1e-3, temperature=0.0, seed=None, precession=True,
@@ -75,8 +77,12 @@ class LLGSolver:
             raise NotImplementedError(f"integrator '{integrator}' is not on the device path (heun and depondt are)")
         if thermostat != "classical":
             raise NotImplementedError("only the classical (white-noise) thermostat is on the device path")
-        if stt or sot:
-            raise NotImplementedError("spin-transfer / spin-orbit torques are not on the device path")
+        if stt:
+            raise NotImplementedError("spin-transfer torque (a lattice-gradient term) is not on the device path")
+        self.sot = None
+        if sot:                                  # magex/llg.py:83-86
+            p = np.asarray(sot.get("p", [0, 1, 0]), float)
+            self.sot = (p / np.linalg.norm(p), float(sot.get("B_DL", 0.0)), float(sot.get("B_FL", 0.0)))
         if backend not in ("auto", "hip", "numba"):
             if backend == "numpy":
                 raise RuntimeError("magex_amd has no NumPy path: use backend='auto' (HIP kernels)")
@@ -86,7 +92,7 @@ class LLGSolver:
         self.time = 0.0
         self.field_func = None
         self.extra_field = None
-        self.stt = self.sot = None
+        self.stt = None
         # the noise key: the seed itself, or 64 bits drawn from fresh entropy
         self.noise_seed = int(seed) if seed is not None else int(self.rng.integers(0, 2**63))
         self.step_count = 0
@@ -94,9 +100,11 @@ class LLGSolver:
 
     # ------------------------------------------------------------------ device plumbing
     def _sync(self):
-        if self.field_func is not None or self.extra_field is not None:
-            raise NotImplementedError("field_func / extra_field are not on the device path")
+        if self.extra_field is not None:
+            raise NotImplementedError("extra_field (an arbitrary function of the spins) is not on the device path")
         self.dm = self.sys.device_model(self.precision)
+        p, bdl, bfl = self.sot if self.sot is not None else ((0.0, 1.0, 0.0), 0.0, 0.0)
+        self.dm.set_sot(p, bdl, bfl)
         self.fast = self.dm
         from .structure import site_temperature
         T = site_temperature(self.sys, self.T)
@@ -129,7 +137,21 @@ class LLGSolver:
         """Advance a device tensor in place by nsteps (the CudaLLG.step_device shape,
         magex/cuda_backend.py:362-369).  noise: injected (nsteps, N, 3) standard normals or None."""
         dm = self._sync()
-        dm.run(e_dev, nsteps, self.noise_seed, self.step_count, noise)
+        if self.field_func is not None:
+            if noise is not None:
+                raise NotImplementedError("injected noise together with field_func")
+            t = self.time + self.dt * np.arange(int(nsteps))
+            B = np.empty((int(nsteps), 2, 3))
+            for n, tn in enumerate(t):
+                for k, tk in enumerate((tn, tn + self.dt)):
+                    f = np.asarray(self.field_func(tk), float)
+                    if f.size != 3:
+                        raise NotImplementedError("a site-dependent field_func is not on the device path "
+                                                  "(return a uniform (3,) field)")
+                    B[n, k] = self.sys.B + f.reshape(3)
+            dm.run_driven(e_dev, B, self.noise_seed, self.step_count)
+        else:
+            dm.run(e_dev, nsteps, self.noise_seed, self.step_count, noise)
         self.step_count += int(nsteps)
         self.time += nsteps * self.dt
         return e_dev

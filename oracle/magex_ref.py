@@ -7,6 +7,11 @@ checker.  Each function cites the magex file:line it restates (all paths inside 
 Pinning (what each part is checked against, see tests/test_oracle.py):
   make_model / load_results / SpinSystem assembly  -> golden vectors of magex itself (tests/golden)
   exchange_field / gradient / energy               -> golden vectors + central finite differences
+  CubicAnisotropy / UniaxialHigher                 -> closed-form energies on symmetry directions,
+                                                      central differences of their energies
+  solver_field (spin-orbit torque, field_func)     -> closed-form single-spin solutions: precession
+                                                      phase under B_z(t); damping-like relaxation
+                                                      tan(theta/2) = tan(theta0/2) exp(-gamma B_DL t)
   heun_step                                        -> golden trajectories (T=0 and injected noise),
                                                       Larmor closed form, damped-spin closed form
   rotate / depondt_step                            -> Rodrigues closed form (rotation about an axis),
@@ -223,6 +228,7 @@ class SpinSystem:
         self.A = A
         self.pinned = None
         self.site_T = None
+        self.terms = []
         self.B = np.asarray(field, float).reshape(3)
 
     def set_field(self, B):
@@ -238,6 +244,8 @@ class SpinSystem:
     def gradient(self, e):                                   # spinmodel.py:458-469
         g = -2.0 * self.exchange_field(e) - MU_B * self.mu[:, None] * self.B[None, :]
         g += 2.0 * np.einsum("iab,ib->ia", self.A, e)
+        for t in self.terms:
+            g += t.gradient(e)
         if self.pinned is not None:
             g[self.pinned] = 0.0
         return g
@@ -248,6 +256,8 @@ class SpinSystem:
     def energy(self, e, per_spin=False):                     # spinmodel.py:475-484
         E = -np.sum(e * self.exchange_field(e)) - MU_B * np.sum(self.mu * (e @ self.B))
         E += np.einsum("ia,iab,ib->", e, self.A, e)
+        for t in self.terms:
+            E += t.energy(e)
         return E / self.N if per_spin else E
 
     def init_random(self, seed):                             # spinmodel.py:495-496, 512
@@ -255,6 +265,42 @@ class SpinSystem:
 
     def order_parameter(self, e):                            # spinmodel.py:517-519
         return float(np.linalg.norm((self.signs[:, None] * e).sum(0)) / self.N)
+
+
+class CubicAnisotropy:
+    """E_i = Kc_i (a1^2 a2^2 + a2^2 a3^2 + a3^2 a1^2), a = C e_i (terms.py:35-55)."""
+
+    def __init__(self, Kc, axes=None):
+        self.Kc = np.asarray(Kc, float)
+        self.C = np.eye(3) if axes is None else np.asarray(axes, float)
+
+    def energy(self, e):
+        q = (e @ self.C.T) ** 2
+        return float(np.sum(self.Kc * (q[:, 0] * q[:, 1] + q[:, 1] * q[:, 2] + q[:, 2] * q[:, 0])))
+
+    def gradient(self, e):
+        a = e @ self.C.T
+        q = a**2
+        da = 2 * a * np.stack([q[:, 1] + q[:, 2], q[:, 0] + q[:, 2], q[:, 0] + q[:, 1]], 1)
+        return self.Kc[:, None] * (da @ self.C)
+
+
+class UniaxialHigher:
+    """E_i = -K2_i (e_i.u)^4 - K3_i (e_i.u)^6 (terms.py:58-75)."""
+
+    def __init__(self, K2, K3=0.0, axis=(0, 0, 1)):
+        self.K2 = np.asarray(K2, float)
+        self.K3 = np.asarray(K3, float) * np.ones_like(self.K2)
+        u = np.asarray(axis, float)
+        self.u = u / np.linalg.norm(u)
+
+    def energy(self, e):
+        c = e @ self.u
+        return float(np.sum(-self.K2 * c**4 - self.K3 * c**6))
+
+    def gradient(self, e):
+        c = e @ self.u
+        return (-4 * self.K2 * c**3 - 6 * self.K3 * c**5)[:, None] * self.u[None, :]
 
 
 def normalize(e):                                            # spinmodel.py:553-554
@@ -277,12 +323,30 @@ def sigma(system, alpha, T, dt):
     return np.broadcast_to(s, (system.N,)).astype(float)
 
 
-def heun_step(system, e, alpha, dt, b=0.0, precession=True):
-    """One stochastic Heun step with thermal field b held over both stages (llg.py:216-227)."""
+def solver_field(system, sot=None, field_func=None):          # llg.py:124-136
+    """LLGSolver.field: B_eff + spin-orbit torque fields + field_func(t), zero on pinned sites when any
+    of the extras is present.  sot = (p unit vector, B_DL, B_FL)."""
+    def field(e, t):
+        B = system.effective_field(e)
+        if sot is not None:
+            p, bdl, bfl = sot
+            B = B + bdl * np.cross(e, p[None, :]) + bfl * p[None, :]
+        if field_func is not None:
+            B = B + field_func(t)
+        if system.pinned is not None and (sot is not None or field_func is not None):
+            B[system.pinned] = 0.0
+        return B
+    return field
+
+
+def heun_step(system, e, alpha, dt, b=0.0, precession=True, field=None, t=0.0):
+    """One stochastic Heun step with thermal field b held over both stages (llg.py:216-227); field(e, t)
+    defaults to the system's B_eff (time t in the predictor, t + dt in the corrector)."""
+    field = field or (lambda x, _t: system.effective_field(x))
     a = alpha[:, None] if np.ndim(alpha) else alpha
-    k1 = llg_rhs(e, system.effective_field(e) + b, a, precession)
+    k1 = llg_rhs(e, field(e, t) + b, a, precession)
     ep = normalize(e + dt * k1)
-    k2 = llg_rhs(ep, system.effective_field(ep) + b, a, precession)
+    k2 = llg_rhs(ep, field(ep, t + dt) + b, a, precession)
     return normalize(e + 0.5 * dt * (k1 + k2))
 
 
@@ -294,19 +358,23 @@ def rotate(e, w):                                            # llg.py:31-36
     return e * c + np.cross(k, e) * s_ + k * np.einsum("ia,ia->i", k, e)[:, None] * (1 - c)
 
 
-def depondt_step(system, e, alpha, dt, b=0.0, precession=True):
+def depondt_step(system, e, alpha, dt, b=0.0, precession=True, field=None, t=0.0):
     """One Depondt step (Heun on rotation vectors), thermal field b held over both stages
     (llg.py:236-240; fast.py:284-306)."""
+    field = field or (lambda x, _t: system.effective_field(x))
     a = alpha[:, None] if np.ndim(alpha) else alpha
-    w1 = np.cross(e, llg_rhs(e, system.effective_field(e) + b, a, precession))
+    w1 = np.cross(e, llg_rhs(e, field(e, t) + b, a, precession))
     ep = rotate(e, dt * w1)
-    w2 = np.cross(ep, llg_rhs(ep, system.effective_field(ep) + b, a, precession))
+    w2 = np.cross(ep, llg_rhs(ep, field(ep, t + dt) + b, a, precession))
     return normalize(rotate(e, 0.5 * dt * (w1 + w2)))
 
 
-def run_heun(system, e, steps, alpha, dt, T=0.0, normals=None, precession=True, integrator="heun"):
-    """steps Heun (or Depondt) steps; normals(n) -> (N,3) standard normals of step n (injected noise)."""
+def run_heun(system, e, steps, alpha, dt, T=0.0, normals=None, precession=True, integrator="heun", sot=None,
+             field_func=None):
+    """steps Heun (or Depondt) steps from t = 0; normals(n) -> (N,3) standard normals of step n
+    (injected noise); sot / field_func as solver_field."""
     step = {"heun": heun_step, "depondt": depondt_step}[integrator]
+    field = solver_field(system, sot, field_func) if (sot is not None or field_func is not None) else None
     sg = sigma(system, alpha, T, dt) if (T > 0 or system.site_T is not None) else None
     for n in range(steps):
         b = 0.0
@@ -314,7 +382,7 @@ def run_heun(system, e, steps, alpha, dt, T=0.0, normals=None, precession=True, 
             b = sg[:, None] * normals(n)
             if system.pinned is not None:
                 b[system.pinned] = 0.0
-        e = step(system, e, alpha, dt, b, precession)
+        e = step(system, e, alpha, dt, b, precession, field, n * dt)
     return e
 
 

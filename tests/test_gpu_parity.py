@@ -343,3 +343,52 @@ def test_device_class_matches_solver(gpu, integrator):
     assert [n for n, _ in seen] == [8, 16]
     e8, _, _ = LLGSolver(s, alpha=0.2, dt=1e-3, seed=1, integrator=integrator).run(e0, 8, output_every=8)
     assert np.array_equal(seen[0][1], e8)
+
+
+def test_cubic_and_higher_uniaxial_terms_match_oracle(gpu):
+    """CubicAnisotropy (rotated axes, per-site Kc) and UniaxialHigher (per-site K2, K3, tilted axis)
+    folded into the kernels: gradient, energy and 60 thermal Heun steps against the oracle's terms."""
+    print("case: cubic + higher uniaxial", flush=True)
+    from scipy.spatial.transform import Rotation
+    from magex_amd.terms import CubicAnisotropy, UniaxialHigher
+    s, o = _pair((5, 4, 6), (0.1, 0.0, 0.4), lattice="sc", J=[1.0, 0.2], K=0.05, moment=2.0)
+    rng = np.random.default_rng(8)
+    C = Rotation.from_rotvec([0.2, 0.4, -0.1]).as_matrix()
+    Kc, K2, K3 = rng.uniform(0.05, 0.3, s.N), rng.uniform(0.05, 0.2, s.N), rng.uniform(0.0, 0.1, s.N)
+    s.terms += [CubicAnisotropy(Kc, C), UniaxialHigher(K2, K3, axis=(0.3, 0, 1))]
+    o.terms += [R.CubicAnisotropy(Kc, C), R.UniaxialHigher(K2, K3, axis=(0.3, 0, 1))]
+    e0 = s.init_spins("random", seed=3)
+    assert _rel(s.gradient(e0), o.gradient(e0)) < 1e-13
+    assert abs(s.energy(e0) - o.energy(e0)) < 1e-12 * abs(o.energy(e0))
+    sol = LLGSolver(s, alpha=0.1, dt=2e-4, temperature=5.0, seed=21)
+    e, _, _ = sol.run(e0, 60, output_every=60)
+    dm = s.device_model("fp64")
+    normals = [dm.noise(21, n).cpu().numpy() for n in range(60)]
+    ref = R.run_heun(o, e0.copy(), 60, 0.1, 2e-4, T=5.0, normals=lambda n: normals[n])
+    assert _rel(e, ref) < 1e-11
+
+
+@pytest.mark.parametrize("integrator", ["heun", "depondt"])
+def test_sot_and_field_func_match_oracle(gpu, integrator):
+    """Spin-orbit torque fields and a uniform time-dependent field_func(t) (evaluated at t and t + dt),
+    with pinned sites and thermal noise, against the oracle's LLGSolver.field restatement."""
+    print(f"case: sot + field_func {integrator}", flush=True)
+    s, o = _pair((10, 8, 1), (0, 0, 0.3), lattice="square", J=[2.0, 0.3], D=0.8, K=0.1)
+    pin = np.zeros(s.N, bool)
+    pin[::7] = True
+    s.pinned = pin
+    o.pinned = pin
+    ff = lambda t: np.array([0.4 * np.cos(40.0 * t), 0.4 * np.sin(40.0 * t), 0.0])      # noqa: E731
+    sot = {"p": [0.0, 1.0, 1.0], "B_DL": 0.6, "B_FL": -0.2}
+    sol = LLGSolver(s, alpha=0.1, dt=5e-4, temperature=8.0, seed=13, integrator=integrator, sot=sot)
+    sol.field_func = ff
+    e0 = s.init_spins("random", seed=6)
+    e, _, _ = sol.run(e0, 50, output_every=50)
+    dm = s.device_model("fp64")
+    normals = [dm.noise(13, n).cpu().numpy() for n in range(50)]
+    p = np.array(sot["p"]) / np.linalg.norm(sot["p"])
+    ref = R.run_heun(o, e0.copy(), 50, 0.1, 5e-4, T=8.0, normals=lambda n: normals[n], integrator=integrator,
+                     sot=(p, sot["B_DL"], sot["B_FL"]), field_func=ff)
+    assert np.array_equal(e[pin], e0[pin])
+    assert _rel(e, ref) < 1e-11
+    assert abs(sol.time - 50 * 5e-4) < 1e-15

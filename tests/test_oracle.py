@@ -326,3 +326,71 @@ def test_depondt_is_second_order():
     h = R.run_heun(s, e0.copy(), 50, 0.1, 2e-3)
     d = R.run_heun(s, e0.copy(), 50, 0.1, 2e-3, integrator="depondt")
     assert np.abs(h - d).max() < 1e-4
+
+
+# ------------------------------------------------------------------ on-site terms
+def test_onsite_terms_closed_forms_and_gradients():
+    rng = np.random.default_rng(3)
+    N = 7
+    Kc, K2, K3 = rng.uniform(0.1, 1, N), rng.uniform(0.1, 1, N), rng.uniform(0.1, 1, N)
+    from scipy.spatial.transform import Rotation
+    C = Rotation.from_rotvec([0.3, -0.2, 0.5]).as_matrix()
+    cub, uni = R.CubicAnisotropy(Kc, C), R.UniaxialHigher(K2, K3, axis=(1, 1, 0))
+    # symmetry directions: cube axis -> 0; body diagonal -> Kc / 3; along u -> -K2 - K3; normal to u -> 0
+    assert abs(cub.energy(np.tile(C[1], (N, 1)))) < 1e-15
+    diag = (C.T @ np.ones(3)) / np.sqrt(3)
+    assert abs(cub.energy(np.tile(diag, (N, 1))) - Kc.sum() / 3) < 1e-14
+    u = np.array([1, 1, 0]) / np.sqrt(2)
+    assert abs(uni.energy(np.tile(u, (N, 1))) + (K2 + K3).sum()) < 1e-14
+    assert abs(uni.energy(np.tile([0, 0, 1.0], (N, 1)))) < 1e-15
+    # gradients by central differences
+    e = R.normalize(rng.normal(size=(N, 3)))
+    for t in (cub, uni):
+        g = t.gradient(e)
+        h = 1e-6
+        for i, a in ((0, 0), (3, 1), (6, 2)):
+            d = np.zeros_like(e)
+            d[i, a] = h
+            fd = (t.energy(e + d) - t.energy(e - d)) / (2 * h)
+            assert abs(fd - g[i, a]) < 1e-8 * max(1.0, abs(fd))
+
+
+# ------------------------------------------------------------------ solver fields: field_func, SOT
+def _single_spin(field=(0, 0, 0)):
+    return _sys((1, 1, 1), field=field, pbc=(False, False, False), lattice="sc", J=[1.0], moment=2.0)
+
+
+def test_field_func_precession_phase():
+    """alpha = 0, B(t) = (0, 0, 1 + 0.5 sin(2 pi t / 0.5)) T: the spin keeps its polar angle and turns
+    right-handed about z by phi(t) = gamma int B_z dt (de/dt = gamma B x e).  Evaluating field_func at t in the corrector too (instead of t + dt)
+    is a first-order error that the same check catches."""
+    s = _single_spin()
+    w = 2 * np.pi / 0.5
+    ff = lambda t: np.array([0.0, 0.0, 1.0 + 0.5 * np.sin(w * t)])          # noqa: E731
+    th0 = 0.7
+    e0 = np.array([[np.sin(th0), 0.0, np.cos(th0)]])
+    dt, n = 1e-4, 4000
+    e = R.run_heun(s, e0.copy(), n, 0.0, dt, field_func=ff)
+    t = n * dt
+    phi = R.GAMMA * (t + 0.5 * (1 - np.cos(w * t)) / w)
+    ref = np.array([[np.sin(th0) * np.cos(phi), np.sin(th0) * np.sin(phi), np.cos(th0)]])
+    assert np.abs(e - ref).max() < 1e-7
+    lagged = e0.copy()
+    for k in range(n):
+        lagged = R.heun_step(s, lagged, 0.0, dt, field=lambda x, _t: R.solver_field(s, None, ff)(x, k * dt))
+    assert np.abs(lagged - ref).max() > 1e-6
+
+
+def test_sot_damping_like_closed_form():
+    """B = B_DL (e x p), alpha = 0: theta(t) to p follows tan(theta/2) = tan(theta0/2) exp(-gamma B_DL t);
+    the field-like part B_FL p is a plain field along p (precession about p at fixed theta)."""
+    s = _single_spin()
+    p = np.array([0.0, 1.0, 0.0])
+    th0, bdl = 1.2, 0.8
+    e0 = np.array([[np.sin(th0), np.cos(th0), 0.0]])
+    dt, n = 1e-4, 5000
+    e = R.run_heun(s, e0.copy(), n, 0.0, dt, sot=(p, bdl, 0.0))
+    th = 2 * np.arctan(np.tan(th0 / 2) * np.exp(-R.GAMMA * bdl * n * dt))
+    assert abs(np.arccos(e[0] @ p) - th) < 1e-7
+    e = R.run_heun(s, e0.copy(), n, 0.0, dt, sot=(p, 0.0, 2.0))
+    assert abs(e[0] @ p - np.cos(th0)) < 1e-10
