@@ -81,10 +81,12 @@ __device__ __forceinline__ int neighbour_cell(const Static<R>* __restrict__ S, c
 }
 
 // Entries per batch of the stencil loop: every load of a batch is issued before the first is used, so
-// a wavefront keeps CH gathers (and, per bond, CH tensors) in flight instead of one.
+// a wavefront keeps CH gathers (and, per bond, CH tensors) in flight instead of one.  Per-bond batches
+// were timed on MI355X with the 14-entry bcc stencil: FP32 7 (two full batches; 5, 6, 8 and 9 were
+// 1-15 % slower), FP64 2 (3 and 5 slower: more tensors in flight evict their partners from L2).
 template <typename R, int CM>
 struct Chunk {
-  static constexpr int value = CM == COUP_BOND ? (sizeof(R) == 8 ? 3 : 5) : 8;
+  static constexpr int value = CM == COUP_BOND ? (sizeof(R) == 8 ? 2 : 7) : 8;
 };
 
 // One batch of CH stencil entries: gather() issues every load of the batch without a branch (entries
@@ -201,8 +203,24 @@ __device__ __forceinline__ void onsite_grad(const Static<R>* __restrict__ S, con
   } else if (amode == ONSITE_SITE6) {
     const int N = S->N, i = st.i;
     const R* A = d.A;
-    const R axx = A[i], ayy = A[N + i], azz = A[2 * N + i];
-    const R axy = A[3 * N + i], axz = A[4 * N + i], ayz = A[5 * N + i];
+    R axx, ayy, azz, axy, axz, ayz;
+    if constexpr (sizeof(R) == 8) {
+      // FP64: the per-site tensor is read once per pass; streaming it past L2 leaves room for the
+      // partner bond tensors (measured 4-6 % faster on bcc64; slower in FP32, where it stays cached)
+      axx = __builtin_nontemporal_load(A + i);
+      ayy = __builtin_nontemporal_load(A + N + i);
+      azz = __builtin_nontemporal_load(A + 2 * N + i);
+      axy = __builtin_nontemporal_load(A + 3 * N + i);
+      axz = __builtin_nontemporal_load(A + 4 * N + i);
+      ayz = __builtin_nontemporal_load(A + 5 * N + i);
+    } else {
+      axx = A[i];
+      ayy = A[N + i];
+      azz = A[2 * N + i];
+      axy = A[3 * N + i];
+      axz = A[4 * N + i];
+      ayz = A[5 * N + i];
+    }
     gx = R(2) * (axx * x + axy * y + axz * z);
     gy = R(2) * (axy * x + ayy * y + ayz * z);
     gz = R(2) * (axz * x + ayz * y + azz * z);
