@@ -81,6 +81,27 @@ def gpu_rate(system, p, precision, steps, warmup):
     return dict(wall_s=wall, event_ms=ev_ms, step_ms=ev_ms / steps)
 
 
+def fp32_angle_error(system, p, steps=200):
+    """Mean and max per-spin angle (rad) between the FP32 and FP64 trajectories of the workload after
+    `steps` steps from the same start on the same thermal noise (the FP64 kernels match the oracle to
+    1e-10, see tests/test_gpu_parity.py)."""
+    import numpy as np
+    import torch
+    from magex_amd.llg import LLGSolver
+    e0 = system.init_spins("random", seed=0)
+    out = {}
+    for prec, dt in (("fp64", torch.float64), ("fp32", torch.float32)):
+        sol = LLGSolver(system, p["alpha"], p["dt"], p["temperature"], p["seed"], precision=prec)
+        e = torch.as_tensor(e0, device="cuda", dtype=dt).contiguous()
+        sol.step_device(e, steps)
+        out[prec] = e.double()
+    a, b = out["fp64"], out["fp32"]
+    b = b / b.norm(dim=1, keepdim=True)
+    ang = torch.atan2(torch.linalg.cross(a, b).norm(dim=1), (a * b).sum(1))
+    return {"steps": steps, "mean_rad": float(ang.mean()), "max_rad": float(ang.max()),
+            "note": "FP32 vs FP64 device trajectories, same start and noise"}
+
+
 def cpu_baseline(system, p, seconds):
     """magex's fused Heun restated in C++/OpenMP, FP64, timed on a bounded sample of the workload."""
     import numpy as np
@@ -245,6 +266,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "Heun step = k_heun pass 1 + pass 2 (event time per step)"},
     }
+    if a.precision == "fp32" and not a.no_fp64:
+        out["fp32_angle_error"] = fp32_angle_error(system, p)
     if r2 is not None:
         out[other] = {"spin_steps_per_s": N / (r2["step_ms"] * 1e-3), "ms_per_step": r2["step_ms"],
                       "roofline_frac": alg_bytes(system, other) * N / (r2["step_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}

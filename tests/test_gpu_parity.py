@@ -392,3 +392,40 @@ def test_sot_and_field_func_match_oracle(gpu, integrator):
     assert np.array_equal(e[pin], e0[pin])
     assert _rel(e, ref) < 1e-11
     assert abs(sol.time - 50 * 5e-4) < 1e-15
+
+
+def test_largest_cell_count_and_self_images(gpu):
+    """Sizes at the edges of the index arithmetic and of the stencil:
+    - 256^3 = 2^24 cells (the largest supercell class, where cell indices leave the float-division
+      path): a spin spiral of period 8 gives, site for site, the field and the 5-step T=0 trajectory
+      of the 8^3 supercell (translation invariance);
+    - a supercell one cell thick on a periodic axis (every bond along it a self-image, folded into the
+      on-site tensor) and two cells thick (both directions reach the same neighbour) against the oracle."""
+    print("case: 256^3 cells and self images", flush=True)
+    import torch
+    from magex_amd.device import DeviceModel
+    uc = S.load_results(M.make_model(lattice="sc", J=[1.0, 0.1], K=0.05, D=0.2, moment=2.0))
+    small = S.SpinSystem(uc, (8, 8, 8), field=(0, 0, 0.4))
+    big = S.SpinSystem(uc, (256, 256, 256), field=(0, 0, 0.4))
+    c = small.cells.astype(float)
+    q = 2 * np.pi / 8 * np.array([1.0, 2.0, 3.0])
+    ph = c @ q
+    es = np.stack([np.cos(ph) * 0.8, np.sin(ph) * 0.8, np.full_like(ph, 0.6)], 1)
+    a = DeviceModel(small, "fp32")
+    b = DeviceModel(big, "fp32")
+    idx = torch.as_tensor(((big.cells % 8) @ np.array([64, 8, 1])).astype(np.int64), device="cuda")
+    xs = torch.as_tensor(es, device="cuda", dtype=torch.float32)
+    xb = xs[idx].contiguous()
+    assert torch.equal(b.field(xb), a.field(xs)[idx])
+    for m in (a, b):
+        m.set_dynamics(2e-4, 0.1, 0.0)
+    a.run(xs, 5, seed=1)
+    b.run(xb, 5, seed=1)
+    assert torch.equal(xb, xs[idx])
+    del xb, b
+    for L in ((1, 3, 4), (2, 3, 4)):
+        s, o = _pair(L, (0.1, 0, 0.3), lattice="bcc_conventional", J=[20.0, 10.0], moment=2.2)
+        e0 = s.init_spins("random", seed=5)
+        assert _rel(s.effective_field(e0), o.effective_field(e0)) < 1e-13
+        e, _, _ = LLGSolver(s, alpha=0.1, dt=1e-4).run(e0, 20, output_every=20)
+        assert _rel(e, R.run_heun(o, e0.copy(), 20, 0.1, 1e-4)) < 1e-12
