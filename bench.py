@@ -26,6 +26,9 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the CPU baseline runs one OpenMP thread per core, pinned; set before any OpenMP runtime starts
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 METRIC = "spin-steps/sec + % HBM roofline (FP32 vs FP64); batched speedup; grad cost/forward"
 HBM_PEAK_GBS = 8000.0
@@ -102,6 +105,106 @@ def fp32_angle_error(system, p, steps=200):
             "note": "FP32 vs FP64 device trajectories, same start and noise"}
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _best_of_3(h, e, n):
+    best = float("inf")
+    for _ in range(3):
+        t = time.perf_counter()
+        h.run(e, n)
+        best = min(best, (time.perf_counter() - t) / n)
+    return best
+
+
+def cpu_calibration():
+    """The CPU restatement on magex's published set-up (benchmarks/speed_llg.py, sc.json): sc 32^3,
+    J = 1 meV, mu = 2, alpha = 1, dt = 1e-4 ps, T = 10 K, ferro start; best of 3 x 50 steps after one
+    warm-up step, 1 and 2 threads.  Published: 55-64 and 29 ns/spin/step (docs/benchmarks.md:11-13,
+    which include magex's per-step Python overhead)."""
+    import numpy as np
+    from oracle import cpu
+    from oracle import magex_ref as R
+    o = R.SpinSystem(R.load_results(R.make_model("sc", a=3.0, J=[1.0], moment=2.0)), (32, 32, 32))
+    out = {"published_ns_per_spin_step": {"1": "55-64", "2": "29"}}
+    for th in (1, 2):
+        cpu.set_threads(th)
+        h = cpu.CpuHeun(o, 1.0, 1e-4, 10.0, seed=1)
+        e = np.tile([0.0, 0.0, 1.0], (o.N, 1))
+        h.run(e, 1)
+        out[f"{th}_thread_ns_per_spin_step"] = _best_of_3(h, e, 50) / o.N * 1e9
+    cpu.set_threads(cpu.max_threads_available())
+    return out
+
+
+def cpu_rate(system, p, seconds, threads=None, per_bond=False):
+    """spin-steps/s of the CPU restatement (FP64) on a bounded sample of the workload."""
+    import numpy as np
+    from oracle import cpu
+    from oracle import magex_ref as R
+    if threads:
+        cpu.set_threads(threads)
+    if per_bond:
+        src, dst, J = [], [], []
+        for k in range(len(system.stencil)):
+            a, b, blk = system._entry_blocks(k)
+            ok = b >= 0
+            src.append(a[ok])
+            dst.append(b[ok])
+            J.append(np.asarray(blk)[ok])
+        blocks = cpu.per_bond_blocks(system.N, np.concatenate(src), np.concatenate(dst), np.concatenate(J))
+    else:
+        blocks = R.neighbour_blocks(system.Jsp, system.N)
+    h = cpu.CpuHeun(system, p["alpha"], p["dt"], p["temperature"], seed=1, blocks=blocks)
+    e = system.init_spins("random", seed=0)
+    h.run(e, 1)
+    t = time.perf_counter()
+    h.run(e, 1)
+    one = time.perf_counter() - t
+    n = max(1, int(seconds / 3 / max(one, 1e-9)))
+    best = _best_of_3(h, e, n)
+    th = cpu.max_threads()
+    cpu.set_threads(cpu.max_threads_available())
+    return system.N / best, n, th
+
+
+def config_rates():
+    """The other benchmark configurations, GPU beside CPU: sc 16^3 (FP64, T = 0) and tri 256^2 (FP32 and
+    FP64 on identical noise, with the topological charge after 2000 steps)."""
+    import torch
+    from magex_amd import configs, texture
+    from magex_amd.llg import LLGSolver
+    out = {}
+    s, p = configs.sc16()
+    r = gpu_rate(s, p, "fp64", 2000, 200)
+    c, n, th = cpu_rate(s, p, 2.0)
+    out["sc16_fp64"] = {"gpu_spin_steps_per_s": s.N / (r["step_ms"] * 1e-3), "gpu_us_per_step": 1e3 * r["step_ms"],
+                        "cpu_spin_steps_per_s": c, "cpu_threads": th, "alg_bytes_per_spin_step": alg_bytes(s, "fp64")}
+    s, p = configs.tri256()
+    c, n, th = cpu_rate(s, p, 3.0)
+    q = {}
+    for prec in ("fp32", "fp64"):
+        r = gpu_rate(s, p, prec, 2000, 200)
+        sol = LLGSolver(s, p["alpha"], p["dt"], p["temperature"], p["seed"], precision=prec)
+        e = torch.as_tensor(s.init_spins("random", seed=0), device="cuda",
+                            dtype=torch.float32 if prec == "fp32" else torch.float64)
+        sol.step_device(e, 2000)
+        q[prec] = float(texture.topological_charge(s, e))
+        out[f"tri256_{prec}"] = {"gpu_spin_steps_per_s": s.N / (r["step_ms"] * 1e-3),
+                                 "gpu_us_per_step": 1e3 * r["step_ms"], "Q_after_2000_steps": q[prec],
+                                 "roofline_frac": alg_bytes(s, prec) * s.N / (r["step_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    out["tri256_cpu_spin_steps_per_s"] = c
+    out["tri256_cpu_threads"] = th
+    return out
+
+
 def cpu_baseline(system, p, seconds):
     """magex's fused Heun restated in C++/OpenMP, FP64, timed on a bounded sample of the workload."""
     import numpy as np
@@ -127,9 +230,14 @@ def cpu_baseline(system, p, seconds):
         h.run(e, n)
         best = min(best, (time.perf_counter() - t) / n)
     threads = cpu.max_threads()
+    one, n1, _ = cpu_rate(system, p, min(3.0, seconds / 4), threads=1, per_bond=True)
     return dict(value=system.N / best, unit="spin-steps/s", cores=threads, kind="port",
                 sample=f"best of 3 x {n} Heun steps of the full {system.N}-spin lattice, FP64, {threads} OpenMP "
-                       f"threads (magex's fused two-pass Heun, per-bond 3x3 blocks, ziggurat noise)")
+                       f"threads (magex's fused two-pass Heun, per-bond 3x3 blocks, ziggurat noise)",
+                cpu_model=cpu_model(), ns_per_spin_step=1e9 * best / system.N,
+                one_thread={"spin_steps_per_s": one, "ns_per_spin_step": 1e9 / one, "steps": n1},
+                omp={k: os.environ.get(k) for k in ("OMP_PROC_BIND", "OMP_PLACES", "OMP_NUM_THREADS")},
+                calibration=cpu_calibration())
 
 
 def batch_rates(steps, per_replica_steps):
@@ -231,6 +339,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=300)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the sc16 / tri256 GPU-beside-CPU table")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 256 MiB device copy first (the reference kernel for rocprof FETCH/WRITE_SIZE)")
     a = ap.parse_args()
@@ -277,6 +386,9 @@ def main():
     if not a.no_adjoint:
         print("adjoint ...", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_rates()
+    if not a.no_configs:
+        print("other configurations ...", file=sys.stderr, flush=True)
+        out["configs"] = config_rates()
     if not a.no_cpu_baseline:
         print("cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(system, p, a.cpu_seconds)

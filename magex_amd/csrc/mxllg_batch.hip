@@ -155,7 +155,8 @@ template <typename R, int CM, int BS, int PASS>
 __global__ __launch_bounds__(BS) void k_bheun(const BStatic<R>* __restrict__ S, const Rep* __restrict__ reps,
                                               const int32_t* __restrict__ tile_rep, const BDyn<R> d,
                                               R* __restrict__ e, R* __restrict__ ep, R* __restrict__ k1,
-                                              uint64_t step, int spt, const uint64_t* __restrict__ step_ptr) {
+                                              R* __restrict__ bn, uint64_t step, int spt,
+                                              const uint64_t* __restrict__ step_ptr) {
   const int b = xcd_tile(blockIdx.x, gridDim.x);
   const Rep* __restrict__ rp = reps + tile_rep[b];  // descriptor read through the pointer: scalar loads
   if (step_ptr) step += *step_ptr;
@@ -168,14 +169,25 @@ __global__ __launch_bounds__(BS) void k_bheun(const BStatic<R>* __restrict__ S, 
     const R* src = PASS == 1 ? e : ep;
     const R x = src[3 * i], y = src[3 * i + 1], z = src[3 * i + 2];
     R hx, hy, hz, gx, gy, gz, nx = R(0), ny = R(0), nz = R(0);
+    // the thermal field is drawn in pass 1 (while the first neighbour batch is in flight) and kept for
+    // pass 2: the batched kernels are bound by arithmetic more than by bytes, and the generator is the
+    // largest part of it
+    if (PASS == 2 && d.noisy) {
+      nx = bn[3 * i];
+      ny = bn[3 * i + 1];
+      nz = bn[3 * i + 2];
+    }
     bexchange<R, CM>(S, rp, src, st, hx, hy, hz, [&] {
-      if (d.noisy) {
+      if (PASS == 1 && d.noisy) {
         float n0, n1, n2;
         gauss3(rp->seed_lo, rp->seed_hi, step, uint32_t(st.iref), n0, n1, n2);
         const R sg = R(rp->sig[st.s]);
         nx = sg * R(n0);
         ny = sg * R(n1);
         nz = sg * R(n2);
+        bn[3 * i] = nx;
+        bn[3 * i + 1] = ny;
+        bn[3 * i + 2] = nz;
       }
     });
     bonsite(S, st.s, x, y, z, gx, gy, gz);
@@ -283,6 +295,7 @@ struct mx_batch {
   int64_t total = 0;
   int ntiles = 0;
   void *static_d = nullptr, *reps_d = nullptr, *ep_d = nullptr, *k1_d = nullptr;
+  void* bn_d = nullptr;  // thermal field of the current step, drawn in pass 1 and reused in pass 2
   int32_t* tile_rep_d = nullptr;
   double* part_d = nullptr;
   size_t part_cap = 0;
@@ -382,10 +395,11 @@ void launch_pair(mx_batch& b, R* e, uint64_t step, const uint64_t* step_ptr, hip
   const Rep* reps = static_cast<const Rep*>(b.reps_d);
   R* ep = static_cast<R*>(b.ep_d);
   R* k1 = static_cast<R*>(b.k1_d);
+  R* bn = static_cast<R*>(b.bn_d);
   hipLaunchKernelGGL((k_bheun<R, CM, BS, 1>), dim3(b.ntiles), dim3(BS), 0, s, S, reps, b.tile_rep_d, d, e, ep, k1,
-                     step, b.spt, step_ptr);
+                     bn, step, b.spt, step_ptr);
   hipLaunchKernelGGL((k_bheun<R, CM, BS, 2>), dim3(b.ntiles), dim3(BS), 0, s, S, reps, b.tile_rep_d, d, e, ep, k1,
-                     step, b.spt, step_ptr);
+                     bn, step, b.spt, step_ptr);
 }
 
 template <typename R, int CM>
@@ -505,6 +519,7 @@ int mx_batch_create(const mx_model* proto, int32_t nrep, const int32_t* L, mx_ba
       bok(hipMemcpy(b->static_d, b->static_h.data(), b->static_h.size(), hipMemcpyHostToDevice), "hipMemcpy constants");
       bok(hipMalloc(&b->ep_d, size_t(b->total) * 3 * rsz(*b)), "hipMalloc ep");
       bok(hipMalloc(&b->k1_d, size_t(b->total) * 3 * rsz(*b)), "hipMalloc k1");
+      bok(hipMalloc(&b->bn_d, size_t(b->total) * 3 * rsz(*b)), "hipMalloc thermal field");
       layout(*b);
       upload_reps(*b);
     } catch (...) {
@@ -520,7 +535,7 @@ void mx_batch_destroy(mx_batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->device);
   drop_bgraph(*b);
-  for (void** p : {&b->static_d, &b->reps_d, &b->ep_d, &b->k1_d, reinterpret_cast<void**>(&b->tile_rep_d),
+  for (void** p : {&b->static_d, &b->reps_d, &b->ep_d, &b->k1_d, &b->bn_d, reinterpret_cast<void**>(&b->tile_rep_d),
                    reinterpret_cast<void**>(&b->part_d), reinterpret_cast<void**>(&b->counter_d)})
     bfree(*p);
   if (b->cap) (void)hipStreamDestroy(b->cap);

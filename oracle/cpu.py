@@ -97,11 +97,23 @@ class CpuHeun:
 
 
 def set_threads(n):
+    max_threads_available()
     lib().cpu_set_threads(int(n))
 
 
 def max_threads():
     return lib().cpu_max_threads()
+
+
+_AVAILABLE = None
+
+
+def max_threads_available():
+    """The thread count OpenMP started with (restored after single-thread runs)."""
+    global _AVAILABLE
+    if _AVAILABLE is None:
+        _AVAILABLE = max_threads()
+    return _AVAILABLE
 
 
 def per_bond_blocks(N, src, dst, blocks):
