@@ -120,6 +120,23 @@ int mx_llg_run_driven(mx_model* m, void* e, int64_t nsteps, const double* B, uin
  * switches it off. */
 int mx_model_set_sot(mx_model* m, const double p[3], double B_DL, double B_FL);
 
+/* ---------------------------------------------------------------------------------------------
+ * Metropolis Monte Carlo on the same stencil gather: the colour sweep of magex's numba path
+ * (magex/fast.py:393-503 `_mc_group`, driven by magex/montecarlo.py:152-179).  A colour is a set of
+ * mutually uncoupled sites; colours move one after another, the sites of a colour all at once.
+ * off: ncolor + 1 offsets into sites (magex site order; pinned sites left out). */
+int mx_mc_set_colors(mx_model* m, int32_t ncolor, const int64_t* off, const int32_t* sites);
+/* Temperature of the sampler (K): per_site = 0: one value, 1: N values; beta = 1 / (k_B T), infinite
+ * at T = 0 (magex/montecarlo.py:156-158). */
+int mx_mc_set_temperature(mx_model* m, int per_site, const double* T);
+/* nsweeps sweeps of e (device, in place): per sweep every colour with move 0 (gaussian cone of width
+ * sigma), 1 (uniform) or 2 (heat bath; gaussian where some beta is infinite), then overrelax passes of
+ * over-relaxation.  Random numbers: counter-based, keyed by seed, sweep0 + k, site. */
+int mx_mc_sweep(mx_model* m, void* e, int64_t nsweeps, int move, int overrelax, double sigma, uint64_t seed,
+                uint64_t sweep0, void* stream);
+/* Accepted moves (not counting over-relaxation) since the last reset; synchronises. */
+int mx_mc_accepted(mx_model* m, int64_t* out, int reset);
+
 /* Steps per captured step loop (hipGraph replayed by mx_llg_run when no noise is injected and at least
  * two loops fit; default 0 = always launch step by step).  Results are identical either way: the
  * captured kernels read the step number from a device counter, so the noise sequence is the same.

@@ -44,6 +44,10 @@ SIGNATURES = {
     "mx_model_set_graph_steps": (_I, [_P, ctypes.c_int32]),
     "mx_llg_run_driven": (_I, [_P, _P, _I64, _P, _U64, _U64, _P]),
     "mx_model_set_sot": (_I, [_P, _P, _D, _D]),
+    "mx_mc_set_colors": (_I, [_P, ctypes.c_int32, _P, _P]),
+    "mx_mc_set_temperature": (_I, [_P, _I, _P]),
+    "mx_mc_sweep": (_I, [_P, _P, _I64, _I, _I, _D, _U64, _U64, _P]),
+    "mx_mc_accepted": (_I, [_P, _P, _I]),
     "mx_llg_run": (_I, [_P, _P, _I64, _U64, _U64, _P, _P]),
     "mx_noise": (_I, [_P, _P, _U64, _U64, _P]),
     "mx_field": (_I, [_P, _P, _P, _I, _P]),

@@ -265,7 +265,8 @@ void mx_model_destroy(mx_model* m) {
   if (!m) return;
   (void)hipSetDevice(m->device);
   for (void** p : {&m->bond_d, &m->A_d, &m->Kc_d, &m->K2_d, &m->K3_d, &m->inv_mu_d, &m->sign_d, &m->pinned_d,
-                   &m->alpha_d, &m->sigma_d, &m->ep_d, &m->k1_d, &m->eint_d, &m->static_d})
+                   &m->alpha_d, &m->sigma_d, &m->ep_d, &m->k1_d, &m->eint_d, &m->static_d, &m->mc_beta_d,
+                   reinterpret_cast<void**>(&m->mc_sites_d), reinterpret_cast<void**>(&m->mc_acc_d)})
     dfree(*p);
   if (m->perm_d) (void)hipFree(m->perm_d);
   if (m->part_d) (void)hipFree(m->part_d);
@@ -740,6 +741,120 @@ int mx_model_set_sot(mx_model* m, const double p[3], double B_DL, double B_FL) {
     for (int a = 0; a < 3; ++a) m->sot_p[a] = p[a];
     m->sot_dl = B_DL;
     m->sot_fl = B_FL;
+    return MX_OK;
+  });
+}
+
+int mx_mc_set_colors(mx_model* m, int32_t ncolor, const int64_t* off, const int32_t* sites) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (ncolor < 0 || (ncolor > 0 && (!off || !sites))) fail(MX_EINVAL, "null colour lists");
+    if (ncolor > 0 && off[0] != 0) fail(MX_EINVAL, "colour offsets must start at 0");
+    for (int c = 0; c < ncolor; ++c)
+      if (off[c + 1] < off[c]) fail(MX_EINVAL, "colour offsets must not decrease");
+    const int64_t n = ncolor > 0 ? off[ncolor] : 0;
+    if (n > m->N) fail(MX_EINVAL, "more colour entries than sites");
+    std::vector<int32_t> q(static_cast<size_t>(n));
+    std::vector<char> seen(static_cast<size_t>(m->N), 0);
+    for (int64_t k = 0; k < n; ++k) {
+      const int32_t r = sites[k];
+      if (r < 0 || r >= m->N) fail(MX_EINVAL, "colour site out of range");
+      if (seen[r]) fail(MX_EINVAL, "a site appears in two colours");
+      seen[r] = 1;
+      q[k] = internal_index(*m, r);
+    }
+    use_device(*m);
+    m->layout_frozen = 1;
+    if (m->mc_sites_d) hip_ok(hipFree(m->mc_sites_d), "hipFree");
+    m->mc_sites_d = nullptr;
+    if (n > 0) {
+      hip_ok(hipMalloc(reinterpret_cast<void**>(&m->mc_sites_d), sizeof(int32_t) * n), "hipMalloc colours");
+      hip_ok(hipMemcpy(m->mc_sites_d, q.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice), "hipMemcpy colours");
+    }
+    if (ncolor > 0) m->mc_off.assign(off, off + ncolor + 1);
+    else m->mc_off.assign(1, 0);
+    return MX_OK;
+  });
+}
+
+int mx_mc_set_temperature(mx_model* m, int per_site, const double* T) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (!T) fail(MX_EINVAL, "null temperature");
+    auto beta = [](double t) { return t > 0 ? 1.0 / (K_B * t) : INFINITY; };
+    use_device(*m);
+    m->layout_frozen = 1;
+    dfree(m->mc_beta_d);
+    m->mc_beta_finite = 1;
+    if (per_site) {
+      std::vector<double> b(static_cast<size_t>(m->N));
+      for (int r = 0; r < m->N; ++r) {
+        if (!(T[r] >= 0)) fail(MX_EINVAL, "temperatures must be >= 0");
+        b[internal_index(*m, r)] = beta(T[r]);
+        if (!std::isfinite(b[internal_index(*m, r)])) m->mc_beta_finite = 0;
+      }
+      hip_ok(hipMalloc(&m->mc_beta_d, b.size() * rsize(*m)), "hipMalloc beta");
+      upload(*m, m->mc_beta_d, b.data(), b.size());
+    } else {
+      if (!(T[0] >= 0)) fail(MX_EINVAL, "temperature must be >= 0");
+      m->mc_beta = beta(T[0]);
+      m->mc_beta_finite = std::isfinite(m->mc_beta) ? 1 : 0;
+    }
+    return MX_OK;
+  });
+}
+
+int mx_mc_sweep(mx_model* m, void* e, int64_t nsweeps, int move, int overrelax, double sigma, uint64_t seed,
+                uint64_t sweep0, void* stream) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (!e) fail(MX_EINVAL, "null spins");
+    if (nsweeps < 0 || overrelax < 0) fail(MX_EINVAL, "negative sweep or over-relaxation count");
+    if (move < 0 || move > 2) fail(MX_EINVAL, "move must be 0 (gaussian), 1 (uniform) or 2 (heat bath)");
+    if (m->mc_off.empty()) fail(MX_EINVAL, "set the colours first (mx_mc_set_colors)");
+    use_device(*m);
+    ensure_scratch(*m);
+    sync_static(*m);
+    if (!m->mc_acc_d) {
+      hip_ok(hipMalloc(reinterpret_cast<void**>(&m->mc_acc_d), sizeof(unsigned long long)), "hipMalloc counter");
+      hip_ok(hipMemset(m->mc_acc_d, 0, sizeof(unsigned long long)), "hipMemset counter");
+    }
+    // heat bath needs a finite beta everywhere; otherwise gaussian moves (magex/montecarlo.py:166-167)
+    const int mv = (move == 2 && !m->mc_beta_finite) ? 0 : move;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    void* E = e;
+    if (needs_permute(*m)) {
+      int rc = launch_permute(*m, e, m->eint_d, 1, s);
+      if (rc) return rc;
+      E = m->eint_d;
+    }
+    const int nc = int(m->mc_off.size()) - 1;
+    for (int64_t k = 0; k < nsweeps; ++k) {
+      const uint64_t sweep = sweep0 + uint64_t(k);
+      for (int pass = 0; pass <= overrelax; ++pass)
+        for (int c = 0; c < nc; ++c) {
+          const int64_t a = m->mc_off[c], b = m->mc_off[c + 1];
+          int rc = launch_mc(*m, E, m->mc_sites_d + a, int(b - a), m->mc_beta_d, m->mc_beta, mv, pass, sigma, seed,
+                             sweep, m->mc_acc_d, s);
+          if (rc) return rc;
+        }
+    }
+    if (needs_permute(*m)) return launch_permute(*m, m->eint_d, e, 0, s);
+    return MX_OK;
+  });
+}
+
+int mx_mc_accepted(mx_model* m, int64_t* out, int reset) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (!out) fail(MX_EINVAL, "null output");
+    use_device(*m);
+    unsigned long long v = 0;
+    if (m->mc_acc_d) {
+      hip_ok(hipMemcpy(&v, m->mc_acc_d, sizeof(v), hipMemcpyDeviceToHost), "hipMemcpy counter");
+      if (reset) hip_ok(hipMemset(m->mc_acc_d, 0, sizeof(v)), "hipMemset counter");
+    }
+    *out = int64_t(v);
     return MX_OK;
   });
 }

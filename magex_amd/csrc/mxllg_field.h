@@ -48,6 +48,29 @@ __device__ __forceinline__ bool site_of(const Static<R>* __restrict__ S, Site& s
   return true;
 }
 
+// Site record of internal site i (sublattice-major order), for kernels that take a list of sites.
+template <typename R, int GEO>
+__device__ __forceinline__ void site_at(const Static<R>* __restrict__ S, int i, Site& st) {
+  const int ncell = S->ncell, L1 = S->L1, L2 = S->L2;
+  st.s = i / ncell;
+  st.c = i - st.s * ncell;
+  if constexpr (GEO == GEO_TILED) {
+    const int b0 = S->tb0, b1 = S->tb1, b2 = S->tb2;
+    const int tile = st.c / BLOCK, l = st.c - tile * BLOCK;
+    const int t2 = tile % S->nt2, tq = tile / S->nt2, t1 = tq % S->nt1, t0 = tq / S->nt1;
+    st.c0 = (t0 << b0) + (l >> (b1 + b2));
+    st.c1 = (t1 << b1) + ((l >> b2) & ((1 << b1) - 1));
+    st.c2 = (t2 << b2) + (l & ((1 << b2) - 1));
+  } else {
+    st.c2 = st.c % L2;
+    const int t = st.c / L2;
+    st.c1 = t % L1;
+    st.c0 = t / L1;
+  }
+  st.crm = (st.c0 * L1 + st.c1) * L2 + st.c2;
+  st.i = i;
+}
+
 // Neighbour cell (internal index) of st along stencil entry en; offsets are reduced, so one conditional
 // wrap per periodic axis.  GEO_OPEN: ok = false past an open face, and the returned cell is the site's
 // own (a valid address; the caller zeroes its contribution).

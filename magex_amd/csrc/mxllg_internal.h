@@ -126,6 +126,13 @@ struct Model {
   void *ep_d = nullptr, *k1_d = nullptr, *eint_d = nullptr;  // predictor ep; k1_d holds w = e + dt/2 k1
   double* part_d = nullptr;
   int npart = 0;
+  // Monte Carlo: colour site lists (internal order, colours concatenated), inverse temperatures
+  int32_t* mc_sites_d = nullptr;
+  std::vector<int64_t> mc_off;  // ncolor + 1 offsets into mc_sites_d
+  void* mc_beta_d = nullptr;    // per-site beta (model precision, internal order) or null
+  double mc_beta = 0.0;         // uniform beta (1/meV; +inf at T = 0)
+  int mc_beta_finite = 1;       // every beta finite (heat bath possible)
+  unsigned long long* mc_acc_d = nullptr;
   // captured step loop (hipGraph of graph_steps Heun steps + counter advance), keyed by what it bakes in
   int graph_steps = 0;
   uint64_t* counter_d = nullptr;
@@ -151,6 +158,9 @@ int launch_noise(const Model& m, void* out_ref, uint64_t seed, uint64_t step, hi
 int launch_permute(const Model& m, const void* src, void* dst, int to_internal, hipStream_t s);
 bool needs_permute(const Model& m);
 int launch_rhs(const Model& m, const void* e, const void* B, double alpha, int prec, void* out, hipStream_t s);
+// One Monte Carlo pass over n sites of one colour (mxllg_mc.hip); pass 0 = the move, > 0 = over-relaxation.
+int launch_mc(const Model& m, void* e, const int32_t* sites, int n, const void* beta_site, double beta, int move,
+              int pass, double sigma, uint64_t seed, uint64_t sweep, unsigned long long* acc, hipStream_t s);
 int launch_topo(const Model& m, const void* e_ref, const int32_t* tri, int64_t ntri, double hand, double* out,
                 hipStream_t s);
 

@@ -1,0 +1,219 @@
+// Metropolis Monte Carlo colour sweep on the stencil gather (magex/fast.py:393-503 `_mc_group`,
+// driven as magex/montecarlo.py:152-179 `MetropolisMC._fast_sweep`).
+//
+// A colour is a list of sites none of which couple to each other, so all of them move at once: one
+// launch per colour, one thread per site.  Per site: H = 2 h + mu mu_B B (the linear local field, meV),
+// a proposal, dE = E_onsite(new) - E_onsite(old) - H.(new - old) for the linear moves (the heat-bath and
+// over-relaxation proposals already account for H exactly, so only the on-site change remains), and
+// acceptance if dE <= 0 or u < exp(-min(beta dE, 700)).
+//   move 0 (gaussian): new = n(old + sigma g), g three standard normals
+//   move 1 (uniform):  new = n(g)
+//   move 2 (heat bath): cos(theta) = 1 + log1p(-r (-expm1(-2x))) / x around H, x = beta |H| (uniform
+//                      cos(theta) below x = 1e-8), azimuth 2 pi u in a frame built on H
+//   over-relaxation passes: reflection of old about H (skipped where |H|^2 < 1e-28)
+// Random numbers: Philox4x32-10 keyed by the seed, counter (site in magex order, sweep lo, sweep hi,
+// 'MC' tag | pass << 8 | draw), converted to 53-bit uniforms (two words each) and Box-Muller normals
+// in double precision, so a sweep does not depend on the launch shape or on the cell layout.
+#include <cmath>
+
+#include "mxllg_field.h"
+
+namespace mx {
+
+constexpr uint32_t MC_TAG = 0x4D430000u;
+
+struct Uni4 {
+  double u0, u1;  // two 53-bit uniforms in (0, 1) from one Philox call
+};
+
+__device__ __forceinline__ Uni4 mc_draw(uint32_t k0, uint32_t k1, uint32_t site, uint64_t sweep, uint32_t tag) {
+  uint32_t c0 = site, c1 = uint32_t(sweep), c2 = uint32_t(sweep >> 32), c3 = tag;
+  philox10(c0, c1, c2, c3, k0, k1);
+  Uni4 u;
+  u.u0 = (double(c0 >> 5) * 67108864.0 + double(c1 >> 6) + 0.5) * 0x1.0p-53;
+  u.u1 = (double(c2 >> 5) * 67108864.0 + double(c3 >> 6) + 0.5) * 0x1.0p-53;
+  return u;
+}
+
+template <typename R, int CM, int GEO>
+__global__ __launch_bounds__(BLOCK) void k_mc(const Static<R>* __restrict__ S, const Dyn<R> d, R* __restrict__ e,
+                                              const int32_t* __restrict__ sites, int n,
+                                              const R* __restrict__ beta_site, R beta_u, int move, int pass,
+                                              R sigma, uint64_t sweep, unsigned long long* __restrict__ acc) {
+  const int t = blockIdx.x * BLOCK + threadIdx.x;
+  int ok_count = 0;
+  if (t < n) {
+    Site st;
+    site_at<R, GEO>(S, sites[t], st);
+    const int i = st.i;
+    R hx, hy, hz;
+    exchange<R, CM, GEO>(S, d, e, st, hx, hy, hz);
+    const R zf = R(1) / inv_mu_of(S, d, st);  // mu mu_B
+    const R Hx = R(2) * hx + zf * d.B0, Hy = R(2) * hy + zf * d.B1, Hz = R(2) * hz + zf * d.B2;
+    const R ox = e[3 * i], oy = e[3 * i + 1], oz = e[3 * i + 2];
+    const R bt = beta_site ? beta_site[i] : beta_u;
+    const uint32_t iref = uint32_t(st.crm * S->ns + st.s);
+    const uint32_t tag = MC_TAG | (uint32_t(pass) << 8);
+    R nx = ox, ny = oy, nz = oz;
+    bool lin = true, live = true;
+    double ua;  // acceptance uniform
+    if (pass > 0) {
+      // over-relaxation: reflection about H (H.e conserved)
+      const R HH = Hx * Hx + Hy * Hy + Hz * Hz;
+      if (HH < R(1e-28)) {
+        live = false;
+      } else {
+        const R p = R(2) * (ox * Hx + oy * Hy + oz * Hz) / HH;
+        nx = p * Hx - ox;
+        ny = p * Hy - oy;
+        nz = p * Hz - oz;
+        const R q = rnorm3(nx, ny, nz);
+        nx *= q;
+        ny *= q;
+        nz *= q;
+      }
+      lin = false;
+      ua = mc_draw(d.seed_lo, d.seed_hi, iref, sweep, tag).u0;
+    } else if (move == 2) {
+      const Uni4 a = mc_draw(d.seed_lo, d.seed_hi, iref, sweep, tag);
+      ua = mc_draw(d.seed_lo, d.seed_hi, iref, sweep, tag | 1u).u0;
+      const double H0 = Hx, H1 = Hy, H2 = Hz;
+      const double Hn = sqrt(H0 * H0 + H1 * H1 + H2 * H2);
+      const double xx = double(bt) * Hn;
+      double cth;
+      if (xx < 1e-8 || !isfinite(xx)) cth = xx < 1e-8 ? 2.0 * a.u0 - 1.0 : 1.0;
+      else cth = 1.0 + log1p(-a.u0 * (-expm1(-2.0 * xx))) / xx;
+      cth = fmin(1.0, fmax(-1.0, cth));
+      const double ph = 2.0 * M_PI * a.u1;
+      double zx = 0.0, zy = 0.0, zz = 1.0;
+      if (Hn > 0) {
+        zx = H0 / Hn;
+        zy = H1 / Hn;
+        zz = H2 / Hn;
+      }
+      double ax = 0.0, ay = 1.0, az = 0.0;
+      if (fabs(zx) < 0.9) {
+        ax = 1.0;
+        ay = 0.0;
+      }
+      double t1x = zy * az - zz * ay, t1y = zz * ax - zx * az, t1z = zx * ay - zy * ax;
+      const double n1 = sqrt(t1x * t1x + t1y * t1y + t1z * t1z);
+      t1x /= n1;
+      t1y /= n1;
+      t1z /= n1;
+      const double t2x = zy * t1z - zz * t1y, t2y = zz * t1x - zx * t1z, t2z = zx * t1y - zy * t1x;
+      const double sn = sqrt(fmax(0.0, 1.0 - cth * cth));
+      const double cp = cos(ph), sp = sin(ph);
+      nx = R(cth * zx + sn * (cp * t1x + sp * t2x));
+      ny = R(cth * zy + sn * (cp * t1y + sp * t2y));
+      nz = R(cth * zz + sn * (cp * t1z + sp * t2z));
+      lin = false;
+    } else {
+      const Uni4 a = mc_draw(d.seed_lo, d.seed_hi, iref, sweep, tag);
+      const Uni4 b = mc_draw(d.seed_lo, d.seed_hi, iref, sweep, tag | 1u);
+      ua = mc_draw(d.seed_lo, d.seed_hi, iref, sweep, tag | 2u).u0;
+      const double r0 = sqrt(-2.0 * log(a.u0)), r2 = sqrt(-2.0 * log(b.u0));
+      const double g0 = r0 * cos(2.0 * M_PI * a.u1), g1 = r0 * sin(2.0 * M_PI * a.u1);
+      const double g2 = r2 * cos(2.0 * M_PI * b.u1);
+      if (move == 0) {
+        nx = ox + sigma * R(g0);
+        ny = oy + sigma * R(g1);
+        nz = oz + sigma * R(g2);
+      } else {
+        nx = R(g0);
+        ny = R(g1);
+        nz = R(g2);
+      }
+      const R q = rnorm3(nx, ny, nz);
+      nx *= q;
+      ny *= q;
+      nz *= q;
+    }
+    if (live) {
+      R dE = onsite_energy(S, d, st, nx, ny, nz) - onsite_energy(S, d, st, ox, oy, oz);
+      if (lin) dE -= Hx * (nx - ox) + Hy * (ny - oy) + Hz * (nz - oz);
+      bool ok = dE <= R(0);
+      if (!ok) ok = ua < exp(-fmin(double(dE) * double(bt), 700.0));
+      if (ok) {
+        e[3 * i] = nx;
+        e[3 * i + 1] = ny;
+        e[3 * i + 2] = nz;
+        ok_count = pass == 0 ? 1 : 0;
+      }
+    }
+  }
+  // accepted moves: integer counts, so the order of the additions does not matter
+  __shared__ int cnt[BLOCK / 64];
+  int v = ok_count;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) cnt[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int w = 0; w < BLOCK / 64; ++w) s += cnt[w];
+    if (s) atomicAdd(acc, (unsigned long long)s);
+  }
+}
+
+template <typename R>
+static int mc_t(const Model& m, void* e, const int32_t* sites, int n, const void* beta_site, double beta, int move,
+                int pass, double sigma, uint64_t seed, uint64_t sweep, unsigned long long* acc, hipStream_t s) {
+  if (n <= 0) return MX_OK;
+  const Static<R>* S = static_cast<const Static<R>*>(m.static_d);
+  Dyn<R> d{};
+  d.B0 = R(m.B[0]);
+  d.B1 = R(m.B[1]);
+  d.B2 = R(m.B[2]);
+  d.seed_lo = uint32_t(seed);
+  d.seed_hi = uint32_t(seed >> 32);
+  d.bond = static_cast<const R*>(m.bond_d);
+  d.A = static_cast<const R*>(m.A_d);
+  d.Kc = static_cast<const R*>(m.Kc_d);
+  d.K2 = static_cast<const R*>(m.K2_d);
+  d.K3 = static_cast<const R*>(m.K3_d);
+  d.inv_mu = static_cast<const R*>(m.inv_mu_d);
+  R* E = static_cast<R*>(e);
+  const R* bs = static_cast<const R*>(beta_site);
+  const R bu = std::isinf(beta) ? R(INFINITY) : R(beta);
+  const dim3 g((n + BLOCK - 1) / BLOCK), b(BLOCK);
+#define MX_MC_CALL(CM, GEO) \
+  hipLaunchKernelGGL((k_mc<R, CM, GEO>), g, b, 0, s, S, d, E, sites, n, bs, bu, move, pass, R(sigma), sweep, acc)
+  switch (m.coup) {
+    case COUP_TABLE1:
+      if (m.geo == GEO_TILED) MX_MC_CALL(COUP_TABLE1, GEO_TILED);
+      else if (m.geo == GEO_PERIODIC) MX_MC_CALL(COUP_TABLE1, GEO_PERIODIC);
+      else MX_MC_CALL(COUP_TABLE1, GEO_OPEN);
+      break;
+    case COUP_TABLE4:
+      if (m.geo == GEO_TILED) MX_MC_CALL(COUP_TABLE4, GEO_TILED);
+      else if (m.geo == GEO_PERIODIC) MX_MC_CALL(COUP_TABLE4, GEO_PERIODIC);
+      else MX_MC_CALL(COUP_TABLE4, GEO_OPEN);
+      break;
+    case COUP_TABLE9:
+      if (m.geo == GEO_TILED) MX_MC_CALL(COUP_TABLE9, GEO_TILED);
+      else if (m.geo == GEO_PERIODIC) MX_MC_CALL(COUP_TABLE9, GEO_PERIODIC);
+      else MX_MC_CALL(COUP_TABLE9, GEO_OPEN);
+      break;
+    default:
+      if (m.geo == GEO_TILED) MX_MC_CALL(COUP_BOND, GEO_TILED);
+      else if (m.geo == GEO_PERIODIC) MX_MC_CALL(COUP_BOND, GEO_PERIODIC);
+      else MX_MC_CALL(COUP_BOND, GEO_OPEN);
+      break;
+  }
+#undef MX_MC_CALL
+  const hipError_t err = hipGetLastError();
+  if (err != hipSuccess) {
+    set_error(std::string("launch mc: ") + hipGetErrorString(err));
+    return MX_EHIP;
+  }
+  return MX_OK;
+}
+
+int launch_mc(const Model& m, void* e, const int32_t* sites, int n, const void* beta_site, double beta, int move,
+              int pass, double sigma, uint64_t seed, uint64_t sweep, unsigned long long* acc, hipStream_t s) {
+  return m.prec == MX_FP64 ? mc_t<double>(m, e, sites, n, beta_site, beta, move, pass, sigma, seed, sweep, acc, s)
+                           : mc_t<float>(m, e, sites, n, beta_site, beta, move, pass, sigma, seed, sweep, acc, s);
+}
+
+}  // namespace mx

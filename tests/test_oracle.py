@@ -394,3 +394,85 @@ def test_sot_damping_like_closed_form():
     assert abs(np.arccos(e[0] @ p) - th) < 1e-7
     e = R.run_heun(s, e0.copy(), n, 0.0, dt, sot=(p, 0.0, 2.0))
     assert abs(e[0] @ p - np.cos(th0)) < 1e-10
+
+
+# ------------------------------------------------------------------ Monte Carlo restatement
+def _langevin(x):
+    return 1 / np.tanh(x) - 1 / x
+
+
+def _free_spins(N, B, mu=2.0):
+    s = _sys((N, 1, 1), field=(0, 0, B), pbc=(True, True, True), lattice="chain", J=[1.0], moment=mu)
+    s.Jsp = sp.csr_matrix((3 * s.N, 3 * s.N))
+    return s
+
+
+def _mc_mean(s, groups, T, move, f, sweeps=240, burn=40, sigma=0.6, overrelax=0, seed=3):
+    from oracle import mc_ref
+    rng = np.random.default_rng(seed)
+    e = R.normalize(rng.normal(size=(s.N, 3)))
+    beta = 1 / (R.K_B * T)
+    vals = []
+    for k in range(sweeps):
+        mc_ref.sweep(s, e, groups, beta, move, overrelax, sigma, seed, k)
+        if k >= burn:
+            vals.append(f(e))
+    return float(np.mean(vals))
+
+
+@pytest.mark.parametrize("move", [0, 1, 2])
+def test_mc_single_spin_langevin(move):
+    """Free spins in a field: <cos theta> = coth x - 1/x, x = beta mu mu_B B, for every move."""
+    s = _free_spins(4096, 1.5)
+    T = 1.0
+    x = 2.0 * R.MU_B * 1.5 / (R.K_B * T)
+    m = _mc_mean(s, [np.arange(s.N)], T, move, lambda e: e[:, 2].mean())
+    assert abs(m - _langevin(x)) < 0.01, (m, _langevin(x))
+
+
+@pytest.mark.parametrize("move", [0, 2])
+def test_mc_dimer_correlation(move):
+    """Independent dimers E = -2 J e1.e2 (every ordered pair counted): <e1.e2> = L(2 beta J).  A factor
+    of 2 dropped from H = 2h fails it."""
+    N = 2048
+    s = _free_spins(N, 0.0)
+    J = 1.0
+    a, b = np.arange(0, N, 2), np.arange(1, N, 2)
+    rows, cols = [], []
+    for c in range(3):
+        rows += [3 * a + c, 3 * b + c]
+        cols += [3 * b + c, 3 * a + c]
+    rows, cols = np.concatenate(rows), np.concatenate(cols)
+    s.Jsp = sp.csr_matrix((np.full(len(rows), J), (rows, cols)), shape=(3 * N, 3 * N))
+    T = 1.0 / R.K_B                                          # 2 beta J = 2
+    m = _mc_mean(s, [a, b], T, move, lambda e: np.einsum("ij,ij->i", e[a], e[b]).mean())
+    assert abs(m - _langevin(2.0)) < 0.01, (m, _langevin(2.0))
+    assert abs(m - _langevin(1.0)) > 0.05
+
+
+def test_mc_heatbath_with_anisotropy_quadrature():
+    """Heat bath around H, Metropolis-corrected for an easy axis K: <cos theta> equals the quadrature
+    of exp(beta (mu mu_B B cos + K cos^2)) sin."""
+    from scipy.integrate import quad
+    s = _free_spins(4096, 0.8)
+    K, T = 0.3, 2.0
+    s.A = np.tile(-K * np.diag([0.0, 0.0, 1.0]), (s.N, 1, 1))
+    beta = 1 / (R.K_B * T)
+    w = lambda c: np.exp(beta * (2.0 * R.MU_B * 0.8 * c + K * c * c))            # noqa: E731
+    ref = quad(lambda c: c * w(c), -1, 1)[0] / quad(w, -1, 1)[0]
+    m = _mc_mean(s, [np.arange(s.N)], T, 2, lambda e: e[:, 2].mean())
+    assert abs(m - ref) < 0.01, (m, ref)
+
+
+def test_mc_overrelaxation_conserves_energy():
+    from oracle import mc_ref
+    s = _sys((4, 4, 4), field=(0.2, 0, 0.5), lattice="sc", J=[1.0], D=0.3)
+    s.A[:] = 0.0
+    e = s.init_random(1)
+    groups = [np.where((s.cells.sum(1) % 2) == c)[0] for c in (0, 1)]  # checkerboard: no NN inside a colour
+    E0 = s.energy(e)
+    before = e.copy()
+    # over-relaxation passes only: move pass uses gaussian with sigma 0 (no change), then two reflections
+    mc_ref.sweep(s, e, groups, 1 / (R.K_B * 5.0), 0, 2, 0.0, 7, 0)
+    assert abs(s.energy(e) - E0) < 1e-12 * abs(E0)
+    assert np.abs(e - before).max() > 0.1
