@@ -317,6 +317,30 @@ def adjoint_rates(steps=500, checkpoint_every=50, reps=3):
             "grad_theta": [float(x) for x in gt.cpu().numpy()]}
 
 
+def mc_rates(system, p, sweeps=50):
+    """Metropolis colour sweeps (gaussian moves, one over-relaxation pass) on the bcc64 workload, FP32:
+    site updates per second counting both passes, and the HBM rate by the same algorithmic bytes per
+    site visit as a Heun pass (a visit gathers every coupling of the site once)."""
+    import torch
+    from magex_amd.montecarlo import MetropolisMC
+    mc = MetropolisMC(system, p["temperature"], seed=1, sigma=0.3, overrelax=1, move="gaussian", precision="fp32")
+    e = torch.as_tensor(system.init_spins("random", seed=0), device="cuda", dtype=torch.float32)
+    mc.sweep_device(e, 3)
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    mc.sweep_device(e, sweeps)
+    t1.record()
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / sweeps
+    visits = 2 * system.N
+    B = alg_bytes(system, "fp32") / 2
+    return {"workload": "bcc64 Metropolis (gaussian sigma 0.3, 1 over-relaxation pass), T = 300 K, FP32",
+            "colours": mc.ncolors, "ms_per_sweep": ms, "site_visits_per_s": visits / (ms * 1e-3),
+            "acceptance": mc.acceptance(), "alg_bytes_per_visit": B,
+            "roofline_frac": B * visits / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
 def traffic_from_profiles(workload_name, precision):
     """HBM bytes per step measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE), if committed."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload_name}_{precision}.txt")
@@ -340,6 +364,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=300)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="skip the sc16 / tri256 GPU-beside-CPU table")
+    ap.add_argument("--no-mc", action="store_true", help="skip the Monte Carlo sweep leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 256 MiB device copy first (the reference kernel for rocprof FETCH/WRITE_SIZE)")
     a = ap.parse_args()
@@ -386,6 +411,9 @@ def main():
     if not a.no_adjoint:
         print("adjoint ...", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_rates()
+    if not a.no_mc:
+        print("monte carlo ...", file=sys.stderr, flush=True)
+        out["monte_carlo"] = mc_rates(system, p) if a.workload == "bcc64" else None
     if not a.no_configs:
         print("other configurations ...", file=sys.stderr, flush=True)
         out["configs"] = config_rates()

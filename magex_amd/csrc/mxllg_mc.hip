@@ -12,8 +12,10 @@
 //                      cos(theta) below x = 1e-8), azimuth 2 pi u in a frame built on H
 //   over-relaxation passes: reflection of old about H (skipped where |H|^2 < 1e-28)
 // Random numbers: Philox4x32-10 keyed by the seed, counter (site in magex order, sweep lo, sweep hi,
-// 'MC' tag | pass << 8 | draw), converted to 53-bit uniforms (two words each) and Box-Muller normals
-// in double precision, so a sweep does not depend on the launch shape or on the cell layout.
+// 'MC' tag | pass << 8 | draw), converted to 53-bit uniforms (two words each), so a sweep does not
+// depend on the launch shape or on the cell layout.  The normals, the heat-bath angle and the
+// acceptance test are evaluated in the model's precision (double for FP64, which the CPU restatement
+// reproduces; float for FP32).
 #include <cmath>
 
 #include "mxllg_field.h"
@@ -51,12 +53,13 @@ __global__ __launch_bounds__(BLOCK) void k_mc(const Static<R>* __restrict__ S, c
     const R zf = R(1) / inv_mu_of(S, d, st);  // mu mu_B
     const R Hx = R(2) * hx + zf * d.B0, Hy = R(2) * hy + zf * d.B1, Hz = R(2) * hz + zf * d.B2;
     const R ox = e[3 * i], oy = e[3 * i + 1], oz = e[3 * i + 2];
+    using F = R;  // precision of the random transforms
     const R bt = beta_site ? beta_site[i] : beta_u;
     const uint32_t iref = uint32_t(st.crm * S->ns + st.s);
     const uint32_t tag = MC_TAG | (uint32_t(pass) << 8);
     R nx = ox, ny = oy, nz = oz;
     bool lin = true, live = true;
-    double ua;  // acceptance uniform
+    F ua;  // acceptance uniform
     if (pass > 0) {
       // over-relaxation: reflection about H (H.e conserved)
       const R HH = Hx * Hx + Hy * Hy + Hz * Hz;
@@ -73,48 +76,49 @@ __global__ __launch_bounds__(BLOCK) void k_mc(const Static<R>* __restrict__ S, c
         nz *= q;
       }
       lin = false;
-      ua = mc_draw(d.seed_lo, d.seed_hi, iref, sweep, tag).u0;
+      ua = F(mc_draw(d.seed_lo, d.seed_hi, iref, sweep, tag).u0);
     } else if (move == 2) {
       const Uni4 a = mc_draw(d.seed_lo, d.seed_hi, iref, sweep, tag);
-      ua = mc_draw(d.seed_lo, d.seed_hi, iref, sweep, tag | 1u).u0;
-      const double H0 = Hx, H1 = Hy, H2 = Hz;
-      const double Hn = sqrt(H0 * H0 + H1 * H1 + H2 * H2);
-      const double xx = double(bt) * Hn;
-      double cth;
-      if (xx < 1e-8 || !isfinite(xx)) cth = xx < 1e-8 ? 2.0 * a.u0 - 1.0 : 1.0;
-      else cth = 1.0 + log1p(-a.u0 * (-expm1(-2.0 * xx))) / xx;
-      cth = fmin(1.0, fmax(-1.0, cth));
-      const double ph = 2.0 * M_PI * a.u1;
-      double zx = 0.0, zy = 0.0, zz = 1.0;
-      if (Hn > 0) {
+      ua = F(mc_draw(d.seed_lo, d.seed_hi, iref, sweep, tag | 1u).u0);
+      const F u0 = F(a.u0), u1 = F(a.u1);
+      const F H0 = Hx, H1 = Hy, H2 = Hz;
+      const F Hn = sqrt(H0 * H0 + H1 * H1 + H2 * H2);
+      const F xx = F(bt) * Hn;
+      F cth;
+      if (xx < F(1e-8) || !isfinite(xx)) cth = xx < F(1e-8) ? F(2) * u0 - F(1) : F(1);
+      else cth = F(1) + log1p(-u0 * (-expm1(F(-2) * xx))) / xx;
+      cth = fmin(F(1), fmax(F(-1), cth));
+      const F ph = F(2 * M_PI) * u1;
+      F zx = F(0), zy = F(0), zz = F(1);
+      if (Hn > F(0)) {
         zx = H0 / Hn;
         zy = H1 / Hn;
         zz = H2 / Hn;
       }
-      double ax = 0.0, ay = 1.0, az = 0.0;
-      if (fabs(zx) < 0.9) {
-        ax = 1.0;
-        ay = 0.0;
+      F ax = F(0), ay = F(1), az = F(0);
+      if (fabs(zx) < F(0.9)) {
+        ax = F(1);
+        ay = F(0);
       }
-      double t1x = zy * az - zz * ay, t1y = zz * ax - zx * az, t1z = zx * ay - zy * ax;
-      const double n1 = sqrt(t1x * t1x + t1y * t1y + t1z * t1z);
+      F t1x = zy * az - zz * ay, t1y = zz * ax - zx * az, t1z = zx * ay - zy * ax;
+      const F n1 = sqrt(t1x * t1x + t1y * t1y + t1z * t1z);
       t1x /= n1;
       t1y /= n1;
       t1z /= n1;
-      const double t2x = zy * t1z - zz * t1y, t2y = zz * t1x - zx * t1z, t2z = zx * t1y - zy * t1x;
-      const double sn = sqrt(fmax(0.0, 1.0 - cth * cth));
-      const double cp = cos(ph), sp = sin(ph);
-      nx = R(cth * zx + sn * (cp * t1x + sp * t2x));
-      ny = R(cth * zy + sn * (cp * t1y + sp * t2y));
-      nz = R(cth * zz + sn * (cp * t1z + sp * t2z));
+      const F t2x = zy * t1z - zz * t1y, t2y = zz * t1x - zx * t1z, t2z = zx * t1y - zy * t1x;
+      const F sn = sqrt(fmax(F(0), F(1) - cth * cth));
+      const F cp = cos(ph), sp = sin(ph);
+      nx = cth * zx + sn * (cp * t1x + sp * t2x);
+      ny = cth * zy + sn * (cp * t1y + sp * t2y);
+      nz = cth * zz + sn * (cp * t1z + sp * t2z);
       lin = false;
     } else {
       const Uni4 a = mc_draw(d.seed_lo, d.seed_hi, iref, sweep, tag);
       const Uni4 b = mc_draw(d.seed_lo, d.seed_hi, iref, sweep, tag | 1u);
-      ua = mc_draw(d.seed_lo, d.seed_hi, iref, sweep, tag | 2u).u0;
-      const double r0 = sqrt(-2.0 * log(a.u0)), r2 = sqrt(-2.0 * log(b.u0));
-      const double g0 = r0 * cos(2.0 * M_PI * a.u1), g1 = r0 * sin(2.0 * M_PI * a.u1);
-      const double g2 = r2 * cos(2.0 * M_PI * b.u1);
+      ua = F(mc_draw(d.seed_lo, d.seed_hi, iref, sweep, tag | 2u).u0);
+      const F r0 = sqrt(F(-2) * log(F(a.u0))), r2 = sqrt(F(-2) * log(F(b.u0)));
+      const F g0 = r0 * cos(F(2 * M_PI) * F(a.u1)), g1 = r0 * sin(F(2 * M_PI) * F(a.u1));
+      const F g2 = r2 * cos(F(2 * M_PI) * F(b.u1));
       if (move == 0) {
         nx = ox + sigma * R(g0);
         ny = oy + sigma * R(g1);
@@ -133,7 +137,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(const Static<R>* __restrict__ S, c
       R dE = onsite_energy(S, d, st, nx, ny, nz) - onsite_energy(S, d, st, ox, oy, oz);
       if (lin) dE -= Hx * (nx - ox) + Hy * (ny - oy) + Hz * (nz - oz);
       bool ok = dE <= R(0);
-      if (!ok) ok = ua < exp(-fmin(double(dE) * double(bt), 700.0));
+      if (!ok) ok = ua < exp(-fmin(F(dE) * F(bt), F(700)));
       if (ok) {
         e[3 * i] = nx;
         e[3 * i + 1] = ny;
