@@ -136,6 +136,15 @@ int mx_mc_sweep(mx_model* m, void* e, int64_t nsweeps, int move, int overrelax, 
                 uint64_t sweep0, void* stream);
 /* Accepted moves (not counting over-relaxation) since the last reset; synchronises. */
 int mx_mc_accepted(mx_model* m, int64_t* out, int reset);
+/* Replica sweeps for parallel tempering (magex/advanced_mc.py:38-94): nrep copies of the model's
+ * lattice, replica r's spins at e + r * N * 3 (device), each with its own beta (1/meV, inf at T = 0),
+ * cone width sigma and generator key seeds[r] (host arrays of nrep), all in one launch per colour and
+ * pass.  One-site cells, row-major layout, uniform temperatures only.  The acceptance counts of the
+ * replicas accumulate on the device; mx_mc_replica_accepted reads them (synchronising). */
+int mx_mc_sweep_replicas(mx_model* m, void* e, int32_t nrep, int64_t nsweeps, int move, int overrelax,
+                         const double* beta, const double* sigma, const uint64_t* seeds, uint64_t sweep0,
+                         void* stream);
+int mx_mc_replica_accepted(mx_model* m, int32_t nrep, uint64_t* out, int reset);
 
 /* Steps per captured step loop (hipGraph replayed by mx_llg_run when no noise is injected and at least
  * two loops fit; default 0 = always launch step by step).  Results are identical either way: the

@@ -133,6 +133,9 @@ struct Model {
   double mc_beta = 0.0;         // uniform beta (1/meV; +inf at T = 0)
   int mc_beta_finite = 1;       // every beta finite (heat bath possible)
   unsigned long long* mc_acc_d = nullptr;
+  void* mc_rep_d = nullptr;  // replica sweeps: per-replica parameters and acceptance counters
+  unsigned long long* mc_repacc_d = nullptr;
+  int mc_rep_cap = 0;
   // captured step loop (hipGraph of graph_steps Heun steps + counter advance), keyed by what it bakes in
   int graph_steps = 0;
   uint64_t* counter_d = nullptr;

@@ -37,12 +37,30 @@ __device__ __forceinline__ Uni4 mc_draw(uint32_t k0, uint32_t k1, uint32_t site,
   return u;
 }
 
+// Per-replica parameters of a replica sweep (parallel tempering): replica r owns spins
+// e + r * N * 3 and its own inverse temperature, cone width, generator key and acceptance counter.
+struct McRep {
+  double beta, sigma;
+  uint32_t seed_lo, seed_hi;
+};
+
 template <typename R, int CM, int GEO>
-__global__ __launch_bounds__(BLOCK) void k_mc(const Static<R>* __restrict__ S, const Dyn<R> d, R* __restrict__ e,
+__global__ __launch_bounds__(BLOCK) void k_mc(const Static<R>* __restrict__ S, Dyn<R> d, R* __restrict__ e,
                                               const int32_t* __restrict__ sites, int n,
                                               const R* __restrict__ beta_site, R beta_u, int move, int pass,
-                                              R sigma, uint64_t sweep, unsigned long long* __restrict__ acc) {
+                                              R sigma, uint64_t sweep, unsigned long long* __restrict__ acc,
+                                              const McRep* __restrict__ reps) {
   const int t = blockIdx.x * BLOCK + threadIdx.x;
+  if (reps) {
+    // replica blockIdx.y: wavefront-uniform scalar loads of its parameters
+    const McRep* rp = reps + blockIdx.y;
+    e += size_t(blockIdx.y) * size_t(S->N) * 3;
+    acc += blockIdx.y;
+    beta_u = R(rp->beta);
+    sigma = R(rp->sigma);
+    d.seed_lo = rp->seed_lo;
+    d.seed_hi = rp->seed_hi;
+  }
   int ok_count = 0;
   if (t < n) {
     Site st;
@@ -162,8 +180,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc(const Static<R>* __restrict__ S, c
 
 template <typename R>
 static int mc_t(const Model& m, void* e, const int32_t* sites, int n, const void* beta_site, double beta, int move,
-                int pass, double sigma, uint64_t seed, uint64_t sweep, unsigned long long* acc, hipStream_t s) {
-  if (n <= 0) return MX_OK;
+                int pass, double sigma, uint64_t seed, uint64_t sweep, unsigned long long* acc, hipStream_t s,
+                const McRep* reps, int nrep) {
+  if (n <= 0 || nrep <= 0) return MX_OK;
   const Static<R>* S = static_cast<const Static<R>*>(m.static_d);
   Dyn<R> d{};
   d.B0 = R(m.B[0]);
@@ -180,9 +199,10 @@ static int mc_t(const Model& m, void* e, const int32_t* sites, int n, const void
   R* E = static_cast<R*>(e);
   const R* bs = static_cast<const R*>(beta_site);
   const R bu = std::isinf(beta) ? R(INFINITY) : R(beta);
-  const dim3 g((n + BLOCK - 1) / BLOCK), b(BLOCK);
-#define MX_MC_CALL(CM, GEO) \
-  hipLaunchKernelGGL((k_mc<R, CM, GEO>), g, b, 0, s, S, d, E, sites, n, bs, bu, move, pass, R(sigma), sweep, acc)
+  const dim3 g((n + BLOCK - 1) / BLOCK, reps ? nrep : 1), b(BLOCK);
+#define MX_MC_CALL(CM, GEO)                                                                                    \
+  hipLaunchKernelGGL((k_mc<R, CM, GEO>), g, b, 0, s, S, d, E, sites, n, bs, bu, move, pass, R(sigma), sweep, acc, \
+                     reps)
   switch (m.coup) {
     case COUP_TABLE1:
       if (m.geo == GEO_TILED) MX_MC_CALL(COUP_TABLE1, GEO_TILED);
@@ -216,8 +236,109 @@ static int mc_t(const Model& m, void* e, const int32_t* sites, int n, const void
 
 int launch_mc(const Model& m, void* e, const int32_t* sites, int n, const void* beta_site, double beta, int move,
               int pass, double sigma, uint64_t seed, uint64_t sweep, unsigned long long* acc, hipStream_t s) {
-  return m.prec == MX_FP64 ? mc_t<double>(m, e, sites, n, beta_site, beta, move, pass, sigma, seed, sweep, acc, s)
-                           : mc_t<float>(m, e, sites, n, beta_site, beta, move, pass, sigma, seed, sweep, acc, s);
+  return m.prec == MX_FP64
+             ? mc_t<double>(m, e, sites, n, beta_site, beta, move, pass, sigma, seed, sweep, acc, s, nullptr, 1)
+             : mc_t<float>(m, e, sites, n, beta_site, beta, move, pass, sigma, seed, sweep, acc, s, nullptr, 1);
 }
 
 }  // namespace mx
+
+// ------------------------------------------------------------------------------------- replica sweeps
+using namespace mx;
+
+extern "C" int mx_mc_sweep_replicas(mx_model* m, void* e, int32_t nrep, int64_t nsweeps, int move, int overrelax,
+                                    const double* beta, const double* sigma, const uint64_t* seeds, uint64_t sweep0,
+                                    void* stream) {
+  if (!m || !e || !beta || !sigma || !seeds) {
+    set_error("null argument");
+    return MX_EINVAL;
+  }
+  if (nrep < 1 || nsweeps < 0 || overrelax < 0 || move < 0 || move > 2) {
+    set_error("bad replica count, sweep count, over-relaxation count or move");
+    return MX_EINVAL;
+  }
+  if (m->mc_off.empty()) {
+    set_error("set the colours first (mx_mc_set_colors)");
+    return MX_EINVAL;
+  }
+  if (m->mc_beta_d) {
+    set_error("replica sweeps take one temperature per replica: the model has per-site temperatures");
+    return MX_EUNSUPPORTED;
+  }
+  if (m->ns > 1 || m->geo == GEO_TILED) {
+    set_error("replica sweeps need one-site cells in row-major order (their spins are not permuted)");
+    return MX_EUNSUPPORTED;
+  }
+  hipError_t err = hipSetDevice(m->device);
+  int rc = err == hipSuccess ? sync_model_static(*m) : MX_EHIP;
+  if (rc) {
+    if (err != hipSuccess) set_error(std::string("hipSetDevice: ") + hipGetErrorString(err));
+    return rc;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  std::vector<McRep> h(nrep);
+  bool finite = true;
+  for (int r = 0; r < nrep; ++r) {
+    h[r].beta = beta[r];
+    h[r].sigma = sigma[r];
+    h[r].seed_lo = uint32_t(seeds[r]);
+    h[r].seed_hi = uint32_t(seeds[r] >> 32);
+    finite = finite && std::isfinite(beta[r]);
+  }
+  // parameter block and per-replica counters: reallocated when the replica count grows; the earlier
+  // sweeps on this stream finish before the block is rewritten
+  err = hipStreamSynchronize(s);
+  if (err == hipSuccess && nrep > m->mc_rep_cap) {
+    if (m->mc_rep_d) (void)hipFree(m->mc_rep_d);
+    if (m->mc_repacc_d) (void)hipFree(m->mc_repacc_d);
+    m->mc_rep_d = nullptr;
+    m->mc_repacc_d = nullptr;
+    m->mc_rep_cap = 0;
+    err = hipMalloc(&m->mc_rep_d, sizeof(McRep) * nrep);
+    if (err == hipSuccess)
+      err = hipMalloc(reinterpret_cast<void**>(&m->mc_repacc_d), sizeof(unsigned long long) * nrep);
+    if (err == hipSuccess) err = hipMemset(m->mc_repacc_d, 0, sizeof(unsigned long long) * nrep);
+    if (err == hipSuccess) m->mc_rep_cap = nrep;
+  }
+  if (err == hipSuccess) err = hipMemcpy(m->mc_rep_d, h.data(), sizeof(McRep) * nrep, hipMemcpyHostToDevice);
+  if (err != hipSuccess) {
+    set_error(std::string("replica sweep set-up: ") + hipGetErrorString(err));
+    return err == hipErrorOutOfMemory ? MX_ENOMEM : MX_EHIP;
+  }
+  const McRep* reps = static_cast<const McRep*>(m->mc_rep_d);
+  const int mv = (move == 2 && !finite) ? 0 : move;
+  const int nc = int(m->mc_off.size()) - 1;
+  for (int64_t k = 0; rc == MX_OK && k < nsweeps; ++k)
+    for (int pass = 0; rc == MX_OK && pass <= overrelax; ++pass)
+      for (int c = 0; rc == MX_OK && c < nc; ++c) {
+        const int64_t a = m->mc_off[c], b = m->mc_off[c + 1];
+        const uint64_t sw = sweep0 + uint64_t(k);
+        rc = m->prec == MX_FP64 ? mc_t<double>(*m, e, m->mc_sites_d + a, int(b - a), nullptr, 0.0, mv, pass, 0.0, 0,
+                                               sw, m->mc_repacc_d, s, reps, nrep)
+                                : mc_t<float>(*m, e, m->mc_sites_d + a, int(b - a), nullptr, 0.0, mv, pass, 0.0, 0,
+                                              sw, m->mc_repacc_d, s, reps, nrep);
+      }
+  return rc;
+}
+
+extern "C" int mx_mc_replica_accepted(mx_model* m, int32_t nrep, uint64_t* out, int reset) {
+  if (!m || !out || nrep < 1) {
+    set_error("null argument or bad replica count");
+    return MX_EINVAL;
+  }
+  if (nrep > m->mc_rep_cap) {
+    for (int r = 0; r < nrep; ++r) out[r] = 0;
+    return MX_OK;
+  }
+  std::vector<unsigned long long> v(nrep);
+  hipError_t err = hipSetDevice(m->device);
+  if (err == hipSuccess) err = hipMemcpy(v.data(), m->mc_repacc_d, sizeof(unsigned long long) * nrep,
+                                         hipMemcpyDeviceToHost);
+  if (err == hipSuccess && reset) err = hipMemset(m->mc_repacc_d, 0, sizeof(unsigned long long) * nrep);
+  if (err != hipSuccess) {
+    set_error(std::string("replica counters: ") + hipGetErrorString(err));
+    return MX_EHIP;
+  }
+  for (int r = 0; r < nrep; ++r) out[r] = uint64_t(v[r]);
+  return MX_OK;
+}

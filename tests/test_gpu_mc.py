@@ -1,9 +1,10 @@
 """Parity of the device Monte Carlo colour sweep with its CPU restatement (oracle/mc_ref.py) on the same
 counter-based random numbers, and its sampling against closed forms in FP32.
 
-Bars: FP64 spins within 1e-10 of the oracle after several sweeps and identical accepted counts (a
+Bars: FP64 spins within 1e-10 of the oracle after a few sweeps and identical accepted counts (a
 round-off difference could flip one acceptance, which would show as an O(1) difference: none occur
-in these cases); FP32 Langevin averages within 0.01 of coth x - 1/x.
+in these cases); parallel tempering over 120 sweeps within 1e-6 with identical swaps and cone widths;
+FP32 Langevin averages within 0.01 of coth x - 1/x.
 """
 import json
 import os
@@ -112,3 +113,66 @@ def test_mc_fp32_langevin(gpu, move):
         vals.append(float(e[:, 2].mean()))
     x = 2.0 * R.MU_B * 1.5 / (R.K_B * 1.0)
     assert abs(np.mean(vals) - (1 / np.tanh(x) - 1 / x)) < 0.01
+
+
+def _pt_reference(o, groups, Ts, therm, meas, swap_every, sample_every, seed, sigma, overrelax, init):
+    """The parallel-tempering schedule of magex/advanced_mc.py:38-94 on the CPU restatement of the sweep."""
+    rng = np.random.default_rng(seed)
+    M = len(Ts)
+    betas = 1 / (R.K_B * np.asarray(Ts))
+    conf = [c.copy() for c in init]
+    sig = np.full(M, sigma)
+    acc, tried = np.zeros(M), np.zeros(M)
+    nfree = sum(len(g) for g in groups)
+    swaps_acc, swaps_try = np.zeros(M - 1), np.zeros(M - 1)
+    for n in range(therm + meas):
+        for k in range(M):
+            acc[k] += mc_ref.sweep(o, conf[k], groups, betas[k], 0, overrelax, sig[k], seed + 17 * k, n)
+            tried[k] += nfree
+            if n < therm and (n + 1) % 50 == 0:
+                sig[k] = np.clip(sig[k] * 0.5 / max(1 - acc[k] / tried[k], 0.05), 0.02, 4.0)
+                acc[k] = tried[k] = 0
+        if (n + 1) % swap_every == 0:
+            E = np.array([o.energy(c) for c in conf])
+            for k in range((n // swap_every) % 2, M - 1, 2):
+                d = (betas[k] - betas[k + 1]) * (E[k] - E[k + 1])
+                swaps_try[k] += 1
+                if d >= 0 or rng.random() < np.exp(d):
+                    conf[k], conf[k + 1] = conf[k + 1], conf[k]
+                    E[k], E[k + 1] = E[k + 1], E[k]
+                    swaps_acc[k] += 1
+    return conf, sig, (swaps_acc / np.maximum(swaps_try, 1)).tolist()
+
+
+def test_parallel_tempering_matches_oracle(gpu):
+    """Four replicas swept together on the device, swaps and cone adaptation, against the same schedule
+    on the CPU restatement (FP64, identical draws)."""
+    print("case: parallel tempering vs oracle", flush=True)
+    from magex_amd.advanced_mc import parallel_tempering
+    s, o = _pair((4, 4, 4), (0, 0, 0.3), lattice="sc", J=[1.0], K=0.2, moment=2.0)
+    Ts = [10.0, 11.0, 12.2, 13.5]
+    rows, swap, conf = parallel_tempering(s, Ts, therm_sweeps=100, meas_sweeps=20, sample_every=2, swap_every=5,
+                                          seed=3, init="random", sigma=0.6, overrelax=1, return_spins=True)
+    mc = MetropolisMC(s, Ts[0], seed=3)
+    init = [s.init_spins("random", (0, 0, 1), 3 + k) for k in range(4)]
+    ref, sig, swap_ref = _pt_reference(o, mc.groups, Ts, 100, 20, 5, 2, 3, 0.6, 1, init)
+    got = conf.cpu().numpy()
+    # 120 sweeps carry round-off of two evaluation orders (about 1e-9 here); a single differing accept,
+    # reject or swap decision would show as an O(0.1) difference
+    assert max(np.abs(got[k] - ref[k]).max() for k in range(4)) < 1e-6
+    assert swap == swap_ref and 0 < min(swap)
+    assert np.allclose([r["sigma"] for r in rows], sig, rtol=1e-12)
+
+
+def test_parallel_tempering_free_spins(gpu):
+    """Free spins in a field at a ladder of temperatures: E/N = -mu mu_B B L(x) at every rung."""
+    print("case: parallel tempering free spins", flush=True)
+    from magex_amd.advanced_mc import parallel_tempering
+    s = S.SpinSystem(S.load_results(M.make_model("sc", J=[0.0], moment=2.0)), (16, 16, 16), field=(0, 0, 1.5))
+    Ts = [0.5, 1.0, 2.0, 4.0]
+    rows, swap = parallel_tempering(s, Ts, therm_sweeps=100, meas_sweeps=200, sample_every=2, swap_every=5,
+                                    seed=2, init="random", precision="fp32", move="heatbath")
+    zf = 2.0 * R.MU_B * 1.5
+    for r in rows:
+        x = zf / (R.K_B * r["T"])
+        assert abs(r["E"] + zf * (1 / np.tanh(x) - 1 / x)) < 0.01 * zf, r
