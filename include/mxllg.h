@@ -146,6 +146,19 @@ int mx_mc_sweep_replicas(mx_model* m, void* e, int32_t nrep, int64_t nsweeps, in
                          void* stream);
 int mx_mc_replica_accepted(mx_model* m, int32_t nrep, uint64_t* out, int reset);
 
+/* ---------------------------------------------------------------------------------------------
+ * Spin-wave sampler: m_q = (1/sqrt N) sum_i e_i exp(-i q.r_i) for nq wave vectors, the sampling step of
+ * the dynamical structure factor (magex/spectra.py:49-84, `mq[s] = ph.T @ e`).  Positions
+ * r_i = c0 a0 + c1 a1 + c2 a2 + tau_s come from the cell rows `cell` (9 doubles, Angstrom) and the
+ * sublattice positions `basis` (nsub x 3); q: nq x 3 Cartesian (1/Angstrom); host arrays, copied. */
+typedef struct mx_sampler mx_sampler;
+int mx_sampler_create(const mx_model* m, int32_t nq, const double* q, const double* cell, const double* basis,
+                      mx_sampler** out);
+void mx_sampler_destroy(mx_sampler* s);
+/* m_q of spins e (device, N x 3, magex order, model precision) into out (device, nq x 6 doubles:
+ * Re x, Im x, Re y, Im y, Re z, Im z); nothing synchronises. */
+int mx_sample_mq(mx_sampler* s, const void* e, double* out, void* stream);
+
 /* Steps per captured step loop (hipGraph replayed by mx_llg_run when no noise is injected and at least
  * two loops fit; default 0 = always launch step by step).  Results are identical either way: the
  * captured kernels read the step number from a device counter, so the noise sequence is the same.

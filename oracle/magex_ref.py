@@ -451,3 +451,34 @@ def neighbour_blocks(Jsp, N):
         D = np.stack([Aq[:, 1, 2], Aq[:, 2, 0], Aq[:, 0, 1]], 1)
         table = iso[:, None].copy() if np.all(np.abs(D) <= tol) else np.column_stack([iso, D])
     return ptr, nbr, typ.astype(np.int32), np.ascontiguousarray(table)
+
+
+# ----------------------------------------------------------------------------- spectra
+HBAR_MEV_PS = 0.6582119569                                   # quantum.py:34
+
+
+def mq_sample(system_positions, qc, e):
+    """m_q = ph.T @ e with ph = exp(-i r q^T) / sqrt(N) (spectra.py:55-66)."""
+    ph = np.exp(-1j * system_positions @ qc.T) / np.sqrt(len(e))
+    return ph.T @ e
+
+
+def welch_sqw(mq, ref, dt, nseg):
+    """Hann-windowed Welch average of m_q(t) into (E, S_perp, S_par) (spectra.py:67-84)."""
+    n = np.asarray(ref, float) / np.linalg.norm(ref)
+    a = np.array([1.0, 0, 0]) if abs(n[0]) < 0.9 else np.array([0, 1.0, 0])
+    e1 = np.cross(n, a)
+    e1 /= np.linalg.norm(e1)
+    e2 = np.cross(n, e1)
+    seg = len(mq) // nseg
+    win = np.hanning(seg)
+    Sperp = Spar = 0.0
+    for k in range(nseg):
+        x = mq[k * seg:(k + 1) * seg]
+        x = x - x.mean(0, keepdims=True)
+        F = np.fft.fft(x * win[:, None, None], axis=0)
+        Sperp = Sperp + np.abs(F @ e1) ** 2 + np.abs(F @ e2) ** 2
+        Spar = Spar + np.abs(F @ n) ** 2
+    E = 2 * np.pi * np.fft.fftfreq(seg, dt) * HBAR_MEV_PS
+    order = np.argsort(E)
+    return E[order], (Sperp / nseg)[order].T, (Spar / nseg)[order].T

@@ -476,3 +476,31 @@ def test_mc_overrelaxation_conserves_energy():
     mc_ref.sweep(s, e, groups, 1 / (R.K_B * 5.0), 0, 2, 0.0, 7, 0)
     assert abs(s.energy(e) - E0) < 1e-12 * abs(E0)
     assert np.abs(e - before).max() > 0.1
+
+
+# ------------------------------------------------------------------ spectra restatement
+def test_mq_sample_plane_wave_closed_form():
+    """A commensurate spin spiral e = (cos k.r, sin k.r, 0): m_k = (sqrt N / 2)(1, -i, 0), zero elsewhere."""
+    s = _sys((12, 8, 1), lattice="square", J=[1.0])
+    B = 2 * np.pi * np.linalg.inv(s.uc.cell).T
+    k = np.array([3 / 12, 2 / 8, 0]) @ B
+    ph = s.positions @ k
+    e = np.stack([np.cos(ph), np.sin(ph), np.zeros_like(ph)], 1)
+    qs = np.array([[3 / 12, 2 / 8, 0], [1 / 12, 0, 0], [5 / 12, 3 / 8, 0]]) @ B
+    m = R.mq_sample(s.positions, qs, e)
+    rN = np.sqrt(s.N)
+    assert np.allclose(m[0], [rN / 2, -1j * rN / 2, 0], atol=1e-12)
+    assert np.abs(m[1:]).max() < 1e-12
+
+
+def test_welch_peak_of_a_precessing_mode():
+    """m_q(t) precessing at omega: S_perp peaks at E = +-hbar omega (within one frequency bin), S_par ~ 0."""
+    dt, n, w = 0.02, 2048, 3.0
+    t = np.arange(n) * dt
+    mq = np.zeros((n, 1, 3), complex)
+    mq[:, 0, 0] = np.exp(-1j * w * t)
+    mq[:, 0, 1] = 1j * np.exp(-1j * w * t)
+    E, Sp, Sl = R.welch_sqw(mq, [0, 0, 1.0], dt, 2)
+    dE = E[1] - E[0]
+    assert abs(abs(E[np.argmax(Sp[0])]) - R.HBAR_MEV_PS * w) <= dE
+    assert Sl.max() < 1e-20
