@@ -173,6 +173,13 @@ int mx_noise(mx_model* m, void* out, uint64_t seed, uint64_t step, void* stream)
  * spinmodel.py:454-473.  out: device N x 3. */
 int mx_field(mx_model* m, const void* e, void* out, int kind, void* stream);
 
+/* The field (as mx_field) and the observables (as mx_observables) of nimg spin configurations stored one
+ * after another (e: nimg x N x 3; out: nimg x N x 3, resp. nimg x MX_OBS_LEN doubles), in one launch
+ * for all images: the batched energy and gradient of magex's GNEB image chain (magex/gneb.py:171-187,
+ * which evaluates the images one by one). */
+int mx_field_images(mx_model* m, const void* e, int32_t nimg, void* out, int kind, void* stream);
+int mx_observables_images(mx_model* m, const void* e, int32_t nimg, double* out, void* stream);
+
 /* Observables of e into out (device, MX_OBS_LEN doubles): [E (meV, SpinSystem.energy), sum e (3),
  * sum s_i e_i (3, order parameter numerator), max_i |e_i x B_eff,i| (T, LLGSolver.max_torque)];
  * spinmodel.py:475-484,517-524, llg.py:250-252. */

@@ -50,6 +50,8 @@ SIGNATURES = {
     "mx_mc_accepted": (_I, [_P, _P, _I]),
     "mx_mc_sweep_replicas": (_I, [_P, _P, ctypes.c_int32, _I64, _I, _I, _P, _P, _P, _U64, _P]),
     "mx_mc_replica_accepted": (_I, [_P, ctypes.c_int32, _P, _I]),
+    "mx_field_images": (_I, [_P, _P, ctypes.c_int32, _P, _I, _P]),
+    "mx_observables_images": (_I, [_P, _P, ctypes.c_int32, _P, _P]),
     "mx_sampler_create": (_I, [_P, ctypes.c_int32, _P, _P, _P, _P]),
     "mx_sampler_destroy": (None, [_P]),
     "mx_sample_mq": (_I, [_P, _P, _P, _P]),

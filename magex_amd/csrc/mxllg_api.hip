@@ -267,7 +267,7 @@ void mx_model_destroy(mx_model* m) {
   for (void** p : {&m->bond_d, &m->A_d, &m->Kc_d, &m->K2_d, &m->K3_d, &m->inv_mu_d, &m->sign_d, &m->pinned_d,
                    &m->alpha_d, &m->sigma_d, &m->ep_d, &m->k1_d, &m->eint_d, &m->static_d, &m->mc_beta_d,
                    reinterpret_cast<void**>(&m->mc_sites_d), reinterpret_cast<void**>(&m->mc_acc_d), &m->mc_rep_d,
-                   reinterpret_cast<void**>(&m->mc_repacc_d)})
+                   reinterpret_cast<void**>(&m->mc_repacc_d), reinterpret_cast<void**>(&m->img_part_d)})
     dfree(*p);
   if (m->perm_d) (void)hipFree(m->perm_d);
   if (m->part_d) (void)hipFree(m->part_d);
@@ -904,6 +904,61 @@ int mx_observables(mx_model* m, const void* e, double* out, void* stream) {
     const void* E = internal_spins(m, e, s, rc);
     if (rc) return rc;
     return launch_observables(*m, E, out, s);
+  });
+}
+
+int mx_field_images(mx_model* m, const void* e, int32_t nimg, void* out, int kind, void* stream) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (!e || !out) fail(MX_EINVAL, "null buffer");
+    if (nimg < 1) fail(MX_EINVAL, "no images");
+    if (kind < MX_FIELD_BEFF || kind > MX_FIELD_EXCHANGE) fail(MX_EINVAL, "bad field kind");
+    use_device(*m);
+    ensure_scratch(*m);
+    sync_static(*m);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!needs_permute(*m)) return launch_field(*m, e, out, kind, s, nimg);
+    // images in magex order that the kernels read in another order: one image at a time
+    const size_t b = size_t(m->N) * 3 * rsize(*m);
+    for (int k = 0; k < nimg; ++k) {
+      int rc;
+      const void* E = internal_spins(m, static_cast<const char*>(e) + k * b, s, rc);
+      if (rc == MX_OK) rc = launch_field(*m, E, static_cast<char*>(out) + k * b, kind, s);
+      if (rc) return rc;
+    }
+    return MX_OK;
+  });
+}
+
+int mx_observables_images(mx_model* m, const void* e, int32_t nimg, double* out, void* stream) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (!e || !out) fail(MX_EINVAL, "null buffer");
+    if (nimg < 1) fail(MX_EINVAL, "no images");
+    use_device(*m);
+    ensure_scratch(*m);
+    sync_static(*m);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!needs_permute(*m)) {
+      if (nimg > m->img_part_cap) {
+        if (m->img_part_d) hip_ok(hipFree(m->img_part_d), "hipFree");
+        m->img_part_d = nullptr;
+        m->img_part_cap = 0;
+        const int nblk = m->ns * ((m->ncell + BLOCK - 1) / BLOCK);
+        hip_ok(hipMalloc(reinterpret_cast<void**>(&m->img_part_d), sizeof(double) * MX_OBS_LEN * nblk * size_t(nimg)),
+               "hipMalloc image partials");
+        m->img_part_cap = nimg;
+      }
+      return launch_observables(*m, e, out, s, nimg, m->img_part_d);
+    }
+    const size_t b = size_t(m->N) * 3 * rsize(*m);
+    for (int k = 0; k < nimg; ++k) {
+      int rc;
+      const void* E = internal_spins(m, static_cast<const char*>(e) + k * b, s, rc);
+      if (rc == MX_OK) rc = launch_observables(*m, E, out + size_t(k) * MX_OBS_LEN, s);
+      if (rc) return rc;
+    }
+    return MX_OK;
   });
 }
 

@@ -141,10 +141,13 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], int nmax_from, dou
   }
 }
 
-// Second stage: one workgroup folds the per-block partials in a fixed order, then scales them.
+// Second stage: one workgroup folds the per-block partials in a fixed order, then scales them; with a
+// grid of several workgroups, workgroup r folds row r (partials r * nblk .. into out + r * NV).
 template <int NV>
 __global__ __launch_bounds__(BLOCK) void k_final(const double* __restrict__ part, int nblk, int nmax_from,
                                                  double scale, double* __restrict__ out) {
+  part += size_t(blockIdx.x) * nblk * NV;
+  out += size_t(blockIdx.x) * NV;
   __shared__ double res[NV];
   double v[NV];
 #pragma unroll

@@ -136,6 +136,8 @@ struct Model {
   void* mc_rep_d = nullptr;  // replica sweeps: per-replica parameters and acceptance counters
   unsigned long long* mc_repacc_d = nullptr;
   int mc_rep_cap = 0;
+  double* img_part_d = nullptr;  // partials of image-stack observables
+  int img_part_cap = 0;          // images they hold
   // captured step loop (hipGraph of graph_steps Heun steps + counter advance), keyed by what it bakes in
   int graph_steps = 0;
   uint64_t* counter_d = nullptr;
@@ -155,8 +157,11 @@ int launch_heun_pass(const Model& m, int pass, void* e, void* ep, void* k1, uint
                      void* save_e, void* save_B, hipStream_t s, const double* B_ext = nullptr);
 int sync_model_static(Model& m);
 size_t prec_size(const Model& m);
-int launch_field(const Model& m, const void* e_int, void* out_ref, int kind, hipStream_t s);
-int launch_observables(const Model& m, const void* e_int, double* out, hipStream_t s);
+// nimg images of N x 3 values each, one after another (field: in and out; observables: nimg x MX_OBS_LEN
+// out, partials in `part`, nimg x blocks x MX_OBS_LEN, default the model's own for one image).
+int launch_field(const Model& m, const void* e_int, void* out_ref, int kind, hipStream_t s, int nimg = 1);
+int launch_observables(const Model& m, const void* e_int, double* out, hipStream_t s, int nimg = 1,
+                       double* part = nullptr);
 int launch_noise(const Model& m, void* out_ref, uint64_t seed, uint64_t step, hipStream_t s);
 int launch_permute(const Model& m, const void* src, void* dst, int to_internal, hipStream_t s);
 bool needs_permute(const Model& m);

@@ -120,7 +120,11 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S,
 // kind: MX_FIELD_BEFF (T), MX_FIELD_GRADIENT (meV), MX_FIELD_EXCHANGE (meV); output in magex order.
 template <typename R, int CM, int GEO>
 __global__ __launch_bounds__(BLOCK) void k_field(const Static<R>* __restrict__ S, const Dyn<R> d,
-                                                 const R* __restrict__ e, R* __restrict__ out, int kind) {
+                                                 const R* __restrict__ e, R* __restrict__ out, int kind,
+                                                 size_t istride) {
+  // image blockIdx.y of a stack of spin configurations istride values apart (one image: y = 0)
+  e += blockIdx.y * istride;
+  out += blockIdx.y * istride;
   Site st;
   if (!site_of<R, GEO>(S, st)) return;
   const int i = st.i, o = 3 * (st.crm * S->ns + st.s);
@@ -156,7 +160,9 @@ __global__ __launch_bounds__(BLOCK) void k_field(const Static<R>* __restrict__ S
 // llg.py:250-252).
 template <typename R, int CM, int GEO>
 __global__ __launch_bounds__(BLOCK) void k_obs(const Static<R>* __restrict__ S, const Dyn<R> d,
-                                               const R* __restrict__ e, double* __restrict__ part) {
+                                               const R* __restrict__ e, double* __restrict__ part, size_t istride) {
+  e += blockIdx.y * istride;
+  part += size_t(blockIdx.y) * gridDim.x * MX_OBS_LEN;
   double v[MX_OBS_LEN] = {0, 0, 0, 0, 0, 0, 0, 0};
   Site st;
   if (site_of<R, GEO>(S, st)) {
@@ -434,38 +440,41 @@ int launch_counter(uint64_t* c, uint64_t value, int add, hipStream_t s) {
 }
 
 template <typename R>
-static int field_t(const Model& m, const void* e, void* out, int kind, hipStream_t s) {
+static int field_t(const Model& m, const void* e, void* out, int kind, int nimg, hipStream_t s) {
   const Static<R>* S = static_cast<const Static<R>*>(m.static_d);
   const Dyn<R> d = make_dyn<R>(m, 0);
   const R* E = static_cast<const R*>(e);
   R* O = static_cast<R*>(out);
-  const dim3 g(nblk_of(m)), b(BLOCK);
-#define MX_FIELD_CALL(CM, GEO) hipLaunchKernelGGL((k_field<R, CM, GEO>), g, b, 0, s, S, d, E, O, kind)
+  const size_t istride = size_t(m.N) * 3;
+  const dim3 g(nblk_of(m), nimg), b(BLOCK);
+#define MX_FIELD_CALL(CM, GEO) hipLaunchKernelGGL((k_field<R, CM, GEO>), g, b, 0, s, S, d, E, O, kind, istride)
   MX_DISPATCH(m, MX_FIELD_CALL);
 #undef MX_FIELD_CALL
   return check_launch("field");
 }
 
-int launch_field(const Model& m, const void* e, void* out, int kind, hipStream_t s) {
-  return m.prec == MX_FP64 ? field_t<double>(m, e, out, kind, s) : field_t<float>(m, e, out, kind, s);
+int launch_field(const Model& m, const void* e, void* out, int kind, hipStream_t s, int nimg) {
+  return m.prec == MX_FP64 ? field_t<double>(m, e, out, kind, nimg, s) : field_t<float>(m, e, out, kind, nimg, s);
 }
 
 template <typename R>
-static int obs_t(const Model& m, const void* e, double* out, hipStream_t s) {
+static int obs_t(const Model& m, const void* e, double* out, int nimg, double* part, hipStream_t s) {
   const Static<R>* S = static_cast<const Static<R>*>(m.static_d);
   const Dyn<R> d = make_dyn<R>(m, 0);
   const R* E = static_cast<const R*>(e);
   const int nb = nblk_of(m);
-  const dim3 g(nb), b(BLOCK);
-#define MX_OBS_CALL(CM, GEO) hipLaunchKernelGGL((k_obs<R, CM, GEO>), g, b, 0, s, S, d, E, m.part_d)
+  const size_t istride = size_t(m.N) * 3;
+  const dim3 g(nb, nimg), b(BLOCK);
+#define MX_OBS_CALL(CM, GEO) hipLaunchKernelGGL((k_obs<R, CM, GEO>), g, b, 0, s, S, d, E, part, istride)
   MX_DISPATCH(m, MX_OBS_CALL);
 #undef MX_OBS_CALL
-  hipLaunchKernelGGL((k_final<MX_OBS_LEN>), dim3(1), dim3(BLOCK), 0, s, m.part_d, nb, 7, 1.0, out);
+  hipLaunchKernelGGL((k_final<MX_OBS_LEN>), dim3(nimg), dim3(BLOCK), 0, s, part, nb, 7, 1.0, out);
   return check_launch("observables");
 }
 
-int launch_observables(const Model& m, const void* e, double* out, hipStream_t s) {
-  return m.prec == MX_FP64 ? obs_t<double>(m, e, out, s) : obs_t<float>(m, e, out, s);
+int launch_observables(const Model& m, const void* e, double* out, hipStream_t s, int nimg, double* part) {
+  if (!part) part = m.part_d;
+  return m.prec == MX_FP64 ? obs_t<double>(m, e, out, nimg, part, s) : obs_t<float>(m, e, out, nimg, part, s);
 }
 
 int launch_noise(const Model& m, void* out, uint64_t seed, uint64_t step, hipStream_t s) {

@@ -504,3 +504,39 @@ def test_welch_peak_of_a_precessing_mode():
     dE = E[1] - E[0]
     assert abs(abs(E[np.argmax(Sp[0])]) - R.HBAR_MEV_PS * w) <= dE
     assert Sl.max() < 1e-20
+
+
+# ------------------------------------------------------------------ GNEB restatement
+def _sw_spin(K, h, mu=2.0):
+    """One spin, easy axis z (A = -K(zz - I/3)), field along +z with h = mu mu_B B / (2K)."""
+    B = 2 * K * h / (mu * R.MU_B)
+    s = _sys((1, 1, 1), field=(0, 0, B), pbc=(False, False, False), lattice="sc", J=[1.0], K=K, moment=mu)
+    return s
+
+
+def test_geodesic_of_a_uniform_rotation():
+    from oracle import gneb_ref as G
+    rng = np.random.default_rng(0)
+    a = R.normalize(rng.normal(size=(50, 3)))
+    th = 0.3
+    b = R.rotate(a, np.tile([0, 0, 1.0], (50, 1)) * th)
+    perp = np.linalg.norm(a[:, :2], axis=1)
+    ang = np.arccos(np.clip((a * b).sum(1), -1, 1))
+    assert abs(G.geodesic(a, b) - np.sqrt(np.sum(ang**2))) < 1e-12
+    c = np.tile([1.0, 0, 0], (9, 1))
+    assert abs(G.geodesic(c, R.rotate(c, np.tile([0, 0, th], (9, 1)))) - 3 * th) < 1e-12
+    assert perp.min() >= 0
+
+
+def test_gneb_stoner_wohlfarth_saddle():
+    """Climbing-image GNEB between +z and -z of a uniaxial spin in a field: barriers K (1 +- h)^2 and
+    the saddle at cos(theta) = -h."""
+    from oracle import gneb_ref as G
+    K, h = 0.5, 0.3
+    s = _sw_spin(K, h)
+    th = np.linspace(0, np.pi, 9)
+    images = [np.array([[np.sin(t) * np.cos(0.4), np.sin(t) * np.sin(0.4), np.cos(t)]]) for t in th]
+    x, E, ci, it = G.run(s, images, rel_tol=1e-8, max_iter=5000)
+    assert abs((E[ci] - E[0]) - K * (1 + h) ** 2) < 1e-6
+    assert abs((E[ci] - E[-1]) - K * (1 - h) ** 2) < 1e-6
+    assert abs(x[ci][0, 2] + h) < 1e-4
