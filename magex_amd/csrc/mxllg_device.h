@@ -7,9 +7,13 @@ namespace mx {
 
 // The dispatcher deals workgroups round-robin to the 8 XCDs (XCD = block mod 8, checked on MI355X);
 // this map gives XCD x the contiguous logical tiles [x G/8, (x+1) G/8), so each XCD's L2 sees one slab.
-__device__ __forceinline__ int xcd_tile(int b, int G) {
+// With rev, each XCD walks its slab from the far end: a kernel that follows one which walked forward
+// starts on the tiles whose data the previous kernel left in that XCD's L2 (L2 contents survive a
+// kernel boundary), so alternating directions saves about one L2 of refetch per launch.
+__device__ __forceinline__ int xcd_tile(int b, int G, bool rev = false) {
   const int x = b & 7, q = b >> 3, per = G >> 3, rem = G & 7;
-  return x * per + min(x, rem) + q;
+  const int cnt = per + (x < rem ? 1 : 0);
+  return x * per + min(x, rem) + (rev ? cnt - 1 - q : q);
 }
 
 // Philox4x32-10 (Salmon et al., SC'11): counter (c0..c3), key (k0, k1).
@@ -17,8 +21,10 @@ __device__ __forceinline__ void philox10(uint32_t& c0, uint32_t& c1, uint32_t& c
                                          uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-    const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-    const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+    // one 32 x 32 -> 64-bit product per multiplier (v_mad_u64_u32) gives both halves
+    const uint64_t p0 = uint64_t(0xD2511F53u) * c0, p1 = uint64_t(0xCD9E8D57u) * c2;
+    const uint32_t lo0 = uint32_t(p0), hi0 = uint32_t(p0 >> 32);
+    const uint32_t lo1 = uint32_t(p1), hi1 = uint32_t(p1 >> 32);
     const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
     c0 = n0;
     c1 = lo1;

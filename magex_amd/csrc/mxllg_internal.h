@@ -42,6 +42,7 @@ struct Static {
   int pbc0, pbc1, pbc2, allpbc;
   int tb0, tb1, tb2, nt1, nt2;  // TILED: log2 of the tile sides; tiles along axes 1 and 2
   int fdiv;                     // ncell < 2^24: divide cell indices by L1, L2 through rL1, rL2
+  int xcol;                     // step kernels' column map: row-major tiles per XCD per axis-0 plane (0: slabs)
   float rL1, rL2;
   int amode, has_A, has_c, has_u;
   int sptr[MAXSUB + 1];
@@ -60,6 +61,7 @@ struct Dyn {
   int prec, noisy;
   int integ;  // MX_HEUN or MX_DEPONDT
   int sot;    // spin-orbit torque fields on: B += bdl (e x p) + bfl p
+  int rev2;   // the corrector pass walks each XCD's slab backwards (starts on the predictor's last tiles)
   R sp0, sp1, sp2, bdl, bfl;
   uint32_t seed_lo, seed_hi;
   const R* bond;  // COUP_BOND: [nfwd][ncell][9], row-major tensor per bond (36 B FP32 / 72 B FP64)

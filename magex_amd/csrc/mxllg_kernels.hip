@@ -16,7 +16,9 @@
 // Workgroups: 256 threads, one site each; a workgroup never mixes sublattices, so the stencil loop is
 // wavefront-uniform.  The block -> tile map is XCD-aware: the dispatcher deals workgroups round-robin
 // to the 8 XCDs, so tile = xcd * (G / 8) + (block / 8) gives every XCD one contiguous slab of the
-// lattice and the neighbour re-reads of that slab hit its own L2.
+// lattice and the neighbour re-reads of that slab hit its own L2.  The step kernels use per-XCD columns
+// instead where axis-0 planes split into whole tiles (site_of), and in FP64 the corrector pass walks
+// them backwards, starting on the tiles the predictor pass left in L2.
 #include <cstring>
 
 #include "mxllg_field.h"
@@ -36,7 +38,7 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S,
                                                 R* __restrict__ ep, R* __restrict__ k1, uint64_t step,
                                                 const R* __restrict__ noise) {
   Site st;
-  if (!site_of<R, GEO>(S, st)) return;
+  if (!site_of<R, GEO>(S, st, PASS == 2 && d.rev2, true)) return;
   if (d.step_ptr) step += *d.step_ptr;  // graph replays: step = device counter + offset of this node
   const int i = st.i;
   const bool pin = d.pinned && d.pinned[i];
@@ -285,6 +287,12 @@ static void fill_static(const Model& m, Static<R>& S) {
   S.fdiv = m.ncell < (1 << 24);
   S.rL1 = 1.0f / float(m.L[1]);
   S.rL2 = 1.0f / float(m.L[2]);
+  {
+    // column map of the step kernels when an axis-0 plane is a whole number of tiles, 8 or more per XCD
+    const long plane = long(m.L[1]) * m.L[2];
+    S.xcol = (m.geo != GEO_TILED && plane % BLOCK == 0 && (plane / BLOCK) % 8 == 0)
+                 ? int(plane / BLOCK / 8) : 0;
+  }
   S.amode = m.amode;
   S.has_A = m.has_A;
   S.has_c = m.has_c;
@@ -336,6 +344,8 @@ static Dyn<R> make_dyn(const Model& m, uint64_t seed) {
   d.noisy = m.noisy;
   d.integ = m.integrator;
   d.sot = (m.sot_dl != 0.0 || m.sot_fl != 0.0) ? 1 : 0;
+  // measured on bcc 64^3 per-bond: the reversed corrector walk is 7 % faster in FP64, neutral in FP32
+  d.rev2 = sizeof(R) == 8 ? 1 : 0;
   d.sp0 = R(m.sot_p[0]);
   d.sp1 = R(m.sot_p[1]);
   d.sp2 = R(m.sot_p[2]);

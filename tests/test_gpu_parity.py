@@ -220,16 +220,18 @@ def test_captured_step_loop_equals_direct_launches(gpu, precision, lattice):
     assert gpu.equal(ea, eb)
 
 
-@pytest.mark.parametrize("case", ["bcc_perbond", "tri_dmi"])
+@pytest.mark.parametrize("case", ["bcc_perbond", "bcc_columns", "tri_dmi"])
 def test_tiled_layout_equals_row_major(gpu, case):
     """The tiled cell layout (tile-major internal order) changes no result bit of the field or of 25
     thermal steps against the row-major layout (FP64 and FP32); the observables, sums taken in the
-    internal site order, agree to round-off."""
+    internal site order, agree to round-off.  bcc_columns: axis-0 planes of 32 x 64 cells (8 tiles),
+    where the row-major step kernels walk per-XCD columns instead of slabs."""
     from magex_amd.configs import random_bond_tensors, random_sia
     from magex_amd.device import DeviceModel
     print(f"case: tiled vs row-major {case}", flush=True)
-    if case == "bcc_perbond":
-        s, _ = _pair((8, 8, 16), lattice="bcc_conventional", J=[20.0, 10.0], moment=2.2)
+    if case in ("bcc_perbond", "bcc_columns"):
+        L = (8, 8, 16) if case == "bcc_perbond" else (4, 32, 64)
+        s, _ = _pair(L, lattice="bcc_conventional", J=[20.0, 10.0], moment=2.2)
         s.set_bond_tensors(random_bond_tensors(s, seed=3))
         s.A = random_sia(s.N, seed=4)
     else:
