@@ -31,6 +31,15 @@ struct Rep {
   double sig[MAXSUB];  // sigma of sublattice s: sqrt(2 alpha k_B T / (gamma mu_s mu_B dt))
   uint32_t seed_lo, seed_hi;
   uint32_t pad2[2];
+  uint32_t Q0, Q1, Q2;  // byte period of axes 0, 1, 2 in the replica's spin array
+  uint32_t pad3;
+};
+
+// Per (replica, stencil entry): byte offset from a site's spin to its neighbour's before wrapping, and
+// the cell coordinate from which each axis wraps (L_k - d_k with the offset d reduced into [0, L_k)).
+struct REnt {
+  uint32_t D;
+  int32_t th0, th1, th2;
 };
 
 struct BEnt {
@@ -55,46 +64,63 @@ struct BDyn {
 
 struct BSite {
   int s, c, c0, c1, c2, i, iref;
+  uint32_t ob;  // byte offset of the site's spin from its replica's first spin
 };
 
-// h_i over the stencil with raw offsets (|R| < L on every axis), accumulated in stencil order; each
-// batch of CH gathers is branch-free (entries past the end re-read the last one and count zero), and
-// mid() runs between the first batch's loads and their use.
+// Spin of a site in array a: the replica's base (wavefront-uniform, 64-bit) plus a 32-bit byte offset.
+template <typename R>
+__device__ __forceinline__ R* rep_at(R* a, const Rep* __restrict__ rp, uint32_t ob) {
+  return reinterpret_cast<R*>(reinterpret_cast<char*>(a + size_t(rp->off) * 3) + ob);
+}
+template <typename R>
+__device__ __forceinline__ const R* rep_at(const R* a, const Rep* __restrict__ rp, uint32_t ob) {
+  return reinterpret_cast<const R*>(reinterpret_cast<const char*>(a + size_t(rp->off) * 3) + ob);
+}
+
+// h_i over the stencil with raw offsets (|R| < L on every axis), accumulated in stencil order.  Each
+// batch of CH gathers is branch-free: every load is issued unconditionally (entries past the end re-read
+// the last one, a valid address) and a select zeroes what does not count, so the CH loads of a wavefront
+// are in flight together; mid() runs between the first batch's loads and their use.  A neighbour's
+// address needs no multiply per lane: with the offset d reduced into [0, L) per axis (scalar), its byte
+// offset is the site's own plus a wavefront-uniform delta, less one period per axis that wraps
+// (c_k >= L_k - d_k).  Offsets are 32-bit from the replica's base (a replica has fewer than 2^24 cells
+// of at most 8 sites of 24 B).
 template <typename R, int CM, typename Mid>
 __device__ __forceinline__ void bexchange(const BStatic<R>* __restrict__ S, const Rep* __restrict__ rp,
-                                          const R* __restrict__ e, const BSite& st, R& hx, R& hy, R& hz, Mid&& mid) {
+                                          const REnt* __restrict__ re, const R* __restrict__ e, const BSite& st,
+                                          R& hx, R& hy, R& hz, Mid&& mid) {
   constexpr int CH = 8;
   hx = hy = hz = R(0);
-  const int kb = S->sptr[st.s], ke = S->sptr[st.s + 1], ns = S->ns;
-  const int L0 = rp->L0, L1 = rp->L1, L2 = rp->L2, off = rp->off;
+  const int kb = S->sptr[st.s], ke = S->sptr[st.s + 1];
+  const uint32_t Q0 = rp->Q0, Q1 = rp->Q1, Q2 = rp->Q2;
   if (kb >= ke) {
     mid();
     return;
   }
+  const char* eb = reinterpret_cast<const char*>(e + size_t(rp->off) * 3);
   for (int k0 = kb; k0 < ke; k0 += CH) {
     R ex[CH], ey[CH], ez[CH];
     int slot[CH];
+    bool live[CH];
 #pragma unroll
     for (int u = 0; u < CH; ++u) {
-      const bool live = k0 + u < ke;
-      const BEnt en = S->ent[live ? k0 + u : ke - 1];
-      slot[u] = en.slot;
-      int n0 = st.c0 + int(int16_t(en.r01 & 0xffff)), n1 = st.c1 + (en.r01 >> 16), n2 = st.c2 + en.r2;
-      n0 += n0 < 0 ? L0 : 0;
-      n0 -= n0 >= L0 ? L0 : 0;
-      n1 += n1 < 0 ? L1 : 0;
-      n1 -= n1 >= L1 ? L1 : 0;
-      n2 += n2 < 0 ? L2 : 0;
-      n2 -= n2 >= L2 ? L2 : 0;
-      const int j = off + ((n0 * L1 + n1) * L2 + n2) * ns + en.t;
-      ex[u] = live ? e[3 * j] : R(0);
-      ey[u] = live ? e[3 * j + 1] : R(0);
-      ez[u] = live ? e[3 * j + 2] : R(0);
+      live[u] = k0 + u < ke;
+      const int k = min(k0 + u, ke - 1);
+      slot[u] = S->ent[k].slot;
+      const REnt q = re[k];
+      uint32_t o = st.ob + q.D;
+      o -= st.c0 >= q.th0 ? Q0 : 0u;
+      o -= st.c1 >= q.th1 ? Q1 : 0u;
+      o -= st.c2 >= q.th2 ? Q2 : 0u;
+      const R* p = reinterpret_cast<const R*>(eb + o);
+      ex[u] = p[0];
+      ey[u] = p[1];
+      ez[u] = p[2];
     }
     if (k0 == kb) mid();
 #pragma unroll
     for (int u = 0; u < CH; ++u) {
-      const R x = ex[u], y = ey[u], z = ez[u];
+      const R x = live[u] ? ex[u] : R(0), y = live[u] ? ey[u] : R(0), z = live[u] ? ez[u] : R(0);
       if constexpr (CM == COUP_TABLE1) {
         const R Jv = S->tab[slot[u]];
         hx += Jv * x;
@@ -117,8 +143,9 @@ __device__ __forceinline__ void bexchange(const BStatic<R>* __restrict__ S, cons
 
 template <typename R, int CM>
 __device__ __forceinline__ void bexchange(const BStatic<R>* __restrict__ S, const Rep* __restrict__ rp,
-                                          const R* __restrict__ e, const BSite& st, R& hx, R& hy, R& hz) {
-  bexchange<R, CM>(S, rp, e, st, hx, hy, hz, [] {});
+                                          const REnt* __restrict__ re, const R* __restrict__ e, const BSite& st,
+                                          R& hx, R& hy, R& hz) {
+  bexchange<R, CM>(S, rp, re, e, st, hx, hy, hz, [] {});
 }
 
 template <typename R>
@@ -148,36 +175,41 @@ __device__ __forceinline__ bool bsite(const BStatic<R>* __restrict__ S, const Re
   st.c1 = t - st.c0 * L1;
   st.iref = st.c * ns + st.s;
   st.i = rp->off + st.iref;
+  st.ob = uint32_t(st.iref) * uint32_t(3 * sizeof(R));
   return true;
 }
 
 template <typename R, int CM, int BS, int PASS>
 __global__ __launch_bounds__(BS) void k_bheun(const BStatic<R>* __restrict__ S, const Rep* __restrict__ reps,
-                                              const int32_t* __restrict__ tile_rep, const BDyn<R> d,
-                                              R* __restrict__ e, R* __restrict__ ep, R* __restrict__ k1,
-                                              R* __restrict__ bn, uint64_t step, int spt,
+                                              const int32_t* __restrict__ tile_rep, const REnt* __restrict__ rent,
+                                              int nent, const BDyn<R> d, R* __restrict__ e, R* __restrict__ ep,
+                                              R* __restrict__ k1, R* __restrict__ bn, uint64_t step, int spt,
                                               const uint64_t* __restrict__ step_ptr) {
   const int b = xcd_tile(blockIdx.x, gridDim.x);
-  const Rep* __restrict__ rp = reps + tile_rep[b];  // descriptor read through the pointer: scalar loads
+  const int r = tile_rep[b];
+  const Rep* __restrict__ rp = reps + r;  // descriptor read through the pointer: scalar loads
+  const REnt* __restrict__ re = rent + size_t(r) * nent;
   if (step_ptr) step += *step_ptr;
   const int lt = b - rp->tile0;
   const R a = R(rp->alpha);
   for (int u = 0; u < spt; ++u) {
     BSite st;
     if (!bsite<R, BS>(S, rp, lt, spt, u, st)) return;
-    const int i = st.i;
     const R* src = PASS == 1 ? e : ep;
-    const R x = src[3 * i], y = src[3 * i + 1], z = src[3 * i + 2];
+    const R* sp = rep_at(src, rp, st.ob);
+    R* bnp = rep_at(bn, rp, st.ob);
+    R* k1p = rep_at(k1, rp, st.ob);
+    const R x = sp[0], y = sp[1], z = sp[2];
     R hx, hy, hz, gx, gy, gz, nx = R(0), ny = R(0), nz = R(0);
     // the thermal field is drawn in pass 1 (while the first neighbour batch is in flight) and kept for
     // pass 2: the batched kernels are bound by arithmetic more than by bytes, and the generator is the
     // largest part of it
     if (PASS == 2 && d.noisy) {
-      nx = bn[3 * i];
-      ny = bn[3 * i + 1];
-      nz = bn[3 * i + 2];
+      nx = bnp[0];
+      ny = bnp[1];
+      nz = bnp[2];
     }
-    bexchange<R, CM>(S, rp, src, st, hx, hy, hz, [&] {
+    bexchange<R, CM>(S, rp, re, src, st, hx, hy, hz, [&] {
       if (PASS == 1 && d.noisy) {
         float n0, n1, n2;
         gauss3(rp->seed_lo, rp->seed_hi, step, uint32_t(st.iref), n0, n1, n2);
@@ -185,9 +217,9 @@ __global__ __launch_bounds__(BS) void k_bheun(const BStatic<R>* __restrict__ S, 
         nx = sg * R(n0);
         ny = sg * R(n1);
         nz = sg * R(n2);
-        bn[3 * i] = nx;
-        bn[3 * i + 1] = ny;
-        bn[3 * i + 2] = nz;
+        bnp[0] = nx;
+        bnp[1] = ny;
+        bnp[2] = nz;
       }
     });
     bonsite(S, st.s, x, y, z, gx, gy, gz);
@@ -199,22 +231,24 @@ __global__ __launch_bounds__(BS) void k_bheun(const BStatic<R>* __restrict__ S, 
     const R hdt = R(0.5) * d.dt;
     if (PASS == 1) {
       // k1 keeps w = e + dt/2 k1, so pass 2 does not read e again
-      k1[3 * i] = x + hdt * fx;
-      k1[3 * i + 1] = y + hdt * fy;
-      k1[3 * i + 2] = z + hdt * fz;
+      k1p[0] = x + hdt * fx;
+      k1p[1] = y + hdt * fy;
+      k1p[2] = z + hdt * fz;
       const R px = x + d.dt * fx, py = y + d.dt * fy, pz = z + d.dt * fz;
       const R q = rnorm3(px, py, pz);
-      ep[3 * i] = px * q;
-      ep[3 * i + 1] = py * q;
-      ep[3 * i + 2] = pz * q;
+      R* o = rep_at(ep, rp, st.ob);
+      o[0] = px * q;
+      o[1] = py * q;
+      o[2] = pz * q;
     } else {
-      const R cx = k1[3 * i] + hdt * fx;
-      const R cy = k1[3 * i + 1] + hdt * fy;
-      const R cz = k1[3 * i + 2] + hdt * fz;
+      const R cx = k1p[0] + hdt * fx;
+      const R cy = k1p[1] + hdt * fy;
+      const R cz = k1p[2] + hdt * fz;
       const R q = rnorm3(cx, cy, cz);
-      e[3 * i] = cx * q;
-      e[3 * i + 1] = cy * q;
-      e[3 * i + 2] = cz * q;
+      R* o = rep_at(e, rp, st.ob);
+      o[0] = cx * q;
+      o[1] = cy * q;
+      o[2] = cz * q;
     }
   }
 }
@@ -223,10 +257,13 @@ __global__ __launch_bounds__(BS) void k_bheun(const BStatic<R>* __restrict__ S, 
 // that a replica's partials are contiguous.
 template <typename R, int CM, int BS>
 __global__ __launch_bounds__(BS) void k_bobs(const BStatic<R>* __restrict__ S, const Rep* __restrict__ reps,
-                                             const int32_t* __restrict__ tile_rep, const BDyn<R> d,
-                                             const R* __restrict__ e, int spt, double* __restrict__ part) {
+                                             const int32_t* __restrict__ tile_rep, const REnt* __restrict__ rent,
+                                             int nent, const BDyn<R> d, const R* __restrict__ e, int spt,
+                                             double* __restrict__ part) {
   const int b = xcd_tile(blockIdx.x, gridDim.x);
-  const Rep* __restrict__ rp = reps + tile_rep[b];
+  const int r = tile_rep[b];
+  const Rep* __restrict__ rp = reps + r;
+  const REnt* __restrict__ re = rent + size_t(r) * nent;
   const int lt = b - rp->tile0;
   double v[MX_OBS_LEN] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int u = 0; u < spt; ++u) {
@@ -235,7 +272,7 @@ __global__ __launch_bounds__(BS) void k_bobs(const BStatic<R>* __restrict__ S, c
     const int i = st.i;
     const R x = e[3 * i], y = e[3 * i + 1], z = e[3 * i + 2];
     R hx, hy, hz, gx, gy, gz;
-    bexchange<R, CM>(S, rp, e, st, hx, hy, hz);
+    bexchange<R, CM>(S, rp, re, e, st, hx, hy, hz);
     bonsite(S, st.s, x, y, z, gx, gy, gz);
     const R im = S->inv_mu_sub[st.s];
     const R Eon = (gx * x + gy * y + gz * z) * R(0.5);
@@ -296,6 +333,8 @@ struct mx_batch {
   int ntiles = 0;
   void *static_d = nullptr, *reps_d = nullptr, *ep_d = nullptr, *k1_d = nullptr;
   void* bn_d = nullptr;  // thermal field of the current step, drawn in pass 1 and reused in pass 2
+  void* rent_d = nullptr;  // REnt per (replica, stencil entry)
+  int nent = 0;
   int32_t* tile_rep_d = nullptr;
   double* part_d = nullptr;
   size_t part_cap = 0;
@@ -359,6 +398,36 @@ void build_static(mx_batch& b, const Model& m) {
   b.static_h.assign(reinterpret_cast<char*>(&S), reinterpret_cast<char*>(&S) + sizeof(S));
 }
 
+// Byte periods of every replica and the REnt table (internal stencil order: entries grouped by source
+// sublattice), uploaded once: the neighbour address arithmetic of the kernels is then one table read
+// and three conditional subtractions per entry.
+void build_rent(mx_batch& b, const Model& m) {
+  const uint32_t W = uint32_t(3 * rsz(b));
+  b.nent = m.nent;
+  std::vector<REnt> t(size_t(b.nrep) * std::max(m.nent, 1));
+  for (int r = 0; r < b.nrep; ++r) {
+    Rep& p = b.reps[r];
+    const int L[3] = {p.L0, p.L1, p.L2};
+    p.Q2 = uint32_t(p.L2) * uint32_t(m.ns) * W;
+    p.Q1 = p.Q2 * uint32_t(p.L1);
+    p.Q0 = p.Q1 * uint32_t(p.L0);
+    for (int s = 0; s < m.ns; ++s)
+      for (int k = m.sptr[s]; k < m.sptr[s + 1]; ++k) {
+        const int* raw = &m.ent_raw[6 * size_t(m.ent_order[k])];
+        int dd[3] = {raw[2], raw[3], raw[4]};
+        for (int a = 0; a < 3; ++a) dd[a] = ((dd[a] % L[a]) + L[a]) % L[a];
+        const int64_t cell = (int64_t(dd[0]) * p.L1 + dd[1]) * p.L2 + dd[2];
+        REnt& q = t[size_t(r) * m.nent + k];
+        q.D = uint32_t(uint64_t(cell * m.ns + raw[1] - s) * W);  // modulo 2^32, as the kernels add it
+        q.th0 = p.L0 - dd[0];
+        q.th1 = p.L1 - dd[1];
+        q.th2 = p.L2 - dd[2];
+      }
+  }
+  bok(hipMalloc(&b.rent_d, sizeof(REnt) * t.size()), "hipMalloc stencil table");
+  bok(hipMemcpy(b.rent_d, t.data(), sizeof(REnt) * t.size(), hipMemcpyHostToDevice), "hipMemcpy stencil table");
+}
+
 // Workgroup layout for the current launch shape: tile_rep[logical workgroup] = replica.
 void layout(mx_batch& b) {
   std::vector<int32_t> map;
@@ -396,10 +465,11 @@ void launch_pair(mx_batch& b, R* e, uint64_t step, const uint64_t* step_ptr, hip
   R* ep = static_cast<R*>(b.ep_d);
   R* k1 = static_cast<R*>(b.k1_d);
   R* bn = static_cast<R*>(b.bn_d);
-  hipLaunchKernelGGL((k_bheun<R, CM, BS, 1>), dim3(b.ntiles), dim3(BS), 0, s, S, reps, b.tile_rep_d, d, e, ep, k1,
-                     bn, step, b.spt, step_ptr);
-  hipLaunchKernelGGL((k_bheun<R, CM, BS, 2>), dim3(b.ntiles), dim3(BS), 0, s, S, reps, b.tile_rep_d, d, e, ep, k1,
-                     bn, step, b.spt, step_ptr);
+  const REnt* rent = static_cast<const REnt*>(b.rent_d);
+  hipLaunchKernelGGL((k_bheun<R, CM, BS, 1>), dim3(b.ntiles), dim3(BS), 0, s, S, reps, b.tile_rep_d, rent, b.nent, d,
+                     e, ep, k1, bn, step, b.spt, step_ptr);
+  hipLaunchKernelGGL((k_bheun<R, CM, BS, 2>), dim3(b.ntiles), dim3(BS), 0, s, S, reps, b.tile_rep_d, rent, b.nent, d,
+                     e, ep, k1, bn, step, b.spt, step_ptr);
 }
 
 template <typename R, int CM>
@@ -432,7 +502,8 @@ template <typename R, int CM, int BS>
 void obs_launch(mx_batch& b, const R* e, hipStream_t s) {
   BDyn<R> d{R(b.B[0]), R(b.B[1]), R(b.B[2]), R(b.dt), b.precession, b.noisy};
   hipLaunchKernelGGL((k_bobs<R, CM, BS>), dim3(b.ntiles), dim3(BS), 0, s, static_cast<const BStatic<R>*>(b.static_d),
-                     static_cast<const Rep*>(b.reps_d), b.tile_rep_d, d, e, b.spt, b.part_d);
+                     static_cast<const Rep*>(b.reps_d), b.tile_rep_d, static_cast<const REnt*>(b.rent_d), b.nent, d,
+                     e, b.spt, b.part_d);
 }
 
 template <typename R, int CM>
@@ -515,6 +586,7 @@ int mx_batch_create(const mx_model* proto, int32_t nrep, const int32_t* L, mx_ba
       bok(hipSetDevice(b->device), "hipSetDevice");
       if (b->prec == MX_FP64) build_static<double>(*b, *proto);
       else build_static<float>(*b, *proto);
+      build_rent(*b, *proto);
       bok(hipMalloc(&b->static_d, b->static_h.size()), "hipMalloc batch constants");
       bok(hipMemcpy(b->static_d, b->static_h.data(), b->static_h.size(), hipMemcpyHostToDevice), "hipMemcpy constants");
       bok(hipMalloc(&b->ep_d, size_t(b->total) * 3 * rsz(*b)), "hipMalloc ep");
@@ -535,7 +607,7 @@ void mx_batch_destroy(mx_batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->device);
   drop_bgraph(*b);
-  for (void** p : {&b->static_d, &b->reps_d, &b->ep_d, &b->k1_d, &b->bn_d, reinterpret_cast<void**>(&b->tile_rep_d),
+  for (void** p : {&b->static_d, &b->reps_d, &b->rent_d, &b->ep_d, &b->k1_d, &b->bn_d, reinterpret_cast<void**>(&b->tile_rep_d),
                    reinterpret_cast<void**>(&b->part_d), reinterpret_cast<void**>(&b->counter_d)})
     bfree(*p);
   if (b->cap) (void)hipStreamDestroy(b->cap);
