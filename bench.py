@@ -202,6 +202,9 @@ def config_rates():
                                  "roofline_frac": alg_bytes(s, prec) * s.N / (r["step_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
     out["tri256_cpu_spin_steps_per_s"] = c
     out["tri256_cpu_threads"] = th
+    # random start at T = 20 K forms a skyrmion texture: round-off grows through that dynamics, so the
+    # angle error is given at two lengths (the charge Q is the same in both precisions)
+    out["tri256_fp32_vs_fp64"] = [fp32_angle_error(s, p, steps=n) for n in (200, 2000)]
     return out
 
 
@@ -268,13 +271,26 @@ def batch_rates(steps, per_replica_steps):
     rate = total * steps / wall
     base, _ = per_replica_rate(systems, alpha, p["dt"], T, seeds, spins, per_replica_steps)
     m = batch.magnetization(e)
+    # per-replica |M| of FP32 against FP64 (the FP64 kernels match the oracle to 1e-10) after 300 steps
+    # from the same start on the same noise
+    mags = {}
+    for prec in ("fp32", "fp64"):
+        bp = ReplicaBatch(systems, alpha, p["dt"], T, seeds, prec)
+        ep = bp.pack(spins)
+        bp.run(ep, 300)
+        mags[prec] = bp.magnetization(ep)
+        del bp, ep
+    dm = np.abs(mags["fp32"] - mags["fp64"])
     return {"workload": "96 triangular replicas, L in {32..192} (seeded), J=1 K=0.1 meV B=0.5 T, T 5-300 K, "
                         "alpha=0.1, dt=0.1 fs, FP32", "spins": total, "steps": steps,
             "spin_steps_per_s": rate, "ms_per_step": 1e3 * wall / steps,
             "event_ms_per_step": t0.elapsed_time(t1) / steps, "launch": {"block": shape[0],
             "sites_per_thread": shape[1], "graph_steps": shape[2]},
             "per_replica_spin_steps_per_s": base, "per_replica_steps": per_replica_steps,
-            "speedup_vs_per_replica": rate / base, "mean_abs_m_range": [float(m.min()), float(m.max())]}
+            "speedup_vs_per_replica": rate / base, "mean_abs_m_range": [float(m.min()), float(m.max())],
+            "abs_m_fp32_vs_fp64_300_steps": {"max_abs_diff": float(dm.max()), "mean_abs_diff": float(dm.mean()),
+                                             "abs_m_fp64_range": [float(mags["fp64"].min()),
+                                                                  float(mags["fp64"].max())]}}
 
 
 def adjoint_rates(steps=500, checkpoint_every=50, reps=3):

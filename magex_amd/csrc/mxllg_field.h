@@ -20,12 +20,12 @@ struct Site {
 template <typename R, int GEO>
 __device__ __forceinline__ bool site_of(const Static<R>* __restrict__ S, Site& st, bool rev = false,
                                         bool col = false) {
-  const int ncell = S->ncell, L1 = S->L1, L2 = S->L2;
+  // every header field is read before the first branch, so they arrive in one round of scalar loads
+  const int ncell = S->ncell, L1 = S->L1, L2 = S->L2, ns = S->ns, xcol = S->xcol, use_fdiv = S->fdiv;
+  const float rL1 = S->rL1, rL2 = S->rL2;
   // sublattices interleave tile by tile, so one XCD works on every sublattice of its slab of cells and
   // the partner reads of per-bond tensors stored by another sublattice hit the same L2
-  const int ns = S->ns;
   int tile;
-  const int xcol = S->xcol;
   if (GEO != GEO_TILED && col && xcol > 0) {
     // every XCD gets exactly G / 8 workgroups: planes x 8 k tiles x ns
     const int x = blockIdx.x & 7, cnt = gridDim.x >> 3, q0 = blockIdx.x >> 3;
@@ -51,10 +51,10 @@ __device__ __forceinline__ bool site_of(const Static<R>* __restrict__ S, Site& s
   } else {
     st.c = tile * BLOCK + threadIdx.x;
     if (st.c >= ncell) return false;
-    if (S->fdiv) {
-      const int t = fdiv(st.c, L2, S->rL2);
+    if (use_fdiv) {
+      const int t = fdiv(st.c, L2, rL2);
       st.c2 = st.c - t * L2;
-      st.c0 = fdiv(t, L1, S->rL1);
+      st.c0 = fdiv(t, L1, rL1);
       st.c1 = t - st.c0 * L1;
     } else {
       st.c2 = st.c % L2;
@@ -231,7 +231,9 @@ __device__ __forceinline__ void exchange(const Static<R>* __restrict__ S, const 
 }
 
 // On-site gradient dE/de (magex/fast.py:39-64): 2 A e + cubic + higher uniaxial.
-template <typename R>
+// LEAN (here and below): compiled for the plain model -- no cubic or higher-uniaxial terms, uniform
+// moments and temperatures per sublattice, generated noise, no spin-orbit torque fields.
+template <typename R, bool LEAN = false>
 __device__ __forceinline__ void onsite_grad(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st, R x, R y,
                                             R z, R& gx, R& gy, R& gz) {
   gx = gy = gz = R(0);
@@ -274,6 +276,7 @@ __device__ __forceinline__ void onsite_grad(const Static<R>* __restrict__ S, con
     gy = R(2) * (a[3 * N] * x + a[4 * N] * y + a[5 * N] * z);
     gz = R(2) * (a[6 * N] * x + a[7 * N] * y + a[8 * N] * z);
   }
+  if (LEAN) return;
   if (S->has_c) {
     const R* C = S->C;
     const R a1 = C[0] * x + C[1] * y + C[2] * z;
@@ -336,12 +339,12 @@ __device__ __forceinline__ R onsite_energy(const Static<R>* __restrict__ S, cons
   return E;
 }
 
-template <typename R>
+template <typename R, bool LEAN = false>
 __device__ __forceinline__ void thermal(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st,
                                         uint64_t step, const R* __restrict__ noise, R& bx, R& by, R& bz) {
-  const R sg = d.sigma ? d.sigma[st.i] : S->sigma_sub[st.s];
+  const R sg = (!LEAN && d.sigma) ? d.sigma[st.i] : S->sigma_sub[st.s];
   const int iref = st.crm * S->ns + st.s;
-  if (noise) {
+  if (!LEAN && noise) {
     bx = sg * noise[3 * iref];
     by = sg * noise[3 * iref + 1];
     bz = sg * noise[3 * iref + 2];
@@ -354,27 +357,27 @@ __device__ __forceinline__ void thermal(const Static<R>* __restrict__ S, const D
   }
 }
 
-template <typename R>
+template <typename R, bool LEAN = false>
 __device__ __forceinline__ R inv_mu_of(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st) {
-  return d.inv_mu ? d.inv_mu[st.i] : S->inv_mu_sub[st.s];
+  return (!LEAN && d.inv_mu) ? d.inv_mu[st.i] : S->inv_mu_sub[st.s];
 }
 
 // B_i = (2 h_i - g_i) / (mu_i mu_B) + B (+ thermal) at spin (x, y, z) (magex/fast.py:309-317).  The
 // generator's normals are computed while the first neighbour batch is in flight.
-template <typename R, int CM, int GEO>
+template <typename R, int CM, int GEO, bool LEAN = false>
 __device__ __forceinline__ void local_field(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
                                             const Site& st, R x, R y, R z, uint64_t step,
                                             const R* __restrict__ noise, R& bx, R& by, R& bz) {
   R hx, hy, hz, gx, gy, gz, nx = R(0), ny = R(0), nz = R(0);
   exchange<R, CM, GEO>(S, d, e, st, hx, hy, hz, [&] {
-    if (d.noisy) thermal(S, d, st, step, noise, nx, ny, nz);
+    if (d.noisy) thermal<R, LEAN>(S, d, st, step, noise, nx, ny, nz);
   });
-  onsite_grad(S, d, st, x, y, z, gx, gy, gz);
-  const R im = inv_mu_of(S, d, st);
+  onsite_grad<R, LEAN>(S, d, st, x, y, z, gx, gy, gz);
+  const R im = inv_mu_of<R, LEAN>(S, d, st);
   bx = (R(2) * hx - gx) * im + d.B0 + nx;
   by = (R(2) * hy - gy) * im + d.B1 + ny;
   bz = (R(2) * hz - gz) * im + d.B2 + nz;
-  if (d.sot) {
+  if (!LEAN && d.sot) {
     // spin-orbit (Slonczewski-like) torque as fields: B_DL (e x p) + B_FL p (magex/llg.py:126-128)
     bx += d.bdl * (y * d.sp2 - z * d.sp1) + d.bfl * d.sp0;
     by += d.bdl * (z * d.sp0 - x * d.sp2) + d.bfl * d.sp1;

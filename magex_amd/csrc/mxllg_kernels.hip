@@ -33,7 +33,12 @@ namespace mx {
 // Depondt (d.integ == MX_DEPONDT, the same passes' depondt branches): pass 1 keeps the rotation vector
 //         w1 = e x f(e) in k1 and rotates ep = R(dt w1) e; pass 2 rotates e = n(R(dt/2 (w1 + w2)) e) with
 //         w2 = ep x f(ep).
-template <typename R, int CM, int GEO, int PASS>
+//
+// LEAN: the same passes compiled for the plain model (Heun, nothing pinned, damping, moments and
+// temperatures uniform per sublattice, no cubic or higher-uniaxial terms, no spin-orbit torque, generated
+// noise, no adjoint stores): the host picks it when the model is that, and its shorter code and scalar
+// load chain matter on lattices too small to hide them.  Same arithmetic, so the same result bits.
+template <typename R, int CM, int GEO, int PASS, bool LEAN = false>
 __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S, const Dyn<R> d, R* __restrict__ e,
                                                 R* __restrict__ ep, R* __restrict__ k1, uint64_t step,
                                                 const R* __restrict__ noise) {
@@ -41,8 +46,8 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S,
   if (!site_of<R, GEO>(S, st, PASS == 2 && d.rev2, true)) return;
   if (d.step_ptr) step += *d.step_ptr;  // graph replays: step = device counter + offset of this node
   const int i = st.i;
-  const bool pin = d.pinned && d.pinned[i];
-  const R a = d.alpha ? d.alpha[i] : S->alpha_sub[st.s];
+  const bool pin = !LEAN && d.pinned && d.pinned[i];
+  const R a = (!LEAN && d.alpha) ? d.alpha[i] : S->alpha_sub[st.s];
   if (PASS == 1) {
     const R x = e[3 * i], y = e[3 * i + 1], z = e[3 * i + 2];
     if (pin) {
@@ -55,8 +60,8 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S,
       return;
     }
     R bx, by, bz, fx, fy, fz;
-    local_field<R, CM, GEO>(S, d, e, st, x, y, z, step, noise, bx, by, bz);
-    if (d.save_e) {
+    local_field<R, CM, GEO, LEAN>(S, d, e, st, x, y, z, step, noise, bx, by, bz);
+    if (!LEAN && d.save_e) {
       d.save_e[3 * i] = x;
       d.save_e[3 * i + 1] = y;
       d.save_e[3 * i + 2] = z;
@@ -65,7 +70,7 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S,
       d.save_B[3 * i + 2] = bz;
     }
     llg_rhs(x, y, z, bx, by, bz, a, d.prec, fx, fy, fz);
-    if (d.integ == MX_DEPONDT) {
+    if (!LEAN && d.integ == MX_DEPONDT) {
       R wx, wy, wz, px = x, py = y, pz = z;
       omega(x, y, z, fx, fy, fz, wx, wy, wz);
       k1[3 * i] = wx;
@@ -90,8 +95,8 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S,
     if (pin) return;
     const R x = ep[3 * i], y = ep[3 * i + 1], z = ep[3 * i + 2];
     R bx, by, bz, fx, fy, fz;
-    local_field<R, CM, GEO>(S, d, ep, st, x, y, z, step, noise, bx, by, bz);
-    if (d.save_B) {
+    local_field<R, CM, GEO, LEAN>(S, d, ep, st, x, y, z, step, noise, bx, by, bz);
+    if (!LEAN && d.save_B) {
       d.save_B[3 * i] = bx;
       d.save_B[3 * i + 1] = by;
       d.save_B[3 * i + 2] = bz;
@@ -99,7 +104,7 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S,
     llg_rhs(x, y, z, bx, by, bz, a, d.prec, fx, fy, fz);
     const R hdt = R(0.5) * d.dt;
     R cx, cy, cz;
-    if (d.integ == MX_DEPONDT) {
+    if (!LEAN && d.integ == MX_DEPONDT) {
       R wx, wy, wz;
       omega(x, y, z, fx, fy, fz, wx, wy, wz);
       cx = e[3 * i];
@@ -371,6 +376,12 @@ static Dyn<R> make_dyn(const Model& m, uint64_t seed) {
 
 static inline int nblk_of(const Model& m) { return m.ns * ((m.ncell + BLOCK - 1) / BLOCK); }
 
+// The model the LEAN step kernels are compiled for (injected noise is checked by the caller).
+static inline bool lean_model(const Model& m) {
+  return m.integrator == MX_HEUN && !m.pinned_d && !m.alpha_d && !m.sigma_d && !m.inv_mu_d && !m.has_c &&
+         !m.has_u && m.sot_dl == 0.0 && m.sot_fl == 0.0;
+}
+
 // Dispatch a kernel template over (coupling mode, geometry) at run time.
 #define MX_GEO(m, CALL, CM)                                       \
   do {                                                            \
@@ -399,11 +410,19 @@ static int heun_t(const Model& m, void* e, uint64_t seed, uint64_t step, const v
   R* k1 = static_cast<R*>(m.k1_d);
   const R* nz = static_cast<const R*>(noise);
   const dim3 g(nblk_of(m)), b(BLOCK);
+  if (lean_model(m) && !noise) {
+#define MX_HEUN_CALL(CM, GEO)                                                                   \
+  hipLaunchKernelGGL((k_heun<R, CM, GEO, 1, true>), g, b, 0, s, S, d, E, ep, k1, step, nz); \
+  hipLaunchKernelGGL((k_heun<R, CM, GEO, 2, true>), g, b, 0, s, S, d, E, ep, k1, step, nz)
+    MX_DISPATCH(m, MX_HEUN_CALL);
+#undef MX_HEUN_CALL
+  } else {
 #define MX_HEUN_CALL(CM, GEO)                                                             \
   hipLaunchKernelGGL((k_heun<R, CM, GEO, 1>), g, b, 0, s, S, d, E, ep, k1, step, nz); \
   hipLaunchKernelGGL((k_heun<R, CM, GEO, 2>), g, b, 0, s, S, d, E, ep, k1, step, nz)
-  MX_DISPATCH(m, MX_HEUN_CALL);
+    MX_DISPATCH(m, MX_HEUN_CALL);
 #undef MX_HEUN_CALL
+  }
   return check_launch("heun step");
 }
 
