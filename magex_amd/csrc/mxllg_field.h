@@ -132,24 +132,31 @@ struct Chunk {
   static constexpr int value = CM == COUP_BOND ? (sizeof(R) == 8 ? 2 : 7) : 8;
 };
 
-// One batch of CH stencil entries: gather() issues every load of the batch without a branch (entries
-// past the end re-read the last entry and contribute zero), accumulate() adds the batch to h in stencil
-// order.  Splitting the two lets ALU work (the thermal noise) run while the first batch is in flight.
+// One batch of CH stencil entries: gather() issues every load of the batch without a branch, accumulate()
+// adds the batch to h in stencil order.  Splitting the two lets ALU work (the thermal noise) run while the
+// first batch is in flight.  The CH records are read as one contiguous block (merged scalar loads, one
+// wait); an entry past the end becomes a self-gather that counts zero.  Table couplings are fetched while
+// the gathers are in flight, so accumulate() waits on nothing but the gathers.
 template <typename R, int CM, int GEO>
 struct Batch {
   static constexpr int CH = Chunk<R, CM>::value;
-  static constexpr int NJ = CM == COUP_BOND ? 9 : 1;
+  static constexpr int NJ = CM == COUP_BOND ? 9 : (CM == COUP_TABLE9 ? 9 : (CM == COUP_TABLE4 ? 4 : 1));
   R ex[CH], ey[CH], ez[CH], J[CH][NJ];
-  int slot[CH];
 
   __device__ __forceinline__ void gather(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
                                          const Site& st, int k0, int ke) {
     const int ncell = S->ncell;
+    Ent raw[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) raw[u] = S->ent[k0 + u];
 #pragma unroll
     for (int u = 0; u < CH; ++u) {
       const bool live = k0 + u < ke;
-      const Ent en = S->ent[live ? k0 + u : ke - 1];
-      slot[u] = en.slot;
+      Ent en;
+      en.slot = live ? raw[u].slot : 0;
+      en.tb = live ? raw[u].tb : st.s;
+      en.d01 = live ? raw[u].d01 : 0;
+      en.d2 = live ? raw[u].d2 : 0;
       bool ok;
       const int cn = neighbour_cell<R, GEO>(S, st, en, ok);
       ok = ok && live;
@@ -170,6 +177,10 @@ struct Batch {
         for (int q = 0; q < 9; ++q) t[q] = Jb[q];
 #pragma unroll
         for (int q = 0; q < 9; ++q) J[u][q] = back ? t[(q % 3) * 3 + q / 3] : t[q];
+      } else {
+        const R* T = S->tab + NJ * en.slot;
+#pragma unroll
+        for (int q = 0; q < NJ; ++q) J[u][q] = T[q];
       }
     }
   }
@@ -178,26 +189,21 @@ struct Batch {
 #pragma unroll
     for (int u = 0; u < CH; ++u) {
       const R x = ex[u], y = ey[u], z = ez[u];
+      const R* T = J[u];
       if constexpr (CM == COUP_TABLE1) {
-        const R Jv = S->tab[slot[u]];
+        const R Jv = T[0];
         hx += Jv * x;
         hy += Jv * y;
         hz += Jv * z;
       } else if constexpr (CM == COUP_TABLE4) {
         // J I + antisymmetric part: (J^A e_j) = e_j x D
-        const R* T = S->tab + 4 * slot[u];
         hx += T[0] * x + (y * T[3] - z * T[2]);
         hy += T[0] * y + (z * T[1] - x * T[3]);
         hz += T[0] * z + (x * T[2] - y * T[1]);
-      } else if constexpr (CM == COUP_TABLE9) {
-        const R* T = S->tab + 9 * slot[u];
+      } else {
         hx += T[0] * x + T[1] * y + T[2] * z;
         hy += T[3] * x + T[4] * y + T[5] * z;
         hz += T[6] * x + T[7] * y + T[8] * z;
-      } else {
-        hx += J[u][0] * x + J[u][1] * y + J[u][2] * z;
-        hy += J[u][3] * x + J[u][4] * y + J[u][5] * z;
-        hz += J[u][6] * x + J[u][7] * y + J[u][8] * z;
       }
     }
   }
