@@ -53,10 +53,14 @@ int mx_model_info(const mx_model* m, int64_t* n_sites, int32_t* precision, int32
 
 /* Cell layout inside the library (the ABI's arrays are always in magex's site order): tiled = 1
  * groups the cells of a fully periodic supercell into 3D tiles of 256 cells with power-of-two sides
- * when the supercell allows it (each side divisible, at least 1024 cells); tiled = 0 (the default)
- * keeps them row-major.  Results are identical; on MI355X the row-major order measured faster.
- * Only callable before any per-site data is set.  mx_model_layout reports the tile sides (0 when
- * row-major).  No counterpart in magex (its arrays are always row-major, spinmodel.py:197-206). */
+ * when the supercell allows it (each side divisible, at least 1024 cells); tiled = 2 (checker) stores
+ * the cells of even c0 + c1 + c2 first and the odd ones after, on fully periodic supercells with an
+ * even axis-2 side, so a two-colour Monte Carlo sweep reads each colour contiguously (the adjoint
+ * entry points refuse it); tiled = 0 (the default) keeps them row-major.  A layout the supercell does
+ * not allow falls back to row-major.  Results are identical; on MI355X the row-major order measured
+ * fastest for the LLG step.  Only callable before any per-site data is set.  mx_model_layout reports
+ * the tile sides (0 when row-major, -1 for checker).  No counterpart in magex (its arrays are always
+ * row-major, spinmodel.py:197-206). */
 int mx_model_set_layout(mx_model* m, int tiled);
 int mx_model_layout(const mx_model* m, int32_t tile[3]);
 

@@ -520,6 +520,7 @@ int mx_adjoint_create(mx_model* m, int32_t nparam, const double* unit_bonds, con
     if (m->pinned_d || m->has_c || m->has_u || m->amode != ONSITE_SUB || m->coup == COUP_BOND)
       afail(MX_EUNSUPPORTED, "the adjoint covers stencil couplings and per-sublattice on-site tensors");
     if (m->nent * 9 > MAXTAB) afail(MX_EUNSUPPORTED, "stencil too large for a per-entry coupling table");
+    if (m->geo == GEO_CHECKER) afail(MX_EUNSUPPORTED, "the adjoint runs on the row-major or tiled cell layouts");
     // unit tensors arrive in the caller's stencil order; reorder into the library's, and check that
     // every entry's partner carries the transposed tensors (the adjoint gather relies on J_ji = J_ij^T)
     const int n = m->nent;

@@ -65,10 +65,12 @@ static void upload(const Model& m, void*& dst, const double* src, size_t n) {
   }
 }
 
-// Internal cell of row-major cell c: itself, or its tile-major index in the TILED layout.
+// Internal cell of row-major cell c: itself, its tile-major index in the TILED layout, or its place in
+// its parity half in the CHECKER layout.
 static inline int internal_cell(const Model& m, int c) {
-  if (m.geo != GEO_TILED) return c;
+  if (m.geo != GEO_TILED && m.geo != GEO_CHECKER) return c;
   const int c2 = c % m.L[2], t = c / m.L[2], c1 = t % m.L[1], c0 = t / m.L[1];
+  if (m.geo == GEO_CHECKER) return ((c0 + c1 + c2) & 1) * (m.ncell / 2) + t * (m.L[2] / 2) + (c2 >> 1);
   const int b0 = m.tb[0], b1 = m.tb[1], b2 = m.tb[2];
   const int tile = ((c0 >> b0) * (m.L[1] >> b1) + (c1 >> b1)) * (m.L[2] >> b2) + (c2 >> b2);
   const int l = ((c0 & ((1 << b0) - 1)) << (b1 + b2)) | ((c1 & ((1 << b1) - 1)) << b2) | (c2 & ((1 << b2) - 1));
@@ -103,11 +105,15 @@ static bool choose_tiles(const Model& m, int tb[3]) {
   return true;
 }
 
-// Fix the cell layout and upload perm[q] = magex site of internal site q.
+// Fix the cell layout (want: 0 row-major, 1 tiled, 2 checker; each where the supercell allows it, else
+// row-major) and upload perm[q] = magex site of internal site q.
 static void set_layout(Model& m, int want_tiled) {
   int tb[3];
   const bool allp = m.pbc[0] && m.pbc[1] && m.pbc[2];
-  if (want_tiled && choose_tiles(m, tb)) {
+  if (want_tiled == 2 && allp && m.L[2] % 2 == 0) {
+    m.geo = GEO_CHECKER;
+    m.tb[0] = m.tb[1] = m.tb[2] = 0;
+  } else if (want_tiled == 1 && choose_tiles(m, tb)) {
     m.geo = GEO_TILED;
     for (int a = 0; a < 3; ++a) m.tb[a] = tb[a];
   } else {
@@ -643,8 +649,9 @@ int mx_model_set_layout(mx_model* m, int tiled) {
   return guarded([&]() -> int {
     need_model(m);
     if (m->layout_frozen) fail(MX_EINVAL, "the cell layout is fixed once site data has been set");
+    if (tiled < 0 || tiled > 2) fail(MX_EINVAL, "layout must be 0 (row-major), 1 (tiled) or 2 (checker)");
     use_device(*m);
-    set_layout(*m, tiled ? 1 : 0);
+    set_layout(*m, tiled);
     return MX_OK;
   });
 }
@@ -652,7 +659,7 @@ int mx_model_set_layout(mx_model* m, int tiled) {
 int mx_model_layout(const mx_model* m, int32_t tile[3]) {
   return guarded([&]() -> int {
     need_model(m);
-    for (int a = 0; a < 3; ++a) tile[a] = m->geo == GEO_TILED ? (1 << m->tb[a]) : 0;
+    for (int a = 0; a < 3; ++a) tile[a] = m->geo == GEO_TILED ? (1 << m->tb[a]) : (m->geo == GEO_CHECKER ? -1 : 0);
     return MX_OK;
   });
 }

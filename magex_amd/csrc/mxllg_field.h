@@ -10,6 +10,24 @@ struct Site {
   int s, c, c0, c1, c2, i, crm;  // sublattice, internal cell, coordinates, internal site, magex cell
 };
 
+// CHECKER layout: coordinates of internal cell st.c (parity half, then row-major with c2 / 2 fastest).
+template <typename R>
+__device__ __forceinline__ void checker_cell(const Static<R>* __restrict__ S, Site& st) {
+  const int half = S->half, L1 = S->L1, L2h = S->L2h;
+  const int p = st.c >= half ? 1 : 0, idx = st.c - p * half;
+  int t, k;
+  if (S->fdiv) {
+    t = fdiv(idx, L2h, S->rL2h);
+    st.c0 = fdiv(t, L1, S->rL1);
+  } else {
+    t = idx / L2h;
+    st.c0 = t / L1;
+  }
+  k = idx - t * L2h;
+  st.c1 = t - st.c0 * L1;
+  st.c2 = 2 * k + ((p + st.c0 + st.c1) & 1);
+}
+
 // Two block -> tile maps.  Slabs (xcd_tile): XCD x walks a contiguous run of row-major tiles, i.e. whole
 // planes of axis 0.  Columns (col and S->xcol > 0, row-major layouts whose axis-0 planes hold 8 k whole
 // tiles): XCD x owns tiles [x k, (x + 1) k) of every plane and walks them plane by plane, so the two
@@ -48,6 +66,10 @@ __device__ __forceinline__ bool site_of(const Static<R>* __restrict__ S, Site& s
     st.c1 = (t1 << b1) + ((l >> b2) & ((1 << b1) - 1));
     st.c2 = (t2 << b2) + (l & ((1 << b2) - 1));
     st.c = tile * BLOCK + l;
+  } else if constexpr (GEO == GEO_CHECKER) {
+    st.c = tile * BLOCK + threadIdx.x;
+    if (st.c >= ncell) return false;
+    checker_cell(S, st);
   } else {
     st.c = tile * BLOCK + threadIdx.x;
     if (st.c >= ncell) return false;
@@ -81,6 +103,8 @@ __device__ __forceinline__ void site_at(const Static<R>* __restrict__ S, int i, 
     st.c0 = (t0 << b0) + (l >> (b1 + b2));
     st.c1 = (t1 << b1) + ((l >> b2) & ((1 << b1) - 1));
     st.c2 = (t2 << b2) + (l & ((1 << b2) - 1));
+  } else if constexpr (GEO == GEO_CHECKER) {
+    checker_cell(S, st);
   } else {
     st.c2 = st.c % L2;
     const int t = st.c / L2;
@@ -117,6 +141,8 @@ __device__ __forceinline__ int neighbour_cell(const Static<R>* __restrict__ S, c
       const int tile = ((n0 >> b0) * S->nt1 + (n1 >> b1)) * S->nt2 + (n2 >> b2);
       const int l = ((n0 & ((1 << b0) - 1)) << (b1 + b2)) | ((n1 & ((1 << b1) - 1)) << b2) | (n2 & ((1 << b2) - 1));
       return tile * BLOCK + l;
+    } else if constexpr (GEO == GEO_CHECKER) {
+      return ((n0 + n1 + n2) & 1) * S->half + (n0 * L1 + n1) * S->L2h + (n2 >> 1);
     } else {
       return (n0 * L1 + n1) * L2 + n2;
     }

@@ -22,8 +22,10 @@ constexpr int BLOCK = 256;   // threads per workgroup (4 wavefronts of 64)
 
 enum CoupMode { COUP_TABLE1 = 0, COUP_TABLE4 = 1, COUP_TABLE9 = 2, COUP_BOND = 3 };
 // Cell layouts: OPEN (row-major, some axis open), PERIODIC (row-major, all axes periodic), TILED (all
-// axes periodic, cells grouped in 3D tiles of BLOCK cells, power-of-two sides, one tile per workgroup).
-enum Geometry { GEO_OPEN = 0, GEO_PERIODIC = 1, GEO_TILED = 2 };
+// axes periodic, cells grouped in 3D tiles of BLOCK cells, power-of-two sides, one tile per workgroup),
+// CHECKER (all axes periodic, L2 even: the cells of even c0 + c1 + c2, then the odd ones, each half
+// row-major with c2 / 2 fastest -- a two-colour Monte Carlo sweep then reads each colour contiguously).
+enum Geometry { GEO_OPEN = 0, GEO_PERIODIC = 1, GEO_TILED = 2, GEO_CHECKER = 3 };
 enum OnsiteMode { ONSITE_SUB = 0, ONSITE_SITE6 = 1, ONSITE_SITE9 = 2 };
 
 // One stencil entry (16 bytes): slot = coupling slot (type-table row, or forward-bond slot),
@@ -43,6 +45,8 @@ struct Static {
   int tb0, tb1, tb2, nt1, nt2;  // TILED: log2 of the tile sides; tiles along axes 1 and 2
   int fdiv;                     // ncell < 2^24: divide cell indices by L1, L2 through rL1, rL2
   int xcol;                     // step kernels' column map: row-major tiles per XCD per axis-0 plane (0: slabs)
+  int half, L2h;                // CHECKER: cells per parity, L2 / 2
+  float rL2h;
   float rL1, rL2;
   int amode, has_A, has_c, has_u;
   int sptr[MAXSUB + 1];

@@ -295,9 +295,12 @@ static void fill_static(const Model& m, Static<R>& S) {
   {
     // column map of the step kernels when an axis-0 plane is a whole number of tiles, 8 or more per XCD
     const long plane = long(m.L[1]) * m.L[2];
-    S.xcol = (m.geo != GEO_TILED && plane % BLOCK == 0 && (plane / BLOCK) % 8 == 0)
+    S.xcol = ((m.geo == GEO_PERIODIC || m.geo == GEO_OPEN) && plane % BLOCK == 0 && (plane / BLOCK) % 8 == 0)
                  ? int(plane / BLOCK / 8) : 0;
   }
+  S.half = m.ncell / 2;
+  S.L2h = m.L[2] / 2;
+  S.rL2h = S.L2h > 0 ? 1.0f / float(S.L2h) : 0.0f;
   S.amode = m.amode;
   S.has_A = m.has_A;
   S.has_c = m.has_c;
@@ -386,6 +389,7 @@ static inline bool lean_model(const Model& m) {
 #define MX_GEO(m, CALL, CM)                                       \
   do {                                                            \
     if ((m).geo == GEO_TILED) { CALL(CM, GEO_TILED); }            \
+    else if ((m).geo == GEO_CHECKER) { CALL(CM, GEO_CHECKER); }   \
     else if ((m).geo == GEO_PERIODIC) { CALL(CM, GEO_PERIODIC); } \
     else { CALL(CM, GEO_OPEN); }                                  \
   } while (0)
@@ -517,7 +521,7 @@ int launch_noise(const Model& m, void* out, uint64_t seed, uint64_t step, hipStr
   return check_launch("noise");
 }
 
-bool needs_permute(const Model& m) { return m.ns > 1 || m.geo == GEO_TILED; }
+bool needs_permute(const Model& m) { return m.ns > 1 || m.geo == GEO_TILED || m.geo == GEO_CHECKER; }
 
 int launch_permute(const Model& m, const void* src, void* dst, int to_internal, hipStream_t s) {
   const dim3 g((m.N + BLOCK - 1) / BLOCK);

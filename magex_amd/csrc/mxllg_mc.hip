@@ -206,21 +206,25 @@ static int mc_t(const Model& m, void* e, const int32_t* sites, int n, const void
   switch (m.coup) {
     case COUP_TABLE1:
       if (m.geo == GEO_TILED) MX_MC_CALL(COUP_TABLE1, GEO_TILED);
+      else if (m.geo == GEO_CHECKER) MX_MC_CALL(COUP_TABLE1, GEO_CHECKER);
       else if (m.geo == GEO_PERIODIC) MX_MC_CALL(COUP_TABLE1, GEO_PERIODIC);
       else MX_MC_CALL(COUP_TABLE1, GEO_OPEN);
       break;
     case COUP_TABLE4:
       if (m.geo == GEO_TILED) MX_MC_CALL(COUP_TABLE4, GEO_TILED);
+      else if (m.geo == GEO_CHECKER) MX_MC_CALL(COUP_TABLE4, GEO_CHECKER);
       else if (m.geo == GEO_PERIODIC) MX_MC_CALL(COUP_TABLE4, GEO_PERIODIC);
       else MX_MC_CALL(COUP_TABLE4, GEO_OPEN);
       break;
     case COUP_TABLE9:
       if (m.geo == GEO_TILED) MX_MC_CALL(COUP_TABLE9, GEO_TILED);
+      else if (m.geo == GEO_CHECKER) MX_MC_CALL(COUP_TABLE9, GEO_CHECKER);
       else if (m.geo == GEO_PERIODIC) MX_MC_CALL(COUP_TABLE9, GEO_PERIODIC);
       else MX_MC_CALL(COUP_TABLE9, GEO_OPEN);
       break;
     default:
       if (m.geo == GEO_TILED) MX_MC_CALL(COUP_BOND, GEO_TILED);
+      else if (m.geo == GEO_CHECKER) MX_MC_CALL(COUP_BOND, GEO_CHECKER);
       else if (m.geo == GEO_PERIODIC) MX_MC_CALL(COUP_BOND, GEO_PERIODIC);
       else MX_MC_CALL(COUP_BOND, GEO_OPEN);
       break;
@@ -265,7 +269,7 @@ extern "C" int mx_mc_sweep_replicas(mx_model* m, void* e, int32_t nrep, int64_t 
     set_error("replica sweeps take one temperature per replica: the model has per-site temperatures");
     return MX_EUNSUPPORTED;
   }
-  if (m->ns > 1 || m->geo == GEO_TILED) {
+  if (needs_permute(*m)) {
     set_error("replica sweeps need one-site cells in row-major order (their spins are not permuted)");
     return MX_EUNSUPPORTED;
   }

@@ -83,7 +83,7 @@ DEFAULT_LAYOUT = os.environ.get("MAGEX_AMD_LAYOUT", "rowmajor")
 class DeviceModel:
     """libmxllg model of a SpinSystem in precision "fp64" or "fp32"."""
 
-    def __init__(self, system, precision="fp64", tiled=None):
+    def __init__(self, system, precision="fp64", tiled=None, layout=None):
         from .terms import CubicAnisotropy, UniaxialHigher
         torch = _torch()
         if precision not in ("fp64", "fp32"):
@@ -101,9 +101,13 @@ class DeviceModel:
                                 hptr(np.array(system.L, np.int32)), hptr(np.array(system.pbc, np.int32)),
                                 ctypes.byref(h)))
         self.h = h
-        if tiled is None:
-            tiled = DEFAULT_LAYOUT == "tiled"
-        check(L.mx_model_set_layout(h, int(bool(tiled))))
+        # layout: "rowmajor", "tiled" or "checker" (tiled=True/False is the older spelling)
+        if layout is None:
+            layout = ("tiled" if tiled else "rowmajor") if tiled is not None else DEFAULT_LAYOUT
+        codes = {"rowmajor": 0, "tiled": 1, "checker": 2}
+        if layout not in codes:
+            raise ValueError(f"layout must be one of {tuple(codes)}")
+        check(L.mx_model_set_layout(h, codes[layout]))
         # exchange stencil + coupling table (or per-bond tensors)
         st = system.stencil
         table, typ = coupling_table([J for *_, J in st])

@@ -97,6 +97,12 @@ class MetropolisMC:
         self.overrelax = int(overrelax)
         self.precision = precision
         col = coloring(system)
+        # colours that are the parity classes of c0 + c1 + c2 (bcc, sc): the library's checkerboard cell
+        # order then keeps every colour contiguous in memory (where the supercell allows that order)
+        lin = linear_coloring(system)
+        self.layout = None
+        if lin is not None and lin[1] == 2 and all(int(a) % 2 == 1 or L == 1 for a, L in zip(lin[0], system.L)):
+            self.layout = "checker"
         pinned = system.pinned
         self.groups = []
         for c in range(int(col.max()) + 1 if len(col) else 0):
@@ -113,7 +119,7 @@ class MetropolisMC:
         self._key = None
 
     def _sync(self):
-        dm = self.sys.device_model(self.precision)
+        dm = self.sys.device_model(self.precision, layout=self.layout)
         key = (id(dm), self.T, id(getattr(self.sys, "site_T", None)))
         if dm is not self._dm:
             off = np.concatenate([[0], np.cumsum([len(g) for g in self.groups])]).astype(np.int64)

@@ -341,14 +341,17 @@ class SpinSystem:
         return out
 
     # ------------------------------------------------------------------ device evaluation
-    def device_model(self, precision="fp64"):
-        """Cached DeviceModel of the current arrays (rebuilt after touch() / setters)."""
+    def device_model(self, precision="fp64", layout=None):
+        """Cached DeviceModel of the current arrays (rebuilt after touch() / setters); layout None is the
+        library default, "checker" the two-colour cell order of the Monte Carlo sweeps."""
         from .device import DeviceModel
-        key = (precision, self._version, len(self.terms), tuple(id(t) for t in self.terms))
+        state = (self._version, len(self.terms), tuple(id(t) for t in self.terms))
+        key = (precision, layout) + state
         dm = self._dev.get(key)
         if dm is None:
-            self._dev.clear()
-            dm = self._dev[key] = DeviceModel(self, precision)
+            for k in [k for k in self._dev if k[2:] != state]:   # models of older arrays
+                del self._dev[k]
+            dm = self._dev[key] = DeviceModel(self, precision, layout=layout)
         dm.set_field(self.B)
         return dm
 

@@ -97,6 +97,26 @@ def test_mc_perbond_terms_uniform_matches_oracle(gpu):
     assert mc.acceptance() == acc / (3 * s.N)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_mc_checker_layout_equals_row_major(gpu, precision):
+    """Sweeps on the checkerboard cell layout (each parity colour contiguous) give the same spins and
+    acceptance counts, bit for bit, as on the row-major layout."""
+    print(f"case: mc checker vs row-major {precision}", flush=True)
+    out = []
+    for layout in ("checker", None):
+        s, _ = _pair((6, 8, 10), (0.1, 0, 0.4), lattice="bcc_conventional", J=[2.0, 1.0], K=0.1, moment=2.2)
+        mc = MetropolisMC(s, 40.0, seed=21, sigma=0.4, overrelax=1, move="gaussian", precision=precision)
+        assert mc.layout == "checker"
+        mc.layout = layout
+        dt = gpu.float64 if precision == "fp64" else gpu.float32
+        e = gpu.as_tensor(s.init_spins("random", seed=4), device="cuda", dtype=dt)
+        for _ in range(4):
+            mc.sweep_device(e)
+        assert s.device_model(precision, layout=layout).layout() == ((-1, -1, -1) if layout else (0, 0, 0))
+        out.append((e.clone(), mc.accepted))
+    assert gpu.equal(out[0][0], out[1][0]) and out[0][1] == out[1][1]
+
+
 @pytest.mark.parametrize("move", ["gaussian", "heatbath"])
 def test_mc_fp32_langevin(gpu, move):
     """FP32 sweeps of free spins in a field sample <cos theta> = coth x - 1/x."""

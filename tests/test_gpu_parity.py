@@ -253,6 +253,36 @@ def test_tiled_layout_equals_row_major(gpu, case):
         assert gpu.equal(ea, eb)
 
 
+@pytest.mark.parametrize("case", ["bcc_perbond", "sc_dmi"])
+def test_checker_layout_equals_row_major(gpu, case):
+    """The checkerboard cell layout (even c0 + c1 + c2 first) changes no result bit of the field or of
+    25 thermal steps against the row-major layout (FP64 and FP32)."""
+    from magex_amd.configs import random_bond_tensors, random_sia
+    from magex_amd.device import DeviceModel
+    print(f"case: checker vs row-major {case}", flush=True)
+    if case == "bcc_perbond":
+        s, _ = _pair((6, 4, 8), lattice="bcc_conventional", J=[20.0, 10.0], moment=2.2)
+        s.set_bond_tensors(random_bond_tensors(s, seed=3))
+        s.A = random_sia(s.N, seed=4)
+    else:
+        s, _ = _pair((5, 6, 10), (0.2, 0, 0.7), lattice="sc", J=[1.0, 0.1], D=0.3, K=0.1, moment=2.0)
+    e0 = s.init_spins("random", seed=2)
+    for prec, dt in (("fp64", gpu.float64), ("fp32", gpu.float32)):
+        a = DeviceModel(s, prec, layout="checker")
+        b = DeviceModel(s, prec, layout="rowmajor")
+        assert a.layout() == (-1, -1, -1) and b.layout() == (0, 0, 0)
+        ed = gpu.as_tensor(e0, device="cuda", dtype=dt)
+        assert gpu.equal(a.field(ed), b.field(ed))
+        oa, ob = a.observables(ed), b.observables(ed)
+        assert np.allclose(oa, ob, rtol=1e-12, atol=1e-12 * np.abs(ob).max())
+        for m in (a, b):
+            m.set_dynamics(1e-4, 0.1, 300.0)
+        ea, eb = ed.clone(), ed.clone()
+        a.run(ea, 25, seed=9)
+        b.run(eb, 25, seed=9)
+        assert gpu.equal(ea, eb)
+
+
 def test_topological_charge_matches_magex(gpu):
     print("case: topological charge", flush=True)
     from magex_amd.texture import topological_charge
