@@ -1,0 +1,87 @@
+"""The benchmark configurations at their full sizes, through checks that do not depend on size:
+
+- bcc 64^3 (524 288 spins, per-bond random tensors, random SIA): reciprocity of the exchange operator,
+  sum_i e1_i . h(e2)_i = sum_i e2_i . h(e1)_i, which holds only if every partner entry applies the
+  transposed tensor of its bond (J_ji = J_ij^T); exact norm preservation; the stated FP32 bar against
+  FP64 on identical noise (mean angle below 1e-5 rad);
+- triangular 256^2 (65 536 spins, three shells + DMI): field and ten Heun steps on injected noise
+  against the CPU oracle at the full size (FP64, 1e-10 relative), integer topological charge;
+- the 96-replica batch (1.66 M spins, L from 32 to 192): the smallest and the largest replica of the
+  batched FP64 run equal separate plain runs bit for bit.
+"""
+import numpy as np
+import pytest
+
+from oracle import magex_ref as R
+from magex_amd import _lib, configs, texture
+from magex_amd import models as M
+from magex_amd.batch import ReplicaBatch
+from magex_amd.llg import LLGSolver
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300))
+
+
+def test_bcc64_exchange_reciprocity_and_norms(gpu):
+    print("case: bcc64 reciprocity, norms, fp32 angle", flush=True)
+    s, p = configs.bcc64()
+    dm = s.device_model("fp64")
+    e1 = gpu.as_tensor(s.init_spins("random", seed=11), device="cuda")
+    e2 = gpu.as_tensor(s.init_spins("random", seed=12), device="cuda")
+    h1 = dm.field(e1, _lib.MX_FIELD_EXCHANGE)
+    h2 = dm.field(e2, _lib.MX_FIELD_EXCHANGE)
+    a, b = float((e1 * h2).sum()), float((e2 * h1).sum())
+    # a transposition slip in the partner entries changes these sums by ~1e-2 relative
+    assert abs(a - b) <= 1e-11 * max(abs(a), abs(b)), (a, b)
+    e = {}
+    for prec, dt in (("fp64", gpu.float64), ("fp32", gpu.float32)):
+        sol = LLGSolver(s, p["alpha"], p["dt"], p["temperature"], p["seed"], precision=prec)
+        x = gpu.as_tensor(s.init_spins("random", seed=0), device="cuda", dtype=dt)
+        sol.step_device(x, 50)
+        n = x.double().norm(dim=1)
+        assert float((n - 1).abs().max()) < (1e-13 if prec == "fp64" else 1e-6)
+        e[prec] = x.double()
+    b32 = e["fp32"] / e["fp32"].norm(dim=1, keepdim=True)
+    ang = gpu.atan2(gpu.linalg.cross(e["fp64"], b32).norm(dim=1), (e["fp64"] * b32).sum(1))
+    print(f"bcc64 fp32 vs fp64 after 50 steps: mean {float(ang.mean()):.3e} rad, max {float(ang.max()):.3e}")
+    assert float(ang.mean()) < 1e-5
+
+
+def test_tri256_full_size_matches_oracle(gpu):
+    print("case: tri256 full size vs oracle", flush=True)
+    s, p = configs.tri256()
+    o = R.SpinSystem(R.load_results(M.make_model("triangular", J=[5.0, -0.5, 0.2], D=1.5, K=0.3)), (256, 256, 1),
+                     field=(0, 0, 2.0))
+    e0 = s.init_spins("random", seed=0)
+    assert _rel(s.effective_field(e0), o.effective_field(e0)) < 1e-12
+    rng = np.random.default_rng(5)
+    noise = rng.standard_normal((10, s.N, 3))
+    sol = LLGSolver(s, p["alpha"], p["dt"], p["temperature"], p["seed"], precision="fp64")
+    e = gpu.as_tensor(e0, device="cuda")
+    sol.step_device(e, 10, noise=gpu.as_tensor(noise, device="cuda"))
+    ref = R.run_heun(o, e0.copy(), 10, p["alpha"], p["dt"], T=p["temperature"], normals=lambda n: noise[n])
+    assert _rel(e.cpu().numpy(), ref) < 1e-10
+    # the charge of any texture on the periodic lattice is an integer
+    sol.step_device(e, 400)
+    q = texture.topological_charge(s, e)
+    assert abs(q - round(q)) < 1e-6, q
+
+
+def test_batch96_full_size_equals_separate_runs(gpu):
+    print("case: batch96 full size vs separate runs", flush=True)
+    systems, p = configs.batch96()
+    T, seeds = p["temperatures"], p["seeds"]
+    alpha = np.full(len(systems), p["alpha"])
+    spins = [s.init_spins("random", seed=seeds[k]) for k, s in enumerate(systems)]
+    batch = ReplicaBatch(systems, alpha, p["dt"], T, seeds, "fp64")
+    e = batch.pack(spins)
+    batch.run(e, 20)
+    parts = batch.unpack(e)
+    sizes = [s.N for s in systems]
+    for k in (int(np.argmin(sizes)), int(np.argmax(sizes))):
+        ek = gpu.as_tensor(spins[k], device="cuda")
+        LLGSolver(systems[k], alpha[k], p["dt"], T[k], seeds[k], precision="fp64").step_device(ek, 20)
+        assert gpu.equal(parts[k], ek), k

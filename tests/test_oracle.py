@@ -110,6 +110,33 @@ def test_bcc_perbond_matches_magex():
     assert np.abs(e - g["e10"]).max() < 1e-13
 
 
+def test_exchange_reciprocity_detects_untransposed_partners():
+    """The full-size GPU check sum_i e1_i . h(e2)_i == sum_i e2_i . h(e1)_i has teeth: it holds for
+    magex's per-bond couplings (golden bcc per-bond lattice, J_ji = J_ij^T) and fails when the partner
+    entries reuse the forward tensor untransposed, a plausible slip in a once-per-bond storage."""
+    g = golden("bcc_perbond.npz")
+    src, dst, blocks = g["src"], g["dst"], g["blocks"]
+    N = int(max(src.max(), dst.max())) + 1
+    rng = np.random.default_rng(3)
+    e1, e2 = R.normalize(rng.normal(size=(N, 3))), R.normalize(rng.normal(size=(N, 3)))
+
+    def recip(b):
+        rows = (3 * src[:, None, None] + np.arange(3)[None, :, None]).repeat(3, 2).ravel()
+        cols = (3 * dst[:, None, None] + np.arange(3)[None, None, :]).repeat(3, 1).ravel()
+        J = sp.csr_matrix((b.ravel(), (rows, cols)), shape=(3 * N, 3 * N))
+        a, c = e1.ravel() @ (J @ e2.ravel()), e2.ravel() @ (J @ e1.ravel())
+        return abs(a - c) / max(abs(a), abs(c))
+
+    assert recip(blocks) < 1e-12
+    # the slip: for every pair with src > dst, the block of the reverse pair used as it is
+    idx = {(int(a), int(b)): k for k, (a, b) in enumerate(zip(src, dst))}
+    wrong = blocks.copy()
+    for k, (a, b) in enumerate(zip(src, dst)):
+        if a > b:
+            wrong[k] = blocks[idx[(int(b), int(a))]]
+    assert recip(wrong) > 1e-4
+
+
 def test_charge_matches_magex():
     g = golden("charge.npz")
     uc = R.load_results(R.make_model(lattice="triangular", J=[1.0]))
