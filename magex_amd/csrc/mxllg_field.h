@@ -163,9 +163,11 @@ struct Chunk {
 // first batch is in flight.  The CH records are read as one contiguous block (merged scalar loads, one
 // wait); an entry past the end becomes a self-gather that counts zero.  Table couplings are fetched while
 // the gathers are in flight, so accumulate() waits on nothing but the gathers.
-template <typename R, int CM, int GEO>
+// CHW > 0 overrides the batch width (at most 16, the padding of the record array).
+template <typename R, int CM, int GEO, int CHW = 0>
 struct Batch {
-  static constexpr int CH = Chunk<R, CM>::value;
+  static constexpr int CH = CHW > 0 ? CHW : Chunk<R, CM>::value;
+  static_assert(CH <= 16, "a batch may read at most 16 records past its first");
   static constexpr int NJ = CM == COUP_BOND ? 9 : (CM == COUP_TABLE9 ? 9 : (CM == COUP_TABLE4 ? 4 : 1));
   R ex[CH], ey[CH], ez[CH], J[CH][NJ];
 
@@ -237,7 +239,8 @@ struct Batch {
 
 // h_i = sum_j J_ij e_j over the stencil of site st (magex/fast.py:86-120 semantics), accumulated in
 // stencil order; mid() runs after the first batch's loads are issued and before they are used.
-template <typename R, int CM, int GEO, typename Mid>
+// The batch width does not change the result: entries are summed in stencil order either way.
+template <typename R, int CM, int GEO, int CHW = 0, typename Mid>
 __device__ __forceinline__ void exchange(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
                                          const Site& st, R& hx, R& hy, R& hz, Mid&& mid) {
   hx = hy = hz = R(0);
@@ -246,20 +249,21 @@ __device__ __forceinline__ void exchange(const Static<R>* __restrict__ S, const 
     mid();
     return;
   }
-  Batch<R, CM, GEO> bt;
+  using B = Batch<R, CM, GEO, CHW>;
+  B bt;
   bt.gather(S, d, e, st, kb, ke);
   mid();
   bt.accumulate(S, hx, hy, hz);
-  for (int k0 = kb + Batch<R, CM, GEO>::CH; k0 < ke; k0 += Batch<R, CM, GEO>::CH) {
+  for (int k0 = kb + B::CH; k0 < ke; k0 += B::CH) {
     bt.gather(S, d, e, st, k0, ke);
     bt.accumulate(S, hx, hy, hz);
   }
 }
 
-template <typename R, int CM, int GEO>
+template <typename R, int CM, int GEO, int CHW = 0>
 __device__ __forceinline__ void exchange(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
                                          const Site& st, R& hx, R& hy, R& hz) {
-  exchange<R, CM, GEO>(S, d, e, st, hx, hy, hz, [] {});
+  exchange<R, CM, GEO, CHW>(S, d, e, st, hx, hy, hz, [] {});
 }
 
 // On-site gradient dE/de (magex/fast.py:39-64): 2 A e + cubic + higher uniaxial.

@@ -50,7 +50,7 @@ struct Static {
   float rL1, rL2;
   int amode, has_A, has_c, has_u;
   int sptr[MAXSUB + 1];
-  Ent ent[MAXENT + 8];  // padded: a batch of up to 8 entries may be read past the last one (never used)
+  Ent ent[MAXENT + 16];  // padded: a batch of up to 16 entries may be read past the last one (never used)
   R tab[MAXTAB];  // COUP_TABLE*: ntypes x width
   R Asub[MAXSUB][9];
   R inv_mu_sub[MAXSUB], alpha_sub[MAXSUB], sigma_sub[MAXSUB], sign_sub[MAXSUB];
