@@ -156,6 +156,50 @@ class LLGSolver:
         self.time += nsteps * self.dt
         return e_dev
 
+    def field(self, e, t=None):
+        """B_eff + spin-orbit torque fields + field_func(t) (T), zero on pinned sites (magex/llg.py:124-136);
+        numpy in -> numpy out, device tensors stay on the device.  t defaults to the solver's time."""
+        from . import _lib
+        from .device import like_input
+        if self.extra_field is not None:
+            raise NotImplementedError("extra_field (an arbitrary function of the spins) is not on the device path")
+        dm = self._sync()
+        ed = self._to_device(e)
+        B = dm.field(ed, _lib.MX_FIELD_BEFF)          # B_eff from the kernels, zero on pinned sites
+        extra = self.sot is not None or self.field_func is not None
+        if self.sot is not None:
+            p, bdl, bfl = self.sot
+            pt = B.new_tensor(p)
+            B = B + bdl * ed.cross(pt.expand_as(ed), dim=1) + bfl * pt
+        if self.field_func is not None:
+            f = np.asarray(self.field_func(self.time if t is None else t), float)
+            if f.size != 3:
+                raise NotImplementedError("a site-dependent field_func is not on the device path")
+            B = B + B.new_tensor(f.reshape(3))
+        if extra and self.sys.pinned is not None:
+            B[_pinned_mask(self.sys, B)] = 0.0
+        return like_input(B, e)
+
+    def rhs(self, e, b=0.0, t=None):
+        """Tangent de/dt (1/ps) in the field field(e, t) + b (magex/llg.py:138-150); b a thermal field
+        (N,3) or 0.  Same input/output convention as field()."""
+        from .device import as_device, like_input
+        dm = self._sync()
+        ed = self._to_device(e)
+        B = self.field(ed, t)
+        if not np.isscalar(b) or b != 0.0:
+            B = B + as_device(b, dm.dtype)
+        a = self.alpha
+        if np.ndim(a) == 0:
+            return like_input(dm.rhs(ed, B.contiguous(), float(a), self.precession), e)
+        # per-site damping: the same expression with alpha per row
+        at = B.new_tensor(np.asarray(a, float))[:, None]
+        exB = ed.cross(B, dim=1)
+        f = at * ed.cross(exB, dim=1)
+        if self.precession:
+            f = f + exB
+        return like_input(-GAMMA / (1.0 + at * at) * f, e)
+
     def step(self, e):
         """One integration step; returns new spins (numpy in -> numpy out, the input is not mutated)."""
         from .device import like_input
@@ -216,6 +260,11 @@ class LLGSolver:
         if log:
             log(f"  LLG wall time {time.time() - t0:.1f} s")
         return like_input(ed, e), np.array(rows), snaps
+
+
+def _pinned_mask(system, like):
+    import torch
+    return torch.as_tensor(np.asarray(system.pinned, bool), device=like.device)
 
 
 def save_spins(path, system, e):

@@ -426,6 +426,37 @@ def test_sot_and_field_func_match_oracle(gpu, integrator):
     assert abs(sol.time - 50 * 5e-4) < 1e-15
 
 
+@pytest.mark.parametrize("per_site_alpha", [False, True])
+def test_solver_field_and_rhs_match_oracle(gpu, per_site_alpha):
+    """LLGSolver.field(e, t) and .rhs(e, b, t) (magex/llg.py:124-150) with spin-orbit torque fields, a
+    time-dependent field_func, pinned sites and a thermal field b, against the oracle's solver_field and
+    llg_rhs; numpy in -> numpy out, device tensors stay on the device."""
+    print(f"case: solver field/rhs per_site_alpha={per_site_alpha}", flush=True)
+    s, o = _pair((9, 7, 1), (0.1, 0, 0.3), lattice="square", J=[2.0, 0.3], D=0.8, K=0.1)
+    pin = np.zeros(s.N, bool)
+    pin[::5] = True
+    s.pinned = pin
+    o.pinned = pin
+    ff = lambda t: np.array([0.4 * np.cos(40.0 * t), 0.4 * np.sin(40.0 * t), 0.1])      # noqa: E731
+    sot = {"p": [0.0, 1.0, 1.0], "B_DL": 0.6, "B_FL": -0.2}
+    alpha = np.linspace(0.05, 0.3, s.N) if per_site_alpha else 0.12
+    sol = LLGSolver(s, alpha=alpha, dt=5e-4, sot=sot)
+    sol.field_func = ff
+    p = np.array(sot["p"]) / np.linalg.norm(sot["p"])
+    ref_field = R.solver_field(o, (p, sot["B_DL"], sot["B_FL"]), ff)
+    e = s.init_spins("random", seed=8)
+    b = np.random.default_rng(2).normal(size=(s.N, 3)) * 0.7
+    for t in (0.0, 0.013):
+        B = sol.field(e, t)
+        assert isinstance(B, np.ndarray) and _rel(B, ref_field(e, t)) < 1e-13
+        assert np.all(B[pin] == 0.0)
+        a = np.asarray(alpha)[:, None] if per_site_alpha else alpha
+        f = sol.rhs(e, b, t)
+        assert _rel(f, R.llg_rhs(e, ref_field(e, t) + b, a)) < 1e-13
+        fd = sol.rhs(gpu.as_tensor(e, device="cuda"), gpu.as_tensor(b, device="cuda"), t)
+        assert fd.is_cuda and _rel(fd.cpu().numpy(), f) < 1e-15
+
+
 def test_largest_cell_count_and_self_images(gpu):
     """Sizes at the edges of the index arithmetic and of the stencil:
     - 256^3 = 2^24 cells (the largest supercell class, where cell indices leave the float-division
