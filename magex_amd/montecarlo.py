@@ -16,6 +16,7 @@ dipolar / four-spin terms are not on the device path (NotImplementedError).
 from __future__ import annotations
 
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -121,6 +122,18 @@ class MetropolisMC:
     def _sync(self):
         dm = self.sys.device_model(self.precision, layout=self.layout)
         key = (id(dm), self.T, id(getattr(self.sys, "site_T", None)))
+        # the device model (colour lists, temperatures, acceptance counter) is shared by every sampler
+        # of this system: a sampler that was not the last one to sweep it uploads its own set-up again,
+        # and the accepted moves still on the device are credited to the sampler that made them
+        owner = getattr(dm, "_mc_owner", None)
+        prev = owner() if owner is not None else None
+        if prev is not self:
+            if prev is not None:
+                n = ctypes.c_int64()
+                check(dm.lib.mx_mc_accepted(dm.h, ctypes.byref(n), 1))
+                prev.accepted += n.value
+            self._dm = None
+            dm._mc_owner = weakref.ref(self)
         if dm is not self._dm:
             off = np.concatenate([[0], np.cumsum([len(g) for g in self.groups])]).astype(np.int64)
             sites = np.ascontiguousarray(np.concatenate(self.groups) if self.groups else np.zeros(0), np.int32)

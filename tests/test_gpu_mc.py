@@ -117,6 +117,58 @@ def test_mc_checker_layout_equals_row_major(gpu, precision):
     assert gpu.equal(out[0][0], out[1][0]) and out[0][1] == out[1][1]
 
 
+def test_cuda_mc_class_shape(gpu):
+    """magex's GPU Metropolis class (CudaMC(system, T, seed, sigma, move), sweep_device, acceptance):
+    the same sweeps as MetropolisMC without over-relaxation, acceptance resetting per call, and the
+    heat-bath move falling back to gaussian moves at T = 0 (where every accepted move lowers E)."""
+    print("case: CudaMC", flush=True)
+    from magex_amd.hip_backend import CudaMC
+    s, _ = _pair((6, 6, 6), (0.1, 0, 0.4), lattice="sc", J=[1.0], D=0.2, K=0.1, moment=2.0)
+    e0 = gpu.as_tensor(s.init_spins("random", seed=3), device="cuda")
+    a, b = e0.clone(), e0.clone()
+    cm = CudaMC(s, 5.0, seed=4, sigma=0.4, move="gaussian")
+    mc = MetropolisMC(s, 5.0, seed=4, sigma=0.4, overrelax=0, move="gaussian")
+    for _ in range(3):
+        cm.sweep_device(a)
+        mc.sweep_device(b)
+    assert gpu.equal(a, b)
+    acc = cm.acceptance()
+    assert 0.0 < acc < 1.0 and cm.tried == 0 and acc == mc.acceptance()
+    z = CudaMC(s, 0.0, seed=1, move="heatbath")
+    assert z.mc.move == "gaussian"
+    e = e0.clone()
+    E0 = s.energy(e.cpu().numpy())
+    for _ in range(5):
+        z.sweep_device(e)
+    assert s.energy(e.cpu().numpy()) <= E0
+
+
+def test_two_samplers_on_one_system_do_not_interfere(gpu):
+    """Samplers share the system's device model (colour lists, temperatures, acceptance counter): two
+    at different temperatures, sweeping alternately, each equal a sampler running alone, and each
+    counts only its own accepted moves."""
+    print("case: two samplers on one system", flush=True)
+    def system():
+        return _pair((6, 6, 6), (0.1, 0, 0.4), lattice="sc", J=[1.0], D=0.2, K=0.1, moment=2.0)[0]
+    s = system()
+    e0 = s.init_spins("random", seed=3)
+    a = MetropolisMC(s, 5.0, seed=4, sigma=0.4, overrelax=1)
+    b = MetropolisMC(s, 50.0, seed=9, sigma=0.4, overrelax=0, move="heatbath")
+    ea, eb = gpu.as_tensor(e0, device="cuda"), gpu.as_tensor(e0, device="cuda")
+    for _ in range(3):
+        a.sweep_device(ea)
+        b.sweep_device(eb)
+    solo = []
+    for T, seed, orx, mv in ((5.0, 4, 1, "gaussian"), (50.0, 9, 0, "heatbath")):
+        m = MetropolisMC(system(), T, seed=seed, sigma=0.4, overrelax=orx, move=mv)
+        e = gpu.as_tensor(e0, device="cuda")
+        for _ in range(3):
+            m.sweep_device(e)
+        solo.append((e, m.acceptance()))
+    assert gpu.equal(ea, solo[0][0]) and gpu.equal(eb, solo[1][0])
+    assert a.acceptance() == solo[0][1] and b.acceptance() == solo[1][1]
+
+
 @pytest.mark.parametrize("move", ["gaussian", "heatbath"])
 def test_mc_fp32_langevin(gpu, move):
     """FP32 sweeps of free spins in a field sample <cos theta> = coth x - 1/x."""
