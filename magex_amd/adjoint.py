@@ -25,7 +25,7 @@ import torch
 from ._lib import check, hptr
 from .device import DeviceModel, _stream, _torch
 
-MAXP = 8
+MAXP = 16   # parameters the library differentiates in one backward (mxllg_adjoint.hip)
 
 
 def linear_units(build, theta_ref, tol=1e-12):

@@ -24,7 +24,7 @@
 
 namespace mx {
 
-constexpr int MAXP = 8;  // parameters
+constexpr int MAXP = 16;  // parameters (a full 3x3 tensor and a few more)
 
 template <typename R>
 struct AStatic {
@@ -72,7 +72,7 @@ __device__ __forceinline__ void onsite_T(const Static<R>* __restrict__ S, int s,
 
 // Adjoint gather at site st: H = sum_k J_k nu_j (J from the model's width-9 table), and for every
 // parameter t_p = sum_k U[k][p] x_j; then G_p = nu_i . (t_p - Aun[s][p] x_i) is accumulated into gp.
-// NP (4 or 8) is the parameter count rounded up; padded parameters have zero units.
+// NP (4, 8 or 16) is the parameter count rounded up; padded parameters have zero units.
 // Entries go in batches of CH whose loads are all issued before the first is used (entries past the
 // end re-read the last one with weight zero), so a wavefront keeps 2 CH gathers in flight.
 template <typename R, int GEO, int NP>
@@ -498,10 +498,14 @@ void sweep(mx_adjoint& a, char* seg, size_t b, int C, int n_seg, hipStream_t s) 
     if (geo == GEO_OPEN) sweep_t<R, GEO_OPEN, 4>(a, rec.data(), n_seg, s);
     else if (geo == GEO_PERIODIC) sweep_t<R, GEO_PERIODIC, 4>(a, rec.data(), n_seg, s);
     else sweep_t<R, GEO_TILED, 4>(a, rec.data(), n_seg, s);
-  } else {
+  } else if (a.P <= 8) {
     if (geo == GEO_OPEN) sweep_t<R, GEO_OPEN, 8>(a, rec.data(), n_seg, s);
     else if (geo == GEO_PERIODIC) sweep_t<R, GEO_PERIODIC, 8>(a, rec.data(), n_seg, s);
     else sweep_t<R, GEO_TILED, 8>(a, rec.data(), n_seg, s);
+  } else {
+    if (geo == GEO_OPEN) sweep_t<R, GEO_OPEN, 16>(a, rec.data(), n_seg, s);
+    else if (geo == GEO_PERIODIC) sweep_t<R, GEO_PERIODIC, 16>(a, rec.data(), n_seg, s);
+    else sweep_t<R, GEO_TILED, 16>(a, rec.data(), n_seg, s);
   }
 }
 
@@ -514,7 +518,7 @@ int mx_adjoint_create(mx_model* m, int32_t nparam, const double* unit_bonds, con
   return aguard([&]() -> int {
     if (!m || !out || !unit_bonds || !unit_onsite) afail(MX_EINVAL, "null argument");
     *out = nullptr;
-    if (nparam < 1 || nparam > MAXP) afail(MX_EUNSUPPORTED, "1 to 8 parameters");
+    if (nparam < 1 || nparam > MAXP) afail(MX_EUNSUPPORTED, "1 to 16 parameters");
     if (checkpoint_every < 1) afail(MX_EINVAL, "checkpoint interval must be positive");
     if (m->nent < 1) afail(MX_EINVAL, "set the stencil first");
     if (m->pinned_d || m->has_c || m->has_u || m->amode != ONSITE_SUB || m->coup == COUP_BOND)
