@@ -251,6 +251,17 @@ int mx_adjoint_forward(mx_adjoint* a, void* e, int64_t nsteps, uint64_t seed, ui
 /* Given grad_e = dL/de_final (device, N x 3), writes dL/de_0 into grad_e0 (device, N x 3, may be null)
  * and dL/dtheta into grad_theta (device, nparam doubles), for the last mx_adjoint_forward. */
 int mx_adjoint_backward(mx_adjoint* a, const void* grad_e, void* grad_e0, double* grad_theta, void* stream);
+/* mx_adjoint_backward plus the dense gradients at the current couplings (either may be null):
+ * grad_bonds (device, nbonds x ncell x 9 doubles, the layout of mx_model_set_bond_tensors) receives
+ * dL/dT for every unordered bond's tensor T (J_ij = T, J_ji = T^T), and grad_onsite (device, N x 9
+ * doubles, magex site order) dL/dA_i for every site's on-site tensor.  The antisymmetric part of
+ * dL/dT gives dL/dD of the bond's DMI vector (T_ab = eps_abc D_c: dL/dD_c = eps_abc dL/dT_ab).  No
+ * counterpart in magex (its solvers do not differentiate; row-major DMI convention of
+ * magex/models.py:37). */
+int mx_adjoint_backward_dense(mx_adjoint* a, const void* grad_e, void* grad_e0, double* grad_theta,
+                              double* grad_bonds, double* grad_onsite, void* stream);
+/* nbonds: forward stencil entries (s < t, or s == t and R > 0), as mx_model_set_bond_tensors counts them. */
+int mx_adjoint_num_bonds(const mx_adjoint* a);
 
 #ifdef __cplusplus
 }

@@ -13,7 +13,9 @@ differencing, and are checked to reproduce the builder's couplings.
     loss = ((e - target) ** 2).mean(); loss.backward()
 
 The backward replays each checkpoint interval (default 50 steps) and runs the adjoint kernels on the
-device; see mxllg_adjoint.hip for the equations.
+device; see mxllg_adjoint.hip for the equations.  ``backward_dense`` also returns the gradient with
+respect to every bond's 3x3 tensor and every site's on-site tensor at the current couplings
+(``dmi_gradient`` takes the DMI vectors' part of the former).
 """
 from __future__ import annotations
 
@@ -81,10 +83,10 @@ class DifferentiableLLG:
     build(theta) -> SpinSystem defines the Hamiltonian (its field, moments and stencil are taken at
     theta0; couplings and on-site tensors must be linear in theta).  alpha, dt (ps), temperature (K) and
     seed as LLGSolver; the thermal field is a fixed realization (the library's Philox stream), so the
-    gradient is that of one sample path.  precision "fp64" or "fp32"."""
+    gradient is that of one sample path.  precision "fp64" or "fp32"; layout as DeviceModel."""
 
     def __init__(self, build, theta0, alpha, dt, temperature=0.0, seed=0, precision="fp64", checkpoint_every=50,
-                 precession=True):
+                 precession=True, layout=None):
         _torch()
         U, Aun, offset, system = linear_units(build, theta0)
         self.P = len(theta0)
@@ -101,7 +103,7 @@ class DifferentiableLLG:
         self.dtype = torch.float64 if precision == "fp64" else torch.float32
         self.N = system.N
         self.seed = int(seed)
-        self.dm = DeviceModel(system, precision)
+        self.dm = DeviceModel(system, precision, layout=layout)
         self.dm.set_dynamics(dt, alpha, temperature, precession)
         self.checkpoint_every = int(checkpoint_every)
         L = self.dm.lib
@@ -157,6 +159,28 @@ class DifferentiableLLG:
                                               ctypes.c_void_p(gt.data_ptr()), _stream()))
         return gt[:self.P], g0
 
+    @property
+    def nbonds(self):
+        """Unordered bonds per cell: the forward stencil entries (SpinSystem.forward_entries)."""
+        return int(self.dm.lib.mx_adjoint_num_bonds(self.h))
+
+    def backward_dense(self, grad_e, want_e0=True):
+        """(dL/dtheta, dL/de0 or None, dL/dT (nbonds, ncell, 3, 3), dL/dA (N, 3, 3)) for the last forward().
+
+        T is each unordered bond's tensor in the layout of SpinSystem.set_bond_tensors (it acts as J_ij = T
+        from the forward entry's site and J_ji = T^T back), A each site's on-site tensor; both are
+        evaluated at the current couplings.  float64 device tensors."""
+        g = grad_e.detach().to(device="cuda", dtype=self.dtype).contiguous()
+        gt = torch.empty(MAXP, dtype=torch.float64, device="cuda")
+        g0 = torch.empty_like(g) if want_e0 else None
+        gb = torch.empty((self.nbonds, self.system.ncell, 3, 3), dtype=torch.float64, device="cuda")
+        ga = torch.empty((self.N, 3, 3), dtype=torch.float64, device="cuda")
+        check(self.dm.lib.mx_adjoint_backward_dense(self.h, ctypes.c_void_p(g.data_ptr()),
+                                                    ctypes.c_void_p(g0.data_ptr() if g0 is not None else 0),
+                                                    ctypes.c_void_p(gt.data_ptr()), ctypes.c_void_p(gb.data_ptr()),
+                                                    ctypes.c_void_p(ga.data_ptr()), _stream()))
+        return gt[:self.P], g0, gb, ga
+
     def trajectory(self, theta, e0, nsteps, step0=0):
         """Differentiable e(nsteps) as a function of theta (torch tensor, P) and e0 (N, 3)."""
         if not isinstance(theta, torch.Tensor):
@@ -192,3 +216,10 @@ class _HeunTrajectory(torch.autograd.Function):
         if g0 is not None:
             g0 = g0.to(dtype=edt, device=edev)
         return gt, g0, None, None, None
+
+
+def dmi_gradient(grad_bonds):
+    """dL/dD (..., 3) of each bond's DMI vector from dL/dT (..., 3, 3): the DMI part of a bond tensor is
+    T_ab = eps_abc D_c (magex/models.py:37), so dL/dD_c = eps_abc dL/dT_ab."""
+    G = grad_bonds
+    return torch.stack([G[..., 1, 2] - G[..., 2, 1], G[..., 2, 0] - G[..., 0, 2], G[..., 0, 1] - G[..., 1, 0]], -1)

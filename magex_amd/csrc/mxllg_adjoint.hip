@@ -12,12 +12,16 @@
 //   field:          nu_i = 2 dL/dB_i / (mu_i mu_B);  dL/de_j += sum_i J_ij^T nu_i - A_j^T nu_j, and since
 //                   J_ji = J_ij^T the first term is the forward gather applied to nu
 //   parameters:     dL/dtheta_p += sum_i nu_i . (sum_k U[p][k] x_j(k) - Aun[p] x_i)  (x = e or ep).
+//   dense:          per unordered bond b = (i, j) stored once (J_ij = T_b, J_ji = T_b^T):
+//                   dL/dT_b += nu_i x_j^T + x_i nu_j^T;  per site dL/dA_i += -nu_i x_i^T.
 // The forward run keeps the spins every C steps; the backward replays each segment from its
 // checkpoint (saving e, ep, w = e + dt/2 k1, B(e), B(ep) per step, noise regenerated from the same counters) and
 // runs three kernels per step: one local, two gathers.
 #include <cmath>
 #include <cstring>
+#include <algorithm>
 #include <new>
+#include <tuple>
 #include <vector>
 
 #include "mxllg_field.h"
@@ -29,6 +33,7 @@ constexpr int MAXP = 16;  // parameters (a full 3x3 tensor and a few more)
 template <typename R>
 struct AStatic {
   int P;
+  int fslot[MAXENT];       // forward-bond slot of each stencil entry (library order), -1 for reverse entries
   R U[MAXENT][MAXP][9];    // unit bond tensors per stencil entry (library order) and parameter
   R Aun[MAXSUB][MAXP][9];  // unit on-site tensors per sublattice and parameter
 };
@@ -75,11 +80,14 @@ __device__ __forceinline__ void onsite_T(const Static<R>* __restrict__ S, int s,
 // NP (4, 8 or 16) is the parameter count rounded up; padded parameters have zero units.
 // Entries go in batches of CH whose loads are all issued before the first is used (entries past the
 // end re-read the last one with weight zero), so a wavefront keeps 2 CH gathers in flight.
-template <typename R, int GEO, int NP>
+// DENSE also adds nu_i x_j^T + x_i nu_j^T to gb[(f * 9 + q) * ncell + cell] for every forward entry f
+// of the site (each unordered bond has one owner thread, so no two threads write one element) and
+// -nu_i x_i^T to ga[q * N + site].
+template <typename R, int GEO, int NP, bool DENSE>
 __device__ __forceinline__ void adj_gather(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
                                            const R* __restrict__ nu, const R* __restrict__ x, const Site& st,
                                            R nix, R niy, R niz, R xix, R xiy, R xiz, R& hx, R& hy, R& hz,
-                                           double (&gp)[NP]) {
+                                           double (&gp)[NP], double* __restrict__ gb, double* __restrict__ ga) {
   constexpr int CH = 8;
   const int ncell = S->ncell;
   R t[NP][3];
@@ -126,7 +134,29 @@ __device__ __forceinline__ void adj_gather(const Static<R>* __restrict__ S, cons
         t[p][1] += U[3] * qx[u] + U[4] * qy[u] + U[5] * qz[u];
         t[p][2] += U[6] * qx[u] + U[7] * qy[u] + U[8] * qz[u];
       }
+      if constexpr (DENSE) {
+        const int f = A->fslot[kk[u]];
+        if (k0 + u < ke && f >= 0) {
+          const double ni[3] = {double(nix), double(niy), double(niz)}, xi[3] = {double(xix), double(xiy), double(xiz)};
+          const double nj[3] = {double(vx[u]), double(vy[u]), double(vz[u])};
+          const double xj[3] = {double(qx[u]), double(qy[u]), double(qz[u])};
+          double* g = gb + size_t(f) * 9 * ncell + st.c;
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) g[size_t(3 * r + c) * ncell] += ni[r] * xj[c] + xi[r] * nj[c];
+        }
+      }
     }
+  }
+  if constexpr (DENSE) {
+    const double ni[3] = {double(nix), double(niy), double(niz)}, xi[3] = {double(xix), double(xiy), double(xiz)};
+    double* g = ga + st.i;
+    const size_t N = size_t(S->N);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) g[(3 * r + c) * N] -= ni[r] * xi[c];
   }
 #pragma unroll
   for (int p = 0; p < NP; ++p) {
@@ -201,11 +231,12 @@ __device__ __forceinline__ void store_partials(double (&gp)[NP], double old, dou
 }
 
 // Gather nu2 (dL/dep) and the predictor's adjoint: writes nu1 and the partial dL/de (part).
-template <typename R, int GEO, int NP>
+template <typename R, int GEO, int NP, bool DENSE>
 __global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
                                                  const Dyn<R> d, const StepRec<R> r, const R* __restrict__ lamu,
                                                  const R* __restrict__ gep, const R* __restrict__ nu2,
-                                                 R* __restrict__ nu1, R* __restrict__ part, double* __restrict__ gpart) {
+                                                 R* __restrict__ nu1, R* __restrict__ part, double* __restrict__ gpart,
+                                                 double* __restrict__ gb, double* __restrict__ ga) {
   const double old = partial_row<NP>(gpart);
   double gp[NP];
 #pragma unroll
@@ -217,7 +248,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S
     const R px = r.ep[3 * i], py = r.ep[3 * i + 1], pz = r.ep[3 * i + 2];
     const R vx2 = nu2[3 * i], vy2 = nu2[3 * i + 1], vz2 = nu2[3 * i + 2];
     R hx, hy, hz, ox, oy, oz;
-    adj_gather<R, GEO, NP>(S, A, nu2, r.ep, st, vx2, vy2, vz2, px, py, pz, hx, hy, hz, gp);
+    adj_gather<R, GEO, NP, DENSE>(S, A, nu2, r.ep, st, vx2, vy2, vz2, px, py, pz, hx, hy, hz, gp, gb, ga);
     onsite_T(S, st.s, vx2, vy2, vz2, ox, oy, oz);
     R lx = gep[3 * i] + hx - ox, ly = gep[3 * i + 1] + hy - oy, lz = gep[3 * i + 2] + hz - oz;  // dL/dep
     const R ex = r.e[3 * i], ey = r.e[3 * i + 1], ez = r.e[3 * i + 2];
@@ -246,12 +277,13 @@ __global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S
 // Gather nu1: lambda = dL/de of the step's input, and the parameter terms at e.  With FUSE the
 // previous step's local part (adj_local on its record prev) follows at once from lambda, which then
 // never goes through memory; otherwise lambda is stored for a later k_adj_a.
-template <typename R, int GEO, int NP, bool FUSE>
+template <typename R, int GEO, int NP, bool FUSE, bool DENSE>
 __global__ __launch_bounds__(BLOCK) void k_adj_c(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
                                                  const Dyn<R> d, const R* __restrict__ e, const R* __restrict__ nu1,
                                                  const R* __restrict__ part, R* __restrict__ lam,
                                                  const StepRec<R> prev, R* __restrict__ lamu, R* __restrict__ gep,
-                                                 R* __restrict__ nu2, double* __restrict__ gpart) {
+                                                 R* __restrict__ nu2, double* __restrict__ gpart,
+                                                 double* __restrict__ gb, double* __restrict__ ga) {
   const double old = partial_row<NP>(gpart);
   double gp[NP];
 #pragma unroll
@@ -261,7 +293,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_c(const Static<R>* __restrict__ S
     const int i = st.i;
     const R vx = nu1[3 * i], vy = nu1[3 * i + 1], vz = nu1[3 * i + 2];
     R hx, hy, hz, ox, oy, oz;
-    adj_gather<R, GEO, NP>(S, A, nu1, e, st, vx, vy, vz, e[3 * i], e[3 * i + 1], e[3 * i + 2], hx, hy, hz, gp);
+    adj_gather<R, GEO, NP, DENSE>(S, A, nu1, e, st, vx, vy, vz, e[3 * i], e[3 * i + 1], e[3 * i + 2], hx, hy, hz, gp,
+                                  gb, ga);
     onsite_T(S, st.s, vx, vy, vz, ox, oy, oz);
     const R lx = part[3 * i] + hx - ox, ly = part[3 * i + 1] + hy - oy, lz = part[3 * i + 2] + hz - oz;
     if constexpr (FUSE) {
@@ -287,6 +320,31 @@ __global__ void k_copy(const T* __restrict__ a, T* __restrict__ b, size_t n) {
   if (i < n) b[i] = a[i];
 }
 
+// Dense gradients from the accumulators (component-major, internal cells) to the caller's layouts:
+// bonds [f][magex cell][9], on-site [magex site][9]; cmap[c] is the internal cell of magex cell c
+// (null: the identity).
+__global__ void k_dense_bonds(const double* __restrict__ gb, const int* __restrict__ cmap, int nfwd, int ncell,
+                              double* __restrict__ out) {
+  const size_t n = size_t(nfwd) * ncell * 9;
+  const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const size_t fc = idx / 9;
+  const int q = int(idx - fc * 9);
+  const int f = int(fc / size_t(ncell)), c = int(fc - size_t(f) * ncell);
+  out[idx] = gb[(size_t(f) * 9 + q) * ncell + (cmap ? cmap[c] : c)];
+}
+
+__global__ void k_dense_onsite(const double* __restrict__ ga, const int* __restrict__ cmap, int ns, int ncell,
+                               double* __restrict__ out) {
+  const size_t N = size_t(ns) * ncell;
+  const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= N * 9) return;
+  const size_t r = idx / 9;
+  const int q = int(idx - r * 9);
+  const int c = int(r / size_t(ns)), s = int(r - size_t(c) * ns);
+  out[idx] = ga[size_t(q) * N + size_t(s) * ncell + (cmap ? cmap[c] : c)];
+}
+
 }  // namespace mx
 
 // ============================================================================================ host
@@ -297,6 +355,8 @@ struct mx_adjoint {
   int P = 0, C = 50;
   std::vector<double> U, Aun;  // P x nent x 9 (library order), P x ns x 9
   std::vector<double> theta;
+  std::vector<int> fslot;  // forward-bond slot per library entry (-1: reverse entry)
+  int nfwd = 0;
   void* astatic_d = nullptr;
   // forward record
   bool recorded = false;
@@ -313,6 +373,11 @@ struct mx_adjoint {
   double* gpart_d = nullptr;  // per-workgroup parameter-gradient partials, nblk x MAXP
   double* gout_d = nullptr;   // their fold, MAXP
   int nblk = 0;
+  // dense gradients: per forward bond 9 x ncell, per site 9 x N (FP64, internal cells), and the magex
+  // cell -> internal cell map of the layout cmap_geo (null for row-major)
+  double *gb_d = nullptr, *ga_d = nullptr;
+  int* cmap_d = nullptr;
+  int cmap_geo = -1;
 };
 
 namespace {
@@ -351,6 +416,7 @@ void upload_astatic(mx_adjoint& a) {
   AStatic<R> A;
   std::memset(&A, 0, sizeof(A));
   A.P = a.P;
+  for (int k = 0; k < MAXENT; ++k) A.fslot[k] = k < m.nent ? a.fslot[k] : -1;
   for (int p = 0; p < a.P; ++p) {
     for (int k = 0; k < m.nent; ++k)
       for (int q = 0; q < 9; ++q) A.U[k][p][q] = R(a.U[(size_t(p) * m.nent + k) * 9 + q]);
@@ -457,7 +523,7 @@ void ensure_buffers(mx_adjoint& a) {
 // Adjoint sweep over one replayed segment of n_seg steps (records rec(n), newest last): the local
 // part of the segment's last step from lam, then per step (newest first) k_adj_b and k_adj_c, the
 // latter fused with the next-older step's local part.  Leaves dL/de of the segment's first input in lam.
-template <typename R, int GEO, int NP>
+template <typename R, int GEO, int NP, bool DENSE>
 void sweep_t(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
   const Model& m = *a.m;
   const Static<R>* S = static_cast<const Static<R>*>(m.static_d);
@@ -476,37 +542,104 @@ void sweep_t(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
   const dim3 g(a.nblk), b(BLOCK);
   hipLaunchKernelGGL((k_adj_a<R, GEO>), g, b, 0, s, S, d, lam, rec[n_seg - 1], lamu, gep, nu2);
   for (int n = n_seg - 1; n >= 0; --n) {
-    hipLaunchKernelGGL((k_adj_b<R, GEO, NP>), g, b, 0, s, S, A, d, rec[n], lamu, gep, nu2, nu1, part, a.gpart_d);
+    hipLaunchKernelGGL((k_adj_b<R, GEO, NP, DENSE>), g, b, 0, s, S, A, d, rec[n], lamu, gep, nu2, nu1, part, a.gpart_d,
+                       a.gb_d, a.ga_d);
     if (n > 0)
-      hipLaunchKernelGGL((k_adj_c<R, GEO, NP, true>), g, b, 0, s, S, A, d, rec[n].e, nu1, part, lam, rec[n - 1], lamu,
-                         gep, nu2, a.gpart_d);
+      hipLaunchKernelGGL((k_adj_c<R, GEO, NP, true, DENSE>), g, b, 0, s, S, A, d, rec[n].e, nu1, part, lam, rec[n - 1],
+                         lamu, gep, nu2, a.gpart_d, a.gb_d, a.ga_d);
     else
-      hipLaunchKernelGGL((k_adj_c<R, GEO, NP, false>), g, b, 0, s, S, A, d, rec[n].e, nu1, part, lam, rec[n], lamu, gep,
-                         nu2, a.gpart_d);
+      hipLaunchKernelGGL((k_adj_c<R, GEO, NP, false, DENSE>), g, b, 0, s, S, A, d, rec[n].e, nu1, part, lam, rec[n],
+                         lamu, gep, nu2, a.gpart_d, a.gb_d, a.ga_d);
   }
 }
 
+template <typename R, int NP, bool DENSE>
+void sweep_geo(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
+  const int geo = a.m->geo;
+  if (geo == GEO_OPEN) sweep_t<R, GEO_OPEN, NP, DENSE>(a, rec, n_seg, s);
+  else if (geo == GEO_PERIODIC) sweep_t<R, GEO_PERIODIC, NP, DENSE>(a, rec, n_seg, s);
+  else sweep_t<R, GEO_TILED, NP, DENSE>(a, rec, n_seg, s);
+}
+
+template <typename R, bool DENSE>
+void sweep_np(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
+  if (a.P <= 4) sweep_geo<R, 4, DENSE>(a, rec, n_seg, s);
+  else if (a.P <= 8) sweep_geo<R, 8, DENSE>(a, rec, n_seg, s);
+  else sweep_geo<R, 16, DENSE>(a, rec, n_seg, s);
+}
+
 template <typename R>
-void sweep(mx_adjoint& a, char* seg, size_t b, int C, int n_seg, hipStream_t s) {
+void sweep(mx_adjoint& a, char* seg, size_t b, int C, int n_seg, bool dense, hipStream_t s) {
   std::vector<StepRec<R>> rec(n_seg);
   for (int n = 0; n < n_seg; ++n) {
     auto at = [&](int which) { return reinterpret_cast<const R*>(seg + (size_t(which) * C + n) * b); };
     rec[n] = StepRec<R>{at(0), at(1), at(2), at(3), at(4)};
   }
-  const int geo = a.m->geo;
-  if (a.P <= 4) {
-    if (geo == GEO_OPEN) sweep_t<R, GEO_OPEN, 4>(a, rec.data(), n_seg, s);
-    else if (geo == GEO_PERIODIC) sweep_t<R, GEO_PERIODIC, 4>(a, rec.data(), n_seg, s);
-    else sweep_t<R, GEO_TILED, 4>(a, rec.data(), n_seg, s);
-  } else if (a.P <= 8) {
-    if (geo == GEO_OPEN) sweep_t<R, GEO_OPEN, 8>(a, rec.data(), n_seg, s);
-    else if (geo == GEO_PERIODIC) sweep_t<R, GEO_PERIODIC, 8>(a, rec.data(), n_seg, s);
-    else sweep_t<R, GEO_TILED, 8>(a, rec.data(), n_seg, s);
-  } else {
-    if (geo == GEO_OPEN) sweep_t<R, GEO_OPEN, 16>(a, rec.data(), n_seg, s);
-    else if (geo == GEO_PERIODIC) sweep_t<R, GEO_PERIODIC, 16>(a, rec.data(), n_seg, s);
-    else sweep_t<R, GEO_TILED, 16>(a, rec.data(), n_seg, s);
+  if (dense) sweep_np<R, true>(a, rec.data(), n_seg, s);
+  else sweep_np<R, false>(a, rec.data(), n_seg, s);
+}
+
+// Accumulators of the dense gradients (cleared) and, for a permuted layout, the cell map.
+void ensure_dense(mx_adjoint& a, hipStream_t s) {
+  const Model& m = *a.m;
+  const size_t nb = size_t(a.nfwd) * 9 * m.ncell, na = size_t(9) * m.N;
+  if (!a.gb_d) aok(hipMalloc(reinterpret_cast<void**>(&a.gb_d), sizeof(double) * std::max<size_t>(nb, 1)), "hipMalloc dense bond gradient");
+  if (!a.ga_d) aok(hipMalloc(reinterpret_cast<void**>(&a.ga_d), sizeof(double) * na), "hipMalloc dense on-site gradient");
+  if (nb) hipLaunchKernelGGL((k_zero<double>), dim3((nb + 255) / 256), dim3(256), 0, s, a.gb_d, nb);
+  hipLaunchKernelGGL((k_zero<double>), dim3((na + 255) / 256), dim3(256), 0, s, a.ga_d, na);
+  if (m.geo == GEO_TILED && a.cmap_geo != m.geo) {
+    std::vector<int> map(m.ncell);
+    for (int c = 0; c < m.ncell; ++c) map[c] = cell_to_internal(m, c);
+    if (!a.cmap_d) aok(hipMalloc(reinterpret_cast<void**>(&a.cmap_d), sizeof(int) * m.ncell), "hipMalloc cell map");
+    aok(hipMemcpy(a.cmap_d, map.data(), sizeof(int) * m.ncell, hipMemcpyHostToDevice), "hipMemcpy cell map");
+    a.cmap_geo = m.geo;
   }
+}
+
+// The adjoint sweep over every segment, newest first; with grad_bonds / grad_onsite also the dense
+// gradients (either may be null).
+int backward_impl(mx_adjoint* a, const void* grad_e, void* grad_e0, double* grad_theta, double* grad_bonds,
+                  double* grad_onsite, void* stream) {
+  if (!a || !grad_e || !grad_theta) afail(MX_EINVAL, "null argument");
+  Model& m = *a->m;
+  if (!a->recorded) afail(MX_EINVAL, "no forward recorded for the current parameters: run mx_adjoint_forward");
+  aok(hipSetDevice(m.device), "hipSetDevice");
+  aok_rc(sync_model_static(m), "model constants");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t b = nbytes3(m);
+  const bool fp64 = m.prec == MX_FP64;
+  const bool dense = grad_bonds || grad_onsite;
+  if (dense) ensure_dense(*a, s);
+  // the adjoint works in the library's internal site order; permute at entry and exit
+  to_internal(*a, grad_e, a->lam_d, s);
+  hipLaunchKernelGGL((k_zero<double>), dim3((MAXP * a->nblk + 255) / 256), dim3(256), 0, s, a->gpart_d,
+                     size_t(MAXP) * a->nblk);
+  char* ck = static_cast<char*>(a->ckpt_d);
+  const int C = a->C;
+  for (int64_t k = (a->nsteps - 1) / C; k >= 0 && a->nsteps > 0; --k) {
+    const int n_seg = int(std::min<int64_t>(C, a->nsteps - k * C));
+    // replay the segment from its checkpoint unless its records are still held
+    if (a->seg_k != k) record_segment(*a, ck + size_t(k) * b, k, n_seg, s);
+    if (fp64) sweep<double>(*a, static_cast<char*>(a->seg_d), b, a->seg_cap, n_seg, dense, s);
+    else sweep<float>(*a, static_cast<char*>(a->seg_d), b, a->seg_cap, n_seg, dense, s);
+  }
+  hipLaunchKernelGGL((k_final<MAXP>), dim3(1), dim3(BLOCK), 0, s, a->gpart_d, a->nblk, MAXP, 1.0, a->gout_d);
+  hipLaunchKernelGGL((k_copy<double>), dim3(1), dim3(64), 0, s, a->gout_d, grad_theta, size_t(a->P));
+  if (grad_e0) to_magex(*a, a->lam_d, grad_e0, s);
+  const int* cmap = m.geo == GEO_TILED ? a->cmap_d : nullptr;
+  if (grad_bonds && a->nfwd) {
+    const size_t n = size_t(a->nfwd) * m.ncell * 9;
+    hipLaunchKernelGGL(k_dense_bonds, dim3((n + 255) / 256), dim3(256), 0, s, a->gb_d, cmap, a->nfwd, m.ncell,
+                       grad_bonds);
+  }
+  if (grad_onsite) {
+    const size_t n = size_t(m.N) * 9;
+    hipLaunchKernelGGL(k_dense_onsite, dim3((n + 255) / 256), dim3(256), 0, s, a->ga_d, cmap, m.ns, m.ncell,
+                       grad_onsite);
+  }
+  const hipError_t err = hipGetLastError();
+  if (err != hipSuccess) afail(MX_EHIP, std::string("adjoint: ") + hipGetErrorString(err));
+  return MX_OK;
 }
 
 }  // namespace
@@ -559,6 +692,15 @@ int mx_adjoint_create(mx_model* m, int32_t nparam, const double* unit_bonds, con
             a->U[(size_t(p) * n + pos_of[k]) * 9 + q] = unit_bonds[(size_t(p) * n + k) * 9 + q];
       a->Aun.assign(unit_onsite, unit_onsite + size_t(nparam) * m->ns * 9);
       a->theta.assign(nparam, 0.0);
+      // forward entries (s < t, or s == t and R > 0) get bond slots in the caller's order
+      std::vector<int> fw(n, -1);
+      for (int k = 0; k < n; ++k) {
+        const std::tuple<int, int, int> R{raw(k, 2), raw(k, 3), raw(k, 4)};
+        if (raw(k, 0) < raw(k, 1) || (raw(k, 0) == raw(k, 1) && R > std::tuple<int, int, int>{0, 0, 0}))
+          fw[k] = a->nfwd++;
+      }
+      a->fslot.assign(n, -1);
+      for (int k = 0; k < n; ++k) a->fslot[pos_of[k]] = fw[k];
       aok(hipSetDevice(m->device), "hipSetDevice");
       if (m->prec == MX_FP64) upload_astatic<double>(*a);
       else upload_astatic<float>(*a);
@@ -577,7 +719,8 @@ void mx_adjoint_destroy(mx_adjoint* a) {
   if (a->m) (void)hipSetDevice(a->m->device);
   for (void** p : {&a->astatic_d, &a->ckpt_d, &a->seg_d, &a->lam_d, &a->lamu_d, &a->gep_d, &a->nu1_d, &a->nu2_d,
                    &a->part_d, &a->ework_d, reinterpret_cast<void**>(&a->gpart_d),
-                   reinterpret_cast<void**>(&a->gout_d)})
+                   reinterpret_cast<void**>(&a->gout_d), reinterpret_cast<void**>(&a->gb_d),
+                   reinterpret_cast<void**>(&a->ga_d), reinterpret_cast<void**>(&a->cmap_d)})
     afree(*p);
   delete a;
 }
@@ -643,35 +786,14 @@ int mx_adjoint_forward(mx_adjoint* a, void* e, int64_t nsteps, uint64_t seed, ui
 }
 
 int mx_adjoint_backward(mx_adjoint* a, const void* grad_e, void* grad_e0, double* grad_theta, void* stream) {
-  return aguard([&]() -> int {
-    if (!a || !grad_e || !grad_theta) afail(MX_EINVAL, "null argument");
-    Model& m = *a->m;
-    if (!a->recorded) afail(MX_EINVAL, "no forward recorded for the current parameters: run mx_adjoint_forward");
-    aok(hipSetDevice(m.device), "hipSetDevice");
-    aok_rc(sync_model_static(m), "model constants");
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    const size_t b = nbytes3(m), nel = size_t(m.N) * 3;
-    const bool fp64 = m.prec == MX_FP64;
-    // the adjoint works in the library's internal site order; permute at entry and exit
-    to_internal(*a, grad_e, a->lam_d, s);
-    hipLaunchKernelGGL((k_zero<double>), dim3((MAXP * a->nblk + 255) / 256), dim3(256), 0, s, a->gpart_d,
-                       size_t(MAXP) * a->nblk);
-    char* ck = static_cast<char*>(a->ckpt_d);
-    const int C = a->C;
-    for (int64_t k = (a->nsteps - 1) / C; k >= 0 && a->nsteps > 0; --k) {
-      const int n_seg = int(std::min<int64_t>(C, a->nsteps - k * C));
-      // replay the segment from its checkpoint unless its records are still held
-      if (a->seg_k != k) record_segment(*a, ck + size_t(k) * b, k, n_seg, s);
-      if (fp64) sweep<double>(*a, static_cast<char*>(a->seg_d), b, a->seg_cap, n_seg, s);
-      else sweep<float>(*a, static_cast<char*>(a->seg_d), b, a->seg_cap, n_seg, s);
-    }
-    hipLaunchKernelGGL((k_final<MAXP>), dim3(1), dim3(BLOCK), 0, s, a->gpart_d, a->nblk, MAXP, 1.0, a->gout_d);
-    hipLaunchKernelGGL((k_copy<double>), dim3(1), dim3(64), 0, s, a->gout_d, grad_theta, size_t(a->P));
-    if (grad_e0) to_magex(*a, a->lam_d, grad_e0, s);
-    const hipError_t err = hipGetLastError();
-    if (err != hipSuccess) afail(MX_EHIP, std::string("adjoint: ") + hipGetErrorString(err));
-    return MX_OK;
-  });
+  return aguard([&]() -> int { return backward_impl(a, grad_e, grad_e0, grad_theta, nullptr, nullptr, stream); });
 }
+
+int mx_adjoint_backward_dense(mx_adjoint* a, const void* grad_e, void* grad_e0, double* grad_theta,
+                              double* grad_bonds, double* grad_onsite, void* stream) {
+  return aguard([&]() -> int { return backward_impl(a, grad_e, grad_e0, grad_theta, grad_bonds, grad_onsite, stream); });
+}
+
+int mx_adjoint_num_bonds(const mx_adjoint* a) { return a ? a->nfwd : -1; }
 
 }  // extern "C"

@@ -77,6 +77,8 @@ static inline int internal_cell(const Model& m, int c) {
   return tile * BLOCK + l;
 }
 
+int cell_to_internal(const Model& m, int c) { return internal_cell(m, c); }
+
 // magex order site r = c * ns + s -> internal order q = s * ncell + internal_cell(c)
 static inline int internal_index(const Model& m, int r) {
   const int c = r / m.ns, s = r - c * m.ns;

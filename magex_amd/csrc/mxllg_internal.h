@@ -171,6 +171,7 @@ int launch_observables(const Model& m, const void* e_int, double* out, hipStream
 int launch_noise(const Model& m, void* out_ref, uint64_t seed, uint64_t step, hipStream_t s);
 int launch_permute(const Model& m, const void* src, void* dst, int to_internal, hipStream_t s);
 bool needs_permute(const Model& m);
+int cell_to_internal(const Model& m, int c);  // internal cell of row-major (magex) cell c
 int launch_rhs(const Model& m, const void* e, const void* B, double alpha, int prec, void* out, hipStream_t s);
 // One Monte Carlo pass over n sites of one colour (mxllg_mc.hip); pass 0 = the move, > 0 = over-relaxation.
 int launch_mc(const Model& m, void* e, const int32_t* sites, int n, const void* beta_site, double beta, int move,

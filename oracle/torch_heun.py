@@ -8,7 +8,10 @@ theta, Jsp(theta) = sum_p theta_p Jsp_p (+ a fixed remainder), A(theta) likewise
 
 Pinning (tests/test_oracle.py): the forward equals magex_ref.run_heun at the same theta to round-off
 (magex_ref is pinned to magex's golden trajectories); the gradient is torch's reverse mode of that
-forward, checked against central finite differences of the same forward.
+forward, checked against central finite differences of the same forward.  ``run_dense`` is the same
+trajectory with the coupling matrix (dense 3N x 3N) and the on-site tensors as free tensors, for
+gradients with respect to every bond and site; pinned to ``run`` (same trajectory, and the chain rule
+dL/dtheta_p = <Jsp_p, dL/dJ> + <A_p, dL/dA>) and to central differences of single elements.
 """
 from __future__ import annotations
 
@@ -77,5 +80,29 @@ class TorchHeun:
             k1 = self.rhs(e, self.effective_field(th, e) + b)
             ep = self.normalize(e + dt * k1)
             k2 = self.rhs(ep, self.effective_field(th, ep) + b)
+            e = self.normalize(e + 0.5 * dt * (k1 + k2))
+        return e
+
+    def dense(self, th):
+        """(J (3N, 3N), A (N, 3, 3)) at theta th as dense torch FP64 tensors."""
+        thf = torch.cat([torch.as_tensor(th, dtype=torch.float64).reshape(-1), torch.ones(1, dtype=torch.float64)])
+        J = sum(thf[p] * self.Jp[p].to_dense() for p in range(len(self.Jp)))
+        A = torch.einsum("p,piab->iab", thf, self.Ap)
+        return J, A
+
+    def effective_field_dense(self, J, A, e):
+        """B_eff (spinmodel.py:458-473) with the coupling matrix J (3N, 3N) and on-site tensors A (N, 3, 3)."""
+        h = (J @ e.reshape(-1)).reshape(-1, 3)
+        g = -2.0 * h - R.MU_B * self.mu[:, None] * self.B[None, :] + 2.0 * torch.einsum("iab,ib->ia", A, e)
+        return -g / (R.MU_B * self.mu[:, None])
+
+    def run_dense(self, J, A, e, steps, thermal=None):
+        """run() with the couplings given as the dense matrix J and on-site tensors A."""
+        dt = self.dt
+        for n in range(steps):
+            b = 0.0 if thermal is None else thermal(n)
+            k1 = self.rhs(e, self.effective_field_dense(J, A, e) + b)
+            ep = self.normalize(e + dt * k1)
+            k2 = self.rhs(ep, self.effective_field_dense(J, A, ep) + b)
             e = self.normalize(e + 0.5 * dt * (k1 + k2))
         return e

@@ -249,3 +249,91 @@ def test_autograd_function(gpu):
     assert theta.grad.dtype == torch.float32
     assert _relerr(theta.grad.double().numpy(), gt) < 1e-6
     assert _relerr(x0.grad.cpu().numpy(), g0) < 1e-12
+
+
+def _forward_pairs(system):
+    """(src, dst) magex site arrays of every forward stencil entry (s < t, or s == t and R > 0), in
+    stencil order, for a lattice periodic on every axis: the bond's tensor acts at block (src, dst)
+    of the coupling matrix and its transpose at (dst, src)."""
+    L = np.array(system.L)
+    ns = system.uc.nsites
+    cells = np.stack(np.meshgrid(*[np.arange(n) for n in L], indexing="ij"), -1).reshape(-1, 3)
+    out = []
+    for s, t, Rv, _ in system.stencil:
+        if not (s < t or (s == t and tuple(Rv) > (0, 0, 0))):
+            continue
+        tgt = (cells + np.asarray(Rv)) % L
+        cn = (tgt[:, 0] * L[1] + tgt[:, 1]) * L[2] + tgt[:, 2]
+        out.append((np.arange(len(cells)) * ns + s, cn * ns + t))
+    return out
+
+
+def test_adjoint_dense_gradients_match_autograd(gpu):
+    """dL/dT per bond and dL/dA per site (FP64, square 8^2 with J2, DMI and anisotropy, 60 steps over
+    three checkpoint segments) against FP64 autograd of the dense coupling matrix and on-site tensors:
+    dL/dT_b = G(i, j) + G(j, i)^T from the matrix gradient G, to 1e-9 of the largest element.  The
+    parameter gradient of the same call equals mx_adjoint_backward's bit for bit."""
+    print("case: dense adjoint gradients vs autograd, square 8^2, 60 steps, C=25", flush=True)
+    from magex_amd.adjoint import dmi_gradient
+    L, steps = 8, 60
+    sol = _solver(L, C=25)
+    _, ref_build = _builders(L)
+    o = ref_build(THETA0)
+    e0 = o.init_random(6)
+    tgt = torch.as_tensor(o.init_random(7))
+    e = sol.forward(THETA0, e0, steps)
+    gt, g0, gb, ga = sol.backward_dense(_mse_grad(e, tgt.cuda()))
+    gt_plain, g0_plain = sol.backward(_mse_grad(e, tgt.cuda()))
+    torch.cuda.synchronize()
+    assert torch.equal(gt, gt_plain) and torch.equal(g0, g0_plain)
+    Th = TorchHeun(ref_build, THETA0, alpha=0.3, dt=1e-3)
+    J, A = Th.dense(THETA0)
+    J = J.clone().requires_grad_(True)
+    A = A.clone().requires_grad_(True)
+    ((Th.run_dense(J, A, torch.as_tensor(e0), steps) - tgt) ** 2).mean().backward()
+    N = o.N
+    G = J.grad.reshape(N, 3, N, 3).permute(0, 2, 1, 3).numpy()           # G[i, j] = dL/dJ block (i, j)
+    pairs = _forward_pairs(sol.system)
+    assert sol.nbonds == len(pairs) == 4
+    ref_b = np.stack([G[i, j] + G[j, i].transpose(0, 2, 1) for i, j in pairs])
+    got_b = gb.cpu().numpy()
+    assert _relerr(got_b, ref_b) < 1e-9
+    assert _relerr(ga.cpu().numpy(), A.grad.numpy()) < 1e-9
+    # the DMI vectors' gradient is the antisymmetric part, read with eps_abc
+    dD = dmi_gradient(gb).cpu().numpy()
+    assert np.allclose(dD[..., 2], ref_b[..., 0, 1] - ref_b[..., 1, 0], rtol=0, atol=1e-12 * np.abs(ref_b).max())
+
+
+def test_adjoint_dense_gradients_tiled_and_fp32(gpu):
+    """square 64^2 (tiled cells in the library): the dense gradients equal the row-major layout's to
+    round-off, satisfy the chain rule dL/dtheta_p = sum_b <U_p(b), dL/dT_b> + sum_i <Aun_p(i), dL/dA_i>
+    to 1e-10, and FP32 stays within 2e-3 of FP64 (of the largest element) over 80 steps."""
+    print("case: dense adjoint gradients, square 64^2 tiled vs row-major, FP64 and FP32", flush=True)
+    from magex_amd.adjoint import DifferentiableLLG
+    L, steps = 64, 80
+    dev, _ = _builders(L)
+    s0 = dev(THETA0)
+    e0 = s0.init_spins("random", seed=8)
+    tgt = torch.as_tensor(s0.init_spins("random", seed=9), device="cuda")
+    out = {}
+    for layout, prec in (("rowmajor", "fp64"), ("tiled", "fp64"), ("tiled", "fp32")):
+        sol = DifferentiableLLG(dev, THETA0, alpha=0.3, dt=1e-3, precision=prec, checkpoint_every=30,
+                                layout=layout)
+        if layout == "tiled":
+            assert tuple(sol.dm.layout()) != (0, 0, 0)
+        e = sol.forward(THETA0, e0, steps)
+        gt, _, gb, ga = sol.backward_dense(_mse_grad(e.double(), tgt).to(e.dtype), want_e0=False)
+        torch.cuda.synchronize()
+        out[(layout, prec)] = (gt.cpu().numpy(), gb.cpu().numpy(), ga.cpu().numpy(), sol)
+    gt, gb, ga, sol = out[("tiled", "fp64")]
+    gt_r, gb_r, ga_r, _ = out[("rowmajor", "fp64")]
+    assert _relerr(gb, gb_r) < 1e-12 and _relerr(ga, ga_r) < 1e-12
+    fw = sol.system.forward_entries()
+    U = sol._U.reshape(len(sol._U), -1, 3, 3)[:, fw]                      # (P, nbonds, 3, 3)
+    Aun = sol._Aun.reshape(len(sol._Aun), -1, 3, 3)                       # (P, ns, 3, 3)
+    ns = sol.system.uc.nsites
+    ga_s = ga.reshape(-1, ns, 3, 3).sum(0)
+    chain = np.einsum("pfab,fcab->p", U, gb) + np.einsum("psab,sab->p", Aun, ga_s)
+    assert _relerr(chain[:sol.P], gt) < 1e-10, (chain, gt)
+    gt32, gb32, ga32, _ = out[("tiled", "fp32")]
+    assert _relerr(gb32, gb) < 2e-3 and _relerr(ga32, ga) < 2e-3

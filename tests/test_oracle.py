@@ -270,6 +270,40 @@ def test_torch_heun_forward_matches_oracle():
     assert np.abs(got3 - ref3).max() < 1e-13
 
 
+def test_torch_heun_dense_form_pinned():
+    """run_dense at the dense couplings of theta is run(theta); its gradients obey the chain rule
+    dL/dtheta_p = <Jsp_p, dL/dJ> + <A_p, dL/dA> and match central differences of single elements of J
+    (an exchange bond, both triangles of the matrix) and of A."""
+    import torch
+    from oracle.torch_heun import TorchHeun
+    th = np.array([4.0, -0.4, 1.0, 0.2])
+    T = TorchHeun(_square_builder(5), th, alpha=0.3, dt=1e-3)
+    e0 = torch.as_tensor(T.base.init_random(2))
+    tgt = torch.as_tensor(T.base.init_random(3))
+    J, A = T.dense(th)
+    J = J.clone().requires_grad_(True)
+    A = A.clone().requires_grad_(True)
+    with torch.no_grad():
+        assert (T.run_dense(J, A, e0, 20) - T.run(torch.as_tensor(th), e0, 20)).abs().max() < 1e-13
+    loss = lambda J_, A_: ((T.run_dense(J_, A_, e0, 20) - tgt) ** 2).mean()      # noqa: E731
+    loss(J, A).backward()
+    t = torch.tensor(th, requires_grad=True)
+    ((T.run(t, e0, 20) - tgt) ** 2).mean().backward()
+    chain = [float((T.Jp[p].to_dense() * J.grad).sum() + (T.Ap[p] * A.grad).sum()) for p in range(4)]
+    assert np.allclose(chain, t.grad.numpy(), rtol=1e-10, atol=1e-14)
+    h = 1e-4
+    with torch.no_grad():
+        for (r, c) in [(0, 3), (3, 0), (4, 5)]:                    # sites 0 and 1 are nearest neighbours
+            d = torch.zeros_like(J)
+            d[r, c] = h
+            fd = (loss(J + d, A) - loss(J - d, A)) / (2 * h)
+            assert abs(fd - J.grad[r, c]) <= 1e-7 * abs(fd) + 1e-12, (r, c)
+        d = torch.zeros_like(A)
+        d[7, 0, 2] = h
+        fd = (loss(J, A + d) - loss(J, A - d)) / (2 * h)
+        assert abs(fd - A.grad[7, 0, 2]) <= 1e-7 * abs(fd) + 1e-12
+
+
 def test_torch_heun_gradient_matches_central_differences():
     import torch
     from oracle.torch_heun import TorchHeun
