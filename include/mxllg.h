@@ -143,8 +143,10 @@ int mx_mc_accepted(mx_model* m, int64_t* out, int reset);
 /* Replica sweeps for parallel tempering (magex/advanced_mc.py:38-94): nrep copies of the model's
  * lattice, replica r's spins at e + r * N * 3 (device), each with its own beta (1/meV, inf at T = 0),
  * cone width sigma and generator key seeds[r] (host arrays of nrep), all in one launch per colour and
- * pass.  One-site cells, row-major layout, uniform temperatures only.  The acceptance counts of the
- * replicas accumulate on the device; mx_mc_replica_accepted reads them (synchronising). */
+ * pass.  Uniform temperatures only.  Spins are in magex site order; where the model's layout differs
+ * (multi-site cells, tiled or checker cells) the call sweeps an internal-order copy of the replicas and
+ * writes them back.  The acceptance counts of the replicas accumulate on the device;
+ * mx_mc_replica_accepted reads them (synchronising). */
 int mx_mc_sweep_replicas(mx_model* m, void* e, int32_t nrep, int64_t nsweeps, int move, int overrelax,
                          const double* beta, const double* sigma, const uint64_t* seeds, uint64_t sweep0,
                          void* stream);

@@ -34,7 +34,6 @@ def parallel_tempering(system, temperatures, therm_sweeps=2000, meas_sweeps=4000
     M = len(Ts)
     rng = np.random.default_rng(seed)
     mc = MetropolisMC(system, Ts[0], seed=seed, sigma=sigma, overrelax=overrelax, move=move, precision=precision)
-    mc.layout = None   # replica sweeps take their (M, N, 3) spins as they are: row-major cells
     dm = mc._sync()
     if getattr(system, "site_T", None) is not None:
         raise NotImplementedError("parallel tempering with per-site temperatures")

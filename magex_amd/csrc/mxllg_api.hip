@@ -275,7 +275,7 @@ void mx_model_destroy(mx_model* m) {
   for (void** p : {&m->bond_d, &m->A_d, &m->Kc_d, &m->K2_d, &m->K3_d, &m->inv_mu_d, &m->sign_d, &m->pinned_d,
                    &m->alpha_d, &m->sigma_d, &m->ep_d, &m->k1_d, &m->eint_d, &m->static_d, &m->mc_beta_d,
                    reinterpret_cast<void**>(&m->mc_sites_d), reinterpret_cast<void**>(&m->mc_acc_d), &m->mc_rep_d,
-                   reinterpret_cast<void**>(&m->mc_repacc_d), reinterpret_cast<void**>(&m->img_part_d)})
+                   reinterpret_cast<void**>(&m->mc_repacc_d), reinterpret_cast<void**>(&m->img_part_d), &m->mc_repe_d})
     dfree(*p);
   if (m->perm_d) (void)hipFree(m->perm_d);
   if (m->part_d) (void)hipFree(m->part_d);

@@ -142,6 +142,8 @@ struct Model {
   void* mc_rep_d = nullptr;  // replica sweeps: per-replica parameters and acceptance counters
   unsigned long long* mc_repacc_d = nullptr;
   int mc_rep_cap = 0;
+  void* mc_repe_d = nullptr;  // replica spins in internal site order (layouts that permute), mc_repe_cap of them
+  int mc_repe_cap = 0;
   double* img_part_d = nullptr;  // partials of image-stack observables
   int img_part_cap = 0;          // images they hold
   // captured step loop (hipGraph of graph_steps Heun steps + counter advance), keyed by what it bakes in

@@ -272,10 +272,6 @@ extern "C" int mx_mc_sweep_replicas(mx_model* m, void* e, int32_t nrep, int64_t 
     set_error("replica sweeps take one temperature per replica: the model has per-site temperatures");
     return MX_EUNSUPPORTED;
   }
-  if (needs_permute(*m)) {
-    set_error("replica sweeps need one-site cells in row-major order (their spins are not permuted)");
-    return MX_EUNSUPPORTED;
-  }
   hipError_t err = hipSetDevice(m->device);
   int rc = err == hipSuccess ? sync_model_static(*m) : MX_EHIP;
   if (rc) {
@@ -308,6 +304,20 @@ extern "C" int mx_mc_sweep_replicas(mx_model* m, void* e, int32_t nrep, int64_t 
     if (err == hipSuccess) m->mc_rep_cap = nrep;
   }
   if (err == hipSuccess) err = hipMemcpy(m->mc_rep_d, h.data(), sizeof(McRep) * nrep, hipMemcpyHostToDevice);
+  // multi-site cells and tiled / checker layouts: the replicas are swept in the internal site order
+  const bool perm = needs_permute(*m);
+  if (perm && !m->perm_d) {
+    set_error("replica sweeps: the model's cell layout is not set");
+    return MX_EINVAL;
+  }
+  const size_t rbytes = size_t(m->N) * 3 * (m->prec == MX_FP64 ? sizeof(double) : sizeof(float));
+  if (err == hipSuccess && perm && nrep > m->mc_repe_cap) {
+    if (m->mc_repe_d) (void)hipFree(m->mc_repe_d);
+    m->mc_repe_d = nullptr;
+    m->mc_repe_cap = 0;
+    err = hipMalloc(&m->mc_repe_d, rbytes * nrep);
+    if (err == hipSuccess) m->mc_repe_cap = nrep;
+  }
   if (err != hipSuccess) {
     set_error(std::string("replica sweep set-up: ") + hipGetErrorString(err));
     return err == hipErrorOutOfMemory ? MX_ENOMEM : MX_EHIP;
@@ -315,6 +325,12 @@ extern "C" int mx_mc_sweep_replicas(mx_model* m, void* e, int32_t nrep, int64_t 
   const McRep* reps = static_cast<const McRep*>(m->mc_rep_d);
   const int mv = (move == 2 && !finite) ? 0 : move;
   const int nc = int(m->mc_off.size()) - 1;
+  char* const user = static_cast<char*>(e);
+  if (perm) {
+    for (int r = 0; rc == MX_OK && r < nrep; ++r)
+      rc = launch_permute(*m, user + r * rbytes, static_cast<char*>(m->mc_repe_d) + r * rbytes, 1, s);
+    e = m->mc_repe_d;
+  }
   for (int64_t k = 0; rc == MX_OK && k < nsweeps; ++k)
     for (int pass = 0; rc == MX_OK && pass <= overrelax; ++pass)
       for (int c = 0; rc == MX_OK && c < nc; ++c) {
@@ -325,6 +341,9 @@ extern "C" int mx_mc_sweep_replicas(mx_model* m, void* e, int32_t nrep, int64_t 
                                 : mc_t<float>(*m, e, m->mc_sites_d + a, int(b - a), nullptr, 0.0, mv, pass, 0.0, 0,
                                               sw, m->mc_repacc_d, s, reps, nrep);
       }
+  if (perm)
+    for (int r = 0; rc == MX_OK && r < nrep; ++r)
+      rc = launch_permute(*m, static_cast<char*>(m->mc_repe_d) + r * rbytes, user + r * rbytes, 0, s);
   return rc;
 }
 
