@@ -235,8 +235,9 @@ int mx_batch_observables(mx_batch* b, const void* e, double* out, void* stream);
  * sublattice (shells, DMI and anisotropy constants).  No counterpart in magex, whose solvers are not
  * differentiable (magex/llg.py:87-120 is the forward it differentiates).
  * The adjoint takes over the model's couplings and on-site tensors (the model keeps its stencil,
- * moments, field and dynamics); per-bond tensors, per-site on-site tensors, cubic / higher uniaxial
- * terms and pinned sites are not supported.  Spins and gradients are in magex site order. */
+ * moments, field and dynamics); fixed per-bond and per-site parts come through mx_adjoint_set_offsets.
+ * Cubic / higher uniaxial terms and pinned sites are not supported.  Spins and gradients are in magex
+ * site order. */
 typedef struct mx_adjoint mx_adjoint;
 
 /* unit_bonds: nparam x nent x 9 (row-major 3x3 per stencil entry, the order given to
@@ -247,6 +248,13 @@ int mx_adjoint_create(mx_model* m, int32_t nparam, const double* unit_bonds, con
 void mx_adjoint_destroy(mx_adjoint* a);
 /* theta: nparam values (host); sets the model's couplings and on-site tensors. */
 int mx_adjoint_set_params(mx_adjoint* a, const double* theta);
+/* Fixed parts of a disordered model (host arrays, copied; either may be null for none): bond_offset
+ * (nbonds x ncell x 9, the layout of mx_model_set_bond_tensors) makes every unordered bond's tensor
+ * bond_offset_b + sum_p theta_p unit_bonds[p][f(b)], and onsite_offset (N x 9, magex site order) every
+ * site's on-site tensor onsite_offset_i + sum_p theta_p unit_onsite[p][s(i)].  Applies at once, as
+ * mx_adjoint_set_params does.  No counterpart in magex (the per-bond storage of fast.py:612-614 with
+ * parameters on top). */
+int mx_adjoint_set_offsets(mx_adjoint* a, const double* bond_offset, const double* onsite_offset);
 /* nsteps Heun steps of e (device, N x 3, in place) as mx_llg_run with the same seed and step0,
  * recording checkpoints for mx_adjoint_backward. */
 int mx_adjoint_forward(mx_adjoint* a, void* e, int64_t nsteps, uint64_t seed, uint64_t step0, void* stream);

@@ -71,6 +71,7 @@ SIGNATURES = {
     "mx_adjoint_create": (_I, [_P, ctypes.c_int32, _P, _P, ctypes.c_int32, _P]),
     "mx_adjoint_destroy": (None, [_P]),
     "mx_adjoint_set_params": (_I, [_P, _P]),
+    "mx_adjoint_set_offsets": (_I, [_P, _P, _P]),
     "mx_adjoint_forward": (_I, [_P, _P, ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint64, _P]),
     "mx_adjoint_backward": (_I, [_P, _P, _P, _P, _P]),
     "mx_adjoint_backward_dense": (_I, [_P, _P, _P, _P, _P, _P, _P]),

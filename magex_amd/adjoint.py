@@ -13,7 +13,9 @@ differencing, and are checked to reproduce the builder's couplings.
     loss = ((e - target) ** 2).mean(); loss.backward()
 
 The backward replays each checkpoint interval (default 50 steps) and runs the adjoint kernels on the
-device; see mxllg_adjoint.hip for the equations.  ``backward_dense`` also returns the gradient with
+device; see mxllg_adjoint.hip for the equations.  The builder's systems may carry per-bond tensors and
+per-site on-site tensors (a disordered model) as long as theta moves them uniformly per stencil entry
+and sublattice; the rest is held fixed.  ``backward_dense`` also returns the gradient with
 respect to every bond's 3x3 tensor and every site's on-site tensor at the current couplings
 (``dmi_gradient`` takes the DMI vectors' part of the former).
 """
@@ -31,50 +33,78 @@ MAXP = 16   # parameters the library differentiates in one backward (mxllg_adjoi
 
 
 def linear_units(build, theta_ref, tol=1e-12):
-    """Unit tensors (U (P, nent, 3, 3), Aun (P, ns, 3, 3)) and the system at theta_ref.
+    """Units of a builder linear in theta: (U (P', nent, 3, 3), Aun (P', ns, 3, 3), offset, system, boff, aoff).
 
-    U_p = J(theta_ref + e_p) - J(theta_ref) entry by entry; a theta-independent remainder
-    J(theta_ref) - sum_p theta_ref_p U_p, if any, is returned as an extra fixed unit (offset, or None).
-    Raises ValueError when the builder's stencil changes with theta or its couplings are not linear."""
+    U_p = J(theta_ref + e_p) - J(theta_ref) entry by entry, Aun_p likewise per sublattice.  A
+    theta-independent remainder J(theta_ref) - sum_p theta_ref_p U_p, if any, is appended to U and Aun as
+    a fixed unit (offset True).  Per-bond tensors (SpinSystem.set_bond_tensors) and per-site on-site
+    tensors are allowed when their dependence on theta is uniform per stencil entry / sublattice: what
+    remains is a fixed per-bond part boff (nbonds, ncell, 3, 3) and per-site part aoff (N, 3, 3) (None
+    when the model has none).  Raises ValueError when the builder's stencil changes with theta or its
+    couplings are not linear in theta with such units."""
     theta_ref = np.asarray(theta_ref, float)
     P = len(theta_ref)
     base = build(theta_ref)
     key = [(s, t, tuple(R)) for s, t, R, _ in base.stencil]
     ns = base.uc.nsites
+    fw = base.forward_entries()
 
     def arrays(sys_):
+        """(stencil tensors (nent, 3, 3), per-bond tensors or None, per-site A (ncell, ns, 3, 3))."""
         if [(s, t, tuple(R)) for s, t, R, _ in sys_.stencil] != key:
             raise ValueError("the builder's stencil changes with the parameters (keep every shell nonzero at the "
                              "reference point)")
-        if sys_.bond_tensors is not None:
-            raise NotImplementedError("per-bond tensors are not parameterized")
-        A = np.asarray(sys_.A, float).reshape(sys_.ncell, ns, 3, 3)
-        if not np.all(A == A[:1]):
-            raise NotImplementedError("per-site on-site tensors are not parameterized")
-        return np.array([J for *_, J in sys_.stencil], float).reshape(-1, 3, 3), A[0]
+        Js = np.array([J for *_, J in sys_.stencil], float).reshape(-1, 3, 3)
+        Jb = None if sys_.bond_tensors is None else np.asarray(sys_.bond_tensors, float)
+        return Js, Jb, np.asarray(sys_.A, float).reshape(sys_.ncell, ns, 3, 3)
 
-    J0, A0 = arrays(base)
-    U = np.empty((P,) + J0.shape)
-    Aun = np.empty((P,) + A0.shape)
+    Js0, Jb0, A0 = arrays(base)
+    disordered_J = Jb0 is not None
+    disordered_A = not np.all(A0 == A0[:1])
+    U = np.empty((P,) + Js0.shape)
+    Aun = np.empty((P, ns, 3, 3))
+    scale = max(np.abs(Js0).max(initial=0.0), np.abs(A0).max(initial=0.0),
+                np.abs(Jb0).max(initial=0.0) if disordered_J else 0.0, 1e-300)
     for p in range(P):
         th = theta_ref.copy()
         th[p] += 1.0
-        J1, A1 = arrays(build(th))
-        U[p], Aun[p] = J1 - J0, A1 - A0
-    offJ = J0 - np.tensordot(theta_ref, U, 1)
-    offA = A0 - np.tensordot(theta_ref, Aun, 1)
-    scale = max(np.abs(J0).max(initial=0.0), np.abs(A0).max(initial=0.0), 1e-300)
+        Js1, Jb1, A1 = arrays(build(th))
+        if (Jb1 is not None) != disordered_J:
+            raise ValueError("the builder sets per-bond tensors at some parameters only")
+        U[p] = Js1 - Js0
+        dA = A1 - A0
+        Aun[p] = dA[0]
+        if np.abs(dA - dA[:1]).max(initial=0.0) > tol * scale:
+            raise ValueError("the on-site tensors depend on the parameters per site (only per sublattice is "
+                             "parameterized)")
+        if disordered_J:
+            dJ = Jb1 - Jb0
+            if np.abs(dJ - U[p][fw][:, None]).max(initial=0.0) > tol * scale:
+                raise ValueError("the per-bond tensors must follow the parameters as the stencil does (uniformly "
+                                 "per entry)")
+    offJ = Js0 - np.tensordot(theta_ref, U, 1)
+    offA = A0[0] - np.tensordot(theta_ref, Aun, 1)
     offset = None
     if np.abs(offJ).max(initial=0.0) > tol * scale or np.abs(offA).max(initial=0.0) > tol * scale:
         offset = (offJ, offA)
-    # linearity: a second point must be reproduced by the units
+    Uf = np.concatenate([U, offJ[None]]) if offset else U
+    Af = np.concatenate([Aun, offA[None]]) if offset else Aun
+    thf = np.append(theta_ref, 1.0) if offset else theta_ref
+    boff = Jb0 - np.tensordot(thf, Uf, 1)[fw][:, None] if disordered_J else None
+    aoff = (A0 - np.tensordot(thf, Af, 1)[None]).reshape(-1, 3, 3) if disordered_A else None
+
+    # linearity: a second point must be reproduced by the units and fixed parts
     th = theta_ref * 0.5 + 0.25
-    J2, A2 = arrays(build(th))
-    pJ = np.tensordot(th, U, 1) + (offset[0] if offset else 0.0)
-    pA = np.tensordot(th, Aun, 1) + (offset[1] if offset else 0.0)
-    if np.abs(pJ - J2).max(initial=0.0) > 1e-10 * scale or np.abs(pA - A2).max(initial=0.0) > 1e-10 * scale:
+    thf2 = np.append(th, 1.0) if offset else th
+    Js2, Jb2, A2 = arrays(build(th))
+    pJ = np.tensordot(thf2, Uf, 1)
+    pA = np.tensordot(thf2, Af, 1)[None] + (0.0 if aoff is None else aoff.reshape(A0.shape))
+    bad = np.abs(pJ - Js2).max(initial=0.0) > 1e-10 * scale or np.abs(pA - A2).max(initial=0.0) > 1e-10 * scale
+    if disordered_J:
+        bad = bad or np.abs(boff + pJ[fw][:, None] - Jb2).max(initial=0.0) > 1e-10 * scale
+    if bad:
         raise ValueError("the builder's couplings are not linear in the parameters")
-    return U, Aun, offset, base
+    return Uf, Af, offset is not None, base, boff, aoff
 
 
 class DifferentiableLLG:
@@ -88,16 +118,13 @@ class DifferentiableLLG:
     def __init__(self, build, theta0, alpha, dt, temperature=0.0, seed=0, precision="fp64", checkpoint_every=50,
                  precession=True, layout=None):
         _torch()
-        U, Aun, offset, system = linear_units(build, theta0)
+        U, Aun, offset, system, boff, aoff = linear_units(build, theta0)
         self.P = len(theta0)
-        if offset is not None:
-            U = np.concatenate([U, offset[0][None]])
-            Aun = np.concatenate([Aun, offset[1][None]])
         if len(U) > MAXP:
             raise ValueError(f"at most {MAXP} parameters (including a fixed offset)")
         if system.terms or system.pinned is not None:
             raise NotImplementedError("extra on-site terms and pinned sites are not differentiated")
-        self.offset = offset is not None
+        self.offset = offset
         self.system = system
         self.precision = precision
         self.dtype = torch.float64 if precision == "fp64" else torch.float32
@@ -113,6 +140,11 @@ class DifferentiableLLG:
         check(L.mx_adjoint_create(self.dm.h, len(U), hptr(self._U), hptr(self._Aun), self.checkpoint_every,
                                   ctypes.byref(h)))
         self.h = h
+        if boff is not None or aoff is not None:
+            self._boff = None if boff is None else np.ascontiguousarray(boff.reshape(-1))
+            self._aoff = None if aoff is None else np.ascontiguousarray(aoff.reshape(-1))
+            check(L.mx_adjoint_set_offsets(h, hptr(self._boff) if boff is not None else None,
+                                           hptr(self._aoff) if aoff is not None else None))
         self._theta = None
         self._token = 0          # id of the forward the adjoint record belongs to
 

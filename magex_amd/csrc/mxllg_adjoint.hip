@@ -66,16 +66,38 @@ __device__ __forceinline__ void unproject(R yx, R yy, R yz, R n, R& vx, R& vy, R
   vz = (vz - yz * d) / n;
 }
 
+// A_i^T v for the site's on-site tensor: per sublattice, or per site (symmetric 6 x N or full 9 x N).
 template <typename R>
-__device__ __forceinline__ void onsite_T(const Static<R>* __restrict__ S, int s, R x, R y, R z, R& ox, R& oy, R& oz) {
-  // A^T v for the sublattice's on-site tensor
-  const R* a = S->Asub[s];
-  ox = a[0] * x + a[3] * y + a[6] * z;
-  oy = a[1] * x + a[4] * y + a[7] * z;
-  oz = a[2] * x + a[5] * y + a[8] * z;
+__device__ __forceinline__ void onsite_T(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st, R x, R y,
+                                         R z, R& ox, R& oy, R& oz) {
+  const int amode = S->amode;
+  if (amode == ONSITE_SUB) {
+    const R* a = S->Asub[st.s];
+    ox = a[0] * x + a[3] * y + a[6] * z;
+    oy = a[1] * x + a[4] * y + a[7] * z;
+    oz = a[2] * x + a[5] * y + a[8] * z;
+  } else {
+    const size_t N = size_t(S->N), i = size_t(st.i);
+    const R* A = d.A;
+    if (amode == ONSITE_SITE6) {
+      const R axx = A[i], ayy = A[N + i], azz = A[2 * N + i], axy = A[3 * N + i], axz = A[4 * N + i],
+              ayz = A[5 * N + i];
+      ox = axx * x + axy * y + axz * z;
+      oy = axy * x + ayy * y + ayz * z;
+      oz = axz * x + ayz * y + azz * z;
+    } else {
+      R a[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) a[q] = A[q * N + i];
+      ox = a[0] * x + a[3] * y + a[6] * z;
+      oy = a[1] * x + a[4] * y + a[7] * z;
+      oz = a[2] * x + a[5] * y + a[8] * z;
+    }
+  }
 }
 
-// Adjoint gather at site st: H = sum_k J_k nu_j (J from the model's width-9 table), and for every
+// Adjoint gather at site st: H = sum_k J_k nu_j (J from the model's width-9 table, or with CM ==
+// COUP_BOND its per-bond tensors: a forward entry's own, a reverse entry's partner's transposed), and for every
 // parameter t_p = sum_k U[k][p] x_j; then G_p = nu_i . (t_p - Aun[s][p] x_i) is accumulated into gp.
 // NP (4, 8 or 16) is the parameter count rounded up; padded parameters have zero units.
 // Entries go in batches of CH whose loads are all issued before the first is used (entries past the
@@ -83,9 +105,10 @@ __device__ __forceinline__ void onsite_T(const Static<R>* __restrict__ S, int s,
 // DENSE also adds nu_i x_j^T + x_i nu_j^T to gb[(f * 9 + q) * ncell + cell] for every forward entry f
 // of the site (each unordered bond has one owner thread, so no two threads write one element) and
 // -nu_i x_i^T to ga[q * N + site].
-template <typename R, int GEO, int NP, bool DENSE>
+template <typename R, int GEO, int NP, bool DENSE, int CM>
 __device__ __forceinline__ void adj_gather(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
-                                           const R* __restrict__ nu, const R* __restrict__ x, const Site& st,
+                                           const Dyn<R>& d, const R* __restrict__ nu, const R* __restrict__ x,
+                                           const Site& st,
                                            R nix, R niy, R niz, R xix, R xiy, R xiz, R& hx, R& hy, R& hz,
                                            double (&gp)[NP], double* __restrict__ gb, double* __restrict__ ga) {
   constexpr int CH = 8;
@@ -97,6 +120,7 @@ __device__ __forceinline__ void adj_gather(const Static<R>* __restrict__ S, cons
   const int kb = S->sptr[st.s], ke = S->sptr[st.s + 1];
   for (int k0 = kb; k0 < ke; k0 += CH) {
     R vx[CH], vy[CH], vz[CH], qx[CH], qy[CH], qz[CH];
+    R Jb[CM == COUP_BOND ? CH : 1][9];
     int kk[CH];
 #pragma unroll
     for (int u = 0; u < CH; ++u) {
@@ -113,6 +137,15 @@ __device__ __forceinline__ void adj_gather(const Static<R>* __restrict__ S, cons
       qx[u] = x[3 * j];
       qy[u] = x[3 * j + 1];
       qz[u] = x[3 * j + 2];
+      if constexpr (CM == COUP_BOND) {
+        const bool back = (en.tb >> 16) != 0;
+        const R* T = d.bond + (size_t(en.slot) * ncell + (back ? cn : st.c)) * 9;
+        R tt[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) tt[q] = T[q];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) Jb[u][q] = back ? tt[(q % 3) * 3 + q / 3] : tt[q];
+      }
       const R w = ok ? R(1) : R(0);
       vx[u] *= w;
       vy[u] *= w;
@@ -123,7 +156,9 @@ __device__ __forceinline__ void adj_gather(const Static<R>* __restrict__ S, cons
     }
 #pragma unroll
     for (int u = 0; u < CH; ++u) {
-      const R* T = S->tab + 9 * S->ent[kk[u]].slot;
+      const R* T;
+      if constexpr (CM == COUP_BOND) T = Jb[u];
+      else T = S->tab + 9 * S->ent[kk[u]].slot;
       hx += T[0] * vx[u] + T[1] * vy[u] + T[2] * vz[u];
       hy += T[3] * vx[u] + T[4] * vy[u] + T[5] * vz[u];
       hz += T[6] * vx[u] + T[7] * vy[u] + T[8] * vz[u];
@@ -231,7 +266,7 @@ __device__ __forceinline__ void store_partials(double (&gp)[NP], double old, dou
 }
 
 // Gather nu2 (dL/dep) and the predictor's adjoint: writes nu1 and the partial dL/de (part).
-template <typename R, int GEO, int NP, bool DENSE>
+template <typename R, int GEO, int NP, bool DENSE, int CM>
 __global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
                                                  const Dyn<R> d, const StepRec<R> r, const R* __restrict__ lamu,
                                                  const R* __restrict__ gep, const R* __restrict__ nu2,
@@ -248,8 +283,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S
     const R px = r.ep[3 * i], py = r.ep[3 * i + 1], pz = r.ep[3 * i + 2];
     const R vx2 = nu2[3 * i], vy2 = nu2[3 * i + 1], vz2 = nu2[3 * i + 2];
     R hx, hy, hz, ox, oy, oz;
-    adj_gather<R, GEO, NP, DENSE>(S, A, nu2, r.ep, st, vx2, vy2, vz2, px, py, pz, hx, hy, hz, gp, gb, ga);
-    onsite_T(S, st.s, vx2, vy2, vz2, ox, oy, oz);
+    adj_gather<R, GEO, NP, DENSE, CM>(S, A, d, nu2, r.ep, st, vx2, vy2, vz2, px, py, pz, hx, hy, hz, gp, gb, ga);
+    onsite_T(S, d, st, vx2, vy2, vz2, ox, oy, oz);
     R lx = gep[3 * i] + hx - ox, ly = gep[3 * i + 1] + hy - oy, lz = gep[3 * i + 2] + hz - oz;  // dL/dep
     const R ex = r.e[3 * i], ey = r.e[3 * i + 1], ez = r.e[3 * i + 2];
     const R bx = r.B1[3 * i], by = r.B1[3 * i + 1], bz = r.B1[3 * i + 2];
@@ -277,7 +312,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S
 // Gather nu1: lambda = dL/de of the step's input, and the parameter terms at e.  With FUSE the
 // previous step's local part (adj_local on its record prev) follows at once from lambda, which then
 // never goes through memory; otherwise lambda is stored for a later k_adj_a.
-template <typename R, int GEO, int NP, bool FUSE, bool DENSE>
+template <typename R, int GEO, int NP, bool FUSE, bool DENSE, int CM>
 __global__ __launch_bounds__(BLOCK) void k_adj_c(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
                                                  const Dyn<R> d, const R* __restrict__ e, const R* __restrict__ nu1,
                                                  const R* __restrict__ part, R* __restrict__ lam,
@@ -293,9 +328,9 @@ __global__ __launch_bounds__(BLOCK) void k_adj_c(const Static<R>* __restrict__ S
     const int i = st.i;
     const R vx = nu1[3 * i], vy = nu1[3 * i + 1], vz = nu1[3 * i + 2];
     R hx, hy, hz, ox, oy, oz;
-    adj_gather<R, GEO, NP, DENSE>(S, A, nu1, e, st, vx, vy, vz, e[3 * i], e[3 * i + 1], e[3 * i + 2], hx, hy, hz, gp,
-                                  gb, ga);
-    onsite_T(S, st.s, vx, vy, vz, ox, oy, oz);
+    adj_gather<R, GEO, NP, DENSE, CM>(S, A, d, nu1, e, st, vx, vy, vz, e[3 * i], e[3 * i + 1], e[3 * i + 2], hx, hy,
+                                      hz, gp, gb, ga);
+    onsite_T(S, d, st, vx, vy, vz, ox, oy, oz);
     const R lx = part[3 * i] + hx - ox, ly = part[3 * i + 1] + hy - oy, lz = part[3 * i + 2] + hz - oz;
     if constexpr (FUSE) {
       adj_local(S, d, st, prev, lx, ly, lz, lamu, gep, nu2);
@@ -357,6 +392,7 @@ struct mx_adjoint {
   std::vector<double> theta;
   std::vector<int> fslot;  // forward-bond slot per library entry (-1: reverse entry)
   int nfwd = 0;
+  std::vector<double> boff, aoff;  // fixed parts: per bond nfwd x ncell x 9, per site N x 9 (empty: none)
   void* astatic_d = nullptr;
   // forward record
   bool recorded = false;
@@ -428,27 +464,54 @@ void upload_astatic(mx_adjoint& a) {
 }
 
 // The model's couplings become sum_p theta_p U[p] (one width-9 table row per entry) and its on-site
-// tensors sum_p theta_p Aun[p].
+// tensors sum_p theta_p Aun[p].  With fixed parts (boff, aoff) they are per bond, boff_b + sum_p theta_p
+// U[p][f(b)], and per site, aoff_i + sum_p theta_p Aun[p][s(i)], uploaded as the model's per-bond and
+// per-site tensors.
 void apply_theta(mx_adjoint& a) {
   Model& m = *a.m;
-  m.table.assign(size_t(m.nent) * 9, 0.0);
-  for (int k = 0; k < m.nent; ++k) {
-    m.ent[k].slot = k;
-    for (int p = 0; p < a.P; ++p)
-      for (int q = 0; q < 9; ++q) m.table[9 * size_t(k) + q] += a.theta[p] * a.U[(size_t(p) * m.nent + k) * 9 + q];
+  if (a.boff.empty()) {
+    m.table.assign(size_t(m.nent) * 9, 0.0);
+    for (int k = 0; k < m.nent; ++k) {
+      m.ent[k].slot = k;
+      for (int p = 0; p < a.P; ++p)
+        for (int q = 0; q < 9; ++q) m.table[9 * size_t(k) + q] += a.theta[p] * a.U[(size_t(p) * m.nent + k) * 9 + q];
+    }
+    m.ntypes = m.nent;
+    m.width = 9;
+    m.coup = COUP_TABLE9;
+  } else {
+    std::vector<int> pos_of_f(a.nfwd, -1);
+    for (int k = 0; k < m.nent; ++k)
+      if (a.fslot[k] >= 0) pos_of_f[a.fslot[k]] = k;
+    std::vector<double> J(a.boff);
+    for (int f = 0; f < a.nfwd; ++f) {
+      double u[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int p = 0; p < a.P; ++p)
+        for (int q = 0; q < 9; ++q) u[q] += a.theta[p] * a.U[(size_t(p) * m.nent + pos_of_f[f]) * 9 + q];
+      for (int c = 0; c < m.ncell; ++c)
+        for (int q = 0; q < 9; ++q) J[(size_t(f) * m.ncell + c) * 9 + q] += u[q];
+    }
+    aok_rc(mx_model_set_bond_tensors(a.m, J.data()), "per-bond tensors");
   }
-  m.ntypes = m.nent;
-  m.width = 9;
-  m.coup = COUP_TABLE9;
-  m.amode = ONSITE_SUB;
-  m.has_A = 0;
-  std::fill(m.Asub.begin(), m.Asub.end(), 0.0);
-  for (int s = 0; s < m.ns; ++s)
-    for (int p = 0; p < a.P; ++p)
-      for (int q = 0; q < 9; ++q) {
-        m.Asub[9 * s + q] += a.theta[p] * a.Aun[(size_t(p) * m.ns + s) * 9 + q];
-        if (m.Asub[9 * s + q] != 0.0) m.has_A = 1;
-      }
+  if (a.aoff.empty()) {
+    m.amode = ONSITE_SUB;
+    m.has_A = 0;
+    std::fill(m.Asub.begin(), m.Asub.end(), 0.0);
+    for (int s = 0; s < m.ns; ++s)
+      for (int p = 0; p < a.P; ++p)
+        for (int q = 0; q < 9; ++q) {
+          m.Asub[9 * s + q] += a.theta[p] * a.Aun[(size_t(p) * m.ns + s) * 9 + q];
+          if (m.Asub[9 * s + q] != 0.0) m.has_A = 1;
+        }
+  } else {
+    std::vector<double> A(a.aoff);
+    for (int r = 0; r < m.N; ++r) {
+      const int s = r % m.ns;
+      for (int p = 0; p < a.P; ++p)
+        for (int q = 0; q < 9; ++q) A[size_t(r) * 9 + q] += a.theta[p] * a.Aun[(size_t(p) * m.ns + s) * 9 + q];
+    }
+    aok_rc(mx_model_set_onsite(a.m, 1, A.data()), "per-site on-site tensors");
+  }
   m.static_dirty = 1;
 }
 
@@ -523,7 +586,7 @@ void ensure_buffers(mx_adjoint& a) {
 // Adjoint sweep over one replayed segment of n_seg steps (records rec(n), newest last): the local
 // part of the segment's last step from lam, then per step (newest first) k_adj_b and k_adj_c, the
 // latter fused with the next-older step's local part.  Leaves dL/de of the segment's first input in lam.
-template <typename R, int GEO, int NP, bool DENSE>
+template <typename R, int GEO, int NP, bool DENSE, int CM>
 void sweep_t(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
   const Model& m = *a.m;
   const Static<R>* S = static_cast<const Static<R>*>(m.static_d);
@@ -533,6 +596,8 @@ void sweep_t(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
   d.prec = m.precession;
   d.alpha = static_cast<const R*>(m.alpha_d);
   d.inv_mu = static_cast<const R*>(m.inv_mu_d);
+  d.bond = static_cast<const R*>(m.bond_d);
+  d.A = static_cast<const R*>(m.A_d);
   R* lam = static_cast<R*>(a.lam_d);
   R* lamu = static_cast<R*>(a.lamu_d);
   R* gep = static_cast<R*>(a.gep_d);
@@ -542,30 +607,36 @@ void sweep_t(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
   const dim3 g(a.nblk), b(BLOCK);
   hipLaunchKernelGGL((k_adj_a<R, GEO>), g, b, 0, s, S, d, lam, rec[n_seg - 1], lamu, gep, nu2);
   for (int n = n_seg - 1; n >= 0; --n) {
-    hipLaunchKernelGGL((k_adj_b<R, GEO, NP, DENSE>), g, b, 0, s, S, A, d, rec[n], lamu, gep, nu2, nu1, part, a.gpart_d,
+    hipLaunchKernelGGL((k_adj_b<R, GEO, NP, DENSE, CM>), g, b, 0, s, S, A, d, rec[n], lamu, gep, nu2, nu1, part, a.gpart_d,
                        a.gb_d, a.ga_d);
     if (n > 0)
-      hipLaunchKernelGGL((k_adj_c<R, GEO, NP, true, DENSE>), g, b, 0, s, S, A, d, rec[n].e, nu1, part, lam, rec[n - 1],
+      hipLaunchKernelGGL((k_adj_c<R, GEO, NP, true, DENSE, CM>), g, b, 0, s, S, A, d, rec[n].e, nu1, part, lam, rec[n - 1],
                          lamu, gep, nu2, a.gpart_d, a.gb_d, a.ga_d);
     else
-      hipLaunchKernelGGL((k_adj_c<R, GEO, NP, false, DENSE>), g, b, 0, s, S, A, d, rec[n].e, nu1, part, lam, rec[n],
+      hipLaunchKernelGGL((k_adj_c<R, GEO, NP, false, DENSE, CM>), g, b, 0, s, S, A, d, rec[n].e, nu1, part, lam, rec[n],
                          lamu, gep, nu2, a.gpart_d, a.gb_d, a.ga_d);
   }
 }
 
-template <typename R, int NP, bool DENSE>
+template <typename R, int NP, bool DENSE, int CM>
 void sweep_geo(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
   const int geo = a.m->geo;
-  if (geo == GEO_OPEN) sweep_t<R, GEO_OPEN, NP, DENSE>(a, rec, n_seg, s);
-  else if (geo == GEO_PERIODIC) sweep_t<R, GEO_PERIODIC, NP, DENSE>(a, rec, n_seg, s);
-  else sweep_t<R, GEO_TILED, NP, DENSE>(a, rec, n_seg, s);
+  if (geo == GEO_OPEN) sweep_t<R, GEO_OPEN, NP, DENSE, CM>(a, rec, n_seg, s);
+  else if (geo == GEO_PERIODIC) sweep_t<R, GEO_PERIODIC, NP, DENSE, CM>(a, rec, n_seg, s);
+  else sweep_t<R, GEO_TILED, NP, DENSE, CM>(a, rec, n_seg, s);
+}
+
+template <typename R, int NP, bool DENSE>
+void sweep_cm(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
+  if (a.m->coup == COUP_BOND) sweep_geo<R, NP, DENSE, COUP_BOND>(a, rec, n_seg, s);
+  else sweep_geo<R, NP, DENSE, COUP_TABLE9>(a, rec, n_seg, s);
 }
 
 template <typename R, bool DENSE>
 void sweep_np(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
-  if (a.P <= 4) sweep_geo<R, 4, DENSE>(a, rec, n_seg, s);
-  else if (a.P <= 8) sweep_geo<R, 8, DENSE>(a, rec, n_seg, s);
-  else sweep_geo<R, 16, DENSE>(a, rec, n_seg, s);
+  if (a.P <= 4) sweep_cm<R, 4, DENSE>(a, rec, n_seg, s);
+  else if (a.P <= 8) sweep_cm<R, 8, DENSE>(a, rec, n_seg, s);
+  else sweep_cm<R, 16, DENSE>(a, rec, n_seg, s);
 }
 
 template <typename R>
@@ -654,8 +725,11 @@ int mx_adjoint_create(mx_model* m, int32_t nparam, const double* unit_bonds, con
     if (nparam < 1 || nparam > MAXP) afail(MX_EUNSUPPORTED, "1 to 16 parameters");
     if (checkpoint_every < 1) afail(MX_EINVAL, "checkpoint interval must be positive");
     if (m->nent < 1) afail(MX_EINVAL, "set the stencil first");
-    if (m->pinned_d || m->has_c || m->has_u || m->amode != ONSITE_SUB || m->coup == COUP_BOND)
-      afail(MX_EUNSUPPORTED, "the adjoint covers stencil couplings and per-sublattice on-site tensors");
+    // the couplings and on-site tensors are replaced by the parameterized ones (and any fixed parts
+    // given by mx_adjoint_set_offsets); the other on-site terms and pinning are not differentiated
+    if (m->pinned_d || m->has_c || m->has_u)
+      afail(MX_EUNSUPPORTED, "the adjoint covers couplings and on-site tensors (no pinned sites, cubic or higher "
+                             "uniaxial terms)");
     if (m->nent * 9 > MAXTAB) afail(MX_EUNSUPPORTED, "stencil too large for a per-entry coupling table");
     if (m->geo == GEO_CHECKER) afail(MX_EUNSUPPORTED, "the adjoint runs on the row-major or tiled cell layouts");
     // unit tensors arrive in the caller's stencil order; reorder into the library's, and check that
@@ -795,5 +869,21 @@ int mx_adjoint_backward_dense(mx_adjoint* a, const void* grad_e, void* grad_e0, 
 }
 
 int mx_adjoint_num_bonds(const mx_adjoint* a) { return a ? a->nfwd : -1; }
+
+int mx_adjoint_set_offsets(mx_adjoint* a, const double* bond_offset, const double* onsite_offset) {
+  return aguard([&]() -> int {
+    if (!a) afail(MX_EINVAL, "null argument");
+    Model& m = *a->m;
+    if (bond_offset && a->nfwd == 0) afail(MX_EINVAL, "per-bond offsets need forward stencil entries");
+    aok(hipSetDevice(m.device), "hipSetDevice");
+    a->boff.clear();
+    a->aoff.clear();
+    if (bond_offset) a->boff.assign(bond_offset, bond_offset + size_t(a->nfwd) * m.ncell * 9);
+    if (onsite_offset) a->aoff.assign(onsite_offset, onsite_offset + size_t(m.N) * 9);
+    apply_theta(*a);
+    a->recorded = false;
+    return MX_OK;
+  });
+}
 
 }  // extern "C"
