@@ -169,17 +169,37 @@ __device__ __forceinline__ void adj_gather(const Static<R>* __restrict__ S, cons
         t[p][1] += U[3] * qx[u] + U[4] * qy[u] + U[5] * qz[u];
         t[p][2] += U[6] * qx[u] + U[7] * qy[u] + U[8] * qz[u];
       }
-      if constexpr (DENSE) {
-        const int f = A->fslot[kk[u]];
-        if (k0 + u < ke && f >= 0) {
-          const double ni[3] = {double(nix), double(niy), double(niz)}, xi[3] = {double(xix), double(xiy), double(xiz)};
-          const double nj[3] = {double(vx[u]), double(vy[u]), double(vz[u])};
-          const double xj[3] = {double(qx[u]), double(qy[u]), double(qz[u])};
-          double* g = gb + size_t(f) * 9 * ncell + st.c;
+    }
+    if constexpr (DENSE) {
+      // accumulator loads of DG entries are issued before any of their stores: loads and stores share
+      // one completion counter, so interleaving them would make every read-modify-write a round trip
+      constexpr int DG = 4;
+      static_assert(CH % DG == 0, "dense groups must tile the batch");
+      const double ni[3] = {double(nix), double(niy), double(niz)}, xi[3] = {double(xix), double(xiy), double(xiz)};
 #pragma unroll
-          for (int r = 0; r < 3; ++r)
+      for (int u0 = 0; u0 < CH; u0 += DG) {
+        double acc[DG][9];
+        double* gq[DG];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) g[size_t(3 * r + c) * ncell] += ni[r] * xj[c] + xi[r] * nj[c];
+        for (int v = 0; v < DG; ++v) {
+          const int u = u0 + v, f = A->fslot[kk[u]];
+          gq[v] = (k0 + u < ke && f >= 0) ? gb + size_t(f) * 9 * ncell + st.c : nullptr;
+          if (gq[v]) {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) acc[v][q] = gq[v][size_t(q) * ncell];
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < DG; ++v) {
+          if (gq[v]) {
+            const int u = u0 + v;
+            const double nj[3] = {double(vx[u]), double(vy[u]), double(vz[u])};
+            const double xj[3] = {double(qx[u]), double(qy[u]), double(qz[u])};
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+              for (int c = 0; c < 3; ++c) gq[v][size_t(3 * r + c) * ncell] = acc[v][3 * r + c] + (ni[r] * xj[c] + xi[r] * nj[c]);
+          }
         }
       }
     }
