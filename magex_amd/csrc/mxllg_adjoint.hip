@@ -486,10 +486,11 @@ void upload_astatic(mx_adjoint& a) {
 // The model's couplings become sum_p theta_p U[p] (one width-9 table row per entry) and its on-site
 // tensors sum_p theta_p Aun[p].  With fixed parts (boff, aoff) they are per bond, boff_b + sum_p theta_p
 // U[p][f(b)], and per site, aoff_i + sum_p theta_p Aun[p][s(i)], uploaded as the model's per-bond and
-// per-site tensors.
+// per-site tensors.  A stencil too large for the constant table (nent x 9 > MAXTAB values) also takes
+// the per-bond form, with a zero fixed part.
 void apply_theta(mx_adjoint& a) {
   Model& m = *a.m;
-  if (a.boff.empty()) {
+  if (a.boff.empty() && m.nent * 9 <= MAXTAB) {
     m.table.assign(size_t(m.nent) * 9, 0.0);
     for (int k = 0; k < m.nent; ++k) {
       m.ent[k].slot = k;
@@ -504,6 +505,7 @@ void apply_theta(mx_adjoint& a) {
     for (int k = 0; k < m.nent; ++k)
       if (a.fslot[k] >= 0) pos_of_f[a.fslot[k]] = k;
     std::vector<double> J(a.boff);
+    if (J.empty()) J.assign(size_t(a.nfwd) * m.ncell * 9, 0.0);
     for (int f = 0; f < a.nfwd; ++f) {
       double u[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
       for (int p = 0; p < a.P; ++p)
@@ -750,7 +752,6 @@ int mx_adjoint_create(mx_model* m, int32_t nparam, const double* unit_bonds, con
     if (m->pinned_d || m->has_c || m->has_u)
       afail(MX_EUNSUPPORTED, "the adjoint covers couplings and on-site tensors (no pinned sites, cubic or higher "
                              "uniaxial terms)");
-    if (m->nent * 9 > MAXTAB) afail(MX_EUNSUPPORTED, "stencil too large for a per-entry coupling table");
     if (m->geo == GEO_CHECKER) afail(MX_EUNSUPPORTED, "the adjoint runs on the row-major or tiled cell layouts");
     // unit tensors arrive in the caller's stencil order; reorder into the library's, and check that
     // every entry's partner carries the transposed tensors (the adjoint gather relies on J_ji = J_ij^T)
