@@ -230,12 +230,21 @@ struct StepRec {
 };
 
 // Step-local part (corrector) at site i: from lambda = dL/de' and the step's record: lambda_u, the
-// cotangent of ep through k2 (gep) and nu2.
+// cotangent of ep through k2 (gep) and nu2.  A pinned spin is carried unchanged (e' = ep = e), so
+// lambda passes through and no field acts on it (gep = nu2 = 0).
 template <typename R>
 __device__ __forceinline__ void adj_local(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st,
                                           const StepRec<R>& r, R lx, R ly, R lz, R* __restrict__ lamu,
                                           R* __restrict__ gep, R* __restrict__ nu2) {
   const int i = st.i;
+  if (d.pinned && d.pinned[i]) {
+    lamu[3 * i] = lx;
+    lamu[3 * i + 1] = ly;
+    lamu[3 * i + 2] = lz;
+    gep[3 * i] = gep[3 * i + 1] = gep[3 * i + 2] = R(0);
+    nu2[3 * i] = nu2[3 * i + 1] = nu2[3 * i + 2] = R(0);
+    return;
+  }
   const R a = d.alpha ? d.alpha[i] : S->alpha_sub[st.s];
   const R px = r.ep[3 * i], py = r.ep[3 * i + 1], pz = r.ep[3 * i + 2];
   const R bx = r.B2[3 * i], by = r.B2[3 * i + 1], bz = r.B2[3 * i + 2];
@@ -306,25 +315,34 @@ __global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S
     adj_gather<R, GEO, NP, DENSE, CM>(S, A, d, nu2, r.ep, st, vx2, vy2, vz2, px, py, pz, hx, hy, hz, gp, gb, ga);
     onsite_T(S, d, st, vx2, vy2, vz2, ox, oy, oz);
     R lx = gep[3 * i] + hx - ox, ly = gep[3 * i + 1] + hy - oy, lz = gep[3 * i + 2] + hz - oz;  // dL/dep
-    const R ex = r.e[3 * i], ey = r.e[3 * i + 1], ez = r.e[3 * i + 2];
-    const R bx = r.B1[3 * i], by = r.B1[3 * i + 1], bz = r.B1[3 * i + 2];
-    R fx, fy, fz;
-    llg_rhs(ex, ey, ez, bx, by, bz, a, d.prec, fx, fy, fz);
-    const R vx = ex + d.dt * fx, vy = ey + d.dt * fy, vz = ez + d.dt * fz;
-    const R n = sqrt(vx * vx + vy * vy + vz * vz);
-    unproject(vx / n, vy / n, vz / n, n, lx, ly, lz);  // dL/dv, v = e + dt k1
-    const R ux = lamu[3 * i], uy = lamu[3 * i + 1], uz = lamu[3 * i + 2];
-    const R hdt = R(0.5) * d.dt;
-    const R kx = d.dt * lx + hdt * ux, ky = d.dt * ly + hdt * uy, kz = d.dt * lz + hdt * uz;  // dL/dk1
-    R gx, gy, gz, qx, qy, qz;
-    rhs_vjp(ex, ey, ez, bx, by, bz, a, d.prec, kx, ky, kz, gx, gy, gz, qx, qy, qz);
-    const R im2 = R(2) * (d.inv_mu ? d.inv_mu[i] : S->inv_mu_sub[st.s]);
-    nu1[3 * i] = im2 * qx;
-    nu1[3 * i + 1] = im2 * qy;
-    nu1[3 * i + 2] = im2 * qz;
-    part[3 * i] = ux + lx + gx;
-    part[3 * i + 1] = uy + ly + gy;
-    part[3 * i + 2] = uz + lz + gz;
+    if (d.pinned && d.pinned[i]) {
+      // ep = e' = e: dL/de gathers lambda (in lamu) and dL/dep; the spin feels no field (nu1 = 0).
+      // No early return: every thread reaches the workgroup reduction below.
+      nu1[3 * i] = nu1[3 * i + 1] = nu1[3 * i + 2] = R(0);
+      part[3 * i] = lamu[3 * i] + lx;
+      part[3 * i + 1] = lamu[3 * i + 1] + ly;
+      part[3 * i + 2] = lamu[3 * i + 2] + lz;
+    } else {
+      const R ex = r.e[3 * i], ey = r.e[3 * i + 1], ez = r.e[3 * i + 2];
+      const R bx = r.B1[3 * i], by = r.B1[3 * i + 1], bz = r.B1[3 * i + 2];
+      R fx, fy, fz;
+      llg_rhs(ex, ey, ez, bx, by, bz, a, d.prec, fx, fy, fz);
+      const R vx = ex + d.dt * fx, vy = ey + d.dt * fy, vz = ez + d.dt * fz;
+      const R n = sqrt(vx * vx + vy * vy + vz * vz);
+      unproject(vx / n, vy / n, vz / n, n, lx, ly, lz);  // dL/dv, v = e + dt k1
+      const R ux = lamu[3 * i], uy = lamu[3 * i + 1], uz = lamu[3 * i + 2];
+      const R hdt = R(0.5) * d.dt;
+      const R kx = d.dt * lx + hdt * ux, ky = d.dt * ly + hdt * uy, kz = d.dt * lz + hdt * uz;  // dL/dk1
+      R gx, gy, gz, qx, qy, qz;
+      rhs_vjp(ex, ey, ez, bx, by, bz, a, d.prec, kx, ky, kz, gx, gy, gz, qx, qy, qz);
+      const R im2 = R(2) * (d.inv_mu ? d.inv_mu[i] : S->inv_mu_sub[st.s]);
+      nu1[3 * i] = im2 * qx;
+      nu1[3 * i + 1] = im2 * qy;
+      nu1[3 * i + 2] = im2 * qz;
+      part[3 * i] = ux + lx + gx;
+      part[3 * i + 1] = uy + ly + gy;
+      part[3 * i + 2] = uz + lz + gz;
+    }
   }
   store_partials<NP>(gp, old, gpart);
 }
@@ -620,6 +638,7 @@ void sweep_t(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
   d.inv_mu = static_cast<const R*>(m.inv_mu_d);
   d.bond = static_cast<const R*>(m.bond_d);
   d.A = static_cast<const R*>(m.A_d);
+  d.pinned = static_cast<const uint8_t*>(m.pinned_d);
   R* lam = static_cast<R*>(a.lam_d);
   R* lamu = static_cast<R*>(a.lamu_d);
   R* gep = static_cast<R*>(a.gep_d);
@@ -748,9 +767,9 @@ int mx_adjoint_create(mx_model* m, int32_t nparam, const double* unit_bonds, con
     if (checkpoint_every < 1) afail(MX_EINVAL, "checkpoint interval must be positive");
     if (m->nent < 1) afail(MX_EINVAL, "set the stencil first");
     // the couplings and on-site tensors are replaced by the parameterized ones (and any fixed parts
-    // given by mx_adjoint_set_offsets); the other on-site terms and pinning are not differentiated
-    if (m->pinned_d || m->has_c || m->has_u)
-      afail(MX_EUNSUPPORTED, "the adjoint covers couplings and on-site tensors (no pinned sites, cubic or higher "
+    // given by mx_adjoint_set_offsets); the cubic and higher uniaxial terms are not differentiated
+    if (m->has_c || m->has_u)
+      afail(MX_EUNSUPPORTED, "the adjoint covers couplings, on-site tensors and pinning (no cubic or higher "
                              "uniaxial terms)");
     if (m->geo == GEO_CHECKER) afail(MX_EUNSUPPORTED, "the adjoint runs on the row-major or tiled cell layouts");
     // unit tensors arrive in the caller's stencil order; reorder into the library's, and check that

@@ -57,6 +57,12 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S,
       k1[3 * i] = x;
       k1[3 * i + 1] = y;
       k1[3 * i + 2] = z;
+      if (!LEAN && d.save_e) {  // a recorded step (adjoint): the spin, and no field acting on it
+        d.save_e[3 * i] = x;
+        d.save_e[3 * i + 1] = y;
+        d.save_e[3 * i + 2] = z;
+        d.save_B[3 * i] = d.save_B[3 * i + 1] = d.save_B[3 * i + 2] = R(0);
+      }
       return;
     }
     R bx, by, bz, fx, fy, fz;
@@ -92,7 +98,10 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S,
     ep[3 * i + 1] = py * q;
     ep[3 * i + 2] = pz * q;
   } else {
-    if (pin) return;
+    if (pin) {
+      if (!LEAN && d.save_B) d.save_B[3 * i] = d.save_B[3 * i + 1] = d.save_B[3 * i + 2] = R(0);
+      return;
+    }
     const R x = ep[3 * i], y = ep[3 * i + 1], z = ep[3 * i + 2];
     R bx, by, bz, fx, fy, fz;
     local_field<R, CM, GEO, LEAN>(S, d, ep, st, x, y, z, step, noise, bx, by, bz);

@@ -49,6 +49,9 @@ class TorchHeun:
         self.B = torch.as_tensor(base.B)
         self.alpha, self.dt, self.precession = float(alpha), float(dt), precession
         self.base = base
+        # pinned spins are carried unchanged (a copy, as the device kernels do; magex renormalises them,
+        # which changes nothing but round-off in the value and drops the radial part of the derivative)
+        self.pin = None if base.pinned is None else torch.as_tensor(np.asarray(base.pinned, bool))[:, None]
 
     def effective_field(self, th, e):
         """B_eff (spinmodel.py:458-473) with theta th (torch, P) and e (N, 3)."""
@@ -72,15 +75,18 @@ class TorchHeun:
     def normalize(u):
         return u / torch.linalg.norm(u, dim=1, keepdim=True)
 
+    def _keep_pinned(self, new, old):
+        return new if self.pin is None else torch.where(self.pin, old, new)
+
     def run(self, th, e, steps, thermal=None):
         """steps Heun steps; thermal(n) -> (N, 3) thermal field of step n (torch FP64) or None."""
         dt = self.dt
         for n in range(steps):
             b = 0.0 if thermal is None else thermal(n)
             k1 = self.rhs(e, self.effective_field(th, e) + b)
-            ep = self.normalize(e + dt * k1)
+            ep = self._keep_pinned(self.normalize(e + dt * k1), e)
             k2 = self.rhs(ep, self.effective_field(th, ep) + b)
-            e = self.normalize(e + 0.5 * dt * (k1 + k2))
+            e = self._keep_pinned(self.normalize(e + 0.5 * dt * (k1 + k2)), e)
         return e
 
     def dense(self, th):
@@ -102,7 +108,7 @@ class TorchHeun:
         for n in range(steps):
             b = 0.0 if thermal is None else thermal(n)
             k1 = self.rhs(e, self.effective_field_dense(J, A, e) + b)
-            ep = self.normalize(e + dt * k1)
+            ep = self._keep_pinned(self.normalize(e + dt * k1), e)
             k2 = self.rhs(ep, self.effective_field_dense(J, A, ep) + b)
-            e = self.normalize(e + 0.5 * dt * (k1 + k2))
+            e = self._keep_pinned(self.normalize(e + 0.5 * dt * (k1 + k2)), e)
         return e

@@ -467,3 +467,56 @@ def test_adjoint_stencil_beyond_the_table(gpu):
     o = ref(th0)
     _check_against_autograd(sol, ref, th0, o.init_random(12), torch.as_tensor(o.init_random(13)), steps,
                             _forward_pairs(sol.system))
+
+
+def test_adjoint_pinned_sites_match_autograd(gpu):
+    """Pinned sites (a stripe and scattered sites, as structure.apply_structure makes them): they keep
+    their spins, feel no field and still act on their neighbours.  FP64, square 12^2, 40 steps at 50 K
+    over three checkpoint segments: parameter, spin and dense gradients match FP64 autograd of the
+    restatement that holds pinned spins (to 1e-9)."""
+    print("case: adjoint with pinned sites, square 12^2, 40 steps, T=50 K", flush=True)
+    from magex_amd.adjoint import DifferentiableLLG
+    L, steps, T, seed = 12, 40, 50.0, 4
+    dev0, ref0 = _builders(L)
+    mask = np.zeros(L * L, bool)
+    mask[:L] = True                                   # the first row of cells
+    mask[[30, 57, 100, 143]] = True
+
+    def dev(th):
+        s = dev0(th)
+        s.pinned = mask
+        return s
+
+    def ref(th):
+        o = ref0(th)
+        o.pinned = mask
+        return o
+
+    sol = DifferentiableLLG(dev, THETA0, alpha=0.3, dt=1e-3, temperature=T, seed=seed, precision="fp64",
+                            checkpoint_every=16)
+    o = ref(THETA0)
+    e0 = o.init_random(14)
+    tgt = torch.as_tensor(o.init_random(15))
+    e = sol.forward(THETA0, e0, steps)
+    gt, g0, gb, ga = sol.backward_dense(_mse_grad(e, tgt.cuda()))
+    torch.cuda.synchronize()
+    assert np.array_equal(e.cpu().numpy()[mask], e0[mask])
+    sg = R.sigma(o, 0.3, T, 1e-3)
+    fields = [torch.as_tensor(sg[:, None] * sol.dm.noise(seed, n).double().cpu().numpy()) for n in range(steps)]
+    Th = TorchHeun(ref, THETA0, alpha=0.3, dt=1e-3)
+    t = torch.tensor(THETA0, requires_grad=True)
+    x0 = torch.tensor(e0, requires_grad=True)
+    eT = Th.run(t, x0, steps, thermal=lambda n: fields[n])
+    ((eT - tgt) ** 2).mean().backward()
+    assert np.abs(e.cpu().numpy() - eT.detach().numpy()).max() < 1e-12
+    assert _relerr(gt.cpu().numpy(), t.grad.numpy()) < 1e-9, (gt, t.grad)
+    assert _relerr(g0.cpu().numpy(), x0.grad.numpy()) < 1e-9
+    J, A = Th.dense(THETA0)
+    J = J.clone().requires_grad_(True)
+    A = A.clone().requires_grad_(True)
+    ((Th.run_dense(J, A, torch.as_tensor(e0), steps, thermal=lambda n: fields[n]) - tgt) ** 2).mean().backward()
+    N = L * L
+    G = J.grad.reshape(N, 3, N, 3).permute(0, 2, 1, 3).numpy()
+    ref_b = np.stack([G[i, j] + G[j, i].transpose(0, 2, 1) for i, j in _forward_pairs(sol.system)])
+    assert _relerr(gb.cpu().numpy(), ref_b) < 1e-9
+    assert _relerr(ga.cpu().numpy(), A.grad.numpy()) < 1e-9

@@ -304,6 +304,33 @@ def test_torch_heun_dense_form_pinned():
         assert abs(fd - A.grad[7, 0, 2]) <= 1e-7 * abs(fd) + 1e-12
 
 
+def test_torch_heun_pinned_matches_oracle():
+    """With pinned sites the restatement's trajectory is magex_ref.run_heun's (pinned spins held; magex
+    renormalises them, which differs by round-off only), and a planted mistake, pinned spins left free,
+    is far off."""
+    import torch
+    from oracle.torch_heun import TorchHeun
+    th = np.array([4.0, -0.4, 1.0, 0.2])
+    mask = np.zeros(36, bool)
+    mask[[0, 7, 14, 20, 33]] = True
+
+    def build(t):
+        o = _square_builder(6)(t)
+        o.pinned = mask
+        return o
+
+    T = TorchHeun(build, th, alpha=0.3, dt=1e-3)
+    o = build(th)
+    e0 = o.init_random(5)
+    ref = R.run_heun(o, e0.copy(), 25, 0.3, 1e-3)
+    got = T.run(torch.as_tensor(th), torch.as_tensor(e0), 25).numpy()
+    assert np.abs(got - ref).max() < 1e-13
+    assert np.abs(got[mask] - e0[mask]).max() < 1e-15
+    T.pin = None
+    free = T.run(torch.as_tensor(th), torch.as_tensor(e0), 25).numpy()
+    assert np.abs(free - ref).max() > 1e-4
+
+
 def test_torch_heun_gradient_matches_central_differences():
     import torch
     from oracle.torch_heun import TorchHeun
