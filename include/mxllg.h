@@ -237,8 +237,8 @@ int mx_batch_observables(mx_batch* b, const void* e, double* out, void* stream);
  * The adjoint takes over the model's couplings and on-site tensors (the model keeps its stencil,
  * moments, field, pinned sites and dynamics); fixed per-bond and per-site parts come through
  * mx_adjoint_set_offsets.  Pinned spins are carried unchanged, so their gradient is lambda plus the
- * field terms of their neighbours.  Cubic / higher uniaxial terms are not supported.  Spins and
- * gradients are in magex site order. */
+ * field terms of their neighbours.  Cubic / higher uniaxial terms are held fixed and enter through
+ * their Hessians.  Spins and gradients are in magex site order. */
 typedef struct mx_adjoint mx_adjoint;
 
 /* unit_bonds: nparam x nent x 9 (row-major 3x3 per stencil entry, the order given to

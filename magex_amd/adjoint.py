@@ -122,8 +122,6 @@ class DifferentiableLLG:
         self.P = len(theta0)
         if len(U) > MAXP:
             raise ValueError(f"at most {MAXP} parameters (including a fixed offset)")
-        if system.terms:
-            raise NotImplementedError("extra on-site terms (cubic, higher uniaxial) are not differentiated")
         self.offset = offset
         self.system = system
         self.precision = precision

@@ -66,10 +66,13 @@ __device__ __forceinline__ void unproject(R yx, R yy, R yz, R n, R& vx, R& vy, R
   vz = (vz - yz * d) / n;
 }
 
-// A_i^T v for the site's on-site tensor: per sublattice, or per site (symmetric 6 x N or full 9 x N).
+// (dg/de)^T v / 2 for the site's on-site gradient g at the state (sx, sy, sz): A_i^T v for the tensor
+// (per sublattice, or per site: symmetric 6 x N or full 9 x N), plus the cubic term's Hessian
+// C^T M C v / 2 (M_kk = 2 Kc (q - q_k), M_kl = 4 Kc a_k a_l, a = C s, q_k = a_k^2) and the higher
+// uniaxial term's f'(c) (u . v) u / 2 (c = u . s, f' = -12 K2 c^2 - 30 K3 c^4); both Hessians are symmetric.
 template <typename R>
-__device__ __forceinline__ void onsite_T(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st, R x, R y,
-                                         R z, R& ox, R& oy, R& oz) {
+__device__ __forceinline__ void onsite_T(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st, R sx,
+                                         R sy, R sz, R x, R y, R z, R& ox, R& oy, R& oz) {
   const int amode = S->amode;
   if (amode == ONSITE_SUB) {
     const R* a = S->Asub[st.s];
@@ -93,6 +96,28 @@ __device__ __forceinline__ void onsite_T(const Static<R>* __restrict__ S, const 
       oy = a[1] * x + a[4] * y + a[7] * z;
       oz = a[2] * x + a[5] * y + a[8] * z;
     }
+  }
+  if (S->has_c) {
+    const R* C = S->C;
+    const R a1 = C[0] * sx + C[1] * sy + C[2] * sz, a2 = C[3] * sx + C[4] * sy + C[5] * sz,
+            a3 = C[6] * sx + C[7] * sy + C[8] * sz;
+    const R w1 = C[0] * x + C[1] * y + C[2] * z, w2 = C[3] * x + C[4] * y + C[5] * z, w3 = C[6] * x + C[7] * y + C[8] * z;
+    const R kc = d.Kc[st.i], q1 = a1 * a1, q2 = a2 * a2, q3 = a3 * a3;
+    // z = M w / 2
+    const R z1 = kc * ((q2 + q3) * w1 + R(2) * a1 * (a2 * w2 + a3 * w3));
+    const R z2 = kc * ((q1 + q3) * w2 + R(2) * a2 * (a1 * w1 + a3 * w3));
+    const R z3 = kc * ((q1 + q2) * w3 + R(2) * a3 * (a1 * w1 + a2 * w2));
+    ox += C[0] * z1 + C[3] * z2 + C[6] * z3;
+    oy += C[1] * z1 + C[4] * z2 + C[7] * z3;
+    oz += C[2] * z1 + C[5] * z2 + C[8] * z3;
+  }
+  if (S->has_u) {
+    const R* u = S->u;
+    const R c = u[0] * sx + u[1] * sy + u[2] * sz, c2 = c * c;
+    const R h = (R(-6) * d.K2[st.i] * c2 - R(15) * d.K3[st.i] * c2 * c2) * (u[0] * x + u[1] * y + u[2] * z);
+    ox += h * u[0];
+    oy += h * u[1];
+    oz += h * u[2];
   }
 }
 
@@ -313,7 +338,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S
     const R vx2 = nu2[3 * i], vy2 = nu2[3 * i + 1], vz2 = nu2[3 * i + 2];
     R hx, hy, hz, ox, oy, oz;
     adj_gather<R, GEO, NP, DENSE, CM>(S, A, d, nu2, r.ep, st, vx2, vy2, vz2, px, py, pz, hx, hy, hz, gp, gb, ga);
-    onsite_T(S, d, st, vx2, vy2, vz2, ox, oy, oz);
+    onsite_T(S, d, st, px, py, pz, vx2, vy2, vz2, ox, oy, oz);
     R lx = gep[3 * i] + hx - ox, ly = gep[3 * i + 1] + hy - oy, lz = gep[3 * i + 2] + hz - oz;  // dL/dep
     if (d.pinned && d.pinned[i]) {
       // ep = e' = e: dL/de gathers lambda (in lamu) and dL/dep; the spin feels no field (nu1 = 0).
@@ -368,7 +393,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_c(const Static<R>* __restrict__ S
     R hx, hy, hz, ox, oy, oz;
     adj_gather<R, GEO, NP, DENSE, CM>(S, A, d, nu1, e, st, vx, vy, vz, e[3 * i], e[3 * i + 1], e[3 * i + 2], hx, hy,
                                       hz, gp, gb, ga);
-    onsite_T(S, d, st, vx, vy, vz, ox, oy, oz);
+    onsite_T(S, d, st, e[3 * i], e[3 * i + 1], e[3 * i + 2], vx, vy, vz, ox, oy, oz);
     const R lx = part[3 * i] + hx - ox, ly = part[3 * i + 1] + hy - oy, lz = part[3 * i + 2] + hz - oz;
     if constexpr (FUSE) {
       adj_local(S, d, st, prev, lx, ly, lz, lamu, gep, nu2);
@@ -639,6 +664,9 @@ void sweep_t(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
   d.bond = static_cast<const R*>(m.bond_d);
   d.A = static_cast<const R*>(m.A_d);
   d.pinned = static_cast<const uint8_t*>(m.pinned_d);
+  d.Kc = static_cast<const R*>(m.Kc_d);
+  d.K2 = static_cast<const R*>(m.K2_d);
+  d.K3 = static_cast<const R*>(m.K3_d);
   R* lam = static_cast<R*>(a.lam_d);
   R* lamu = static_cast<R*>(a.lamu_d);
   R* gep = static_cast<R*>(a.gep_d);
@@ -767,10 +795,7 @@ int mx_adjoint_create(mx_model* m, int32_t nparam, const double* unit_bonds, con
     if (checkpoint_every < 1) afail(MX_EINVAL, "checkpoint interval must be positive");
     if (m->nent < 1) afail(MX_EINVAL, "set the stencil first");
     // the couplings and on-site tensors are replaced by the parameterized ones (and any fixed parts
-    // given by mx_adjoint_set_offsets); the cubic and higher uniaxial terms are not differentiated
-    if (m->has_c || m->has_u)
-      afail(MX_EUNSUPPORTED, "the adjoint covers couplings, on-site tensors and pinning (no cubic or higher "
-                             "uniaxial terms)");
+    // given by mx_adjoint_set_offsets); cubic and higher uniaxial terms stay as the model has them
     if (m->geo == GEO_CHECKER) afail(MX_EUNSUPPORTED, "the adjoint runs on the row-major or tiled cell layouts");
     // unit tensors arrive in the caller's stencil order; reorder into the library's, and check that
     // every entry's partner carries the transposed tensors (the adjoint gather relies on J_ji = J_ij^T)

@@ -53,6 +53,27 @@ class TorchHeun:
         # which changes nothing but round-off in the value and drops the radial part of the derivative)
         self.pin = None if base.pinned is None else torch.as_tensor(np.asarray(base.pinned, bool))[:, None]
 
+    def terms_gradient(self, e):
+        """Gradient of the system's cubic and higher-uniaxial terms (magex_ref.CubicAnisotropy /
+        UniaxialHigher, terms.py:35-75), fixed in theta."""
+        g = torch.zeros_like(e)
+        for t in self.base.terms:
+            if isinstance(t, R.CubicAnisotropy):
+                C = torch.as_tensor(t.C)
+                a = e @ C.T
+                q = a * a
+                da = 2.0 * a * torch.stack([q[:, 1] + q[:, 2], q[:, 0] + q[:, 2], q[:, 0] + q[:, 1]], 1)
+                g = g + torch.as_tensor(np.broadcast_to(t.Kc, (self.N,)).copy())[:, None] * (da @ C)
+            elif isinstance(t, R.UniaxialHigher):
+                u = torch.as_tensor(t.u)
+                c = e @ u
+                K2 = torch.as_tensor(np.broadcast_to(t.K2, (self.N,)).copy())
+                K3 = torch.as_tensor(np.broadcast_to(t.K3, (self.N,)).copy())
+                g = g + (-4.0 * K2 * c**3 - 6.0 * K3 * c**5)[:, None] * u[None, :]
+            else:
+                raise NotImplementedError(type(t).__name__)
+        return g
+
     def effective_field(self, th, e):
         """B_eff (spinmodel.py:458-473) with theta th (torch, P) and e (N, 3)."""
         thf = torch.cat([th, th.new_ones(1)])
@@ -60,6 +81,7 @@ class TorchHeun:
         h = sum(thf[p] * torch.sparse.mm(self.Jp[p], x) for p in range(len(self.Jp))).reshape(-1, 3)
         A = torch.einsum("p,piab->iab", thf, self.Ap)
         g = -2.0 * h - R.MU_B * self.mu[:, None] * self.B[None, :] + 2.0 * torch.einsum("iab,ib->ia", A, e)
+        g = g + self.terms_gradient(e)
         return -g / (R.MU_B * self.mu[:, None])
 
     def rhs(self, e, B):
@@ -100,6 +122,7 @@ class TorchHeun:
         """B_eff (spinmodel.py:458-473) with the coupling matrix J (3N, 3N) and on-site tensors A (N, 3, 3)."""
         h = (J @ e.reshape(-1)).reshape(-1, 3)
         g = -2.0 * h - R.MU_B * self.mu[:, None] * self.B[None, :] + 2.0 * torch.einsum("iab,ib->ia", A, e)
+        g = g + self.terms_gradient(e)
         return -g / (R.MU_B * self.mu[:, None])
 
     def run_dense(self, J, A, e, steps, thermal=None):

@@ -331,6 +331,39 @@ def test_torch_heun_pinned_matches_oracle():
     assert np.abs(free - ref).max() > 1e-4
 
 
+def test_torch_heun_terms_match_oracle():
+    """Cubic and higher-uniaxial terms in the restatement: its trajectory is magex_ref.run_heun's, and
+    its spin gradient matches central differences (the terms' Hessians enter only there)."""
+    import torch
+    from oracle.torch_heun import TorchHeun
+    th = np.array([4.0, -0.4, 1.0, 0.2])
+    Kc = np.random.default_rng(3).uniform(0.2, 0.6, 25)
+    C = np.linalg.qr(np.random.default_rng(4).normal(size=(3, 3)))[0]
+
+    def build(t):
+        o = _square_builder(5)(t)
+        o.terms = [R.CubicAnisotropy(Kc, C), R.UniaxialHigher(0.3, 0.1, (0.2, 0.1, 1.0))]
+        return o
+
+    T = TorchHeun(build, th, alpha=0.3, dt=1e-3)
+    o = build(th)
+    e0 = o.init_random(6)
+    ref = R.run_heun(o, e0.copy(), 20, 0.3, 1e-3)
+    got = T.run(torch.as_tensor(th), torch.as_tensor(e0), 20).numpy()
+    assert np.abs(got - ref).max() < 1e-13
+    tgt = torch.as_tensor(o.init_random(7))
+    e = torch.tensor(e0, requires_grad=True)
+    loss = lambda x: ((T.run(torch.as_tensor(th), x, 20) - tgt) ** 2).mean()      # noqa: E731
+    loss(e).backward()
+    h = 1e-5
+    with torch.no_grad():
+        for (r, c) in [(3, 0), (11, 2)]:
+            d = torch.zeros_like(e)
+            d[r, c] = h
+            fd = (loss(e + d) - loss(e - d)) / (2 * h)
+            assert abs(fd - e.grad[r, c]) <= 1e-6 * abs(fd) + 1e-12, (r, c)
+
+
 def test_torch_heun_gradient_matches_central_differences():
     import torch
     from oracle.torch_heun import TorchHeun
