@@ -113,10 +113,11 @@ class DifferentiableLLG:
     build(theta) -> SpinSystem defines the Hamiltonian (its field, moments and stencil are taken at
     theta0; couplings and on-site tensors must be linear in theta).  alpha, dt (ps), temperature (K) and
     seed as LLGSolver; the thermal field is a fixed realization (the library's Philox stream), so the
-    gradient is that of one sample path.  precision "fp64" or "fp32"; layout as DeviceModel."""
+    gradient is that of one sample path.  precision "fp64" or "fp32"; layout as DeviceModel; sot as
+    LLGSolver's ({"p": direction, "B_DL": T, "B_FL": T}, magex/llg.py:83-86), held fixed."""
 
     def __init__(self, build, theta0, alpha, dt, temperature=0.0, seed=0, precision="fp64", checkpoint_every=50,
-                 precession=True, layout=None):
+                 precession=True, layout=None, sot=None):
         _torch()
         U, Aun, offset, system, boff, aoff = linear_units(build, theta0)
         self.P = len(theta0)
@@ -130,6 +131,9 @@ class DifferentiableLLG:
         self.seed = int(seed)
         self.dm = DeviceModel(system, precision, layout=layout)
         self.dm.set_dynamics(dt, alpha, temperature, precession)
+        if sot:
+            p = np.asarray(sot.get("p", [0, 1, 0]), float)
+            self.dm.set_sot(p / np.linalg.norm(p), float(sot.get("B_DL", 0.0)), float(sot.get("B_FL", 0.0)))
         self.checkpoint_every = int(checkpoint_every)
         L = self.dm.lib
         h = ctypes.c_void_p()

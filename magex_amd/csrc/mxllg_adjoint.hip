@@ -70,6 +70,7 @@ __device__ __forceinline__ void unproject(R yx, R yy, R yz, R n, R& vx, R& vy, R
 // (per sublattice, or per site: symmetric 6 x N or full 9 x N), plus the cubic term's Hessian
 // C^T M C v / 2 (M_kk = 2 Kc (q - q_k), M_kl = 4 Kc a_k a_l, a = C s, q_k = a_k^2) and the higher
 // uniaxial term's f'(c) (u . v) u / 2 (c = u . s, f' = -12 K2 c^2 - 30 K3 c^4); both Hessians are symmetric.
+// The spin-orbit field B_DL (e x p) adds -B_DL (p x v) mu mu_B / 2 (it enters B directly, not through g).
 template <typename R>
 __device__ __forceinline__ void onsite_T(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st, R sx,
                                          R sy, R sz, R x, R y, R z, R& ox, R& oy, R& oz) {
@@ -118,6 +119,12 @@ __device__ __forceinline__ void onsite_T(const Static<R>* __restrict__ S, const 
     ox += h * u[0];
     oy += h * u[1];
     oz += h * u[2];
+  }
+  if (d.sot && d.bdl != R(0)) {
+    const R k = -d.bdl * R(0.5) / inv_mu_of(S, d, st);
+    ox += k * (d.sp1 * z - d.sp2 * y);
+    oy += k * (d.sp2 * x - d.sp0 * z);
+    oz += k * (d.sp0 * y - d.sp1 * x);
   }
 }
 
@@ -667,6 +674,12 @@ void sweep_t(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
   d.Kc = static_cast<const R*>(m.Kc_d);
   d.K2 = static_cast<const R*>(m.K2_d);
   d.K3 = static_cast<const R*>(m.K3_d);
+  d.sot = (m.sot_dl != 0.0 || m.sot_fl != 0.0) ? 1 : 0;
+  d.sp0 = R(m.sot_p[0]);
+  d.sp1 = R(m.sot_p[1]);
+  d.sp2 = R(m.sot_p[2]);
+  d.bdl = R(m.sot_dl);
+  d.bfl = R(m.sot_fl);
   R* lam = static_cast<R*>(a.lam_d);
   R* lamu = static_cast<R*>(a.lamu_d);
   R* gep = static_cast<R*>(a.gep_d);
@@ -881,7 +894,6 @@ int mx_adjoint_forward(mx_adjoint* a, void* e, int64_t nsteps, uint64_t seed, ui
     Model& m = *a->m;
     if (!m.dyn_set) afail(MX_EINVAL, "set the dynamics first");
     if (m.integrator != MX_HEUN) afail(MX_EUNSUPPORTED, "the adjoint differentiates the Heun integrator");
-    if (m.sot_dl != 0.0 || m.sot_fl != 0.0) afail(MX_EUNSUPPORTED, "spin-orbit torque fields are not differentiated");
     aok(hipSetDevice(m.device), "hipSetDevice");
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int64_t nck = nsteps / a->C + 1;

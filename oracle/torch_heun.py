@@ -30,7 +30,7 @@ def _coo(M):
 class TorchHeun:
     """build(theta) -> magex_ref.SpinSystem, linear in theta; units by differences at theta_ref."""
 
-    def __init__(self, build, theta_ref, alpha, dt, precession=True):
+    def __init__(self, build, theta_ref, alpha, dt, precession=True, sot=None):
         theta_ref = np.asarray(theta_ref, float)
         base = build(theta_ref)
         self.N = base.N
@@ -49,6 +49,8 @@ class TorchHeun:
         self.B = torch.as_tensor(base.B)
         self.alpha, self.dt, self.precession = float(alpha), float(dt), precession
         self.base = base
+        # spin-orbit torque fields B_DL (e x p) + B_FL p (magex/llg.py:126-128), sot = (p unit, B_DL, B_FL)
+        self.sot = None if sot is None else (torch.as_tensor(np.asarray(sot[0], float)), float(sot[1]), float(sot[2]))
         # pinned spins are carried unchanged (a copy, as the device kernels do; magex renormalises them,
         # which changes nothing but round-off in the value and drops the radial part of the derivative)
         self.pin = None if base.pinned is None else torch.as_tensor(np.asarray(base.pinned, bool))[:, None]
@@ -82,7 +84,13 @@ class TorchHeun:
         A = torch.einsum("p,piab->iab", thf, self.Ap)
         g = -2.0 * h - R.MU_B * self.mu[:, None] * self.B[None, :] + 2.0 * torch.einsum("iab,ib->ia", A, e)
         g = g + self.terms_gradient(e)
-        return -g / (R.MU_B * self.mu[:, None])
+        return self._sot(-g / (R.MU_B * self.mu[:, None]), e)
+
+    def _sot(self, B, e):
+        if self.sot is None:
+            return B
+        p, bdl, bfl = self.sot
+        return B + bdl * torch.cross(e, p.expand_as(e), dim=1) + bfl * p[None, :]
 
     def rhs(self, e, B):
         a = self.alpha
@@ -123,7 +131,7 @@ class TorchHeun:
         h = (J @ e.reshape(-1)).reshape(-1, 3)
         g = -2.0 * h - R.MU_B * self.mu[:, None] * self.B[None, :] + 2.0 * torch.einsum("iab,ib->ia", A, e)
         g = g + self.terms_gradient(e)
-        return -g / (R.MU_B * self.mu[:, None])
+        return self._sot(-g / (R.MU_B * self.mu[:, None]), e)
 
     def run_dense(self, J, A, e, steps, thermal=None):
         """run() with the couplings given as the dense matrix J and on-site tensors A."""

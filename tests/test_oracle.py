@@ -364,6 +364,23 @@ def test_torch_heun_terms_match_oracle():
             assert abs(fd - e.grad[r, c]) <= 1e-6 * abs(fd) + 1e-12, (r, c)
 
 
+def test_torch_heun_sot_matches_oracle():
+    """Spin-orbit torque fields in the restatement: its trajectory is magex_ref.run_heun's with the same
+    sot, and differs from the one without (a dropped term shows)."""
+    import torch
+    from oracle.torch_heun import TorchHeun
+    th = np.array([4.0, -0.4, 1.0, 0.2])
+    p = np.array([0.3, 1.0, 0.2])
+    sot = (p / np.linalg.norm(p), 1.5, 0.7)
+    T = TorchHeun(_square_builder(5), th, alpha=0.3, dt=1e-3, sot=sot)
+    o = _square_builder(5)(th)
+    e0 = o.init_random(8)
+    ref = R.run_heun(o, e0.copy(), 20, 0.3, 1e-3, sot=sot)
+    got = T.run(torch.as_tensor(th), torch.as_tensor(e0), 20).numpy()
+    assert np.abs(got - ref).max() < 1e-13
+    assert np.abs(R.run_heun(o, e0.copy(), 20, 0.3, 1e-3) - ref).max() > 1e-5
+
+
 def test_torch_heun_gradient_matches_central_differences():
     import torch
     from oracle.torch_heun import TorchHeun
