@@ -259,6 +259,11 @@ int mx_adjoint_set_offsets(mx_adjoint* a, const double* bond_offset, const doubl
 /* nsteps Heun steps of e (device, N x 3, in place) as mx_llg_run with the same seed and step0,
  * recording checkpoints for mx_adjoint_backward. */
 int mx_adjoint_forward(mx_adjoint* a, void* e, int64_t nsteps, uint64_t seed, uint64_t step0, void* stream);
+/* mx_adjoint_forward under a time-dependent uniform field, as mx_llg_run_driven: B (host, nsteps x 2 x 3)
+ * is the field of the predictor and corrector pass of each step (magex's field_func at t and t + dt,
+ * magex/llg.py:129-130, 204-211); kept for the backward's replays. */
+int mx_adjoint_forward_driven(mx_adjoint* a, void* e, int64_t nsteps, const double* B, uint64_t seed,
+                              uint64_t step0, void* stream);
 /* Given grad_e = dL/de_final (device, N x 3), writes dL/de_0 into grad_e0 (device, N x 3, may be null)
  * and dL/dtheta into grad_theta (device, nparam doubles), for the last mx_adjoint_forward. */
 int mx_adjoint_backward(mx_adjoint* a, const void* grad_e, void* grad_e0, double* grad_theta, void* stream);
@@ -271,6 +276,11 @@ int mx_adjoint_backward(mx_adjoint* a, const void* grad_e, void* grad_e0, double
  * magex/models.py:37). */
 int mx_adjoint_backward_dense(mx_adjoint* a, const void* grad_e, void* grad_e0, double* grad_theta,
                               double* grad_bonds, double* grad_onsite, void* stream);
+/* mx_adjoint_backward plus dL/dB of the uniform field in every pass of every step of the last forward
+ * (driven or not): grad_field (device, nsteps x 2 x 3 doubles; predictor, then corrector pass), the sum
+ * over sites of dL/dB_i.  No counterpart in magex (it does not differentiate). */
+int mx_adjoint_backward_field(mx_adjoint* a, const void* grad_e, void* grad_e0, double* grad_theta, double* grad_field,
+                              void* stream);
 /* nbonds: forward stencil entries (s < t, or s == t and R > 0), as mx_model_set_bond_tensors counts them. */
 int mx_adjoint_num_bonds(const mx_adjoint* a);
 

@@ -74,6 +74,8 @@ SIGNATURES = {
     "mx_adjoint_set_offsets": (_I, [_P, _P, _P]),
     "mx_adjoint_forward": (_I, [_P, _P, ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint64, _P]),
     "mx_adjoint_backward": (_I, [_P, _P, _P, _P, _P]),
+    "mx_adjoint_forward_driven": (_I, [_P, _P, ctypes.c_int64, _P, ctypes.c_uint64, ctypes.c_uint64, _P]),
+    "mx_adjoint_backward_field": (_I, [_P, _P, _P, _P, _P, _P]),
     "mx_adjoint_backward_dense": (_I, [_P, _P, _P, _P, _P, _P, _P]),
     "mx_adjoint_num_bonds": (_I, [_P]),
 }

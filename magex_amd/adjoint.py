@@ -131,6 +131,8 @@ class DifferentiableLLG:
         self.seed = int(seed)
         self.dm = DeviceModel(system, precision, layout=layout)
         self.dm.set_dynamics(dt, alpha, temperature, precession)
+        self.dt = float(dt)
+        self._nsteps = 0
         if sot:
             p = np.asarray(sot.get("p", [0, 1, 0]), float)
             self.dm.set_sot(p / np.linalg.norm(p), float(sot.get("B_DL", 0.0)), float(sot.get("B_FL", 0.0)))
@@ -174,14 +176,33 @@ class DifferentiableLLG:
             raise ValueError(f"spins must be ({self.N}, 3)")
         return e.clone()
 
-    def forward(self, theta, e0, nsteps, step0=0):
-        """Spins after nsteps steps from e0 (new device tensor); records checkpoints for backward()."""
+    def forward(self, theta, e0, nsteps, step0=0, field=None):
+        """Spins after nsteps steps from e0 (new device tensor); records checkpoints for backward().
+        field: the uniform field (T) of the predictor and corrector pass of each step, (nsteps, 2, 3)
+        (field_sequence builds it from a field_func), or None for the system's own field."""
         self.set_params(theta)
         e = self._spins(e0)
-        check(self.dm.lib.mx_adjoint_forward(self.h, ctypes.c_void_p(e.data_ptr()), int(nsteps), self.seed,
-                                             int(step0), _stream()))
+        L = self.dm.lib
+        if field is None:
+            check(L.mx_adjoint_forward(self.h, ctypes.c_void_p(e.data_ptr()), int(nsteps), self.seed, int(step0),
+                                       _stream()))
+        else:
+            B = np.ascontiguousarray(np.asarray(field, float).reshape(int(nsteps), 2, 3))
+            check(L.mx_adjoint_forward_driven(self.h, ctypes.c_void_p(e.data_ptr()), int(nsteps), hptr(B), self.seed,
+                                              int(step0), _stream()))
+        self._nsteps = int(nsteps)
         self._token += 1
         return e
+
+    def field_sequence(self, field_func, nsteps, t0=0.0):
+        """(nsteps, 2, 3) total fields B + field_func(t) at t = t0 + n dt and t + dt, as LLGSolver drives
+        them (magex/llg.py:129-130)."""
+        out = np.empty((int(nsteps), 2, 3))
+        for n in range(int(nsteps)):
+            t = t0 + n * self.dt
+            for k, tk in enumerate((t, t + self.dt)):
+                out[n, k] = self.system.B + np.asarray(field_func(tk), float).reshape(3)
+        return out
 
     def backward(self, grad_e, want_e0=True):
         """(dL/dtheta (P,) float64 device tensor, dL/de0 or None) for the last forward()."""
@@ -215,41 +236,64 @@ class DifferentiableLLG:
                                                     ctypes.c_void_p(ga.data_ptr()), _stream()))
         return gt[:self.P], g0, gb, ga
 
-    def trajectory(self, theta, e0, nsteps, step0=0):
-        """Differentiable e(nsteps) as a function of theta (torch tensor, P) and e0 (N, 3)."""
+    def backward_field(self, grad_e, want_e0=True):
+        """(dL/dtheta, dL/de0 or None, dL/dB (nsteps, 2, 3)) for the last forward(): dL/dB is the gradient
+        with respect to the uniform field of each pass of each step (driven or not), float64 device tensor."""
+        g = grad_e.detach().to(device="cuda", dtype=self.dtype).contiguous()
+        gt = torch.empty(MAXP, dtype=torch.float64, device="cuda")
+        g0 = torch.empty_like(g) if want_e0 else None
+        gB = torch.empty((self._nsteps, 2, 3), dtype=torch.float64, device="cuda")
+        check(self.dm.lib.mx_adjoint_backward_field(self.h, ctypes.c_void_p(g.data_ptr()),
+                                                    ctypes.c_void_p(g0.data_ptr() if g0 is not None else 0),
+                                                    ctypes.c_void_p(gt.data_ptr()), ctypes.c_void_p(gB.data_ptr()),
+                                                    _stream()))
+        return gt[:self.P], g0, gB
+
+    def trajectory(self, theta, e0, nsteps, step0=0, field=None):
+        """Differentiable e(nsteps) as a function of theta (torch tensor, P), e0 (N, 3) and, when given,
+        the driving field (nsteps, 2, 3) of forward()."""
         if not isinstance(theta, torch.Tensor):
             theta = torch.as_tensor(np.asarray(theta, float))
         if not isinstance(e0, torch.Tensor):
             e0 = torch.as_tensor(np.asarray(e0, float))
-        return _HeunTrajectory.apply(theta, e0, self, int(nsteps), int(step0))
+        if field is not None and not isinstance(field, torch.Tensor):
+            field = torch.as_tensor(np.asarray(field, float))
+        return _HeunTrajectory.apply(theta, e0, field, self, int(nsteps), int(step0))
 
 
 class _HeunTrajectory(torch.autograd.Function):
-    """e_final = Heun^nsteps(e0; theta): forward through mx_adjoint_forward, backward through
-    mx_adjoint_backward (re-running the forward first if another forward has run since)."""
+    """e_final = Heun^nsteps(e0; theta, field): forward through mx_adjoint_forward(_driven), backward
+    through mx_adjoint_backward(_field) (re-running the forward first if another forward has run since)."""
 
     @staticmethod
-    def forward(ctx, theta, e0, sol, nsteps, step0):
+    def forward(ctx, theta, e0, field, sol, nsteps, step0):
         th = theta.detach().cpu().double().numpy()
-        e = sol.forward(th, e0, nsteps, step0)
-        ctx.sol, ctx.nsteps, ctx.step0, ctx.theta = sol, nsteps, step0, th
+        fld = None if field is None else field.detach().cpu().double().numpy()
+        e = sol.forward(th, e0, nsteps, step0, field=fld)
+        ctx.sol, ctx.nsteps, ctx.step0, ctx.theta, ctx.field = sol, nsteps, step0, th, fld
         ctx.token = sol._token
         ctx.e0 = e0.detach().clone()
-        ctx.meta = (theta.dtype, theta.device, e0.dtype, e0.device)
+        ctx.meta = (theta.dtype, theta.device, e0.dtype, e0.device,
+                    None if field is None else (field.dtype, field.device))
         return e.to(dtype=e0.dtype, device=e0.device)
 
     @staticmethod
     def backward(ctx, g):
         sol = ctx.sol
         if sol._token != ctx.token:
-            sol.forward(ctx.theta, ctx.e0, ctx.nsteps, ctx.step0)
+            sol.forward(ctx.theta, ctx.e0, ctx.nsteps, ctx.step0, field=ctx.field)
             ctx.token = sol._token
-        gt, g0 = sol.backward(g, want_e0=ctx.needs_input_grad[1])
-        tdt, tdev, edt, edev = ctx.meta
+        tdt, tdev, edt, edev, fmeta = ctx.meta
+        gB = None
+        if fmeta is not None and ctx.needs_input_grad[2]:
+            gt, g0, gB = sol.backward_field(g, want_e0=ctx.needs_input_grad[1])
+            gB = gB.to(dtype=fmeta[0], device=fmeta[1])
+        else:
+            gt, g0 = sol.backward(g, want_e0=ctx.needs_input_grad[1])
         gt = gt.to(dtype=tdt, device=tdev)
         if g0 is not None:
             g0 = g0.to(dtype=edt, device=edev)
-        return gt, g0, None, None, None
+        return gt, g0, gB, None, None, None
 
 
 def dmi_gradient(grad_bonds):

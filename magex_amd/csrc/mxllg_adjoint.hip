@@ -326,23 +326,41 @@ __device__ __forceinline__ void store_partials(double (&gp)[NP], double old, dou
   if (threadIdx.x < NP) gpart[size_t(blockIdx.x) * MAXP + threadIdx.x] = old + res[threadIdx.x];
 }
 
+// dL/dB of the uniform field in this pass: the workgroup's sum of nu mu mu_B / 2 (= dL/dB_i) into row
+// gf (nblk x 3), when the caller asks for it (gf is uniform over the grid).
+__device__ __forceinline__ void store_field_partials(double (&fv)[3], double* __restrict__ gf) {
+  if (!gf) return;
+  __shared__ double fres[3];
+  block_reduce<3>(fv, 3, fres);
+  __syncthreads();
+  if (threadIdx.x < 3) gf[size_t(blockIdx.x) * 3 + threadIdx.x] = fres[threadIdx.x];
+}
+
 // Gather nu2 (dL/dep) and the predictor's adjoint: writes nu1 and the partial dL/de (part).
 template <typename R, int GEO, int NP, bool DENSE, int CM>
 __global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
                                                  const Dyn<R> d, const StepRec<R> r, const R* __restrict__ lamu,
                                                  const R* __restrict__ gep, const R* __restrict__ nu2,
                                                  R* __restrict__ nu1, R* __restrict__ part, double* __restrict__ gpart,
-                                                 double* __restrict__ gb, double* __restrict__ ga) {
+                                                 double* __restrict__ gb, double* __restrict__ ga,
+                                                 double* __restrict__ gf) {
   const double old = partial_row<NP>(gpart);
   double gp[NP];
 #pragma unroll
   for (int p = 0; p < NP; ++p) gp[p] = 0.0;
+  double fv[3] = {0.0, 0.0, 0.0};
   Site st;
   if (site_of<R, GEO>(S, st)) {
     const int i = st.i;
     const R a = d.alpha ? d.alpha[i] : S->alpha_sub[st.s];
     const R px = r.ep[3 * i], py = r.ep[3 * i + 1], pz = r.ep[3 * i + 2];
     const R vx2 = nu2[3 * i], vy2 = nu2[3 * i + 1], vz2 = nu2[3 * i + 2];
+    if (gf) {
+      const double h = 0.5 / double(d.inv_mu ? d.inv_mu[i] : S->inv_mu_sub[st.s]);
+      fv[0] = h * double(vx2);
+      fv[1] = h * double(vy2);
+      fv[2] = h * double(vz2);
+    }
     R hx, hy, hz, ox, oy, oz;
     adj_gather<R, GEO, NP, DENSE, CM>(S, A, d, nu2, r.ep, st, vx2, vy2, vz2, px, py, pz, hx, hy, hz, gp, gb, ga);
     onsite_T(S, d, st, px, py, pz, vx2, vy2, vz2, ox, oy, oz);
@@ -377,6 +395,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S
     }
   }
   store_partials<NP>(gp, old, gpart);
+  store_field_partials(fv, gf);
 }
 
 // Gather nu1: lambda = dL/de of the step's input, and the parameter terms at e.  With FUSE the
@@ -388,15 +407,23 @@ __global__ __launch_bounds__(BLOCK) void k_adj_c(const Static<R>* __restrict__ S
                                                  const R* __restrict__ part, R* __restrict__ lam,
                                                  const StepRec<R> prev, R* __restrict__ lamu, R* __restrict__ gep,
                                                  R* __restrict__ nu2, double* __restrict__ gpart,
-                                                 double* __restrict__ gb, double* __restrict__ ga) {
+                                                 double* __restrict__ gb, double* __restrict__ ga,
+                                                 double* __restrict__ gf) {
   const double old = partial_row<NP>(gpart);
   double gp[NP];
 #pragma unroll
   for (int p = 0; p < NP; ++p) gp[p] = 0.0;
+  double fv[3] = {0.0, 0.0, 0.0};
   Site st;
   if (site_of<R, GEO>(S, st)) {
     const int i = st.i;
     const R vx = nu1[3 * i], vy = nu1[3 * i + 1], vz = nu1[3 * i + 2];
+    if (gf) {
+      const double h = 0.5 / double(d.inv_mu ? d.inv_mu[i] : S->inv_mu_sub[st.s]);
+      fv[0] = h * double(vx);
+      fv[1] = h * double(vy);
+      fv[2] = h * double(vz);
+    }
     R hx, hy, hz, ox, oy, oz;
     adj_gather<R, GEO, NP, DENSE, CM>(S, A, d, nu1, e, st, vx, vy, vz, e[3 * i], e[3 * i + 1], e[3 * i + 2], hx, hy,
                                       hz, gp, gb, ga);
@@ -411,6 +438,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_c(const Static<R>* __restrict__ S
     }
   }
   store_partials<NP>(gp, old, gpart);
+  store_field_partials(fv, gf);
 }
 
 template <typename T>
@@ -484,6 +512,13 @@ struct mx_adjoint {
   double *gb_d = nullptr, *ga_d = nullptr;
   int* cmap_d = nullptr;
   int cmap_geo = -1;
+  // driven forward: the uniform field of each pass of each step (nsteps x 2 x 3; empty: the model's)
+  std::vector<double> drive;
+  // dL/dB of the uniform field per step and pass: per-workgroup partials (nsteps x 2) x nblk x 3
+  double* gf_d = nullptr;
+  int64_t gf_rows = 0;
+  bool want_field = false;
+  int64_t sweep_step0 = 0;  // global step of the first record of the segment being swept
 };
 
 namespace {
@@ -634,11 +669,12 @@ void record_segment(mx_adjoint& a, const void* src, int64_t k, int n_seg, hipStr
   to_internal(a, src, ew, s);
   for (int n = 0; n < n_seg; ++n) {
     const uint64_t step = a.step0 + uint64_t(k * a.C + n);
+    const double* B = a.drive.empty() ? nullptr : a.drive.data() + 6 * size_t(k * a.C + n);
     aok_rc(launch_heun_pass(m, 1, ew, seg_slot(a, 1, n), seg_slot(a, 2, n), a.seed, step, seg_slot(a, 0, n),
-                            seg_slot(a, 3, n), s),
+                            seg_slot(a, 3, n), s, B),
            "segment pass 1");
     aok_rc(launch_heun_pass(m, 2, ew, seg_slot(a, 1, n), seg_slot(a, 2, n), a.seed, step, nullptr,
-                            seg_slot(a, 4, n), s),
+                            seg_slot(a, 4, n), s, B ? B + 3 : nullptr),
            "segment pass 2");
   }
   a.seg_k = k;
@@ -687,16 +723,20 @@ void sweep_t(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
   R* nu2 = static_cast<R*>(a.nu2_d);
   R* part = static_cast<R*>(a.part_d);
   const dim3 g(a.nblk), b(BLOCK);
+  // field-gradient rows: (global step * 2 + pass index) x nblk x 3, pass index 0 = predictor, 1 = corrector
+  auto frow = [&](int n, int pass) -> double* {
+    return a.want_field ? a.gf_d + (size_t(a.sweep_step0 + n) * 2 + pass) * a.nblk * 3 : nullptr;
+  };
   hipLaunchKernelGGL((k_adj_a<R, GEO>), g, b, 0, s, S, d, lam, rec[n_seg - 1], lamu, gep, nu2);
   for (int n = n_seg - 1; n >= 0; --n) {
     hipLaunchKernelGGL((k_adj_b<R, GEO, NP, DENSE, CM>), g, b, 0, s, S, A, d, rec[n], lamu, gep, nu2, nu1, part, a.gpart_d,
-                       a.gb_d, a.ga_d);
+                       a.gb_d, a.ga_d, frow(n, 1));
     if (n > 0)
       hipLaunchKernelGGL((k_adj_c<R, GEO, NP, true, DENSE, CM>), g, b, 0, s, S, A, d, rec[n].e, nu1, part, lam, rec[n - 1],
-                         lamu, gep, nu2, a.gpart_d, a.gb_d, a.ga_d);
+                         lamu, gep, nu2, a.gpart_d, a.gb_d, a.ga_d, frow(n, 0));
     else
       hipLaunchKernelGGL((k_adj_c<R, GEO, NP, false, DENSE, CM>), g, b, 0, s, S, A, d, rec[n].e, nu1, part, lam, rec[n],
-                         lamu, gep, nu2, a.gpart_d, a.gb_d, a.ga_d);
+                         lamu, gep, nu2, a.gpart_d, a.gb_d, a.ga_d, frow(n, 0));
   }
 }
 
@@ -752,7 +792,7 @@ void ensure_dense(mx_adjoint& a, hipStream_t s) {
 // The adjoint sweep over every segment, newest first; with grad_bonds / grad_onsite also the dense
 // gradients (either may be null).
 int backward_impl(mx_adjoint* a, const void* grad_e, void* grad_e0, double* grad_theta, double* grad_bonds,
-                  double* grad_onsite, void* stream) {
+                  double* grad_onsite, double* grad_field, void* stream) {
   if (!a || !grad_e || !grad_theta) afail(MX_EINVAL, "null argument");
   Model& m = *a->m;
   if (!a->recorded) afail(MX_EINVAL, "no forward recorded for the current parameters: run mx_adjoint_forward");
@@ -763,6 +803,15 @@ int backward_impl(mx_adjoint* a, const void* grad_e, void* grad_e0, double* grad
   const bool fp64 = m.prec == MX_FP64;
   const bool dense = grad_bonds || grad_onsite;
   if (dense) ensure_dense(*a, s);
+  a->want_field = grad_field != nullptr && a->nsteps > 0;
+  if (a->want_field && a->gf_rows < 2 * a->nsteps) {
+    if (a->gf_d) (void)hipFree(a->gf_d);
+    a->gf_d = nullptr;
+    a->gf_rows = 0;
+    aok(hipMalloc(reinterpret_cast<void**>(&a->gf_d), sizeof(double) * 2 * size_t(a->nsteps) * a->nblk * 3),
+        "hipMalloc field-gradient partials");
+    a->gf_rows = 2 * a->nsteps;
+  }
   // the adjoint works in the library's internal site order; permute at entry and exit
   to_internal(*a, grad_e, a->lam_d, s);
   hipLaunchKernelGGL((k_zero<double>), dim3((MAXP * a->nblk + 255) / 256), dim3(256), 0, s, a->gpart_d,
@@ -773,12 +822,17 @@ int backward_impl(mx_adjoint* a, const void* grad_e, void* grad_e0, double* grad
     const int n_seg = int(std::min<int64_t>(C, a->nsteps - k * C));
     // replay the segment from its checkpoint unless its records are still held
     if (a->seg_k != k) record_segment(*a, ck + size_t(k) * b, k, n_seg, s);
+    a->sweep_step0 = k * C;
     if (fp64) sweep<double>(*a, static_cast<char*>(a->seg_d), b, a->seg_cap, n_seg, dense, s);
     else sweep<float>(*a, static_cast<char*>(a->seg_d), b, a->seg_cap, n_seg, dense, s);
   }
   hipLaunchKernelGGL((k_final<MAXP>), dim3(1), dim3(BLOCK), 0, s, a->gpart_d, a->nblk, MAXP, 1.0, a->gout_d);
   hipLaunchKernelGGL((k_copy<double>), dim3(1), dim3(64), 0, s, a->gout_d, grad_theta, size_t(a->P));
   if (grad_e0) to_magex(*a, a->lam_d, grad_e0, s);
+  if (a->want_field)  // every (step, pass) row was written once by the sweep
+    hipLaunchKernelGGL((k_final<3>), dim3(unsigned(2 * a->nsteps)), dim3(BLOCK), 0, s, a->gf_d, a->nblk, 3, 1.0,
+                       grad_field);
+  a->want_field = false;
   const int* cmap = m.geo == GEO_TILED ? a->cmap_d : nullptr;
   if (grad_bonds && a->nfwd) {
     const size_t n = size_t(a->nfwd) * m.ncell * 9;
@@ -872,7 +926,8 @@ void mx_adjoint_destroy(mx_adjoint* a) {
   for (void** p : {&a->astatic_d, &a->ckpt_d, &a->seg_d, &a->lam_d, &a->lamu_d, &a->gep_d, &a->nu1_d, &a->nu2_d,
                    &a->part_d, &a->ework_d, reinterpret_cast<void**>(&a->gpart_d),
                    reinterpret_cast<void**>(&a->gout_d), reinterpret_cast<void**>(&a->gb_d),
-                   reinterpret_cast<void**>(&a->ga_d), reinterpret_cast<void**>(&a->cmap_d)})
+                   reinterpret_cast<void**>(&a->ga_d), reinterpret_cast<void**>(&a->cmap_d),
+                   reinterpret_cast<void**>(&a->gf_d)})
     afree(*p);
   delete a;
 }
@@ -887,62 +942,93 @@ int mx_adjoint_set_params(mx_adjoint* a, const double* theta) {
   });
 }
 
+namespace {
+
+// The recorded forward: nsteps steps of e from step0, checkpoints every C steps; B (host, nsteps x 2 x 3)
+// drives the uniform field of each pass, or is null for the model's own field.
+int forward_impl(mx_adjoint* a, void* e, int64_t nsteps, const double* B, uint64_t seed, uint64_t step0,
+                 void* stream) {
+  if (!a || !e) afail(MX_EINVAL, "null argument");
+  if (nsteps < 0) afail(MX_EINVAL, "negative step count");
+  if (B) a->drive.assign(B, B + 6 * size_t(nsteps));
+  else a->drive.clear();
+  Model& m = *a->m;
+  if (!m.dyn_set) afail(MX_EINVAL, "set the dynamics first");
+  if (m.integrator != MX_HEUN) afail(MX_EUNSUPPORTED, "the adjoint differentiates the Heun integrator");
+  aok(hipSetDevice(m.device), "hipSetDevice");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int64_t nck = nsteps / a->C + 1;
+  const size_t b = nbytes3(m);
+  if (nck > a->ckpt_cap) {
+    afree(a->ckpt_d);
+    aok(hipMalloc(&a->ckpt_d, size_t(nck) * b), "hipMalloc checkpoints");
+    a->ckpt_cap = nck;
+  }
+  ensure_buffers(*a);
+  a->recorded = false;
+  a->nsteps = nsteps;
+  a->seed = seed;
+  a->step0 = step0;
+  // checkpoint k holds the (magex-order) spins after k * C steps
+  char* ck = static_cast<char*>(a->ckpt_d);
+  const size_t nel = size_t(m.N) * 3;
+  // the last segment runs with the single-pass kernels and keeps its records, so the backward starts
+  // without a replay
+  const int64_t k_last = nsteps > 0 ? (nsteps - 1) / a->C : -1;
+  ensure_segment(*a, int(std::min<int64_t>(a->C, std::max<int64_t>(nsteps, 1))));
+  a->seg_k = -1;
+  for (int64_t k = 0; k * a->C <= nsteps; ++k) {
+    if (m.prec == MX_FP64) copy_dev<double>(e, ck + size_t(k) * b, nel, s);
+    else copy_dev<float>(e, ck + size_t(k) * b, nel, s);
+    const int64_t n = std::min<int64_t>(a->C, nsteps - k * a->C);
+    if (n <= 0) continue;
+    if (k == k_last) {
+      aok_rc(sync_model_static(m), "model constants");
+      record_segment(*a, e, k, int(n), s);
+      to_magex(*a, a->ework_d, e, s);
+    } else if (B) {
+      aok_rc(mx_llg_run_driven(a->m, e, n, B + 6 * size_t(k * a->C), seed, step0 + uint64_t(k * a->C), stream),
+             "forward");
+    } else {
+      aok_rc(mx_llg_run(a->m, e, n, seed, step0 + uint64_t(k * a->C), nullptr, stream), "forward");
+    }
+  }
+  const hipError_t err = hipGetLastError();
+  if (err != hipSuccess) afail(MX_EHIP, std::string("adjoint forward: ") + hipGetErrorString(err));
+  a->recorded = true;
+  return MX_OK;
+}
+
+}  // namespace
+
 int mx_adjoint_forward(mx_adjoint* a, void* e, int64_t nsteps, uint64_t seed, uint64_t step0, void* stream) {
+  return aguard([&]() -> int { return forward_impl(a, e, nsteps, nullptr, seed, step0, stream); });
+}
+
+int mx_adjoint_forward_driven(mx_adjoint* a, void* e, int64_t nsteps, const double* B, uint64_t seed,
+                              uint64_t step0, void* stream) {
   return aguard([&]() -> int {
-    if (!a || !e) afail(MX_EINVAL, "null argument");
-    if (nsteps < 0) afail(MX_EINVAL, "negative step count");
-    Model& m = *a->m;
-    if (!m.dyn_set) afail(MX_EINVAL, "set the dynamics first");
-    if (m.integrator != MX_HEUN) afail(MX_EUNSUPPORTED, "the adjoint differentiates the Heun integrator");
-    aok(hipSetDevice(m.device), "hipSetDevice");
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    const int64_t nck = nsteps / a->C + 1;
-    const size_t b = nbytes3(m);
-    if (nck > a->ckpt_cap) {
-      afree(a->ckpt_d);
-      aok(hipMalloc(&a->ckpt_d, size_t(nck) * b), "hipMalloc checkpoints");
-      a->ckpt_cap = nck;
-    }
-    ensure_buffers(*a);
-    a->recorded = false;
-    a->nsteps = nsteps;
-    a->seed = seed;
-    a->step0 = step0;
-    // checkpoint k holds the (magex-order) spins after k * C steps
-    char* ck = static_cast<char*>(a->ckpt_d);
-    const size_t nel = size_t(m.N) * 3;
-    // the last segment runs with the single-pass kernels and keeps its records, so the backward starts
-    // without a replay
-    const int64_t k_last = nsteps > 0 ? (nsteps - 1) / a->C : -1;
-    ensure_segment(*a, int(std::min<int64_t>(a->C, std::max<int64_t>(nsteps, 1))));
-    a->seg_k = -1;
-    for (int64_t k = 0; k * a->C <= nsteps; ++k) {
-      if (m.prec == MX_FP64) copy_dev<double>(e, ck + size_t(k) * b, nel, s);
-      else copy_dev<float>(e, ck + size_t(k) * b, nel, s);
-      const int64_t n = std::min<int64_t>(a->C, nsteps - k * a->C);
-      if (n <= 0) continue;
-      if (k == k_last) {
-        aok_rc(sync_model_static(m), "model constants");
-        record_segment(*a, e, k, int(n), s);
-        to_magex(*a, a->ework_d, e, s);
-      } else {
-        aok_rc(mx_llg_run(a->m, e, n, seed, step0 + uint64_t(k * a->C), nullptr, stream), "forward");
-      }
-    }
-    const hipError_t err = hipGetLastError();
-    if (err != hipSuccess) afail(MX_EHIP, std::string("adjoint forward: ") + hipGetErrorString(err));
-    a->recorded = true;
-    return MX_OK;
+    if (!B) afail(MX_EINVAL, "null field sequence");
+    return forward_impl(a, e, nsteps, B, seed, step0, stream);
   });
 }
 
 int mx_adjoint_backward(mx_adjoint* a, const void* grad_e, void* grad_e0, double* grad_theta, void* stream) {
-  return aguard([&]() -> int { return backward_impl(a, grad_e, grad_e0, grad_theta, nullptr, nullptr, stream); });
+  return aguard([&]() -> int { return backward_impl(a, grad_e, grad_e0, grad_theta, nullptr, nullptr, nullptr, stream); });
 }
 
 int mx_adjoint_backward_dense(mx_adjoint* a, const void* grad_e, void* grad_e0, double* grad_theta,
                               double* grad_bonds, double* grad_onsite, void* stream) {
-  return aguard([&]() -> int { return backward_impl(a, grad_e, grad_e0, grad_theta, grad_bonds, grad_onsite, stream); });
+  return aguard(
+      [&]() -> int { return backward_impl(a, grad_e, grad_e0, grad_theta, grad_bonds, grad_onsite, nullptr, stream); });
+}
+
+int mx_adjoint_backward_field(mx_adjoint* a, const void* grad_e, void* grad_e0, double* grad_theta, double* grad_field,
+                              void* stream) {
+  return aguard([&]() -> int {
+    if (!grad_field) afail(MX_EINVAL, "null field-gradient output");
+    return backward_impl(a, grad_e, grad_e0, grad_theta, nullptr, nullptr, grad_field, stream);
+  });
 }
 
 int mx_adjoint_num_bonds(const mx_adjoint* a) { return a ? a->nfwd : -1; }

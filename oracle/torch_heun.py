@@ -76,13 +76,15 @@ class TorchHeun:
                 raise NotImplementedError(type(t).__name__)
         return g
 
-    def effective_field(self, th, e):
-        """B_eff (spinmodel.py:458-473) with theta th (torch, P) and e (N, 3)."""
+    def effective_field(self, th, e, B=None):
+        """B_eff (spinmodel.py:458-473) with theta th (torch, P) and e (N, 3); B replaces the system's
+        uniform field (a driven pass: B + field_func(t), magex/llg.py:129-130)."""
         thf = torch.cat([th, th.new_ones(1)])
         x = e.reshape(-1, 1)
         h = sum(thf[p] * torch.sparse.mm(self.Jp[p], x) for p in range(len(self.Jp))).reshape(-1, 3)
         A = torch.einsum("p,piab->iab", thf, self.Ap)
-        g = -2.0 * h - R.MU_B * self.mu[:, None] * self.B[None, :] + 2.0 * torch.einsum("iab,ib->ia", A, e)
+        Bu = self.B if B is None else B
+        g = -2.0 * h - R.MU_B * self.mu[:, None] * Bu[None, :] + 2.0 * torch.einsum("iab,ib->ia", A, e)
         g = g + self.terms_gradient(e)
         return self._sot(-g / (R.MU_B * self.mu[:, None]), e)
 
@@ -108,14 +110,17 @@ class TorchHeun:
     def _keep_pinned(self, new, old):
         return new if self.pin is None else torch.where(self.pin, old, new)
 
-    def run(self, th, e, steps, thermal=None):
-        """steps Heun steps; thermal(n) -> (N, 3) thermal field of step n (torch FP64) or None."""
+    def run(self, th, e, steps, thermal=None, field=None):
+        """steps Heun steps; thermal(n) -> (N, 3) thermal field of step n (torch FP64) or None; field
+        (steps, 2, 3): the uniform field of each step's predictor and corrector pass, or None."""
         dt = self.dt
         for n in range(steps):
             b = 0.0 if thermal is None else thermal(n)
-            k1 = self.rhs(e, self.effective_field(th, e) + b)
+            B1 = None if field is None else field[n, 0]
+            B2 = None if field is None else field[n, 1]
+            k1 = self.rhs(e, self.effective_field(th, e, B1) + b)
             ep = self._keep_pinned(self.normalize(e + dt * k1), e)
-            k2 = self.rhs(ep, self.effective_field(th, ep) + b)
+            k2 = self.rhs(ep, self.effective_field(th, ep, B2) + b)
             e = self._keep_pinned(self.normalize(e + 0.5 * dt * (k1 + k2)), e)
         return e
 

@@ -381,6 +381,34 @@ def test_torch_heun_sot_matches_oracle():
     assert np.abs(R.run_heun(o, e0.copy(), 20, 0.3, 1e-3) - ref).max() > 1e-5
 
 
+def test_torch_heun_driven_field_matches_oracle():
+    """A time-dependent uniform field: the restatement, given B + field_func at t and t + dt per step,
+    follows magex_ref.run_heun with that field_func; its field gradient matches central differences."""
+    import torch
+    from oracle.torch_heun import TorchHeun
+    th = np.array([4.0, -0.4, 1.0, 0.2])
+    dt, steps = 1e-3, 20
+    ff = lambda t: np.array([3.0 * np.sin(40.0 * t), 0.0, 2.0 * np.cos(25.0 * t)])      # noqa: E731
+    T = TorchHeun(_square_builder(5), th, alpha=0.3, dt=dt)
+    o = _square_builder(5)(th)
+    e0 = o.init_random(9)
+    B = np.array([[o.B + ff(n * dt), o.B + ff(n * dt + dt)] for n in range(steps)])
+    ref = R.run_heun(o, e0.copy(), steps, 0.3, dt, field_func=ff)
+    got = T.run(torch.as_tensor(th), torch.as_tensor(e0), steps, field=torch.as_tensor(B)).numpy()
+    assert np.abs(got - ref).max() < 1e-13
+    tgt = torch.as_tensor(o.init_random(10))
+    Bt = torch.tensor(B, requires_grad=True)
+    loss = lambda b: ((T.run(torch.as_tensor(th), torch.as_tensor(e0), steps, field=b) - tgt) ** 2).mean()  # noqa
+    loss(Bt).backward()
+    h = 1e-5
+    with torch.no_grad():
+        for idx in [(0, 0, 0), (7, 1, 2), (19, 1, 1)]:
+            d = torch.zeros_like(Bt)
+            d[idx] = h
+            fd = (loss(Bt + d) - loss(Bt - d)) / (2 * h)
+            assert abs(fd - Bt.grad[idx]) <= 1e-6 * abs(fd) + 1e-13, idx
+
+
 def test_torch_heun_gradient_matches_central_differences():
     import torch
     from oracle.torch_heun import TorchHeun
