@@ -683,3 +683,33 @@ def test_adjoint_field_gradient_of_the_static_field(gpu):
             fd.append(float(((x - tgt) ** 2).mean()))
         d = (fd[0] - fd[1]) / (2 * h)
         assert abs(total[c] - d) <= 1e-6 * max(abs(d), 1e-12) + 1e-12, (c, total[c], d)
+
+
+def test_adjoint_entry_points_reject_bad_arguments(gpu):
+    """Host-side validation of the adjoint's entry points: null outputs and inputs, a backward before
+    any forward, and a backward after the parameters changed raise ValueError with the library's
+    message; nothing is launched for them."""
+    print("case: adjoint argument errors", flush=True)
+    import ctypes
+    from magex_amd._lib import check
+    sol = _solver(6, C=5)
+    L, h = sol.dm.lib, sol.h
+    g = torch.zeros((sol.N, 3), dtype=torch.float64, device="cuda")
+    out = torch.zeros(16, dtype=torch.float64, device="cuda")
+    null = ctypes.c_void_p(0)
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr())                                    # noqa: E731
+    with pytest.raises(ValueError, match="no forward recorded"):
+        check(L.mx_adjoint_backward(h, ptr(g), null, ptr(out), None))
+    with pytest.raises(ValueError, match="null field sequence"):
+        check(L.mx_adjoint_forward_driven(h, ptr(g), 3, None, 0, 0, None))
+    e = sol.forward(THETA0, sol.system.init_spins("random", seed=1), 4)
+    with pytest.raises(ValueError, match="null field-gradient output"):
+        check(L.mx_adjoint_backward_field(h, ptr(g), null, ptr(out), None, None))
+    with pytest.raises(ValueError, match="null argument"):
+        check(L.mx_adjoint_backward(h, None, null, ptr(out), None))
+    with pytest.raises(ValueError, match="negative step count"):
+        check(L.mx_adjoint_forward(h, ptr(e), -1, 0, 0, None))
+    sol.set_params(THETA0 + 0.5)                      # the recorded forward belongs to other parameters
+    with pytest.raises(ValueError, match="no forward recorded"):
+        sol.backward(g)
+    assert L.mx_adjoint_num_bonds(h) == 4 and L.mx_adjoint_num_bonds(None) == -1
