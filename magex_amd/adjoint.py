@@ -114,10 +114,11 @@ class DifferentiableLLG:
     theta0; couplings and on-site tensors must be linear in theta).  alpha, dt (ps), temperature (K) and
     seed as LLGSolver; the thermal field is a fixed realization (the library's Philox stream), so the
     gradient is that of one sample path.  precision "fp64" or "fp32"; layout as DeviceModel; sot as
-    LLGSolver's ({"p": direction, "B_DL": T, "B_FL": T}, magex/llg.py:83-86), held fixed."""
+    LLGSolver's ({"p": direction, "B_DL": T, "B_FL": T}, magex/llg.py:83-86), held fixed; integrator
+    "heun" or "depondt" (magex/llg.py:216-240)."""
 
     def __init__(self, build, theta0, alpha, dt, temperature=0.0, seed=0, precision="fp64", checkpoint_every=50,
-                 precession=True, layout=None, sot=None):
+                 precession=True, layout=None, sot=None, integrator="heun"):
         _torch()
         U, Aun, offset, system, boff, aoff = linear_units(build, theta0)
         self.P = len(theta0)
@@ -130,7 +131,7 @@ class DifferentiableLLG:
         self.N = system.N
         self.seed = int(seed)
         self.dm = DeviceModel(system, precision, layout=layout)
-        self.dm.set_dynamics(dt, alpha, temperature, precession)
+        self.dm.set_dynamics(dt, alpha, temperature, precession, integrator)
         self.dt = float(dt)
         self._nsteps = 0
         if sot:

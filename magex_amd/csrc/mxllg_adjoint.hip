@@ -71,6 +71,42 @@ __device__ __forceinline__ void unproject(R yx, R yy, R yz, R n, R& vx, R& vy, R
 // C^T M C v / 2 (M_kk = 2 Kc (q - q_k), M_kl = 4 Kc a_k a_l, a = C s, q_k = a_k^2) and the higher
 // uniaxial term's f'(c) (u . v) u / 2 (c = u . s, f' = -12 K2 c^2 - 30 K3 c^4); both Hessians are symmetric.
 // The spin-orbit field B_DL (e x p) adds -B_DL (p x v) mu mu_B / 2 (it enters B directly, not through g).
+// Depondt steps rotate: y = Rot(v) x = x cos th + A (v x x) + Bc v (v . x), th = |v|, A = sin th / th,
+// Bc = (1 - cos th) / th^2 (the identity below th = 1e-14, as the forward).  Rot(v)^T = Rot(-v); the
+// cotangent of v for the cotangent g of y is
+//   v [-(g.x) A + (g.(v x x)) A'/th + (g.v)(v.x) Bc'/th] + A (x x g) + Bc [(v.x) g + (g.v) x],
+// with the ratios from their series below th = 1e-2 (FP64) / 0.1 (FP32), where the closed forms cancel.
+template <typename R>
+__device__ __forceinline__ void rotate_vjp_v(R vx, R vy, R vz, R x, R y, R z, R gx, R gy, R gz, R& ox, R& oy,
+                                             R& oz) {
+  const R th2 = vx * vx + vy * vy + vz * vz, th = sqrt(th2);
+  if (!(th >= R(1e-14))) {
+    ox = oy = oz = R(0);
+    return;
+  }
+  R A, Bc, Ap, Bp;  // A, Bc and A'/th, Bc'/th
+  if (th < R(sizeof(R) == 8 ? 1e-2 : 1e-1)) {
+    const R t4 = th2 * th2;
+    A = R(1) - th2 / R(6) + t4 / R(120);
+    Bc = R(0.5) - th2 / R(24) + t4 / R(720);
+    Ap = R(-1) / R(3) + th2 / R(30) - t4 / R(840);
+    Bp = R(-1) / R(12) + th2 / R(180) - t4 / R(6720);
+  } else {
+    const R sn = sin(th), h = sin(R(0.5) * th), omc = R(2) * h * h, c = R(1) - omc;
+    A = sn / th;
+    Bc = omc / th2;
+    Ap = (th * c - sn) / (th2 * th);
+    Bp = (th * sn - R(2) * omc) / (th2 * th2);
+  }
+  const R gdx = gx * x + gy * y + gz * z, vdx = vx * x + vy * y + vz * z, gdv = gx * vx + gy * vy + gz * vz;
+  const R cx = vy * z - vz * y, cy = vz * x - vx * z, cz = vx * y - vy * x;  // v x x
+  const R gdc = gx * cx + gy * cy + gz * cz;
+  const R k = -gdx * A + gdc * Ap + gdv * vdx * Bp;
+  ox = k * vx + A * (y * gz - z * gy) + Bc * (vdx * gx + gdv * x);
+  oy = k * vy + A * (z * gx - x * gz) + Bc * (vdx * gy + gdv * y);
+  oz = k * vz + A * (x * gy - y * gx) + Bc * (vdx * gz + gdv * z);
+}
+
 template <typename R>
 __device__ __forceinline__ void onsite_T(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st, R sx,
                                          R sy, R sz, R x, R y, R z, R& ox, R& oy, R& oz) {
@@ -267,7 +303,7 @@ struct StepRec {
 template <typename R>
 __device__ __forceinline__ void adj_local(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st,
                                           const StepRec<R>& r, R lx, R ly, R lz, R* __restrict__ lamu,
-                                          R* __restrict__ gep, R* __restrict__ nu2) {
+                                          R* __restrict__ gep, R* __restrict__ nu2, R* __restrict__ gwb) {
   const int i = st.i;
   if (d.pinned && d.pinned[i]) {
     lamu[3 * i] = lx;
@@ -275,6 +311,48 @@ __device__ __forceinline__ void adj_local(const Static<R>* __restrict__ S, const
     lamu[3 * i + 2] = lz;
     gep[3 * i] = gep[3 * i + 1] = gep[3 * i + 2] = R(0);
     nu2[3 * i] = nu2[3 * i + 1] = nu2[3 * i + 2] = R(0);
+    if (d.integ == MX_DEPONDT) gwb[3 * i] = gwb[3 * i + 1] = gwb[3 * i + 2] = R(0);
+    return;
+  }
+  if (d.integ == MX_DEPONDT) {
+    // e' = N(Rot(v) e), v = dt/2 (w1 + w2), w2 = ep x f(ep, B2); the record's k1 slot holds w1.
+    // Writes lamu = Rot(v)^T dL/du (the direct part of dL/de), gwb = dL/dw1 from this stage
+    // (= dL/dw2 = dt/2 dL/dv), gep = dL/dep through w2 and nu2.
+    const R a = d.alpha ? d.alpha[i] : S->alpha_sub[st.s];
+    const R ex = r.e[3 * i], ey = r.e[3 * i + 1], ez = r.e[3 * i + 2];
+    const R px = r.ep[3 * i], py = r.ep[3 * i + 1], pz = r.ep[3 * i + 2];
+    const R bx = r.B2[3 * i], by = r.B2[3 * i + 1], bz = r.B2[3 * i + 2];
+    R fx, fy, fz, w2x, w2y, w2z;
+    llg_rhs(px, py, pz, bx, by, bz, a, d.prec, fx, fy, fz);
+    omega(px, py, pz, fx, fy, fz, w2x, w2y, w2z);
+    const R hdt = R(0.5) * d.dt;
+    const R vx = hdt * (r.w[3 * i] + w2x), vy = hdt * (r.w[3 * i + 1] + w2y), vz = hdt * (r.w[3 * i + 2] + w2z);
+    R ux = ex, uy = ey, uz = ez;
+    rotate(ux, uy, uz, vx, vy, vz);
+    const R n = sqrt(ux * ux + uy * uy + uz * uz);
+    unproject(ux / n, uy / n, uz / n, n, lx, ly, lz);  // dL/du
+    R tx = lx, ty = ly, tz = lz;
+    rotate(tx, ty, tz, -vx, -vy, -vz);
+    lamu[3 * i] = tx;
+    lamu[3 * i + 1] = ty;
+    lamu[3 * i + 2] = tz;
+    R gvx, gvy, gvz;
+    rotate_vjp_v(vx, vy, vz, ex, ey, ez, lx, ly, lz, gvx, gvy, gvz);
+    const R gwx = hdt * gvx, gwy = hdt * gvy, gwz = hdt * gvz;
+    gwb[3 * i] = gwx;
+    gwb[3 * i + 1] = gwy;
+    gwb[3 * i + 2] = gwz;
+    // w2 = ep x f2: dL/dep += f2 x gw, dL/df2 = gw x ep
+    const R lfx = gwy * pz - gwz * py, lfy = gwz * px - gwx * pz, lfz = gwx * py - gwy * px;
+    R gx, gy, gz, hx, hy, hz;
+    rhs_vjp(px, py, pz, bx, by, bz, a, d.prec, lfx, lfy, lfz, gx, gy, gz, hx, hy, hz);
+    gep[3 * i] = (fy * gwz - fz * gwy) + gx;
+    gep[3 * i + 1] = (fz * gwx - fx * gwz) + gy;
+    gep[3 * i + 2] = (fx * gwy - fy * gwx) + gz;
+    const R im2 = R(2) * (d.inv_mu ? d.inv_mu[i] : S->inv_mu_sub[st.s]);
+    nu2[3 * i] = im2 * hx;
+    nu2[3 * i + 1] = im2 * hy;
+    nu2[3 * i + 2] = im2 * hz;
     return;
   }
   const R a = d.alpha ? d.alpha[i] : S->alpha_sub[st.s];
@@ -304,11 +382,11 @@ __device__ __forceinline__ void adj_local(const Static<R>* __restrict__ S, const
 template <typename R, int GEO>
 __global__ __launch_bounds__(BLOCK) void k_adj_a(const Static<R>* __restrict__ S, const Dyn<R> d,
                                                  const R* __restrict__ lam, const StepRec<R> r, R* __restrict__ lamu,
-                                                 R* __restrict__ gep, R* __restrict__ nu2) {
+                                                 R* __restrict__ gep, R* __restrict__ nu2, R* __restrict__ gwb) {
   Site st;
   if (!site_of<R, GEO>(S, st)) return;
   const int i = st.i;
-  adj_local(S, d, st, r, lam[3 * i], lam[3 * i + 1], lam[3 * i + 2], lamu, gep, nu2);
+  adj_local(S, d, st, r, lam[3 * i], lam[3 * i + 1], lam[3 * i + 2], lamu, gep, nu2, gwb);
 }
 
 // Adds the workgroup's sums of gp to its row of gpart (stride MAXP); old is that row's entry for this
@@ -343,7 +421,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S
                                                  const R* __restrict__ gep, const R* __restrict__ nu2,
                                                  R* __restrict__ nu1, R* __restrict__ part, double* __restrict__ gpart,
                                                  double* __restrict__ gb, double* __restrict__ ga,
-                                                 double* __restrict__ gf) {
+                                                 double* __restrict__ gf, const R* __restrict__ gwb) {
   const double old = partial_row<NP>(gpart);
   double gp[NP];
 #pragma unroll
@@ -372,6 +450,29 @@ __global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S
       part[3 * i] = lamu[3 * i] + lx;
       part[3 * i + 1] = lamu[3 * i + 1] + ly;
       part[3 * i + 2] = lamu[3 * i + 2] + lz;
+    } else if (d.integ == MX_DEPONDT) {
+      // ep = Rot(dt w1) e, w1 = e x f(e, B1): dL/de = lamu + Rot^T dL/dep + f1 x gw1 + rhs part,
+      // gw1 = dt dL/dv1 + the corrector's share (gwb)
+      const R ex = r.e[3 * i], ey = r.e[3 * i + 1], ez = r.e[3 * i + 2];
+      const R bx = r.B1[3 * i], by = r.B1[3 * i + 1], bz = r.B1[3 * i + 2];
+      R fx, fy, fz;
+      llg_rhs(ex, ey, ez, bx, by, bz, a, d.prec, fx, fy, fz);
+      const R vx = d.dt * r.w[3 * i], vy = d.dt * r.w[3 * i + 1], vz = d.dt * r.w[3 * i + 2];
+      R tx = lx, ty = ly, tz = lz;
+      rotate(tx, ty, tz, -vx, -vy, -vz);
+      R gvx, gvy, gvz;
+      rotate_vjp_v(vx, vy, vz, ex, ey, ez, lx, ly, lz, gvx, gvy, gvz);
+      const R gwx = d.dt * gvx + gwb[3 * i], gwy = d.dt * gvy + gwb[3 * i + 1], gwz = d.dt * gvz + gwb[3 * i + 2];
+      const R lfx = gwy * ez - gwz * ey, lfy = gwz * ex - gwx * ez, lfz = gwx * ey - gwy * ex;  // gw1 x e
+      R gx, gy, gz, qx, qy, qz;
+      rhs_vjp(ex, ey, ez, bx, by, bz, a, d.prec, lfx, lfy, lfz, gx, gy, gz, qx, qy, qz);
+      const R im2 = R(2) * (d.inv_mu ? d.inv_mu[i] : S->inv_mu_sub[st.s]);
+      nu1[3 * i] = im2 * qx;
+      nu1[3 * i + 1] = im2 * qy;
+      nu1[3 * i + 2] = im2 * qz;
+      part[3 * i] = lamu[3 * i] + tx + (fy * gwz - fz * gwy) + gx;
+      part[3 * i + 1] = lamu[3 * i + 1] + ty + (fz * gwx - fx * gwz) + gy;
+      part[3 * i + 2] = lamu[3 * i + 2] + tz + (fx * gwy - fy * gwx) + gz;
     } else {
       const R ex = r.e[3 * i], ey = r.e[3 * i + 1], ez = r.e[3 * i + 2];
       const R bx = r.B1[3 * i], by = r.B1[3 * i + 1], bz = r.B1[3 * i + 2];
@@ -408,7 +509,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_c(const Static<R>* __restrict__ S
                                                  const StepRec<R> prev, R* __restrict__ lamu, R* __restrict__ gep,
                                                  R* __restrict__ nu2, double* __restrict__ gpart,
                                                  double* __restrict__ gb, double* __restrict__ ga,
-                                                 double* __restrict__ gf) {
+                                                 double* __restrict__ gf, R* __restrict__ gwb) {
   const double old = partial_row<NP>(gpart);
   double gp[NP];
 #pragma unroll
@@ -430,7 +531,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_c(const Static<R>* __restrict__ S
     onsite_T(S, d, st, e[3 * i], e[3 * i + 1], e[3 * i + 2], vx, vy, vz, ox, oy, oz);
     const R lx = part[3 * i] + hx - ox, ly = part[3 * i + 1] + hy - oy, lz = part[3 * i + 2] + hz - oz;
     if constexpr (FUSE) {
-      adj_local(S, d, st, prev, lx, ly, lz, lamu, gep, nu2);
+      adj_local(S, d, st, prev, lx, ly, lz, lamu, gep, nu2, gwb);
     } else {
       lam[3 * i] = lx;
       lam[3 * i + 1] = ly;
@@ -503,6 +604,7 @@ struct mx_adjoint {
   int seg_cap = 0;        // steps the records hold
   int64_t seg_k = -1;     // segment whose records seg_d holds (-1: none)
   void *lam_d = nullptr, *lamu_d = nullptr, *gep_d = nullptr, *nu1_d = nullptr, *nu2_d = nullptr, *part_d = nullptr;
+  void* gw_d = nullptr;  // Depondt: the corrector's share of dL/dw1, N x 3
   void* ework_d = nullptr;
   double* gpart_d = nullptr;  // per-workgroup parameter-gradient partials, nblk x MAXP
   double* gout_d = nullptr;   // their fold, MAXP
@@ -683,7 +785,7 @@ void record_segment(mx_adjoint& a, const void* src, int64_t k, int n_seg, hipStr
 void ensure_buffers(mx_adjoint& a) {
   const Model& m = *a.m;
   const size_t b = nbytes3(m);
-  for (void** p : {&a.lam_d, &a.lamu_d, &a.gep_d, &a.nu1_d, &a.nu2_d, &a.part_d, &a.ework_d})
+  for (void** p : {&a.lam_d, &a.lamu_d, &a.gep_d, &a.nu1_d, &a.nu2_d, &a.part_d, &a.ework_d, &a.gw_d})
     if (!*p) aok(hipMalloc(p, b), "hipMalloc adjoint state");
   a.nblk = m.ns * ((m.ncell + BLOCK - 1) / BLOCK);
   if (!a.gpart_d) aok(hipMalloc(reinterpret_cast<void**>(&a.gpart_d), sizeof(double) * MAXP * a.nblk), "hipMalloc gpart");
@@ -707,6 +809,7 @@ void sweep_t(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
   d.bond = static_cast<const R*>(m.bond_d);
   d.A = static_cast<const R*>(m.A_d);
   d.pinned = static_cast<const uint8_t*>(m.pinned_d);
+  d.integ = m.integrator;
   d.Kc = static_cast<const R*>(m.Kc_d);
   d.K2 = static_cast<const R*>(m.K2_d);
   d.K3 = static_cast<const R*>(m.K3_d);
@@ -722,21 +825,22 @@ void sweep_t(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
   R* nu1 = static_cast<R*>(a.nu1_d);
   R* nu2 = static_cast<R*>(a.nu2_d);
   R* part = static_cast<R*>(a.part_d);
+  R* gwb = static_cast<R*>(a.gw_d);
   const dim3 g(a.nblk), b(BLOCK);
   // field-gradient rows: (global step * 2 + pass index) x nblk x 3, pass index 0 = predictor, 1 = corrector
   auto frow = [&](int n, int pass) -> double* {
     return a.want_field ? a.gf_d + (size_t(a.sweep_step0 + n) * 2 + pass) * a.nblk * 3 : nullptr;
   };
-  hipLaunchKernelGGL((k_adj_a<R, GEO>), g, b, 0, s, S, d, lam, rec[n_seg - 1], lamu, gep, nu2);
+  hipLaunchKernelGGL((k_adj_a<R, GEO>), g, b, 0, s, S, d, lam, rec[n_seg - 1], lamu, gep, nu2, gwb);
   for (int n = n_seg - 1; n >= 0; --n) {
     hipLaunchKernelGGL((k_adj_b<R, GEO, NP, DENSE, CM>), g, b, 0, s, S, A, d, rec[n], lamu, gep, nu2, nu1, part, a.gpart_d,
-                       a.gb_d, a.ga_d, frow(n, 1));
+                       a.gb_d, a.ga_d, frow(n, 1), gwb);
     if (n > 0)
       hipLaunchKernelGGL((k_adj_c<R, GEO, NP, true, DENSE, CM>), g, b, 0, s, S, A, d, rec[n].e, nu1, part, lam, rec[n - 1],
-                         lamu, gep, nu2, a.gpart_d, a.gb_d, a.ga_d, frow(n, 0));
+                         lamu, gep, nu2, a.gpart_d, a.gb_d, a.ga_d, frow(n, 0), gwb);
     else
       hipLaunchKernelGGL((k_adj_c<R, GEO, NP, false, DENSE, CM>), g, b, 0, s, S, A, d, rec[n].e, nu1, part, lam, rec[n],
-                         lamu, gep, nu2, a.gpart_d, a.gb_d, a.ga_d, frow(n, 0));
+                         lamu, gep, nu2, a.gpart_d, a.gb_d, a.ga_d, frow(n, 0), gwb);
   }
 }
 
@@ -927,7 +1031,7 @@ void mx_adjoint_destroy(mx_adjoint* a) {
                    &a->part_d, &a->ework_d, reinterpret_cast<void**>(&a->gpart_d),
                    reinterpret_cast<void**>(&a->gout_d), reinterpret_cast<void**>(&a->gb_d),
                    reinterpret_cast<void**>(&a->ga_d), reinterpret_cast<void**>(&a->cmap_d),
-                   reinterpret_cast<void**>(&a->gf_d)})
+                   reinterpret_cast<void**>(&a->gf_d), &a->gw_d})
     afree(*p);
   delete a;
 }
@@ -954,7 +1058,6 @@ int forward_impl(mx_adjoint* a, void* e, int64_t nsteps, const double* B, uint64
   else a->drive.clear();
   Model& m = *a->m;
   if (!m.dyn_set) afail(MX_EINVAL, "set the dynamics first");
-  if (m.integrator != MX_HEUN) afail(MX_EUNSUPPORTED, "the adjoint differentiates the Heun integrator");
   aok(hipSetDevice(m.device), "hipSetDevice");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int64_t nck = nsteps / a->C + 1;

@@ -30,7 +30,7 @@ def _coo(M):
 class TorchHeun:
     """build(theta) -> magex_ref.SpinSystem, linear in theta; units by differences at theta_ref."""
 
-    def __init__(self, build, theta_ref, alpha, dt, precession=True, sot=None):
+    def __init__(self, build, theta_ref, alpha, dt, precession=True, sot=None, integrator="heun"):
         theta_ref = np.asarray(theta_ref, float)
         base = build(theta_ref)
         self.N = base.N
@@ -48,6 +48,7 @@ class TorchHeun:
         self.mu = torch.as_tensor(base.mu)
         self.B = torch.as_tensor(base.B)
         self.alpha, self.dt, self.precession = float(alpha), float(dt), precession
+        self.integrator = integrator
         self.base = base
         # spin-orbit torque fields B_DL (e x p) + B_FL p (magex/llg.py:126-128), sot = (p unit, B_DL, B_FL)
         self.sot = None if sot is None else (torch.as_tensor(np.asarray(sot[0], float)), float(sot[1]), float(sot[2]))
@@ -110,18 +111,36 @@ class TorchHeun:
     def _keep_pinned(self, new, old):
         return new if self.pin is None else torch.where(self.pin, old, new)
 
+    @staticmethod
+    def rotate(e, w):
+        """Rodrigues rotation of e by the rotation vectors w (magex_ref.rotate, llg.py:31-36)."""
+        th = torch.linalg.norm(w, dim=1, keepdim=True)
+        big = th > 1e-14
+        k = torch.where(big, w / torch.where(big, th, torch.ones_like(th)), torch.zeros_like(w))
+        c, s = torch.cos(th), torch.sin(th)
+        return e * c + torch.cross(k, e, dim=1) * s + k * (k * e).sum(1, keepdim=True) * (1 - c)
+
+    def _step(self, e, field1, field2, dt):
+        """One Heun or Depondt step (magex_ref.heun_step / depondt_step) from the two pass fields."""
+        k1 = self.rhs(e, field1(e))
+        if self.integrator == "depondt":
+            w1 = torch.cross(e, k1, dim=1)
+            ep = self._keep_pinned(self.rotate(e, dt * w1), e)
+            w2 = torch.cross(ep, self.rhs(ep, field2(ep)), dim=1)
+            return self._keep_pinned(self.normalize(self.rotate(e, 0.5 * dt * (w1 + w2))), e)
+        ep = self._keep_pinned(self.normalize(e + dt * k1), e)
+        k2 = self.rhs(ep, field2(ep))
+        return self._keep_pinned(self.normalize(e + 0.5 * dt * (k1 + k2)), e)
+
     def run(self, th, e, steps, thermal=None, field=None):
         """steps Heun steps; thermal(n) -> (N, 3) thermal field of step n (torch FP64) or None; field
         (steps, 2, 3): the uniform field of each step's predictor and corrector pass, or None."""
-        dt = self.dt
         for n in range(steps):
             b = 0.0 if thermal is None else thermal(n)
             B1 = None if field is None else field[n, 0]
             B2 = None if field is None else field[n, 1]
-            k1 = self.rhs(e, self.effective_field(th, e, B1) + b)
-            ep = self._keep_pinned(self.normalize(e + dt * k1), e)
-            k2 = self.rhs(ep, self.effective_field(th, ep, B2) + b)
-            e = self._keep_pinned(self.normalize(e + 0.5 * dt * (k1 + k2)), e)
+            e = self._step(e, lambda x: self.effective_field(th, x, B1) + b,
+                           lambda x: self.effective_field(th, x, B2) + b, self.dt)
         return e
 
     def dense(self, th):
@@ -140,11 +159,8 @@ class TorchHeun:
 
     def run_dense(self, J, A, e, steps, thermal=None):
         """run() with the couplings given as the dense matrix J and on-site tensors A."""
-        dt = self.dt
         for n in range(steps):
             b = 0.0 if thermal is None else thermal(n)
-            k1 = self.rhs(e, self.effective_field_dense(J, A, e) + b)
-            ep = self._keep_pinned(self.normalize(e + dt * k1), e)
-            k2 = self.rhs(ep, self.effective_field_dense(J, A, ep) + b)
-            e = self._keep_pinned(self.normalize(e + 0.5 * dt * (k1 + k2)), e)
+            f = lambda x: self.effective_field_dense(J, A, x) + b      # noqa: E731
+            e = self._step(e, f, f, self.dt)
         return e

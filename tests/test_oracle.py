@@ -409,6 +409,44 @@ def test_torch_heun_driven_field_matches_oracle():
             assert abs(fd - Bt.grad[idx]) <= 1e-6 * abs(fd) + 1e-13, idx
 
 
+def test_torch_depondt_matches_oracle():
+    """The Depondt form of the restatement: its trajectory is magex_ref.run_heun(integrator="depondt")'s
+    (with a thermal field too), its gradients match central differences, and it differs from Heun's."""
+    import torch
+    from oracle.torch_heun import TorchHeun
+    th = np.array([4.0, -0.4, 1.0, 0.2])
+    T = TorchHeun(_square_builder(5), th, alpha=0.3, dt=1e-3, integrator="depondt")
+    o = _square_builder(5)(th)
+    e0 = o.init_random(11)
+    ref = R.run_heun(o, e0.copy(), 20, 0.3, 1e-3, integrator="depondt")
+    got = T.run(torch.as_tensor(th), torch.as_tensor(e0), 20).numpy()
+    assert np.abs(got - ref).max() < 1e-13
+    assert np.abs(R.run_heun(o, e0.copy(), 20, 0.3, 1e-3) - ref).max() > 1e-8
+    rng = np.random.default_rng(2)
+    nz = [rng.normal(size=(o.N, 3)) for _ in range(10)]
+    sg = R.sigma(o, 0.3, 30.0, 1e-3)
+    ref3 = R.run_heun(o, e0.copy(), 10, 0.3, 1e-3, T=30.0, normals=lambda n: nz[n], integrator="depondt")
+    got3 = T.run(torch.as_tensor(th), torch.as_tensor(e0), 10,
+                 thermal=lambda n: torch.as_tensor(sg[:, None] * nz[n])).numpy()
+    assert np.abs(got3 - ref3).max() < 1e-13
+    tgt = torch.as_tensor(o.init_random(12))
+    t = torch.tensor(th, requires_grad=True)
+    e = torch.tensor(e0, requires_grad=True)
+    loss = lambda a, b: ((T.run(a, b, 20) - tgt) ** 2).mean()                        # noqa: E731
+    loss(t, e).backward()
+    h = 1e-5
+    with torch.no_grad():
+        for p in range(4):
+            d = torch.zeros(4, dtype=torch.float64)
+            d[p] = h
+            fd = (loss(t + d, e) - loss(t - d, e)) / (2 * h)
+            assert abs(fd - t.grad[p]) <= 1e-6 * abs(fd) + 1e-12, p
+        d = torch.zeros_like(e)
+        d[6, 2] = h
+        fd = (loss(t, e + d) - loss(t, e - d)) / (2 * h)
+        assert abs(fd - e.grad[6, 2]) <= 1e-6 * abs(fd) + 1e-12
+
+
 def test_torch_heun_gradient_matches_central_differences():
     import torch
     from oracle.torch_heun import TorchHeun
