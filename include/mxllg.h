@@ -229,8 +229,8 @@ int mx_batch_run(mx_batch* b, void* e, int64_t nsteps, const uint64_t* seeds, ui
 int mx_batch_observables(mx_batch* b, const void* e, double* out, void* stream);
 
 /* ---------------------------------------------------------------------------------------------
- * Differentiable form: reverse-mode adjoint of a deterministic-noise Heun trajectory with respect to
- * the initial spins and to P <= 16 parameters theta of a linearly parameterized Hamiltonian,
+ * Differentiable form: reverse-mode adjoint of a deterministic-noise Heun (or Depondt) trajectory with
+ * respect to the initial spins and to P <= 16 parameters theta of a linearly parameterized Hamiltonian,
  * J_k = sum_p theta_p U[p][k] on every stencil entry k and A_s = sum_p theta_p Aun[p][s] on every
  * sublattice (shells, DMI and anisotropy constants).  No counterpart in magex, whose solvers are not
  * differentiable (magex/llg.py:87-120 is the forward it differentiates).

@@ -1,5 +1,5 @@
-"""Differentiable LLG: gradients of a Heun trajectory with respect to the initial spins and to the
-constants of the Hamiltonian, through the library's hand-written adjoint.
+"""Differentiable LLG: gradients of a Heun or Depondt trajectory with respect to the initial spins and
+to the constants of the Hamiltonian, through the library's hand-written adjoint.
 
 No counterpart in magex: its solvers (magex/llg.py:87-120) only run forward.  Here a Hamiltonian whose
 couplings and on-site tensors are linear in a few constants theta (exchange shells, DMI, anisotropy)
@@ -108,7 +108,7 @@ def linear_units(build, theta_ref, tol=1e-12):
 
 
 class DifferentiableLLG:
-    """Deterministic-noise Heun trajectories of a parameterized SpinSystem with adjoint gradients.
+    """Deterministic-noise Heun (or Depondt) trajectories of a parameterized SpinSystem with adjoint gradients.
 
     build(theta) -> SpinSystem defines the Hamiltonian (its field, moments and stencil are taken at
     theta0; couplings and on-site tensors must be linear in theta).  alpha, dt (ps), temperature (K) and

@@ -1,5 +1,6 @@
-// Differentiable form: reverse-mode adjoint of the stochastic Heun trajectory with respect to the
-// spins and to the parameters of a linearly parameterized Hamiltonian.  No counterpart in magex.
+// Differentiable form: reverse-mode adjoint of the stochastic Heun trajectory (and of the Depondt one,
+// whose rotations are differentiated in rotate_vjp_v) with respect to the spins and to the parameters
+// of a linearly parameterized Hamiltonian.  No counterpart in magex.
 //
 // Model: couplings J_k = sum_p theta_p U[p][k] on the stencil entries k, on-site A_s = sum_p theta_p
 // Aun[p][s]; B_i = 2 (h_i - A_i e_i) / (mu_i mu_B) + B_ext + b_i with h_i = sum_k J_k e_j(k).
