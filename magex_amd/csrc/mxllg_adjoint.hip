@@ -108,7 +108,9 @@ __device__ __forceinline__ void rotate_vjp_v(R vx, R vy, R vz, R x, R y, R z, R 
   oz = k * vz + A * (x * gy - y * gx) + Bc * (vdx * gz + gdv * z);
 }
 
-template <typename R>
+// LEAN (here and in the kernels below): compiled for the plain model -- Heun, nothing pinned, no cubic,
+// higher-uniaxial or spin-orbit terms (on-site tensors per sublattice or per site).
+template <typename R, bool LEAN = false>
 __device__ __forceinline__ void onsite_T(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st, R sx,
                                          R sy, R sz, R x, R y, R z, R& ox, R& oy, R& oz) {
   const int amode = S->amode;
@@ -135,6 +137,7 @@ __device__ __forceinline__ void onsite_T(const Static<R>* __restrict__ S, const 
       oz = a[2] * x + a[5] * y + a[8] * z;
     }
   }
+  if (LEAN) return;
   if (S->has_c) {
     const R* C = S->C;
     const R a1 = C[0] * sx + C[1] * sy + C[2] * sz, a2 = C[3] * sx + C[4] * sy + C[5] * sz,
@@ -301,12 +304,12 @@ struct StepRec {
 // Step-local part (corrector) at site i: from lambda = dL/de' and the step's record: lambda_u, the
 // cotangent of ep through k2 (gep) and nu2.  A pinned spin is carried unchanged (e' = ep = e), so
 // lambda passes through and no field acts on it (gep = nu2 = 0).
-template <typename R>
+template <typename R, bool LEAN = false>
 __device__ __forceinline__ void adj_local(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st,
                                           const StepRec<R>& r, R lx, R ly, R lz, R* __restrict__ lamu,
                                           R* __restrict__ gep, R* __restrict__ nu2, R* __restrict__ gwb) {
   const int i = st.i;
-  if (d.pinned && d.pinned[i]) {
+  if (!LEAN && d.pinned && d.pinned[i]) {
     lamu[3 * i] = lx;
     lamu[3 * i + 1] = ly;
     lamu[3 * i + 2] = lz;
@@ -315,7 +318,7 @@ __device__ __forceinline__ void adj_local(const Static<R>* __restrict__ S, const
     if (d.integ == MX_DEPONDT) gwb[3 * i] = gwb[3 * i + 1] = gwb[3 * i + 2] = R(0);
     return;
   }
-  if (d.integ == MX_DEPONDT) {
+  if (!LEAN && d.integ == MX_DEPONDT) {
     // e' = N(Rot(v) e), v = dt/2 (w1 + w2), w2 = ep x f(ep, B2); the record's k1 slot holds w1.
     // Writes lamu = Rot(v)^T dL/du (the direct part of dL/de), gwb = dL/dw1 from this stage
     // (= dL/dw2 = dt/2 dL/dv), gep = dL/dep through w2 and nu2.
@@ -380,14 +383,14 @@ __device__ __forceinline__ void adj_local(const Static<R>* __restrict__ S, const
   nu2[3 * i + 2] = im2 * hz;
 }
 
-template <typename R, int GEO>
+template <typename R, int GEO, bool LEAN>
 __global__ __launch_bounds__(BLOCK) void k_adj_a(const Static<R>* __restrict__ S, const Dyn<R> d,
                                                  const R* __restrict__ lam, const StepRec<R> r, R* __restrict__ lamu,
                                                  R* __restrict__ gep, R* __restrict__ nu2, R* __restrict__ gwb) {
   Site st;
   if (!site_of<R, GEO>(S, st)) return;
   const int i = st.i;
-  adj_local(S, d, st, r, lam[3 * i], lam[3 * i + 1], lam[3 * i + 2], lamu, gep, nu2, gwb);
+  adj_local<R, LEAN>(S, d, st, r, lam[3 * i], lam[3 * i + 1], lam[3 * i + 2], lamu, gep, nu2, gwb);
 }
 
 // Adds the workgroup's sums of gp to its row of gpart (stride MAXP); old is that row's entry for this
@@ -416,7 +419,7 @@ __device__ __forceinline__ void store_field_partials(double (&fv)[3], double* __
 }
 
 // Gather nu2 (dL/dep) and the predictor's adjoint: writes nu1 and the partial dL/de (part).
-template <typename R, int GEO, int NP, bool DENSE, int CM>
+template <typename R, int GEO, int NP, bool DENSE, int CM, bool LEAN>
 __global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
                                                  const Dyn<R> d, const StepRec<R> r, const R* __restrict__ lamu,
                                                  const R* __restrict__ gep, const R* __restrict__ nu2,
@@ -442,16 +445,16 @@ __global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S
     }
     R hx, hy, hz, ox, oy, oz;
     adj_gather<R, GEO, NP, DENSE, CM>(S, A, d, nu2, r.ep, st, vx2, vy2, vz2, px, py, pz, hx, hy, hz, gp, gb, ga);
-    onsite_T(S, d, st, px, py, pz, vx2, vy2, vz2, ox, oy, oz);
+    onsite_T<R, LEAN>(S, d, st, px, py, pz, vx2, vy2, vz2, ox, oy, oz);
     R lx = gep[3 * i] + hx - ox, ly = gep[3 * i + 1] + hy - oy, lz = gep[3 * i + 2] + hz - oz;  // dL/dep
-    if (d.pinned && d.pinned[i]) {
+    if (!LEAN && d.pinned && d.pinned[i]) {
       // ep = e' = e: dL/de gathers lambda (in lamu) and dL/dep; the spin feels no field (nu1 = 0).
       // No early return: every thread reaches the workgroup reduction below.
       nu1[3 * i] = nu1[3 * i + 1] = nu1[3 * i + 2] = R(0);
       part[3 * i] = lamu[3 * i] + lx;
       part[3 * i + 1] = lamu[3 * i + 1] + ly;
       part[3 * i + 2] = lamu[3 * i + 2] + lz;
-    } else if (d.integ == MX_DEPONDT) {
+    } else if (!LEAN && d.integ == MX_DEPONDT) {
       // ep = Rot(dt w1) e, w1 = e x f(e, B1): dL/de = lamu + Rot^T dL/dep + f1 x gw1 + rhs part,
       // gw1 = dt dL/dv1 + the corrector's share (gwb)
       const R ex = r.e[3 * i], ey = r.e[3 * i + 1], ez = r.e[3 * i + 2];
@@ -503,7 +506,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S
 // Gather nu1: lambda = dL/de of the step's input, and the parameter terms at e.  With FUSE the
 // previous step's local part (adj_local on its record prev) follows at once from lambda, which then
 // never goes through memory; otherwise lambda is stored for a later k_adj_a.
-template <typename R, int GEO, int NP, bool FUSE, bool DENSE, int CM>
+template <typename R, int GEO, int NP, bool FUSE, bool DENSE, int CM, bool LEAN>
 __global__ __launch_bounds__(BLOCK) void k_adj_c(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
                                                  const Dyn<R> d, const R* __restrict__ e, const R* __restrict__ nu1,
                                                  const R* __restrict__ part, R* __restrict__ lam,
@@ -529,10 +532,10 @@ __global__ __launch_bounds__(BLOCK) void k_adj_c(const Static<R>* __restrict__ S
     R hx, hy, hz, ox, oy, oz;
     adj_gather<R, GEO, NP, DENSE, CM>(S, A, d, nu1, e, st, vx, vy, vz, e[3 * i], e[3 * i + 1], e[3 * i + 2], hx, hy,
                                       hz, gp, gb, ga);
-    onsite_T(S, d, st, e[3 * i], e[3 * i + 1], e[3 * i + 2], vx, vy, vz, ox, oy, oz);
+    onsite_T<R, LEAN>(S, d, st, e[3 * i], e[3 * i + 1], e[3 * i + 2], vx, vy, vz, ox, oy, oz);
     const R lx = part[3 * i] + hx - ox, ly = part[3 * i + 1] + hy - oy, lz = part[3 * i + 2] + hz - oz;
     if constexpr (FUSE) {
-      adj_local(S, d, st, prev, lx, ly, lz, lamu, gep, nu2, gwb);
+      adj_local<R, LEAN>(S, d, st, prev, lx, ly, lz, lamu, gep, nu2, gwb);
     } else {
       lam[3 * i] = lx;
       lam[3 * i + 1] = ly;
@@ -797,7 +800,7 @@ void ensure_buffers(mx_adjoint& a) {
 // Adjoint sweep over one replayed segment of n_seg steps (records rec(n), newest last): the local
 // part of the segment's last step from lam, then per step (newest first) k_adj_b and k_adj_c, the
 // latter fused with the next-older step's local part.  Leaves dL/de of the segment's first input in lam.
-template <typename R, int GEO, int NP, bool DENSE, int CM>
+template <typename R, int GEO, int NP, bool DENSE, int CM, bool LEAN>
 void sweep_t(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
   const Model& m = *a.m;
   const Static<R>* S = static_cast<const Static<R>*>(m.static_d);
@@ -832,31 +835,44 @@ void sweep_t(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
   auto frow = [&](int n, int pass) -> double* {
     return a.want_field ? a.gf_d + (size_t(a.sweep_step0 + n) * 2 + pass) * a.nblk * 3 : nullptr;
   };
-  hipLaunchKernelGGL((k_adj_a<R, GEO>), g, b, 0, s, S, d, lam, rec[n_seg - 1], lamu, gep, nu2, gwb);
+  hipLaunchKernelGGL((k_adj_a<R, GEO, LEAN>), g, b, 0, s, S, d, lam, rec[n_seg - 1], lamu, gep, nu2, gwb);
   for (int n = n_seg - 1; n >= 0; --n) {
-    hipLaunchKernelGGL((k_adj_b<R, GEO, NP, DENSE, CM>), g, b, 0, s, S, A, d, rec[n], lamu, gep, nu2, nu1, part, a.gpart_d,
+    hipLaunchKernelGGL((k_adj_b<R, GEO, NP, DENSE, CM, LEAN>), g, b, 0, s, S, A, d, rec[n], lamu, gep, nu2, nu1, part, a.gpart_d,
                        a.gb_d, a.ga_d, frow(n, 1), gwb);
     if (n > 0)
-      hipLaunchKernelGGL((k_adj_c<R, GEO, NP, true, DENSE, CM>), g, b, 0, s, S, A, d, rec[n].e, nu1, part, lam, rec[n - 1],
+      hipLaunchKernelGGL((k_adj_c<R, GEO, NP, true, DENSE, CM, LEAN>), g, b, 0, s, S, A, d, rec[n].e, nu1, part, lam, rec[n - 1],
                          lamu, gep, nu2, a.gpart_d, a.gb_d, a.ga_d, frow(n, 0), gwb);
     else
-      hipLaunchKernelGGL((k_adj_c<R, GEO, NP, false, DENSE, CM>), g, b, 0, s, S, A, d, rec[n].e, nu1, part, lam, rec[n],
+      hipLaunchKernelGGL((k_adj_c<R, GEO, NP, false, DENSE, CM, LEAN>), g, b, 0, s, S, A, d, rec[n].e, nu1, part, lam, rec[n],
                          lamu, gep, nu2, a.gpart_d, a.gb_d, a.ga_d, frow(n, 0), gwb);
   }
 }
 
-template <typename R, int NP, bool DENSE, int CM>
+template <typename R, int NP, bool DENSE, int CM, bool LEAN>
 void sweep_geo(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
   const int geo = a.m->geo;
-  if (geo == GEO_OPEN) sweep_t<R, GEO_OPEN, NP, DENSE, CM>(a, rec, n_seg, s);
-  else if (geo == GEO_PERIODIC) sweep_t<R, GEO_PERIODIC, NP, DENSE, CM>(a, rec, n_seg, s);
-  else sweep_t<R, GEO_TILED, NP, DENSE, CM>(a, rec, n_seg, s);
+  if (geo == GEO_OPEN) sweep_t<R, GEO_OPEN, NP, DENSE, CM, LEAN>(a, rec, n_seg, s);
+  else if (geo == GEO_PERIODIC) sweep_t<R, GEO_PERIODIC, NP, DENSE, CM, LEAN>(a, rec, n_seg, s);
+  else sweep_t<R, GEO_TILED, NP, DENSE, CM, LEAN>(a, rec, n_seg, s);
+}
+
+// The plain model (see LEAN above); the dense sweeps are compiled in the general form only.
+bool lean_adjoint(const Model& m) {
+  return m.integrator == MX_HEUN && !m.pinned_d && !m.has_c && !m.has_u && m.sot_dl == 0.0 && m.sot_fl == 0.0;
 }
 
 template <typename R, int NP, bool DENSE>
 void sweep_cm(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
-  if (a.m->coup == COUP_BOND) sweep_geo<R, NP, DENSE, COUP_BOND>(a, rec, n_seg, s);
-  else sweep_geo<R, NP, DENSE, COUP_TABLE9>(a, rec, n_seg, s);
+  const bool bond = a.m->coup == COUP_BOND;
+  if constexpr (!DENSE) {
+    if (lean_adjoint(*a.m)) {
+      if (bond) sweep_geo<R, NP, DENSE, COUP_BOND, true>(a, rec, n_seg, s);
+      else sweep_geo<R, NP, DENSE, COUP_TABLE9, true>(a, rec, n_seg, s);
+      return;
+    }
+  }
+  if (bond) sweep_geo<R, NP, DENSE, COUP_BOND, false>(a, rec, n_seg, s);
+  else sweep_geo<R, NP, DENSE, COUP_TABLE9, false>(a, rec, n_seg, s);
 }
 
 template <typename R, bool DENSE>
