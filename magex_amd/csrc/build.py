@@ -14,6 +14,8 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
@@ -29,24 +31,48 @@ def hipcc():
     return os.path.join(rocm, "bin", "hipcc")
 
 
-def command(out=TARGET):
-    return [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-ffp-contract=on",
-            "-Wall", "-Wno-unused-function", "-I", INCLUDE, "-I", HERE,
-            *[os.path.join(HERE, s) for s in SOURCES], "-o", out]
+FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-ffp-contract=on",
+         "-Wall", "-Wno-unused-function", "-I", INCLUDE, "-I", HERE]
 
 
-def build(verbose=False):
-    """Compile every source into libmxllg.so (always recompiles); returns the library path."""
-    tmp = TARGET + ".tmp"
-    cmd = command(tmp)
-    if verbose:
-        print(" ".join(cmd), flush=True)
+def compile_command(src, obj):
+    """One translation unit to an object file (device code is whole-program per unit: no -fgpu-rdc)."""
+    return [hipcc(), *FLAGS, "-c", os.path.join(HERE, src), "-o", obj]
+
+
+def link_command(objs, out=TARGET):
+    return [hipcc(), f"--offload-arch={ARCH}", "-fPIC", "-shared", *objs, "-o", out]
+
+
+def _run(cmd):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
-    if verbose and r.stderr.strip():
-        print(r.stderr)
+        raise RuntimeError("hipcc failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    return r.stderr
+
+
+def build(verbose=False, jobs=None):
+    """Compile every source into libmxllg.so (always recompiles); returns the library path.
+
+    The units compile in parallel (one hipcc per source, at most `jobs` at once, default the CPU
+    count capped at the number of sources) and are then linked into one shared library.
+    """
+    jobs = jobs or min(len(SOURCES), os.cpu_count() or 1)
+    with tempfile.TemporaryDirectory(prefix="mxllg_build_") as tmpdir:
+        objs = [os.path.join(tmpdir, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+        cmds = [compile_command(s, o) for s, o in zip(SOURCES, objs)]
+        if verbose:
+            for c in cmds:
+                print(" ".join(c), flush=True)
+        with ThreadPoolExecutor(max_workers=jobs) as pool:
+            logs = list(pool.map(_run, cmds))
+        tmp = TARGET + ".tmp"
+        logs.append(_run(link_command(objs, tmp)))
+    if verbose:
+        for log in logs:
+            if log.strip():
+                print(log)
     os.replace(tmp, TARGET)
     return TARGET
 
