@@ -152,10 +152,13 @@ __device__ __forceinline__ int neighbour_cell(const Static<R>* __restrict__ S, c
 // Entries per batch of the stencil loop: every load of a batch is issued before the first is used, so
 // a wavefront keeps CH gathers (and, per bond, CH tensors) in flight instead of one.  Per-bond batches
 // were timed on MI355X with the 14-entry bcc stencil: FP32 7 (two full batches; 5, 6, 8 and 9 were
-// 1-15 % slower), FP64 2 (3 and 5 slower: more tensors in flight evict their partners from L2).
+// 1-15 % slower), FP64 2 (3 and 5 slower: more tensors in flight evict their partners from L2).  Table
+// batches of 9 take the 18-entry triangular stencil (three shells) in two round trips instead of three:
+// on 256^2 spins, 12.9 -> 11.2 us per Heun step FP32 and 14.4 -> 12.8 FP64, against 2 % more on the
+// 6-entry simple cubic (three dead entries); one batch of 18 was slower on both (register pressure).
 template <typename R, int CM>
 struct Chunk {
-  static constexpr int value = CM == COUP_BOND ? (sizeof(R) == 8 ? 2 : 7) : 8;
+  static constexpr int value = CM == COUP_BOND ? (sizeof(R) == 8 ? 2 : 7) : 9;
 };
 
 // One batch of CH stencil entries: gather() issues every load of the batch without a branch, accumulate()
@@ -163,11 +166,11 @@ struct Chunk {
 // first batch is in flight.  The CH records are read as one contiguous block (merged scalar loads, one
 // wait); an entry past the end becomes a self-gather that counts zero.  Table couplings are fetched while
 // the gathers are in flight, so accumulate() waits on nothing but the gathers.
-// CHW > 0 overrides the batch width (at most 16, the padding of the record array).
+// CHW > 0 overrides the batch width (at most ENT_PAD, the padding of the record array).
 template <typename R, int CM, int GEO, int CHW = 0>
 struct Batch {
   static constexpr int CH = CHW > 0 ? CHW : Chunk<R, CM>::value;
-  static_assert(CH <= 16, "a batch may read at most 16 records past its first");
+  static_assert(CH <= ENT_PAD, "a batch may read at most ENT_PAD records past its first");
   static constexpr int NJ = CM == COUP_BOND ? 9 : (CM == COUP_TABLE9 ? 9 : (CM == COUP_TABLE4 ? 4 : 1));
   R ex[CH], ey[CH], ez[CH], J[CH][NJ];
 

@@ -17,6 +17,7 @@ constexpr double GAMMA = 0.1760859630;    // rad / (ps T)
 
 constexpr int MAXSUB = 8;    // sublattices per cell
 constexpr int MAXENT = 64;   // stencil entries over all sublattices
+constexpr int ENT_PAD = 16;  // records readable past the last entry: the widest gather batch
 constexpr int MAXTAB = 256;  // coupling-table values (ntypes x width)
 constexpr int BLOCK = 256;   // threads per workgroup (4 wavefronts of 64)
 
@@ -50,7 +51,7 @@ struct Static {
   float rL1, rL2;
   int amode, has_A, has_c, has_u;
   int sptr[MAXSUB + 1];
-  Ent ent[MAXENT + 16];  // padded: a batch of up to 16 entries may be read past the last one (never used)
+  Ent ent[MAXENT + ENT_PAD];  // padded: a batch of up to ENT_PAD entries may be read past the last one (never used)
   R tab[MAXTAB];  // COUP_TABLE*: ntypes x width
   R Asub[MAXSUB][9];
   R inv_mu_sub[MAXSUB], alpha_sub[MAXSUB], sigma_sub[MAXSUB], sign_sub[MAXSUB];
