@@ -70,7 +70,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(const Static<R>* __restrict__ S, D
     // a colour is a fraction of the lattice (a few workgroups per CU): one wide batch keeps all of a
     // site's loads in flight at once (measured on bcc64: FP32 per-bond 4 % faster; FP64 per-bond keeps
     // its narrow batches, 22 % slower wide)
-    exchange<R, CM, GEO, (CM == COUP_BOND ? (sizeof(R) == 8 ? 0 : 14) : 16)>(S, d, e, st, hx, hy, hz);
+    exchange<R, CM, GEO, (CM == COUP_BOND ? (sizeof(R) == 8 ? 2 : 14) : 16)>(S, d, e, st, hx, hy, hz);
     const R zf = R(1) / inv_mu_of(S, d, st);  // mu mu_B
     const R Hx = R(2) * hx + zf * d.B0, Hy = R(2) * hy + zf * d.B1, Hz = R(2) * hz + zf * d.B2;
     const R ox = e[3 * i], oy = e[3 * i + 1], oz = e[3 * i + 2];
