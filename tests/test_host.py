@@ -147,3 +147,23 @@ def test_product_has_no_cpu_path():
     s = S.SpinSystem(S.load_results(M.make_model("sc", J=[1.0])), (2, 2, 2))
     with pytest.raises((RuntimeError, ValueError)):
         LLGSolver(s, backend="numpy")
+
+
+def test_build_recipe_compiles_every_source_for_gfx950_without_packed_fp32():
+    """Every HIP source is one parallel compile unit with the gfx950 target, packed FP32 instructions
+    switched off and single-expression FMA contraction, and the link step takes all of their objects."""
+    from magex_amd.csrc import build as B
+    here = os.path.dirname(B.__file__)
+    on_disk = sorted(f for f in os.listdir(here) if f.endswith(".hip"))
+    assert sorted(B.SOURCES) == on_disk
+    objs = []
+    for src in B.SOURCES:
+        cmd = B.compile_command(src, src + ".o")
+        assert "--offload-arch=gfx950" in cmd and "-c" in cmd and "-ffp-contract=on" in cmd
+        k = cmd.index("-target-feature")
+        assert cmd[k - 1] == "-Xclang" and cmd[k + 1] == "-Xclang" and cmd[k + 2] == "-packed-fp32-ops"
+        assert os.path.join(here, src) in cmd
+        objs.append(cmd[cmd.index("-o") + 1])
+    link = B.link_command(objs, "out.so")
+    assert "-shared" in link and "--offload-arch=gfx950" in link
+    assert all(o in link for o in objs) and link[-2:] == ["-o", "out.so"]
