@@ -171,6 +171,20 @@ int mx_sample_mq(mx_sampler* s, const void* e, double* out, void* stream);
  * No counterpart in magex (its step loop is a Python for loop, llg.py:263-280). */
 int mx_model_set_graph_steps(mx_model* m, int32_t steps);
 
+/* How mx_llg_run and mx_llg_run_driven's uniform-field steps launch: mode 0 = two kernels per step (the
+ * predictor pass, then the corrector pass: magex's _fused1 / _fused2 pair, fast.py:336-390); mode 1
+ * (default) = one kernel per step where the model allows it (row-major cells; table couplings, or FP32
+ * per-bond tensors with at most 14 stencil entries per sublattice; axis-0 planes small enough that the
+ * workgroups a tile waits for are always resident), else two.  In one launch each workgroup keeps its
+ * sites' couplings and state in registers from the predictor to the corrector and waits only for the
+ * predictor spins of the axis-0 planes its stencil reaches, so per-bond tensors come from memory once
+ * per step instead of twice.  Results are identical in either mode.  No counterpart in magex. */
+int mx_model_set_step_mode(mx_model* m, int32_t mode);
+/* one_launch: whether mx_llg_run uses one kernel per step for this model (mode and model permitting);
+ * given_up: waits of one-launch steps that timed out since the model was created, each finished by the
+ * launch's last workgroup instead (0 in normal operation; synchronises the device).  Either may be NULL. */
+int mx_model_step_info(mx_model* m, int32_t* one_launch, int64_t* given_up);
+
 /* The standard normals mx_llg_run draws for (seed, step): out (device, N x 3). */
 int mx_noise(mx_model* m, void* out, uint64_t seed, uint64_t step, void* stream);
 

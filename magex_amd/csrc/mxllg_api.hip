@@ -144,6 +144,11 @@ static void ensure_scratch(Model& m) {
   if (!m.ep_d) hip_ok(hipMalloc(&m.ep_d, size_t(m.N) * 3 * rsize(m)), "hipMalloc ep");
   if (!m.k1_d) hip_ok(hipMalloc(&m.k1_d, size_t(m.N) * 3 * rsize(m)), "hipMalloc k1");
   if (needs_permute(m) && !m.eint_d) hip_ok(hipMalloc(&m.eint_d, size_t(m.N) * 3 * rsize(m)), "hipMalloc e");
+  if (!m.stepsync_d) {
+    const size_t n = stepsync_words(m) * sizeof(uint32_t);
+    hip_ok(hipMalloc(reinterpret_cast<void**>(&m.stepsync_d), n), "hipMalloc step control words");
+    hip_ok(hipMemset(m.stepsync_d, 0, n), "hipMemset step control words");
+  }
   if (!m.part_d) {
     const int nblk = m.ns * ((m.ncell + BLOCK - 1) / BLOCK);
     m.npart = std::max(nblk, (2 * m.N + BLOCK - 1) / BLOCK + 1);
@@ -193,7 +198,8 @@ static void ensure_graph(Model& m, void* E, uint64_t seed) {
   for (double b : m.B) put(key, b);
   for (double v : {m.sot_p[0], m.sot_p[1], m.sot_p[2], m.sot_dl, m.sot_fl}) put(key, v);
   put(key, m.dt);
-  for (int v : {m.precession, m.noisy, m.integrator, m.coup, m.pbc[0], m.pbc[1], m.pbc[2], m.graph_steps, m.prec})
+  for (int v : {m.precession, m.noisy, m.integrator, m.coup, m.pbc[0], m.pbc[1], m.pbc[2], m.graph_steps, m.prec,
+                m.step_mode})
     put(key, v);
   if (m.gexec && key == m.gkey) return;
   drop_graph(m);
@@ -278,6 +284,7 @@ void mx_model_destroy(mx_model* m) {
                    reinterpret_cast<void**>(&m->mc_repacc_d), reinterpret_cast<void**>(&m->img_part_d), &m->mc_repe_d})
     dfree(*p);
   if (m->perm_d) (void)hipFree(m->perm_d);
+  if (m->stepsync_d) (void)hipFree(m->stepsync_d);
   if (m->part_d) (void)hipFree(m->part_d);
   drop_graph(*m);
   if (m->counter_d) (void)hipFree(m->counter_d);
@@ -675,6 +682,37 @@ int mx_model_set_graph_steps(mx_model* m, int32_t steps) {
       drop_graph(*m);
     }
     m->graph_steps = steps;
+    return MX_OK;
+  });
+}
+
+int mx_model_set_step_mode(mx_model* m, int32_t mode) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (mode < 0 || mode > 1) fail(MX_EINVAL, "step mode must be 0 (two launches) or 1 (one launch)");
+    if (mode != m->step_mode) {
+      use_device(*m);
+      drop_graph(*m);
+    }
+    m->step_mode = mode;
+    return MX_OK;
+  });
+}
+
+int mx_model_step_info(mx_model* m, int32_t* one_launch, int64_t* given_up) {
+  return guarded([&]() -> int {
+    need_model(m);
+    use_device(*m);
+    if (one_launch) *one_launch = fused_step(*m) ? 1 : 0;
+    if (given_up) {
+      uint32_t v = 0;
+      if (m->stepsync_d) {
+        hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        hip_ok(hipMemcpy(&v, m->stepsync_d + size_t(m->L[0]) * CNT_STRIDE + 2, sizeof(v), hipMemcpyDeviceToHost),
+               "hipMemcpy");
+      }
+      *given_up = int64_t(v);
+    }
     return MX_OK;
   });
 }

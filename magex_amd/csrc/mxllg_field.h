@@ -28,6 +28,21 @@ __device__ __forceinline__ void checker_cell(const Static<R>* __restrict__ S, Si
   st.c2 = 2 * k + ((p + st.c0 + st.c1) & 1);
 }
 
+// Row-major coordinates of cell st.c (c2 fastest).
+__device__ __forceinline__ void rowmajor_cell(Site& st, int L1, int L2, int use_fdiv, float rL1, float rL2) {
+  if (use_fdiv) {
+    const int t = fdiv(st.c, L2, rL2);
+    st.c2 = st.c - t * L2;
+    st.c0 = fdiv(t, L1, rL1);
+    st.c1 = t - st.c0 * L1;
+  } else {
+    st.c2 = st.c % L2;
+    const int t = st.c / L2;
+    st.c1 = t % L1;
+    st.c0 = t / L1;
+  }
+}
+
 // Two block -> tile maps.  Slabs (xcd_tile): XCD x walks a contiguous run of row-major tiles, i.e. whole
 // planes of axis 0.  Columns (col and S->xcol > 0, row-major layouts whose axis-0 planes hold 8 k whole
 // tiles): XCD x owns tiles [x k, (x + 1) k) of every plane and walks them plane by plane, so the two
@@ -73,18 +88,34 @@ __device__ __forceinline__ bool site_of(const Static<R>* __restrict__ S, Site& s
   } else {
     st.c = tile * BLOCK + threadIdx.x;
     if (st.c >= ncell) return false;
-    if (use_fdiv) {
-      const int t = fdiv(st.c, L2, rL2);
-      st.c2 = st.c - t * L2;
-      st.c0 = fdiv(t, L1, rL1);
-      st.c1 = t - st.c0 * L1;
-    } else {
-      st.c2 = st.c % L2;
-      const int t = st.c / L2;
-      st.c1 = t % L1;
-      st.c0 = t / L1;
-    }
+    rowmajor_cell(st, L1, L2, use_fdiv, rL1, rL2);
   }
+  st.crm = (st.c0 * L1 + st.c1) * L2 + st.c2;
+  st.i = st.s * ncell + st.c;
+  return true;
+}
+
+// The one-launch step's block -> (sublattice, tile) map on row-major layouts: the per-XCD column walk
+// where an axis-0 plane splits into 8 k whole tiles, else tiles in order (sublattices interleaved).
+// Either way block b works on plane b / (blocks per plane), so blocks are dispatched plane after plane
+// and the planes a block waits for are held by blocks dispatched at most about one plane after it.
+// False for threads past the last cell (st.s and tile are still set).
+template <typename R>
+__device__ __forceinline__ bool step_site(const Static<R>* __restrict__ S, int b, Site& st, int& tile) {
+  const int ncell = S->ncell, L1 = S->L1, L2 = S->L2, ns = S->ns, xcol = S->xcol, use_fdiv = S->fdiv;
+  const float rL1 = S->rL1, rL2 = S->rL2;
+  if (xcol > 0) {
+    const int x = b & 7, q = b >> 3, per = xcol * ns;
+    const int p = q / per, rem = q - p * per, rr = rem / ns;
+    st.s = rem - rr * ns;
+    tile = (p * 8 + x) * xcol + rr;
+  } else {
+    tile = b / ns;
+    st.s = b - tile * ns;
+  }
+  st.c = tile * BLOCK + threadIdx.x;
+  if (st.c >= ncell) return false;
+  rowmajor_cell(st, L1, L2, use_fdiv, rL1, rL2);
   st.crm = (st.c0 * L1 + st.c1) * L2 + st.c2;
   st.i = st.s * ncell + st.c;
   return true;
@@ -149,6 +180,15 @@ __device__ __forceinline__ int neighbour_cell(const Static<R>* __restrict__ S, c
   }
 }
 
+// Spin loads of the neighbour gathers.  LD 0: ordinary cached loads.  LD 1: agent-scope coherent loads
+// (global_load ... sc1), for spins that other workgroups of the same launch wrote and published with an
+// agent-scope release: they never return a stale line of this XCD's L2, and need no L2 invalidation.
+template <int LD, typename R>
+__device__ __forceinline__ R spin_load(const R* p) {
+  if constexpr (LD == 1) return __hip_atomic_load(const_cast<R*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else return *p;
+}
+
 // Entries per batch of the stencil loop: every load of a batch is issued before the first is used, so
 // a wavefront keeps CH gathers (and, per bond, CH tensors) in flight instead of one.  Per-bond batches
 // were timed on MI355X with the 14-entry bcc stencil: FP32 7 (two full batches; 5, 6, 8 and 9 were
@@ -180,13 +220,16 @@ struct Batch {
     for (int u = 0; u < CH; ++u) raw[u] = S->ent[k0 + u];
   }
 
+  template <int LD = 0>
   __device__ __forceinline__ void gather(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
                                          const Site& st, int k0, int ke) {
     Ent raw[CH];
     records(S, k0, raw);
-    gather(S, d, e, st, k0, ke, raw);
+    gather<LD>(S, d, e, st, k0, ke, raw);
   }
 
+  // WITH_J = false reloads only the spins (the couplings of these entries are already in J).
+  template <int LD = 0, bool WITH_J = true>
   __device__ __forceinline__ void gather(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
                                          const Site& st, int k0, int ke, const Ent (&raw)[CH]) {
     const int ncell = S->ncell;
@@ -202,13 +245,14 @@ struct Batch {
       const int cn = neighbour_cell<R, GEO>(S, st, en, ok);
       ok = ok && live;
       const int j = (en.tb & 0xffff) * ncell + cn;
-      ex[u] = e[3 * j];
-      ey[u] = e[3 * j + 1];
-      ez[u] = e[3 * j + 2];
+      ex[u] = spin_load<LD>(e + 3 * j);
+      ey[u] = spin_load<LD>(e + 3 * j + 1);
+      ez[u] = spin_load<LD>(e + 3 * j + 2);
       ex[u] = ok ? ex[u] : R(0);
       ey[u] = ok ? ey[u] : R(0);
       ez[u] = ok ? ez[u] : R(0);
-      if constexpr (CM == COUP_BOND) {
+      if constexpr (!WITH_J) {
+      } else if constexpr (CM == COUP_BOND) {
         // per-bond tensor stored once per unordered bond: a forward entry reads its own cell's copy,
         // a backward entry its partner's copy at the neighbour cell, transposed (J_ji = J_ij^T)
         const bool back = (en.tb >> 16) != 0;
@@ -259,7 +303,7 @@ struct Batch {
 // values are scalar loads too, and waiting for them then also waits for the next records (triangular
 // 256^2: 11.2 -> 12.0 us).  An explicit width (the Monte Carlo sweep's single batch) keeps it too: that
 // kernel compiled 7 % slower with the prefetching loop, which it never runs.
-template <typename R, int CM, int GEO, int CHW = 0, typename Mid>
+template <typename R, int CM, int GEO, int CHW = 0, int LD = 0, typename Mid>
 __device__ __forceinline__ void exchange(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
                                          const Site& st, R& hx, R& hy, R& hz, Mid&& mid) {
   hx = hy = hz = R(0);
@@ -270,14 +314,14 @@ __device__ __forceinline__ void exchange(const Static<R>* __restrict__ S, const 
   }
   using B = Batch<R, CM, GEO, CHW>;
   B bt;
-  bt.gather(S, d, e, st, kb, ke);
+  bt.template gather<LD>(S, d, e, st, kb, ke);
   if constexpr (CM == COUP_BOND && CHW == 0) {
     Ent next[B::CH];
     if (kb + B::CH < ke) B::records(S, kb + B::CH, next);
     mid();
     bt.accumulate(S, hx, hy, hz);
     for (int k0 = kb + B::CH; k0 < ke; k0 += B::CH) {
-      bt.gather(S, d, e, st, k0, ke, next);
+      bt.template gather<LD>(S, d, e, st, k0, ke, next);
       if (k0 + B::CH < ke) B::records(S, k0 + B::CH, next);
       bt.accumulate(S, hx, hy, hz);
     }
@@ -285,7 +329,7 @@ __device__ __forceinline__ void exchange(const Static<R>* __restrict__ S, const 
     mid();
     bt.accumulate(S, hx, hy, hz);
     for (int k0 = kb + B::CH; k0 < ke; k0 += B::CH) {
-      bt.gather(S, d, e, st, k0, ke);
+      bt.template gather<LD>(S, d, e, st, k0, ke);
       bt.accumulate(S, hx, hy, hz);
     }
   }
@@ -297,12 +341,41 @@ __device__ __forceinline__ void exchange(const Static<R>* __restrict__ S, const 
   exchange<R, CM, GEO, CHW>(S, d, e, st, hx, hy, hz, [] {});
 }
 
-// On-site gradient dE/de (magex/fast.py:39-64): 2 A e + cubic + higher uniaxial.
+// Per-site on-site tensor of one site (ONSITE_SITE6: xx, yy, zz, xy, xz, yz in v[0..5]; ONSITE_SITE9:
+// row-major), read once; the one-launch step keeps it for both Heun passes.
+template <typename R>
+struct SiteA {
+  R v[9];
+  __device__ __forceinline__ void load(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st) {
+    const int amode = S->amode;
+    if (amode == ONSITE_SITE6) {
+      const int N = S->N, i = st.i;
+      const R* A = d.A;
+      if constexpr (sizeof(R) == 8) {
+        // FP64: the per-site tensor is read once per pass; streaming it past L2 leaves room for the
+        // partner bond tensors (measured 4-6 % faster on bcc64; slower in FP32, where it stays cached)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) v[k] = __builtin_nontemporal_load(A + size_t(k) * N + i);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) v[k] = A[size_t(k) * N + i];
+      }
+    } else if (amode == ONSITE_SITE9) {
+      const int N = S->N;
+      const R* a = d.A + st.i;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) v[k] = a[size_t(k) * N];
+    }
+  }
+};
+
+// On-site gradient dE/de (magex/fast.py:39-64): 2 A e + cubic + higher uniaxial, with the site's tensor
+// already in A (ignored for ONSITE_SUB, whose tensor is in the constant block).
 // LEAN (here and below): compiled for the plain model -- no cubic or higher-uniaxial terms, uniform
 // moments and temperatures per sublattice, generated noise, no spin-orbit torque fields.
 template <typename R, bool LEAN = false>
-__device__ __forceinline__ void onsite_grad(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st, R x, R y,
-                                            R z, R& gx, R& gy, R& gz) {
+__device__ __forceinline__ void onsite_grad(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st,
+                                            const SiteA<R>& A, R x, R y, R z, R& gx, R& gy, R& gz) {
   gx = gy = gz = R(0);
   const int amode = S->amode;
   if (amode == ONSITE_SUB) {
@@ -313,35 +386,15 @@ __device__ __forceinline__ void onsite_grad(const Static<R>* __restrict__ S, con
       gz = R(2) * (a[6] * x + a[7] * y + a[8] * z);
     }
   } else if (amode == ONSITE_SITE6) {
-    const int N = S->N, i = st.i;
-    const R* A = d.A;
-    R axx, ayy, azz, axy, axz, ayz;
-    if constexpr (sizeof(R) == 8) {
-      // FP64: the per-site tensor is read once per pass; streaming it past L2 leaves room for the
-      // partner bond tensors (measured 4-6 % faster on bcc64; slower in FP32, where it stays cached)
-      axx = __builtin_nontemporal_load(A + i);
-      ayy = __builtin_nontemporal_load(A + N + i);
-      azz = __builtin_nontemporal_load(A + 2 * N + i);
-      axy = __builtin_nontemporal_load(A + 3 * N + i);
-      axz = __builtin_nontemporal_load(A + 4 * N + i);
-      ayz = __builtin_nontemporal_load(A + 5 * N + i);
-    } else {
-      axx = A[i];
-      ayy = A[N + i];
-      azz = A[2 * N + i];
-      axy = A[3 * N + i];
-      axz = A[4 * N + i];
-      ayz = A[5 * N + i];
-    }
+    const R axx = A.v[0], ayy = A.v[1], azz = A.v[2], axy = A.v[3], axz = A.v[4], ayz = A.v[5];
     gx = R(2) * (axx * x + axy * y + axz * z);
     gy = R(2) * (axy * x + ayy * y + ayz * z);
     gz = R(2) * (axz * x + ayz * y + azz * z);
   } else {
-    const int N = S->N;
-    const R* a = d.A + st.i;
-    gx = R(2) * (a[0] * x + a[N] * y + a[2 * N] * z);
-    gy = R(2) * (a[3 * N] * x + a[4 * N] * y + a[5 * N] * z);
-    gz = R(2) * (a[6 * N] * x + a[7 * N] * y + a[8 * N] * z);
+    const R* a = A.v;
+    gx = R(2) * (a[0] * x + a[1] * y + a[2] * z);
+    gy = R(2) * (a[3] * x + a[4] * y + a[5] * z);
+    gz = R(2) * (a[6] * x + a[7] * y + a[8] * z);
   }
   if (LEAN) return;
   if (S->has_c) {
@@ -364,6 +417,14 @@ __device__ __forceinline__ void onsite_grad(const Static<R>* __restrict__ S, con
     gy += f * u[1];
     gz += f * u[2];
   }
+}
+
+template <typename R, bool LEAN = false>
+__device__ __forceinline__ void onsite_grad(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st, R x, R y,
+                                            R z, R& gx, R& gy, R& gz) {
+  SiteA<R> A;
+  A.load(S, d, st);
+  onsite_grad<R, LEAN>(S, d, st, A, x, y, z, gx, gy, gz);
 }
 
 // On-site energy (magex/fast.py:66-84).
@@ -429,27 +490,90 @@ __device__ __forceinline__ R inv_mu_of(const Static<R>* __restrict__ S, const Dy
   return (!LEAN && d.inv_mu) ? d.inv_mu[st.i] : S->inv_mu_sub[st.s];
 }
 
-// B_i = (2 h_i - g_i) / (mu_i mu_B) + B (+ thermal) at spin (x, y, z) (magex/fast.py:309-317).  The
-// generator's normals are computed while the first neighbour batch is in flight.
-template <typename R, int CM, int GEO, bool LEAN = false>
-__device__ __forceinline__ void local_field(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
-                                            const Site& st, R x, R y, R z, uint64_t step,
-                                            const R* __restrict__ noise, R& bx, R& by, R& bz) {
-  R hx, hy, hz, gx, gy, gz, nx = R(0), ny = R(0), nz = R(0);
-  exchange<R, CM, GEO>(S, d, e, st, hx, hy, hz, [&] {
-    if (d.noisy) thermal<R, LEAN>(S, d, st, step, noise, nx, ny, nz);
-  });
-  onsite_grad<R, LEAN>(S, d, st, x, y, z, gx, gy, gz);
+// B_i = (2 h_i - g_i) / (mu_i mu_B) + B (+ thermal) at spin (x, y, z) (magex/fast.py:309-317), from the
+// exchange sum h and the on-site gradient g; Bu is the uniform field of this pass.  A caller-supplied
+// field d.extra (magex order, N x 3) is added after the uniform field and before the thermal field, the
+// order of magex's unfused path (fast.py:216-222, 250-269).
+template <typename R, bool LEAN = false>
+__device__ __forceinline__ void field_from(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st, R x, R y,
+                                           R z, R hx, R hy, R hz, R gx, R gy, R gz, R nx, R ny, R nz, R Bu0, R Bu1,
+                                           R Bu2, R& bx, R& by, R& bz) {
   const R im = inv_mu_of<R, LEAN>(S, d, st);
-  bx = (R(2) * hx - gx) * im + d.B0 + nx;
-  by = (R(2) * hy - gy) * im + d.B1 + ny;
-  bz = (R(2) * hz - gz) * im + d.B2 + nz;
+  if (!LEAN && d.extra) {
+    const R* x3 = d.extra + 3 * size_t(st.crm * S->ns + st.s);
+    bx = (R(2) * hx - gx) * im + Bu0;
+    by = (R(2) * hy - gy) * im + Bu1;
+    bz = (R(2) * hz - gz) * im + Bu2;
+    bx += x3[0];
+    by += x3[1];
+    bz += x3[2];
+    bx += nx;
+    by += ny;
+    bz += nz;
+  } else {
+    bx = (R(2) * hx - gx) * im + Bu0 + nx;
+    by = (R(2) * hy - gy) * im + Bu1 + ny;
+    bz = (R(2) * hz - gz) * im + Bu2 + nz;
+  }
   if (!LEAN && d.sot) {
     // spin-orbit (Slonczewski-like) torque as fields: B_DL (e x p) + B_FL p (magex/llg.py:126-128)
     bx += d.bdl * (y * d.sp2 - z * d.sp1) + d.bfl * d.sp0;
     by += d.bdl * (z * d.sp0 - x * d.sp2) + d.bfl * d.sp1;
     bz += d.bdl * (x * d.sp1 - y * d.sp0) + d.bfl * d.sp2;
   }
+}
+
+// The local field at spin (x, y, z) of site st, gathering the neighbours' spins from e.  The generator's
+// normals are computed while the first neighbour batch is in flight.
+template <typename R, int CM, int GEO, bool LEAN = false>
+__device__ __forceinline__ void local_field(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
+                                            const Site& st, R x, R y, R z, uint64_t step,
+                                            const R* __restrict__ noise, R Bu0, R Bu1, R Bu2, R& bx, R& by, R& bz) {
+  R hx, hy, hz, gx, gy, gz, nx = R(0), ny = R(0), nz = R(0);
+  exchange<R, CM, GEO>(S, d, e, st, hx, hy, hz, [&] {
+    if (d.noisy) thermal<R, LEAN>(S, d, st, step, noise, nx, ny, nz);
+  });
+  onsite_grad<R, LEAN>(S, d, st, x, y, z, gx, gy, gz);
+  field_from<R, LEAN>(S, d, st, x, y, z, hx, hy, hz, gx, gy, gz, nx, ny, nz, Bu0, Bu1, Bu2, bx, by, bz);
+}
+
+// Heun / Depondt corrector of site st (magex/fast.py:370-390): the field at the predictor ep (uniform
+// part Bu), then e = n(w + dt/2 f(ep)) with w = e + dt/2 k1 kept in k1 (Heun), or e = n(R(dt/2 (w1 + w2)) e)
+// with the predictor's rotation vector w1 in k1 (Depondt); in place.  save_B (adjoint recordings, may
+// be null) receives the field.
+template <typename R, int CM, int GEO, bool LEAN = false>
+__device__ __forceinline__ void corrector(const Static<R>* __restrict__ S, const Dyn<R>& d, R* __restrict__ e,
+                                          const R* __restrict__ ep, const R* __restrict__ k1, const Site& st, R a,
+                                          uint64_t step, const R* __restrict__ noise, R Bu0, R Bu1, R Bu2,
+                                          R* __restrict__ save_B) {
+  const int i = st.i;
+  const R x = ep[3 * i], y = ep[3 * i + 1], z = ep[3 * i + 2];
+  R bx, by, bz, fx, fy, fz;
+  local_field<R, CM, GEO, LEAN>(S, d, ep, st, x, y, z, step, noise, Bu0, Bu1, Bu2, bx, by, bz);
+  if (!LEAN && save_B) {
+    save_B[3 * i] = bx;
+    save_B[3 * i + 1] = by;
+    save_B[3 * i + 2] = bz;
+  }
+  llg_rhs(x, y, z, bx, by, bz, a, d.prec, fx, fy, fz);
+  const R hdt = R(0.5) * d.dt;
+  R cx, cy, cz;
+  if (!LEAN && d.integ == MX_DEPONDT) {
+    R wx, wy, wz;
+    omega(x, y, z, fx, fy, fz, wx, wy, wz);
+    cx = e[3 * i];
+    cy = e[3 * i + 1];
+    cz = e[3 * i + 2];
+    rotate(cx, cy, cz, hdt * (k1[3 * i] + wx), hdt * (k1[3 * i + 1] + wy), hdt * (k1[3 * i + 2] + wz));
+  } else {
+    cx = k1[3 * i] + hdt * fx;
+    cy = k1[3 * i + 1] + hdt * fy;
+    cz = k1[3 * i + 2] + hdt * fz;
+  }
+  const R q = rnorm3(cx, cy, cz);
+  e[3 * i] = cx * q;
+  e[3 * i + 1] = cy * q;
+  e[3 * i + 2] = cz * q;
 }
 
 }  // namespace mx

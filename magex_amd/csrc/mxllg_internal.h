@@ -82,6 +82,23 @@ struct Dyn {
   const uint64_t* step_ptr;  // captured step loops: device step counter added to the step argument
   R* save_e;                 // adjoint replays: pass 1 stores e here, both passes store B_eff in save_B
   R* save_B;
+  const R* extra;            // caller-supplied field of this pass (magex order, N x 3), or null
+  R Bc0, Bc1, Bc2;           // uniform field of the corrector in the one-launch step (B0..B2: predictor)
+};
+
+// Control words of the one-launch Heun step (k_step) in model-owned device memory.  Every word is zero
+// between launches: the last workgroup of a launch clears what the launch used.
+// The plane counters sit one per 128-byte line: hundreds of wavefronts poll the few counters around
+// their plane while others increment them.
+constexpr int CNT_STRIDE = 32;
+struct StepSync {
+  uint32_t* cnt;  // per axis-0 plane (stride CNT_STRIDE): workgroups whose predictor spins covering the
+                  // plane are published
+  uint32_t* ctl;  // [0] workgroups finished, [1] waits given up in this launch, [2] given up since creation
+  int32_t* gu;    // blocks whose wait gave up (their corrector runs in the launch's last workgroup)
+  int r0;         // stencil reach along axis 0, in planes
+  int PS;         // cells per axis-0 plane
+  uint32_t spin;  // polls of a wait before it gives up
 };
 
 // Host-side model.
@@ -147,6 +164,10 @@ struct Model {
   int mc_repe_cap = 0;
   double* img_part_d = nullptr;  // partials of image-stack observables
   int img_part_cap = 0;          // images they hold
+  // one-launch step (k_step): mode (0: two launches per step; 1: one launch where the model allows it),
+  // control words
+  int step_mode = 1;
+  uint32_t* stepsync_d = nullptr;  // L0 plane counters, 4 control words, nblk give-up slots
   // captured step loop (hipGraph of graph_steps Heun steps + counter advance), keyed by what it bakes in
   int graph_steps = 0;
   uint64_t* counter_d = nullptr;
@@ -160,6 +181,10 @@ int upload_static(Model& m);
 int launch_heun_step(const Model& m, void* e, uint64_t seed, uint64_t step, const void* noise_step, hipStream_t s,
                      const uint64_t* step_ptr = nullptr);
 int launch_counter(uint64_t* c, uint64_t value, int add, hipStream_t s);
+// Whether a step of m runs as one launch (k_step), and the number of held coupling batches it uses.
+bool fused_step(const Model& m, int* nb = nullptr);
+// Device words of the one-launch step (allocated and zeroed with the model's scratch).
+size_t stepsync_words(const Model& m);
 // One Heun pass (1 or 2) with optional stores of e and B_eff (magex order is not used: internal order).
 // B_ext (3 doubles) replaces the model's uniform field for this pass when non-null.
 int launch_heun_pass(const Model& m, int pass, void* e, void* ep, void* k1, uint64_t seed, uint64_t step,

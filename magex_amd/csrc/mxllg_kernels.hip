@@ -19,6 +19,8 @@
 // lattice and the neighbour re-reads of that slab hit its own L2.  The step kernels use per-XCD columns
 // instead where axis-0 planes split into whole tiles (site_of), and the corrector pass walks
 // them backwards, starting on the tiles the predictor pass left in L2.
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "mxllg_field.h"
@@ -66,7 +68,7 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S,
       return;
     }
     R bx, by, bz, fx, fy, fz;
-    local_field<R, CM, GEO, LEAN>(S, d, e, st, x, y, z, step, noise, bx, by, bz);
+    local_field<R, CM, GEO, LEAN>(S, d, e, st, x, y, z, step, noise, d.B0, d.B1, d.B2, bx, by, bz);
     if (!LEAN && d.save_e) {
       d.save_e[3 * i] = x;
       d.save_e[3 * i + 1] = y;
@@ -102,33 +104,235 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S,
       if (!LEAN && d.save_B) d.save_B[3 * i] = d.save_B[3 * i + 1] = d.save_B[3 * i + 2] = R(0);
       return;
     }
-    const R x = ep[3 * i], y = ep[3 * i + 1], z = ep[3 * i + 2];
-    R bx, by, bz, fx, fy, fz;
-    local_field<R, CM, GEO, LEAN>(S, d, ep, st, x, y, z, step, noise, bx, by, bz);
-    if (!LEAN && d.save_B) {
-      d.save_B[3 * i] = bx;
-      d.save_B[3 * i + 1] = by;
-      d.save_B[3 * i + 2] = bz;
+    corrector<R, CM, GEO, LEAN>(S, d, e, ep, k1, st, a, step, noise, d.B0, d.B1, d.B2, d.save_B);
+  }
+}
+
+// ------------------------------------------------------------------------------------------ one-launch step
+// Wavefront 0 polls the plane counters of planes [lo, lo + n) (wrapped onto [0, L0)) until each plane has
+// been published by every workgroup whose tile covers it; false after `spin` unsuccessful polls.
+template <typename R>
+__device__ __forceinline__ bool wait_planes(const Static<R>* __restrict__ S, const StepSync& sy, int lo, int n) {
+  const int L0 = S->L0, PS = sy.PS, ns = S->ns, lane = threadIdx.x & 63;
+  for (uint32_t it = 0;; ++it) {
+    int ok = 1;
+    for (int k = lane; k < n; k += 64) {
+      int P = lo + k;
+      P = P >= L0 ? P - L0 : P;
+      const uint32_t want = uint32_t(ns * (((P + 1) * PS - 1) / BLOCK - (P * PS) / BLOCK + 1));
+      const uint32_t c = __hip_atomic_load(sy.cnt + P * CNT_STRIDE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      ok = ok && c >= want;
     }
-    llg_rhs(x, y, z, bx, by, bz, a, d.prec, fx, fy, fz);
-    const R hdt = R(0.5) * d.dt;
-    R cx, cy, cz;
-    if (!LEAN && d.integ == MX_DEPONDT) {
-      R wx, wy, wz;
-      omega(x, y, z, fx, fy, fz, wx, wy, wz);
-      cx = e[3 * i];
-      cy = e[3 * i + 1];
-      cz = e[3 * i + 2];
-      rotate(cx, cy, cz, hdt * (k1[3 * i] + wx), hdt * (k1[3 * i + 1] + wy), hdt * (k1[3 * i + 2] + wz));
+    if (__all(ok)) return true;
+    if (it >= sy.spin) return false;
+    __builtin_amdgcn_s_sleep(8);
+  }
+}
+
+// One stochastic Heun (or Depondt) step in one launch (magex/fast.py:336-390 as one kernel): each
+// workgroup runs the predictor of its tile, publishes the predictor spins, waits until the predictor
+// spins of every axis-0 plane its stencil reaches are published, and runs the corrector of the same tile.
+// Everything the corrector needs of its own sites stays in registers: the spin, w = e + dt/2 k1 (Heun) or
+// the rotation vector (Depondt), the thermal field, the per-site on-site tensor and, for per-bond
+// couplings (NB batches held), the bond tensors.  Each tensor therefore comes from HBM once per step
+// instead of once per pass.  The arithmetic is that of k_heun's two passes, so the results are the same
+// bits.
+//
+// Ordering: a workgroup's predictor reads e of planes within r0 of its own and never waits; its
+// corrector writes e only after every predictor that reads those planes has published (the wait covers
+// the same planes).  Publication without cache maintenance: the predictor spins are stored with
+// agent-scope stores (global_store ... sc1, written through to the agent-coherent level), the wavefront
+// waits for their acknowledgement, and only then do the plane counters advance.  The corrector then
+// gathers the neighbours' predictor spins with ordinary cached loads: no cache of this launch can hold a
+// stale copy of them, because each predictor line is written once per launch (tiles are whole lines)
+// and no workgroup loads a line of a plane before the plane is published, while copies from earlier
+// launches are made coherent at the kernel boundary (the two-launch step relies on the same).
+// Measured on bcc 64^3 FP32: an agent-scope release and acquire per wavefront (buffer_wbl2 /
+// buffer_inv of the XCD's L2) made the step 7x slower than two launches, agent-scope (sc1) loads of the
+// gathered spins, which bypass L2, 3.8x.  Every wait is bounded: a workgroup that gives up stores w
+// (behind a release fence) and leaves its corrector to the launch's last workgroup, which runs it once
+// every predictor is done, then clears the counters for the next launch.
+template <typename R, int CM, int GEO, int NB, bool LEAN>
+__global__ __launch_bounds__(BLOCK) void k_step(const Static<R>* __restrict__ S, const Dyn<R> d, R* __restrict__ e,
+                                                R* __restrict__ ep, R* __restrict__ k1, uint64_t step,
+                                                const R* __restrict__ noise, const StepSync sy) {
+  using BT = Batch<R, CM, GEO>;
+  constexpr int CH = BT::CH;
+  __shared__ int sh_flag;
+  if (d.step_ptr) step += *d.step_ptr;  // graph replays: step = device counter + offset of this node
+  Site st;
+  int tile;
+  const bool live = step_site<R>(S, blockIdx.x, st, tile);
+  const int i = st.i;
+  const bool pin = live && !LEAN && d.pinned && d.pinned[i];
+  const int kb = S->sptr[st.s], ke = S->sptr[st.s + 1];
+  BT bt[NB > 0 ? NB : 1];
+  SiteA<R> A;
+  R a = R(0), x = R(0), y = R(0), z = R(0), px = R(0), py = R(0), pz = R(0);
+  R wx = R(0), wy = R(0), wz = R(0), nx = R(0), ny = R(0), nz = R(0);
+  // ---- predictor (pass 1)
+  if (live) {
+    a = (!LEAN && d.alpha) ? d.alpha[i] : S->alpha_sub[st.s];
+    x = e[3 * i];
+    y = e[3 * i + 1];
+    z = e[3 * i + 2];
+    if (pin) {
+      px = wx = x;
+      py = wy = y;
+      pz = wz = z;
     } else {
-      cx = k1[3 * i] + hdt * fx;
-      cy = k1[3 * i + 1] + hdt * fy;
-      cz = k1[3 * i + 2] + hdt * fz;
+      R hx = R(0), hy = R(0), hz = R(0), gx, gy, gz, bx, by, bz, fx, fy, fz;
+      if constexpr (NB > 0) {
+        // every held batch in flight before the first is used
+#pragma unroll
+        for (int n = 0; n < NB; ++n) bt[n].gather(S, d, e, st, kb + n * CH, ke);
+        if (d.noisy) thermal<R, LEAN>(S, d, st, step, noise, nx, ny, nz);
+#pragma unroll
+        for (int n = 0; n < NB; ++n)
+          if (kb + n * CH < ke) bt[n].accumulate(S, hx, hy, hz);
+      } else {
+        exchange<R, CM, GEO>(S, d, e, st, hx, hy, hz, [&] {
+          if (d.noisy) thermal<R, LEAN>(S, d, st, step, noise, nx, ny, nz);
+        });
+      }
+      A.load(S, d, st);
+      onsite_grad<R, LEAN>(S, d, st, A, x, y, z, gx, gy, gz);
+      field_from<R, LEAN>(S, d, st, x, y, z, hx, hy, hz, gx, gy, gz, nx, ny, nz, d.B0, d.B1, d.B2, bx, by, bz);
+      llg_rhs(x, y, z, bx, by, bz, a, d.prec, fx, fy, fz);
+      if (!LEAN && d.integ == MX_DEPONDT) {
+        omega(x, y, z, fx, fy, fz, wx, wy, wz);
+        px = x;
+        py = y;
+        pz = z;
+        rotate(px, py, pz, d.dt * wx, d.dt * wy, d.dt * wz);
+      } else {
+        const R hdt = R(0.5) * d.dt;
+        wx = x + hdt * fx;
+        wy = y + hdt * fy;
+        wz = z + hdt * fz;
+        const R qx = x + d.dt * fx, qy = y + d.dt * fy, qz = z + d.dt * fz;
+        const R q = rnorm3(qx, qy, qz);
+        px = qx * q;
+        py = qy * q;
+        pz = qz * q;
+      }
     }
-    const R q = rnorm3(cx, cy, cz);
-    e[3 * i] = cx * q;
-    e[3 * i + 1] = cy * q;
-    e[3 * i + 2] = cz * q;
+    __hip_atomic_store(ep + 3 * i, px, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(ep + 3 * i + 1, py, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(ep + 3 * i + 2, pz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  // ---- publish the predictor spins of this tile's planes: stores acknowledged, then the counters
+  const int L0 = S->L0, PS = sy.PS;
+  const int c_first = tile * BLOCK, c_last = min(S->ncell, c_first + BLOCK) - 1;
+  const int pa = c_first / PS, pb = c_last / PS;
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (int(threadIdx.x) <= pb - pa)
+    __hip_atomic_fetch_add(sy.cnt + (pa + threadIdx.x) * CNT_STRIDE, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // ---- wait for the planes within r0
+  if (threadIdx.x < 64) {
+    int lo = pa - sy.r0, n = pb - pa + 1 + 2 * sy.r0;
+    if (S->pbc0) {
+      if (n >= L0) {
+        lo = 0;
+        n = L0;
+      } else if (lo < 0) {
+        lo += L0;
+      }
+    } else {
+      const int hi = min(L0 - 1, pb + sy.r0);
+      lo = max(lo, 0);
+      n = hi - lo + 1;
+    }
+    const bool ok = wait_planes<R>(S, sy, lo, n);
+    if (threadIdx.x == 0) sh_flag = ok ? 1 : 0;
+  }
+  __syncthreads();
+  const bool ok = sh_flag != 0;
+  if (ok) {
+    // ---- corrector (pass 2) from registers
+    if (live && !pin) {
+      R hx = R(0), hy = R(0), hz = R(0), gx, gy, gz, bx, by, bz, fx, fy, fz;
+      if constexpr (NB > 0) {
+#pragma unroll
+        for (int n = 0; n < NB; ++n) {
+          Ent raw[CH];
+          BT::records(S, kb + n * CH, raw);
+          bt[n].template gather<0, false>(S, d, ep, st, kb + n * CH, ke, raw);
+        }
+#pragma unroll
+        for (int n = 0; n < NB; ++n)
+          if (kb + n * CH < ke) bt[n].accumulate(S, hx, hy, hz);
+      } else {
+        exchange<R, CM, GEO>(S, d, ep, st, hx, hy, hz, [] {});
+      }
+      onsite_grad<R, LEAN>(S, d, st, A, px, py, pz, gx, gy, gz);
+      field_from<R, LEAN>(S, d, st, px, py, pz, hx, hy, hz, gx, gy, gz, nx, ny, nz, d.Bc0, d.Bc1, d.Bc2, bx, by,
+                          bz);
+      llg_rhs(px, py, pz, bx, by, bz, a, d.prec, fx, fy, fz);
+      const R hdt = R(0.5) * d.dt;
+      R cx, cy, cz;
+      if (!LEAN && d.integ == MX_DEPONDT) {
+        R vx, vy, vz;
+        omega(px, py, pz, fx, fy, fz, vx, vy, vz);
+        cx = x;
+        cy = y;
+        cz = z;
+        rotate(cx, cy, cz, hdt * (wx + vx), hdt * (wy + vy), hdt * (wz + vz));
+      } else {
+        cx = wx + hdt * fx;
+        cy = wy + hdt * fy;
+        cz = wz + hdt * fz;
+      }
+      const R q = rnorm3(cx, cy, cz);
+      e[3 * i] = cx * q;
+      e[3 * i + 1] = cy * q;
+      e[3 * i + 2] = cz * q;
+    }
+  } else {
+    // gave up: keep w (or the rotation vector) for the last workgroup's corrector
+    if (live && !pin) {
+      k1[3 * i] = wx;
+      k1[3 * i + 1] = wy;
+      k1[3 * i + 2] = wz;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint32_t k = __hip_atomic_fetch_add(sy.ctl + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(sy.gu + k, int32_t(blockIdx.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  // ---- the last workgroup to finish runs the correctors that were given up and clears the counters
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t done = ok ? __hip_atomic_fetch_add(sy.ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                             : __hip_atomic_fetch_add(sy.ctl, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    sh_flag = done == gridDim.x - 1 ? 1 : 0;
+  }
+  __syncthreads();
+  if (!sh_flag) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  const uint32_t ngu = __hip_atomic_load(sy.ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (uint32_t g = 0; g < ngu; ++g) {
+    const int b = __hip_atomic_load(sy.gu + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    Site s2;
+    int t2;
+    if (step_site<R>(S, b, s2, t2)) {
+      const int j = s2.i;
+      if (LEAN || !(d.pinned && d.pinned[j])) {
+        const R a2 = (!LEAN && d.alpha) ? d.alpha[j] : S->alpha_sub[s2.s];
+        corrector<R, CM, GEO, LEAN>(S, d, e, ep, k1, s2, a2, step, noise, d.Bc0, d.Bc1, d.Bc2, nullptr);
+      }
+    }
+  }
+  for (int P = threadIdx.x; P < L0; P += BLOCK)
+    __hip_atomic_store(sy.cnt + P * CNT_STRIDE, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (ngu) __hip_atomic_fetch_add(sy.ctl + 2, ngu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(sy.ctl + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(sy.ctl, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -384,6 +588,10 @@ static Dyn<R> make_dyn(const Model& m, uint64_t seed) {
   d.step_ptr = nullptr;
   d.save_e = nullptr;
   d.save_B = nullptr;
+  d.extra = nullptr;
+  d.Bc0 = d.B0;
+  d.Bc1 = d.B1;
+  d.Bc2 = d.B2;
   return d;
 }
 
@@ -413,6 +621,82 @@ static inline bool lean_model(const Model& m) {
     }                                                             \
   } while (0)
 
+// ------------------------------------------------------------------------------------------ one-launch step (host)
+// Stencil reach along axis 0, in planes (offsets on a periodic axis are stored reduced into [0, L0)).
+static int reach0(const Model& m) {
+  int r = 0;
+  for (const Ent& en : m.ent) {
+    int d0 = int(int16_t(en.d01 & 0xffff));
+    if (m.pbc[0]) d0 = std::min(d0, m.L[0] - d0);
+    r = std::max(r, std::abs(d0));
+  }
+  return std::min(r, m.L[0]);
+}
+
+size_t stepsync_words(const Model& m) { return size_t(m.L[0]) * CNT_STRIDE + 4 + size_t(nblk_of(m)); }
+
+// Per-bond couplings keep all their tensors in registers across the two passes: FP32 up to two batches
+// of 7 (the 14-entry bcc stencil); FP64 tensors (twice the registers) are not held, and FP64 per-bond
+// models keep two launches per step.
+static int held_batches(const Model& m) {
+  if (m.coup != COUP_BOND) return 0;
+  if (m.prec != MX_FP32) return -1;
+  int mx = 0;
+  for (int s = 0; s < m.ns; ++s) mx = std::max(mx, m.sptr[s + 1] - m.sptr[s]);
+  const int ch = Chunk<float, COUP_BOND>::value;
+  const int nb = (mx + ch - 1) / ch;
+  return nb <= 2 ? std::max(nb, 1) : -1;
+}
+
+// Workgroups a block of the one-launch step may wait on beyond its own plane, in dispatch order.
+static long step_window(const Model& m) {
+  const long PS = long(m.L[1]) * m.L[2];
+  return long(m.ns) * ((2L * reach0(m) + 2) * PS / BLOCK + 3);
+}
+
+bool fused_step(const Model& m, int* nb_out) {
+  if (m.step_mode == 0 || !(m.geo == GEO_PERIODIC || m.geo == GEO_OPEN)) return false;
+  const int nb = held_batches(m);
+  if (nb < 0) return false;
+  int dev_cu = 0;
+  if (hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, m.device) != hipSuccess || dev_cu <= 0)
+    return false;
+  // every block this many ahead in dispatch order is resident with at least one resident block per CU,
+  // so a wait is always for blocks that are already running (the kernels hold 1 to 4 blocks per CU)
+  if (step_window(m) > dev_cu) return false;
+  // one launch pays where the whole grid is resident at once (one block per CU or fewer): measured on
+  // MI355X, sc 16^3 FP64 11.9 -> 8.8 us per step, triangular 256^2 FP32 15.1 -> 15.0.  On larger grids the
+  // blocks run in lock-step generations whose waits leave memory idle (bcc 64^3 FP32, 2048 blocks at two
+  // per CU: 57 -> 90 us), so they keep two launches per step.
+  if (nblk_of(m) > dev_cu) return false;
+  if (nb_out) *nb_out = nb;
+  return true;
+}
+
+template <typename R, int CM, int GEO, int NB>
+static void step_launch(const Model& m, bool lean, dim3 g, hipStream_t s, const Static<R>* S, const Dyn<R>& d, R* E,
+                        R* ep, R* k1, uint64_t step, const R* nz, const StepSync& sy) {
+  const dim3 b(BLOCK);
+  if (lean) hipLaunchKernelGGL((k_step<R, CM, GEO, NB, true>), g, b, 0, s, S, d, E, ep, k1, step, nz, sy);
+  else hipLaunchKernelGGL((k_step<R, CM, GEO, NB, false>), g, b, 0, s, S, d, E, ep, k1, step, nz, sy);
+}
+
+template <typename R, int GEO>
+static void step_dispatch(const Model& m, int nb, bool lean, dim3 g, hipStream_t s, const Static<R>* S,
+                          const Dyn<R>& d, R* E, R* ep, R* k1, uint64_t step, const R* nz, const StepSync& sy) {
+  switch (m.coup) {
+    case COUP_TABLE1: step_launch<R, COUP_TABLE1, GEO, 0>(m, lean, g, s, S, d, E, ep, k1, step, nz, sy); break;
+    case COUP_TABLE4: step_launch<R, COUP_TABLE4, GEO, 0>(m, lean, g, s, S, d, E, ep, k1, step, nz, sy); break;
+    case COUP_TABLE9: step_launch<R, COUP_TABLE9, GEO, 0>(m, lean, g, s, S, d, E, ep, k1, step, nz, sy); break;
+    default:
+      if constexpr (sizeof(R) == 4) {
+        if (nb == 1) step_launch<R, COUP_BOND, GEO, 1>(m, lean, g, s, S, d, E, ep, k1, step, nz, sy);
+        else step_launch<R, COUP_BOND, GEO, 2>(m, lean, g, s, S, d, E, ep, k1, step, nz, sy);
+      }
+      break;
+  }
+}
+
 template <typename R>
 static int heun_t(const Model& m, void* e, uint64_t seed, uint64_t step, const void* noise, hipStream_t s,
                   const uint64_t* step_ptr) {
@@ -424,6 +708,20 @@ static int heun_t(const Model& m, void* e, uint64_t seed, uint64_t step, const v
   R* k1 = static_cast<R*>(m.k1_d);
   const R* nz = static_cast<const R*>(noise);
   const dim3 g(nblk_of(m)), b(BLOCK);
+  int nb = 0;
+  if (m.stepsync_d && fused_step(m, &nb)) {
+    StepSync sy;
+    sy.cnt = m.stepsync_d;
+    sy.ctl = m.stepsync_d + size_t(m.L[0]) * CNT_STRIDE;
+    sy.gu = reinterpret_cast<int32_t*>(sy.ctl + 4);
+    sy.r0 = reach0(m);
+    sy.PS = m.L[1] * m.L[2];
+    sy.spin = 1u << 16;
+    const bool lean = lean_model(m) && !noise;
+    if (m.geo == GEO_PERIODIC) step_dispatch<R, GEO_PERIODIC>(m, nb, lean, g, s, S, d, E, ep, k1, step, nz, sy);
+    else step_dispatch<R, GEO_OPEN>(m, nb, lean, g, s, S, d, E, ep, k1, step, nz, sy);
+    return check_launch("heun step (one launch)");
+  }
   if (lean_model(m) && !noise) {
 #define MX_HEUN_CALL(CM, GEO)                                                                   \
   hipLaunchKernelGGL((k_heun<R, CM, GEO, 1, true>), g, b, 0, s, S, d, E, ep, k1, step, nz); \

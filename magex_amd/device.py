@@ -208,6 +208,16 @@ class DeviceModel:
         """Steps per captured hipGraph step loop (0: launch every step directly)."""
         check(self.lib.mx_model_set_graph_steps(self.h, int(n)))
 
+    def set_step_mode(self, mode):
+        """0: two kernel launches per Heun step; 1 (default): one launch where the model allows it."""
+        check(self.lib.mx_model_set_step_mode(self.h, int(mode)))
+
+    def step_info(self):
+        """(one launch per step?, waits of one-launch steps given up since creation; synchronises)."""
+        one, gu = ctypes.c_int32(), ctypes.c_int64()
+        check(self.lib.mx_model_step_info(self.h, ctypes.byref(one), ctypes.byref(gu)))
+        return bool(one.value), int(gu.value)
+
     # ------------------------------------------------------------------ kernels
     def _check_spins(self, e):
         if e.shape != (self.N, 3) or e.dtype != self.dtype or not e.is_contiguous() or e.device.type != "cuda":
