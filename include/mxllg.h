@@ -119,6 +119,19 @@ int mx_llg_run(mx_model* m, void* e, int64_t nsteps, uint64_t seed, uint64_t ste
 int mx_llg_run_driven(mx_model* m, void* e, int64_t nsteps, const double* B, uint64_t seed, uint64_t step0,
                       void* stream);
 
+/* One stage of a Heun (or Depondt) step with a caller-supplied field, for fields the host evaluates
+ * between the stages -- magex's unfused fast path with `extra` (magex/fast.py:202-223, 250-269, as
+ * LLGSolver._fast_step drives it with extra_field(e) and an (N, 3) field_func(t), magex/llg.py:152-166,
+ * 197-212).  stage 1 (predictor): thermal noise of (seed, step) + field(e) + predictor, kept in the
+ * model; if ep_out != NULL the predictor spins are written there (device, N x 3, magex order) for the
+ * caller to evaluate the corrector's field.  stage 2 (corrector): field(predictor) + corrector, e updated
+ * in place.  B (host, 3 doubles, NULL = the model's field) is the uniform field of the stage; extra
+ * (device, N x 3 in the model precision, magex order, NULL = none) is added to every site's field after
+ * the uniform field and before the thermal field, as magex adds `extra`.  Pinned sites do not move.
+ * Stage 2 must follow stage 1 on the same model and spins with no other stepping call in between. */
+int mx_llg_stage(mx_model* m, void* e, int32_t stage, const double* B, const void* extra, uint64_t seed,
+                 uint64_t step, void* ep_out, void* stream);
+
 /* Spin-orbit torque as effective fields on the dynamics (not on mx_field / mx_observables, as in
  * magex): B += B_DL (e x p) + B_FL p, p a unit vector (magex/llg.py:83-86, 126-128).  B_DL = B_FL = 0
  * switches it off. */

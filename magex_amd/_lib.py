@@ -45,6 +45,7 @@ SIGNATURES = {
     "mx_model_set_step_mode": (_I, [_P, ctypes.c_int32]),
     "mx_model_step_info": (_I, [_P, _P, _P]),
     "mx_llg_run_driven": (_I, [_P, _P, _I64, _P, _U64, _U64, _P]),
+    "mx_llg_stage": (_I, [_P, _P, ctypes.c_int32, _P, _P, _U64, _U64, _P, _P]),
     "mx_model_set_sot": (_I, [_P, _P, _D, _D]),
     "mx_mc_set_colors": (_I, [_P, ctypes.c_int32, _P, _P]),
     "mx_mc_set_temperature": (_I, [_P, _I, _P]),

@@ -782,6 +782,32 @@ int mx_llg_run_driven(mx_model* m, void* e, int64_t nsteps, const double* B, uin
   });
 }
 
+int mx_llg_stage(mx_model* m, void* e, int32_t stage, const double* B, const void* extra, uint64_t seed,
+                 uint64_t step, void* ep_out, void* stream) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (!e) fail(MX_EINVAL, "null spins");
+    if (stage != 1 && stage != 2) fail(MX_EINVAL, "stage must be 1 (predictor) or 2 (corrector)");
+    if (!m->dyn_set) fail(MX_EINVAL, "set the dynamics first");
+    use_device(*m);
+    ensure_scratch(*m);
+    sync_static(*m);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const bool perm = needs_permute(*m);
+    void* E = perm ? m->eint_d : e;
+    int rc = MX_OK;
+    if (perm && stage == 1) rc = launch_permute(*m, e, m->eint_d, 1, s);
+    if (rc == MX_OK) rc = launch_heun_pass(*m, stage, E, m->ep_d, m->k1_d, seed, step, nullptr, nullptr, s, B, extra);
+    if (rc == MX_OK && stage == 1 && ep_out) {
+      if (perm) rc = launch_permute(*m, m->ep_d, ep_out, 0, s);
+      else hip_ok(hipMemcpyAsync(ep_out, m->ep_d, size_t(m->N) * 3 * rsize(*m), hipMemcpyDeviceToDevice, s),
+                  "hipMemcpyAsync predictor spins");
+    }
+    if (rc == MX_OK && stage == 2 && perm) rc = launch_permute(*m, m->eint_d, e, 0, s);
+    return rc;
+  });
+}
+
 int mx_model_set_sot(mx_model* m, const double p[3], double B_DL, double B_FL) {
   return guarded([&]() -> int {
     need_model(m);

@@ -186,9 +186,11 @@ bool fused_step(const Model& m, int* nb = nullptr);
 // Device words of the one-launch step (allocated and zeroed with the model's scratch).
 size_t stepsync_words(const Model& m);
 // One Heun pass (1 or 2) with optional stores of e and B_eff (magex order is not used: internal order).
-// B_ext (3 doubles) replaces the model's uniform field for this pass when non-null.
+// B_ext (3 doubles) replaces the model's uniform field for this pass when non-null; extra (device, N x 3
+// in the model precision, magex site order) is added to the field of every site when non-null.
 int launch_heun_pass(const Model& m, int pass, void* e, void* ep, void* k1, uint64_t seed, uint64_t step,
-                     void* save_e, void* save_B, hipStream_t s, const double* B_ext = nullptr);
+                     void* save_e, void* save_B, hipStream_t s, const double* B_ext = nullptr,
+                     const void* extra = nullptr);
 int sync_model_static(Model& m);
 size_t prec_size(const Model& m);
 // nimg images of N x 3 values each, one after another (field: in and out; observables: nimg x MX_OBS_LEN

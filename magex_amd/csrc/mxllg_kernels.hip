@@ -746,7 +746,7 @@ int launch_heun_step(const Model& m, void* e, uint64_t seed, uint64_t step, cons
 
 template <typename R>
 static int pass_t(const Model& m, int pass, void* e, void* ep, void* k1, uint64_t seed, uint64_t step, void* save_e,
-                  void* save_B, hipStream_t s, const double* B_ext) {
+                  void* save_B, hipStream_t s, const double* B_ext, const void* extra) {
   const Static<R>* S = static_cast<const Static<R>*>(m.static_d);
   Dyn<R> d = make_dyn<R>(m, seed);
   if (B_ext) {
@@ -754,6 +754,7 @@ static int pass_t(const Model& m, int pass, void* e, void* ep, void* k1, uint64_
     d.B1 = R(B_ext[1]);
     d.B2 = R(B_ext[2]);
   }
+  d.extra = static_cast<const R*>(extra);
   d.save_e = static_cast<R*>(save_e);
   d.save_B = static_cast<R*>(save_B);
   R* E = static_cast<R*>(e);
@@ -770,9 +771,9 @@ static int pass_t(const Model& m, int pass, void* e, void* ep, void* k1, uint64_
 }
 
 int launch_heun_pass(const Model& m, int pass, void* e, void* ep, void* k1, uint64_t seed, uint64_t step,
-                     void* save_e, void* save_B, hipStream_t s, const double* B_ext) {
-  return m.prec == MX_FP64 ? pass_t<double>(m, pass, e, ep, k1, seed, step, save_e, save_B, s, B_ext)
-                           : pass_t<float>(m, pass, e, ep, k1, seed, step, save_e, save_B, s, B_ext);
+                     void* save_e, void* save_B, hipStream_t s, const double* B_ext, const void* extra) {
+  return m.prec == MX_FP64 ? pass_t<double>(m, pass, e, ep, k1, seed, step, save_e, save_B, s, B_ext, extra)
+                           : pass_t<float>(m, pass, e, ep, k1, seed, step, save_e, save_B, s, B_ext, extra);
 }
 
 int launch_counter(uint64_t* c, uint64_t value, int add, hipStream_t s) {

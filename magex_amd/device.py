@@ -244,6 +244,21 @@ class DeviceModel:
                                          _stream()))
         return e
 
+    def stage(self, e, stage, B, extra, seed, step, ep_out=None):
+        """One stage (1: predictor, 2: corrector) of a step on e with the uniform field B ((3,) or None for
+        the model's) and the per-site field extra (device (N, 3) tensor of the model dtype, or None);
+        stage 1 writes the predictor spins into ep_out when given."""
+        self._check_spins(e)
+        for t in (extra, ep_out):
+            if t is not None:
+                self._check_spins(t)
+        Bh = None if B is None else np.ascontiguousarray(np.asarray(B, float).reshape(3))
+        check(self.lib.mx_llg_stage(self.h, ctypes.c_void_p(e.data_ptr()), int(stage), hptr(Bh),
+                                    None if extra is None else ctypes.c_void_p(extra.data_ptr()),
+                                    int(seed) & 0xFFFFFFFFFFFFFFFF, int(step),
+                                    None if ep_out is None else ctypes.c_void_p(ep_out.data_ptr()), _stream()))
+        return e
+
     def set_sot(self, p, B_DL, B_FL):
         """Spin-orbit torque fields B_DL (e x p) + B_FL p on the dynamics (p a unit vector)."""
         check(self.lib.mx_model_set_sot(self.h, hptr(np.ascontiguousarray(np.asarray(p, float).reshape(3))),

@@ -52,10 +52,12 @@ class LLGSolver:
     Same constructor as magex's LLGSolver plus ``precision`` ("fp64", magex's arithmetic, or "fp32").
     backend: "auto", "hip" or "numba" (all mean the device kernels); "numpy" raises: this package
     has no CPU path.  On the device path: the Heun and Depondt integrators, the classical thermostat,
-    spin-orbit torque fields (sot) and a uniform time-dependent ``field_func(t) -> (3,)``, evaluated
-    at t in the predictor and t + dt in the corrector as magex does.  ValueError for an unknown
-    integrator; NotImplementedError for rk4 / sib / euler, STT, the quantum thermostat, a
-    site-dependent field_func and extra_field.
+    spin-orbit torque fields (sot), ``field_func(t) -> (3,) or (N, 3)`` evaluated at t in the predictor
+    and t + dt in the corrector, and ``extra_field(e) -> (N, 3)`` evaluated at the spins of each stage,
+    as magex's fast path does (magex/llg.py:152-166, 197-212).  extra_field receives a NumPy (N, 3) array,
+    as in magex; with ``extra_field_device = True`` it receives the device tensor instead and may return a
+    tensor.  ValueError for an unknown integrator; NotImplementedError for rk4 / sib / euler, STT and the
+    quantum thermostat.
     """
 
     def __init__(self, system, alpha=0.1, dt=1e-3, temperature=0.0, seed=None, precession=True,
@@ -92,6 +94,7 @@ class LLGSolver:
         self.time = 0.0
         self.field_func = None
         self.extra_field = None
+        self.extra_field_device = False
         self.stt = None
         # the noise key: the seed itself, or 64 bits drawn from fresh entropy
         self.noise_seed = int(seed) if seed is not None else int(self.rng.integers(0, 2**63))
@@ -100,8 +103,6 @@ class LLGSolver:
 
     # ------------------------------------------------------------------ device plumbing
     def _sync(self):
-        if self.extra_field is not None:
-            raise NotImplementedError("extra_field (an arbitrary function of the spins) is not on the device path")
         self.dm = self.sys.device_model(self.precision)
         p, bdl, bfl = self.sot if self.sot is not None else ((0.0, 1.0, 0.0), 0.0, 0.0)
         self.dm.set_sot(p, bdl, bfl)
@@ -133,22 +134,55 @@ class LLGSolver:
             b[self.sys.pinned] = 0.0
         return b
 
+    def _stage_fields(self, x, t):
+        """Uniform field (3,) and per-site extra field (device (N, 3) or None) of the stage evaluated at
+        spins x (device) and time t: field_func(t) (uniform or per site) plus extra_field(x)
+        (magex/llg.py:152-166)."""
+        from .device import as_device
+        B = np.asarray(self.sys.B, float).reshape(3).copy()
+        X = None
+        if self.field_func is not None:
+            f = np.asarray(self.field_func(t), float)
+            if f.size == 3:
+                B += f.reshape(3)
+            else:
+                X = as_device(np.array(np.broadcast_to(f, (self.sys.N, 3))), self.dm.dtype)
+        if self.extra_field is not None:
+            arg = x if self.extra_field_device else x.cpu().numpy().astype(float)
+            xe = as_device(self.extra_field(arg), self.dm.dtype)
+            if tuple(xe.shape) != (self.sys.N, 3):
+                raise ValueError(f"extra_field must return an ({self.sys.N}, 3) field")
+            X = xe if X is None else X + xe
+        return B, (None if X is None else X.contiguous())
+
     def step_device(self, e_dev, nsteps=1, noise=None):
         """Advance a device tensor in place by nsteps (the CudaLLG.step_device shape,
         magex/cuda_backend.py:362-369).  noise: injected (nsteps, N, 3) standard normals or None."""
         dm = self._sync()
-        if self.field_func is not None:
+        site_ff = False
+        if self.field_func is not None and self.extra_field is None:
+            site_ff = np.asarray(self.field_func(self.time), float).size != 3
+        if self.extra_field is not None or site_ff:
+            # fields that depend on the spins or on the site: evaluated on the host between the stages
+            if noise is not None:
+                raise NotImplementedError("injected noise together with extra_field or a per-site field_func")
+            import torch
+            ep = torch.empty_like(e_dev)
+            for n in range(int(nsteps)):
+                t = self.time + n * self.dt
+                step = self.step_count + n
+                B1, X1 = self._stage_fields(e_dev, t)
+                dm.stage(e_dev, 1, B1, X1, self.noise_seed, step, ep)
+                B2, X2 = self._stage_fields(ep, t + self.dt)
+                dm.stage(e_dev, 2, B2, X2, self.noise_seed, step)
+        elif self.field_func is not None:
             if noise is not None:
                 raise NotImplementedError("injected noise together with field_func")
             t = self.time + self.dt * np.arange(int(nsteps))
             B = np.empty((int(nsteps), 2, 3))
             for n, tn in enumerate(t):
                 for k, tk in enumerate((tn, tn + self.dt)):
-                    f = np.asarray(self.field_func(tk), float)
-                    if f.size != 3:
-                        raise NotImplementedError("a site-dependent field_func is not on the device path "
-                                                  "(return a uniform (3,) field)")
-                    B[n, k] = self.sys.B + f.reshape(3)
+                    B[n, k] = self.sys.B + np.asarray(self.field_func(tk), float).reshape(3)
             dm.run_driven(e_dev, B, self.noise_seed, self.step_count)
         else:
             dm.run(e_dev, nsteps, self.noise_seed, self.step_count, noise)
@@ -157,25 +191,25 @@ class LLGSolver:
         return e_dev
 
     def field(self, e, t=None):
-        """B_eff + spin-orbit torque fields + field_func(t) (T), zero on pinned sites (magex/llg.py:124-136);
-        numpy in -> numpy out, device tensors stay on the device.  t defaults to the solver's time."""
+        """B_eff + spin-orbit torque fields + field_func(t) + extra_field(e) (T), zero on pinned sites
+        (magex/llg.py:124-136); numpy in -> numpy out, device tensors stay on the device.  t defaults to
+        the solver's time."""
         from . import _lib
-        from .device import like_input
-        if self.extra_field is not None:
-            raise NotImplementedError("extra_field (an arbitrary function of the spins) is not on the device path")
+        from .device import as_device, like_input
         dm = self._sync()
         ed = self._to_device(e)
         B = dm.field(ed, _lib.MX_FIELD_BEFF)          # B_eff from the kernels, zero on pinned sites
-        extra = self.sot is not None or self.field_func is not None
+        extra = self.sot is not None or self.field_func is not None or self.extra_field is not None
         if self.sot is not None:
             p, bdl, bfl = self.sot
             pt = B.new_tensor(p)
             B = B + bdl * ed.cross(pt.expand_as(ed), dim=1) + bfl * pt
         if self.field_func is not None:
             f = np.asarray(self.field_func(self.time if t is None else t), float)
-            if f.size != 3:
-                raise NotImplementedError("a site-dependent field_func is not on the device path")
-            B = B + B.new_tensor(f.reshape(3))
+            B = B + B.new_tensor(f.reshape(3) if f.size == 3 else np.array(np.broadcast_to(f, (self.sys.N, 3))))
+        if self.extra_field is not None:
+            arg = ed if self.extra_field_device else ed.cpu().numpy().astype(float)
+            B = B + as_device(self.extra_field(arg), B.dtype)
         if extra and self.sys.pinned is not None:
             B[_pinned_mask(self.sys, B)] = 0.0
         return like_input(B, e)

@@ -9,9 +9,11 @@ Pinning (what each part is checked against, see tests/test_oracle.py):
   exchange_field / gradient / energy               -> golden vectors + central finite differences
   CubicAnisotropy / UniaxialHigher                 -> closed-form energies on symmetry directions,
                                                       central differences of their energies
-  solver_field (spin-orbit torque, field_func)     -> closed-form single-spin solutions: precession
-                                                      phase under B_z(t); damping-like relaxation
-                                                      tan(theta/2) = tan(theta0/2) exp(-gamma B_DL t)
+  solver_field (spin-orbit torque, field_func,     -> closed-form single-spin solutions: precession
+               extra_field)                           phase under B_z(t); damping-like relaxation
+                                                      tan(theta/2) = tan(theta0/2) exp(-gamma B_DL t);
+                                                      an extra_field equal to an on-site tensor's field
+                                                      follows the model with that tensor
   heun_step                                        -> golden trajectories (T=0 and injected noise),
                                                       Larmor closed form, damped-spin closed form
   rotate / depondt_step                            -> Rodrigues closed form (rotation about an axis),
@@ -323,9 +325,10 @@ def sigma(system, alpha, T, dt):
     return np.broadcast_to(s, (system.N,)).astype(float)
 
 
-def solver_field(system, sot=None, field_func=None):          # llg.py:124-136
-    """LLGSolver.field: B_eff + spin-orbit torque fields + field_func(t), zero on pinned sites when any
-    of the extras is present.  sot = (p unit vector, B_DL, B_FL)."""
+def solver_field(system, sot=None, field_func=None, extra_field=None):   # llg.py:124-136
+    """LLGSolver.field: B_eff + spin-orbit torque fields + field_func(t) ((3,) or (N, 3)) + extra_field(e)
+    ((N, 3), a function of the spins), zero on pinned sites when any of the extras is present.
+    sot = (p unit vector, B_DL, B_FL)."""
     def field(e, t):
         B = system.effective_field(e)
         if sot is not None:
@@ -333,7 +336,9 @@ def solver_field(system, sot=None, field_func=None):          # llg.py:124-136
             B = B + bdl * np.cross(e, p[None, :]) + bfl * p[None, :]
         if field_func is not None:
             B = B + field_func(t)
-        if system.pinned is not None and (sot is not None or field_func is not None):
+        if extra_field is not None:
+            B = B + extra_field(e)
+        if system.pinned is not None and (sot is not None or field_func is not None or extra_field is not None):
             B[system.pinned] = 0.0
         return B
     return field
@@ -370,11 +375,12 @@ def depondt_step(system, e, alpha, dt, b=0.0, precession=True, field=None, t=0.0
 
 
 def run_heun(system, e, steps, alpha, dt, T=0.0, normals=None, precession=True, integrator="heun", sot=None,
-             field_func=None):
+             field_func=None, extra_field=None):
     """steps Heun (or Depondt) steps from t = 0; normals(n) -> (N,3) standard normals of step n
-    (injected noise); sot / field_func as solver_field."""
+    (injected noise); sot / field_func / extra_field as solver_field."""
     step = {"heun": heun_step, "depondt": depondt_step}[integrator]
-    field = solver_field(system, sot, field_func) if (sot is not None or field_func is not None) else None
+    extras = sot is not None or field_func is not None or extra_field is not None
+    field = solver_field(system, sot, field_func, extra_field) if extras else None
     sg = sigma(system, alpha, T, dt) if (T > 0 or system.site_T is not None) else None
     for n in range(steps):
         b = 0.0

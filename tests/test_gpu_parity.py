@@ -426,6 +426,58 @@ def test_sot_and_field_func_match_oracle(gpu, integrator):
     assert abs(sol.time - 50 * 5e-4) < 1e-15
 
 
+def _spin_dependent_extra(A, mu):
+    """A field that depends on the spins (that of an on-site tensor A, scaled) and a radial pulse."""
+    def xf(e):
+        return -1.5 * np.einsum("iab,ib->ia", A, np.asarray(e, float)) / (R.MU_B * mu[:, None])
+    return xf
+
+
+@pytest.mark.parametrize("integrator", ["heun", "depondt"])
+@pytest.mark.parametrize("layout", ["one-site", "two-site"])
+def test_extra_field_and_site_field_func_match_oracle(gpu, integrator, layout):
+    """extra_field(e) (evaluated at the step's spins in the predictor and at the predictor spins in the
+    corrector) and a site-dependent field_func(t) -> (N, 3), with spin-orbit torque, pinned sites and
+    thermal noise, against the oracle within 1e-11 (FP64); LLGSolver.field includes both.  The two-site
+    case runs through the internal sublattice-major order (permuted in and out of each stage)."""
+    print(f"case: extra_field + (N,3) field_func {integrator} {layout}", flush=True)
+    if layout == "one-site":
+        s, o = _pair((10, 8, 1), (0, 0, 0.3), lattice="square", J=[2.0, 0.3], D=0.8, K=0.1)
+    else:
+        s, o = _pair((4, 4, 1), (0, 0, 0.3), lattice="honeycomb", J=[2.0, 0.3], K=0.1)
+    rng = np.random.default_rng(9)
+    pin = rng.random(s.N) < 0.12
+    s.pinned = pin
+    o.pinned = pin
+    A = rng.normal(0.0, 0.2, (s.N, 3, 3))
+    A = A + A.transpose(0, 2, 1)
+    xf = _spin_dependent_extra(A, np.asarray(o.mu, float))
+    prof = np.exp(-((np.arange(s.N) - s.N / 2) / (0.3 * s.N)) ** 2)
+    ff = lambda t: np.outer(prof, [0.5 * np.cos(30.0 * t), 0.0, 0.2])                      # noqa: E731
+    sot = {"p": [0.0, 1.0, 1.0], "B_DL": 0.4, "B_FL": -0.1}
+    sol = LLGSolver(s, alpha=0.1, dt=5e-4, temperature=8.0, seed=17, integrator=integrator, sot=sot)
+    sol.field_func = ff
+    sol.extra_field = xf
+    e0 = s.init_spins("random", seed=6)
+    e, _, _ = sol.run(e0, 40, output_every=40)
+    dm = s.device_model("fp64")
+    normals = [dm.noise(17, n).cpu().numpy() for n in range(40)]
+    p = np.array(sot["p"]) / np.linalg.norm(sot["p"])
+    ref = R.run_heun(o, e0.copy(), 40, 0.1, 5e-4, T=8.0, normals=lambda n: normals[n], integrator=integrator,
+                     sot=(p, sot["B_DL"], sot["B_FL"]), field_func=ff, extra_field=xf)
+    assert np.array_equal(e[pin], e0[pin])
+    assert _rel(e, ref) < 1e-11
+    ref_field = R.solver_field(o, (p, sot["B_DL"], sot["B_FL"]), ff, xf)
+    assert _rel(sol.field(e, 0.01), ref_field(e, 0.01)) < 1e-13
+    # the device-tensor form of the callable gives the same trajectory
+    sol2 = LLGSolver(s, alpha=0.1, dt=5e-4, temperature=8.0, seed=17, integrator=integrator, sot=sot)
+    sol2.field_func = ff
+    sol2.extra_field = lambda ed: gpu.as_tensor(xf(ed.cpu().numpy()), device="cuda")
+    sol2.extra_field_device = True
+    e2, _, _ = sol2.run(e0, 40, output_every=40)
+    assert np.array_equal(e2, e)
+
+
 @pytest.mark.parametrize("per_site_alpha", [False, True])
 def test_solver_field_and_rhs_match_oracle(gpu, per_site_alpha):
     """LLGSolver.field(e, t) and .rhs(e, b, t) (magex/llg.py:124-150) with spin-orbit torque fields, a

@@ -600,6 +600,45 @@ def test_sot_damping_like_closed_form():
     assert abs(e[0] @ p - np.cos(th0)) < 1e-10
 
 
+@pytest.mark.parametrize("integrator", ["heun", "depondt"])
+def test_extra_field_reproducing_an_onsite_tensor(integrator):
+    """An extra_field(e) = -2 A e / (mu mu_B) is, field for field, the on-site tensor A: a model without
+    anisotropy driven by it follows the model with A (thermal, pinned sites, both integrators).  The same
+    extra field evaluated at the step's input spins in the corrector (instead of the predictor spins) is
+    a first-order error that the check catches; an (N, 3) field_func with equal rows is the uniform one."""
+    rng = np.random.default_rng(3)
+    sK = _sys((4, 4, 2), field=(0, 0, 0.3), lattice="sc", J=[1.0, 0.1], K=0.4, moment=2.0)
+    s0 = _sys((4, 4, 2), field=(0, 0, 0.3), lattice="sc", J=[1.0, 0.1], K=0.0, moment=2.0)
+    pinned = rng.random(sK.N) < 0.15
+    sK.pinned = s0.pinned = pinned
+    A = sK.A.copy()
+    xf = lambda e: -2.0 * np.einsum("iab,ib->ia", A, e) / (R.MU_B * s0.mu[:, None])    # noqa: E731
+    e0 = R.normalize(rng.normal(size=(sK.N, 3)))
+    normals = rng.normal(size=(25, sK.N, 3))
+    dt = 2e-3
+    pulse = lambda t: np.array([0.2 * np.sin(40 * t), 0.0, 0.1])                      # noqa: E731
+    ref = R.run_heun(sK, e0.copy(), 25, 0.1, dt, T=30.0, normals=lambda n: normals[n], integrator=integrator,
+                     field_func=pulse)
+    got = R.run_heun(s0, e0.copy(), 25, 0.1, dt, T=30.0, normals=lambda n: normals[n], integrator=integrator,
+                     field_func=pulse, extra_field=xf)
+    assert np.abs(got - ref).max() < 1e-13
+    assert np.abs(got[pinned] - e0[pinned]).max() < 1e-14      # renormalised, as magex's unfused path does
+    rows = lambda t: np.tile(pulse(t), (s0.N, 1))                                      # noqa: E731
+    got_rows = R.run_heun(s0, e0.copy(), 25, 0.1, dt, T=30.0, normals=lambda n: normals[n],
+                          integrator=integrator, field_func=rows, extra_field=xf)
+    assert np.abs(got_rows - got).max() < 1e-14
+    step = R.heun_step if integrator == "heun" else R.depondt_step
+    sg = R.sigma(s0, 0.1, 30.0, dt)
+    stale = e0.copy()
+    for n in range(25):
+        b = sg[:, None] * normals[n]
+        b[pinned] = 0.0
+        x_in = stale.copy()
+        fld = R.solver_field(s0, None, pulse, lambda _e: xf(x_in))
+        stale = step(s0, stale, 0.1, dt, b, True, fld, n * dt)
+    assert np.abs(stale - ref).max() > 1e-9
+
+
 # ------------------------------------------------------------------ Monte Carlo restatement
 def _langevin(x):
     return 1 / np.tanh(x) - 1 / x
