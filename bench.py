@@ -110,7 +110,8 @@ def fp32_angle_error(system, p, steps=200, start="random", control=False):
     out = {}
     for key, x0 in starts.items():
         prec = "fp32" if key == "fp32" else "fp64"
-        sol = LLGSolver(system, p["alpha"], p["dt"], p["temperature"], p["seed"], precision=prec)
+        # the FP64 runs draw the FP32 run's (float-grade) normals: identical noise
+        sol = LLGSolver(system, p["alpha"], p["dt"], p["temperature"], p["seed"], precision=prec, noise="float")
         e = torch.as_tensor(x0, device="cuda", dtype=torch.float32 if prec == "fp32" else torch.float64).contiguous()
         sol.step_device(e, steps)
         out[key] = e.double()
@@ -294,7 +295,7 @@ def batch_rates(steps, per_replica_steps):
     mags = {}
     nsamp = 40
     for prec in ("fp32", "fp64"):
-        bp = ReplicaBatch(systems, alpha, p["dt"], T, seeds, prec)
+        bp = ReplicaBatch(systems, alpha, p["dt"], T, seeds, prec, noise="float")
         ep = bp.pack(spins)
         acc = np.zeros(len(systems))
         for _ in range(nsamp):

@@ -36,6 +36,7 @@ enum { MX_FP64 = 0, MX_FP32 = 1 };
 enum { MX_FIELD_BEFF = 0, MX_FIELD_GRADIENT = 1, MX_FIELD_EXCHANGE = 2 };
 enum { MX_HEUN = 0, MX_DEPONDT = 1 };
 enum { MX_OBS_E = 0, MX_OBS_M = 1, MX_OBS_ORDER = 4, MX_OBS_TORQUE = 7, MX_OBS_LEN = 8 };
+enum { MX_NOISE_FLOAT = 0, MX_NOISE_DOUBLE = 1 };
 
 typedef struct mx_model mx_model;
 
@@ -46,7 +47,9 @@ int mx_version(void);
 /* Model of a periodic (or open) supercell of a unit cell with nsub magnetic sites.
  * Replaces the array set-up of fast.FastModel (magex/fast.py:630-672) and cuda_backend._Device
  * (magex/cuda_backend.py:307-335); the 3N x 3N CSR of SpinSystem (spinmodel.py:211-241) is never built:
- * couplings are a stencil over cells.  L: supercell (cells per axis); pbc: periodic flags. */
+ * couplings are a stencil over cells.  L: supercell (cells per axis, each at most 32767); pbc: periodic
+ * flags.  At most 238 609 293 sites ((2^31 - 1) / 9: the kernels index per-site arrays of up to 9 values
+ * per site with 32-bit offsets); larger models return MX_EUNSUPPORTED. */
 int mx_model_create(int precision, int nsub, const int32_t L[3], const int32_t pbc[3], mx_model** out);
 void mx_model_destroy(mx_model* m);
 int mx_model_info(const mx_model* m, int64_t* n_sites, int32_t* precision, int32_t* nsub);
@@ -197,6 +200,15 @@ int mx_model_set_step_mode(mx_model* m, int32_t mode);
  * given_up: waits of one-launch steps that timed out since the model was created, each finished by the
  * launch's last workgroup instead (0 in normal operation; synchronises the device).  Either may be NULL. */
 int mx_model_step_info(mx_model* m, int32_t* one_launch, int64_t* given_up);
+
+/* Grade of the thermal-noise generator (the counter-based stream that replaces magex's
+ * PCG64 / ziggurat draws, magex/llg.py:119, fast.py:123-200): MX_NOISE_DOUBLE (default for MX_FP64) draws
+ * 53-bit uniforms from two Philox4x32-10 outputs and applies Box-Muller in double precision;
+ * MX_NOISE_FLOAT (default for MX_FP32) draws 24-bit uniforms from one output with float Box-Muller.  An FP64
+ * model set to MX_NOISE_FLOAT sees exactly the noise of an FP32 model with the same seed (for FP32
+ * accuracy measurements against FP64 on identical noise).  Also governs mx_noise, the batched form
+ * (from its prototype) and the adjoint's replays. */
+int mx_model_set_noise(mx_model* m, int32_t grade);
 
 /* The standard normals mx_llg_run draws for (seed, step): out (device, N x 3). */
 int mx_noise(mx_model* m, void* out, uint64_t seed, uint64_t step, void* stream);

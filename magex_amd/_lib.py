@@ -16,6 +16,7 @@ MX_FP64, MX_FP32 = 0, 1
 MX_FIELD_BEFF, MX_FIELD_GRADIENT, MX_FIELD_EXCHANGE = 0, 1, 2
 MX_HEUN, MX_DEPONDT = 0, 1
 MX_OBS_LEN = 8
+MX_NOISE_FLOAT, MX_NOISE_DOUBLE = 0, 1
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -43,6 +44,7 @@ SIGNATURES = {
     "mx_model_set_dynamics": (_I, [_P, _D, _I, _P, _I, _P, _I, _I]),
     "mx_model_set_graph_steps": (_I, [_P, ctypes.c_int32]),
     "mx_model_set_step_mode": (_I, [_P, ctypes.c_int32]),
+    "mx_model_set_noise": (_I, [_P, ctypes.c_int32]),
     "mx_model_step_info": (_I, [_P, _P, _P]),
     "mx_llg_run_driven": (_I, [_P, _P, _I64, _P, _U64, _U64, _P]),
     "mx_llg_stage": (_I, [_P, _P, ctypes.c_int32, _P, _P, _U64, _U64, _P, _P]),

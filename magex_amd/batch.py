@@ -30,7 +30,7 @@ _TUNED = {}
 
 class ReplicaBatch:
     def __init__(self, systems, alpha=0.1, dt=1e-3, temperatures=0.0, seeds=None, precision="fp64",
-                 precession=True):
+                 precession=True, noise=None):
         torch = _torch()
         systems = list(systems)
         if not systems:
@@ -50,6 +50,9 @@ class ReplicaBatch:
         self.dtype = torch.float64 if precision == "fp64" else torch.float32
         self.lib = _lib.lib()
         self._proto = proto.device_model(precision)
+        # generator grade: the precision's own unless given ("float" makes an FP64 batch draw exactly the
+        # noise of the FP32 one)
+        self._proto.set_noise(noise or ("double" if precision == "fp64" else "float"))
         L = np.ascontiguousarray([s.L for s in systems], np.int32)
         h = ctypes.c_void_p()
         check(self.lib.mx_batch_create(self._proto.h, self.n, hptr(L), ctypes.byref(h)))

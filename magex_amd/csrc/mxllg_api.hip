@@ -47,14 +47,24 @@ static int guarded(F&& f) {
 
 static size_t rsize(const Model& m) { return m.prec == MX_FP64 ? sizeof(double) : sizeof(float); }
 
+// Set-up calls replace device buffers and constant blocks that kernels already queued on the caller's
+// stream may still read (the stream may be non-blocking, so a copy on the null stream does not wait for
+// it): the device finishes its queued work first.  Only configuration changes pay for this; the step
+// loop does not.
+static void quiesce() { hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+
 static void dfree(void*& p) {
-  if (p) (void)hipFree(p);
+  if (p) {
+    quiesce();
+    (void)hipFree(p);
+  }
   p = nullptr;
 }
 
 // Device copy of n host doubles, converted to the model precision.  perm: reorder a magex-order
 // per-site array of n = N * comps values (site-major) into internal component-major order.
 static void upload(const Model& m, void*& dst, const double* src, size_t n) {
+  quiesce();
   dfree(dst);
   hip_ok(hipMalloc(&dst, n * rsize(m)), "hipMalloc");
   if (m.prec == MX_FP64) {
@@ -162,9 +172,11 @@ static void use_device(const Model& m) { hip_ok(hipSetDevice(m.device), "hipSetD
 int sync_model_static(Model& m) { return m.static_dirty ? upload_static(m) : MX_OK; }
 size_t prec_size(const Model& m) { return rsize(m); }
 
-// Bring the device constant block up to date (synchronous copy; only after configuration changes).
+// Bring the device constant block up to date (synchronous copy after the device's queued work, which may
+// still read the block; only after configuration changes).
 static void sync_static(Model& m) {
   if (!m.static_dirty) return;
+  quiesce();
   const int rc = upload_static(m);
   if (rc) fail(rc, "model constants: " + std::string(mx_last_error_text()));
 }
@@ -244,9 +256,12 @@ int mx_model_create(int precision, int nsub, const int32_t L[3], const int32_t p
       if (L[a] < 1 || L[a] > 32767) fail(MX_EINVAL, "supercell extents must lie in [1, 32767]");
       n *= L[a];
     }
-    if (n > (int64_t(1) << 30)) fail(MX_EUNSUPPORTED, "more than 2^30 sites");
+    if (n > MAX_SITES)
+      fail(MX_EUNSUPPORTED, "more than " + std::to_string(MAX_SITES) +
+                                " sites (per-site offsets up to 9 N are 32-bit in the kernels)");
     auto* m = new mx_model();
     m->prec = precision;
+    m->noise_dbl = precision == MX_FP64 ? 1 : 0;
     hip_ok(hipGetDevice(&m->device), "hipGetDevice");
     m->ns = nsub;
     for (int a = 0; a < 3; ++a) {
@@ -581,6 +596,7 @@ int mx_model_set_pinned(mx_model* m, const uint8_t* pinned) {
       m->static_dirty = 1;
       return MX_OK;
     }
+    quiesce();
     hip_ok(hipMalloc(&m->pinned_d, m->N), "hipMalloc pinned");
     hip_ok(hipMemcpy(m->pinned_d, v.data(), m->N, hipMemcpyHostToDevice), "hipMemcpy pinned");
     m->static_dirty = 1;
@@ -682,6 +698,21 @@ int mx_model_set_graph_steps(mx_model* m, int32_t steps) {
       drop_graph(*m);
     }
     m->graph_steps = steps;
+    return MX_OK;
+  });
+}
+
+int mx_model_set_noise(mx_model* m, int32_t grade) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (grade != MX_NOISE_FLOAT && grade != MX_NOISE_DOUBLE)
+      fail(MX_EINVAL, "noise grade must be MX_NOISE_FLOAT or MX_NOISE_DOUBLE");
+    if (grade != m->noise_dbl) {
+      m->noise_dbl = grade;
+      m->static_dirty = 1;
+      use_device(*m);
+      drop_graph(*m);
+    }
     return MX_OK;
   });
 }

@@ -60,6 +60,7 @@ template <typename R>
 struct BDyn {
   R B0, B1, B2, dt;
   int prec, noisy;
+  int noise_dbl;  // the prototype model's generator grade
 };
 
 struct BSite {
@@ -211,12 +212,20 @@ __global__ __launch_bounds__(BS) void k_bheun(const BStatic<R>* __restrict__ S, 
     }
     bexchange<R, CM>(S, rp, re, src, st, hx, hy, hz, [&] {
       if (PASS == 1 && d.noisy) {
-        float n0, n1, n2;
-        gauss3(rp->seed_lo, rp->seed_hi, step, uint32_t(st.iref), n0, n1, n2);
         const R sg = R(rp->sig[st.s]);
-        nx = sg * R(n0);
-        ny = sg * R(n1);
-        nz = sg * R(n2);
+        if (d.noise_dbl) {
+          double n0, n1, n2;
+          gauss3d(rp->seed_lo, rp->seed_hi, step, uint32_t(st.iref), n0, n1, n2);
+          nx = sg * R(n0);
+          ny = sg * R(n1);
+          nz = sg * R(n2);
+        } else {
+          float n0, n1, n2;
+          gauss3(rp->seed_lo, rp->seed_hi, step, uint32_t(st.iref), n0, n1, n2);
+          nx = sg * R(n0);
+          ny = sg * R(n1);
+          nz = sg * R(n2);
+        }
         bnp[0] = nx;
         bnp[1] = ny;
         bnp[2] = nz;
@@ -327,6 +336,7 @@ struct mx_batch {
   double B[3] = {0, 0, 0};
   double dt = 1e-3;
   int precession = 1, noisy = 0, dyn_set = 0;
+  int noise_dbl = 0;  // generator grade, the prototype model's
   std::vector<double> alpha, T;
   int block = 256, spt = 1, graph_steps = 0;
   int64_t total = 0;
@@ -452,14 +462,17 @@ void layout(mx_batch& b) {
   drop_bgraph(b);
 }
 
+// Replaced while batched kernels queued on the caller's (possibly non-blocking) stream may still read the
+// table: the device finishes its queued work first (set-up calls only).
 void upload_reps(mx_batch& b) {
+  bok(hipDeviceSynchronize(), "hipDeviceSynchronize");
   if (!b.reps_d) bok(hipMalloc(&b.reps_d, sizeof(Rep) * b.reps.size()), "hipMalloc replicas");
   bok(hipMemcpy(b.reps_d, b.reps.data(), sizeof(Rep) * b.reps.size(), hipMemcpyHostToDevice), "hipMemcpy replicas");
 }
 
 template <typename R, int CM, int BS>
 void launch_pair(mx_batch& b, R* e, uint64_t step, const uint64_t* step_ptr, hipStream_t s) {
-  BDyn<R> d{R(b.B[0]), R(b.B[1]), R(b.B[2]), R(b.dt), b.precession, b.noisy};
+  BDyn<R> d{R(b.B[0]), R(b.B[1]), R(b.B[2]), R(b.dt), b.precession, b.noisy, b.noise_dbl};
   const BStatic<R>* S = static_cast<const BStatic<R>*>(b.static_d);
   const Rep* reps = static_cast<const Rep*>(b.reps_d);
   R* ep = static_cast<R*>(b.ep_d);
@@ -500,7 +513,7 @@ void bstep(mx_batch& b, void* e, uint64_t step, const uint64_t* step_ptr, hipStr
 
 template <typename R, int CM, int BS>
 void obs_launch(mx_batch& b, const R* e, hipStream_t s) {
-  BDyn<R> d{R(b.B[0]), R(b.B[1]), R(b.B[2]), R(b.dt), b.precession, b.noisy};
+  BDyn<R> d{R(b.B[0]), R(b.B[1]), R(b.B[2]), R(b.dt), b.precession, b.noisy, b.noise_dbl};
   hipLaunchKernelGGL((k_bobs<R, CM, BS>), dim3(b.ntiles), dim3(BS), 0, s, static_cast<const BStatic<R>*>(b.static_d),
                      static_cast<const Rep*>(b.reps_d), b.tile_rep_d, static_cast<const REnt*>(b.rent_d), b.nent, d,
                      e, b.spt, b.part_d);
@@ -522,7 +535,7 @@ void ensure_bgraph(mx_batch& b, void* e) {
   put(&e, sizeof(e));
   put(b.B, sizeof(b.B));
   put(&b.dt, sizeof(b.dt));
-  const int ints[6] = {b.precession, b.noisy, b.block, b.spt, b.graph_steps, b.ntiles};
+  const int ints[7] = {b.precession, b.noisy, b.block, b.spt, b.graph_steps, b.ntiles, b.noise_dbl};
   put(ints, sizeof(ints));
   if (b.gexec && key == b.gkey) return;
   drop_bgraph(b);
@@ -557,6 +570,7 @@ int mx_batch_create(const mx_model* proto, int32_t nrep, const int32_t* L, mx_ba
     auto* b = new mx_batch();
     try {
       b->prec = proto->prec;
+      b->noise_dbl = proto->noise_dbl;
       b->device = proto->device;
       b->nrep = nrep;
       b->ns = proto->ns;
@@ -574,7 +588,9 @@ int mx_batch_create(const mx_model* proto, int32_t nrep, const int32_t* L, mx_ba
         for (int a = 0; a < 3; ++a)
           if (L[3 * r + a] < 1 || L[3 * r + a] <= reach[a]) bfail(MX_EINVAL, "replica supercell shorter than the stencil reach");
         const int64_t nc = int64_t(p.L0) * p.L1 * p.L2;
-        if (off + nc * b->ns > (int64_t(1) << 30)) bfail(MX_EUNSUPPORTED, "more than 2^30 sites in a batch");
+        if (off + nc * b->ns > MAX_SITES)
+          bfail(MX_EUNSUPPORTED, "more than " + std::to_string(MAX_SITES) +
+                                     " sites in a batch (per-site offsets up to 9 N are 32-bit in the kernels)");
         if (nc >= (int64_t(1) << 24)) bfail(MX_EUNSUPPORTED, "a replica of 2^24 cells or more");
         p.rL1 = 1.0f / float(p.L1);
         p.rL2 = 1.0f / float(p.L2);

@@ -56,6 +56,30 @@ __device__ __forceinline__ void gauss3(uint32_t k0, uint32_t k1, uint64_t step, 
   n2 = r2 * __builtin_amdgcn_cosf(u3);
 }
 
+// Three standard normals for (seed, step, site) at double precision: two Philox4x32-10 outputs (fourth
+// counter word 0 and 1) give four 64-bit words; the top 52 bits k of each make an odd-multiple uniform
+// u = (2 k + 1) 2^-53 in (0, 1), exact in double (tails to sqrt(106 ln 2) = 8.6 sigma instead of float's
+// 5.8); Box-Muller (1958) with double log and sincospi (sin / cos of pi x, so cos(2 pi u) = cospi(2 u)).
+__device__ __forceinline__ void gauss3d(uint32_t k0, uint32_t k1, uint64_t step, uint32_t site, double& n0,
+                                        double& n1, double& n2) {
+  uint32_t a0 = site, a1 = uint32_t(step), a2 = uint32_t(step >> 32), a3 = 0u;
+  uint32_t b0 = site, b1 = a1, b2 = a2, b3 = 1u;
+  philox10(a0, a1, a2, a3, k0, k1);
+  philox10(b0, b1, b2, b3, k0, k1);
+  const double s53 = 1.1102230246251565404e-16;  // 2^-53
+  const double u0 = double(2 * ((uint64_t(a0) << 20) | (a1 >> 12)) + 1) * s53;
+  const double u1 = double(2 * ((uint64_t(a2) << 20) | (a3 >> 12)) + 1) * s53;
+  const double u2 = double(2 * ((uint64_t(b0) << 20) | (b1 >> 12)) + 1) * s53;
+  const double u3 = double(2 * ((uint64_t(b2) << 20) | (b3 >> 12)) + 1) * s53;
+  const double r0 = sqrt(-2.0 * log(u0)), r2 = sqrt(-2.0 * log(u2));
+  double s1, c1, s3, c3;
+  sincospi(2.0 * u1, &s1, &c1);
+  sincospi(2.0 * u3, &s3, &c3);
+  n0 = r0 * c1;
+  n1 = r0 * s1;
+  n2 = r2 * c3;
+}
+
 // floor(n / d) for 0 <= n < 2^24 and d >= 1, with rcp = 1/d rounded to float: the float product is
 // within one unit of the quotient, and the two integer corrections make it exact.
 __device__ __forceinline__ int fdiv(int n, int d, float rcp) {

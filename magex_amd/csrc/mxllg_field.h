@@ -476,6 +476,12 @@ __device__ __forceinline__ void thermal(const Static<R>* __restrict__ S, const D
     bx = sg * noise[3 * iref];
     by = sg * noise[3 * iref + 1];
     bz = sg * noise[3 * iref + 2];
+  } else if (S->noise_dbl) {
+    double n0, n1, n2;
+    gauss3d(d.seed_lo, d.seed_hi, step, uint32_t(iref), n0, n1, n2);
+    bx = sg * R(n0);
+    by = sg * R(n1);
+    bz = sg * R(n2);
   } else {
     float n0, n1, n2;
     gauss3(d.seed_lo, d.seed_hi, step, uint32_t(iref), n0, n1, n2);

@@ -20,6 +20,9 @@ constexpr int MAXENT = 64;   // stencil entries over all sublattices
 constexpr int ENT_PAD = 16;  // records readable past the last entry: the widest gather batch
 constexpr int MAXTAB = 256;  // coupling-table values (ntypes x width)
 constexpr int BLOCK = 256;   // threads per workgroup (4 wavefronts of 64)
+// Most sites of a model (or batch): the kernels form element offsets of per-site arrays in 32-bit ints,
+// up to 9 N for the row-major per-site on-site tensor (3 N for spins), so 9 N must stay below 2^31.
+constexpr int64_t MAX_SITES = (int64_t(1) << 31) / 9 - 1;
 
 enum CoupMode { COUP_TABLE1 = 0, COUP_TABLE4 = 1, COUP_TABLE9 = 2, COUP_BOND = 3 };
 // Cell layouts: OPEN (row-major, some axis open), PERIODIC (row-major, all axes periodic), TILED (all
@@ -50,6 +53,7 @@ struct Static {
   float rL2h;
   float rL1, rL2;
   int amode, has_A, has_c, has_u;
+  int noise_dbl;  // thermal normals at double grade (gauss3d) instead of float grade (gauss3)
   int sptr[MAXSUB + 1];
   Ent ent[MAXENT + ENT_PAD];  // padded: a batch of up to ENT_PAD entries may be read past the last one (never used)
   R tab[MAXTAB];  // COUP_TABLE*: ntypes x width
@@ -143,6 +147,7 @@ struct Model {
   std::vector<double> alpha_sub_v, sigma_sub_v;
   int noisy = 0;
   int dyn_set = 0;
+  int noise_dbl = 0;  // generator grade (MX_NOISE_DOUBLE = 1); set to the precision's own at creation
   // device constant block (Static<R>) and its dirty flag
   void* static_d = nullptr;
   int static_dirty = 1;

@@ -412,15 +412,23 @@ __global__ __launch_bounds__(BLOCK) void k_obs(const Static<R>* __restrict__ S, 
 
 // ------------------------------------------------------------------------------------------ misc
 template <typename R>
-__global__ __launch_bounds__(BLOCK) void k_noise(int N, uint32_t seed_lo, uint32_t seed_hi, uint64_t step,
+__global__ __launch_bounds__(BLOCK) void k_noise(int N, uint32_t seed_lo, uint32_t seed_hi, uint64_t step, int dbl,
                                                  R* __restrict__ out) {
   const int r = blockIdx.x * BLOCK + threadIdx.x;
   if (r >= N) return;
-  float n0, n1, n2;
-  gauss3(seed_lo, seed_hi, step, uint32_t(r), n0, n1, n2);
-  out[3 * r] = R(n0);
-  out[3 * r + 1] = R(n1);
-  out[3 * r + 2] = R(n2);
+  if (dbl) {
+    double n0, n1, n2;
+    gauss3d(seed_lo, seed_hi, step, uint32_t(r), n0, n1, n2);
+    out[3 * r] = R(n0);
+    out[3 * r + 1] = R(n1);
+    out[3 * r + 2] = R(n2);
+  } else {
+    float n0, n1, n2;
+    gauss3(seed_lo, seed_hi, step, uint32_t(r), n0, n1, n2);
+    out[3 * r] = R(n0);
+    out[3 * r + 1] = R(n1);
+    out[3 * r + 2] = R(n2);
+  }
 }
 
 // Step counter of captured step loops: set before the first replay, advanced by each replay.
@@ -518,6 +526,7 @@ static void fill_static(const Model& m, Static<R>& S) {
   S.has_A = m.has_A;
   S.has_c = m.has_c;
   S.has_u = m.has_u;
+  S.noise_dbl = m.noise_dbl;
   for (int s = 0; s <= MAXSUB; ++s) S.sptr[s] = s <= m.ns ? m.sptr[s] : m.sptr[m.ns];
   for (int k = 0; k < m.nent; ++k) S.ent[k] = m.ent[k];
   for (size_t k = 0; k < m.table.size() && k < size_t(MAXTAB); ++k) S.tab[k] = R(m.table[k]);
@@ -823,10 +832,10 @@ int launch_noise(const Model& m, void* out, uint64_t seed, uint64_t step, hipStr
   const dim3 g((m.N + BLOCK - 1) / BLOCK);
   if (m.prec == MX_FP64)
     hipLaunchKernelGGL((k_noise<double>), g, dim3(BLOCK), 0, s, m.N, uint32_t(seed), uint32_t(seed >> 32), step,
-                       static_cast<double*>(out));
+                       m.noise_dbl, static_cast<double*>(out));
   else
     hipLaunchKernelGGL((k_noise<float>), g, dim3(BLOCK), 0, s, m.N, uint32_t(seed), uint32_t(seed >> 32), step,
-                       static_cast<float*>(out));
+                       m.noise_dbl, static_cast<float*>(out));
   return check_launch("noise");
 }
 

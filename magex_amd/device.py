@@ -158,7 +158,6 @@ class DeviceModel:
         if system.pinned is not None:
             check(L.mx_model_set_pinned(h, hptr(np.ascontiguousarray(system.pinned, np.uint8))))
         self.set_field(system.B)
-        self._obs = torch.empty(_lib.MX_OBS_LEN, dtype=torch.float64, device="cuda")
         self._dyn = None
 
     @staticmethod
@@ -207,6 +206,14 @@ class DeviceModel:
     def set_graph_steps(self, n):
         """Steps per captured hipGraph step loop (0: launch every step directly)."""
         check(self.lib.mx_model_set_graph_steps(self.h, int(n)))
+
+    def set_noise(self, grade):
+        """Thermal-noise generator grade: "double" (default in FP64) or "float" (default in FP32; an FP64
+        model set to "float" draws exactly the FP32 model's noise)."""
+        codes = {"float": _lib.MX_NOISE_FLOAT, "double": _lib.MX_NOISE_DOUBLE}
+        if grade not in codes:
+            raise ValueError("noise grade must be 'float' or 'double'")
+        check(self.lib.mx_model_set_noise(self.h, codes[grade]))
 
     def set_step_mode(self, mode):
         """0: two kernel launches per Heun step; 1 (default): one launch where the model allows it."""
@@ -272,11 +279,12 @@ class DeviceModel:
         return out
 
     def observables_device(self, e):
-        """[E, sum e (3), sum s e (3), max torque] as a device float64 tensor (no synchronisation)."""
+        """[E, sum e (3), sum s e (3), max torque] as a new device float64 tensor (no synchronisation)."""
         self._check_spins(e)
-        check(self.lib.mx_observables(self.h, ctypes.c_void_p(e.data_ptr()), ctypes.c_void_p(self._obs.data_ptr()),
+        out = _torch().empty(_lib.MX_OBS_LEN, dtype=_torch().float64, device="cuda")
+        check(self.lib.mx_observables(self.h, ctypes.c_void_p(e.data_ptr()), ctypes.c_void_p(out.data_ptr()),
                                       _stream()))
-        return self._obs
+        return out
 
     def observables(self, e):
         return self.observables_device(e).cpu().numpy()

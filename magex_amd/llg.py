@@ -49,7 +49,10 @@ _RHS_MODELS = {}
 class LLGSolver:
     """Stochastic Heun LLG of a SpinSystem on the GPU (magex/llg.py:48-283).
 
-    Same constructor as magex's LLGSolver plus ``precision`` ("fp64", magex's arithmetic, or "fp32").
+    Same constructor as magex's LLGSolver plus ``precision`` ("fp64", magex's arithmetic, or "fp32") and
+    ``noise``, the grade of the thermal generator ("double": 53-bit uniforms and double Box-Muller, the
+    FP64 default; "float": 24-bit uniforms and float Box-Muller, the FP32 default; an FP64 solver with
+    noise="float" draws exactly the FP32 solver's noise).
     backend: "auto", "hip" or "numba" (all mean the device kernels); "numpy" raises: this package
     has no CPU path.  On the device path: the Heun and Depondt integrators, the classical thermostat,
     spin-orbit torque fields (sot), ``field_func(t) -> (3,) or (N, 3)`` evaluated at t in the predictor
@@ -61,7 +64,8 @@ class LLGSolver:
     """
 
     def __init__(self, system, alpha=0.1, dt=1e-3, temperature=0.0, seed=None, precession=True,
-                 thermostat="classical", integrator="heun", stt=None, sot=None, backend="auto", precision="fp64"):
+                 thermostat="classical", integrator="heun", stt=None, sot=None, backend="auto", precision="fp64",
+                 noise=None):
         self.sys = system
         a = getattr(system, "alpha_site", None)
         self.alpha = np.asarray(a if (a is not None and np.isscalar(alpha)) else alpha, float)
@@ -91,6 +95,9 @@ class LLGSolver:
             raise ValueError(f"unknown backend '{backend}'")
         self.integrator = integrator
         self.precision = precision
+        if noise not in (None, "float", "double"):
+            raise ValueError("noise must be 'float' or 'double'")
+        self.noise = noise or ("double" if precision == "fp64" else "float")
         self.time = 0.0
         self.field_func = None
         self.extra_field = None
@@ -104,6 +111,7 @@ class LLGSolver:
     # ------------------------------------------------------------------ device plumbing
     def _sync(self):
         self.dm = self.sys.device_model(self.precision)
+        self.dm.set_noise(self.noise)
         p, bdl, bfl = self.sot if self.sot is not None else ((0.0, 1.0, 0.0), 0.0, 0.0)
         self.dm.set_sot(p, bdl, bfl)
         self.fast = self.dm

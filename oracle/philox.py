@@ -7,8 +7,12 @@ use a counter-based generator instead so that a step's noise is a pure function 
 This module restates that generator so tests can pin the device's integer stream bit-exactly and
 its normals to float32 round-off.
 
+Two grades: ``normals`` (float grade, the FP32 kernels' default: 24-bit uniforms from one Philox output)
+and ``normals_double`` (double grade, the FP64 default: 53-bit uniforms from two outputs).
+
 Pinning: ``philox4x32_10`` against the Random123 known-answer vectors (tests/test_oracle.py);
-``normals`` against moments of the standard normal distribution.
+``normals`` and ``normals_double`` against moments of the standard normal distribution, and the
+double grade's uniforms against their defining bit layout (odd multiples of 2^-53, exact).
 """
 from __future__ import annotations
 
@@ -51,6 +55,27 @@ def normals(seed, step, n_sites, stream=STREAM_THERMAL):
     Box-Muller n0 = r0 cos(2 pi u1), n1 = r0 sin(2 pi u1), n2 = r2 cos(2 pi u3), r = sqrt(-2 ln u)."""
     w = raw_words(seed, step, n_sites, stream)
     u = (2.0 * (w >> np.uint32(9)).astype(np.float64) + 1.0) * 2.0**-24
+    r0 = np.sqrt(-2.0 * np.log(u[:, 0]))
+    r2 = np.sqrt(-2.0 * np.log(u[:, 2]))
+    return np.stack([r0 * np.cos(2 * np.pi * u[:, 1]), r0 * np.sin(2 * np.pi * u[:, 1]),
+                     r2 * np.cos(2 * np.pi * u[:, 3])], 1)
+
+
+def uniforms_double(seed, step, n_sites):
+    """(n_sites, 4) double-grade uniforms: two Philox outputs (fourth counter word 0 and 1) give four
+    64-bit words (first word high); the top 52 bits k of each give u = (2 k + 1) 2^-53 in (0, 1)."""
+    a = raw_words(seed, step, n_sites, 0).astype(np.uint64)
+    b = raw_words(seed, step, n_sites, 1).astype(np.uint64)
+    words = [(a[:, 0] << np.uint64(32)) | a[:, 1], (a[:, 2] << np.uint64(32)) | a[:, 3],
+             (b[:, 0] << np.uint64(32)) | b[:, 1], (b[:, 2] << np.uint64(32)) | b[:, 3]]
+    k = np.stack([w >> np.uint64(12) for w in words], 1)
+    return (2.0 * k.astype(np.float64) + 1.0) * 2.0**-53
+
+
+def normals_double(seed, step, n_sites):
+    """(n_sites, 3) double-grade standard normals: Box-Muller n0 = r0 cos(2 pi u1), n1 = r0 sin(2 pi u1),
+    n2 = r2 cos(2 pi u3), r = sqrt(-2 ln u), in double precision."""
+    u = uniforms_double(seed, step, n_sites)
     r0 = np.sqrt(-2.0 * np.log(u[:, 0]))
     r2 = np.sqrt(-2.0 * np.log(u[:, 2]))
     return np.stack([r0 * np.cos(2 * np.pi * u[:, 1]), r0 * np.sin(2 * np.pi * u[:, 1]),

@@ -68,6 +68,7 @@ def test_batch_magnetisation_tracks_fp64_oracle(gpu):
     for k, s in enumerate(systems):
         o = R.SpinSystem(uc, s.L, field=(0, 0, 0.5))
         dm = s.device_model("fp64")
+        dm.set_noise("float")                       # the FP32 batch's normals
         normals = [dm.noise(seeds[k], t).cpu().numpy() for t in range(200)]
         ek = R.run_heun(o, e0[k].astype(float), 200, 0.1, 1e-4, T=T[k], normals=lambda t: normals[t])
         m64 = np.linalg.norm(ek.mean(0))

@@ -39,7 +39,7 @@ def test_bcc64_exchange_reciprocity_and_norms(gpu):
     assert abs(a - b) <= 1e-11 * max(abs(a), abs(b)), (a, b)
     e = {}
     for prec, dt in (("fp64", gpu.float64), ("fp32", gpu.float32)):
-        sol = LLGSolver(s, p["alpha"], p["dt"], p["temperature"], p["seed"], precision=prec)
+        sol = LLGSolver(s, p["alpha"], p["dt"], p["temperature"], p["seed"], precision=prec, noise="float")
         x = gpu.as_tensor(s.init_spins("random", seed=0), device="cuda", dtype=dt)
         sol.step_device(x, 50)
         n = x.double().norm(dim=1)
@@ -58,7 +58,7 @@ def test_bcc64_fp32_bar_over_the_full_run(gpu):
     s, p = configs.bcc64()
     out = {}
     for prec, dt in (("fp64", gpu.float64), ("fp32", gpu.float32)):
-        sol = LLGSolver(s, p["alpha"], p["dt"], p["temperature"], p["seed"], precision=prec)
+        sol = LLGSolver(s, p["alpha"], p["dt"], p["temperature"], p["seed"], precision=prec, noise="float")
         x = gpu.as_tensor(s.init_spins("ferro"), device="cuda", dtype=dt)
         sol.step_device(x, 5000)
         out[prec] = x.double()

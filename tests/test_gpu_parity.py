@@ -99,18 +99,51 @@ def test_bcc_perbond_matches_magex(gpu):
 
 
 def test_device_noise_is_the_philox_stream(gpu):
+    """Both generator grades against their restatements: double grade (the FP64 default) to 1e-13, float
+    grade (the FP32 default) to float32 Box-Muller round-off; an FP64 model set to the float grade draws
+    exactly the FP32 model's normals."""
     print("case: noise vs philox", flush=True)
     s, _ = _pair((64, 64, 1), lattice="square", J=[1.0])
     dm = s.device_model("fp64")
+    dm.set_noise("double")
     for seed, step in ((0, 0), (12345, 7), (2**40 + 3, 2**33 + 5)):
         dev = dm.noise(seed, step).cpu().numpy()
-        ref = philox.normals(seed, step, s.N)
-        assert np.abs(dev - ref).max() < 2e-5
+        assert np.abs(dev - philox.normals_double(seed, step, s.N)).max() < 1e-13
     x = np.concatenate([dm.noise(9, k).cpu().numpy().ravel() for k in range(40)])
     assert abs(x.mean()) < 0.005 and abs(x.var() - 1) < 0.006
     assert abs((x**4).mean() / x.var() ** 2 - 3) < 0.03
+    dm.set_noise("float")
+    for seed, step in ((0, 0), (12345, 7), (2**40 + 3, 2**33 + 5)):
+        dev = dm.noise(seed, step).cpu().numpy()
+        assert np.abs(dev - philox.normals(seed, step, s.N)).max() < 2e-5
     f32 = s.device_model("fp32").noise(12345, 7).cpu().numpy()
     assert np.array_equal(f32.astype(np.float64), dm.noise(12345, 7).cpu().numpy())
+    dm.set_noise("double")
+
+
+def test_double_grade_noise_tails(gpu):
+    """Tail frequencies of the double-grade normals over 5e7 draws: P(|n| > 4) and P(|n| > 5) within five
+    Poisson standard deviations of the normal distribution's (the float grade stops at 5.77 sigma)."""
+    print("case: double-grade noise tails", flush=True)
+    from scipy.special import erfc
+    s, _ = _pair((512, 512, 1), lattice="square", J=[1.0])
+    dm = s.device_model("fp64")
+    dm.set_noise("double")
+    counts = {4.0: 0, 5.0: 0}
+    n = 0
+    big = 0.0
+    for k in range(64):
+        x = dm.noise(2024, k).abs()
+        n += x.numel()
+        big = max(big, float(x.max()))
+        for c in counts:
+            counts[c] += int((x > c).sum())
+    for c, got in counts.items():
+        want = n * erfc(c / np.sqrt(2.0))
+        print(f"  |n| > {c}: {got} of {n} (expected {want:.1f})", flush=True)
+        assert abs(got - want) < 5 * np.sqrt(want)
+    print(f"  largest |n| {big:.3f}")
+    assert big < 8.7
 
 
 def test_thermal_run_matches_oracle_on_device_noise(gpu):
@@ -184,7 +217,7 @@ def test_fp32_angle_error_against_fp64(gpu):
     print("case: fp32 vs fp64 angle", flush=True)
     s, _ = _pair((64, 64, 1), (0, 0, 2.0), lattice="triangular", J=[5.0, -0.5, 0.2], D=1.5, K=0.3)
     e0 = s.init_spins("random", seed=0)
-    a = LLGSolver(s, alpha=0.05, dt=5e-4, temperature=20.0, seed=3).run(e0, 500, output_every=500)[0]
+    a = LLGSolver(s, alpha=0.05, dt=5e-4, temperature=20.0, seed=3, noise="float").run(e0, 500, output_every=500)[0]
     b = LLGSolver(s, alpha=0.05, dt=5e-4, temperature=20.0, seed=3, precision="fp32").run(
         e0.astype(np.float32), 500, output_every=500)[0].astype(np.float64)
     b = R.normalize(b)
@@ -351,7 +384,7 @@ def test_depondt_matches_oracle(gpu, precession):
     # FP64 Depondt is within twice the FP32 Heun angle from FP64 Heun (plus 1e-7 rad)
     def angle(integrator):
         kw = dict(alpha=0.05, dt=5e-4, temperature=20.0, seed=5, integrator=integrator, precession=precession)
-        e64, _, _ = LLGSolver(s, **kw).run(e0, 40, output_every=40)
+        e64, _, _ = LLGSolver(s, noise="float", **kw).run(e0, 40, output_every=40)
         e32, _, _ = LLGSolver(s, precision="fp32", **kw).run(e0, 40, output_every=40)
         return np.arccos(np.clip(np.sum(np.asarray(e32, float) * e64, 1), -1, 1)).mean()
 

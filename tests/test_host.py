@@ -134,6 +134,13 @@ def test_library_argument_errors_without_gpu():
         _lib.check(L.mx_model_create(0, 99, _lib.hptr(Lv), _lib.hptr(pb), ctypes.byref(h)))
     with pytest.raises(ValueError, match="null model"):
         _lib.check(L.mx_model_set_field(None, _lib.hptr(np.zeros(3))))
+    # the 32-bit per-site offsets of the kernels (up to 9 N) cap a model at (2^31 - 1) / 9 sites
+    big = np.array([1024, 1024, 228], np.int32)          # 239 075 328 sites: 9 N > 2^31
+    with pytest.raises(NotImplementedError, match="238609293 sites"):
+        _lib.check(L.mx_model_create(1, 1, _lib.hptr(big), _lib.hptr(pb), ctypes.byref(h)))
+    with pytest.raises(NotImplementedError, match="238609293 sites"):
+        _lib.check(L.mx_model_create(1, 2, _lib.hptr(np.array([1024, 1024, 114], np.int32)), _lib.hptr(pb),
+                                     ctypes.byref(h)))
 
 
 def test_product_has_no_cpu_path():

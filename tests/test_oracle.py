@@ -205,6 +205,23 @@ def test_philox_normals_moments():
     assert abs((np.abs(x) > 3).mean() / 0.0026998 - 1) < 0.08
 
 
+def test_philox_double_grade_normals():
+    """Double-grade restatement: uniforms are odd multiples of 2^-53 in (0, 1) with uniform moments; the
+    normals pass moments, the 3-sigma tail and a Kolmogorov-Smirnov test against the normal CDF."""
+    from scipy import stats
+    u = philox.uniforms_double(11, 5, 200000)
+    k = u * 2.0**53
+    assert np.all(k == np.round(k)) and np.all(np.mod(k, 2) == 1) and u.min() > 0 and u.max() < 1
+    assert abs(u.mean() - 0.5) < 0.002 and abs(u.var() - 1 / 12) < 0.001
+    x = philox.normals_double(7, 3, 400000).ravel()
+    assert abs(x.mean()) < 0.005 and abs(x.var() - 1) < 0.006
+    assert abs((x**4).mean() / x.var() ** 2 - 3) < 0.03
+    assert abs((np.abs(x) > 3).mean() / 0.0026998 - 1) < 0.08
+    assert stats.kstest(x, "norm").pvalue > 1e-3
+    # a different word order (low word first) is a different stream: the restatement pins the layout
+    assert not np.allclose(philox.normals_double(7, 3, 1000), philox.normals(7, 3, 1000), atol=1e-3)
+
+
 def test_c_ziggurat_moments():
     """The C restatement of magex's ziggurat passes magex's own moment test (test_fast.py:39-46)."""
     from oracle import cpu
