@@ -115,6 +115,18 @@ int mx_model_set_dynamics(mx_model* m, double dt, int alpha_per_site, const doub
 int mx_llg_run(mx_model* m, void* e, int64_t nsteps, uint64_t seed, uint64_t step0, const void* noise,
                void* stream);
 
+/* mx_llg_run for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in magex.  It calls no hipMalloc, hipFree or
+ * other allocating routine: its predictor buffers, internal-order spin copy and one-launch control words
+ * live in a 64 MiB device array of the library that every GPU it runs on holds until the process ends,
+ * which bounds the model at 3 x align256(N x 3 x w) + 4 x (32 L0 + 4 + blocks) bytes (about 1.8 M FP32 or
+ * 0.9 M FP64 sites); larger models return MX_EUNSUPPORTED.  Steps launch directly (no captured loop).
+ * The model must be configured and its constants on the device (any other entry point run once after
+ * the last set-up call), else MX_EINVAL.  Calls that overlap on one GPU (other streams, other threads,
+ * other models) share that array and race; results equal mx_llg_run's bit for bit. */
+int mx_llg_run_noalloc(mx_model* m, void* e, int64_t nsteps, uint64_t seed, uint64_t step0, const void* noise,
+                       void* stream);
+
 /* nsteps steps as mx_llg_run with internal noise, under a time-dependent uniform field: B (host,
  * nsteps x 2 x 3 doubles) gives, for step n, the field of the predictor pass (B[6n..6n+2]) and of the
  * corrector pass (B[6n+3..6n+5]) in place of the model's field -- magex evaluates field_func at t and

@@ -61,6 +61,7 @@ SIGNATURES = {
     "mx_sampler_destroy": (None, [_P]),
     "mx_sample_mq": (_I, [_P, _P, _P, _P]),
     "mx_llg_run": (_I, [_P, _P, _I64, _U64, _U64, _P, _P]),
+    "mx_llg_run_noalloc": (_I, [_P, _P, _I64, _U64, _U64, _P, _P]),
     "mx_noise": (_I, [_P, _P, _U64, _U64, _P]),
     "mx_field": (_I, [_P, _P, _P, _I, _P]),
     "mx_observables": (_I, [_P, _P, _P, _P]),

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <tuple>
@@ -234,6 +235,31 @@ static void ensure_graph(Model& m, void* E, uint64_t seed) {
   hip_ok(inst, "hipGraphInstantiate");
   m.gkey = key;
 }
+
+// ---------------------------------------------------------------------- allocation-free step scratch
+// The scratch of mx_llg_run_noalloc: one call's predictor buffers, internal-order spin copy and
+// one-launch control words, in a file-scope device array that every GPU the library runs on holds until
+// the process ends.  In host code `scratch` is only a stand-in symbol: its device address on each GPU
+// comes from hipGetSymbolAddress (once per device, that device current) and only that address is used.
+constexpr size_t SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+constexpr int SCRATCH_DEVICES = 64;
+static void* g_scratch_ptr[SCRATCH_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
+static unsigned char* scratch_of(int device) {
+  if (device < 0 || device >= SCRATCH_DEVICES) fail(MX_EUNSUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (err != hipSuccess || !p) fail(MX_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(err));
+    g_scratch_ptr[device] = p;
+  }
+  return static_cast<unsigned char*>(g_scratch_ptr[device]);
+}
+
+static size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
 
 }  // namespace mx
 
@@ -781,6 +807,46 @@ int mx_llg_run(mx_model* m, void* e, int64_t nsteps, uint64_t seed, uint64_t ste
       if (rc) return rc;
     }
     if (needs_permute(*m)) return launch_permute(*m, m->eint_d, e, 0, s);
+    return MX_OK;
+  });
+}
+
+int mx_llg_run_noalloc(mx_model* m, void* e, int64_t nsteps, uint64_t seed, uint64_t step0, const void* noise,
+                       void* stream) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (!e) fail(MX_EINVAL, "null spins");
+    if (nsteps < 0) fail(MX_EINVAL, "negative step count");
+    if (!m->dyn_set) fail(MX_EINVAL, "set the dynamics first");
+    if (!m->static_d || m->static_dirty)
+      fail(MX_EINVAL, "model constants not on the device: run any other entry point once after configuring");
+    const size_t spins = align256(size_t(m->N) * 3 * rsize(*m));
+    const size_t words = stepsync_words(*m);
+    const size_t need = 3 * spins + words * sizeof(uint32_t);
+    if (need > SCRATCH_BYTES)
+      fail(MX_EUNSUPPORTED, "model too large for the allocation-free scratch (" + std::to_string(need) + " bytes of " +
+                                std::to_string(SCRATCH_BYTES) + "): use mx_llg_run");
+    use_device(*m);
+    unsigned char* base = scratch_of(m->device);
+    StepBufs bufs{base, base + spins, reinterpret_cast<uint32_t*>(base + 3 * spins)};
+    void* eint = base + 2 * spins;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // the control words start every call at zero (a kernel, so the call can sit in captured work)
+    int rc = launch_fill_u32(bufs.sync, words, s);
+    if (rc) return rc;
+    void* E = e;
+    if (needs_permute(*m)) {
+      rc = launch_permute(*m, e, eint, 1, s);
+      if (rc) return rc;
+      E = eint;
+    }
+    const size_t stride = size_t(m->N) * 3 * rsize(*m);
+    for (int64_t n = 0; n < nsteps; ++n) {
+      const void* nz = noise ? static_cast<const char*>(noise) + size_t(n) * stride : nullptr;
+      rc = launch_heun_step(*m, E, seed, step0 + uint64_t(n), nz, s, nullptr, &bufs);
+      if (rc) return rc;
+    }
+    if (needs_permute(*m)) return launch_permute(*m, eint, e, 0, s);
     return MX_OK;
   });
 }

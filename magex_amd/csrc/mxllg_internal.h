@@ -181,10 +181,21 @@ struct Model {
   std::vector<char> gkey;
 };
 
+// Scratch of one step: the predictor spins ep, w = e + dt/2 k1 (or the rotation vector) in k1, and the
+// one-launch step's control words (stepsync_words(m) of them, zero between launches).
+struct StepBufs {
+  void* ep;
+  void* k1;
+  uint32_t* sync;
+};
+
 // Kernel launchers (mxllg_kernels.hip).  All take internal-order device buffers unless noted.
 int upload_static(Model& m);
+// One step on e; bufs = null: the model's own scratch.
 int launch_heun_step(const Model& m, void* e, uint64_t seed, uint64_t step, const void* noise_step, hipStream_t s,
-                     const uint64_t* step_ptr = nullptr);
+                     const uint64_t* step_ptr = nullptr, const StepBufs* bufs = nullptr);
+// Zero n 32-bit words at p (a kernel, so that it can sit in captured work).
+int launch_fill_u32(uint32_t* p, size_t n, hipStream_t s);
 int launch_counter(uint64_t* c, uint64_t value, int add, hipStream_t s);
 // Whether a step of m runs as one launch (k_step), and the number of held coupling batches it uses.
 bool fused_step(const Model& m, int* nb = nullptr);

@@ -708,20 +708,21 @@ static void step_dispatch(const Model& m, int nb, bool lean, dim3 g, hipStream_t
 
 template <typename R>
 static int heun_t(const Model& m, void* e, uint64_t seed, uint64_t step, const void* noise, hipStream_t s,
-                  const uint64_t* step_ptr) {
+                  const uint64_t* step_ptr, const StepBufs* bufs) {
   const Static<R>* S = static_cast<const Static<R>*>(m.static_d);
   Dyn<R> d = make_dyn<R>(m, seed);
   d.step_ptr = step_ptr;
   R* E = static_cast<R*>(e);
-  R* ep = static_cast<R*>(m.ep_d);
-  R* k1 = static_cast<R*>(m.k1_d);
+  R* ep = static_cast<R*>(bufs ? bufs->ep : m.ep_d);
+  R* k1 = static_cast<R*>(bufs ? bufs->k1 : m.k1_d);
+  uint32_t* sync = bufs ? bufs->sync : m.stepsync_d;
   const R* nz = static_cast<const R*>(noise);
   const dim3 g(nblk_of(m)), b(BLOCK);
   int nb = 0;
-  if (m.stepsync_d && fused_step(m, &nb)) {
+  if (sync && fused_step(m, &nb)) {
     StepSync sy;
-    sy.cnt = m.stepsync_d;
-    sy.ctl = m.stepsync_d + size_t(m.L[0]) * CNT_STRIDE;
+    sy.cnt = sync;
+    sy.ctl = sync + size_t(m.L[0]) * CNT_STRIDE;
     sy.gu = reinterpret_cast<int32_t*>(sy.ctl + 4);
     sy.r0 = reach0(m);
     sy.PS = m.L[1] * m.L[2];
@@ -748,9 +749,20 @@ static int heun_t(const Model& m, void* e, uint64_t seed, uint64_t step, const v
 }
 
 int launch_heun_step(const Model& m, void* e, uint64_t seed, uint64_t step, const void* noise, hipStream_t s,
-                     const uint64_t* step_ptr) {
-  return m.prec == MX_FP64 ? heun_t<double>(m, e, seed, step, noise, s, step_ptr)
-                           : heun_t<float>(m, e, seed, step, noise, s, step_ptr);
+                     const uint64_t* step_ptr, const StepBufs* bufs) {
+  return m.prec == MX_FP64 ? heun_t<double>(m, e, seed, step, noise, s, step_ptr, bufs)
+                           : heun_t<float>(m, e, seed, step, noise, s, step_ptr, bufs);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t* __restrict__ p, size_t n) {
+  for (size_t k = size_t(blockIdx.x) * BLOCK + threadIdx.x; k < n; k += size_t(gridDim.x) * BLOCK) p[k] = 0u;
+}
+
+int launch_fill_u32(uint32_t* p, size_t n, hipStream_t s) {
+  if (n == 0) return MX_OK;
+  const int g = int(std::min<size_t>((n + BLOCK - 1) / BLOCK, 1024));
+  hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(BLOCK), 0, s, p, n);
+  return check_launch("fill");
 }
 
 template <typename R>
