@@ -257,6 +257,37 @@ class SpinSystem:
     def bond_tensors(self):
         return self._bond_J
 
+    def scale_exchange(self, f):
+        """Multiply every coupling J_ij by f[material_i, material_j] (interface exchange,
+        magex/spinmodel.py:433-442).  The couplings stop being translation invariant, so they become
+        per-bond tensors; f must be symmetric and partner entries transposed (J_ji = J_ij^T), which
+        per-bond storage (each unordered bond once) needs."""
+        f = np.asarray(f, float)
+        if not np.array_equal(f, f.T):
+            raise NotImplementedError("an asymmetric exchange scale breaks J_ji = J_ij^T")
+        fw = self.forward_entries()
+        base = self._bond_J
+        if base is None:
+            base = self._table_as_bonds(fw)
+        J = np.empty_like(base)
+        for n, k in enumerate(fw):
+            src, dst = self.neighbour(k)
+            w = np.where(dst >= 0, f[self.material[src], self.material[np.maximum(dst, 0)]], 1.0)
+            J[n] = base[n] * w[:, None, None]
+        self.set_bond_tensors(J)
+
+    def _table_as_bonds(self, fw):
+        """The stencil's translation-invariant tensors as per-bond storage on the forward entries."""
+        keys = {(s, t, tuple(R)): J for s, t, R, J in self.stencil}
+        out = np.empty((len(fw), self.ncell, 3, 3))
+        for n, k in enumerate(fw):
+            s, t, R, J = self.stencil[k]
+            P = keys.get((t, s, tuple(-R)))
+            if P is None or not np.allclose(P, J.T, rtol=0, atol=1e-14 * max(np.abs(J).max(), 1e-300)):
+                raise NotImplementedError("per-bond storage needs every entry's partner with J_ji = J_ij^T")
+            out[n] = J[None]
+        return out
+
     @property
     def Jsp(self):
         """The 3N x 3N CSR coupling matrix of magex (spinmodel.py:230-241), built on first access."""

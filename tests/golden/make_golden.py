@@ -232,7 +232,71 @@ def case_charge():
     _save("charge.npz", **out)
 
 
+# [structure] configurations of the drop-in's golden case: a layered film (open axis 2) with two
+# materials, terms, Neel surface anisotropy, interface exchange, a pinned bottom layer and a temperature
+# ramp; and a core-shell alloy on a periodic triangular lattice with a pinned material and a hot spot.
+STRUCTURE_FILM = dict(
+    lattice=dict(lattice="sc", J=[1.0, 0.2], K=0.1), L=(6, 5, 4), pbc=(True, True, False), field=(0, 0, 0.3),
+    cfg={"layers": [{"material": "Co", "thickness": 2}, {"material": "Pt", "thickness": 2}],
+         "materials": [{"name": "Co", "moment_scale": 1.2, "anisotropy_scale": 0.5, "Ku": [0.3, [0, 0, 1]],
+                        "alpha": 0.05, "ks": 0.04},
+                       {"name": "Pt", "moment_scale": 0.6, "Kc": 0.02, "cubic_axes": [[0.8, 0.6, 0], [-0.6, 0.8, 0],
+                                                                                     [0, 0, 1]],
+                        "K2": 0.01, "K3": 0.005, "K_axis": [0, 1, 1], "T": 40.0}],
+         "pinned": [{"region": "bottom", "layers": 1, "direction": [1, 0, 1]}],
+         "temperature_profile": {"axis": "x", "T0": 10.0, "T1": 100.0},
+         "exchange_scale": [[1, 1, 1], [1, 1, 0.5], [1, 0.5, 1]], "neel_surface": 0.05})
+STRUCTURE_ALLOY = dict(
+    lattice=dict(lattice="triangular", J=[1.0]), L=(10, 10, 1), pbc=(True, True, True), field=(0, 0, 0.5),
+    cfg={"core_shell": {"radius": 6.0, "core": "A", "shell": "B"},
+         "alloy": {"host": "B", "guest": "C", "fraction": 0.3, "seed": 3},
+         "materials": [{"name": "A", "moment_scale": 1.5, "T": 30.0},
+                       {"name": "C", "alpha": 0.2, "Ku": [0.2, [1, 0, 0]]}],
+         "pinned": [{"region": "material:A", "direction": [0, 1, 0]}],
+         "temperature_profile": {"spot": [10.0, 5.0, 0.0], "sigma": 4.0, "T0": 5.0, "T1": 50.0}})
+
+
+def case_structure():
+    """magex's apply_structure on both configurations: the arrays it leaves on the system; for the film
+    also B_eff, the energy and 10 thermal Heun steps (T = 20 K, alpha 0.1 where no material sets it) on
+    injected normals."""
+    from magex.structure import apply_structure
+    from magex import terms
+    with open(os.path.join(OUT, "structure_cases.json"), "w") as f:
+        json.dump({"film": STRUCTURE_FILM, "alloy": STRUCTURE_ALLOY}, f, indent=1)
+    print("wrote structure_cases.json")
+    for name, case in (("structure_film.npz", STRUCTURE_FILM), ("structure_alloy.npz", STRUCTURE_ALLOY)):
+        s = _system(case["lattice"], case["L"], case["field"], case["pbc"])
+        apply_structure(s, case["cfg"])
+        out = dict(material=s.material, material_names=np.array(s.material_names), mu=s.mu, A=s.A,
+                   pinned=s.pinned, pinned_dir=s.pinned_dir,
+                   alpha_site=getattr(s, "alpha_site", None) if getattr(s, "alpha_site", None) is not None
+                   else np.full(s.N, np.nan),
+                   site_T=s.site_T if s.site_T is not None else np.full(s.N, np.nan))
+        for t in s.terms:
+            if isinstance(t, terms.CubicAnisotropy):
+                out["Kc"], out["C"] = t.Kc, t.C
+            elif isinstance(t, terms.UniaxialHigher):
+                out["K2"], out["K3"], out["u"] = t.K2, t.K3, t.u
+        if hasattr(s, "surface"):
+            out["surface"] = s.surface
+        J = s.Jsp.tocoo()
+        out.update(J_row=J.row, J_col=J.col, J_val=J.data)
+        if name == "structure_film.npz":
+            e0 = s.init_spins("random", seed=3)
+            out.update(e0=e0, B0=s.effective_field(e0), E0=np.array(s.energy(e0)))
+            g = np.random.default_rng(8).normal(size=(10, s.N, 3))
+            sol = LLGSolver(s, alpha=0.1, dt=2e-4, temperature=20.0, seed=0)
+            sol.rng = _InjectedNormals(g)
+            e = e0.copy()
+            for _ in range(10):
+                e = sol.step(e)
+            out.update(noise=g, e10=e)
+        _save(name, **out)
+
+
 if __name__ == "__main__":
+    case_structure()
     models_json()
     couplings_tri()
     case_c1()

@@ -542,6 +542,48 @@ def test_solver_field_and_rhs_match_oracle(gpu, per_site_alpha):
         assert fd.is_cuda and _rel(fd.cpu().numpy(), f) < 1e-15
 
 
+def _structured_film():
+    from magex_amd import structure
+    case = json.load(open(os.path.join(GOLDEN, "structure_cases.json")))["film"]
+    d = M.make_model(**case["lattice"])
+    s = S.SpinSystem(S.load_results(d), case["L"], case["pbc"], case["field"])
+    structure.apply_structure(s, case["cfg"])
+    o = R.SpinSystem(R.load_results(d), case["L"], case["pbc"], case["field"])
+    return s, o, case
+
+
+def test_structured_film_matches_magex_and_oracle(gpu):
+    """A model set up by apply_structure (two materials in layers, scaled moments and anisotropy, Ku, cubic
+    and higher-uniaxial terms, Neel surface anisotropy, interface exchange as per-bond tensors, a pinned
+    bottom layer with its direction, per-material damping and temperature under a temperature ramp):
+    start vectors, B_eff and energy equal magex's; 10 Heun steps on injected noise within 1e-10 of magex's
+    trajectory; 50 steps on the device generator's noise within 1e-11 of the oracle."""
+    print("case: structured film", flush=True)
+    import scipy.sparse as sp
+    g = golden("structure_film.npz")
+    s, o, case = _structured_film()
+    e0 = s.init_spins("random", seed=3)
+    assert np.array_equal(e0, g["e0"])
+    assert _rel(s.effective_field(e0), g["B0"]) < 1e-13
+    assert abs(s.energy(e0) - g["E0"]) < 1e-12 * abs(g["E0"])
+    sol = LLGSolver(s, alpha=0.1, dt=2e-4, temperature=20.0, seed=0)
+    ed = gpu.as_tensor(e0, device="cuda")
+    sol.step_device(ed, 10, noise=gpu.as_tensor(g["noise"], device="cuda"))
+    assert _rel(ed.cpu().numpy(), g["e10"]) < 1e-10
+    # the oracle's system carries magex's arrays; the kernels' carry the drop-in's
+    o.Jsp = sp.csr_matrix((g["J_val"], (g["J_row"], g["J_col"])), shape=(3 * o.N, 3 * o.N))
+    o.A, o.mu, o.pinned, o.site_T = g["A"], g["mu"], g["pinned"], g["site_T"]
+    o.terms += [R.CubicAnisotropy(g["Kc"], g["C"]), R.UniaxialHigher(g["K2"], g["K3"], axis=g["u"])]
+    alpha = np.where(np.isnan(g["alpha_site"]), 0.1, g["alpha_site"])
+    sol = LLGSolver(s, alpha=0.1, dt=2e-4, temperature=20.0, seed=31)
+    e, _, _ = sol.run(e0, 50, output_every=50)
+    dm = s.device_model("fp64")
+    normals = [dm.noise(31, n).cpu().numpy() for n in range(50)]
+    ref = R.run_heun(o, e0.copy(), 50, alpha, 2e-4, T=20.0, normals=lambda n: normals[n])
+    assert _rel(e, ref) < 1e-11
+    assert np.array_equal(e[g["pinned"]], e0[g["pinned"]])
+
+
 def test_largest_cell_count_and_self_images(gpu):
     """Sizes at the edges of the index arithmetic and of the stencil:
     - 256^3 = 2^24 cells (the largest supercell class, where cell indices leave the float-division

@@ -111,6 +111,71 @@ def _declared_symbols():
     return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(mx_\w+)\(", txt, re.M)))
 
 
+def _structured(name):
+    """The drop-in's apply_structure on one of the golden [structure] configurations."""
+    from magex_amd import structure
+    case = json.load(open(os.path.join(GOLDEN, "structure_cases.json")))[name]
+    s = S.SpinSystem(S.load_results(M.make_model(**case["lattice"])), case["L"], case["pbc"], case["field"])
+    return structure.apply_structure(s, case["cfg"])
+
+
+@pytest.mark.parametrize("case,lattice,L,field,seed", [
+    ("c1_sc16.npz", dict(lattice="sc", J=[1.0], K=0.1, moment=2.0), (16, 16, 16), (0, 0, 0.5), 0),
+    ("tri12.npz", dict(lattice="triangular", J=[5.0, -0.5, 0.2], D=1.5, K=0.3), (12, 12, 1), (0, 0, 2.0), 5),
+])
+def test_init_spins_random_is_magex_start(case, lattice, L, field, seed):
+    """SpinSystem.init_spins("random", seed) reproduces magex's start vectors bit for bit (numpy's PCG64
+    normals, normalised; magex/spinmodel.py:495-496, 512)."""
+    s = S.SpinSystem(S.load_results(M.make_model(**lattice)), L, field=field)
+    assert np.array_equal(s.init_spins("random", seed=seed), golden(case)["e0"])
+
+
+@pytest.mark.parametrize("name", ["film", "alloy"])
+def test_apply_structure_matches_magex(name):
+    """Materials (layers, core-shell, alloy), moments, on-site tensors (scaling, Ku, Neel surface), cubic
+    and higher-uniaxial terms, damping and temperature maps, pinned sites and directions, and interface
+    exchange (as per-bond tensors) equal what magex's apply_structure leaves on its system."""
+    import scipy.sparse as sp
+    from magex_amd.terms import CubicAnisotropy, UniaxialHigher
+    g = golden(f"structure_{name}.npz")
+    s = _structured(name)
+    assert list(s.material_names) == [str(x) for x in g["material_names"]]
+    assert np.array_equal(s.material, g["material"])
+    assert np.array_equal(s.mu, g["mu"])
+    assert np.abs(s.A - g["A"]).max() < 1e-15
+    assert np.array_equal(s.pinned, g["pinned"])
+    assert np.abs(s.pinned_dir - g["pinned_dir"]).max() < 1e-15
+    for key in ("alpha_site", "site_T"):
+        mine = getattr(s, key)
+        mine = np.full(s.N, np.nan) if mine is None else mine
+        assert np.array_equal(np.isnan(mine), np.isnan(g[key])), key
+        assert np.allclose(mine[~np.isnan(mine)], g[key][~np.isnan(g[key])], rtol=1e-15, atol=0), key
+    cub = [t for t in s.terms if isinstance(t, CubicAnisotropy)]
+    uni = [t for t in s.terms if isinstance(t, UniaxialHigher)]
+    if "Kc" in g.files:
+        assert np.array_equal(cub[0].Kc, g["Kc"]) and np.array_equal(cub[0].C, g["C"])
+        assert np.array_equal(uni[0].K2, g["K2"]) and np.array_equal(uni[0].K3, g["K3"])
+        assert np.allclose(uni[0].u, g["u"], rtol=0, atol=1e-16)
+    else:
+        assert not cub and not uni
+    if "surface" in g.files:
+        assert np.array_equal(s.surface, g["surface"])
+    ref = sp.csr_matrix((g["J_val"], (g["J_row"], g["J_col"])), shape=(3 * s.N, 3 * s.N))
+    assert abs(s.Jsp - ref).max() < 1e-15
+
+
+def test_apply_structure_refuses_what_the_device_path_cannot_run():
+    from magex_amd import structure
+    s = S.SpinSystem(S.load_results(M.make_model(lattice="sc", J=[1.0])), (4, 4, 4))
+    for cfg in ({"shape": "sphere", "size": 3.0}, {"vacancies": 0.1}, {"roughness": {"amplitude": 1}},
+                {"dipolar": True}, {"quadruplets": {"K": 0.1, "list": [[0, 1, 2, 3]]}}):
+        with pytest.raises(NotImplementedError):
+            structure.apply_structure(s, cfg)
+    s.material_names += ["a", "b"]
+    with pytest.raises(NotImplementedError, match="asymmetric"):
+        s.scale_exchange([[1, 1, 1], [1, 1, 0.5], [1, 0.7, 1]])
+
+
 def test_library_exports_every_declared_symbol():
     from magex_amd import _lib
     L = ctypes.CDLL(_lib.LIB_PATH)
