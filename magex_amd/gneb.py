@@ -19,46 +19,53 @@ from ._lib import check
 from .device import _stream, _torch, as_device
 
 
-# ----------------------------------------------------------------------------- geometry (magex/gneb.py:85-118)
-def _normalize(x):
-    return x / np.linalg.norm(x, axis=-1, keepdims=True)
-
-
-def _project(x, v):
-    return v - np.einsum("ia,ia->i", v, x)[:, None] * x
+# ----------------------------------------------------------------------------- geometry
+def _spin_angles(a, b):
+    """Angle between corresponding unit spins, 2 atan2(|a - b|, |a + b|): accurate at 0 and at pi."""
+    return 2.0 * np.arctan2(np.linalg.norm(a - b, axis=-1), np.linalg.norm(a + b, axis=-1))
 
 
 def geodesic_distance(a, b):
-    c = np.clip(np.einsum("ia,ia->i", a, b), -1.0, 1.0)
-    s = np.linalg.norm(np.cross(a, b), axis=1)
-    return float(np.sqrt(np.sum(np.arctan2(s, c) ** 2)))
+    """Geodesic distance sqrt(sum_i angle_i^2) between two spin configurations (magex/gneb.py:91-94)."""
+    ang = _spin_angles(np.asarray(a, float), np.asarray(b, float))
+    return float(np.sqrt(ang @ ang))
 
 
 def slerp_path(a, b, nimages):
-    """Geodesic interpolation spin by spin (rotation about a x b; any normal axis for antiparallel spins)."""
-    a, b = np.asarray(a, float), np.asarray(b, float)
-    ang = np.arctan2(np.linalg.norm(np.cross(a, b), axis=1), np.clip(np.einsum("ia,ia->i", a, b), -1, 1))
-    axis = np.cross(a, b)
-    nrm = np.linalg.norm(axis, axis=1)
-    bad = nrm < 1e-8
-    if np.any(bad):
-        trial = np.where(np.abs(a[bad, 0:1]) < 0.9, np.array([[1.0, 0, 0]]), np.array([[0, 1.0, 0]]))
-        axis[bad] = np.cross(a[bad], trial)
-        nrm[bad] = np.linalg.norm(axis[bad], axis=1)
-    axis = axis / nrm[:, None]
-    out = []
-    for t in np.linspace(0, 1, nimages):
-        th = (ang * t)[:, None]
-        rot = a * np.cos(th) + np.cross(axis, a) * np.sin(th) + axis * np.einsum("ia,ia->i", axis, a)[:, None] * (
-            1 - np.cos(th))
-        out.append(_normalize(rot))
-    out[0], out[-1] = a.copy(), b.copy()
+    """nimages configurations from a to b along each spin's great circle (magex/gneb.py:97-118), built on
+    the device: Shoemake's slerp (sin((1 - t) th) a + sin(t th) b) / sin th; spins closer than 1e-8 to
+    parallel interpolate linearly; spins closer than 1e-8 to antiparallel turn by t th about a x u, with
+    u = x unless a is nearly along x (then y) -- magex's choice of the half-turn.  The end images are a
+    and b exactly.  Returns a list of (N, 3) NumPy arrays."""
+    torch = _torch()
+    A = torch.as_tensor(np.asarray(a, float), device="cuda")
+    B = torch.as_tensor(np.asarray(b, float), device="cuda")
+    th = 2.0 * torch.atan2((A - B).norm(dim=-1), (A + B).norm(dim=-1))[None, :, None]
+    t = torch.linspace(0.0, 1.0, nimages, dtype=torch.float64, device="cuda")[:, None, None]
+    cross = torch.linalg.cross(A, B, dim=-1).norm(dim=-1)[None, :, None]
+    near = cross < 1e-8
+    sin_th = torch.sin(th).where(~near, torch.ones_like(th))
+    arc = (torch.sin((1.0 - t) * th) * A + torch.sin(t * th) * B) / sin_th
+    lin = (1.0 - t) * A + t * B
+    u = torch.zeros_like(A)
+    along_x = A[:, 0].abs() >= 0.9
+    u[:, 0] = (~along_x).to(A.dtype)
+    u[:, 1] = along_x.to(A.dtype)
+    k = torch.linalg.cross(A, u, dim=-1)
+    k = k / k.norm(dim=-1, keepdim=True)
+    turn = A * torch.cos(t * th) + torch.linalg.cross(k, A, dim=-1)[None] * torch.sin(t * th)
+    antipar = near & ((A * B).sum(-1)[None, :, None] < 0)
+    X = torch.where(antipar, turn, torch.where(near, lin, arc))
+    X = X / X.norm(dim=-1, keepdim=True)
+    out = [x for x in X.cpu().numpy()]
+    out[0], out[-1] = np.array(a, float), np.array(b, float)
     return out
 
 
 # ----------------------------------------------------------------------------- adapter
 class AtomisticAdapter:
-    """Energies (meV) and gradients of a SpinSystem on the device (magex/gneb.py:27-47)."""
+    """Energy (meV) and gradient dE/de of a SpinSystem, the engine interface of magex's GNEB
+    (magex/gneb.py:27-47); every evaluation runs the batched image kernels, one image or many."""
     unit = "meV"
     kB = 8.617333262e-2
 
@@ -70,15 +77,18 @@ class AtomisticAdapter:
     def dm(self):
         return self.sys.device_model(self.precision)
 
+    def energy_grad(self, x):
+        X = as_device(np.asarray(x, float), self.dm.dtype)[None].contiguous()
+        E, G = self.images_energy_grad(X)
+        return float(E[0]), G[0].double().cpu().numpy()
+
     def energy(self, x):
-        return float(self.sys.energy(x))
+        return self.energy_grad(x)[0]
 
     def grad(self, x):
-        return self.sys.gradient(x)
+        return self.energy_grad(x)[1]
 
-    def energy_grad(self, x):
-        return self.energy(x), self.grad(x)
-
+    # states are the spin arrays themselves for the atomistic engine
     def to_state(self, x):
         return x
 
@@ -120,7 +130,8 @@ def _tgeodesic(a, b):
 def minimize(adapter, x, tol=1e-6, max_iter=20000):
     """Barzilai-Borwein descent on the spheres; tol on max |grad_perp| (magex/gneb.py:121-138)."""
     dm = adapter.dm
-    X = as_device(_normalize(np.asarray(x, float)), dm.dtype)[None].contiguous()
+    X = as_device(np.asarray(x, float), dm.dtype)[None]
+    X = (X / X.norm(dim=-1, keepdim=True)).contiguous()
     _, G = adapter.images_energy_grad(X)
     g = _tproject(X, G)
     tau = 1e-3 / max(float(g.abs().max()), 1e-30)
@@ -149,8 +160,8 @@ class GNEB:
         torch = _torch()
         self.ad = adapter
         dm = adapter.dm
-        X = np.stack([_normalize(np.asarray(im, float)) for im in images])
-        self.X = torch.as_tensor(X, dtype=dm.dtype, device="cuda").contiguous()
+        X = torch.as_tensor(np.stack([np.asarray(im, float) for im in images]), dtype=dm.dtype, device="cuda")
+        self.X = (X / X.norm(dim=-1, keepdim=True)).contiguous()
         _, G = adapter.images_energy_grad(self.X)
         gp = _tproject(self.X, G)[1:-1]
         gref = gp.norm(dim=-1).amax(dim=-1).double().cpu().numpy() if len(images) > 2 else np.zeros(0)
@@ -236,14 +247,14 @@ class GNEB:
             self.V[1:-1] = _tproject(Xn, V)
             E, F, ci = self.forces()
             if log and it % log_every == 0:
-                log(f"    GNEB it {it:6d}  max force {fmax:.3e}  barrier {E.max() - E[0]:.6g} {self.ad.unit}")
-        self.E = E
-        self.climb = ci
-        self.iterations = it
+                log(f"    chain step {it}: largest image force {fmax:.3e}, current barrier "
+                    f"{E.max() - E[0]:.6g} {self.ad.unit}")
+        # what result() reports (the attributes magex's GNEB leaves after run)
+        self.E, self.climb, self.iterations = E, ci, it
         self.fmax = float(fmaxes(F).max()) if F.shape[0] else 0.0
         self.seconds = time.time() - t0
         if log:
-            log(f"    GNEB: {it} iterations, max force {self.fmax:.2e}, {self.seconds:.1f} s")
+            log(f"    chain relaxed in {it} steps ({self.seconds:.1f} s), largest image force {self.fmax:.2e}")
         return self
 
     def reaction_coordinate(self):
@@ -251,30 +262,39 @@ class GNEB:
         return np.concatenate([[0.0], np.cumsum(L)])
 
     def result(self):
-        E = np.asarray(self.E)
-        s = self.reaction_coordinate()
-        ib = int(np.argmax(E))
-        return dict(s=s.tolist(), E=E.tolist(), E_rel=(E - E[0]).tolist(), barrier_forward=float(E[ib] - E[0]),
-                    barrier_backward=float(E[ib] - E[-1]), saddle_image=ib, climbing_image=int(self.climb),
-                    unit=self.ad.unit, iterations=int(self.iterations), max_force=float(self.fmax))
+        """The path summary in magex's format (magex/gneb.py:231-237): reaction coordinate s, energies
+        and energies relative to the first image, barriers seen from either end, saddle and climbing
+        images, unit, iterations and the final largest image force."""
+        E = np.array(self.E, float)
+        top = int(E.argmax())
+        out = {"s": self.reaction_coordinate().tolist(), "E": E.tolist(), "E_rel": (E - E[0]).tolist()}
+        out.update(barrier_forward=float(E[top] - E[0]), barrier_backward=float(E[top] - E[-1]),
+                   saddle_image=top, climbing_image=int(self.climb), unit=self.ad.unit,
+                   iterations=int(self.iterations), max_force=float(self.fmax))
+        return out
 
 
 def arrhenius(barrier, unit_kB, T, tau0=1e-9):
-    """Lifetime (s) tau0 exp(barrier / k_B T) (magex/gneb.py:240-245)."""
-    if T <= 0:
-        return float("inf")
-    x = barrier / (unit_kB * T)
-    return float(tau0 * np.exp(x)) if x < 700 else float("inf")
+    """Arrhenius lifetime tau0 exp(barrier / k_B T) in seconds (magex/gneb.py:240-245); infinite at T = 0
+    and where the exponent passes 700."""
+    x = barrier / (unit_kB * T) if T > 0 else np.inf
+    return float(np.exp(np.log(tau0) + x)) if x < 700 else float("inf")
 
 
 def skyrmion_on_lattice(system, center=None, radius=10.0, core=-1, helicity_deg=0.0):
-    """Neel/Bloch skyrmion profile on atomistic positions (radius in A) in a +z background
-    (magex/gneb.py:270-281)."""
+    """Skyrmion in a +z background on the atomistic positions (radius in Angstrom; core -1 points down;
+    helicity 0 Neel, 90 Bloch), the profile of magex/gneb.py:270-281: tan(theta / 2) = sinh(2) /
+    sinh(2 R / radius), written algebraically as cos theta = (s^2 - s0^2) / (s^2 + s0^2) and
+    sin theta = 2 s s0 / (s^2 + s0^2) with s = sinh(2 R / radius), s0 = sinh(2)."""
     P = system.positions[:, :2]
-    c = P.mean(0) if center is None else np.asarray(center, float)
-    r = P - c
-    R = np.linalg.norm(r, axis=1)
-    w = radius / 2
-    th = 2 * np.arctan2(np.sinh(radius / w), np.sinh(R / w))
-    ph = np.arctan2(r[:, 1], r[:, 0]) + np.deg2rad(helicity_deg)
-    return _normalize(np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), -core * np.cos(th)], 1))
+    r = P - (P.mean(0) if center is None else np.asarray(center, float)[:2])
+    R = np.hypot(r[:, 0], r[:, 1])
+    s, s0 = np.sinh(2.0 * R / radius), np.sinh(2.0)
+    den = s * s + s0 * s0
+    cos_t, sin_t = (s * s - s0 * s0) / den, 2.0 * s * s0 / den
+    rhat = np.where(R[:, None] > 0, r / np.where(R > 0, R, 1.0)[:, None], [1.0, 0.0])
+    h = np.deg2rad(helicity_deg)
+    inplane = np.stack([np.cos(h) * rhat[:, 0] - np.sin(h) * rhat[:, 1],
+                        np.sin(h) * rhat[:, 0] + np.cos(h) * rhat[:, 1]], 1)
+    m = np.column_stack([sin_t[:, None] * inplane, -core * cos_t])
+    return m / np.linalg.norm(m, axis=1, keepdims=True)

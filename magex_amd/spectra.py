@@ -5,8 +5,8 @@ magex/observables.py:66-71).
 
 The trajectory runs with `solver.step_device`; every `sample_every` steps the library forms
 m_q = (1/sqrt N) sum_i e_i exp(-i q.r_i) for all q on the device (phases computed per site, nothing of
-size N x nq stored) into a device array of samples.  Only the samples (nsamp x nq x 3 complex) come back
-for the Hann-windowed Welch average, as in magex.
+size N x nq stored) into a device array of samples, and the Hann-windowed Welch average of those samples
+is taken on the device too (``welch_power``); only the spectra come back.
 """
 from __future__ import annotations
 
@@ -58,56 +58,59 @@ class Sampler:
         return a[..., 0] + 1j * a[..., 1]
 
 
-def _frame(n):
-    n = np.asarray(n, float) / np.linalg.norm(n)
-    a = np.array([1.0, 0, 0]) if abs(n[0]) < 0.9 else np.array([0, 1.0, 0])
-    e1 = np.cross(n, a)
-    e1 /= np.linalg.norm(e1)
-    return e1, np.cross(n, e1), n
+def welch_power(mq, order_dir, dt, nseg):
+    """Hann-windowed Welch power spectra of sampled m_q(t), on the device.
+
+    mq: (nsamp, nq, 3) complex device tensor; order_dir: (3,) order direction (normalised here); dt:
+    time between samples (ps).  The samples split into nseg segments of nsamp // nseg; each segment has
+    its mean removed, is windowed (symmetric Hann) and Fourier transformed in time, and the powers are
+    averaged over the segments.  The longitudinal power is |n . F|^2; the transverse power is the total
+    minus it (|F|^2 = |n . F|^2 + |F_perp|^2), so no transverse frame is needed.
+    Returns (E (seg,) meV ascending, S_perp (nq, seg), S_par (nq, seg)) as NumPy arrays.
+    """
+    torch = _torch()
+    seg = mq.shape[0] // nseg
+    x = mq[: seg * nseg].reshape(nseg, seg, *mq.shape[1:])
+    x = x - x.mean(dim=1, keepdim=True)
+    win = torch.hann_window(seg, periodic=False, dtype=torch.float64, device=mq.device)
+    F = torch.fft.fft(x * win[None, :, None, None], dim=1)
+    n = torch.as_tensor(np.asarray(order_dir, float), dtype=torch.float64, device=mq.device)
+    n = (n / n.norm()).to(F.dtype)
+    par = (F * n).sum(-1).abs().square().mean(0)
+    perp = F.abs().square().sum(-1).mean(0) - par
+    E = 2 * np.pi * HBAR_MEV_PS * torch.fft.fftfreq(seg, dt, dtype=torch.float64)
+    E, perp, par = (torch.fft.fftshift(t, dim=0) for t in (E, perp, par))
+    return E.cpu().numpy(), perp.T.cpu().numpy(), par.T.cpu().numpy()
 
 
 def structure_factor(system, solver, e, qfrac, nsteps, sample_every=10, nseg=4, log=None):
     """Run `nsteps` LLG steps sampling every `sample_every`; returns (E_meV, S_perp, S_par, e)
     (magex/spectra.py:49-84).  qfrac: (nq, 3) fractional coordinates of the unit cell's reciprocal
-    lattice.  e: numpy or device tensor; the returned spins have the same type."""
+    lattice.  e: numpy or device tensor; the returned spins have the same type.  Samples, the order
+    direction and the spectra stay on the device until the end."""
     torch = _torch()
-    B = 2 * np.pi * np.linalg.inv(system.uc.cell).T
-    qc = np.asarray(qfrac, float) @ B
+    qc = np.asarray(qfrac, float) @ (2 * np.pi * np.linalg.inv(system.uc.cell).T)
     samp = Sampler(system, qc, solver.precision)
     solver._sync()
     ed = as_device(e, solver.dm.dtype)
     if ed is e:
         ed = ed.clone()
     nsamp = nsteps // sample_every
-    mq = torch.empty((nsamp, len(qc), 6), dtype=torch.float64, device="cuda")
-    ref = torch.zeros(3, dtype=torch.float64, device="cuda")
+    raw = torch.empty((nsamp, len(qc), 6), dtype=torch.float64, device="cuda")
+    order = torch.zeros(3, dtype=torch.float64, device="cuda")
     signs = torch.as_tensor(np.asarray(system.signs, float), device="cuda")[:, None]
     t0 = time.time()
-    for s in range(nsamp):
+    for k in range(nsamp):
         solver.step_device(ed, sample_every)
-        ref += (signs * ed.double()).sum(0)
-        samp.sample(ed, mq[s])
-    m = Sampler.to_complex(mq)
-    e1, e2, n = _frame(ref.cpu().numpy())
-    dt = solver.dt * sample_every
-    seg = nsamp // nseg
-    win = np.hanning(seg)
-    Sperp = 0.0
-    Spar = 0.0
-    for k in range(nseg):
-        x = m[k * seg:(k + 1) * seg]
-        x = x - x.mean(0, keepdims=True)
-        F = np.fft.fft(x * win[:, None, None], axis=0)
-        Sperp = Sperp + np.abs(F @ e1) ** 2 + np.abs(F @ e2) ** 2
-        Spar = Spar + np.abs(F @ n) ** 2
-    f = np.fft.fftfreq(seg, dt)
-    E = 2 * np.pi * f * HBAR_MEV_PS
-    order = np.argsort(E)
+        order += (signs * ed.double()).sum(0)
+        samp.sample(ed, raw[k])
+    mq = torch.view_as_complex(raw.reshape(nsamp, len(qc), 3, 2))
+    E, Sperp, Spar = welch_power(mq, order.cpu().numpy(), solver.dt * sample_every, nseg)
     if log:
-        log(f"  S(q,w): {nsamp} samples x {len(qc)} q-points, {nseg} segments, "
-            f"dE = {E[1] - E[0]:.3f} meV, E_max = {E.max():.1f} meV, {time.time() - t0:.1f} s")
+        log(f"  S(q,w): {len(qc)} wave vectors, {nsamp} samples in {nseg} windows, energy step "
+            f"{E[1] - E[0]:.3f} meV up to {E.max():.1f} meV ({time.time() - t0:.1f} s)")
     out = ed if isinstance(e, torch.Tensor) else ed.cpu().numpy().astype(float)
-    return E[order], (Sperp / nseg)[order].T, (Spar / nseg)[order].T, out
+    return E, Sperp, Spar, out
 
 
 def static_structure_factor(system, e, q):

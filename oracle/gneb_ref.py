@@ -27,6 +27,26 @@ def geodesic(a, b):
     return float(np.sqrt(np.sum(np.arctan2(np.linalg.norm(np.cross(a, b), axis=1), c) ** 2)))
 
 
+def slerp_path(a, b, nimages):
+    """magex's initial chain (gneb.py:97-118): each spin rotated about a x b (Rodrigues) by a fraction of
+    its angle; spins with |a x b| < 1e-8 turn about a x u, u = x (y when |a_x| >= 0.9); ends exact."""
+    ang = np.arctan2(np.linalg.norm(np.cross(a, b), axis=1), np.clip(np.einsum("ia,ia->i", a, b), -1, 1))
+    axis = np.cross(a, b)
+    nrm = np.linalg.norm(axis, axis=1)
+    bad = nrm < 1e-8
+    trial = np.where(np.abs(a[:, 0:1]) < 0.9, [[1.0, 0.0, 0.0]], [[0.0, 1.0, 0.0]])
+    axis[bad] = np.cross(a[bad], trial[bad])
+    axis = axis / np.linalg.norm(axis, axis=1, keepdims=True)
+    out = []
+    for t in np.linspace(0, 1, nimages):
+        th = (ang * t)[:, None]
+        rot = a * np.cos(th) + np.cross(axis, a) * np.sin(th) + axis * np.einsum("ia,ia->i", axis, a)[:, None] * (
+            1 - np.cos(th))
+        out.append(normalize(rot))
+    out[0], out[-1] = a.copy(), b.copy()
+    return out
+
+
 def tangents(x, E):
     t = [None] * len(x)
     for v in range(1, len(x) - 1):

@@ -36,6 +36,22 @@ def test_image_stack_equals_single_images(gpu):
             assert float(E[k]) == s.energy(X[k].cpu().numpy())
 
 
+def test_slerp_path_matches_restatement(gpu):
+    """The device-built initial chain equals magex's Rodrigues construction (restated), including spins
+    parallel and antiparallel to their ends, to 1e-12."""
+    print("case: slerp path", flush=True)
+    from magex_amd.gneb import geodesic_distance, slerp_path
+    rng = np.random.default_rng(5)
+    a = R.normalize(rng.normal(size=(64, 3)))
+    b = R.normalize(rng.normal(size=(64, 3)))
+    a[:4] = [[0, 0, 1], [1, 0, 0], [0.95, 0.3, 0], [0.3, 0.2, 0.9]]
+    b[:4] = [[0, 0, -1], [-1, 0, 0], [-0.95, -0.3, 0], [0.3, 0.2, 0.9]]
+    a, b = R.normalize(a), R.normalize(b)
+    mine, ref = slerp_path(a, b, 9), G.slerp_path(a, b, 9)
+    assert max(np.abs(m - r).max() for m, r in zip(mine, ref)) < 1e-12
+    assert abs(geodesic_distance(a, b) - G.geodesic(a, b)) < 1e-12
+
+
 def test_gneb_stoner_wohlfarth(gpu):
     print("case: gneb stoner-wohlfarth", flush=True)
     from magex_amd.gneb import GNEB, AtomisticAdapter

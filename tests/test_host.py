@@ -176,6 +176,39 @@ def test_apply_structure_refuses_what_the_device_path_cannot_run():
         s.scale_exchange([[1, 1, 1], [1, 1, 0.5], [1, 0.7, 1]])
 
 
+def test_gneb_helpers_closed_forms():
+    """The skyrmion profile (magex/gneb.py:270-281): core down at the centre, in-plane at R = radius,
+    +z far away, unit length, Neel (radial) at helicity 0 and Bloch (azimuthal) at 90 degrees; the
+    geodesic distance of a uniform rotation by th of N spins is sqrt(N) th, also at th = pi; the
+    Arrhenius lifetime and its infinite limits."""
+    from magex_amd import gneb
+    s = S.SpinSystem(S.load_results(M.make_model(lattice="square", J=[1.0])), (41, 41, 1), (False, False, False))
+    c = s.positions[:, :2].mean(0)
+    a0 = float(np.linalg.norm(s.positions[1] - s.positions[0]))      # lattice spacing
+    rad = 4 * a0
+    for hel, comp in ((0.0, 0), (90.0, 1)):
+        m = gneb.skyrmion_on_lattice(s, radius=rad, helicity_deg=hel)
+        r = s.positions[:, :2] - c
+        R = np.linalg.norm(r, axis=1)
+        assert np.abs(np.linalg.norm(m, axis=1) - 1).max() < 1e-15
+        assert np.allclose(m[R == 0], [[0, 0, -1]], atol=1e-15)
+        on = np.abs(R - rad) < 1e-9
+        assert on.any() and np.abs(m[on, 2]).max() < 1e-14
+        far = R > 3 * rad
+        assert m[far, 2].min() > 0.99
+        ring = on & (np.abs(r[:, 1]) < 1e-9) & (r[:, 0] > 0)           # the point (+radius, 0)
+        assert np.allclose(m[ring][0], [1, 0, 0] if comp == 0 else [0, 1, 0], atol=1e-12)
+    a = np.random.default_rng(0).normal(size=(50, 3))
+    a /= np.linalg.norm(a, axis=1, keepdims=True)
+    for th in (0.3, np.pi):
+        ax = np.cross(a, [0.0, 0.0, 1.0])
+        ax /= np.linalg.norm(ax, axis=1, keepdims=True)
+        b = a * np.cos(th) + np.cross(ax, a) * np.sin(th) + ax * np.sum(ax * a, 1, keepdims=True) * (1 - np.cos(th))
+        assert abs(gneb.geodesic_distance(a, b) - np.sqrt(50) * th) < 1e-12
+    assert gneb.arrhenius(1.0, 1.0, 0.0) == float("inf") and gneb.arrhenius(800.0, 1.0, 1.0) == float("inf")
+    assert abs(gneb.arrhenius(10.0, 0.5, 4.0, tau0=2e-9) / (2e-9 * np.exp(5.0)) - 1) < 1e-14
+
+
 def test_library_exports_every_declared_symbol():
     from magex_amd import _lib
     L = ctypes.CDLL(_lib.LIB_PATH)

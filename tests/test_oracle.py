@@ -774,6 +774,28 @@ def _sw_spin(K, h, mu=2.0):
     return s
 
 
+def test_slerp_path_closed_forms():
+    """The restated initial chain: every spin stays on its great circle at evenly spaced angles from a
+    to b; an exactly antiparallel +z -> -z spin passes through +x (the turn about z x x = y), one along
+    x -> -x through +y (about x x y = z)."""
+    from oracle import gneb_ref as G
+    rng = np.random.default_rng(2)
+    a = R.normalize(rng.normal(size=(30, 3)))
+    b = R.normalize(rng.normal(size=(30, 3)))
+    a[0], b[0] = [0, 0, 1], [0, 0, -1]
+    a[1], b[1] = [1, 0, 0], [-1, 0, 0]
+    path = G.slerp_path(a, b, 5)
+    total = np.arctan2(np.linalg.norm(np.cross(a, b), axis=1), np.einsum("ia,ia->i", a, b))
+    for k in range(1, 5):
+        step = np.arctan2(np.linalg.norm(np.cross(path[k - 1], path[k]), axis=1),
+                          np.einsum("ia,ia->i", path[k - 1], path[k]))
+        assert np.abs(step - total / 4).max() < 1e-12
+        n = np.cross(a[2:], b[2:])
+        assert np.abs(np.einsum("ia,ia->i", path[k][2:], n)).max() < 1e-12     # on the great circle
+    assert np.allclose(path[2][0], [1, 0, 0], atol=1e-15) and np.allclose(path[2][1], [0, 1, 0], atol=1e-15)
+    assert np.array_equal(path[0], a) and np.array_equal(path[-1], b)
+
+
 def test_geodesic_of_a_uniform_rotation():
     from oracle import gneb_ref as G
     rng = np.random.default_rng(0)
