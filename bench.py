@@ -122,6 +122,39 @@ def fp32_angle_error(system, p, steps=200, start="random", control=False):
     return r
 
 
+def fp32_error_curve(system, p, checkpoints=(50, 200, 1000, 2000, 5000), start="random"):
+    """Mean per-spin angle (rad) of FP32 from FP64 along one trajectory, beside the angle of an FP64 run
+    started 1e-7 rad away (about FP32's rounding of the start) from FP64: the three runs advance in lock
+    step on identical noise (the FP32 run's float-grade normals) and are compared at each checkpoint.
+    The control shows how fast the dynamics themselves amplify a difference of FP32's size; the ratio
+    FP32 / control is the part FP32 adds beyond one rounding of the start."""
+    import numpy as np
+    import torch
+    from magex_amd.llg import LLGSolver
+    e0 = system.init_spins("random", seed=0) if start == "random" else system.init_spins("ferro")
+    d = np.random.default_rng(7).normal(size=e0.shape)
+    d -= (d * e0).sum(1, keepdims=True) * e0
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    x1 = e0 + 1e-7 * d
+    runs = {}
+    for key, prec, x0 in (("fp64", "fp64", e0), ("fp32", "fp32", e0),
+                          ("control", "fp64", x1 / np.linalg.norm(x1, axis=1, keepdims=True))):
+        sol = LLGSolver(system, p["alpha"], p["dt"], p["temperature"], p["seed"], precision=prec, noise="float")
+        runs[key] = (sol, torch.as_tensor(x0, device="cuda",
+                                          dtype=torch.float32 if prec == "fp32" else torch.float64).contiguous())
+    curve, done = [], 0
+    for n in checkpoints:
+        for sol, e in runs.values():
+            sol.step_device(e, n - done)
+        done = n
+        ref = runs["fp64"][1]
+        a32 = _angle(ref, runs["fp32"][1].double())["mean_rad"]
+        actl = _angle(ref, runs["control"][1])["mean_rad"]
+        curve.append({"steps": n, "fp32_mean_rad": a32, "control_mean_rad": actl, "ratio": a32 / max(actl, 1e-300)})
+    return {"start": start, "curve": curve,
+            "note": "mean angle to FP64 on identical noise: FP32, and FP64 started 1e-7 rad away (control)"}
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -130,6 +163,27 @@ def cpu_model():
     except OSError:
         pass
     return "unknown"
+
+
+def cpu_topology():
+    """Model, sockets, physical cores and threads per core of this host (lscpu), plus the CPUs this
+    process may run on."""
+    import subprocess
+    info = {"cpus_usable": len(os.sched_getaffinity(0))}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+    except (OSError, subprocess.SubprocessError):
+        return info
+    keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+            "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+    for line in txt.splitlines():
+        k, _, v = line.partition(":")
+        if k.strip() in keys:
+            v = v.strip()
+            info[keys[k.strip()]] = int(v) if v.isdigit() else v
+    if "sockets" in info and "cores_per_socket" in info:
+        info["physical_cores"] = info["sockets"] * info["cores_per_socket"]
+    return info
 
 
 def _best_of_3(h, e, n):
@@ -222,8 +276,7 @@ def config_rates():
     # random start at T = 20 K forms a skyrmion texture: the dynamics amplify small differences, so the
     # angle error is given at two lengths, the longer with the FP64 control started 1e-7 rad away (the
     # charge Q is the same in both precisions)
-    out["tri256_fp32_vs_fp64"] = [fp32_angle_error(s, p, steps=200),
-                                  fp32_angle_error(s, p, steps=2000, control=True)]
+    out["tri256_fp32_vs_fp64"] = fp32_error_curve(s, p)
     return out
 
 
@@ -256,7 +309,7 @@ def cpu_baseline(system, p, seconds):
     return dict(value=system.N / best, unit="spin-steps/s", cores=threads, kind="port",
                 sample=f"best of 3 x {n} Heun steps of the full {system.N}-spin lattice, FP64, {threads} OpenMP "
                        f"threads (magex's fused two-pass Heun, per-bond 3x3 blocks, ziggurat noise)",
-                cpu_model=cpu_model(), ns_per_spin_step=1e9 * best / system.N,
+                cpu_model=cpu_model(), topology=cpu_topology(), ns_per_spin_step=1e9 * best / system.N,
                 one_thread={"spin_steps_per_s": one, "ns_per_spin_step": 1e9 / one, "steps": n1},
                 omp={k: os.environ.get(k) for k in ("OMP_PROC_BIND", "OMP_PLACES", "OMP_NUM_THREADS")},
                 calibration=cpu_calibration())
@@ -503,7 +556,7 @@ def main():
         # control started 1e-7 rad away shows how much of that is the dynamics rather than FP32
         n = p.get("steps", 5000)
         out["fp32_angle_error"] = {"ferro_start": fp32_angle_error(system, p, steps=n, start="ferro"),
-                                   "random_start": fp32_angle_error(system, p, steps=n, control=True)}
+                                   "random_start_curve": fp32_error_curve(system, p, start="random")}
     if r2 is not None:
         out[other] = {"spin_steps_per_s": N / (r2["step_ms"] * 1e-3), "ms_per_step": r2["step_ms"],
                       "roofline_frac": alg_bytes(system, other) * N / (r2["step_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
@@ -525,6 +578,9 @@ def main():
         print("cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(system, p, a.cpu_seconds)
         out["vs_cpu"] = value / out["cpu_baseline"]["value"]
+        if r2 is not None and other == "fp64":
+            # like for like: the FP64 kernels against the FP64 CPU restatement
+            out["vs_cpu_fp64"] = out["fp64"]["spin_steps_per_s"] / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
 
 
