@@ -117,10 +117,11 @@ int mx_llg_run(mx_model* m, void* e, int64_t nsteps, uint64_t seed, uint64_t ste
 
 /* mx_llg_run for callers that cannot allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc).  No counterpart in magex.  It calls no hipMalloc, hipFree or
- * other allocating routine: its predictor buffers, internal-order spin copy and one-launch control words
- * live in a 64 MiB device array of the library that every GPU it runs on holds until the process ends,
- * which bounds the model at 3 x align256(N x 3 x w) + 4 x (32 L0 + 4 + blocks) bytes (about 1.8 M FP32 or
- * 0.9 M FP64 sites); larger models return MX_EUNSUPPORTED.  Steps launch directly (no captured loop).
+ * other allocating routine: its predictor buffers, internal-order spin copy, one-launch control words and
+ * (step mode 2) tagged spin records live in a 64 MiB device array of the library that every GPU it runs
+ * on holds until the process ends, which bounds the model at 3 x align256(N x 3 x w) +
+ * align256(4 x (32 L0 + 4 + blocks)) (+ 8 N x w for persistent runs) bytes (about 1.8 M FP32 or 0.9 M
+ * FP64 sites); larger models return MX_EUNSUPPORTED.  Steps launch directly (no captured loop).
  * The model must be configured and its constants on the device (any other entry point run once after
  * the last set-up call), else MX_EINVAL.  Calls that overlap on one GPU (other streams, other threads,
  * other models) share that array and race; results equal mx_llg_run's bit for bit. */
@@ -199,18 +200,25 @@ int mx_sample_mq(mx_sampler* s, const void* e, double* out, void* stream);
  * No counterpart in magex (its step loop is a Python for loop, llg.py:263-280). */
 int mx_model_set_graph_steps(mx_model* m, int32_t steps);
 
-/* How mx_llg_run and mx_llg_run_driven's uniform-field steps launch: mode 0 = two kernels per step (the
- * predictor pass, then the corrector pass: magex's _fused1 / _fused2 pair, fast.py:336-390); mode 1
- * (default) = one kernel per step where the model allows it (row-major cells; table couplings, or FP32
- * per-bond tensors with at most 14 stencil entries per sublattice; axis-0 planes small enough that the
- * workgroups a tile waits for are always resident), else two.  In one launch each workgroup keeps its
- * sites' couplings and state in registers from the predictor to the corrector and waits only for the
- * predictor spins of the axis-0 planes its stencil reaches, so per-bond tensors come from memory once
- * per step instead of twice.  Results are identical in either mode.  No counterpart in magex. */
+/* How mx_llg_run, mx_llg_run_noalloc and mx_llg_run_driven's uniform-field steps launch: mode 0 = two
+ * kernels per step (the predictor pass, then the corrector pass: magex's _fused1 / _fused2 pair,
+ * fast.py:336-390); mode 1 = one kernel per step where the model allows it (row-major cells; table
+ * couplings, or FP32 per-bond tensors with at most 14 stencil entries per sublattice; a grid of at most
+ * one workgroup per compute unit; FP32, or FP64 up to 16384 sites), else two; mode 2 (default) = as
+ * mode 1, and mx_llg_run / mx_llg_run_noalloc run two or more steps of a model with row-major cells, at
+ * most one workgroup per compute unit and at most 24 MiB of stencil records per poll (N x stencil
+ * entries x 16 bytes FP32, 32 bytes FP64; any coupling form) in persistent cooperative launches of up to
+ * 1024 steps, each workgroup keeping its sites' state in registers across them and exchanging spins
+ * with its neighbours through tagged records.  In one launch per step each workgroup keeps its sites'
+ * couplings and state in registers from the predictor to the corrector and waits only for the predictor
+ * spins of the axis-0 planes its stencil reaches, so per-bond tensors come from memory once per step
+ * instead of twice.  Results are identical in every mode.  No counterpart in magex. */
 int mx_model_set_step_mode(mx_model* m, int32_t mode);
-/* one_launch: whether mx_llg_run uses one kernel per step for this model (mode and model permitting);
- * given_up: waits of one-launch steps that timed out since the model was created, each finished by the
- * launch's last workgroup instead (0 in normal operation; synchronises the device).  Either may be NULL. */
+/* one_launch: how mx_llg_run steps this model (mode and model permitting): 0 two kernels per step, 1 one
+ * kernel per step, 2 runs of two or more steps in persistent launches; given_up: waits of one-launch
+ * steps that timed out since the model was created, each finished by the launch's last workgroup
+ * instead, plus persistent launches that stopped early on a timed-out wait (0 in normal operation;
+ * synchronises the device).  Either may be NULL. */
 int mx_model_step_info(mx_model* m, int32_t* one_launch, int64_t* given_up);
 
 /* Grade of the thermal-noise generator (the counter-based stream that replaces magex's

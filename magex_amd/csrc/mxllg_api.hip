@@ -160,6 +160,11 @@ static void ensure_scratch(Model& m) {
     hip_ok(hipMalloc(reinterpret_cast<void**>(&m.stepsync_d), n), "hipMalloc step control words");
     hip_ok(hipMemset(m.stepsync_d, 0, n), "hipMemset step control words");
   }
+  if (!m.rec_d && persistent_steps(m)) {
+    const size_t n = rec_words(m) * sizeof(uint64_t);
+    hip_ok(hipMalloc(reinterpret_cast<void**>(&m.rec_d), n), "hipMalloc tagged spin records");
+    hip_ok(hipMemset(m.rec_d, 0, n), "hipMemset tagged spin records");
+  }
   if (!m.part_d) {
     const int nblk = m.ns * ((m.ncell + BLOCK - 1) / BLOCK);
     m.npart = std::max(nblk, (2 * m.N + BLOCK - 1) / BLOCK + 1);
@@ -237,9 +242,9 @@ static void ensure_graph(Model& m, void* E, uint64_t seed) {
 }
 
 // ---------------------------------------------------------------------- allocation-free step scratch
-// The scratch of mx_llg_run_noalloc: one call's predictor buffers, internal-order spin copy and
-// one-launch control words, in a file-scope device array that every GPU the library runs on holds until
-// the process ends.  In host code `scratch` is only a stand-in symbol: its device address on each GPU
+// The scratch of mx_llg_run_noalloc: one call's predictor buffers, internal-order spin copy, one-launch
+// control words and tagged spin records, in a file-scope device array that every GPU the library runs on
+// holds until the process ends.  In host code `scratch` is only a stand-in symbol: its device address on each GPU
 // comes from hipGetSymbolAddress (once per device, that device current) and only that address is used.
 constexpr size_t SCRATCH_BYTES = size_t(64) << 20;
 static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
@@ -326,6 +331,7 @@ void mx_model_destroy(mx_model* m) {
     dfree(*p);
   if (m->perm_d) (void)hipFree(m->perm_d);
   if (m->stepsync_d) (void)hipFree(m->stepsync_d);
+  if (m->rec_d) (void)hipFree(m->rec_d);
   if (m->part_d) (void)hipFree(m->part_d);
   drop_graph(*m);
   if (m->counter_d) (void)hipFree(m->counter_d);
@@ -746,7 +752,8 @@ int mx_model_set_noise(mx_model* m, int32_t grade) {
 int mx_model_set_step_mode(mx_model* m, int32_t mode) {
   return guarded([&]() -> int {
     need_model(m);
-    if (mode < 0 || mode > 1) fail(MX_EINVAL, "step mode must be 0 (two launches) or 1 (one launch)");
+    if (mode < 0 || mode > 2)
+      fail(MX_EINVAL, "step mode must be 0 (two launches), 1 (one launch per step) or 2 (persistent runs)");
     if (mode != m->step_mode) {
       use_device(*m);
       drop_graph(*m);
@@ -760,12 +767,12 @@ int mx_model_step_info(mx_model* m, int32_t* one_launch, int64_t* given_up) {
   return guarded([&]() -> int {
     need_model(m);
     use_device(*m);
-    if (one_launch) *one_launch = fused_step(*m) ? 1 : 0;
+    if (one_launch) *one_launch = persistent_steps(*m) ? 2 : (fused_step(*m) ? 1 : 0);
     if (given_up) {
       uint32_t v = 0;
       if (m->stepsync_d) {
         hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize");
-        hip_ok(hipMemcpy(&v, m->stepsync_d + size_t(m->L[0]) * CNT_STRIDE + 2, sizeof(v), hipMemcpyDeviceToHost),
+        hip_ok(hipMemcpy(&v, m->stepsync_d + stepsync_ctl(*m) + 2, sizeof(v), hipMemcpyDeviceToHost),
                "hipMemcpy");
       }
       *given_up = int64_t(v);
@@ -794,7 +801,12 @@ int mx_llg_run(mx_model* m, void* e, int64_t nsteps, uint64_t seed, uint64_t ste
     const size_t stride = size_t(m->N) * 3 * rsize(*m);
     int64_t n = 0;
     const int G = m->graph_steps;
-    if (!noise && G > 0 && nsteps >= 2 * int64_t(G)) {
+    if (nsteps >= 2 && persistent_steps(*m)) {
+      // the whole run in persistent launches (every workgroup resident, couplings held across steps)
+      int rc = launch_heun_steps(*m, E, seed, step0, nsteps, noise, s);
+      if (rc) return rc;
+      n = nsteps;
+    } else if (!noise && G > 0 && nsteps >= 2 * int64_t(G)) {
       // replay a captured loop of G steps; the kernels take the step number from a device counter
       ensure_graph(*m, E, seed);
       int rc = launch_counter(m->counter_d, step0, 0, s);
@@ -822,18 +834,26 @@ int mx_llg_run_noalloc(mx_model* m, void* e, int64_t nsteps, uint64_t seed, uint
       fail(MX_EINVAL, "model constants not on the device: run any other entry point once after configuring");
     const size_t spins = align256(size_t(m->N) * 3 * rsize(*m));
     const size_t words = stepsync_words(*m);
-    const size_t need = 3 * spins + words * sizeof(uint32_t);
+    const bool persistent = nsteps >= 2 && persistent_steps(*m);
+    const size_t recs = persistent ? rec_words(*m) : 0;
+    const size_t need = 3 * spins + align256(words * sizeof(uint32_t)) + recs * sizeof(uint64_t);
     if (need > SCRATCH_BYTES)
       fail(MX_EUNSUPPORTED, "model too large for the allocation-free scratch (" + std::to_string(need) + " bytes of " +
                                 std::to_string(SCRATCH_BYTES) + "): use mx_llg_run");
     use_device(*m);
     unsigned char* base = scratch_of(m->device);
-    StepBufs bufs{base, base + spins, reinterpret_cast<uint32_t*>(base + 3 * spins)};
+    StepBufs bufs{base, base + spins, reinterpret_cast<uint32_t*>(base + 3 * spins),
+                  reinterpret_cast<uint64_t*>(base + 3 * spins + align256(words * sizeof(uint32_t)))};
     void* eint = base + 2 * spins;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    // the control words start every call at zero (a kernel, so the call can sit in captured work)
+    // the control words and the tagged records (which other models' calls may have left) start every
+    // call at zero (a kernel, so the call can sit in captured work)
     int rc = launch_fill_u32(bufs.sync, words, s);
     if (rc) return rc;
+    if (recs) {
+      rc = launch_fill_u32(reinterpret_cast<uint32_t*>(bufs.rec), 2 * recs, s);
+      if (rc) return rc;
+    }
     void* E = e;
     if (needs_permute(*m)) {
       rc = launch_permute(*m, e, eint, 1, s);
@@ -841,6 +861,11 @@ int mx_llg_run_noalloc(mx_model* m, void* e, int64_t nsteps, uint64_t seed, uint
       E = eint;
     }
     const size_t stride = size_t(m->N) * 3 * rsize(*m);
+    if (persistent) {
+      rc = launch_heun_steps(*m, E, seed, step0, nsteps, noise, s, &bufs);
+      if (rc) return rc;
+      nsteps = 0;
+    }
     for (int64_t n = 0; n < nsteps; ++n) {
       const void* nz = noise ? static_cast<const char*>(noise) + size_t(n) * stride : nullptr;
       rc = launch_heun_step(*m, E, seed, step0 + uint64_t(n), nz, s, nullptr, &bufs);

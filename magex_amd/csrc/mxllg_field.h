@@ -189,6 +189,53 @@ __device__ __forceinline__ R spin_load(const R* p) {
   else return *p;
 }
 
+// Tagged spin records of the persistent step launch: a spin's bits (96 in FP32, 192 in FP64) cut into
+// 48-bit chunks, each stored in a 64-bit word under a 16-bit tag (the high bits), so that one
+// single-copy-atomic 64-bit access carries a chunk and its tag.  FP32: 2 words; FP64: 4.
+template <typename R>
+constexpr int TAG_WORDS = sizeof(R) == 8 ? 4 : 2;
+constexpr uint64_t CHUNK48 = (1ull << 48) - 1;
+
+template <typename R>
+__device__ __forceinline__ void tag3(R x, R y, R z, uint32_t tag, uint64_t (&w)[TAG_WORDS<R>]) {
+  const uint64_t t = uint64_t(tag) << 48;
+  if constexpr (sizeof(R) == 8) {
+    const uint64_t b0 = __builtin_bit_cast(uint64_t, x), b1 = __builtin_bit_cast(uint64_t, y),
+                   b2 = __builtin_bit_cast(uint64_t, z);
+    w[0] = t | (b0 & CHUNK48);
+    w[1] = t | (b0 >> 48) | ((b1 & 0xffffffffull) << 16);
+    w[2] = t | (b1 >> 32) | ((b2 & 0xffffull) << 32);
+    w[3] = t | (b2 >> 16);
+  } else {
+    const uint64_t u0 = __builtin_bit_cast(uint32_t, x), u1 = __builtin_bit_cast(uint32_t, y),
+                   u2 = __builtin_bit_cast(uint32_t, z);
+    w[0] = t | u0 | ((u1 & 0xffffull) << 32);
+    w[1] = t | (u1 >> 16) | (u2 << 16);
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ void untag3(const uint64_t (&w)[TAG_WORDS<R>], R& x, R& y, R& z) {
+  if constexpr (sizeof(R) == 8) {
+    x = __builtin_bit_cast(double, (w[0] & CHUNK48) | ((w[1] & 0xffffull) << 48));
+    y = __builtin_bit_cast(double, ((w[1] >> 16) & 0xffffffffull) | ((w[2] & 0xffffffffull) << 32));
+    z = __builtin_bit_cast(double, ((w[2] >> 32) & 0xffffull) | ((w[3] & CHUNK48) << 16));
+  } else {
+    x = __builtin_bit_cast(float, uint32_t(w[0]));
+    y = __builtin_bit_cast(float, uint32_t(((w[0] >> 32) & 0xffffull) | ((w[1] & 0xffffull) << 16)));
+    z = __builtin_bit_cast(float, uint32_t((w[1] >> 16) & 0xffffffffull));
+  }
+}
+
+// Publish a spin as tagged record p (agent-scope 64-bit stores).
+template <typename R>
+__device__ __forceinline__ void publish_tagged(uint64_t* p, R x, R y, R z, uint32_t tag) {
+  uint64_t w[TAG_WORDS<R>];
+  tag3<R>(x, y, z, tag, w);
+#pragma unroll
+  for (int q = 0; q < TAG_WORDS<R>; ++q) __hip_atomic_store(p + q, w[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Entries per batch of the stencil loop: every load of a batch is issued before the first is used, so
 // a wavefront keeps CH gathers (and, per bond, CH tensors) in flight instead of one.  Per-bond batches
 // were timed on MI355X with the 14-entry bcc stencil: FP32 7 (two full batches; 5, 6, 8 and 9 were
@@ -220,12 +267,12 @@ struct Batch {
     for (int u = 0; u < CH; ++u) raw[u] = S->ent[k0 + u];
   }
 
-  template <int LD = 0>
+  template <int LD = 0, bool WITH_J = true>
   __device__ __forceinline__ void gather(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
                                          const Site& st, int k0, int ke) {
     Ent raw[CH];
     records(S, k0, raw);
-    gather<LD>(S, d, e, st, k0, ke, raw);
+    gather<LD, WITH_J>(S, d, e, st, k0, ke, raw);
   }
 
   // WITH_J = false reloads only the spins (the couplings of these entries are already in J).
@@ -267,6 +314,85 @@ struct Batch {
 #pragma unroll
         for (int q = 0; q < NJ; ++q) J[u][q] = T[q];
       }
+    }
+  }
+
+  // As gather(), with the neighbours' spins taken from tagged records instead of a spin array: record j
+  // holds TAG_WORDS<R> words (see tag3), read with agent-scope 64-bit loads (single-copy atomic).  Each
+  // live entry is re-read until all its words carry the tag `want`; entries past the stencil's end and
+  // neighbours outside an open boundary are not read (their spins count zero).  False when a lane gives
+  // up: after `spin` unsuccessful polls, or once *abort is raised.
+  template <bool WITH_J>
+  __device__ __forceinline__ bool gather_tagged(const Static<R>* __restrict__ S, const Dyn<R>& d,
+                                                const uint64_t* __restrict__ rec, const Site& st, int k0, int ke,
+                                                uint32_t want, uint32_t spin, const uint32_t* abort) {
+    constexpr int NW = TAG_WORDS<R>;
+    static_assert(CH <= 32, "pending entries are a 32-bit mask");
+    Ent raw[CH];
+    records(S, k0, raw);
+    const int ncell = S->ncell;
+    int jj[CH];
+    uint32_t pend = 0;
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const bool live = k0 + u < ke;
+      Ent en;
+      en.slot = live ? raw[u].slot : 0;
+      en.tb = live ? raw[u].tb : st.s;
+      en.d01 = live ? raw[u].d01 : 0;
+      en.d2 = live ? raw[u].d2 : 0;
+      bool ok;
+      const int cn = neighbour_cell<R, GEO>(S, st, en, ok);
+      if (ok && live) pend |= 1u << u;
+      jj[u] = (en.tb & 0xffff) * ncell + cn;
+      ex[u] = ey[u] = ez[u] = R(0);
+      if constexpr (!WITH_J) {
+      } else if constexpr (CM == COUP_BOND) {
+        const bool back = (en.tb >> 16) != 0;
+        const R* Jb = d.bond + (size_t(en.slot) * ncell + (back ? cn : st.c)) * 9;
+        R t[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) t[q] = Jb[q];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) J[u][q] = back ? t[(q % 3) * 3 + q / 3] : t[q];
+      } else {
+        const R* T = S->tab + NJ * en.slot;
+#pragma unroll
+        for (int q = 0; q < NJ; ++q) J[u][q] = T[q];
+      }
+    }
+    for (uint32_t it = 0;; ++it) {
+      // every live entry's words in flight at once (a branch per entry would wait on each in turn)
+      uint64_t w[CH][NW];
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        if (k0 + u < ke) {  // wavefront-uniform
+          const uint64_t* r = rec + size_t(jj[u]) * NW;
+#pragma unroll
+          for (int q = 0; q < NW; ++q)
+            w[u][q] = __hip_atomic_load(const_cast<uint64_t*>(r + q), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        if (k0 + u < ke) {
+          bool g = true;
+#pragma unroll
+          for (int q = 0; q < NW; ++q) g = g && uint32_t(w[u][q] >> 48) == want;
+          R x, y, z;
+          untag3<R>(w[u], x, y, z);
+          const bool take = g && (pend >> u & 1u);
+          ex[u] = take ? x : ex[u];
+          ey[u] = take ? y : ey[u];
+          ez[u] = take ? z : ez[u];
+          pend &= g ? ~(1u << u) : ~0u;
+        }
+      }
+      if (!pend) return true;
+      if (it >= spin) return false;
+      if ((it & 15) == 15 && __hip_atomic_load(const_cast<uint32_t*>(abort), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        return false;
+      __builtin_amdgcn_s_sleep(2);
     }
   }
 
@@ -346,25 +472,29 @@ __device__ __forceinline__ void exchange(const Static<R>* __restrict__ S, const 
 template <typename R>
 struct SiteA {
   R v[9];
+  // Every element is written on every path (zero where the mode has none): stores that differ between
+  // the branches would be merged behind a computed address, which keeps a holder of SiteA (and all it
+  // is part of) in scratch memory instead of registers.
   __device__ __forceinline__ void load(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st) {
     const int amode = S->amode;
-    if (amode == ONSITE_SITE6) {
-      const int N = S->N, i = st.i;
-      const R* A = d.A;
-      if constexpr (sizeof(R) == 8) {
-        // FP64: the per-site tensor is read once per pass; streaming it past L2 leaves room for the
-        // partner bond tensors (measured 4-6 % faster on bcc64; slower in FP32, where it stays cached)
+    const int nk = amode == ONSITE_SITE6 ? 6 : (amode == ONSITE_SITE9 ? 9 : 0);
+    const size_t N = size_t(S->N);
+    const R* a = d.A + st.i;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) v[k] = __builtin_nontemporal_load(A + size_t(k) * N + i);
-      } else {
-#pragma unroll
-        for (int k = 0; k < 6; ++k) v[k] = A[size_t(k) * N + i];
+    for (int k = 0; k < 9; ++k) {
+      R t = R(0);
+      if (k < nk) {
+        if constexpr (sizeof(R) == 8) {
+          // FP64: the per-site six-component tensor is read once per pass; streaming it past L2 leaves
+          // room for the partner bond tensors (measured 4-6 % faster on bcc64; slower in FP32, where it
+          // stays cached)
+          if (amode == ONSITE_SITE6) t = __builtin_nontemporal_load(a + size_t(k) * N);
+          else t = a[size_t(k) * N];
+        } else {
+          t = a[size_t(k) * N];
+        }
       }
-    } else if (amode == ONSITE_SITE9) {
-      const int N = S->N;
-      const R* a = d.A + st.i;
-#pragma unroll
-      for (int k = 0; k < 9; ++k) v[k] = a[size_t(k) * N];
+      v[k] = t;
     }
   }
 };

@@ -90,15 +90,16 @@ struct Dyn {
   R Bc0, Bc1, Bc2;           // uniform field of the corrector in the one-launch step (B0..B2: predictor)
 };
 
-// Control words of the one-launch Heun step (k_step) in model-owned device memory.  Every word is zero
-// between launches: the last workgroup of a launch clears what the launch used.
-// The plane counters sit one per 128-byte line: hundreds of wavefronts poll the few counters around
-// their plane while others increment them.
+// Control words of the one-launch Heun steps (k_step, k_steps) in model-owned device memory: L0 plane
+// counters, 4 control words, one give-up slot per workgroup.  Every word is zero between launches: the
+// last workgroup of a launch clears what the launch used.  The plane counters sit one per 128-byte line:
+// hundreds of wavefronts poll the few counters around their plane while others increment them.
 constexpr int CNT_STRIDE = 32;
 struct StepSync {
   uint32_t* cnt;  // per axis-0 plane (stride CNT_STRIDE): workgroups whose predictor spins covering the
-                  // plane are published
-  uint32_t* ctl;  // [0] workgroups finished, [1] waits given up in this launch, [2] given up since creation
+                  // plane are published (k_step)
+  uint32_t* ctl;  // [0] workgroups finished, [1] waits given up in this launch, [2] given up (k_step) or
+                  // launches aborted (k_steps) since creation, [3] abort raised in this launch (k_steps)
   int32_t* gu;    // blocks whose wait gave up (their corrector runs in the launch's last workgroup)
   int r0;         // stencil reach along axis 0, in planes
   int PS;         // cells per axis-0 plane
@@ -169,10 +170,15 @@ struct Model {
   int mc_repe_cap = 0;
   double* img_part_d = nullptr;  // partials of image-stack observables
   int img_part_cap = 0;          // images they hold
-  // one-launch step (k_step): mode (0: two launches per step; 1: one launch where the model allows it),
-  // control words
-  int step_mode = 1;
+  // one-launch steps: mode (0: two launches per step; 1: one launch per step where the model allows it;
+  // 2, the default: also runs of steps in persistent launches where they pay), control words
+  int step_mode = 2;
   uint32_t* stepsync_d = nullptr;  // L0 plane counters, 4 control words, nblk give-up slots
+  // persistent launches (k_steps): tagged spin records (2 x N x TAG_WORDS 64-bit words: spins, predictor
+  // spins; zeroed when allocated) and the epoch of the last launch on the model (each launch takes the
+  // next, 1 .. 15, so that no record a launch finds matches a tag it waits for)
+  uint64_t* rec_d = nullptr;
+  mutable uint32_t rec_epoch = 0;
   // captured step loop (hipGraph of graph_steps Heun steps + counter advance), keyed by what it bakes in
   int graph_steps = 0;
   uint64_t* counter_d = nullptr;
@@ -187,6 +193,7 @@ struct StepBufs {
   void* ep;
   void* k1;
   uint32_t* sync;
+  uint64_t* rec;  // tagged spin records of persistent launches (rec_words(m) words, zero or tagged)
 };
 
 // Kernel launchers (mxllg_kernels.hip).  All take internal-order device buffers unless noted.
@@ -199,8 +206,20 @@ int launch_fill_u32(uint32_t* p, size_t n, hipStream_t s);
 int launch_counter(uint64_t* c, uint64_t value, int add, hipStream_t s);
 // Whether a step of m runs as one launch (k_step), and the number of held coupling batches it uses.
 bool fused_step(const Model& m, int* nb = nullptr);
-// Device words of the one-launch step (allocated and zeroed with the model's scratch).
+// Whether runs of steps of m run in persistent launches (k_steps: step mode 2, row-major cells, a grid of
+// at most one workgroup per compute unit, a small round of polls).
+bool persistent_steps(const Model& m);
+// nsteps steps on e in persistent launches of up to STEPS_PER_LAUNCH steps (persistent_steps(m) must hold);
+// noise: nsteps x N x 3 normals or null; bufs = null: the model's own scratch and records.
+constexpr int STEPS_PER_LAUNCH = 1024;
+int launch_heun_steps(const Model& m, void* e, uint64_t seed, uint64_t step0, int64_t nsteps, const void* noise,
+                      hipStream_t s, const StepBufs* bufs = nullptr);
+// Device words of the one-launch steps (allocated and zeroed with the model's scratch), and the offset of
+// the control words among them.
 size_t stepsync_words(const Model& m);
+inline size_t stepsync_ctl(const Model& m) { return size_t(m.L[0]) * CNT_STRIDE; }
+// 64-bit words of the tagged spin records of persistent launches.
+inline size_t rec_words(const Model& m) { return 2 * size_t(m.N) * (m.prec == MX_FP64 ? 4 : 2); }
 // One Heun pass (1 or 2) with optional stores of e and B_eff (magex order is not used: internal order).
 // B_ext (3 doubles) replaces the model's uniform field for this pass when non-null; extra (device, N x 3
 // in the model precision, magex site order) is added to the field of every site when non-null.

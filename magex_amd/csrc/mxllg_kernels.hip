@@ -109,34 +109,216 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S,
 }
 
 // ------------------------------------------------------------------------------------------ one-launch step
-// Wavefront 0 polls the plane counters of planes [lo, lo + n) (wrapped onto [0, L0)) until each plane has
-// been published by every workgroup whose tile covers it; false after `spin` unsuccessful polls.
+// Wavefront 0 polls the plane counters `cnt` of planes [lo, lo + n) (wrapped onto [0, L0)) until each
+// reaches `times` x the number of workgroups whose tiles cover the plane; false after `spin` unsuccessful
+// polls or once the launch's abort word (ctl[3]) is raised.
 template <typename R>
-__device__ __forceinline__ bool wait_planes(const Static<R>* __restrict__ S, const StepSync& sy, int lo, int n) {
+__device__ __forceinline__ bool wait_planes(const Static<R>* __restrict__ S, const StepSync& sy,
+                                            const uint32_t* cnt, uint32_t times, int lo, int n) {
   const int L0 = S->L0, PS = sy.PS, ns = S->ns, lane = threadIdx.x & 63;
   for (uint32_t it = 0;; ++it) {
     int ok = 1;
     for (int k = lane; k < n; k += 64) {
       int P = lo + k;
       P = P >= L0 ? P - L0 : P;
-      const uint32_t want = uint32_t(ns * (((P + 1) * PS - 1) / BLOCK - (P * PS) / BLOCK + 1));
-      const uint32_t c = __hip_atomic_load(sy.cnt + P * CNT_STRIDE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t want = times * uint32_t(ns * (((P + 1) * PS - 1) / BLOCK - (P * PS) / BLOCK + 1));
+      const uint32_t c = __hip_atomic_load(const_cast<uint32_t*>(cnt) + P * CNT_STRIDE, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
       ok = ok && c >= want;
     }
     if (__all(ok)) return true;
     if (it >= sy.spin) return false;
+    if ((it & 15) == 15 && __hip_atomic_load(sy.ctl + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
     __builtin_amdgcn_s_sleep(8);
+  }
+}
+
+// State a workgroup of the one-launch kernels keeps per site from the predictor to the corrector: the
+// coupling batches (per-bond tensors when NB > 0), the per-site on-site tensor, damping, the spin e, the
+// predictor spin ep, w = e + dt/2 k1 (Heun) or the rotation vector (Depondt), and the thermal field.
+template <typename R, int CM, int GEO, int NB>
+struct Held {
+  Batch<R, CM, GEO> bt[NB > 0 ? NB : 1];
+  SiteA<R> A;
+  R a, x, y, z, px, py, pz, wx, wy, wz, nx, ny, nz;
+};
+
+// Where the one-launch kernels take the neighbours' spins from: a spin array read with load policy LD
+// (SpinsFrom), or tagged records polled until they carry the tag `want` (TaggedFrom).
+template <typename R, int LD>
+struct SpinsFrom {
+  static constexpr bool tagged = false;
+  static constexpr int LD_ = LD;
+  const R* e;
+};
+struct TaggedFrom {
+  static constexpr bool tagged = true;
+  const uint64_t* rec;
+  uint32_t want, spin;
+  const uint32_t* abort;
+};
+
+template <typename R, int CM, int GEO, bool WITH_J, typename Src>
+__device__ __forceinline__ bool fill_batch(Batch<R, CM, GEO>& bt, const Static<R>* __restrict__ S, const Dyn<R>& d,
+                                           const Src& src, const Site& st, int k0, int ke) {
+  if constexpr (Src::tagged) {
+    return bt.template gather_tagged<WITH_J>(S, d, src.rec, st, k0, ke, src.want, src.spin, src.abort);
+  } else {
+    bt.template gather<Src::LD_, WITH_J>(S, d, src.e, st, k0, ke);
+    return true;
+  }
+}
+
+// Predictor of site st (magex/fast.py:336-368) with the arithmetic of k_heun's pass 1: gathers the
+// neighbours' spins from src (with the couplings unless LOADJ is false, i.e. already held), draws the
+// thermal field of `step`, and leaves ep and w (or the rotation vector) in h.  h.x, h.y, h.z hold e.
+// False if a tagged gather gave up.
+template <typename R, int CM, int GEO, int NB, bool LEAN, bool LOADJ, typename Src>
+__device__ __forceinline__ bool predictor_held(const Static<R>* __restrict__ S, const Dyn<R>& d, const Src& src,
+                                               const Site& st, int kb, int ke, uint64_t step,
+                                               const R* __restrict__ noise, Held<R, CM, GEO, NB>& h) {
+  constexpr int CH = Batch<R, CM, GEO>::CH;
+  R hx = R(0), hy = R(0), hz = R(0), gx, gy, gz, bx, by, bz, fx, fy, fz;
+  bool ok = true;
+  h.nx = h.ny = h.nz = R(0);
+  if constexpr (NB > 0) {
+    // every held batch in flight before the first is used
+#pragma unroll
+    for (int n = 0; n < NB; ++n) {
+      const bool g = fill_batch<R, CM, GEO, LOADJ>(h.bt[n], S, d, src, st, kb + n * CH, ke);
+      ok = ok && g;
+    }
+    if (d.noisy) thermal<R, LEAN>(S, d, st, step, noise, h.nx, h.ny, h.nz);
+#pragma unroll
+    for (int n = 0; n < NB; ++n)
+      if (kb + n * CH < ke) h.bt[n].accumulate(S, hx, hy, hz);
+  } else if constexpr (Src::tagged) {
+    if (d.noisy) thermal<R, LEAN>(S, d, st, step, noise, h.nx, h.ny, h.nz);
+    for (int k0 = kb; k0 < ke; k0 += CH) {
+      Batch<R, CM, GEO> bt;
+      const bool g = fill_batch<R, CM, GEO, true>(bt, S, d, src, st, k0, ke);
+      ok = ok && g;
+      bt.accumulate(S, hx, hy, hz);
+    }
+  } else {
+    exchange<R, CM, GEO, 0, Src::LD_>(S, d, src.e, st, hx, hy, hz, [&] {
+      if (d.noisy) thermal<R, LEAN>(S, d, st, step, noise, h.nx, h.ny, h.nz);
+    });
+  }
+  if (LOADJ) h.A.load(S, d, st);
+  const R x = h.x, y = h.y, z = h.z;
+  onsite_grad<R, LEAN>(S, d, st, h.A, x, y, z, gx, gy, gz);
+  field_from<R, LEAN>(S, d, st, x, y, z, hx, hy, hz, gx, gy, gz, h.nx, h.ny, h.nz, d.B0, d.B1, d.B2, bx, by, bz);
+  llg_rhs(x, y, z, bx, by, bz, h.a, d.prec, fx, fy, fz);
+  if (!LEAN && d.integ == MX_DEPONDT) {
+    omega(x, y, z, fx, fy, fz, h.wx, h.wy, h.wz);
+    h.px = x;
+    h.py = y;
+    h.pz = z;
+    rotate(h.px, h.py, h.pz, d.dt * h.wx, d.dt * h.wy, d.dt * h.wz);
+  } else {
+    const R hdt = R(0.5) * d.dt;
+    h.wx = x + hdt * fx;
+    h.wy = y + hdt * fy;
+    h.wz = z + hdt * fz;
+    const R qx = x + d.dt * fx, qy = y + d.dt * fy, qz = z + d.dt * fz;
+    const R q = rnorm3(qx, qy, qz);
+    h.px = qx * q;
+    h.py = qy * q;
+    h.pz = qz * q;
+  }
+  return ok;
+}
+
+// Corrector of site st (magex/fast.py:370-390) with the arithmetic of k_heun's pass 2, from the held
+// state and the neighbours' predictor spins from src; the new spin goes to h.x, h.y, h.z.  False if a
+// tagged gather gave up.
+template <typename R, int CM, int GEO, int NB, bool LEAN, typename Src>
+__device__ __forceinline__ bool corrector_held(const Static<R>* __restrict__ S, const Dyn<R>& d, const Src& src,
+                                               const Site& st, int kb, int ke, Held<R, CM, GEO, NB>& h) {
+  constexpr int CH = Batch<R, CM, GEO>::CH;
+  R hx = R(0), hy = R(0), hz = R(0), gx, gy, gz, bx, by, bz, fx, fy, fz;
+  bool ok = true;
+  if constexpr (NB > 0) {
+#pragma unroll
+    for (int n = 0; n < NB; ++n) {
+      const bool g = fill_batch<R, CM, GEO, false>(h.bt[n], S, d, src, st, kb + n * CH, ke);
+      ok = ok && g;
+    }
+#pragma unroll
+    for (int n = 0; n < NB; ++n)
+      if (kb + n * CH < ke) h.bt[n].accumulate(S, hx, hy, hz);
+  } else if constexpr (Src::tagged) {
+    for (int k0 = kb; k0 < ke; k0 += CH) {
+      Batch<R, CM, GEO> bt;
+      const bool g = fill_batch<R, CM, GEO, true>(bt, S, d, src, st, k0, ke);
+      ok = ok && g;
+      bt.accumulate(S, hx, hy, hz);
+    }
+  } else {
+    exchange<R, CM, GEO, 0, Src::LD_>(S, d, src.e, st, hx, hy, hz, [] {});
+  }
+  const R px = h.px, py = h.py, pz = h.pz;
+  onsite_grad<R, LEAN>(S, d, st, h.A, px, py, pz, gx, gy, gz);
+  field_from<R, LEAN>(S, d, st, px, py, pz, hx, hy, hz, gx, gy, gz, h.nx, h.ny, h.nz, d.Bc0, d.Bc1, d.Bc2, bx, by,
+                      bz);
+  llg_rhs(px, py, pz, bx, by, bz, h.a, d.prec, fx, fy, fz);
+  const R hdt = R(0.5) * d.dt;
+  R cx, cy, cz;
+  if (!LEAN && d.integ == MX_DEPONDT) {
+    R vx, vy, vz;
+    omega(px, py, pz, fx, fy, fz, vx, vy, vz);
+    cx = h.x;
+    cy = h.y;
+    cz = h.z;
+    rotate(cx, cy, cz, hdt * (h.wx + vx), hdt * (h.wy + vy), hdt * (h.wz + vz));
+  } else {
+    cx = h.wx + hdt * fx;
+    cy = h.wy + hdt * fy;
+    cz = h.wz + hdt * fz;
+  }
+  const R q = rnorm3(cx, cy, cz);
+  h.x = cx * q;
+  h.y = cy * q;
+  h.z = cz * q;
+  return ok;
+}
+
+template <typename R>
+__device__ __forceinline__ void store3_coherent(R* p, R x, R y, R z) {
+  __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p + 1, y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p + 2, z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Planes a workgroup's stencil reaches: [lo, lo + n) wrapped onto [0, L0), from its tile's planes
+// [pa, pb] and the reach r0 along axis 0.
+template <typename R>
+__device__ __forceinline__ void wait_window(const Static<R>* __restrict__ S, int pa, int pb, int r0, int& lo, int& n) {
+  const int L0 = S->L0;
+  lo = pa - r0;
+  n = pb - pa + 1 + 2 * r0;
+  if (S->pbc0) {
+    if (n >= L0) {
+      lo = 0;
+      n = L0;
+    } else if (lo < 0) {
+      lo += L0;
+    }
+  } else {
+    const int hi = min(L0 - 1, pb + r0);
+    lo = max(lo, 0);
+    n = hi - lo + 1;
   }
 }
 
 // One stochastic Heun (or Depondt) step in one launch (magex/fast.py:336-390 as one kernel): each
 // workgroup runs the predictor of its tile, publishes the predictor spins, waits until the predictor
 // spins of every axis-0 plane its stencil reaches are published, and runs the corrector of the same tile.
-// Everything the corrector needs of its own sites stays in registers: the spin, w = e + dt/2 k1 (Heun) or
-// the rotation vector (Depondt), the thermal field, the per-site on-site tensor and, for per-bond
-// couplings (NB batches held), the bond tensors.  Each tensor therefore comes from HBM once per step
-// instead of once per pass.  The arithmetic is that of k_heun's two passes, so the results are the same
-// bits.
+// Everything the corrector needs of its own sites stays in registers (Held): the coupling batches (for
+// per-bond couplings the bond tensors, so each tensor comes from HBM once per step instead of once per
+// pass), the on-site tensor, the thermal field and w.  The arithmetic is that of k_heun's two passes,
+// so the results are the same bits.
 //
 // Ordering: a workgroup's predictor reads e of planes within r0 of its own and never waits; its
 // corrector writes e only after every predictor that reads those planes has published (the wait covers
@@ -156,8 +338,6 @@ template <typename R, int CM, int GEO, int NB, bool LEAN>
 __global__ __launch_bounds__(BLOCK) void k_step(const Static<R>* __restrict__ S, const Dyn<R> d, R* __restrict__ e,
                                                 R* __restrict__ ep, R* __restrict__ k1, uint64_t step,
                                                 const R* __restrict__ noise, const StepSync sy) {
-  using BT = Batch<R, CM, GEO>;
-  constexpr int CH = BT::CH;
   __shared__ int sh_flag;
   if (d.step_ptr) step += *d.step_ptr;  // graph replays: step = device counter + offset of this node
   Site st;
@@ -166,63 +346,24 @@ __global__ __launch_bounds__(BLOCK) void k_step(const Static<R>* __restrict__ S,
   const int i = st.i;
   const bool pin = live && !LEAN && d.pinned && d.pinned[i];
   const int kb = S->sptr[st.s], ke = S->sptr[st.s + 1];
-  BT bt[NB > 0 ? NB : 1];
-  SiteA<R> A;
-  R a = R(0), x = R(0), y = R(0), z = R(0), px = R(0), py = R(0), pz = R(0);
-  R wx = R(0), wy = R(0), wz = R(0), nx = R(0), ny = R(0), nz = R(0);
+  Held<R, CM, GEO, NB> h;
   // ---- predictor (pass 1)
   if (live) {
-    a = (!LEAN && d.alpha) ? d.alpha[i] : S->alpha_sub[st.s];
-    x = e[3 * i];
-    y = e[3 * i + 1];
-    z = e[3 * i + 2];
+    h.a = (!LEAN && d.alpha) ? d.alpha[i] : S->alpha_sub[st.s];
+    h.x = e[3 * i];
+    h.y = e[3 * i + 1];
+    h.z = e[3 * i + 2];
     if (pin) {
-      px = wx = x;
-      py = wy = y;
-      pz = wz = z;
+      h.px = h.wx = h.x;
+      h.py = h.wy = h.y;
+      h.pz = h.wz = h.z;
     } else {
-      R hx = R(0), hy = R(0), hz = R(0), gx, gy, gz, bx, by, bz, fx, fy, fz;
-      if constexpr (NB > 0) {
-        // every held batch in flight before the first is used
-#pragma unroll
-        for (int n = 0; n < NB; ++n) bt[n].gather(S, d, e, st, kb + n * CH, ke);
-        if (d.noisy) thermal<R, LEAN>(S, d, st, step, noise, nx, ny, nz);
-#pragma unroll
-        for (int n = 0; n < NB; ++n)
-          if (kb + n * CH < ke) bt[n].accumulate(S, hx, hy, hz);
-      } else {
-        exchange<R, CM, GEO>(S, d, e, st, hx, hy, hz, [&] {
-          if (d.noisy) thermal<R, LEAN>(S, d, st, step, noise, nx, ny, nz);
-        });
-      }
-      A.load(S, d, st);
-      onsite_grad<R, LEAN>(S, d, st, A, x, y, z, gx, gy, gz);
-      field_from<R, LEAN>(S, d, st, x, y, z, hx, hy, hz, gx, gy, gz, nx, ny, nz, d.B0, d.B1, d.B2, bx, by, bz);
-      llg_rhs(x, y, z, bx, by, bz, a, d.prec, fx, fy, fz);
-      if (!LEAN && d.integ == MX_DEPONDT) {
-        omega(x, y, z, fx, fy, fz, wx, wy, wz);
-        px = x;
-        py = y;
-        pz = z;
-        rotate(px, py, pz, d.dt * wx, d.dt * wy, d.dt * wz);
-      } else {
-        const R hdt = R(0.5) * d.dt;
-        wx = x + hdt * fx;
-        wy = y + hdt * fy;
-        wz = z + hdt * fz;
-        const R qx = x + d.dt * fx, qy = y + d.dt * fy, qz = z + d.dt * fz;
-        const R q = rnorm3(qx, qy, qz);
-        px = qx * q;
-        py = qy * q;
-        pz = qz * q;
-      }
+      predictor_held<R, CM, GEO, NB, LEAN, true>(S, d, SpinsFrom<R, 0>{e}, st, kb, ke, step, noise, h);
     }
-    __hip_atomic_store(ep + 3 * i, px, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(ep + 3 * i + 1, py, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(ep + 3 * i + 2, pz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    store3_coherent(ep + 3 * i, h.px, h.py, h.pz);
   }
   // ---- publish the predictor spins of this tile's planes: stores acknowledged, then the counters
-  const int L0 = S->L0, PS = sy.PS;
+  const int PS = sy.PS;
   const int c_first = tile * BLOCK, c_last = min(S->ncell, c_first + BLOCK) - 1;
   const int pa = c_first / PS, pb = c_last / PS;
   __builtin_amdgcn_s_waitcnt(0);
@@ -231,20 +372,9 @@ __global__ __launch_bounds__(BLOCK) void k_step(const Static<R>* __restrict__ S,
     __hip_atomic_fetch_add(sy.cnt + (pa + threadIdx.x) * CNT_STRIDE, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   // ---- wait for the planes within r0
   if (threadIdx.x < 64) {
-    int lo = pa - sy.r0, n = pb - pa + 1 + 2 * sy.r0;
-    if (S->pbc0) {
-      if (n >= L0) {
-        lo = 0;
-        n = L0;
-      } else if (lo < 0) {
-        lo += L0;
-      }
-    } else {
-      const int hi = min(L0 - 1, pb + sy.r0);
-      lo = max(lo, 0);
-      n = hi - lo + 1;
-    }
-    const bool ok = wait_planes<R>(S, sy, lo, n);
+    int lo, n;
+    wait_window<R>(S, pa, pb, sy.r0, lo, n);
+    const bool ok = wait_planes<R>(S, sy, sy.cnt, 1u, lo, n);
     if (threadIdx.x == 0) sh_flag = ok ? 1 : 0;
   }
   __syncthreads();
@@ -252,49 +382,17 @@ __global__ __launch_bounds__(BLOCK) void k_step(const Static<R>* __restrict__ S,
   if (ok) {
     // ---- corrector (pass 2) from registers
     if (live && !pin) {
-      R hx = R(0), hy = R(0), hz = R(0), gx, gy, gz, bx, by, bz, fx, fy, fz;
-      if constexpr (NB > 0) {
-#pragma unroll
-        for (int n = 0; n < NB; ++n) {
-          Ent raw[CH];
-          BT::records(S, kb + n * CH, raw);
-          bt[n].template gather<0, false>(S, d, ep, st, kb + n * CH, ke, raw);
-        }
-#pragma unroll
-        for (int n = 0; n < NB; ++n)
-          if (kb + n * CH < ke) bt[n].accumulate(S, hx, hy, hz);
-      } else {
-        exchange<R, CM, GEO>(S, d, ep, st, hx, hy, hz, [] {});
-      }
-      onsite_grad<R, LEAN>(S, d, st, A, px, py, pz, gx, gy, gz);
-      field_from<R, LEAN>(S, d, st, px, py, pz, hx, hy, hz, gx, gy, gz, nx, ny, nz, d.Bc0, d.Bc1, d.Bc2, bx, by,
-                          bz);
-      llg_rhs(px, py, pz, bx, by, bz, a, d.prec, fx, fy, fz);
-      const R hdt = R(0.5) * d.dt;
-      R cx, cy, cz;
-      if (!LEAN && d.integ == MX_DEPONDT) {
-        R vx, vy, vz;
-        omega(px, py, pz, fx, fy, fz, vx, vy, vz);
-        cx = x;
-        cy = y;
-        cz = z;
-        rotate(cx, cy, cz, hdt * (wx + vx), hdt * (wy + vy), hdt * (wz + vz));
-      } else {
-        cx = wx + hdt * fx;
-        cy = wy + hdt * fy;
-        cz = wz + hdt * fz;
-      }
-      const R q = rnorm3(cx, cy, cz);
-      e[3 * i] = cx * q;
-      e[3 * i + 1] = cy * q;
-      e[3 * i + 2] = cz * q;
+      corrector_held<R, CM, GEO, NB, LEAN>(S, d, SpinsFrom<R, 0>{ep}, st, kb, ke, h);
+      e[3 * i] = h.x;
+      e[3 * i + 1] = h.y;
+      e[3 * i + 2] = h.z;
     }
   } else {
     // gave up: keep w (or the rotation vector) for the last workgroup's corrector
     if (live && !pin) {
-      k1[3 * i] = wx;
-      k1[3 * i + 1] = wy;
-      k1[3 * i + 2] = wz;
+      k1[3 * i] = h.wx;
+      k1[3 * i + 1] = h.wy;
+      k1[3 * i + 2] = h.wz;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
@@ -326,7 +424,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(const Static<R>* __restrict__ S,
       }
     }
   }
-  for (int P = threadIdx.x; P < L0; P += BLOCK)
+  for (int P = threadIdx.x; P < S->L0; P += BLOCK)
     __hip_atomic_store(sy.cnt + P * CNT_STRIDE, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -334,6 +432,105 @@ __global__ __launch_bounds__(BLOCK) void k_step(const Static<R>* __restrict__ S,
     __hip_atomic_store(sy.ctl + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(sy.ctl, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+}
+
+// nsteps stochastic Heun (or Depondt) steps in one launch, for grids that are resident all at once
+// (cooperative launch: every workgroup runs from the start).  Each workgroup owns its tile for the whole
+// launch and keeps its sites' state (spin, damping, on-site tensor) in registers across every step; the
+// couplings are fetched with each gather (at these sizes they stay in cache, and their loads overlap the
+// polls).  Holding per-bond tensors across steps as well took every register of the kernel and gave
+// wrong results in the non-lean FP32 instantiation, while the lean one and every other form matched.
+// Sites exchange spins through tagged records instead of plane
+// counters: every site publishes its spin (tag k = tag0 + 2n), its predictor spin (k + 1) and its
+// corrected spin (k + 2) with agent-scope 64-bit stores, each word carrying the tag beside its payload,
+// and each gather polls its neighbours' records until every word carries the tag it needs -- one
+// store-to-load latency per phase, with no acknowledgement wait, barrier or counter.
+//
+// Why a record is never overwritten before every reader has it: site b publishes its predictor spin of
+// step n + 1 only after reading the corrected spins of step n of all its neighbours, and a neighbour c
+// publishes that only after reading b's predictor spin of step n; likewise for corrected spins (the
+// stencil is symmetric: c reads b whenever b reads c).  So no tag is ever passed over.  Pinned sites take
+// part in every gather (their own spin stays as it is) so that this holds for them too.  Lanes of one
+// wavefront publish together after each phase, so a lane never waits for a lane of its own wavefront.
+// tag0 differs from every tag of the previous launch on these records (a fresh epoch per launch), so no
+// record left by an earlier launch matches.
+//
+// Every poll is bounded: a lane that gives up raises the launch's abort word, which ends every other
+// lane's polls; the launch's last workgroup counts the aborted launch and clears the control words.
+// Same arithmetic as k_heun: same result bits.
+template <typename R, int CM, int GEO, bool LEAN>
+__global__ __launch_bounds__(BLOCK) void k_steps(const Static<R>* __restrict__ S, const Dyn<R> d, R* __restrict__ e,
+                                                 uint64_t* __restrict__ rec, uint64_t step0, int nsteps,
+                                                 const R* __restrict__ noise, uint32_t tag0, const StepSync sy) {
+  __shared__ int sh_flag;
+  constexpr int NW = TAG_WORDS<R>;
+  Site st;
+  int tile;
+  const bool live = step_site<R>(S, blockIdx.x, st, tile);
+  const int i = st.i;
+  const bool pin = live && !LEAN && d.pinned && d.pinned[i];
+  const int kb = S->sptr[st.s], ke = S->sptr[st.s + 1];
+  uint64_t* recE = rec;
+  uint64_t* recP = rec + size_t(S->N) * NW;
+  const uint32_t* abort = sy.ctl + 3;
+  const size_t nstride = size_t(S->N) * 3;
+  bool ok = true;
+  if (live) {
+    Held<R, CM, GEO, 0> h;
+    h.a = (!LEAN && d.alpha) ? d.alpha[i] : S->alpha_sub[st.s];
+    h.x = e[3 * i];
+    h.y = e[3 * i + 1];
+    h.z = e[3 * i + 2];
+    publish_tagged<R>(recE + size_t(i) * NW, h.x, h.y, h.z, tag0);
+    for (int n = 0; n < nsteps; ++n) {
+      const uint64_t step = step0 + uint64_t(n);
+      const R* nz = noise ? noise + size_t(n) * nstride : nullptr;
+      const uint32_t k = tag0 + 2u * uint32_t(n);
+      const R x0 = h.x, y0 = h.y, z0 = h.z;
+      const TaggedFrom fromE{recE, k, sy.spin, abort};
+      bool g = n == 0 ? predictor_held<R, CM, GEO, 0, LEAN, true>(S, d, fromE, st, kb, ke, step, nz, h)
+                      : predictor_held<R, CM, GEO, 0, LEAN, false>(S, d, fromE, st, kb, ke, step, nz, h);
+      if (pin) {
+        h.px = x0;
+        h.py = y0;
+        h.pz = z0;
+      }
+      publish_tagged<R>(recP + size_t(i) * NW, h.px, h.py, h.pz, k + 1);
+      const TaggedFrom fromP{recP, k + 1, sy.spin, abort};
+      const bool g2 = corrector_held<R, CM, GEO, 0, LEAN>(S, d, fromP, st, kb, ke, h);
+      g = g && g2;
+      if (pin) {
+        h.x = x0;
+        h.y = y0;
+        h.z = z0;
+      }
+      publish_tagged<R>(recE + size_t(i) * NW, h.x, h.y, h.z, k + 2);
+      if (!g) {
+        ok = false;
+        break;
+      }
+    }
+    if (!pin) {
+      e[3 * i] = h.x;
+      e[3 * i + 1] = h.y;
+      e[3 * i + 2] = h.z;
+    }
+  }
+  if (!ok) __hip_atomic_store(const_cast<uint32_t*>(abort), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // ---- the last workgroup to finish counts an aborted launch and clears the control words
+  const int bad = __syncthreads_or(ok ? 0 : 1);
+  if (threadIdx.x == 0) {
+    const uint32_t done = bad ? __hip_atomic_fetch_add(sy.ctl, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT)
+                              : __hip_atomic_fetch_add(sy.ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sh_flag = done == gridDim.x - 1 ? 1 : 0;
+  }
+  __syncthreads();
+  if (!sh_flag || threadIdx.x != 0) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  if (__hip_atomic_load(sy.ctl + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    __hip_atomic_fetch_add(sy.ctl + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(sy.ctl + 3, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(sy.ctl, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ------------------------------------------------------------------------------------------ field
@@ -642,7 +839,7 @@ static int reach0(const Model& m) {
   return std::min(r, m.L[0]);
 }
 
-size_t stepsync_words(const Model& m) { return size_t(m.L[0]) * CNT_STRIDE + 4 + size_t(nblk_of(m)); }
+size_t stepsync_words(const Model& m) { return stepsync_ctl(m) + 4 + size_t(nblk_of(m)); }
 
 // Per-bond couplings keep all their tensors in registers across the two passes: FP32 up to two batches
 // of 7 (the 14-entry bcc stencil); FP64 tensors (twice the registers) are not held, and FP64 per-bond
@@ -678,6 +875,9 @@ bool fused_step(const Model& m, int* nb_out) {
   // blocks run in lock-step generations whose waits leave memory idle (bcc 64^3 FP32, 2048 blocks at two
   // per CU: 57 -> 90 us), so they keep two launches per step.
   if (nblk_of(m) > dev_cu) return false;
+  // FP64 beyond 16 k sites: the one-launch step lost on triangular 256^2 FP64 (15.2 -> 18.0 us against
+  // two launches) while it won on sc 16^3 FP64 (12.5 -> 9.5) and triangular 256^2 FP32 (15.9 -> 15.1)
+  if (m.prec == MX_FP64 && m.N > 16384) return false;
   if (nb_out) *nb_out = nb;
   return true;
 }
@@ -706,6 +906,102 @@ static void step_dispatch(const Model& m, int nb, bool lean, dim3 g, hipStream_t
   }
 }
 
+// Bytes one round of a persistent launch's polls reads: every site reads the tagged records of its
+// whole stencil.
+static size_t poll_bytes(const Model& m) {
+  int mx = 0;
+  for (int s = 0; s < m.ns; ++s) mx = std::max(mx, m.sptr[s + 1] - m.sptr[s]);
+  return size_t(m.N) * size_t(mx) * (m.prec == MX_FP64 ? 4 : 2) * sizeof(uint64_t);
+}
+
+// Persistent launches need every workgroup resident at once (at most one per compute unit; each kernel
+// holds at least one) and row-major cells.  They pay where a round of polls is small: measured on
+// MI355X, sc 16^3 FP64 (0.8 MB per round) 9.5 -> 6.3 us per step against one launch per step,
+// triangular 256^2 FP32 (19 MB) 15.1 -> 10.3, but triangular 256^2 FP64 (38 MB) 15.2 -> 18.8 against two
+// launches per step: repeated polls of that much state through the agent-coherent path cost more than
+// the launches they save.
+constexpr size_t PERSISTENT_POLL_BYTES = size_t(24) << 20;
+bool persistent_steps(const Model& m) {
+  if (m.step_mode != 2 || !(m.geo == GEO_PERIODIC || m.geo == GEO_OPEN)) return false;
+  if (poll_bytes(m) > PERSISTENT_POLL_BYTES) return false;
+  int dev_cu = 0;
+  if (hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, m.device) != hipSuccess || dev_cu <= 0)
+    return false;
+  return nblk_of(m) <= dev_cu;
+}
+
+template <typename R, int CM, int GEO, bool LEAN>
+static hipError_t steps_coop(dim3 g, hipStream_t s, const Static<R>* S, Dyn<R> d, R* E, uint64_t* rec,
+                             uint64_t step0, int n, const R* nz, uint32_t tag0, StepSync sy) {
+  void* args[] = {&S, &d, &E, &rec, &step0, &n, &nz, &tag0, &sy};
+  return hipLaunchCooperativeKernel(k_steps<R, CM, GEO, LEAN>, g, dim3(BLOCK), args, 0, s);
+}
+
+template <typename R, int GEO, bool LEAN>
+static hipError_t steps_dispatch(const Model& m, dim3 g, hipStream_t s, const Static<R>* S, const Dyn<R>& d, R* E,
+                                 uint64_t* rec, uint64_t step0, int n, const R* nz, uint32_t tag0,
+                                 const StepSync& sy) {
+  switch (m.coup) {
+    case COUP_TABLE1: return steps_coop<R, COUP_TABLE1, GEO, LEAN>(g, s, S, d, E, rec, step0, n, nz, tag0, sy);
+    case COUP_TABLE4: return steps_coop<R, COUP_TABLE4, GEO, LEAN>(g, s, S, d, E, rec, step0, n, nz, tag0, sy);
+    case COUP_TABLE9: return steps_coop<R, COUP_TABLE9, GEO, LEAN>(g, s, S, d, E, rec, step0, n, nz, tag0, sy);
+    default: return steps_coop<R, COUP_BOND, GEO, LEAN>(g, s, S, d, E, rec, step0, n, nz, tag0, sy);
+  }
+}
+
+template <typename R>
+static int steps_t(const Model& m, void* e, uint64_t seed, uint64_t step0, int64_t nsteps, const void* noise,
+                   hipStream_t s, const StepBufs* bufs) {
+  uint32_t* sync = bufs ? bufs->sync : m.stepsync_d;
+  uint64_t* rec = bufs ? bufs->rec : m.rec_d;
+  if (!sync || !rec || !persistent_steps(m)) {
+    set_error("persistent steps: the model does not take persistent launches");
+    return MX_EINVAL;
+  }
+  const Static<R>* S = static_cast<const Static<R>*>(m.static_d);
+  const Dyn<R> d = make_dyn<R>(m, seed);
+  R* E = static_cast<R*>(e);
+  StepSync sy;
+  sy.cnt = sync;
+  sy.ctl = sync + stepsync_ctl(m);
+  sy.gu = reinterpret_cast<int32_t*>(sy.ctl + 4);
+  sy.r0 = reach0(m);
+  sy.PS = m.L[1] * m.L[2];
+  sy.spin = 1u << 16;
+  static_assert(2 * STEPS_PER_LAUNCH + 2 < (1 << 12), "step tags overflow their 12 bits");
+  const bool lean = lean_model(m) && !noise;
+  const dim3 g(nblk_of(m));
+  const size_t stride = size_t(m.N) * 3;
+  for (int64_t n = 0; n < nsteps; n += STEPS_PER_LAUNCH) {
+    const int k = int(std::min<int64_t>(STEPS_PER_LAUNCH, nsteps - n));
+    const R* nz = noise ? static_cast<const R*>(noise) + size_t(n) * stride : nullptr;
+    const uint64_t st = step0 + uint64_t(n);
+    // 16-bit tags: the launch's epoch (1 .. 15; 0 would match cleared records) in the high 4 bits, the
+    // phase within the launch (up to 2 x STEPS_PER_LAUNCH + 2) in the low 12.  A model's records are
+    // written only by its own launches, and consecutive launches take different epochs.
+    m.rec_epoch = m.rec_epoch % 15u + 1u;
+    const uint32_t tag0 = m.rec_epoch << 12;
+    hipError_t err;
+    if (m.geo == GEO_PERIODIC)
+      err = lean ? steps_dispatch<R, GEO_PERIODIC, true>(m, g, s, S, d, E, rec, st, k, nz, tag0, sy)
+                 : steps_dispatch<R, GEO_PERIODIC, false>(m, g, s, S, d, E, rec, st, k, nz, tag0, sy);
+    else
+      err = lean ? steps_dispatch<R, GEO_OPEN, true>(m, g, s, S, d, E, rec, st, k, nz, tag0, sy)
+                 : steps_dispatch<R, GEO_OPEN, false>(m, g, s, S, d, E, rec, st, k, nz, tag0, sy);
+    if (err != hipSuccess) {
+      set_error(std::string("persistent heun steps: ") + hipGetErrorString(err));
+      return MX_EHIP;
+    }
+  }
+  return check_launch("persistent heun steps");
+}
+
+int launch_heun_steps(const Model& m, void* e, uint64_t seed, uint64_t step0, int64_t nsteps, const void* noise,
+                      hipStream_t s, const StepBufs* bufs) {
+  return m.prec == MX_FP64 ? steps_t<double>(m, e, seed, step0, nsteps, noise, s, bufs)
+                           : steps_t<float>(m, e, seed, step0, nsteps, noise, s, bufs);
+}
+
 template <typename R>
 static int heun_t(const Model& m, void* e, uint64_t seed, uint64_t step, const void* noise, hipStream_t s,
                   const uint64_t* step_ptr, const StepBufs* bufs) {
@@ -722,7 +1018,7 @@ static int heun_t(const Model& m, void* e, uint64_t seed, uint64_t step, const v
   if (sync && fused_step(m, &nb)) {
     StepSync sy;
     sy.cnt = sync;
-    sy.ctl = sync + size_t(m.L[0]) * CNT_STRIDE;
+    sy.ctl = sync + stepsync_ctl(m);
     sy.gu = reinterpret_cast<int32_t*>(sy.ctl + 4);
     sy.r0 = reach0(m);
     sy.PS = m.L[1] * m.L[2];

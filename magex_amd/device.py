@@ -216,14 +216,16 @@ class DeviceModel:
         check(self.lib.mx_model_set_noise(self.h, codes[grade]))
 
     def set_step_mode(self, mode):
-        """0: two kernel launches per Heun step; 1 (default): one launch where the model allows it."""
+        """0: two kernel launches per Heun step; 1: one launch per step where the model allows it; 2
+        (default): as 1, and runs of two or more steps in persistent launches where they pay."""
         check(self.lib.mx_model_set_step_mode(self.h, int(mode)))
 
     def step_info(self):
-        """(one launch per step?, waits of one-launch steps given up since creation; synchronises)."""
+        """(launch form: 0 two per step, 1 one per step, 2 persistent runs; waits given up or persistent
+        launches stopped early since creation -- synchronises)."""
         one, gu = ctypes.c_int32(), ctypes.c_int64()
         check(self.lib.mx_model_step_info(self.h, ctypes.byref(one), ctypes.byref(gu)))
-        return bool(one.value), int(gu.value)
+        return int(one.value), int(gu.value)
 
     # ------------------------------------------------------------------ kernels
     def _check_spins(self, e):

@@ -1,8 +1,9 @@
-"""The one-launch Heun step (k_step) against the two-launch step (k_heun pass 1 + pass 2), bit for bit.
+"""The one-launch Heun step (k_step) and the persistent multi-step launch (k_steps) against the
+two-launch step (k_heun pass 1 + pass 2), bit for bit.
 
-Both compute the same arithmetic in the same order, so every case must agree exactly, over many steps,
-for every coupling form, geometry and model feature the one-launch kernel accepts; and no wait of the
-one-launch step may time out in normal operation.
+All three compute the same arithmetic in the same order, so every case must agree exactly, over many
+steps, for every coupling form, geometry and model feature the one-launch kernels accept; and no wait
+of a one-launch kernel may time out in normal operation.
 """
 import json
 import os
@@ -35,7 +36,7 @@ def _bcc_perbond():
 
 
 def _modes_agree(torch, s, precision, steps, dt, alpha=0.1, T=0.0, integrator="heun", seed=5, noise=None,
-                 graph=None, sot=None):
+                 graph=None, sot=None, forms=(0, 1, 2)):
     dm = s.device_model(precision)
     if graph is not None:
         dm.set_graph_steps(graph)
@@ -45,16 +46,17 @@ def _modes_agree(torch, s, precision, steps, dt, alpha=0.1, T=0.0, integrator="h
     e0 = torch.as_tensor(s.init_spins("random", seed=2), device="cuda", dtype=dm.dtype).contiguous()
     nz = None if noise is None else torch.as_tensor(noise, device="cuda", dtype=dm.dtype).contiguous()
     out = {}
-    for mode in (0, 1):
+    for mode in (0, 1, 2):
         dm.set_step_mode(mode)
         e = e0.clone()
         dm.run(e, steps, seed, 0, nz)
         out[mode] = e
-    one, given_up = dm.step_info()
-    assert one, "the model should take the one-launch step"
-    assert given_up == 0
+        form, given_up = dm.step_info()
+        assert form == forms[mode], f"step mode {mode}: the model runs in form {form}"
+        assert given_up == 0
     assert torch.isfinite(out[1]).all()
-    assert torch.equal(out[0], out[1]), float((out[0] - out[1]).abs().max())
+    for mode in (1, 2):
+        assert torch.equal(out[0], out[mode]), (mode, float((out[0] - out[mode]).abs().max()))
     return out[1]
 
 
@@ -74,6 +76,19 @@ def test_bcc_per_bond_two_held_batches(gpu):
     _modes_agree(gpu, s, "fp32", 30, 1e-4, 0.1, 300.0)
     noise = np.random.default_rng(4).normal(size=(10, s.N, 3))
     _modes_agree(gpu, s, "fp32", 10, 1e-4, 0.1, 300.0, noise=noise)
+    alpha = 0.1 + 0.05 * np.random.default_rng(6).random(s.N)      # the general (non-lean) kernels
+    _modes_agree(gpu, s, "fp32", 10, 1e-4, alpha, 300.0)
+
+
+def test_bcc_per_bond_fp64_persistent(gpu):
+    """FP64 per-bond tensors: two launches per step in mode 1 (tensors not held), persistent launches in
+    mode 2; per-site damping takes the general kernel."""
+    print("case: bcc per-bond fp64", flush=True)
+    s, g = _bcc_perbond()
+    _modes_agree(gpu, s, "fp64", 30, 1e-4, 0.1, 300.0, forms=(0, 0, 2))
+    alpha = 0.1 + 0.05 * np.random.default_rng(6).random(s.N)
+    noise = np.random.default_rng(7).normal(size=(12, s.N, 3))
+    _modes_agree(gpu, s, "fp64", 12, 1e-4, alpha, 300.0, noise=noise, forms=(0, 0, 2))
 
 
 @pytest.mark.parametrize("integrator", ["heun", "depondt"])
@@ -114,3 +129,43 @@ def test_cubic_and_higher_uniaxial(gpu):
     s.terms += [CubicAnisotropy(rng.uniform(0.05, 0.3, s.N), C),
                 UniaxialHigher(rng.uniform(0.05, 0.2, s.N), rng.uniform(0.0, 0.1, s.N), axis=(0.3, 0, 1))]
     _modes_agree(gpu, s, "fp32", 40, 2e-4, 0.1, 5.0)
+
+
+def test_persistent_runs_span_launches_and_continue(gpu):
+    """A run longer than one persistent launch (1024 steps) and a run split in two (step numbers carry
+    on) equal the two-launch steps."""
+    print("case: sc 8^3 fp64 persistent 1500 steps, split 700 + 800", flush=True)
+    s = _system((8, 8, 8), (0, 0, 0.5), lattice="sc", J=[1.0, 0.1], K=0.05)
+    dm = s.device_model("fp64")
+    dm.set_dynamics(2e-4, 0.1, 20.0, True, "heun")
+    e0 = gpu.as_tensor(s.init_spins("random", seed=3), device="cuda").contiguous()
+    dm.set_step_mode(0)
+    ref = e0.clone()
+    dm.run(ref, 1500, 11, 4)
+    dm.set_step_mode(2)
+    one = e0.clone()
+    dm.run(one, 1500, 11, 4)
+    two = e0.clone()
+    dm.run(two, 700, 11, 4)
+    dm.run(two, 800, 11, 704)
+    assert dm.step_info() == (2, 0)
+    assert gpu.equal(one, ref), float((one - ref).abs().max())
+    assert gpu.equal(two, ref), float((two - ref).abs().max())
+
+
+def test_launch_forms_follow_the_measured_policy(gpu):
+    """Default mode: persistent launches where a round of polls is small (sc 16^3 FP64, triangular 256^2
+    FP32), two launches per step for triangular 256^2 FP64 (38 MB of records per poll round) and for
+    grids larger than the GPU holds at once (bcc 64^3)."""
+    from magex_amd import configs
+    print("case: launch forms of the headline configurations", flush=True)
+    for name, prec, form in (("sc16", "fp64", 2), ("tri256", "fp32", 2), ("tri256", "fp64", 0),
+                             ("bcc64", "fp32", 0)):
+        s, p = getattr(configs, name)()
+        dm = s.device_model(prec)
+        dm.set_dynamics(p["dt"], p["alpha"], p["temperature"], True, "heun")
+        assert dm.step_info()[0] == form, (name, prec)
+    s, p = configs.tri256()
+    dm = s.device_model("fp64")
+    dm.set_step_mode(1)
+    assert dm.step_info()[0] == 0     # FP64 beyond 16 k sites keeps two launches per step

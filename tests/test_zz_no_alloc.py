@@ -30,9 +30,10 @@ def _noalloc(torch, dm, e, nsteps, seed, step0=0, noise=None):
                                     None if noise is None else ctypes.c_void_p(noise.data_ptr()), _stream(torch)))
 
 
-def _compare(torch, s, precision, nsteps, dt, alpha, T, seed, noise=None):
+def _compare(torch, s, precision, nsteps, dt, alpha, T, seed, noise=None, mode=1):
     dm = s.device_model(precision)
     dm.set_dynamics(dt, alpha, T, True, "heun")
+    dm.set_step_mode(mode)
     e0 = torch.as_tensor(s.init_spins("random", seed=4), device="cuda", dtype=dm.dtype).contiguous()
     nz = None if noise is None else torch.as_tensor(noise, device="cuda", dtype=dm.dtype).contiguous()
     ref = e0.clone()
@@ -61,19 +62,23 @@ def test_noalloc_equals_run_on_falling_sizes(gpu):
     print("case: noalloc bcc64 fp32 (two launches per step)", flush=True)
     s, p = configs.bcc64()
     dm = _compare(gpu, s, "fp32", 10, p["dt"], p["alpha"], p["temperature"], 7)
-    assert not dm.step_info()[0]
+    assert dm.step_info()[0] == 0
     print("case: noalloc tri64 fp64 (one launch per step)", flush=True)
     t = S.SpinSystem(S.load_results(M.make_model("triangular", J=[5.0, -0.5, 0.2], D=1.5, K=0.3)), (64, 64, 1),
                      field=(0, 0, 2.0))
     dm = _compare(gpu, t, "fp64", 30, 5e-4, 0.05, 20.0, 9)
-    assert dm.step_info()[0]
+    assert dm.step_info()[0] == 1
     print("case: noalloc bcc 4^3 per-bond fp32 (two sublattices), injected noise", flush=True)
     b = _bcc_perbond()
     noise = np.random.default_rng(1).normal(size=(12, b.N, 3))
     dm = _compare(gpu, b, "fp32", 12, 1e-4, 0.1, 300.0, 5, noise=noise)
-    assert dm.step_info() == (True, 0)
+    assert dm.step_info() == (1, 0)
     # the same small call once more, after the scratch held the others' contents
     _compare(gpu, b, "fp32", 5, 1e-4, 0.1, 300.0, 6)
+    # persistent launches: their two counter sets and abort word live in the same scratch
+    print("case: noalloc bcc 4^3 per-bond fp32, persistent launches", flush=True)
+    dm = _compare(gpu, b, "fp32", 7, 1e-4, 0.1, 300.0, 8, mode=2)
+    assert dm.step_info() == (2, 0)
 
 
 def test_noalloc_refuses_oversized_and_unconfigured_models(gpu):
