@@ -40,6 +40,91 @@ namespace mx {
 // temperatures uniform per sublattice, no cubic or higher-uniaxial terms, no spin-orbit torque, generated
 // noise, no adjoint stores): the host picks it when the model is that, and its shorter code and scalar
 // load chain matter on lattices too small to hide them.  Same arithmetic, so the same result bits.
+// Stores of a spin that other workgroups of the same launch read (agent scope: global_store ... sc1).
+template <typename R>
+__device__ __forceinline__ void store3_coherent(R* p, R x, R y, R z) {
+  __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p + 1, y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p + 2, z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Pass 1 of site st: ep and w (or the rotation vector) into ep and k1.  PUB: ep stored with agent-scope
+// stores (written through to the agent-coherent level), for kernels that publish it to other workgroups
+// of the same launch.
+template <typename R, int CM, int GEO, bool LEAN, bool PUB = false>
+__device__ __forceinline__ void heun_predict(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
+                                             R* __restrict__ ep, R* __restrict__ k1, const Site& st, uint64_t step,
+                                             const R* __restrict__ noise) {
+  const int i = st.i;
+  const bool pin = !LEAN && d.pinned && d.pinned[i];
+  const R a = (!LEAN && d.alpha) ? d.alpha[i] : S->alpha_sub[st.s];
+  auto put_ep = [&](R x, R y, R z) {
+    if constexpr (PUB) {
+      store3_coherent(ep + 3 * i, x, y, z);
+    } else {
+      ep[3 * i] = x;
+      ep[3 * i + 1] = y;
+      ep[3 * i + 2] = z;
+    }
+  };
+  const R x = e[3 * i], y = e[3 * i + 1], z = e[3 * i + 2];
+  if (pin) {
+    put_ep(x, y, z);
+    k1[3 * i] = x;
+    k1[3 * i + 1] = y;
+    k1[3 * i + 2] = z;
+    if (!LEAN && d.save_e) {  // a recorded step (adjoint): the spin, and no field acting on it
+      d.save_e[3 * i] = x;
+      d.save_e[3 * i + 1] = y;
+      d.save_e[3 * i + 2] = z;
+      d.save_B[3 * i] = d.save_B[3 * i + 1] = d.save_B[3 * i + 2] = R(0);
+    }
+    return;
+  }
+  R bx, by, bz, fx, fy, fz;
+  local_field<R, CM, GEO, LEAN>(S, d, e, st, x, y, z, step, noise, d.B0, d.B1, d.B2, bx, by, bz);
+  if (!LEAN && d.save_e) {
+    d.save_e[3 * i] = x;
+    d.save_e[3 * i + 1] = y;
+    d.save_e[3 * i + 2] = z;
+    d.save_B[3 * i] = bx;
+    d.save_B[3 * i + 1] = by;
+    d.save_B[3 * i + 2] = bz;
+  }
+  llg_rhs(x, y, z, bx, by, bz, a, d.prec, fx, fy, fz);
+  if (!LEAN && d.integ == MX_DEPONDT) {
+    R wx, wy, wz, px = x, py = y, pz = z;
+    omega(x, y, z, fx, fy, fz, wx, wy, wz);
+    k1[3 * i] = wx;
+    k1[3 * i + 1] = wy;
+    k1[3 * i + 2] = wz;
+    rotate(px, py, pz, d.dt * wx, d.dt * wy, d.dt * wz);
+    put_ep(px, py, pz);
+    return;
+  }
+  const R hdt = R(0.5) * d.dt;
+  k1[3 * i] = x + hdt * fx;
+  k1[3 * i + 1] = y + hdt * fy;
+  k1[3 * i + 2] = z + hdt * fz;
+  const R px = x + d.dt * fx, py = y + d.dt * fy, pz = z + d.dt * fz;
+  const R q = rnorm3(px, py, pz);
+  put_ep(px * q, py * q, pz * q);
+}
+
+// Pass 2 of site st: the corrector from ep (neighbours and own) and w in k1, into e.
+template <typename R, int CM, int GEO, bool LEAN>
+__device__ __forceinline__ void heun_correct(const Static<R>* __restrict__ S, const Dyn<R>& d, R* __restrict__ e,
+                                             const R* __restrict__ ep, const R* __restrict__ k1, const Site& st,
+                                             uint64_t step, const R* __restrict__ noise) {
+  const int i = st.i;
+  if (!LEAN && d.pinned && d.pinned[i]) {
+    if (!LEAN && d.save_B) d.save_B[3 * i] = d.save_B[3 * i + 1] = d.save_B[3 * i + 2] = R(0);
+    return;
+  }
+  const R a = (!LEAN && d.alpha) ? d.alpha[i] : S->alpha_sub[st.s];
+  corrector<R, CM, GEO, LEAN>(S, d, e, ep, k1, st, a, step, noise, d.B0, d.B1, d.B2, d.save_B);
+}
+
 template <typename R, int CM, int GEO, int PASS, bool LEAN = false>
 __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S, const Dyn<R> d, R* __restrict__ e,
                                                 R* __restrict__ ep, R* __restrict__ k1, uint64_t step,
@@ -47,65 +132,8 @@ __global__ __launch_bounds__(BLOCK) void k_heun(const Static<R>* __restrict__ S,
   Site st;
   if (!site_of<R, GEO>(S, st, PASS == 2 && d.rev2, true)) return;
   if (d.step_ptr) step += *d.step_ptr;  // graph replays: step = device counter + offset of this node
-  const int i = st.i;
-  const bool pin = !LEAN && d.pinned && d.pinned[i];
-  const R a = (!LEAN && d.alpha) ? d.alpha[i] : S->alpha_sub[st.s];
-  if (PASS == 1) {
-    const R x = e[3 * i], y = e[3 * i + 1], z = e[3 * i + 2];
-    if (pin) {
-      ep[3 * i] = x;
-      ep[3 * i + 1] = y;
-      ep[3 * i + 2] = z;
-      k1[3 * i] = x;
-      k1[3 * i + 1] = y;
-      k1[3 * i + 2] = z;
-      if (!LEAN && d.save_e) {  // a recorded step (adjoint): the spin, and no field acting on it
-        d.save_e[3 * i] = x;
-        d.save_e[3 * i + 1] = y;
-        d.save_e[3 * i + 2] = z;
-        d.save_B[3 * i] = d.save_B[3 * i + 1] = d.save_B[3 * i + 2] = R(0);
-      }
-      return;
-    }
-    R bx, by, bz, fx, fy, fz;
-    local_field<R, CM, GEO, LEAN>(S, d, e, st, x, y, z, step, noise, d.B0, d.B1, d.B2, bx, by, bz);
-    if (!LEAN && d.save_e) {
-      d.save_e[3 * i] = x;
-      d.save_e[3 * i + 1] = y;
-      d.save_e[3 * i + 2] = z;
-      d.save_B[3 * i] = bx;
-      d.save_B[3 * i + 1] = by;
-      d.save_B[3 * i + 2] = bz;
-    }
-    llg_rhs(x, y, z, bx, by, bz, a, d.prec, fx, fy, fz);
-    if (!LEAN && d.integ == MX_DEPONDT) {
-      R wx, wy, wz, px = x, py = y, pz = z;
-      omega(x, y, z, fx, fy, fz, wx, wy, wz);
-      k1[3 * i] = wx;
-      k1[3 * i + 1] = wy;
-      k1[3 * i + 2] = wz;
-      rotate(px, py, pz, d.dt * wx, d.dt * wy, d.dt * wz);
-      ep[3 * i] = px;
-      ep[3 * i + 1] = py;
-      ep[3 * i + 2] = pz;
-      return;
-    }
-    const R hdt = R(0.5) * d.dt;
-    k1[3 * i] = x + hdt * fx;
-    k1[3 * i + 1] = y + hdt * fy;
-    k1[3 * i + 2] = z + hdt * fz;
-    const R px = x + d.dt * fx, py = y + d.dt * fy, pz = z + d.dt * fz;
-    const R q = rnorm3(px, py, pz);
-    ep[3 * i] = px * q;
-    ep[3 * i + 1] = py * q;
-    ep[3 * i + 2] = pz * q;
-  } else {
-    if (pin) {
-      if (!LEAN && d.save_B) d.save_B[3 * i] = d.save_B[3 * i + 1] = d.save_B[3 * i + 2] = R(0);
-      return;
-    }
-    corrector<R, CM, GEO, LEAN>(S, d, e, ep, k1, st, a, step, noise, d.B0, d.B1, d.B2, d.save_B);
-  }
+  if (PASS == 1) heun_predict<R, CM, GEO, LEAN>(S, d, e, ep, k1, st, step, noise);
+  else heun_correct<R, CM, GEO, LEAN>(S, d, e, ep, k1, st, step, noise);
 }
 
 // ------------------------------------------------------------------------------------------ one-launch step
@@ -284,13 +312,6 @@ __device__ __forceinline__ bool corrector_held(const Static<R>* __restrict__ S, 
   return ok;
 }
 
-template <typename R>
-__device__ __forceinline__ void store3_coherent(R* p, R x, R y, R z) {
-  __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(p + 1, y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(p + 2, z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Planes a workgroup's stencil reaches: [lo, lo + n) wrapped onto [0, L0), from its tile's planes
 // [pa, pb] and the reach r0 along axis 0.
 template <typename R>
@@ -309,6 +330,57 @@ __device__ __forceinline__ void wait_window(const Static<R>* __restrict__ S, int
     const int hi = min(L0 - 1, pb + r0);
     lo = max(lo, 0);
     n = hi - lo + 1;
+  }
+}
+
+// End of a one-launch step (k_step), called by every thread of every workgroup once its own
+// work is done (gave: this workgroup gave up a wait and listed tiles in sy.gu): the last workgroup to
+// get here runs the correctors of the listed tiles (every predictor is done by then; w is in k1) and
+// clears the control words for the next launch.
+template <typename R, int CM, int GEO, bool LEAN>
+__device__ __forceinline__ void finish_step(const Static<R>* __restrict__ S, const Dyn<R>& d, R* __restrict__ e,
+                                            const R* __restrict__ ep, const R* __restrict__ k1, uint64_t step,
+                                            const R* __restrict__ noise, const StepSync& sy, bool gave,
+                                            int& sh_flag) {
+  if (threadIdx.x == 0) {
+    const uint32_t done = gave ? __hip_atomic_fetch_add(sy.ctl, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT)
+                               : __hip_atomic_fetch_add(sy.ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sh_flag = done == gridDim.x - 1 ? 1 : 0;
+  }
+  __syncthreads();
+  if (!sh_flag) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  const uint32_t ngu = __hip_atomic_load(sy.ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (uint32_t g = 0; g < ngu; ++g) {
+    const int b = __hip_atomic_load(sy.gu + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    Site s2;
+    int t2;
+    if (step_site<R>(S, b, s2, t2)) {
+      const int j = s2.i;
+      if (LEAN || !(d.pinned && d.pinned[j])) {
+        const R a2 = (!LEAN && d.alpha) ? d.alpha[j] : S->alpha_sub[s2.s];
+        corrector<R, CM, GEO, LEAN>(S, d, e, ep, k1, s2, a2, step, noise, d.Bc0, d.Bc1, d.Bc2, nullptr);
+      }
+    }
+  }
+  for (int P = threadIdx.x; P < S->L0; P += BLOCK)
+    __hip_atomic_store(sy.cnt + P * CNT_STRIDE, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (ngu) __hip_atomic_fetch_add(sy.ctl + 2, ngu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(sy.ctl + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(sy.ctl, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Tile list of a workgroup that gave up a wait: w of the tile is in k1 (a release makes it, and the
+// tile's predictor spins, visible to the last workgroup), and the tile joins sy.gu.
+__device__ __forceinline__ void give_up_tile(const StepSync& sy, int tile_block) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t k = __hip_atomic_fetch_add(sy.ctl + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(sy.gu + k, int32_t(tile_block), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -331,7 +403,12 @@ __device__ __forceinline__ void wait_window(const Static<R>* __restrict__ S, int
 // launches are made coherent at the kernel boundary (the two-launch step relies on the same).
 // Measured on bcc 64^3 FP32: an agent-scope release and acquire per wavefront (buffer_wbl2 /
 // buffer_inv of the XCD's L2) made the step 7x slower than two launches, agent-scope (sc1) loads of the
-// gathered spins, which bypass L2, 3.8x.  Every wait is bounded: a workgroup that gives up stores w
+// gathered spins, which bypass L2, 3.8x.  A persistent form for grids larger than the GPU holds (G
+// resident workgroups walking the tiles in dispatch order, each running the predictor of one tile and the
+// corrector of the tile it predicted a round before, so the corrector's re-reads hit L2) was measured
+// too: 81-109 us per step on bcc 64^3 FP32 against 59 for two launches, 169-176 against 142 in FP64 --
+// a workgroup idles through its acknowledgement wait and poll, and a step is only 3-8 rounds, so the
+// pipeline's fill and drain are a large share.  Every wait is bounded: a workgroup that gives up stores w
 // (behind a release fence) and leaves its corrector to the launch's last workgroup, which runs it once
 // every predictor is done, then clears the counters for the next launch.
 template <typename R, int CM, int GEO, int NB, bool LEAN>
@@ -394,44 +471,10 @@ __global__ __launch_bounds__(BLOCK) void k_step(const Static<R>* __restrict__ S,
       k1[3 * i + 1] = h.wy;
       k1[3 * i + 2] = h.wz;
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const uint32_t k = __hip_atomic_fetch_add(sy.ctl + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(sy.gu + k, int32_t(blockIdx.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
-  // ---- the last workgroup to finish runs the correctors that were given up and clears the counters
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint32_t done = ok ? __hip_atomic_fetch_add(sy.ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                             : __hip_atomic_fetch_add(sy.ctl, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    sh_flag = done == gridDim.x - 1 ? 1 : 0;
+    give_up_tile(sy, blockIdx.x);
   }
   __syncthreads();
-  if (!sh_flag) return;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  const uint32_t ngu = __hip_atomic_load(sy.ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  for (uint32_t g = 0; g < ngu; ++g) {
-    const int b = __hip_atomic_load(sy.gu + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    Site s2;
-    int t2;
-    if (step_site<R>(S, b, s2, t2)) {
-      const int j = s2.i;
-      if (LEAN || !(d.pinned && d.pinned[j])) {
-        const R a2 = (!LEAN && d.alpha) ? d.alpha[j] : S->alpha_sub[s2.s];
-        corrector<R, CM, GEO, LEAN>(S, d, e, ep, k1, s2, a2, step, noise, d.Bc0, d.Bc1, d.Bc2, nullptr);
-      }
-    }
-  }
-  for (int P = threadIdx.x; P < S->L0; P += BLOCK)
-    __hip_atomic_store(sy.cnt + P * CNT_STRIDE, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (ngu) __hip_atomic_fetch_add(sy.ctl + 2, ngu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(sy.ctl + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(sy.ctl, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  finish_step<R, CM, GEO, LEAN>(S, d, e, ep, k1, step, noise, sy, !ok, sh_flag);
 }
 
 // nsteps stochastic Heun (or Depondt) steps in one launch, for grids that are resident all at once
