@@ -91,10 +91,12 @@ struct Dyn {
 };
 
 // Control words of the one-launch Heun steps (k_step, k_steps) in model-owned device memory: L0 plane
-// counters, 4 control words, one give-up slot per workgroup.  Every word is zero between launches: the
-// last workgroup of a launch clears what the launch used.  The plane counters sit one per 128-byte line:
-// hundreds of wavefronts poll the few counters around their plane while others increment them.
+// counters, CTL_WORDS control words, one give-up slot per workgroup.  Every word is zero between
+// launches: the last workgroup of a launch clears what the launch used.  The plane counters sit one per
+// 128-byte line: hundreds of wavefronts poll the few counters around their plane while others increment
+// them.
 constexpr int CNT_STRIDE = 32;
+constexpr int CTL_WORDS = 4;
 struct StepSync {
   uint32_t* cnt;  // per axis-0 plane (stride CNT_STRIDE): workgroups whose predictor spins covering the
                   // plane are published (k_step)

@@ -403,12 +403,17 @@ __device__ __forceinline__ void give_up_tile(const StepSync& sy, int tile_block)
 // launches are made coherent at the kernel boundary (the two-launch step relies on the same).
 // Measured on bcc 64^3 FP32: an agent-scope release and acquire per wavefront (buffer_wbl2 /
 // buffer_inv of the XCD's L2) made the step 7x slower than two launches, agent-scope (sc1) loads of the
-// gathered spins, which bypass L2, 3.8x.  A persistent form for grids larger than the GPU holds (G
-// resident workgroups walking the tiles in dispatch order, each running the predictor of one tile and the
-// corrector of the tile it predicted a round before, so the corrector's re-reads hit L2) was measured
-// too: 81-109 us per step on bcc 64^3 FP32 against 59 for two launches, 169-176 against 142 in FP64 --
-// a workgroup idles through its acknowledgement wait and poll, and a step is only 3-8 rounds, so the
-// pipeline's fill and drain are a large share.  Every wait is bounded: a workgroup that gives up stores w
+// gathered spins, which bypass L2, 3.8x.  Two single-launch forms for grids larger than the GPU holds
+// were built and measured on bcc 64^3 too, both bit-equal to two launches and both slower: G resident
+// workgroups walking the tiles in rounds, each running one tile's predictor and the corrector of the tile
+// it predicted a round before (81-109 us per step FP32 against 59; 169-176 FP64 against 142), and a plane
+// pipeline per XCD in which workgroups take tickets from their XCD's counter and run the corrector a
+// fixed lag of tickets behind the predictor (1, 2, 3 workgroups per CU: 102, 78, 80 us FP32; 173, 157,
+// 172 FP64).  The pipeline read as many bytes as two launches (FETCH_SIZE 163 101 KiB per step against
+// 162 232) with more L2 misses: a lag long enough that correctors do not wait on the predictors running
+// beside them puts about 5 MB of tensors between a predictor and its corrector, more than an XCD's 4 MB
+// L2, so the re-reads come from HBM anyway and the waits are pure cost.  Every wait is bounded: a
+// workgroup that gives up stores w
 // (behind a release fence) and leaves its corrector to the launch's last workgroup, which runs it once
 // every predictor is done, then clears the counters for the next launch.
 template <typename R, int CM, int GEO, int NB, bool LEAN>
@@ -730,6 +735,14 @@ static int check_launch(const char* what) {
   return MX_OK;
 }
 
+// Column map of the step kernels (row-major tiles per XCD per axis-0 plane) when an axis-0 plane is a
+// whole number of tiles, 8 or more per XCD; 0: slabs.
+static int xcol_of(const Model& m) {
+  const long plane = long(m.L[1]) * m.L[2];
+  return ((m.geo == GEO_PERIODIC || m.geo == GEO_OPEN) && plane % BLOCK == 0 && (plane / BLOCK) % 8 == 0)
+             ? int(plane / BLOCK / 8) : 0;
+}
+
 template <typename R>
 static void fill_static(const Model& m, Static<R>& S) {
   std::memset(&S, 0, sizeof(S));
@@ -753,12 +766,7 @@ static void fill_static(const Model& m, Static<R>& S) {
   S.fdiv = m.ncell < (1 << 24);
   S.rL1 = 1.0f / float(m.L[1]);
   S.rL2 = 1.0f / float(m.L[2]);
-  {
-    // column map of the step kernels when an axis-0 plane is a whole number of tiles, 8 or more per XCD
-    const long plane = long(m.L[1]) * m.L[2];
-    S.xcol = ((m.geo == GEO_PERIODIC || m.geo == GEO_OPEN) && plane % BLOCK == 0 && (plane / BLOCK) % 8 == 0)
-                 ? int(plane / BLOCK / 8) : 0;
-  }
+  S.xcol = xcol_of(m);
   S.half = m.ncell / 2;
   S.L2h = m.L[2] / 2;
   S.rL2h = S.L2h > 0 ? 1.0f / float(S.L2h) : 0.0f;
@@ -882,7 +890,7 @@ static int reach0(const Model& m) {
   return std::min(r, m.L[0]);
 }
 
-size_t stepsync_words(const Model& m) { return stepsync_ctl(m) + 4 + size_t(nblk_of(m)); }
+size_t stepsync_words(const Model& m) { return stepsync_ctl(m) + CTL_WORDS + size_t(nblk_of(m)); }
 
 // Per-bond couplings keep all their tensors in registers across the two passes: FP32 up to two batches
 // of 7 (the 14-entry bcc stencil); FP64 tensors (twice the registers) are not held, and FP64 per-bond
@@ -1007,7 +1015,7 @@ static int steps_t(const Model& m, void* e, uint64_t seed, uint64_t step0, int64
   StepSync sy;
   sy.cnt = sync;
   sy.ctl = sync + stepsync_ctl(m);
-  sy.gu = reinterpret_cast<int32_t*>(sy.ctl + 4);
+  sy.gu = reinterpret_cast<int32_t*>(sy.ctl + CTL_WORDS);
   sy.r0 = reach0(m);
   sy.PS = m.L[1] * m.L[2];
   sy.spin = 1u << 16;
@@ -1062,7 +1070,7 @@ static int heun_t(const Model& m, void* e, uint64_t seed, uint64_t step, const v
     StepSync sy;
     sy.cnt = sync;
     sy.ctl = sync + stepsync_ctl(m);
-    sy.gu = reinterpret_cast<int32_t*>(sy.ctl + 4);
+    sy.gu = reinterpret_cast<int32_t*>(sy.ctl + CTL_WORDS);
     sy.r0 = reach0(m);
     sy.PS = m.L[1] * m.L[2];
     sy.spin = 1u << 16;
