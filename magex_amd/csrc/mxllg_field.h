@@ -322,10 +322,11 @@ struct Batch {
   // live entry is re-read until all its words carry the tag `want`; entries past the stencil's end and
   // neighbours outside an open boundary are not read (their spins count zero).  False when a lane gives
   // up: after `spin` unsuccessful polls, or once *abort is raised.
-  template <bool WITH_J>
+  // mid() runs once, after the first round of loads is issued and before it is used.
+  template <bool WITH_J, typename Mid>
   __device__ __forceinline__ bool gather_tagged(const Static<R>* __restrict__ S, const Dyn<R>& d,
                                                 const uint64_t* __restrict__ rec, const Site& st, int k0, int ke,
-                                                uint32_t want, uint32_t spin, const uint32_t* abort) {
+                                                uint32_t want, uint32_t spin, const uint32_t* abort, Mid&& mid) {
     constexpr int NW = TAG_WORDS<R>;
     static_assert(CH <= 32, "pending entries are a 32-bit mask");
     Ent raw[CH];
@@ -373,6 +374,7 @@ struct Batch {
             w[u][q] = __hip_atomic_load(const_cast<uint64_t*>(r + q), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
       }
+      if (it == 0) mid();
 #pragma unroll
       for (int u = 0; u < CH; ++u) {
         if (k0 + u < ke) {

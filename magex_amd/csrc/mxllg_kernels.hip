@@ -190,7 +190,7 @@ template <typename R, int CM, int GEO, bool WITH_J, typename Src>
 __device__ __forceinline__ bool fill_batch(Batch<R, CM, GEO>& bt, const Static<R>* __restrict__ S, const Dyn<R>& d,
                                            const Src& src, const Site& st, int k0, int ke) {
   if constexpr (Src::tagged) {
-    return bt.template gather_tagged<WITH_J>(S, d, src.rec, st, k0, ke, src.want, src.spin, src.abort);
+    return bt.template gather_tagged<WITH_J>(S, d, src.rec, st, k0, ke, src.want, src.spin, src.abort, [] {});
   } else {
     bt.template gather<Src::LD_, WITH_J>(S, d, src.e, st, k0, ke);
     return true;
@@ -221,10 +221,19 @@ __device__ __forceinline__ bool predictor_held(const Static<R>* __restrict__ S, 
     for (int n = 0; n < NB; ++n)
       if (kb + n * CH < ke) h.bt[n].accumulate(S, hx, hy, hz);
   } else if constexpr (Src::tagged) {
-    if (d.noisy) thermal<R, LEAN>(S, d, st, step, noise, h.nx, h.ny, h.nz);
+    // the thermal field and the on-site gradient (the site's own) while the first poll is in flight
+    if (LOADJ) h.A.load(S, d, st);
+    auto mid = [&] {
+      if (d.noisy) thermal<R, LEAN>(S, d, st, step, noise, h.nx, h.ny, h.nz);
+      onsite_grad<R, LEAN>(S, d, st, h.A, h.x, h.y, h.z, gx, gy, gz);
+    };
+    if (kb >= ke) mid();
     for (int k0 = kb; k0 < ke; k0 += CH) {
       Batch<R, CM, GEO> bt;
-      const bool g = fill_batch<R, CM, GEO, true>(bt, S, d, src, st, k0, ke);
+      const bool g = k0 == kb
+                         ? bt.template gather_tagged<true>(S, d, src.rec, st, k0, ke, src.want, src.spin, src.abort, mid)
+                         : bt.template gather_tagged<true>(S, d, src.rec, st, k0, ke, src.want, src.spin, src.abort,
+                                                           [] {});
       ok = ok && g;
       bt.accumulate(S, hx, hy, hz);
     }
@@ -233,9 +242,11 @@ __device__ __forceinline__ bool predictor_held(const Static<R>* __restrict__ S, 
       if (d.noisy) thermal<R, LEAN>(S, d, st, step, noise, h.nx, h.ny, h.nz);
     });
   }
-  if (LOADJ) h.A.load(S, d, st);
   const R x = h.x, y = h.y, z = h.z;
-  onsite_grad<R, LEAN>(S, d, st, h.A, x, y, z, gx, gy, gz);
+  if constexpr (!(Src::tagged && NB == 0)) {
+    if (LOADJ) h.A.load(S, d, st);
+    onsite_grad<R, LEAN>(S, d, st, h.A, x, y, z, gx, gy, gz);
+  }
   field_from<R, LEAN>(S, d, st, x, y, z, hx, hy, hz, gx, gy, gz, h.nx, h.ny, h.nz, d.B0, d.B1, d.B2, bx, by, bz);
   llg_rhs(x, y, z, bx, by, bz, h.a, d.prec, fx, fy, fz);
   if (!LEAN && d.integ == MX_DEPONDT) {
@@ -277,9 +288,15 @@ __device__ __forceinline__ bool corrector_held(const Static<R>* __restrict__ S, 
     for (int n = 0; n < NB; ++n)
       if (kb + n * CH < ke) h.bt[n].accumulate(S, hx, hy, hz);
   } else if constexpr (Src::tagged) {
+    // the on-site gradient at the site's own predictor spin while the first poll is in flight
+    auto mid = [&] { onsite_grad<R, LEAN>(S, d, st, h.A, h.px, h.py, h.pz, gx, gy, gz); };
+    if (kb >= ke) mid();
     for (int k0 = kb; k0 < ke; k0 += CH) {
       Batch<R, CM, GEO> bt;
-      const bool g = fill_batch<R, CM, GEO, true>(bt, S, d, src, st, k0, ke);
+      const bool g = k0 == kb
+                         ? bt.template gather_tagged<true>(S, d, src.rec, st, k0, ke, src.want, src.spin, src.abort, mid)
+                         : bt.template gather_tagged<true>(S, d, src.rec, st, k0, ke, src.want, src.spin, src.abort,
+                                                           [] {});
       ok = ok && g;
       bt.accumulate(S, hx, hy, hz);
     }
@@ -287,7 +304,7 @@ __device__ __forceinline__ bool corrector_held(const Static<R>* __restrict__ S, 
     exchange<R, CM, GEO, 0, Src::LD_>(S, d, src.e, st, hx, hy, hz, [] {});
   }
   const R px = h.px, py = h.py, pz = h.pz;
-  onsite_grad<R, LEAN>(S, d, st, h.A, px, py, pz, gx, gy, gz);
+  if constexpr (!(Src::tagged && NB == 0)) onsite_grad<R, LEAN>(S, d, st, h.A, px, py, pz, gx, gy, gz);
   field_from<R, LEAN>(S, d, st, px, py, pz, hx, hy, hz, gx, gy, gz, h.nx, h.ny, h.nz, d.Bc0, d.Bc1, d.Bc2, bx, by,
                       bz);
   llg_rhs(px, py, pz, bx, by, bz, h.a, d.prec, fx, fy, fz);
