@@ -212,10 +212,16 @@ int mx_model_set_graph_steps(mx_model* m, int32_t steps);
  * with its neighbours through tagged records.  In one launch per step each workgroup keeps its sites'
  * couplings and state in registers from the predictor to the corrector and waits only for the predictor
  * spins of the axis-0 planes its stencil reaches, so per-bond tensors come from memory once per step
- * instead of twice.  Results are identical in every mode.  No counterpart in magex. */
+ * instead of twice.  Mode 3 = as mode 2, and steps of FP32 per-bond models whose grid is larger than the
+ * GPU holds at once (row-major cells whose axis-0 planes split into 8 k whole tiles) run as one walked
+ * launch: the resident workgroups take tiles from per-XCD ticket counters in plane order and hold each
+ * tile's tensors in registers from its predictor to its corrector (measured slower than two launches on
+ * bcc 64^3, 84 against 57 us per step, hence not the default).  Results are identical in every mode.
+ * No counterpart in magex. */
 int mx_model_set_step_mode(mx_model* m, int32_t mode);
 /* one_launch: how mx_llg_run steps this model (mode and model permitting): 0 two kernels per step, 1 one
- * kernel per step, 2 runs of two or more steps in persistent launches; given_up: waits of one-launch
+ * kernel per step, 2 runs of two or more steps in persistent launches, 3 one walked launch per step;
+ * given_up: waits of one-launch
  * steps that timed out since the model was created, each finished by the launch's last workgroup
  * instead, plus persistent launches that stopped early on a timed-out wait (0 in normal operation;
  * synchronises the device).  Either may be NULL. */

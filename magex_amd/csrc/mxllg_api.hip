@@ -752,8 +752,10 @@ int mx_model_set_noise(mx_model* m, int32_t grade) {
 int mx_model_set_step_mode(mx_model* m, int32_t mode) {
   return guarded([&]() -> int {
     need_model(m);
-    if (mode < 0 || mode > 2)
-      fail(MX_EINVAL, "step mode must be 0 (two launches), 1 (one launch per step) or 2 (persistent runs)");
+    if (mode < 0 || mode > 3)
+      fail(MX_EINVAL,
+           "step mode must be 0 (two launches), 1 (one launch per step), 2 (persistent runs) or 3 (as 2, and the "
+           "walked one-launch step on large FP32 per-bond grids)");
     if (mode != m->step_mode) {
       use_device(*m);
       drop_graph(*m);
@@ -767,7 +769,7 @@ int mx_model_step_info(mx_model* m, int32_t* one_launch, int64_t* given_up) {
   return guarded([&]() -> int {
     need_model(m);
     use_device(*m);
-    if (one_launch) *one_launch = persistent_steps(*m) ? 2 : (fused_step(*m) ? 1 : 0);
+    if (one_launch) *one_launch = persistent_steps(*m) ? 2 : (fused_step(*m) ? 1 : (walk_step(*m) ? 3 : 0));
     if (given_up) {
       uint32_t v = 0;
       if (m->stepsync_d) {

@@ -662,16 +662,20 @@ __device__ __forceinline__ void field_from(const Static<R>* __restrict__ S, cons
 }
 
 // The local field at spin (x, y, z) of site st, gathering the neighbours' spins from e.  The generator's
-// normals are computed while the first neighbour batch is in flight.
+// normals are computed while the first neighbour batch is in flight.  The per-site on-site tensor is
+// loaded before the first gather, so that its latency hides behind the gathers' instead of adding a
+// round trip after them (the loader's branches on the on-site mode kept the compiler from hoisting it).
 template <typename R, int CM, int GEO, bool LEAN = false>
 __device__ __forceinline__ void local_field(const Static<R>* __restrict__ S, const Dyn<R>& d, const R* __restrict__ e,
                                             const Site& st, R x, R y, R z, uint64_t step,
                                             const R* __restrict__ noise, R Bu0, R Bu1, R Bu2, R& bx, R& by, R& bz) {
   R hx, hy, hz, gx, gy, gz, nx = R(0), ny = R(0), nz = R(0);
+  SiteA<R> A;
+  A.load(S, d, st);
   exchange<R, CM, GEO>(S, d, e, st, hx, hy, hz, [&] {
     if (d.noisy) thermal<R, LEAN>(S, d, st, step, noise, nx, ny, nz);
   });
-  onsite_grad<R, LEAN>(S, d, st, x, y, z, gx, gy, gz);
+  onsite_grad<R, LEAN>(S, d, st, A, x, y, z, gx, gy, gz);
   field_from<R, LEAN>(S, d, st, x, y, z, hx, hy, hz, gx, gy, gz, nx, ny, nz, Bu0, Bu1, Bu2, bx, by, bz);
 }
 
@@ -686,6 +690,8 @@ __device__ __forceinline__ void corrector(const Static<R>* __restrict__ S, const
                                           R* __restrict__ save_B) {
   const int i = st.i;
   const R x = ep[3 * i], y = ep[3 * i + 1], z = ep[3 * i + 2];
+  // w (or the rotation vector), loaded ahead of the field like the on-site tensor
+  const R k1x = k1[3 * i], k1y = k1[3 * i + 1], k1z = k1[3 * i + 2];
   R bx, by, bz, fx, fy, fz;
   local_field<R, CM, GEO, LEAN>(S, d, ep, st, x, y, z, step, noise, Bu0, Bu1, Bu2, bx, by, bz);
   if (!LEAN && save_B) {
@@ -702,11 +708,11 @@ __device__ __forceinline__ void corrector(const Static<R>* __restrict__ S, const
     cx = e[3 * i];
     cy = e[3 * i + 1];
     cz = e[3 * i + 2];
-    rotate(cx, cy, cz, hdt * (k1[3 * i] + wx), hdt * (k1[3 * i + 1] + wy), hdt * (k1[3 * i + 2] + wz));
+    rotate(cx, cy, cz, hdt * (k1x + wx), hdt * (k1y + wy), hdt * (k1z + wz));
   } else {
-    cx = k1[3 * i] + hdt * fx;
-    cy = k1[3 * i + 1] + hdt * fy;
-    cz = k1[3 * i + 2] + hdt * fz;
+    cx = k1x + hdt * fx;
+    cy = k1y + hdt * fy;
+    cz = k1z + hdt * fz;
   }
   const R q = rnorm3(cx, cy, cz);
   e[3 * i] = cx * q;

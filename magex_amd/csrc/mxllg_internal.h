@@ -208,6 +208,9 @@ int launch_fill_u32(uint32_t* p, size_t n, hipStream_t s);
 int launch_counter(uint64_t* c, uint64_t value, int add, hipStream_t s);
 // Whether a step of m runs as one launch (k_step), and the number of held coupling batches it uses.
 bool fused_step(const Model& m, int* nb = nullptr);
+// Whether a step of m runs as one walked launch (k_walk: FP32 per-bond tensors held in registers, the
+// per-XCD column map, a grid larger than the GPU holds at once), its held batches and its grid.
+bool walk_step(const Model& m, int* nb = nullptr, int* grid = nullptr);
 // Whether runs of steps of m run in persistent launches (k_steps: step mode 2, row-major cells, a grid of
 // at most one workgroup per compute unit, a small round of polls).
 bool persistent_steps(const Model& m);
@@ -216,8 +219,9 @@ bool persistent_steps(const Model& m);
 constexpr int STEPS_PER_LAUNCH = 1024;
 int launch_heun_steps(const Model& m, void* e, uint64_t seed, uint64_t step0, int64_t nsteps, const void* noise,
                       hipStream_t s, const StepBufs* bufs = nullptr);
-// Device words of the one-launch steps (allocated and zeroed with the model's scratch), and the offset of
-// the control words among them.
+// Device words of the one-launch steps (allocated and zeroed with the model's scratch: plane counters,
+// control words, give-up slots, the walked step's ticket counters), and the offset of the control words
+// among them.
 size_t stepsync_words(const Model& m);
 inline size_t stepsync_ctl(const Model& m) { return size_t(m.L[0]) * CNT_STRIDE; }
 // 64-bit words of the tagged spin records of persistent launches.

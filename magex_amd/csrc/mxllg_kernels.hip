@@ -210,7 +210,8 @@ __device__ __forceinline__ bool predictor_held(const Static<R>* __restrict__ S, 
   bool ok = true;
   h.nx = h.ny = h.nz = R(0);
   if constexpr (NB > 0) {
-    // every held batch in flight before the first is used
+    // the on-site tensor, then every held batch, in flight before the first is used
+    if (LOADJ) h.A.load(S, d, st);
 #pragma unroll
     for (int n = 0; n < NB; ++n) {
       const bool g = fill_batch<R, CM, GEO, LOADJ>(h.bt[n], S, d, src, st, kb + n * CH, ke);
@@ -243,7 +244,9 @@ __device__ __forceinline__ bool predictor_held(const Static<R>* __restrict__ S, 
     });
   }
   const R x = h.x, y = h.y, z = h.z;
-  if constexpr (!(Src::tagged && NB == 0)) {
+  if constexpr (NB > 0) {
+    onsite_grad<R, LEAN>(S, d, st, h.A, x, y, z, gx, gy, gz);
+  } else if constexpr (!Src::tagged) {
     if (LOADJ) h.A.load(S, d, st);
     onsite_grad<R, LEAN>(S, d, st, h.A, x, y, z, gx, gy, gz);
   }
@@ -350,15 +353,15 @@ __device__ __forceinline__ void wait_window(const Static<R>* __restrict__ S, int
   }
 }
 
-// End of a one-launch step (k_step), called by every thread of every workgroup once its own
+// End of a one-launch step (k_step, k_walk), called by every thread of every workgroup once its own
 // work is done (gave: this workgroup gave up a wait and listed tiles in sy.gu): the last workgroup to
 // get here runs the correctors of the listed tiles (every predictor is done by then; w is in k1) and
-// clears the control words for the next launch.
+// clears the control words for the next launch (with k_walk's ticket counters, tick).
 template <typename R, int CM, int GEO, bool LEAN>
 __device__ __forceinline__ void finish_step(const Static<R>* __restrict__ S, const Dyn<R>& d, R* __restrict__ e,
                                             const R* __restrict__ ep, const R* __restrict__ k1, uint64_t step,
                                             const R* __restrict__ noise, const StepSync& sy, bool gave,
-                                            int& sh_flag) {
+                                            int& sh_flag, uint32_t* tick = nullptr) {
   if (threadIdx.x == 0) {
     const uint32_t done = gave ? __hip_atomic_fetch_add(sy.ctl, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT)
                                : __hip_atomic_fetch_add(sy.ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -382,6 +385,8 @@ __device__ __forceinline__ void finish_step(const Static<R>* __restrict__ S, con
   }
   for (int P = threadIdx.x; P < S->L0; P += BLOCK)
     __hip_atomic_store(sy.cnt + P * CNT_STRIDE, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (tick && threadIdx.x < 8)
+    __hip_atomic_store(tick + threadIdx.x * CNT_STRIDE, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
   if (threadIdx.x == 0) {
     if (ngu) __hip_atomic_fetch_add(sy.ctl + 2, ngu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -497,6 +502,103 @@ __global__ __launch_bounds__(BLOCK) void k_step(const Static<R>* __restrict__ S,
   }
   __syncthreads();
   finish_step<R, CM, GEO, LEAN>(S, d, e, ep, k1, step, noise, sy, !ok, sh_flag);
+}
+
+// One Heun step in one launch for grids larger than the GPU holds at once (magex/fast.py:336-390 as one
+// kernel, like k_step).  The grid is the G workgroups the GPU holds at once.  They walk the tiles of the
+// per-XCD column map in plane order: the tiles of each XCD are the tickets of its own counter (workgroup
+// b counts on counter b mod 8, the XCD the dispatcher deals it to).  A workgroup runs the predictor of its
+// ticket's tile, publishes the predictor spins as k_step does, waits for the planes within r0 of the
+// tile, takes its next ticket, and runs the corrector from registers.  Every per-bond tensor then comes
+// from HBM once per step, and the tensors in flight between a predictor and its corrector sit in the
+// registers of the workgroups that hold them rather than in L2.  Same arithmetic as k_heun's two
+// passes: same result bits.
+//
+// Progress: predictors never wait, and a workgroup takes its next ticket only after its wait has ended,
+// so a wait needs only the tickets of the planes within r0 of its tile to be taken.  Let P be the tickets
+// of one plane on one counter.  With every workgroup resident and more than (2 r0 + 1) P workgroups per
+// counter, the counter whose frontier is furthest behind always has a workgroup whose tile's planes are
+// all predicted: at most (r0 + 1) P of its held tiles lie within r0 of the frontier, and at most r0 P are
+// first-plane tiles whose periodic stencil wraps onto the last planes (they hold their workgroup until
+// those are predicted).  Waits are bounded all the same: a workgroup that gives up leaves its corrector
+// to the launch's last workgroup, as in k_step.
+template <typename R, int CM, int GEO, int NB, bool LEAN>
+__global__ __launch_bounds__(BLOCK) void k_walk(const Static<R>* __restrict__ S, const Dyn<R> d, R* __restrict__ e,
+                                                R* __restrict__ ep, R* __restrict__ k1, uint64_t step,
+                                                const R* __restrict__ noise, const StepSync sy,
+                                                uint32_t* __restrict__ tick) {
+  __shared__ int sh_flag, sh_next;
+  if (d.step_ptr) step += *d.step_ptr;  // graph replays: step = device counter + offset of this node
+  const int x = blockIdx.x & 7, per_ctr = int(gridDim.x >> 3), ntick = S->nblk >> 3, PS = sy.PS;
+  bool gave = false;
+  for (int t = int(blockIdx.x >> 3); t < ntick;) {
+    const int b = (t << 3) | x;  // the block of k_step's column walk that this ticket stands for
+    Site st;
+    int tile;
+    const bool live = step_site<R>(S, b, st, tile);
+    const int i = st.i;
+    const bool pin = live && !LEAN && d.pinned && d.pinned[i];
+    const int kb = S->sptr[st.s], ke = S->sptr[st.s + 1];
+    Held<R, CM, GEO, NB> h;
+    // ---- predictor (pass 1)
+    if (live) {
+      h.a = (!LEAN && d.alpha) ? d.alpha[i] : S->alpha_sub[st.s];
+      h.x = e[3 * i];
+      h.y = e[3 * i + 1];
+      h.z = e[3 * i + 2];
+      if (pin) {
+        h.px = h.wx = h.x;
+        h.py = h.wy = h.y;
+        h.pz = h.wz = h.z;
+      } else {
+        predictor_held<R, CM, GEO, NB, LEAN, true>(S, d, SpinsFrom<R, 0>{e}, st, kb, ke, step, noise, h);
+      }
+      store3_coherent(ep + 3 * i, h.px, h.py, h.pz);
+    }
+    // ---- publish: stores acknowledged, then the plane counters
+    const int c_first = tile * BLOCK, c_last = min(S->ncell, c_first + BLOCK) - 1;
+    const int pa = c_first / PS, pb = c_last / PS;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (int(threadIdx.x) <= pb - pa)
+      __hip_atomic_fetch_add(sy.cnt + (pa + threadIdx.x) * CNT_STRIDE, 1u, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    // ---- wait for the planes within r0
+    if (threadIdx.x < 64) {
+      int lo, n;
+      wait_window<R>(S, pa, pb, sy.r0, lo, n);
+      const bool ok = wait_planes<R>(S, sy, sy.cnt, 1u, lo, n);
+      if (threadIdx.x == 0) sh_flag = ok ? 1 : 0;
+    }
+    __syncthreads();
+    const bool ok = sh_flag != 0;
+    // ---- the next ticket, taken now that the wait is over; its latency overlaps the corrector
+    uint32_t nt = 0;
+    if (threadIdx.x == 0) nt = __hip_atomic_fetch_add(tick + x * CNT_STRIDE, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (ok) {
+      // ---- corrector (pass 2) from registers
+      if (live && !pin) {
+        corrector_held<R, CM, GEO, NB, LEAN>(S, d, SpinsFrom<R, 0>{ep}, st, kb, ke, h);
+        e[3 * i] = h.x;
+        e[3 * i + 1] = h.y;
+        e[3 * i + 2] = h.z;
+      }
+    } else {
+      // gave up: keep w (or the rotation vector) for the last workgroup's corrector
+      if (live && !pin) {
+        k1[3 * i] = h.wx;
+        k1[3 * i + 1] = h.wy;
+        k1[3 * i + 2] = h.wz;
+      }
+      give_up_tile(sy, b);
+      gave = true;
+    }
+    // the counter hands out tickets from G / 8 on (the first G / 8 are the workgroups' own)
+    if (threadIdx.x == 0) sh_next = per_ctr + int(nt);
+    __syncthreads();
+    t = sh_next;
+  }
+  finish_step<R, CM, GEO, LEAN>(S, d, e, ep, k1, step, noise, sy, gave, sh_flag, tick);
 }
 
 // nsteps stochastic Heun (or Depondt) steps in one launch, for grids that are resident all at once
@@ -907,7 +1009,13 @@ static int reach0(const Model& m) {
   return std::min(r, m.L[0]);
 }
 
-size_t stepsync_words(const Model& m) { return stepsync_ctl(m) + CTL_WORDS + size_t(nblk_of(m)); }
+// The walked step's 8 ticket counters (one per 128-byte line) follow the give-up slots.
+static size_t stepsync_tick(const Model& m) {
+  const size_t w = stepsync_ctl(m) + CTL_WORDS + size_t(nblk_of(m));
+  return (w + CNT_STRIDE - 1) / CNT_STRIDE * CNT_STRIDE;
+}
+
+size_t stepsync_words(const Model& m) { return stepsync_tick(m) + 8 * CNT_STRIDE; }
 
 // Per-bond couplings keep all their tensors in registers across the two passes: FP32 up to two batches
 // of 7 (the 14-entry bcc stencil); FP64 tensors (twice the registers) are not held, and FP64 per-bond
@@ -950,6 +1058,51 @@ bool fused_step(const Model& m, int* nb_out) {
   return true;
 }
 
+// Workgroups per compute unit of the walked step (the lean and general kernels alike).
+template <int GEO, int NB>
+static int walk_occupancy() {
+  static int occ = -1;
+  if (occ < 0) {
+    int a = 0, b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_walk<float, COUP_BOND, GEO, NB, true>, BLOCK, 0) !=
+        hipSuccess)
+      a = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_walk<float, COUP_BOND, GEO, NB, false>, BLOCK, 0) !=
+        hipSuccess)
+      b = 0;
+    occ = std::min(a, b);
+  }
+  return occ;
+}
+
+bool walk_step(const Model& m, int* nb_out, int* grid_out) {
+  if (m.step_mode != 3 || !(m.geo == GEO_PERIODIC || m.geo == GEO_OPEN)) return false;
+  const int nb = held_batches(m);
+  if (nb < 1 || xcol_of(m) == 0) return false;
+  int dev_cu = 0;
+  if (hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, m.device) != hipSuccess || dev_cu <= 0)
+    return false;
+  const int nblk = nblk_of(m);
+  if (nblk <= dev_cu) return false;  // the whole grid is resident: k_step
+  int occ;
+  if (m.geo == GEO_PERIODIC) occ = nb == 1 ? walk_occupancy<GEO_PERIODIC, 1>() : walk_occupancy<GEO_PERIODIC, 2>();
+  else occ = nb == 1 ? walk_occupancy<GEO_OPEN, 1>() : walk_occupancy<GEO_OPEN, 2>();
+  const int G = std::min(nblk, occ * dev_cu) & ~7;
+  // the progress bound of k_walk: more than (2 r0 + 1) x (tickets per plane per counter) per counter
+  if (long(G / 8) <= (2L * reach0(m) + 1) * long(xcol_of(m)) * m.ns) return false;
+  if (nb_out) *nb_out = nb;
+  if (grid_out) *grid_out = G;
+  return true;
+}
+
+template <typename R, int GEO, int NB>
+static void walk_launch(bool lean, dim3 g, hipStream_t s, const Static<R>* S, const Dyn<R>& d, R* E, R* ep, R* k1,
+                        uint64_t step, const R* nz, const StepSync& sy, uint32_t* tick) {
+  const dim3 b(BLOCK);
+  if (lean) hipLaunchKernelGGL((k_walk<R, COUP_BOND, GEO, NB, true>), g, b, 0, s, S, d, E, ep, k1, step, nz, sy, tick);
+  else hipLaunchKernelGGL((k_walk<R, COUP_BOND, GEO, NB, false>), g, b, 0, s, S, d, E, ep, k1, step, nz, sy, tick);
+}
+
 template <typename R, int CM, int GEO, int NB>
 static void step_launch(const Model& m, bool lean, dim3 g, hipStream_t s, const Static<R>* S, const Dyn<R>& d, R* E,
                         R* ep, R* k1, uint64_t step, const R* nz, const StepSync& sy) {
@@ -990,7 +1143,7 @@ static size_t poll_bytes(const Model& m) {
 // the launches they save.
 constexpr size_t PERSISTENT_POLL_BYTES = size_t(24) << 20;
 bool persistent_steps(const Model& m) {
-  if (m.step_mode != 2 || !(m.geo == GEO_PERIODIC || m.geo == GEO_OPEN)) return false;
+  if (m.step_mode < 2 || !(m.geo == GEO_PERIODIC || m.geo == GEO_OPEN)) return false;
   if (poll_bytes(m) > PERSISTENT_POLL_BYTES) return false;
   int dev_cu = 0;
   if (hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, m.device) != hipSuccess || dev_cu <= 0)
@@ -1095,6 +1248,29 @@ static int heun_t(const Model& m, void* e, uint64_t seed, uint64_t step, const v
     if (m.geo == GEO_PERIODIC) step_dispatch<R, GEO_PERIODIC>(m, nb, lean, g, s, S, d, E, ep, k1, step, nz, sy);
     else step_dispatch<R, GEO_OPEN>(m, nb, lean, g, s, S, d, E, ep, k1, step, nz, sy);
     return check_launch("heun step (one launch)");
+  }
+  int G = 0;
+  if constexpr (sizeof(R) == 4) {
+    if (sync && walk_step(m, &nb, &G)) {
+      StepSync sy;
+      sy.cnt = sync;
+      sy.ctl = sync + stepsync_ctl(m);
+      sy.gu = reinterpret_cast<int32_t*>(sy.ctl + CTL_WORDS);
+      sy.r0 = reach0(m);
+      sy.PS = m.L[1] * m.L[2];
+      sy.spin = 1u << 16;
+      uint32_t* tick = sync + stepsync_tick(m);
+      const bool lean = lean_model(m) && !noise;
+      const dim3 gw(G);
+      if (m.geo == GEO_PERIODIC) {
+        if (nb == 1) walk_launch<R, GEO_PERIODIC, 1>(lean, gw, s, S, d, E, ep, k1, step, nz, sy, tick);
+        else walk_launch<R, GEO_PERIODIC, 2>(lean, gw, s, S, d, E, ep, k1, step, nz, sy, tick);
+      } else {
+        if (nb == 1) walk_launch<R, GEO_OPEN, 1>(lean, gw, s, S, d, E, ep, k1, step, nz, sy, tick);
+        else walk_launch<R, GEO_OPEN, 2>(lean, gw, s, S, d, E, ep, k1, step, nz, sy, tick);
+      }
+      return check_launch("heun step (walked one launch)");
+    }
   }
   if (lean_model(m) && !noise) {
 #define MX_HEUN_CALL(CM, GEO)                                                                   \

@@ -217,12 +217,13 @@ class DeviceModel:
 
     def set_step_mode(self, mode):
         """0: two kernel launches per Heun step; 1: one launch per step where the model allows it; 2
-        (default): as 1, and runs of two or more steps in persistent launches where they pay."""
+        (default): as 1, and runs of two or more steps in persistent launches where they pay; 3: as 2, and
+        one walked launch per step for FP32 per-bond grids larger than the GPU holds at once."""
         check(self.lib.mx_model_set_step_mode(self.h, int(mode)))
 
     def step_info(self):
-        """(launch form: 0 two per step, 1 one per step, 2 persistent runs; waits given up or persistent
-        launches stopped early since creation -- synchronises)."""
+        """(launch form: 0 two per step, 1 one per step, 2 persistent runs, 3 one walked launch per step;
+        waits given up or persistent launches stopped early since creation -- synchronises)."""
         one, gu = ctypes.c_int32(), ctypes.c_int64()
         check(self.lib.mx_model_step_info(self.h, ctypes.byref(one), ctypes.byref(gu)))
         return int(one.value), int(gu.value)

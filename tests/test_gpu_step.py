@@ -36,7 +36,7 @@ def _bcc_perbond():
 
 
 def _modes_agree(torch, s, precision, steps, dt, alpha=0.1, T=0.0, integrator="heun", seed=5, noise=None,
-                 graph=None, sot=None, forms=(0, 1, 2)):
+                 graph=None, sot=None, forms=(0, 1, 2), modes=(0, 1, 2)):
     dm = s.device_model(precision)
     if graph is not None:
         dm.set_graph_steps(graph)
@@ -46,18 +46,18 @@ def _modes_agree(torch, s, precision, steps, dt, alpha=0.1, T=0.0, integrator="h
     e0 = torch.as_tensor(s.init_spins("random", seed=2), device="cuda", dtype=dm.dtype).contiguous()
     nz = None if noise is None else torch.as_tensor(noise, device="cuda", dtype=dm.dtype).contiguous()
     out = {}
-    for mode in (0, 1, 2):
+    for mode, form_want in zip(modes, forms):
         dm.set_step_mode(mode)
         e = e0.clone()
         dm.run(e, steps, seed, 0, nz)
         out[mode] = e
         form, given_up = dm.step_info()
-        assert form == forms[mode], f"step mode {mode}: the model runs in form {form}"
+        assert form == form_want, f"step mode {mode}: the model runs in form {form}"
         assert given_up == 0
-    assert torch.isfinite(out[1]).all()
-    for mode in (1, 2):
-        assert torch.equal(out[0], out[mode]), (mode, float((out[0] - out[mode]).abs().max()))
-    return out[1]
+    assert torch.isfinite(out[modes[1]]).all()
+    for mode in modes[1:]:
+        assert torch.equal(out[modes[0]], out[mode]), (mode, float((out[modes[0]] - out[mode]).abs().max()))
+    return out[modes[1]]
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
@@ -89,6 +89,36 @@ def test_bcc_per_bond_fp64_persistent(gpu):
     alpha = 0.1 + 0.05 * np.random.default_rng(6).random(s.N)
     noise = np.random.default_rng(7).normal(size=(12, s.N, 3))
     _modes_agree(gpu, s, "fp64", 12, 1e-4, alpha, 300.0, noise=noise, forms=(0, 0, 2))
+
+
+def _bcc_large(pbc0=True):
+    """bcc 64 x 64 x 32 with random per-bond tensors and SIA (262 144 spins): 1024 tiles of 256 sites,
+    twice what the GPU holds at once in the walked step, so every workgroup takes a second ticket; axis-0
+    planes of 2048 cells (one tile per XCD per plane and sublattice)."""
+    from magex_amd import configs
+    s = S.SpinSystem(S.load_results(M.make_model("bcc_conventional", a=2.87, J=[20.0, 10.0], moment=2.2)),
+                     (64, 64, 32), (pbc0, True, True))
+    s.set_bond_tensors(configs.random_bond_tensors(s, seed=3))
+    s.A = configs.random_sia(s.N, seed=4)
+    return s
+
+
+def test_walked_step_on_a_grid_larger_than_the_gpu_holds(gpu):
+    """Step mode 3: the walked one-launch step (ticket-ordered tiles, tensors held in registers from the
+    predictor to the corrector) equals two launches bit for bit, with generated and injected noise, the
+    general kernel (per-site damping, pinned sites, Depondt) and an open axis 0, and no wait gives up."""
+    print("case: bcc 64x64x32 per-bond fp32, walked step", flush=True)
+    s = _bcc_large()
+    _modes_agree(gpu, s, "fp32", 12, 1e-4, 0.1, 300.0, forms=(0, 3), modes=(0, 3))
+    noise = np.random.default_rng(9).normal(size=(4, s.N, 3))
+    _modes_agree(gpu, s, "fp32", 4, 1e-4, 0.1, 300.0, noise=noise, forms=(0, 3), modes=(0, 3))
+    rng = np.random.default_rng(10)
+    s.pinned = rng.random(s.N) < 0.05
+    s.touch()
+    alpha = 0.1 + 0.05 * rng.random(s.N)
+    _modes_agree(gpu, s, "fp32", 6, 1e-4, alpha, 300.0, "depondt", forms=(0, 3), modes=(0, 3))
+    o = _bcc_large(pbc0=False)
+    _modes_agree(gpu, o, "fp32", 6, 1e-4, 0.1, 300.0, forms=(0, 3), modes=(0, 3))
 
 
 @pytest.mark.parametrize("integrator", ["heun", "depondt"])
@@ -169,3 +199,12 @@ def test_launch_forms_follow_the_measured_policy(gpu):
     dm = s.device_model("fp64")
     dm.set_step_mode(1)
     assert dm.step_info()[0] == 0     # FP64 beyond 16 k sites keeps two launches per step
+    s, p = configs.bcc64()
+    dm = s.device_model("fp32")
+    dm.set_dynamics(p["dt"], p["alpha"], p["temperature"], True, "heun")
+    dm.set_step_mode(3)
+    assert dm.step_info()[0] == 3     # the walked step only when asked for (slower on this grid)
+    dm = s.device_model("fp64")
+    dm.set_dynamics(p["dt"], p["alpha"], p["temperature"], True, "heun")
+    dm.set_step_mode(3)
+    assert dm.step_info()[0] == 0     # FP64 tensors are not held: two launches
