@@ -21,7 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 ROOT = os.path.dirname(PKG)
 INCLUDE = os.path.join(ROOT, "include")
-SOURCES = ["mxllg_api.hip", "mxllg_kernels.hip", "mxllg_batch.hip", "mxllg_adjoint.hip", "mxllg_mc.hip", "mxllg_sampler.hip"]
+SOURCES = ["mxllg_api.hip", "mxllg_kernels.hip", "mxllg_steps.hip", "mxllg_batch.hip", "mxllg_adjoint.hip", "mxllg_mc.hip", "mxllg_sampler.hip"]
 TARGET = os.path.join(PKG, "libmxllg.so")
 ARCH = "gfx950"
 
