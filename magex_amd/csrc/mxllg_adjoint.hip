@@ -174,15 +174,12 @@ __device__ __forceinline__ void onsite_T(const Static<R>* __restrict__ S, const 
 // NP (4, 8 or 16) is the parameter count rounded up; padded parameters have zero units.
 // Entries go in batches of CH whose loads are all issued before the first is used (entries past the
 // end re-read the last one with weight zero), so a wavefront keeps 2 CH gathers in flight.
-// DENSE also adds nu_i x_j^T + x_i nu_j^T to gb[(f * 9 + q) * ncell + cell] for every forward entry f
-// of the site (each unordered bond has one owner thread, so no two threads write one element) and
-// -nu_i x_i^T to ga[q * N + site].
-template <typename R, int GEO, int NP, bool DENSE, int CM>
+template <typename R, int GEO, int NP, int CM>
 __device__ __forceinline__ void adj_gather(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
                                            const Dyn<R>& d, const R* __restrict__ nu, const R* __restrict__ x,
                                            const Site& st,
                                            R nix, R niy, R niz, R xix, R xiy, R xiz, R& hx, R& hy, R& hz,
-                                           double (&gp)[NP], double* __restrict__ gb, double* __restrict__ ga) {
+                                           double (&gp)[NP]) {
   constexpr int CH = 8;
   const int ncell = S->ncell;
   R t[NP][3];
@@ -242,48 +239,6 @@ __device__ __forceinline__ void adj_gather(const Static<R>* __restrict__ S, cons
         t[p][2] += U[6] * qx[u] + U[7] * qy[u] + U[8] * qz[u];
       }
     }
-    if constexpr (DENSE) {
-      // accumulator loads of DG entries are issued before any of their stores: loads and stores share
-      // one completion counter, so interleaving them would make every read-modify-write a round trip
-      constexpr int DG = 4;
-      static_assert(CH % DG == 0, "dense groups must tile the batch");
-      const double ni[3] = {double(nix), double(niy), double(niz)}, xi[3] = {double(xix), double(xiy), double(xiz)};
-#pragma unroll
-      for (int u0 = 0; u0 < CH; u0 += DG) {
-        double acc[DG][9];
-        double* gq[DG];
-#pragma unroll
-        for (int v = 0; v < DG; ++v) {
-          const int u = u0 + v, f = A->fslot[kk[u]];
-          gq[v] = (k0 + u < ke && f >= 0) ? gb + size_t(f) * 9 * ncell + st.c : nullptr;
-          if (gq[v]) {
-#pragma unroll
-            for (int q = 0; q < 9; ++q) acc[v][q] = gq[v][size_t(q) * ncell];
-          }
-        }
-#pragma unroll
-        for (int v = 0; v < DG; ++v) {
-          if (gq[v]) {
-            const int u = u0 + v;
-            const double nj[3] = {double(vx[u]), double(vy[u]), double(vz[u])};
-            const double xj[3] = {double(qx[u]), double(qy[u]), double(qz[u])};
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-              for (int c = 0; c < 3; ++c) gq[v][size_t(3 * r + c) * ncell] = acc[v][3 * r + c] + (ni[r] * xj[c] + xi[r] * nj[c]);
-          }
-        }
-      }
-    }
-  }
-  if constexpr (DENSE) {
-    const double ni[3] = {double(nix), double(niy), double(niz)}, xi[3] = {double(xix), double(xiy), double(xiz)};
-    double* g = ga + st.i;
-    const size_t N = size_t(S->N);
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) g[(3 * r + c) * N] -= ni[r] * xi[c];
   }
 #pragma unroll
   for (int p = 0; p < NP; ++p) {
@@ -419,12 +374,11 @@ __device__ __forceinline__ void store_field_partials(double (&fv)[3], double* __
 }
 
 // Gather nu2 (dL/dep) and the predictor's adjoint: writes nu1 and the partial dL/de (part).
-template <typename R, int GEO, int NP, bool DENSE, int CM, bool LEAN>
+template <typename R, int GEO, int NP, int CM, bool LEAN>
 __global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
                                                  const Dyn<R> d, const StepRec<R> r, const R* __restrict__ lamu,
                                                  const R* __restrict__ gep, const R* __restrict__ nu2,
                                                  R* __restrict__ nu1, R* __restrict__ part, double* __restrict__ gpart,
-                                                 double* __restrict__ gb, double* __restrict__ ga,
                                                  double* __restrict__ gf, const R* __restrict__ gwb) {
   const double old = partial_row<NP>(gpart);
   double gp[NP];
@@ -444,7 +398,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S
       fv[2] = h * double(vz2);
     }
     R hx, hy, hz, ox, oy, oz;
-    adj_gather<R, GEO, NP, DENSE, CM>(S, A, d, nu2, r.ep, st, vx2, vy2, vz2, px, py, pz, hx, hy, hz, gp, gb, ga);
+    adj_gather<R, GEO, NP, CM>(S, A, d, nu2, r.ep, st, vx2, vy2, vz2, px, py, pz, hx, hy, hz, gp);
     onsite_T<R, LEAN>(S, d, st, px, py, pz, vx2, vy2, vz2, ox, oy, oz);
     R lx = gep[3 * i] + hx - ox, ly = gep[3 * i + 1] + hy - oy, lz = gep[3 * i + 2] + hz - oz;  // dL/dep
     if (!LEAN && d.pinned && d.pinned[i]) {
@@ -506,13 +460,12 @@ __global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S
 // Gather nu1: lambda = dL/de of the step's input, and the parameter terms at e.  With FUSE the
 // previous step's local part (adj_local on its record prev) follows at once from lambda, which then
 // never goes through memory; otherwise lambda is stored for a later k_adj_a.
-template <typename R, int GEO, int NP, bool FUSE, bool DENSE, int CM, bool LEAN>
+template <typename R, int GEO, int NP, bool FUSE, int CM, bool LEAN>
 __global__ __launch_bounds__(BLOCK) void k_adj_c(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
                                                  const Dyn<R> d, const R* __restrict__ e, const R* __restrict__ nu1,
                                                  const R* __restrict__ part, R* __restrict__ lam,
                                                  const StepRec<R> prev, R* __restrict__ lamu, R* __restrict__ gep,
                                                  R* __restrict__ nu2, double* __restrict__ gpart,
-                                                 double* __restrict__ gb, double* __restrict__ ga,
                                                  double* __restrict__ gf, R* __restrict__ gwb) {
   const double old = partial_row<NP>(gpart);
   double gp[NP];
@@ -530,8 +483,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_c(const Static<R>* __restrict__ S
       fv[2] = h * double(vz);
     }
     R hx, hy, hz, ox, oy, oz;
-    adj_gather<R, GEO, NP, DENSE, CM>(S, A, d, nu1, e, st, vx, vy, vz, e[3 * i], e[3 * i + 1], e[3 * i + 2], hx, hy,
-                                      hz, gp, gb, ga);
+    adj_gather<R, GEO, NP, CM>(S, A, d, nu1, e, st, vx, vy, vz, e[3 * i], e[3 * i + 1], e[3 * i + 2], hx, hy, hz,
+                               gp);
     onsite_T<R, LEAN>(S, d, st, e[3 * i], e[3 * i + 1], e[3 * i + 2], vx, vy, vz, ox, oy, oz);
     const R lx = part[3 * i] + hx - ox, ly = part[3 * i + 1] + hy - oy, lz = part[3 * i + 2] + hz - oz;
     if constexpr (FUSE) {
@@ -556,6 +509,103 @@ template <typename T>
 __global__ void k_copy(const T* __restrict__ a, T* __restrict__ b, size_t n) {
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) b[i] = a[i];
+}
+
+// Dense gradients of one replayed segment of n_seg steps, from its records and the adjoint vectors its
+// sweep saved: nu1 (at e) and nu2 (at ep) of step n at nuh[(0 or 1) * C + n], the records' e and ep at
+// seg[(0 or 1) * C + n] (N x 3 each, internal order).  Per unordered bond b = (i, j) stored once
+// (J_ij = T_b, J_ji = T_b^T), owned by the forward entry of site i:
+//   dL/dT_b += nu_i x_j^T + x_i nu_j^T,   per site  dL/dA_i += -nu_i x_i^T,
+// pass by pass in the order the sweep runs them (newest step first, its corrector pass -- nu2 at ep --
+// before its predictor pass -- nu1 at e), so the sums are those of adding every pass's terms to the
+// accumulators as it runs, and every accumulator is read and written once per segment instead of once
+// per pass.  One thread per site, up to FG forward entries at a time in FP64 registers; neighbours past
+// an open face count zero.  Accumulators: gb[(f * 9 + q) * ncell + cell], ga[q * N + site].
+template <typename R, int GEO, int FG>
+__global__ __launch_bounds__(BLOCK) void k_dense_seg(const Static<R>* __restrict__ S,
+                                                     const AStatic<R>* __restrict__ A, const R* __restrict__ seg,
+                                                     const R* __restrict__ nuh, int C, int n_seg,
+                                                     double* __restrict__ gb, double* __restrict__ ga) {
+  Site st;
+  if (!site_of<R, GEO>(S, st)) return;
+  const int i = st.i, ncell = S->ncell;
+  const size_t N3 = size_t(S->N) * 3, N = size_t(S->N);
+  const int kb = S->sptr[st.s], ke = S->sptr[st.s + 1];
+  int k = kb;
+  for (bool first = true;; first = false) {
+    // the next FG forward entries of the sublattice (wavefront-uniform)
+    int kk[FG], nf = 0;
+#pragma unroll
+    for (int u = 0; u < FG; ++u) {
+      while (k < ke && A->fslot[k] < 0) ++k;
+      kk[u] = k < ke ? k++ : -1;
+      nf += kk[u] >= 0 ? 1 : 0;
+    }
+    if (nf == 0 && !first) break;
+    int jj[FG];
+    R w[FG];
+    double acc[FG][9];
+#pragma unroll
+    for (int u = 0; u < FG; ++u) {
+      jj[u] = i;
+      w[u] = R(0);
+      if (u < nf) {
+        const Ent en = S->ent[kk[u]];
+        bool ok;
+        const int cn = neighbour_cell<R, GEO>(S, st, en, ok);
+        jj[u] = (en.tb & 0xffff) * ncell + cn;
+        w[u] = ok ? R(1) : R(0);
+        const double* g = gb + size_t(A->fslot[kk[u]]) * 9 * ncell + st.c;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) acc[u][q] = g[size_t(q) * ncell];
+      }
+    }
+    double aa[9];
+    if (first) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) aa[q] = ga[q * N + i];
+    }
+    for (int n = n_seg - 1; n >= 0; --n) {
+#pragma unroll
+      for (int pass = 1; pass >= 0; --pass) {
+        const R* x = seg + (size_t(pass) * C + n) * N3;
+        const R* v = nuh + (size_t(pass) * C + n) * N3;
+        const double ni[3] = {double(v[3 * i]), double(v[3 * i + 1]), double(v[3 * i + 2])};
+        const double xi[3] = {double(x[3 * i]), double(x[3 * i + 1]), double(x[3 * i + 2])};
+#pragma unroll
+        for (int u = 0; u < FG; ++u) {
+          if (u < nf) {
+            const int j = jj[u];
+            const double nj[3] = {double(v[3 * j] * w[u]), double(v[3 * j + 1] * w[u]), double(v[3 * j + 2] * w[u])};
+            const double xj[3] = {double(x[3 * j] * w[u]), double(x[3 * j + 1] * w[u]), double(x[3 * j + 2] * w[u])};
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+              for (int c = 0; c < 3; ++c) acc[u][3 * r + c] = acc[u][3 * r + c] + (ni[r] * xj[c] + xi[r] * nj[c]);
+          }
+        }
+        if (first) {
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) aa[3 * r + c] -= ni[r] * xi[c];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < FG; ++u) {
+      if (u < nf) {
+        double* g = gb + size_t(A->fslot[kk[u]]) * 9 * ncell + st.c;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) g[size_t(q) * ncell] = acc[u][q];
+      }
+    }
+    if (first) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) ga[q * N + i] = aa[q];
+    }
+    if (k >= ke) break;
+  }
 }
 
 // Dense gradients from the accumulators (component-major, internal cells) to the caller's layouts:
@@ -616,6 +666,8 @@ struct mx_adjoint {
   // dense gradients: per forward bond 9 x ncell, per site 9 x N (FP64, internal cells), and the magex
   // cell -> internal cell map of the layout cmap_geo (null for row-major)
   double *gb_d = nullptr, *ga_d = nullptr;
+  void* nuh_d = nullptr;  // the sweep's nu1 and nu2 of every step of a segment (2 x nuh_cap x N x 3)
+  int nuh_cap = 0;
   int* cmap_d = nullptr;
   int cmap_geo = -1;
   // driven forward: the uniform field of each pass of each step (nsteps x 2 x 3; empty: the model's)
@@ -800,8 +852,10 @@ void ensure_buffers(mx_adjoint& a) {
 // Adjoint sweep over one replayed segment of n_seg steps (records rec(n), newest last): the local
 // part of the segment's last step from lam, then per step (newest first) k_adj_b and k_adj_c, the
 // latter fused with the next-older step's local part.  Leaves dL/de of the segment's first input in lam.
-template <typename R, int GEO, int NP, bool DENSE, int CM, bool LEAN>
-void sweep_t(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
+// With dense gradients (nuh: the segment's nu1 / nu2 slots, seg: its records, C steps apart) every step
+// keeps its own nu1 and nu2, and k_dense_seg adds the segment's bond and site terms afterwards.
+template <typename R, int GEO, int NP, int CM, bool LEAN>
+void sweep_t(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s, R* nuh, const R* seg, int C) {
   const Model& m = *a.m;
   const Static<R>* S = static_cast<const Static<R>*>(m.static_d);
   const AStatic<R>* A = static_cast<const AStatic<R>*>(a.astatic_d);
@@ -826,8 +880,10 @@ void sweep_t(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
   R* lam = static_cast<R*>(a.lam_d);
   R* lamu = static_cast<R*>(a.lamu_d);
   R* gep = static_cast<R*>(a.gep_d);
-  R* nu1 = static_cast<R*>(a.nu1_d);
-  R* nu2 = static_cast<R*>(a.nu2_d);
+  const size_t N3 = size_t(m.N) * 3;
+  // nu1 / nu2 of step n: the shared buffers, or the step's own slots when the dense terms follow
+  auto nu1_of = [&](int n) { return nuh ? nuh + size_t(n) * N3 : static_cast<R*>(a.nu1_d); };
+  auto nu2_of = [&](int n) { return nuh ? nuh + (size_t(C) + n) * N3 : static_cast<R*>(a.nu2_d); };
   R* part = static_cast<R*>(a.part_d);
   R* gwb = static_cast<R*>(a.gw_d);
   const dim3 g(a.nblk), b(BLOCK);
@@ -835,51 +891,45 @@ void sweep_t(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
   auto frow = [&](int n, int pass) -> double* {
     return a.want_field ? a.gf_d + (size_t(a.sweep_step0 + n) * 2 + pass) * a.nblk * 3 : nullptr;
   };
-  hipLaunchKernelGGL((k_adj_a<R, GEO, LEAN>), g, b, 0, s, S, d, lam, rec[n_seg - 1], lamu, gep, nu2, gwb);
+  hipLaunchKernelGGL((k_adj_a<R, GEO, LEAN>), g, b, 0, s, S, d, lam, rec[n_seg - 1], lamu, gep, nu2_of(n_seg - 1),
+                     gwb);
   for (int n = n_seg - 1; n >= 0; --n) {
-    hipLaunchKernelGGL((k_adj_b<R, GEO, NP, DENSE, CM, LEAN>), g, b, 0, s, S, A, d, rec[n], lamu, gep, nu2, nu1, part, a.gpart_d,
-                       a.gb_d, a.ga_d, frow(n, 1), gwb);
+    hipLaunchKernelGGL((k_adj_b<R, GEO, NP, CM, LEAN>), g, b, 0, s, S, A, d, rec[n], lamu, gep, nu2_of(n), nu1_of(n),
+                       part, a.gpart_d, frow(n, 1), gwb);
     if (n > 0)
-      hipLaunchKernelGGL((k_adj_c<R, GEO, NP, true, DENSE, CM, LEAN>), g, b, 0, s, S, A, d, rec[n].e, nu1, part, lam, rec[n - 1],
-                         lamu, gep, nu2, a.gpart_d, a.gb_d, a.ga_d, frow(n, 0), gwb);
+      hipLaunchKernelGGL((k_adj_c<R, GEO, NP, true, CM, LEAN>), g, b, 0, s, S, A, d, rec[n].e, nu1_of(n), part, lam,
+                         rec[n - 1], lamu, gep, nu2_of(n - 1), a.gpart_d, frow(n, 0), gwb);
     else
-      hipLaunchKernelGGL((k_adj_c<R, GEO, NP, false, DENSE, CM, LEAN>), g, b, 0, s, S, A, d, rec[n].e, nu1, part, lam, rec[n],
-                         lamu, gep, nu2, a.gpart_d, a.gb_d, a.ga_d, frow(n, 0), gwb);
+      hipLaunchKernelGGL((k_adj_c<R, GEO, NP, false, CM, LEAN>), g, b, 0, s, S, A, d, rec[n].e, nu1_of(n), part, lam,
+                         rec[n], lamu, gep, nu2_of(n), a.gpart_d, frow(n, 0), gwb);
   }
+  if (nuh)
+    hipLaunchKernelGGL((k_dense_seg<R, GEO, 4>), g, b, 0, s, S, A, seg, nuh, C, n_seg, a.gb_d, a.ga_d);
 }
 
-template <typename R, int NP, bool DENSE, int CM, bool LEAN>
-void sweep_geo(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
+template <typename R, int NP, int CM, bool LEAN>
+void sweep_geo(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s, R* nuh, const R* seg, int C) {
   const int geo = a.m->geo;
-  if (geo == GEO_OPEN) sweep_t<R, GEO_OPEN, NP, DENSE, CM, LEAN>(a, rec, n_seg, s);
-  else if (geo == GEO_PERIODIC) sweep_t<R, GEO_PERIODIC, NP, DENSE, CM, LEAN>(a, rec, n_seg, s);
-  else sweep_t<R, GEO_TILED, NP, DENSE, CM, LEAN>(a, rec, n_seg, s);
+  if (geo == GEO_OPEN) sweep_t<R, GEO_OPEN, NP, CM, LEAN>(a, rec, n_seg, s, nuh, seg, C);
+  else if (geo == GEO_PERIODIC) sweep_t<R, GEO_PERIODIC, NP, CM, LEAN>(a, rec, n_seg, s, nuh, seg, C);
+  else sweep_t<R, GEO_TILED, NP, CM, LEAN>(a, rec, n_seg, s, nuh, seg, C);
 }
 
-// The plain model (see LEAN above); the dense sweeps are compiled in the general form only.
+// The plain model (see LEAN above).
 bool lean_adjoint(const Model& m) {
   return m.integrator == MX_HEUN && !m.pinned_d && !m.has_c && !m.has_u && m.sot_dl == 0.0 && m.sot_fl == 0.0;
 }
 
-template <typename R, int NP, bool DENSE>
-void sweep_cm(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
+template <typename R, int NP>
+void sweep_cm(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s, R* nuh, const R* seg, int C) {
   const bool bond = a.m->coup == COUP_BOND;
-  if constexpr (!DENSE) {
-    if (lean_adjoint(*a.m)) {
-      if (bond) sweep_geo<R, NP, DENSE, COUP_BOND, true>(a, rec, n_seg, s);
-      else sweep_geo<R, NP, DENSE, COUP_TABLE9, true>(a, rec, n_seg, s);
-      return;
-    }
+  if (lean_adjoint(*a.m)) {
+    if (bond) sweep_geo<R, NP, COUP_BOND, true>(a, rec, n_seg, s, nuh, seg, C);
+    else sweep_geo<R, NP, COUP_TABLE9, true>(a, rec, n_seg, s, nuh, seg, C);
+    return;
   }
-  if (bond) sweep_geo<R, NP, DENSE, COUP_BOND, false>(a, rec, n_seg, s);
-  else sweep_geo<R, NP, DENSE, COUP_TABLE9, false>(a, rec, n_seg, s);
-}
-
-template <typename R, bool DENSE>
-void sweep_np(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s) {
-  if (a.P <= 4) sweep_cm<R, 4, DENSE>(a, rec, n_seg, s);
-  else if (a.P <= 8) sweep_cm<R, 8, DENSE>(a, rec, n_seg, s);
-  else sweep_cm<R, 16, DENSE>(a, rec, n_seg, s);
+  if (bond) sweep_geo<R, NP, COUP_BOND, false>(a, rec, n_seg, s, nuh, seg, C);
+  else sweep_geo<R, NP, COUP_TABLE9, false>(a, rec, n_seg, s, nuh, seg, C);
 }
 
 template <typename R>
@@ -889,14 +939,24 @@ void sweep(mx_adjoint& a, char* seg, size_t b, int C, int n_seg, bool dense, hip
     auto at = [&](int which) { return reinterpret_cast<const R*>(seg + (size_t(which) * C + n) * b); };
     rec[n] = StepRec<R>{at(0), at(1), at(2), at(3), at(4)};
   }
-  if (dense) sweep_np<R, true>(a, rec.data(), n_seg, s);
-  else sweep_np<R, false>(a, rec.data(), n_seg, s);
+  R* nuh = dense ? static_cast<R*>(a.nuh_d) : nullptr;
+  const R* sg = reinterpret_cast<const R*>(seg);
+  if (a.P <= 4) sweep_cm<R, 4>(a, rec.data(), n_seg, s, nuh, sg, C);
+  else if (a.P <= 8) sweep_cm<R, 8>(a, rec.data(), n_seg, s, nuh, sg, C);
+  else sweep_cm<R, 16>(a, rec.data(), n_seg, s, nuh, sg, C);
 }
 
-// Accumulators of the dense gradients (cleared) and, for a permuted layout, the cell map.
+// Accumulators of the dense gradients (cleared), the per-step nu1 / nu2 slots of a segment (2 x seg_cap
+// x N x 3) and, for a permuted layout, the cell map.
 void ensure_dense(mx_adjoint& a, hipStream_t s) {
   const Model& m = *a.m;
   const size_t nb = size_t(a.nfwd) * 9 * m.ncell, na = size_t(9) * m.N;
+  if (a.nuh_cap < a.seg_cap) {
+    afree(a.nuh_d);
+    a.nuh_cap = 0;
+    aok(hipMalloc(&a.nuh_d, 2 * size_t(a.seg_cap) * nbytes3(m)), "hipMalloc dense adjoint vectors");
+    a.nuh_cap = a.seg_cap;
+  }
   if (!a.gb_d) aok(hipMalloc(reinterpret_cast<void**>(&a.gb_d), sizeof(double) * std::max<size_t>(nb, 1)), "hipMalloc dense bond gradient");
   if (!a.ga_d) aok(hipMalloc(reinterpret_cast<void**>(&a.ga_d), sizeof(double) * na), "hipMalloc dense on-site gradient");
   if (nb) hipLaunchKernelGGL((k_zero<double>), dim3((nb + 255) / 256), dim3(256), 0, s, a.gb_d, nb);
@@ -1047,7 +1107,7 @@ void mx_adjoint_destroy(mx_adjoint* a) {
   for (void** p : {&a->astatic_d, &a->ckpt_d, &a->seg_d, &a->lam_d, &a->lamu_d, &a->gep_d, &a->nu1_d, &a->nu2_d,
                    &a->part_d, &a->ework_d, reinterpret_cast<void**>(&a->gpart_d),
                    reinterpret_cast<void**>(&a->gout_d), reinterpret_cast<void**>(&a->gb_d),
-                   reinterpret_cast<void**>(&a->ga_d), reinterpret_cast<void**>(&a->cmap_d),
+                   reinterpret_cast<void**>(&a->ga_d), &a->nuh_d, reinterpret_cast<void**>(&a->cmap_d),
                    reinterpret_cast<void**>(&a->gf_d), &a->gw_d})
     afree(*p);
   delete a;
