@@ -85,12 +85,12 @@ __device__ __forceinline__ const R* rep_at(const R* a, const Rep* __restrict__ r
 // address needs no multiply per lane: with the offset d reduced into [0, L) per axis (scalar), its byte
 // offset is the site's own plus a wavefront-uniform delta, less one period per axis that wraps
 // (c_k >= L_k - d_k).  Offsets are 32-bit from the replica's base (a replica has fewer than 2^24 cells
-// of at most 8 sites of 24 B).
-template <typename R, int CM, typename Mid>
+// of at most 8 sites of 24 B).  CH (6 or 8) is the batch width the host picks for the stencil; a full
+// batch skips the selects (only a last, partial batch pays for them).
+template <typename R, int CM, int CH, typename Mid>
 __device__ __forceinline__ void bexchange(const BStatic<R>* __restrict__ S, const Rep* __restrict__ rp,
                                           const REnt* __restrict__ re, const R* __restrict__ e, const BSite& st,
                                           R& hx, R& hy, R& hz, Mid&& mid) {
-  constexpr int CH = 8;
   hx = hy = hz = R(0);
   const int kb = S->sptr[st.s], ke = S->sptr[st.s + 1];
   const uint32_t Q0 = rp->Q0, Q1 = rp->Q1, Q2 = rp->Q2;
@@ -119,9 +119,15 @@ __device__ __forceinline__ void bexchange(const BStatic<R>* __restrict__ S, cons
       ez[u] = p[2];
     }
     if (k0 == kb) mid();
+    const bool full = k0 + CH <= ke;  // wavefront-uniform
 #pragma unroll
     for (int u = 0; u < CH; ++u) {
-      const R x = live[u] ? ex[u] : R(0), y = live[u] ? ey[u] : R(0), z = live[u] ? ez[u] : R(0);
+      R x = ex[u], y = ey[u], z = ez[u];
+      if (!full) {
+        x = live[u] ? x : R(0);
+        y = live[u] ? y : R(0);
+        z = live[u] ? z : R(0);
+      }
       if constexpr (CM == COUP_TABLE1) {
         const R Jv = S->tab[slot[u]];
         hx += Jv * x;
@@ -142,11 +148,11 @@ __device__ __forceinline__ void bexchange(const BStatic<R>* __restrict__ S, cons
   }
 }
 
-template <typename R, int CM>
+template <typename R, int CM, int CH>
 __device__ __forceinline__ void bexchange(const BStatic<R>* __restrict__ S, const Rep* __restrict__ rp,
                                           const REnt* __restrict__ re, const R* __restrict__ e, const BSite& st,
                                           R& hx, R& hy, R& hz) {
-  bexchange<R, CM>(S, rp, re, e, st, hx, hy, hz, [] {});
+  bexchange<R, CM, CH>(S, rp, re, e, st, hx, hy, hz, [] {});
 }
 
 template <typename R>
@@ -180,7 +186,7 @@ __device__ __forceinline__ bool bsite(const BStatic<R>* __restrict__ S, const Re
   return true;
 }
 
-template <typename R, int CM, int BS, int PASS>
+template <typename R, int CM, int BS, int PASS, int CH>
 __global__ __launch_bounds__(BS) void k_bheun(const BStatic<R>* __restrict__ S, const Rep* __restrict__ reps,
                                               const int32_t* __restrict__ tile_rep, const REnt* __restrict__ rent,
                                               int nent, const BDyn<R> d, R* __restrict__ e, R* __restrict__ ep,
@@ -210,7 +216,7 @@ __global__ __launch_bounds__(BS) void k_bheun(const BStatic<R>* __restrict__ S, 
       ny = bnp[1];
       nz = bnp[2];
     }
-    bexchange<R, CM>(S, rp, re, src, st, hx, hy, hz, [&] {
+    bexchange<R, CM, CH>(S, rp, re, src, st, hx, hy, hz, [&] {
       if (PASS == 1 && d.noisy) {
         const R sg = R(rp->sig[st.s]);
         if (d.noise_dbl) {
@@ -281,7 +287,7 @@ __global__ __launch_bounds__(BS) void k_bobs(const BStatic<R>* __restrict__ S, c
     const int i = st.i;
     const R x = e[3 * i], y = e[3 * i + 1], z = e[3 * i + 2];
     R hx, hy, hz, gx, gy, gz;
-    bexchange<R, CM>(S, rp, re, e, st, hx, hy, hz);
+    bexchange<R, CM, 8>(S, rp, re, e, st, hx, hy, hz);
     bonsite(S, st.s, x, y, z, gx, gy, gz);
     const R im = S->inv_mu_sub[st.s];
     const R Eon = (gx * x + gy * y + gz * z) * R(0.5);
@@ -345,6 +351,10 @@ struct mx_batch {
   void* bn_d = nullptr;  // thermal field of the current step, drawn in pass 1 and reused in pass 2
   void* rent_d = nullptr;  // REnt per (replica, stencil entry)
   int nent = 0;
+  // gather batch width of the step kernels: 6 where that takes no more batches than 8 for the longest
+  // sublattice stencil (the 6-entry triangular and simple-cubic neighbours: one full batch, no dead
+  // entries), else 8.  Same summation order either way, so the same result bits.
+  int chw = 8;
   int32_t* tile_rep_d = nullptr;
   double* part_d = nullptr;
   size_t part_cap = 0;
@@ -414,6 +424,9 @@ void build_static(mx_batch& b, const Model& m) {
 void build_rent(mx_batch& b, const Model& m) {
   const uint32_t W = uint32_t(3 * rsz(b));
   b.nent = m.nent;
+  int mx = 0;
+  for (int s = 0; s < m.ns; ++s) mx = std::max(mx, m.sptr[s + 1] - m.sptr[s]);
+  b.chw = (mx + 5) / 6 <= (mx + 7) / 8 ? 6 : 8;
   std::vector<REnt> t(size_t(b.nrep) * std::max(m.nent, 1));
   for (int r = 0; r < b.nrep; ++r) {
     Rep& p = b.reps[r];
@@ -479,10 +492,17 @@ void launch_pair(mx_batch& b, R* e, uint64_t step, const uint64_t* step_ptr, hip
   R* k1 = static_cast<R*>(b.k1_d);
   R* bn = static_cast<R*>(b.bn_d);
   const REnt* rent = static_cast<const REnt*>(b.rent_d);
-  hipLaunchKernelGGL((k_bheun<R, CM, BS, 1>), dim3(b.ntiles), dim3(BS), 0, s, S, reps, b.tile_rep_d, rent, b.nent, d,
-                     e, ep, k1, bn, step, b.spt, step_ptr);
-  hipLaunchKernelGGL((k_bheun<R, CM, BS, 2>), dim3(b.ntiles), dim3(BS), 0, s, S, reps, b.tile_rep_d, rent, b.nent, d,
-                     e, ep, k1, bn, step, b.spt, step_ptr);
+  if (b.chw == 6) {
+    hipLaunchKernelGGL((k_bheun<R, CM, BS, 1, 6>), dim3(b.ntiles), dim3(BS), 0, s, S, reps, b.tile_rep_d, rent, b.nent,
+                       d, e, ep, k1, bn, step, b.spt, step_ptr);
+    hipLaunchKernelGGL((k_bheun<R, CM, BS, 2, 6>), dim3(b.ntiles), dim3(BS), 0, s, S, reps, b.tile_rep_d, rent, b.nent,
+                       d, e, ep, k1, bn, step, b.spt, step_ptr);
+  } else {
+    hipLaunchKernelGGL((k_bheun<R, CM, BS, 1, 8>), dim3(b.ntiles), dim3(BS), 0, s, S, reps, b.tile_rep_d, rent, b.nent,
+                       d, e, ep, k1, bn, step, b.spt, step_ptr);
+    hipLaunchKernelGGL((k_bheun<R, CM, BS, 2, 8>), dim3(b.ntiles), dim3(BS), 0, s, S, reps, b.tile_rep_d, rent, b.nent,
+                       d, e, ep, k1, bn, step, b.spt, step_ptr);
+  }
 }
 
 template <typename R, int CM>
