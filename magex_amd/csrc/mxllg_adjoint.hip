@@ -825,6 +825,16 @@ void record_segment(mx_adjoint& a, const void* src, int64_t k, int n_seg, hipStr
   Model& m = *a.m;
   void* ew = a.ework_d;
   to_internal(a, src, ew, s);
+  // undriven segments of two or more steps on a model that takes persistent launches: one launch writes
+  // the same records, bit for bit, as the two single-pass kernels per step below
+  if (a.drive.empty() && n_seg >= 2 && persistent_steps(m) && ensure_step_scratch(m) == MX_OK) {
+    const StepRecOut ro{seg_slot(a, 0, 0), seg_slot(a, 1, 0), seg_slot(a, 2, 0), seg_slot(a, 3, 0),
+                        seg_slot(a, 4, 0)};
+    aok_rc(launch_heun_steps(m, ew, a.seed, a.step0 + uint64_t(k * a.C), n_seg, nullptr, s, nullptr, &ro),
+           "recorded segment");
+    a.seg_k = k;
+    return;
+  }
   for (int n = 0; n < n_seg; ++n) {
     const uint64_t step = a.step0 + uint64_t(k * a.C + n);
     const double* B = a.drive.empty() ? nullptr : a.drive.data() + 6 * size_t(k * a.C + n);

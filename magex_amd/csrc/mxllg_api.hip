@@ -173,6 +173,13 @@ static void ensure_scratch(Model& m) {
   }
 }
 
+int ensure_step_scratch(Model& m) {
+  return guarded([&]() -> int {
+    ensure_scratch(m);
+    return MX_OK;
+  });
+}
+
 static void use_device(const Model& m) { hip_ok(hipSetDevice(m.device), "hipSetDevice"); }
 
 int sync_model_static(Model& m) { return m.static_dirty ? upload_static(m) : MX_OK; }

@@ -216,9 +216,19 @@ bool walk_step(const Model& m, int* nb = nullptr, int* grid = nullptr);
 bool persistent_steps(const Model& m);
 // nsteps steps on e in persistent launches of up to STEPS_PER_LAUNCH steps (persistent_steps(m) must hold);
 // noise: nsteps x N x 3 normals or null; bufs = null: the model's own scratch and records.
+// rec (optional): records of every step n, internal order, N x 3 each at n * N * 3 values from each base --
+// the step's input spins e, its predictor spins ep, w = e + dt/2 k1 (Depondt: the rotation vector), and the
+// effective fields of both passes (zero on pinned sites, whose w is their spin): what the adjoint's replays
+// store with the single-pass kernels.
+struct StepRecOut {
+  void *e, *ep, *w, *b1, *b2;
+};
 constexpr int STEPS_PER_LAUNCH = 1024;
 int launch_heun_steps(const Model& m, void* e, uint64_t seed, uint64_t step0, int64_t nsteps, const void* noise,
-                      hipStream_t s, const StepBufs* bufs = nullptr);
+                      hipStream_t s, const StepBufs* bufs = nullptr, const StepRecOut* rec = nullptr);
+// The model's own step scratch (predictor buffers, control words, tagged records), allocated if missing
+// (MX_OK or an error code with the message set).
+int ensure_step_scratch(Model& m);
 // Device words of the one-launch steps (allocated and zeroed with the model's scratch: plane counters,
 // control words, give-up slots, the walked step's ticket counters), and the offset of the control words
 // among them.

@@ -47,12 +47,13 @@ __device__ __forceinline__ bool wait_planes(const Static<R>* __restrict__ S, con
 
 // State a workgroup of the one-launch kernels keeps per site from the predictor to the corrector: the
 // coupling batches (per-bond tensors when NB > 0), the per-site on-site tensor, damping, the spin e, the
-// predictor spin ep, w = e + dt/2 k1 (Heun) or the rotation vector (Depondt), and the thermal field.
+// predictor spin ep, w = e + dt/2 k1 (Heun) or the rotation vector (Depondt), the thermal field, and the
+// effective field of the last pass (kept for recorded runs).
 template <typename R, int CM, int GEO, int NB>
 struct Held {
   Batch<R, CM, GEO> bt[NB > 0 ? NB : 1];
   SiteA<R> A;
-  R a, x, y, z, px, py, pz, wx, wy, wz, nx, ny, nz;
+  R a, x, y, z, px, py, pz, wx, wy, wz, nx, ny, nz, bx, by, bz;
 };
 
 // Where the one-launch kernels take the neighbours' spins from: a spin array read with load policy LD
@@ -135,6 +136,9 @@ __device__ __forceinline__ bool predictor_held(const Static<R>* __restrict__ S, 
     onsite_grad<R, LEAN>(S, d, st, h.A, x, y, z, gx, gy, gz);
   }
   field_from<R, LEAN>(S, d, st, x, y, z, hx, hy, hz, gx, gy, gz, h.nx, h.ny, h.nz, d.B0, d.B1, d.B2, bx, by, bz);
+  h.bx = bx;
+  h.by = by;
+  h.bz = bz;
   llg_rhs(x, y, z, bx, by, bz, h.a, d.prec, fx, fy, fz);
   if (!LEAN && d.integ == MX_DEPONDT) {
     omega(x, y, z, fx, fy, fz, h.wx, h.wy, h.wz);
@@ -194,6 +198,9 @@ __device__ __forceinline__ bool corrector_held(const Static<R>* __restrict__ S, 
   if constexpr (!(Src::tagged && NB == 0)) onsite_grad<R, LEAN>(S, d, st, h.A, px, py, pz, gx, gy, gz);
   field_from<R, LEAN>(S, d, st, px, py, pz, hx, hy, hz, gx, gy, gz, h.nx, h.ny, h.nz, d.Bc0, d.Bc1, d.Bc2, bx, by,
                       bz);
+  h.bx = bx;
+  h.by = by;
+  h.bz = bz;
   llg_rhs(px, py, pz, bx, by, bz, h.a, d.prec, fx, fy, fz);
   const R hdt = R(0.5) * d.dt;
   R cx, cy, cz;

@@ -29,10 +29,20 @@ namespace mx {
 // Every poll is bounded: a lane that gives up raises the launch's abort word, which ends every other
 // lane's polls; the launch's last workgroup counts the aborted launch and clears the control words.
 // Same arithmetic as k_heun: same result bits.
+//
+// out (null bases: none) receives every step's record as the single-pass kernels store it in a recorded
+// replay (k_heun with save_e / save_B): the step's input spin, its predictor spin, w (the rotation vector
+// under Depondt), and the effective field of each pass, with a pinned site's fields zero and its w its spin.
+template <typename R>
+struct RecPtrs {
+  R *e, *ep, *w, *b1, *b2;
+};
+
 template <typename R, int CM, int GEO, bool LEAN>
 __global__ __launch_bounds__(BLOCK) void k_steps(const Static<R>* __restrict__ S, const Dyn<R> d, R* __restrict__ e,
                                                  uint64_t* __restrict__ rec, uint64_t step0, int nsteps,
-                                                 const R* __restrict__ noise, uint32_t tag0, const StepSync sy) {
+                                                 const R* __restrict__ noise, uint32_t tag0, const StepSync sy,
+                                                 const RecPtrs<R> out) {
   __shared__ int sh_flag;
   constexpr int NW = TAG_WORDS<R>;
   Site st;
@@ -66,6 +76,21 @@ __global__ __launch_bounds__(BLOCK) void k_steps(const Static<R>* __restrict__ S
         h.py = y0;
         h.pz = z0;
       }
+      const size_t ro = size_t(n) * nstride + 3 * size_t(i);
+      if (out.e) {  // uniform
+        out.e[ro] = x0;
+        out.e[ro + 1] = y0;
+        out.e[ro + 2] = z0;
+        out.ep[ro] = h.px;
+        out.ep[ro + 1] = h.py;
+        out.ep[ro + 2] = h.pz;
+        out.w[ro] = pin ? x0 : h.wx;
+        out.w[ro + 1] = pin ? y0 : h.wy;
+        out.w[ro + 2] = pin ? z0 : h.wz;
+        out.b1[ro] = pin ? R(0) : h.bx;
+        out.b1[ro + 1] = pin ? R(0) : h.by;
+        out.b1[ro + 2] = pin ? R(0) : h.bz;
+      }
       publish_tagged<R>(recP + size_t(i) * NW, h.px, h.py, h.pz, k + 1);
       const TaggedFrom fromP{recP, k + 1, sy.spin, abort};
       const bool g2 = corrector_held<R, CM, GEO, 0, LEAN>(S, d, fromP, st, kb, ke, h);
@@ -74,6 +99,11 @@ __global__ __launch_bounds__(BLOCK) void k_steps(const Static<R>* __restrict__ S
         h.x = x0;
         h.y = y0;
         h.z = z0;
+      }
+      if (out.e) {
+        out.b2[ro] = pin ? R(0) : h.bx;
+        out.b2[ro + 1] = pin ? R(0) : h.by;
+        out.b2[ro + 2] = pin ? R(0) : h.bz;
       }
       publish_tagged<R>(recE + size_t(i) * NW, h.x, h.y, h.z, k + 2);
       if (!g) {
@@ -131,29 +161,29 @@ bool persistent_steps(const Model& m) {
 
 template <typename R, int CM, int GEO, bool LEAN>
 static hipError_t steps_coop(dim3 g, hipStream_t s, const Static<R>* S, Dyn<R> d, R* E, uint64_t* rec,
-                             uint64_t step0, int n, const R* nz, uint32_t tag0, StepSync sy) {
-  void* args[] = {&S, &d, &E, &rec, &step0, &n, &nz, &tag0, &sy};
+                             uint64_t step0, int n, const R* nz, uint32_t tag0, StepSync sy, RecPtrs<R> ro) {
+  void* args[] = {&S, &d, &E, &rec, &step0, &n, &nz, &tag0, &sy, &ro};
   return hipLaunchCooperativeKernel(k_steps<R, CM, GEO, LEAN>, g, dim3(BLOCK), args, 0, s);
 }
 
 template <typename R, int GEO, bool LEAN>
 static hipError_t steps_dispatch(const Model& m, dim3 g, hipStream_t s, const Static<R>* S, const Dyn<R>& d, R* E,
                                  uint64_t* rec, uint64_t step0, int n, const R* nz, uint32_t tag0,
-                                 const StepSync& sy) {
+                                 const StepSync& sy, const RecPtrs<R>& ro) {
   switch (m.coup) {
-    case COUP_TABLE1: return steps_coop<R, COUP_TABLE1, GEO, LEAN>(g, s, S, d, E, rec, step0, n, nz, tag0, sy);
-    case COUP_TABLE4: return steps_coop<R, COUP_TABLE4, GEO, LEAN>(g, s, S, d, E, rec, step0, n, nz, tag0, sy);
-    case COUP_TABLE9: return steps_coop<R, COUP_TABLE9, GEO, LEAN>(g, s, S, d, E, rec, step0, n, nz, tag0, sy);
-    default: return steps_coop<R, COUP_BOND, GEO, LEAN>(g, s, S, d, E, rec, step0, n, nz, tag0, sy);
+    case COUP_TABLE1: return steps_coop<R, COUP_TABLE1, GEO, LEAN>(g, s, S, d, E, rec, step0, n, nz, tag0, sy, ro);
+    case COUP_TABLE4: return steps_coop<R, COUP_TABLE4, GEO, LEAN>(g, s, S, d, E, rec, step0, n, nz, tag0, sy, ro);
+    case COUP_TABLE9: return steps_coop<R, COUP_TABLE9, GEO, LEAN>(g, s, S, d, E, rec, step0, n, nz, tag0, sy, ro);
+    default: return steps_coop<R, COUP_BOND, GEO, LEAN>(g, s, S, d, E, rec, step0, n, nz, tag0, sy, ro);
   }
 }
 
 template <typename R>
 static int steps_t(const Model& m, void* e, uint64_t seed, uint64_t step0, int64_t nsteps, const void* noise,
-                   hipStream_t s, const StepBufs* bufs) {
+                   hipStream_t s, const StepBufs* bufs, const StepRecOut* rec) {
   uint32_t* sync = bufs ? bufs->sync : m.stepsync_d;
-  uint64_t* rec = bufs ? bufs->rec : m.rec_d;
-  if (!sync || !rec || !persistent_steps(m)) {
+  uint64_t* recs = bufs ? bufs->rec : m.rec_d;
+  if (!sync || !recs || !persistent_steps(m)) {
     set_error("persistent steps: the model does not take persistent launches");
     return MX_EINVAL;
   }
@@ -175,6 +205,13 @@ static int steps_t(const Model& m, void* e, uint64_t seed, uint64_t step0, int64
     const int k = int(std::min<int64_t>(STEPS_PER_LAUNCH, nsteps - n));
     const R* nz = noise ? static_cast<const R*>(noise) + size_t(n) * stride : nullptr;
     const uint64_t st = step0 + uint64_t(n);
+    // this launch's records start at its first step
+    RecPtrs<R> ro{nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (rec) {
+      const size_t off = size_t(n) * stride;
+      ro = RecPtrs<R>{static_cast<R*>(rec->e) + off, static_cast<R*>(rec->ep) + off, static_cast<R*>(rec->w) + off,
+                      static_cast<R*>(rec->b1) + off, static_cast<R*>(rec->b2) + off};
+    }
     // 16-bit tags: the launch's epoch (1 .. 15; 0 would match cleared records) in the high 4 bits, the
     // phase within the launch (up to 2 x STEPS_PER_LAUNCH + 2) in the low 12.  A model's records are
     // written only by its own launches, and consecutive launches take different epochs.
@@ -182,11 +219,11 @@ static int steps_t(const Model& m, void* e, uint64_t seed, uint64_t step0, int64
     const uint32_t tag0 = m.rec_epoch << 12;
     hipError_t err;
     if (m.geo == GEO_PERIODIC)
-      err = lean ? steps_dispatch<R, GEO_PERIODIC, true>(m, g, s, S, d, E, rec, st, k, nz, tag0, sy)
-                 : steps_dispatch<R, GEO_PERIODIC, false>(m, g, s, S, d, E, rec, st, k, nz, tag0, sy);
+      err = lean ? steps_dispatch<R, GEO_PERIODIC, true>(m, g, s, S, d, E, recs, st, k, nz, tag0, sy, ro)
+                 : steps_dispatch<R, GEO_PERIODIC, false>(m, g, s, S, d, E, recs, st, k, nz, tag0, sy, ro);
     else
-      err = lean ? steps_dispatch<R, GEO_OPEN, true>(m, g, s, S, d, E, rec, st, k, nz, tag0, sy)
-                 : steps_dispatch<R, GEO_OPEN, false>(m, g, s, S, d, E, rec, st, k, nz, tag0, sy);
+      err = lean ? steps_dispatch<R, GEO_OPEN, true>(m, g, s, S, d, E, recs, st, k, nz, tag0, sy, ro)
+                 : steps_dispatch<R, GEO_OPEN, false>(m, g, s, S, d, E, recs, st, k, nz, tag0, sy, ro);
     if (err != hipSuccess) {
       set_error(std::string("persistent heun steps: ") + hipGetErrorString(err));
       return MX_EHIP;
@@ -196,9 +233,9 @@ static int steps_t(const Model& m, void* e, uint64_t seed, uint64_t step0, int64
 }
 
 int launch_heun_steps(const Model& m, void* e, uint64_t seed, uint64_t step0, int64_t nsteps, const void* noise,
-                      hipStream_t s, const StepBufs* bufs) {
-  return m.prec == MX_FP64 ? steps_t<double>(m, e, seed, step0, nsteps, noise, s, bufs)
-                           : steps_t<float>(m, e, seed, step0, nsteps, noise, s, bufs);
+                      hipStream_t s, const StepBufs* bufs, const StepRecOut* rec) {
+  return m.prec == MX_FP64 ? steps_t<double>(m, e, seed, step0, nsteps, noise, s, bufs, rec)
+                           : steps_t<float>(m, e, seed, step0, nsteps, noise, s, bufs, rec);
 }
 
 }  // namespace mx

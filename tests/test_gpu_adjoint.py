@@ -163,6 +163,40 @@ def test_adjoint_checkpoint_interval_is_exact(gpu):
     assert torch.equal(g0, _mse_grad(e, tgt))
 
 
+@pytest.mark.parametrize("integrator", ["heun", "depondt"])
+def test_recorded_persistent_replays_equal_single_pass_replays(gpu, integrator):
+    """Segments of two or more undriven steps are recorded by the persistent launch (every record in one
+    launch); one-step segments by the single-pass kernels.  Both must store the same records bit for bit,
+    so the gradients with checkpoints every step and every 16 steps are equal -- on the general model:
+    pinned sites, per-site damping, T = 50 K, the dense gradients too."""
+    print(f"case: recorded persistent vs single-pass replays, square 16^2, {integrator}", flush=True)
+    from magex_amd.adjoint import DifferentiableLLG
+    L, steps, T, seed = 16, 40, 50.0, 9
+    dev0, ref0 = _builders(L)
+    mask = np.zeros(L * L, bool)
+    mask[[3, 40, 77, 200]] = True
+    alpha = 0.2 + 0.2 * np.random.default_rng(3).random(L * L)
+
+    def dev(th):
+        s = dev0(th)
+        s.pinned = mask
+        return s
+
+    out = []
+    for C in (1, 16):
+        sol = DifferentiableLLG(dev, THETA0, alpha=alpha, dt=1e-3, temperature=T, seed=seed, precision="fp64",
+                                checkpoint_every=C, integrator=integrator)
+        o = ref0(THETA0)
+        e0 = o.init_random(21)
+        tgt = torch.as_tensor(o.init_random(22)).cuda()
+        e = sol.forward(THETA0, e0, steps)
+        gt, g0, gb, ga = sol.backward_dense(_mse_grad(e, tgt))
+        torch.cuda.synchronize()
+        out.append([x.cpu().numpy() for x in (e, gt, g0, gb, ga)])
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b), float(np.abs(a - b).max())
+
+
 def test_adjoint_forward_is_the_plain_trajectory(gpu):
     """mx_adjoint_forward at theta is the plain solver on the builder's system at theta."""
     print("case: adjoint forward vs LLGSolver, square 32^2", flush=True)
