@@ -25,7 +25,7 @@
 #include <tuple>
 #include <vector>
 
-#include "mxllg_field.h"
+#include "mxllg_step.h"
 
 namespace mx {
 
@@ -168,29 +168,41 @@ __device__ __forceinline__ void onsite_T(const Static<R>* __restrict__ S, const 
   }
 }
 
+// Where the adjoint gathers take the neighbours' nu from: an array (NuArray), or tagged records polled
+// until they carry the tag `want` (TaggedFrom, as the persistent forward's spins).
+template <typename R>
+struct NuArray {
+  static constexpr bool tagged = false;
+  const R* nu;
+};
+
 // Adjoint gather at site st: H = sum_k J_k nu_j (J from the model's width-9 table, or with CM ==
 // COUP_BOND its per-bond tensors: a forward entry's own, a reverse entry's partner's transposed), and for every
 // parameter t_p = sum_k U[k][p] x_j; then G_p = nu_i . (t_p - Aun[s][p] x_i) is accumulated into gp.
 // NP (4, 8 or 16) is the parameter count rounded up; padded parameters have zero units.
 // Entries go in batches of CH whose loads are all issued before the first is used (entries past the
-// end re-read the last one with weight zero), so a wavefront keeps 2 CH gathers in flight.
-template <typename R, int GEO, int NP, int CM>
-__device__ __forceinline__ void adj_gather(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
-                                           const Dyn<R>& d, const R* __restrict__ nu, const R* __restrict__ x,
+// end re-read the last one with weight zero), so a wavefront keeps 2 CH gathers in flight.  From tagged
+// records, each batch's live entries are re-read until every one carries its tag (x and the couplings
+// are in flight meanwhile); false when a lane gives up (after src.spin polls, or once *src.abort is raised).
+template <typename R, int GEO, int NP, int CM, typename Src>
+__device__ __forceinline__ bool adj_gather(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
+                                           const Dyn<R>& d, const Src& src, const R* __restrict__ x,
                                            const Site& st,
                                            R nix, R niy, R niz, R xix, R xiy, R xiz, R& hx, R& hy, R& hz,
                                            double (&gp)[NP]) {
   constexpr int CH = 8;
   const int ncell = S->ncell;
+  bool good = true;
   R t[NP][3];
 #pragma unroll
   for (int p = 0; p < NP; ++p) t[p][0] = t[p][1] = t[p][2] = R(0);
   hx = hy = hz = R(0);
   const int kb = S->sptr[st.s], ke = S->sptr[st.s + 1];
   for (int k0 = kb; k0 < ke; k0 += CH) {
-    R vx[CH], vy[CH], vz[CH], qx[CH], qy[CH], qz[CH];
+    R vx[CH], vy[CH], vz[CH], qx[CH], qy[CH], qz[CH], wt[CH];
     R Jb[CM == COUP_BOND ? CH : 1][9];
-    int kk[CH];
+    int kk[CH], jj[CH];
+    uint32_t pend = 0;
 #pragma unroll
     for (int u = 0; u < CH; ++u) {
       const bool live = k0 + u < ke;
@@ -200,9 +212,15 @@ __device__ __forceinline__ void adj_gather(const Static<R>* __restrict__ S, cons
       const int cn = neighbour_cell<R, GEO>(S, st, en, ok);
       ok = ok && live;
       const int j = (en.tb & 0xffff) * ncell + cn;
-      vx[u] = nu[3 * j];
-      vy[u] = nu[3 * j + 1];
-      vz[u] = nu[3 * j + 2];
+      jj[u] = j;
+      if constexpr (Src::tagged) {
+        vx[u] = vy[u] = vz[u] = R(0);
+        if (ok) pend |= 1u << u;
+      } else {
+        vx[u] = src.nu[3 * j];
+        vy[u] = src.nu[3 * j + 1];
+        vz[u] = src.nu[3 * j + 2];
+      }
       qx[u] = x[3 * j];
       qy[u] = x[3 * j + 1];
       qz[u] = x[3 * j + 2];
@@ -215,13 +233,54 @@ __device__ __forceinline__ void adj_gather(const Static<R>* __restrict__ S, cons
 #pragma unroll
         for (int q = 0; q < 9; ++q) Jb[u][q] = back ? tt[(q % 3) * 3 + q / 3] : tt[q];
       }
-      const R w = ok ? R(1) : R(0);
-      vx[u] *= w;
-      vy[u] *= w;
-      vz[u] *= w;
-      qx[u] *= w;
-      qy[u] *= w;
-      qz[u] *= w;
+      wt[u] = ok ? R(1) : R(0);
+    }
+    if constexpr (Src::tagged) {
+      constexpr int NW = TAG_WORDS<R>;
+      for (uint32_t it = 0;; ++it) {
+        uint64_t w[CH][NW];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+          if (k0 + u < ke) {  // wavefront-uniform
+            const uint64_t* r = src.rec + size_t(jj[u]) * NW;
+#pragma unroll
+            for (int q = 0; q < NW; ++q)
+              w[u][q] = __hip_atomic_load(const_cast<uint64_t*>(r + q), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+          if (k0 + u < ke) {
+            bool g = true;
+#pragma unroll
+            for (int q = 0; q < NW; ++q) g = g && uint32_t(w[u][q] >> 48) == src.want;
+            R a, b, c;
+            untag3<R>(w[u], a, b, c);
+            const bool take = g && (pend >> u & 1u);
+            vx[u] = take ? a : vx[u];
+            vy[u] = take ? b : vy[u];
+            vz[u] = take ? c : vz[u];
+            pend &= g ? ~(1u << u) : ~0u;
+          }
+        }
+        if (!pend) break;
+        if (it >= src.spin ||
+            ((it & 15) == 15 &&
+             __hip_atomic_load(const_cast<uint32_t*>(src.abort), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+          good = false;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      vx[u] *= wt[u];
+      vy[u] *= wt[u];
+      vz[u] *= wt[u];
+      qx[u] *= wt[u];
+      qy[u] *= wt[u];
+      qz[u] *= wt[u];
     }
 #pragma unroll
     for (int u = 0; u < CH; ++u) {
@@ -248,6 +307,7 @@ __device__ __forceinline__ void adj_gather(const Static<R>* __restrict__ S, cons
     const R uz = t[p][2] - (a[6] * xix + a[7] * xiy + a[8] * xiz);
     gp[p] += double(nix * ux + niy * uy + niz * uz);
   }
+  return good;
 }
 
 // Saved state of one step: e, ep, w = e + dt/2 k1, B(e), B(ep), each N x 3 in internal site order.
@@ -256,27 +316,28 @@ struct StepRec {
   const R *e, *ep, *w, *B1, *B2;
 };
 
-// Step-local part (corrector) at site i: from lambda = dL/de' and the step's record: lambda_u, the
-// cotangent of ep through k2 (gep) and nu2.  A pinned spin is carried unchanged (e' = ep = e), so
-// lambda passes through and no field acts on it (gep = nu2 = 0).
+// Step-local part (corrector) at site i: from lambda = dL/de' and the step's record: lambda_u (lu), the
+// cotangent of ep through k2 (ge), nu2 (n2) and, under Depondt, the corrector's share of dL/dw1 (gw).
+// A pinned spin is carried unchanged (e' = ep = e), so lambda passes through and no field acts on it
+// (ge = n2 = gw = 0).
 template <typename R, bool LEAN = false>
-__device__ __forceinline__ void adj_local(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st,
-                                          const StepRec<R>& r, R lx, R ly, R lz, R* __restrict__ lamu,
-                                          R* __restrict__ gep, R* __restrict__ nu2, R* __restrict__ gwb) {
+__device__ __forceinline__ void adj_local_v(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st,
+                                            const StepRec<R>& r, R lx, R ly, R lz, R (&lu)[3], R (&ge)[3],
+                                            R (&n2)[3], R (&gw)[3]) {
   const int i = st.i;
   if (!LEAN && d.pinned && d.pinned[i]) {
-    lamu[3 * i] = lx;
-    lamu[3 * i + 1] = ly;
-    lamu[3 * i + 2] = lz;
-    gep[3 * i] = gep[3 * i + 1] = gep[3 * i + 2] = R(0);
-    nu2[3 * i] = nu2[3 * i + 1] = nu2[3 * i + 2] = R(0);
-    if (d.integ == MX_DEPONDT) gwb[3 * i] = gwb[3 * i + 1] = gwb[3 * i + 2] = R(0);
+    lu[0] = lx;
+    lu[1] = ly;
+    lu[2] = lz;
+    ge[0] = ge[1] = ge[2] = R(0);
+    n2[0] = n2[1] = n2[2] = R(0);
+    gw[0] = gw[1] = gw[2] = R(0);
     return;
   }
   if (!LEAN && d.integ == MX_DEPONDT) {
     // e' = N(Rot(v) e), v = dt/2 (w1 + w2), w2 = ep x f(ep, B2); the record's k1 slot holds w1.
-    // Writes lamu = Rot(v)^T dL/du (the direct part of dL/de), gwb = dL/dw1 from this stage
-    // (= dL/dw2 = dt/2 dL/dv), gep = dL/dep through w2 and nu2.
+    // lu = Rot(v)^T dL/du (the direct part of dL/de), gw = dL/dw1 from this stage (= dL/dw2 =
+    // dt/2 dL/dv), ge = dL/dep through w2, and n2.
     const R a = d.alpha ? d.alpha[i] : S->alpha_sub[st.s];
     const R ex = r.e[3 * i], ey = r.e[3 * i + 1], ez = r.e[3 * i + 2];
     const R px = r.ep[3 * i], py = r.ep[3 * i + 1], pz = r.ep[3 * i + 2];
@@ -292,26 +353,26 @@ __device__ __forceinline__ void adj_local(const Static<R>* __restrict__ S, const
     unproject(ux / n, uy / n, uz / n, n, lx, ly, lz);  // dL/du
     R tx = lx, ty = ly, tz = lz;
     rotate(tx, ty, tz, -vx, -vy, -vz);
-    lamu[3 * i] = tx;
-    lamu[3 * i + 1] = ty;
-    lamu[3 * i + 2] = tz;
+    lu[0] = tx;
+    lu[1] = ty;
+    lu[2] = tz;
     R gvx, gvy, gvz;
     rotate_vjp_v(vx, vy, vz, ex, ey, ez, lx, ly, lz, gvx, gvy, gvz);
     const R gwx = hdt * gvx, gwy = hdt * gvy, gwz = hdt * gvz;
-    gwb[3 * i] = gwx;
-    gwb[3 * i + 1] = gwy;
-    gwb[3 * i + 2] = gwz;
+    gw[0] = gwx;
+    gw[1] = gwy;
+    gw[2] = gwz;
     // w2 = ep x f2: dL/dep += f2 x gw, dL/df2 = gw x ep
     const R lfx = gwy * pz - gwz * py, lfy = gwz * px - gwx * pz, lfz = gwx * py - gwy * px;
     R gx, gy, gz, hx, hy, hz;
     rhs_vjp(px, py, pz, bx, by, bz, a, d.prec, lfx, lfy, lfz, gx, gy, gz, hx, hy, hz);
-    gep[3 * i] = (fy * gwz - fz * gwy) + gx;
-    gep[3 * i + 1] = (fz * gwx - fx * gwz) + gy;
-    gep[3 * i + 2] = (fx * gwy - fy * gwx) + gz;
+    ge[0] = (fy * gwz - fz * gwy) + gx;
+    ge[1] = (fz * gwx - fx * gwz) + gy;
+    ge[2] = (fx * gwy - fy * gwx) + gz;
     const R im2 = R(2) * (d.inv_mu ? d.inv_mu[i] : S->inv_mu_sub[st.s]);
-    nu2[3 * i] = im2 * hx;
-    nu2[3 * i + 1] = im2 * hy;
-    nu2[3 * i + 2] = im2 * hz;
+    n2[0] = im2 * hx;
+    n2[1] = im2 * hy;
+    n2[2] = im2 * hz;
     return;
   }
   const R a = d.alpha ? d.alpha[i] : S->alpha_sub[st.s];
@@ -324,18 +385,98 @@ __device__ __forceinline__ void adj_local(const Static<R>* __restrict__ S, const
   const R ux = r.w[3 * i] + hdt * fx, uy = r.w[3 * i + 1] + hdt * fy, uz = r.w[3 * i + 2] + hdt * fz;
   const R n = sqrt(ux * ux + uy * uy + uz * uz);
   unproject(ux / n, uy / n, uz / n, n, lx, ly, lz);
-  lamu[3 * i] = lx;
-  lamu[3 * i + 1] = ly;
-  lamu[3 * i + 2] = lz;
+  lu[0] = lx;
+  lu[1] = ly;
+  lu[2] = lz;
+  gw[0] = gw[1] = gw[2] = R(0);
   R gx, gy, gz, hx, hy, hz;
   rhs_vjp(px, py, pz, bx, by, bz, a, d.prec, hdt * lx, hdt * ly, hdt * lz, gx, gy, gz, hx, hy, hz);
-  gep[3 * i] = gx;
-  gep[3 * i + 1] = gy;
-  gep[3 * i + 2] = gz;
+  ge[0] = gx;
+  ge[1] = gy;
+  ge[2] = gz;
   const R im2 = R(2) * (d.inv_mu ? d.inv_mu[i] : S->inv_mu_sub[st.s]);
-  nu2[3 * i] = im2 * hx;
-  nu2[3 * i + 1] = im2 * hy;
-  nu2[3 * i + 2] = im2 * hz;
+  n2[0] = im2 * hx;
+  n2[1] = im2 * hy;
+  n2[2] = im2 * hz;
+}
+
+// adj_local_v with its outputs stored at site i of lamu, gep, nu2 and (Depondt) gwb.
+template <typename R, bool LEAN = false>
+__device__ __forceinline__ void adj_local(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st,
+                                          const StepRec<R>& r, R lx, R ly, R lz, R* __restrict__ lamu,
+                                          R* __restrict__ gep, R* __restrict__ nu2, R* __restrict__ gwb) {
+  const int i = st.i;
+  R lu[3], ge[3], n2[3], gw[3];
+  adj_local_v<R, LEAN>(S, d, st, r, lx, ly, lz, lu, ge, n2, gw);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    lamu[3 * i + q] = lu[q];
+    gep[3 * i + q] = ge[q];
+    nu2[3 * i + q] = n2[q];
+  }
+  if (!LEAN && d.integ == MX_DEPONDT)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) gwb[3 * i + q] = gw[q];
+}
+
+// Predictor part at site i, after the nu2 gather: from l = dL/dep, lambda_u (lu) and the corrector's
+// share of dL/dw1 (gw, Depondt): nu1 (n1) and the partial dL/de (pt).
+template <typename R, bool LEAN = false>
+__device__ __forceinline__ void adj_pred_v(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st,
+                                           const StepRec<R>& r, R lx, R ly, R lz, const R (&lu)[3],
+                                           const R (&gw)[3], R (&n1)[3], R (&pt)[3]) {
+  const int i = st.i;
+  const R a = d.alpha ? d.alpha[i] : S->alpha_sub[st.s];
+  if (!LEAN && d.pinned && d.pinned[i]) {
+    // ep = e' = e: dL/de gathers lambda (in lu) and dL/dep; the spin feels no field (nu1 = 0)
+    n1[0] = n1[1] = n1[2] = R(0);
+    pt[0] = lu[0] + lx;
+    pt[1] = lu[1] + ly;
+    pt[2] = lu[2] + lz;
+  } else if (!LEAN && d.integ == MX_DEPONDT) {
+    // ep = Rot(dt w1) e, w1 = e x f(e, B1): dL/de = lu + Rot^T dL/dep + f1 x gw1 + rhs part,
+    // gw1 = dt dL/dv1 + the corrector's share (gw)
+    const R ex = r.e[3 * i], ey = r.e[3 * i + 1], ez = r.e[3 * i + 2];
+    const R bx = r.B1[3 * i], by = r.B1[3 * i + 1], bz = r.B1[3 * i + 2];
+    R fx, fy, fz;
+    llg_rhs(ex, ey, ez, bx, by, bz, a, d.prec, fx, fy, fz);
+    const R vx = d.dt * r.w[3 * i], vy = d.dt * r.w[3 * i + 1], vz = d.dt * r.w[3 * i + 2];
+    R tx = lx, ty = ly, tz = lz;
+    rotate(tx, ty, tz, -vx, -vy, -vz);
+    R gvx, gvy, gvz;
+    rotate_vjp_v(vx, vy, vz, ex, ey, ez, lx, ly, lz, gvx, gvy, gvz);
+    const R gwx = d.dt * gvx + gw[0], gwy = d.dt * gvy + gw[1], gwz = d.dt * gvz + gw[2];
+    const R lfx = gwy * ez - gwz * ey, lfy = gwz * ex - gwx * ez, lfz = gwx * ey - gwy * ex;  // gw1 x e
+    R gx, gy, gz, qx, qy, qz;
+    rhs_vjp(ex, ey, ez, bx, by, bz, a, d.prec, lfx, lfy, lfz, gx, gy, gz, qx, qy, qz);
+    const R im2 = R(2) * (d.inv_mu ? d.inv_mu[i] : S->inv_mu_sub[st.s]);
+    n1[0] = im2 * qx;
+    n1[1] = im2 * qy;
+    n1[2] = im2 * qz;
+    pt[0] = lu[0] + tx + (fy * gwz - fz * gwy) + gx;
+    pt[1] = lu[1] + ty + (fz * gwx - fx * gwz) + gy;
+    pt[2] = lu[2] + tz + (fx * gwy - fy * gwx) + gz;
+  } else {
+    const R ex = r.e[3 * i], ey = r.e[3 * i + 1], ez = r.e[3 * i + 2];
+    const R bx = r.B1[3 * i], by = r.B1[3 * i + 1], bz = r.B1[3 * i + 2];
+    R fx, fy, fz;
+    llg_rhs(ex, ey, ez, bx, by, bz, a, d.prec, fx, fy, fz);
+    const R vx = ex + d.dt * fx, vy = ey + d.dt * fy, vz = ez + d.dt * fz;
+    const R n = sqrt(vx * vx + vy * vy + vz * vz);
+    unproject(vx / n, vy / n, vz / n, n, lx, ly, lz);  // dL/dv, v = e + dt k1
+    const R ux = lu[0], uy = lu[1], uz = lu[2];
+    const R hdt = R(0.5) * d.dt;
+    const R kx = d.dt * lx + hdt * ux, ky = d.dt * ly + hdt * uy, kz = d.dt * lz + hdt * uz;  // dL/dk1
+    R gx, gy, gz, qx, qy, qz;
+    rhs_vjp(ex, ey, ez, bx, by, bz, a, d.prec, kx, ky, kz, gx, gy, gz, qx, qy, qz);
+    const R im2 = R(2) * (d.inv_mu ? d.inv_mu[i] : S->inv_mu_sub[st.s]);
+    n1[0] = im2 * qx;
+    n1[1] = im2 * qy;
+    n1[2] = im2 * qz;
+    pt[0] = ux + lx + gx;
+    pt[1] = uy + ly + gy;
+    pt[2] = uz + lz + gz;
+  }
 }
 
 template <typename R, int GEO, bool LEAN>
@@ -388,7 +529,6 @@ __global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S
   Site st;
   if (site_of<R, GEO>(S, st)) {
     const int i = st.i;
-    const R a = d.alpha ? d.alpha[i] : S->alpha_sub[st.s];
     const R px = r.ep[3 * i], py = r.ep[3 * i + 1], pz = r.ep[3 * i + 2];
     const R vx2 = nu2[3 * i], vy2 = nu2[3 * i + 1], vz2 = nu2[3 * i + 2];
     if (gf) {
@@ -398,59 +538,20 @@ __global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S
       fv[2] = h * double(vz2);
     }
     R hx, hy, hz, ox, oy, oz;
-    adj_gather<R, GEO, NP, CM>(S, A, d, nu2, r.ep, st, vx2, vy2, vz2, px, py, pz, hx, hy, hz, gp);
+    adj_gather<R, GEO, NP, CM>(S, A, d, NuArray<R>{nu2}, r.ep, st, vx2, vy2, vz2, px, py, pz, hx, hy, hz, gp);
     onsite_T<R, LEAN>(S, d, st, px, py, pz, vx2, vy2, vz2, ox, oy, oz);
-    R lx = gep[3 * i] + hx - ox, ly = gep[3 * i + 1] + hy - oy, lz = gep[3 * i + 2] + hz - oz;  // dL/dep
-    if (!LEAN && d.pinned && d.pinned[i]) {
-      // ep = e' = e: dL/de gathers lambda (in lamu) and dL/dep; the spin feels no field (nu1 = 0).
-      // No early return: every thread reaches the workgroup reduction below.
-      nu1[3 * i] = nu1[3 * i + 1] = nu1[3 * i + 2] = R(0);
-      part[3 * i] = lamu[3 * i] + lx;
-      part[3 * i + 1] = lamu[3 * i + 1] + ly;
-      part[3 * i + 2] = lamu[3 * i + 2] + lz;
-    } else if (!LEAN && d.integ == MX_DEPONDT) {
-      // ep = Rot(dt w1) e, w1 = e x f(e, B1): dL/de = lamu + Rot^T dL/dep + f1 x gw1 + rhs part,
-      // gw1 = dt dL/dv1 + the corrector's share (gwb)
-      const R ex = r.e[3 * i], ey = r.e[3 * i + 1], ez = r.e[3 * i + 2];
-      const R bx = r.B1[3 * i], by = r.B1[3 * i + 1], bz = r.B1[3 * i + 2];
-      R fx, fy, fz;
-      llg_rhs(ex, ey, ez, bx, by, bz, a, d.prec, fx, fy, fz);
-      const R vx = d.dt * r.w[3 * i], vy = d.dt * r.w[3 * i + 1], vz = d.dt * r.w[3 * i + 2];
-      R tx = lx, ty = ly, tz = lz;
-      rotate(tx, ty, tz, -vx, -vy, -vz);
-      R gvx, gvy, gvz;
-      rotate_vjp_v(vx, vy, vz, ex, ey, ez, lx, ly, lz, gvx, gvy, gvz);
-      const R gwx = d.dt * gvx + gwb[3 * i], gwy = d.dt * gvy + gwb[3 * i + 1], gwz = d.dt * gvz + gwb[3 * i + 2];
-      const R lfx = gwy * ez - gwz * ey, lfy = gwz * ex - gwx * ez, lfz = gwx * ey - gwy * ex;  // gw1 x e
-      R gx, gy, gz, qx, qy, qz;
-      rhs_vjp(ex, ey, ez, bx, by, bz, a, d.prec, lfx, lfy, lfz, gx, gy, gz, qx, qy, qz);
-      const R im2 = R(2) * (d.inv_mu ? d.inv_mu[i] : S->inv_mu_sub[st.s]);
-      nu1[3 * i] = im2 * qx;
-      nu1[3 * i + 1] = im2 * qy;
-      nu1[3 * i + 2] = im2 * qz;
-      part[3 * i] = lamu[3 * i] + tx + (fy * gwz - fz * gwy) + gx;
-      part[3 * i + 1] = lamu[3 * i + 1] + ty + (fz * gwx - fx * gwz) + gy;
-      part[3 * i + 2] = lamu[3 * i + 2] + tz + (fx * gwy - fy * gwx) + gz;
-    } else {
-      const R ex = r.e[3 * i], ey = r.e[3 * i + 1], ez = r.e[3 * i + 2];
-      const R bx = r.B1[3 * i], by = r.B1[3 * i + 1], bz = r.B1[3 * i + 2];
-      R fx, fy, fz;
-      llg_rhs(ex, ey, ez, bx, by, bz, a, d.prec, fx, fy, fz);
-      const R vx = ex + d.dt * fx, vy = ey + d.dt * fy, vz = ez + d.dt * fz;
-      const R n = sqrt(vx * vx + vy * vy + vz * vz);
-      unproject(vx / n, vy / n, vz / n, n, lx, ly, lz);  // dL/dv, v = e + dt k1
-      const R ux = lamu[3 * i], uy = lamu[3 * i + 1], uz = lamu[3 * i + 2];
-      const R hdt = R(0.5) * d.dt;
-      const R kx = d.dt * lx + hdt * ux, ky = d.dt * ly + hdt * uy, kz = d.dt * lz + hdt * uz;  // dL/dk1
-      R gx, gy, gz, qx, qy, qz;
-      rhs_vjp(ex, ey, ez, bx, by, bz, a, d.prec, kx, ky, kz, gx, gy, gz, qx, qy, qz);
-      const R im2 = R(2) * (d.inv_mu ? d.inv_mu[i] : S->inv_mu_sub[st.s]);
-      nu1[3 * i] = im2 * qx;
-      nu1[3 * i + 1] = im2 * qy;
-      nu1[3 * i + 2] = im2 * qz;
-      part[3 * i] = ux + lx + gx;
-      part[3 * i + 1] = uy + ly + gy;
-      part[3 * i + 2] = uz + lz + gz;
+    // dL/dep; no early return below: every thread reaches the workgroup reduction
+    const R lx = gep[3 * i] + hx - ox, ly = gep[3 * i + 1] + hy - oy, lz = gep[3 * i + 2] + hz - oz;
+    const R lu[3] = {lamu[3 * i], lamu[3 * i + 1], lamu[3 * i + 2]};
+    R gw[3] = {R(0), R(0), R(0)};
+    if (!LEAN && d.integ == MX_DEPONDT && !(d.pinned && d.pinned[i]))
+      for (int q = 0; q < 3; ++q) gw[q] = gwb[3 * i + q];
+    R n1[3], pt[3];
+    adj_pred_v<R, LEAN>(S, d, st, r, lx, ly, lz, lu, gw, n1, pt);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      nu1[3 * i + q] = n1[q];
+      part[3 * i + q] = pt[q];
     }
   }
   store_partials<NP>(gp, old, gpart);
@@ -483,7 +584,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_c(const Static<R>* __restrict__ S
       fv[2] = h * double(vz);
     }
     R hx, hy, hz, ox, oy, oz;
-    adj_gather<R, GEO, NP, CM>(S, A, d, nu1, e, st, vx, vy, vz, e[3 * i], e[3 * i + 1], e[3 * i + 2], hx, hy, hz,
+    adj_gather<R, GEO, NP, CM>(S, A, d, NuArray<R>{nu1}, e, st, vx, vy, vz, e[3 * i], e[3 * i + 1], e[3 * i + 2], hx, hy, hz,
                                gp);
     onsite_T<R, LEAN>(S, d, st, e[3 * i], e[3 * i + 1], e[3 * i + 2], vx, vy, vz, ox, oy, oz);
     const R lx = part[3 * i] + hx - ox, ly = part[3 * i + 1] + hy - oy, lz = part[3 * i + 2] + hz - oz;
@@ -497,6 +598,141 @@ __global__ __launch_bounds__(BLOCK) void k_adj_c(const Static<R>* __restrict__ S
   }
   store_partials<NP>(gp, old, gpart);
   store_field_partials(fv, gf);
+}
+
+// A running row of parameter partials: the workgroup's sums of gp are added to run (thread p holds
+// parameter p), as store_partials adds them to the stored row.
+template <int NP>
+__device__ __forceinline__ void add_partials(double (&gp)[NP], double& run) {
+  __shared__ double res[NP];
+  block_reduce<NP>(gp, NP, res);
+  __syncthreads();
+  if (threadIdx.x < NP) run = run + res[threadIdx.x];
+}
+
+// Steps [n_lo, n_hi) of a replayed segment's adjoint sweep (newest first) in one cooperative launch
+// (every workgroup resident from the start), for models that take the persistent forward launch.  Each
+// workgroup keeps its sites' adjoint state (lambda, lambda_u, gep, the corrector's share of dL/dw1, the
+// partial dL/de) and its row of parameter partials in registers for the whole launch; sites exchange
+// nu2 and nu1 through tagged records (recA, recB: 2 x N x TAG_WORDS words) as the persistent forward
+// exchanges spins.  Per step, with k = tag0 + 2 (n_hi - 1 - n): the local part publishes nu2 (tag k),
+// the nu2 gather publishes nu1 (k + 1), the nu1 gather gives lambda of the step's input.  A record is
+// never overwritten before every reader has it: a site publishes nu2 of step n - 1 only after reading
+// nu1 of step n of all its neighbours, each of which publishes that only after reading the site's nu2
+// of step n (the stencil is symmetric), and likewise for nu1.  Every wait is for an earlier phase of the
+// sweep and every workgroup is resident, so the launch always completes; polls are bounded as the
+// forward's, and an aborted launch is counted the same way.
+// Same site map, arithmetic and reduction order as k_adj_a, k_adj_b and k_adj_c: the same result bits,
+// partial rows included.  seg: the records, C steps apart (e, ep, w, B1, B2); nuh (dense gradients, or
+// null): nu1 and nu2 of step n also to nuh[n] and nuh[C + n]; gf (or null): field-gradient rows of the
+// segment's first step.  lambda comes from lam and goes back to it.
+template <typename R, int GEO, int NP, int CM, bool LEAN>
+__global__ __launch_bounds__(BLOCK) void k_adj_seg(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
+                                                   const Dyn<R> d, const R* __restrict__ seg, int C, int n_hi,
+                                                   int n_lo, R* __restrict__ lam, R* __restrict__ nuh,
+                                                   double* __restrict__ gpart, double* __restrict__ gf,
+                                                   uint64_t* __restrict__ rec, uint32_t tag0, const StepSync sy) {
+  __shared__ int sh_flag;
+  constexpr int NW = TAG_WORDS<R>;
+  Site st;
+  const bool live = site_of<R, GEO>(S, st);
+  const int i = st.i;
+  const size_t N3 = size_t(S->N) * 3;
+  uint64_t* recA = rec;
+  uint64_t* recB = rec + size_t(S->N) * NW;
+  const uint32_t* abort = sy.ctl + 3;
+  const size_t frows = size_t(gridDim.x) * 3;
+  double run = partial_row<NP>(gpart);
+  R lx = R(0), ly = R(0), lz = R(0);
+  if (live) {
+    lx = lam[3 * i];
+    ly = lam[3 * i + 1];
+    lz = lam[3 * i + 2];
+  }
+  bool ok = true;
+  uint32_t k = tag0;
+  for (int n = n_hi - 1; n >= n_lo; --n, k += 2) {
+    const StepRec<R> r{seg + size_t(n) * N3, seg + (size_t(C) + n) * N3, seg + (2 * size_t(C) + n) * N3,
+                       seg + (3 * size_t(C) + n) * N3, seg + (4 * size_t(C) + n) * N3};
+    double gp[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) gp[p] = 0.0;
+    double fv[3] = {0.0, 0.0, 0.0};
+    R pt[3] = {R(0), R(0), R(0)}, n1[3] = {R(0), R(0), R(0)};
+    double hmu = 0.0;
+    // ---- local part, then the nu2 gather and the predictor's adjoint (k_adj_a / k_adj_c, k_adj_b)
+    if (live) {
+      R lu[3], ge[3], n2[3], gw[3];
+      adj_local_v<R, LEAN>(S, d, st, r, lx, ly, lz, lu, ge, n2, gw);
+      publish_tagged<R>(recA + size_t(i) * NW, n2[0], n2[1], n2[2], k);
+      if (nuh)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) nuh[(size_t(C) + n) * N3 + 3 * i + q] = n2[q];
+      if (gf) {
+        hmu = 0.5 / double(d.inv_mu ? d.inv_mu[i] : S->inv_mu_sub[st.s]);
+        fv[0] = hmu * double(n2[0]);
+        fv[1] = hmu * double(n2[1]);
+        fv[2] = hmu * double(n2[2]);
+      }
+      const R px = r.ep[3 * i], py = r.ep[3 * i + 1], pz = r.ep[3 * i + 2];
+      R hx, hy, hz, ox, oy, oz;
+      const bool g = adj_gather<R, GEO, NP, CM>(S, A, d, TaggedFrom{recA, k, sy.spin, abort}, r.ep, st, n2[0], n2[1],
+                                                n2[2], px, py, pz, hx, hy, hz, gp);
+      ok = ok && g;
+      onsite_T<R, LEAN>(S, d, st, px, py, pz, n2[0], n2[1], n2[2], ox, oy, oz);
+      const R mx = ge[0] + hx - ox, my = ge[1] + hy - oy, mz = ge[2] + hz - oz;
+      adj_pred_v<R, LEAN>(S, d, st, r, mx, my, mz, lu, gw, n1, pt);
+      publish_tagged<R>(recB + size_t(i) * NW, n1[0], n1[1], n1[2], k + 1);
+      if (nuh)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) nuh[size_t(n) * N3 + 3 * i + q] = n1[q];
+    }
+    add_partials<NP>(gp, run);
+    store_field_partials(fv, gf ? gf + (size_t(n) * 2 + 1) * frows : nullptr);
+    // ---- the nu1 gather: lambda of the step's input (k_adj_c)
+#pragma unroll
+    for (int p = 0; p < NP; ++p) gp[p] = 0.0;
+    fv[0] = fv[1] = fv[2] = 0.0;
+    if (live) {
+      if (gf) {
+        fv[0] = hmu * double(n1[0]);
+        fv[1] = hmu * double(n1[1]);
+        fv[2] = hmu * double(n1[2]);
+      }
+      const R ex = r.e[3 * i], ey = r.e[3 * i + 1], ez = r.e[3 * i + 2];
+      R hx, hy, hz, ox, oy, oz;
+      const bool g = adj_gather<R, GEO, NP, CM>(S, A, d, TaggedFrom{recB, k + 1, sy.spin, abort}, r.e, st, n1[0],
+                                                n1[1], n1[2], ex, ey, ez, hx, hy, hz, gp);
+      ok = ok && g;
+      onsite_T<R, LEAN>(S, d, st, ex, ey, ez, n1[0], n1[1], n1[2], ox, oy, oz);
+      lx = pt[0] + hx - ox;
+      ly = pt[1] + hy - oy;
+      lz = pt[2] + hz - oz;
+    }
+    add_partials<NP>(gp, run);
+    store_field_partials(fv, gf ? gf + size_t(n) * 2 * frows : nullptr);
+  }
+  if (live) {
+    lam[3 * i] = lx;
+    lam[3 * i + 1] = ly;
+    lam[3 * i + 2] = lz;
+  }
+  if (threadIdx.x < NP) gpart[size_t(blockIdx.x) * MAXP + threadIdx.x] = run;
+  if (!ok) __hip_atomic_store(const_cast<uint32_t*>(abort), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // ---- the last workgroup to finish counts an aborted launch and clears the control words
+  const int bad = __syncthreads_or(ok ? 0 : 1);
+  if (threadIdx.x == 0) {
+    const uint32_t done = bad ? __hip_atomic_fetch_add(sy.ctl, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT)
+                              : __hip_atomic_fetch_add(sy.ctl, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sh_flag = done == gridDim.x - 1 ? 1 : 0;
+  }
+  __syncthreads();
+  if (!sh_flag || threadIdx.x != 0) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  if (__hip_atomic_load(sy.ctl + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    __hip_atomic_fetch_add(sy.ctl + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(sy.ctl + 3, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(sy.ctl, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <typename T>
@@ -859,6 +1095,41 @@ void ensure_buffers(mx_adjoint& a) {
 }
 
 
+// The whole sweep of a segment of two or more steps in persistent launches of up to STEPS_PER_LAUNCH
+// steps (k_adj_seg), on models that take the persistent forward launch (whose tagged records and control
+// words it shares: consecutive launches on them take different epochs); false when the model does not.
+template <typename R, int GEO, int NP, int CM, bool LEAN>
+bool sweep_persistent(mx_adjoint& a, const Dyn<R>& d, const R* seg, int C, int n_seg, R* nuh, hipStream_t s) {
+  if constexpr (GEO != GEO_PERIODIC && GEO != GEO_OPEN) {
+    return false;
+  } else {
+    Model& m = *a.m;
+    if (n_seg < 2 || !persistent_steps(m) || ensure_step_scratch(m) != MX_OK || !m.rec_d || !m.stepsync_d)
+      return false;
+    const Static<R>* S = static_cast<const Static<R>*>(m.static_d);
+    const AStatic<R>* A = static_cast<const AStatic<R>*>(a.astatic_d);
+    StepSync sy{};
+    sy.cnt = m.stepsync_d;
+    sy.ctl = m.stepsync_d + stepsync_ctl(m);
+    sy.gu = reinterpret_cast<int32_t*>(sy.ctl + CTL_WORDS);
+    sy.spin = 1u << 16;
+    R* lam = static_cast<R*>(a.lam_d);
+    double* gpart = a.gpart_d;
+    double* gf = a.want_field ? a.gf_d + size_t(a.sweep_step0) * 2 * a.nblk * 3 : nullptr;
+    uint64_t* rec = m.rec_d;
+    static_assert(2 * STEPS_PER_LAUNCH + 2 < (1 << 12), "sweep tags overflow their 12 bits");
+    for (int hi = n_seg; hi > 0; hi -= STEPS_PER_LAUNCH) {
+      int lo = std::max(0, hi - STEPS_PER_LAUNCH);
+      m.rec_epoch = m.rec_epoch % 15u + 1u;
+      uint32_t tag0 = m.rec_epoch << 12;
+      void* args[] = {&S, &A, const_cast<Dyn<R>*>(&d), &seg, &C, &hi, &lo, &lam, &nuh, &gpart, &gf, &rec, &tag0, &sy};
+      aok(hipLaunchCooperativeKernel(k_adj_seg<R, GEO, NP, CM, LEAN>, dim3(a.nblk), dim3(BLOCK), args, 0, s),
+          "persistent adjoint sweep");
+    }
+    return true;
+  }
+}
+
 // Adjoint sweep over one replayed segment of n_seg steps (records rec(n), newest last): the local
 // part of the segment's last step from lam, then per step (newest first) k_adj_b and k_adj_c, the
 // latter fused with the next-older step's local part.  Leaves dL/de of the segment's first input in lam.
@@ -901,6 +1172,11 @@ void sweep_t(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s, R* 
   auto frow = [&](int n, int pass) -> double* {
     return a.want_field ? a.gf_d + (size_t(a.sweep_step0 + n) * 2 + pass) * a.nblk * 3 : nullptr;
   };
+  if (sweep_persistent<R, GEO, NP, CM, LEAN>(a, d, seg, C, n_seg, nuh, s)) {
+    if (nuh)
+      hipLaunchKernelGGL((k_dense_seg<R, GEO, 4>), g, b, 0, s, S, A, seg, nuh, C, n_seg, a.gb_d, a.ga_d);
+    return;
+  }
   hipLaunchKernelGGL((k_adj_a<R, GEO, LEAN>), g, b, 0, s, S, d, lam, rec[n_seg - 1], lamu, gep, nu2_of(n_seg - 1),
                      gwb);
   for (int n = n_seg - 1; n >= 0; --n) {

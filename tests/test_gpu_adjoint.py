@@ -166,8 +166,9 @@ def test_adjoint_checkpoint_interval_is_exact(gpu):
 @pytest.mark.parametrize("integrator", ["heun", "depondt"])
 def test_recorded_persistent_replays_equal_single_pass_replays(gpu, integrator):
     """Segments of two or more undriven steps are recorded by the persistent launch (every record in one
-    launch); one-step segments by the single-pass kernels.  Both must store the same records bit for bit,
-    so the gradients with checkpoints every step and every 16 steps are equal -- on the general model:
+    launch) and swept by the persistent adjoint launch; one-step segments by the single-pass kernels.  Both
+    must give the same records and sweeps bit for bit, so the gradients with checkpoints every step and
+    every 16 steps are equal -- on the general model:
     pinned sites, per-site damping, T = 50 K, the dense gradients too."""
     print(f"case: recorded persistent vs single-pass replays, square 16^2, {integrator}", flush=True)
     from magex_amd.adjoint import DifferentiableLLG
@@ -195,6 +196,55 @@ def test_recorded_persistent_replays_equal_single_pass_replays(gpu, integrator):
         out.append([x.cpu().numpy() for x in (e, gt, g0, gb, ga)])
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b), float(np.abs(a - b).max())
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_persistent_sweep_equals_per_pass_sweep(gpu, precision):
+    """On a model that takes the persistent forward launch (step mode 2), every segment of two or more
+    steps is swept by one persistent launch; in step mode 1 every step runs three kernels.  Both give the
+    same gradients bit for bit: the bcc 4^3 per-bond model (28 stencil entries, per-bond tensors, per-site
+    anisotropy, three parameters) with pinned sites, per-site damping and T = 30 K, segments of 16 steps
+    and a last one of 9; parameter, spin and dense gradients undriven, field gradients under a drive."""
+    print(f"case: persistent vs per-pass adjoint sweep, bcc 4^3 per-bond, {precision}", flush=True)
+    from magex_amd.adjoint import DifferentiableLLG
+    from magex_amd import models as Md
+    from magex_amd import spinmodel as Sd
+    L, steps, th0 = 4, 41, np.array([20.0, 10.0, 0.5])
+
+    def dev0(th):
+        return Sd.SpinSystem(Sd.load_results(Md.make_model("bcc_conventional", a=2.87, J=[th[0], th[1]], K=th[2],
+                                                           moment=2.2)), (L, L, L))
+
+    dev1, _, _ = _disorder(dev0, dev0, th0, seed=23, sym_A=True)
+    N = 2 * L ** 3
+    mask = np.zeros(N, bool)
+    mask[[1, 30, 77]] = True
+    alpha = 0.15 + 0.2 * np.random.default_rng(4).random(N)
+
+    def dev(th):
+        s = dev1(th)
+        s.pinned = mask
+        return s
+
+    out = {}
+    for mode in (1, 2):
+        sol = DifferentiableLLG(dev, th0, alpha=alpha, dt=1e-3, temperature=30.0, seed=5, precision=precision,
+                                checkpoint_every=16)
+        sol.dm.set_step_mode(mode)
+        assert (sol.dm.step_info()[0] == 2) == (mode == 2)
+        e0 = sol.system.init_spins("random", seed=12)
+        tgt = torch.as_tensor(sol.system.init_spins("random", seed=13), device="cuda",
+                              dtype=torch.float64 if precision == "fp64" else torch.float32)
+        e = sol.forward(th0, e0, steps)
+        res = [e, *sol.backward_dense(_mse_grad(e, tgt))]
+        B = sol.field_sequence(lambda t: np.array([2.0 * np.sin(30.0 * t), 0.3, 1.0]), steps)
+        e = sol.forward(th0, e0, steps, field=B)
+        res += [e, *sol.backward_field(_mse_grad(e, tgt))]
+        torch.cuda.synchronize()
+        assert sol.dm.step_info()[1] == 0
+        out[mode] = [x.cpu().numpy() for x in res]
+    for k, (a, b) in enumerate(zip(out[1], out[2])):
+        assert np.array_equal(a, b), (k, float(np.abs(a - b).max()))
 
 
 def test_adjoint_forward_is_the_plain_trajectory(gpu):
