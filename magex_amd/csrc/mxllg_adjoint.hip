@@ -489,29 +489,54 @@ __global__ __launch_bounds__(BLOCK) void k_adj_a(const Static<R>* __restrict__ S
   adj_local<R, LEAN>(S, d, st, r, lam[3 * i], lam[3 * i + 1], lam[3 * i + 2], lamu, gep, nu2, gwb);
 }
 
-// Adds the workgroup's sums of gp to its row of gpart (stride MAXP); old is that row's entry for this
-// thread, loaded at kernel start so the read is off the critical path.
+// Partial sums are kept per wavefront, not per workgroup: a wavefront's sum is a shuffle reduction that
+// needs no barrier, so no wavefront waits for the others of its workgroup (which would hold every
+// exchange of the persistent sweep to the workgroup's slowest wavefront).  Row of this wavefront:
+constexpr int WAVES = BLOCK / 64;
+__device__ __forceinline__ size_t wave_row() { return size_t(blockIdx.x) * WAVES + (threadIdx.x >> 6); }
+
+// v[lane] for lanes below NV (a select chain: no dynamic register indexing), 0 elsewhere
+template <int NV>
+__device__ __forceinline__ double lane_pick(const double (&v)[NV]) {
+  const int lane = threadIdx.x & 63;
+  double r = 0.0;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) r = lane == k ? v[k] : r;
+  return r;
+}
+
+// Lane p's entry of the wavefront's row of gpart (stride MAXP), loaded at kernel start so the read is
+// off the critical path.
 template <int NP>
 __device__ __forceinline__ double partial_row(const double* __restrict__ gpart) {
-  return threadIdx.x < NP ? gpart[size_t(blockIdx.x) * MAXP + threadIdx.x] : 0.0;
+  const int lane = threadIdx.x & 63;
+  return lane < NP ? gpart[wave_row() * MAXP + lane] : 0.0;
 }
 
+// Adds the wavefront's sums of gp to its running row entry run (lane p holds parameter p).
+template <int NP>
+__device__ __forceinline__ void add_partials(double (&gp)[NP], double& run) {
+#pragma unroll
+  for (int p = 0; p < NP; ++p) gp[p] = wave_sum(gp[p]);
+  if ((threadIdx.x & 63) < NP) run = run + lane_pick<NP>(gp);
+}
+
+// The same, stored to the wavefront's row: old + the wavefront's sums.
 template <int NP>
 __device__ __forceinline__ void store_partials(double (&gp)[NP], double old, double* __restrict__ gpart) {
-  __shared__ double res[NP];
-  block_reduce<NP>(gp, NP, res);
-  __syncthreads();
-  if (threadIdx.x < NP) gpart[size_t(blockIdx.x) * MAXP + threadIdx.x] = old + res[threadIdx.x];
+  add_partials<NP>(gp, old);
+  const int lane = threadIdx.x & 63;
+  if (lane < NP) gpart[wave_row() * MAXP + lane] = old;
 }
 
-// dL/dB of the uniform field in this pass: the workgroup's sum of nu mu mu_B / 2 (= dL/dB_i) into row
-// gf (nblk x 3), when the caller asks for it (gf is uniform over the grid).
+// dL/dB of the uniform field in this pass: the wavefront's sum of nu mu mu_B / 2 (= dL/dB_i) into its row
+// of gf (nblk x WAVES x 3), when the caller asks for it (gf is uniform over the grid).
 __device__ __forceinline__ void store_field_partials(double (&fv)[3], double* __restrict__ gf) {
   if (!gf) return;
-  __shared__ double fres[3];
-  block_reduce<3>(fv, 3, fres);
-  __syncthreads();
-  if (threadIdx.x < 3) gf[size_t(blockIdx.x) * 3 + threadIdx.x] = fres[threadIdx.x];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) fv[k] = wave_sum(fv[k]);
+  const int lane = threadIdx.x & 63;
+  if (lane < 3) gf[wave_row() * 3 + lane] = lane_pick<3>(fv);
 }
 
 // Gather nu2 (dL/dep) and the predictor's adjoint: writes nu1 and the partial dL/de (part).
@@ -600,20 +625,11 @@ __global__ __launch_bounds__(BLOCK) void k_adj_c(const Static<R>* __restrict__ S
   store_field_partials(fv, gf);
 }
 
-// A running row of parameter partials: the workgroup's sums of gp are added to run (thread p holds
-// parameter p), as store_partials adds them to the stored row.
-template <int NP>
-__device__ __forceinline__ void add_partials(double (&gp)[NP], double& run) {
-  __shared__ double res[NP];
-  block_reduce<NP>(gp, NP, res);
-  __syncthreads();
-  if (threadIdx.x < NP) run = run + res[threadIdx.x];
-}
-
 // Steps [n_lo, n_hi) of a replayed segment's adjoint sweep (newest first) in one cooperative launch
 // (every workgroup resident from the start), for models that take the persistent forward launch.  Each
 // workgroup keeps its sites' adjoint state (lambda, lambda_u, gep, the corrector's share of dL/dw1, the
-// partial dL/de) and its row of parameter partials in registers for the whole launch; sites exchange
+// partial dL/de) and each wavefront its row of parameter partials in registers for the whole launch (no
+// barrier between its exchanges); sites exchange
 // nu2 and nu1 through tagged records (recA, recB: 2 x N x TAG_WORDS words) as the persistent forward
 // exchanges spins.  Per step, with k = tag0 + 2 (n_hi - 1 - n): the local part publishes nu2 (tag k),
 // the nu2 gather publishes nu1 (k + 1), the nu1 gather gives lambda of the step's input.  A record is
@@ -641,7 +657,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_seg(const Static<R>* __restrict__
   uint64_t* recA = rec;
   uint64_t* recB = rec + size_t(S->N) * NW;
   const uint32_t* abort = sy.ctl + 3;
-  const size_t frows = size_t(gridDim.x) * 3;
+  const size_t frows = size_t(gridDim.x) * WAVES * 3;
   double run = partial_row<NP>(gpart);
   R lx = R(0), ly = R(0), lz = R(0);
   if (live) {
@@ -717,7 +733,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_seg(const Static<R>* __restrict__
     lam[3 * i + 1] = ly;
     lam[3 * i + 2] = lz;
   }
-  if (threadIdx.x < NP) gpart[size_t(blockIdx.x) * MAXP + threadIdx.x] = run;
+  if ((threadIdx.x & 63) < NP) gpart[wave_row() * MAXP + (threadIdx.x & 63)] = run;
   if (!ok) __hip_atomic_store(const_cast<uint32_t*>(abort), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   // ---- the last workgroup to finish counts an aborted launch and clears the control words
   const int bad = __syncthreads_or(ok ? 0 : 1);
@@ -896,7 +912,7 @@ struct mx_adjoint {
   void *lam_d = nullptr, *lamu_d = nullptr, *gep_d = nullptr, *nu1_d = nullptr, *nu2_d = nullptr, *part_d = nullptr;
   void* gw_d = nullptr;  // Depondt: the corrector's share of dL/dw1, N x 3
   void* ework_d = nullptr;
-  double* gpart_d = nullptr;  // per-workgroup parameter-gradient partials, nblk x MAXP
+  double* gpart_d = nullptr;  // per-wavefront parameter-gradient partials, nblk x WAVES x MAXP
   double* gout_d = nullptr;   // their fold, MAXP
   int nblk = 0;
   // dense gradients: per forward bond 9 x ncell, per site 9 x N (FP64, internal cells), and the magex
@@ -908,7 +924,7 @@ struct mx_adjoint {
   int cmap_geo = -1;
   // driven forward: the uniform field of each pass of each step (nsteps x 2 x 3; empty: the model's)
   std::vector<double> drive;
-  // dL/dB of the uniform field per step and pass: per-workgroup partials (nsteps x 2) x nblk x 3
+  // dL/dB of the uniform field per step and pass: per-wavefront partials (nsteps x 2) x nblk x WAVES x 3
   double* gf_d = nullptr;
   int64_t gf_rows = 0;
   bool want_field = false;
@@ -1090,7 +1106,8 @@ void ensure_buffers(mx_adjoint& a) {
   for (void** p : {&a.lam_d, &a.lamu_d, &a.gep_d, &a.nu1_d, &a.nu2_d, &a.part_d, &a.ework_d, &a.gw_d})
     if (!*p) aok(hipMalloc(p, b), "hipMalloc adjoint state");
   a.nblk = m.ns * ((m.ncell + BLOCK - 1) / BLOCK);
-  if (!a.gpart_d) aok(hipMalloc(reinterpret_cast<void**>(&a.gpart_d), sizeof(double) * MAXP * a.nblk), "hipMalloc gpart");
+  if (!a.gpart_d)
+    aok(hipMalloc(reinterpret_cast<void**>(&a.gpart_d), sizeof(double) * MAXP * WAVES * a.nblk), "hipMalloc gpart");
   if (!a.gout_d) aok(hipMalloc(reinterpret_cast<void**>(&a.gout_d), sizeof(double) * MAXP), "hipMalloc gout");
 }
 
@@ -1115,7 +1132,7 @@ bool sweep_persistent(mx_adjoint& a, const Dyn<R>& d, const R* seg, int C, int n
     sy.spin = 1u << 16;
     R* lam = static_cast<R*>(a.lam_d);
     double* gpart = a.gpart_d;
-    double* gf = a.want_field ? a.gf_d + size_t(a.sweep_step0) * 2 * a.nblk * 3 : nullptr;
+    double* gf = a.want_field ? a.gf_d + size_t(a.sweep_step0) * 2 * a.nblk * WAVES * 3 : nullptr;
     uint64_t* rec = m.rec_d;
     static_assert(2 * STEPS_PER_LAUNCH + 2 < (1 << 12), "sweep tags overflow their 12 bits");
     for (int hi = n_seg; hi > 0; hi -= STEPS_PER_LAUNCH) {
@@ -1168,9 +1185,9 @@ void sweep_t(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s, R* 
   R* part = static_cast<R*>(a.part_d);
   R* gwb = static_cast<R*>(a.gw_d);
   const dim3 g(a.nblk), b(BLOCK);
-  // field-gradient rows: (global step * 2 + pass index) x nblk x 3, pass index 0 = predictor, 1 = corrector
+  // field-gradient rows: (global step * 2 + pass index) x nblk x WAVES x 3, pass index 0 = predictor, 1 = corrector
   auto frow = [&](int n, int pass) -> double* {
-    return a.want_field ? a.gf_d + (size_t(a.sweep_step0 + n) * 2 + pass) * a.nblk * 3 : nullptr;
+    return a.want_field ? a.gf_d + (size_t(a.sweep_step0 + n) * 2 + pass) * a.nblk * WAVES * 3 : nullptr;
   };
   if (sweep_persistent<R, GEO, NP, CM, LEAN>(a, d, seg, C, n_seg, nuh, s)) {
     if (nuh)
@@ -1275,14 +1292,15 @@ int backward_impl(mx_adjoint* a, const void* grad_e, void* grad_e0, double* grad
     if (a->gf_d) (void)hipFree(a->gf_d);
     a->gf_d = nullptr;
     a->gf_rows = 0;
-    aok(hipMalloc(reinterpret_cast<void**>(&a->gf_d), sizeof(double) * 2 * size_t(a->nsteps) * a->nblk * 3),
+    aok(hipMalloc(reinterpret_cast<void**>(&a->gf_d), sizeof(double) * 2 * size_t(a->nsteps) * a->nblk * WAVES * 3),
         "hipMalloc field-gradient partials");
     a->gf_rows = 2 * a->nsteps;
   }
   // the adjoint works in the library's internal site order; permute at entry and exit
   to_internal(*a, grad_e, a->lam_d, s);
-  hipLaunchKernelGGL((k_zero<double>), dim3((MAXP * a->nblk + 255) / 256), dim3(256), 0, s, a->gpart_d,
-                     size_t(MAXP) * a->nblk);
+  const int nrow = a->nblk * WAVES;
+  hipLaunchKernelGGL((k_zero<double>), dim3((MAXP * nrow + 255) / 256), dim3(256), 0, s, a->gpart_d,
+                     size_t(MAXP) * nrow);
   char* ck = static_cast<char*>(a->ckpt_d);
   const int C = a->C;
   for (int64_t k = (a->nsteps - 1) / C; k >= 0 && a->nsteps > 0; --k) {
@@ -1293,11 +1311,11 @@ int backward_impl(mx_adjoint* a, const void* grad_e, void* grad_e0, double* grad
     if (fp64) sweep<double>(*a, static_cast<char*>(a->seg_d), b, a->seg_cap, n_seg, dense, s);
     else sweep<float>(*a, static_cast<char*>(a->seg_d), b, a->seg_cap, n_seg, dense, s);
   }
-  hipLaunchKernelGGL((k_final<MAXP>), dim3(1), dim3(BLOCK), 0, s, a->gpart_d, a->nblk, MAXP, 1.0, a->gout_d);
+  hipLaunchKernelGGL((k_final<MAXP>), dim3(1), dim3(BLOCK), 0, s, a->gpart_d, nrow, MAXP, 1.0, a->gout_d);
   hipLaunchKernelGGL((k_copy<double>), dim3(1), dim3(64), 0, s, a->gout_d, grad_theta, size_t(a->P));
   if (grad_e0) to_magex(*a, a->lam_d, grad_e0, s);
   if (a->want_field)  // every (step, pass) row was written once by the sweep
-    hipLaunchKernelGGL((k_final<3>), dim3(unsigned(2 * a->nsteps)), dim3(BLOCK), 0, s, a->gf_d, a->nblk, 3, 1.0,
+    hipLaunchKernelGGL((k_final<3>), dim3(unsigned(2 * a->nsteps)), dim3(BLOCK), 0, s, a->gf_d, nrow, 3, 1.0,
                        grad_field);
   a->want_field = false;
   const int* cmap = m.geo == GEO_TILED ? a->cmap_d : nullptr;
