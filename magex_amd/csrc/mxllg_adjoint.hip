@@ -316,16 +316,46 @@ struct StepRec {
   const R *e, *ep, *w, *B1, *B2;
 };
 
+// One step's record at one site (e, ep, w, B(e), B(ep)) and the site's constants (damping, the
+// 2 / (mu mu_B) scale of nu, pinned), as the adjoint's local cores use them: loaded one step ahead of
+// use in the persistent sweep.
+template <typename R>
+struct SiteRec {
+  R e[3], ep[3], w[3], b1[3], b2[3];
+};
+template <typename R>
+__device__ __forceinline__ void load_rec(const StepRec<R>& r, int i, SiteRec<R>& v) {
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    v.e[q] = r.e[3 * i + q];
+    v.ep[q] = r.ep[3 * i + q];
+    v.w[q] = r.w[3 * i + q];
+    v.b1[q] = r.B1[3 * i + q];
+    v.b2[q] = r.B2[3 * i + q];
+  }
+}
+template <typename R>
+struct SiteK {
+  R a, im2;
+  bool pin;
+};
+template <typename R, bool LEAN>
+__device__ __forceinline__ SiteK<R> site_k(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st) {
+  SiteK<R> k;
+  k.a = d.alpha ? d.alpha[st.i] : S->alpha_sub[st.s];
+  k.im2 = R(2) * (d.inv_mu ? d.inv_mu[st.i] : S->inv_mu_sub[st.s]);
+  k.pin = !LEAN && d.pinned && d.pinned[st.i];
+  return k;
+}
+
 // Step-local part (corrector) at site i: from lambda = dL/de' and the step's record: lambda_u (lu), the
 // cotangent of ep through k2 (ge), nu2 (n2) and, under Depondt, the corrector's share of dL/dw1 (gw).
 // A pinned spin is carried unchanged (e' = ep = e), so lambda passes through and no field acts on it
 // (ge = n2 = gw = 0).
 template <typename R, bool LEAN = false>
-__device__ __forceinline__ void adj_local_v(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st,
-                                            const StepRec<R>& r, R lx, R ly, R lz, R (&lu)[3], R (&ge)[3],
-                                            R (&n2)[3], R (&gw)[3]) {
-  const int i = st.i;
-  if (!LEAN && d.pinned && d.pinned[i]) {
+__device__ __forceinline__ void adj_local_v(const Dyn<R>& d, const SiteK<R>& k, const SiteRec<R>& v, R lx, R ly,
+                                            R lz, R (&lu)[3], R (&ge)[3], R (&n2)[3], R (&gw)[3]) {
+  if (k.pin) {
     lu[0] = lx;
     lu[1] = ly;
     lu[2] = lz;
@@ -338,15 +368,15 @@ __device__ __forceinline__ void adj_local_v(const Static<R>* __restrict__ S, con
     // e' = N(Rot(v) e), v = dt/2 (w1 + w2), w2 = ep x f(ep, B2); the record's k1 slot holds w1.
     // lu = Rot(v)^T dL/du (the direct part of dL/de), gw = dL/dw1 from this stage (= dL/dw2 =
     // dt/2 dL/dv), ge = dL/dep through w2, and n2.
-    const R a = d.alpha ? d.alpha[i] : S->alpha_sub[st.s];
-    const R ex = r.e[3 * i], ey = r.e[3 * i + 1], ez = r.e[3 * i + 2];
-    const R px = r.ep[3 * i], py = r.ep[3 * i + 1], pz = r.ep[3 * i + 2];
-    const R bx = r.B2[3 * i], by = r.B2[3 * i + 1], bz = r.B2[3 * i + 2];
+    const R a = k.a;
+    const R ex = v.e[0], ey = v.e[1], ez = v.e[2];
+    const R px = v.ep[0], py = v.ep[1], pz = v.ep[2];
+    const R bx = v.b2[0], by = v.b2[1], bz = v.b2[2];
     R fx, fy, fz, w2x, w2y, w2z;
     llg_rhs(px, py, pz, bx, by, bz, a, d.prec, fx, fy, fz);
     omega(px, py, pz, fx, fy, fz, w2x, w2y, w2z);
     const R hdt = R(0.5) * d.dt;
-    const R vx = hdt * (r.w[3 * i] + w2x), vy = hdt * (r.w[3 * i + 1] + w2y), vz = hdt * (r.w[3 * i + 2] + w2z);
+    const R vx = hdt * (v.w[0] + w2x), vy = hdt * (v.w[1] + w2y), vz = hdt * (v.w[2] + w2z);
     R ux = ex, uy = ey, uz = ez;
     rotate(ux, uy, uz, vx, vy, vz);
     const R n = sqrt(ux * ux + uy * uy + uz * uz);
@@ -369,20 +399,20 @@ __device__ __forceinline__ void adj_local_v(const Static<R>* __restrict__ S, con
     ge[0] = (fy * gwz - fz * gwy) + gx;
     ge[1] = (fz * gwx - fx * gwz) + gy;
     ge[2] = (fx * gwy - fy * gwx) + gz;
-    const R im2 = R(2) * (d.inv_mu ? d.inv_mu[i] : S->inv_mu_sub[st.s]);
+    const R im2 = k.im2;
     n2[0] = im2 * hx;
     n2[1] = im2 * hy;
     n2[2] = im2 * hz;
     return;
   }
-  const R a = d.alpha ? d.alpha[i] : S->alpha_sub[st.s];
-  const R px = r.ep[3 * i], py = r.ep[3 * i + 1], pz = r.ep[3 * i + 2];
-  const R bx = r.B2[3 * i], by = r.B2[3 * i + 1], bz = r.B2[3 * i + 2];
+  const R a = k.a;
+  const R px = v.ep[0], py = v.ep[1], pz = v.ep[2];
+  const R bx = v.b2[0], by = v.b2[1], bz = v.b2[2];
   R fx, fy, fz;
   llg_rhs(px, py, pz, bx, by, bz, a, d.prec, fx, fy, fz);
   const R hdt = R(0.5) * d.dt;
   // the record's k1 slot holds w = e + dt/2 k1 (the forward's pass-1 output), so u = w + dt/2 k2
-  const R ux = r.w[3 * i] + hdt * fx, uy = r.w[3 * i + 1] + hdt * fy, uz = r.w[3 * i + 2] + hdt * fz;
+  const R ux = v.w[0] + hdt * fx, uy = v.w[1] + hdt * fy, uz = v.w[2] + hdt * fz;
   const R n = sqrt(ux * ux + uy * uy + uz * uz);
   unproject(ux / n, uy / n, uz / n, n, lx, ly, lz);
   lu[0] = lx;
@@ -394,7 +424,7 @@ __device__ __forceinline__ void adj_local_v(const Static<R>* __restrict__ S, con
   ge[0] = gx;
   ge[1] = gy;
   ge[2] = gz;
-  const R im2 = R(2) * (d.inv_mu ? d.inv_mu[i] : S->inv_mu_sub[st.s]);
+  const R im2 = k.im2;
   n2[0] = im2 * hx;
   n2[1] = im2 * hy;
   n2[2] = im2 * hz;
@@ -406,8 +436,10 @@ __device__ __forceinline__ void adj_local(const Static<R>* __restrict__ S, const
                                           const StepRec<R>& r, R lx, R ly, R lz, R* __restrict__ lamu,
                                           R* __restrict__ gep, R* __restrict__ nu2, R* __restrict__ gwb) {
   const int i = st.i;
+  SiteRec<R> v;
+  load_rec(r, i, v);
   R lu[3], ge[3], n2[3], gw[3];
-  adj_local_v<R, LEAN>(S, d, st, r, lx, ly, lz, lu, ge, n2, gw);
+  adj_local_v<R, LEAN>(d, site_k<R, LEAN>(S, d, st), v, lx, ly, lz, lu, ge, n2, gw);
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     lamu[3 * i + q] = lu[q];
@@ -422,12 +454,10 @@ __device__ __forceinline__ void adj_local(const Static<R>* __restrict__ S, const
 // Predictor part at site i, after the nu2 gather: from l = dL/dep, lambda_u (lu) and the corrector's
 // share of dL/dw1 (gw, Depondt): nu1 (n1) and the partial dL/de (pt).
 template <typename R, bool LEAN = false>
-__device__ __forceinline__ void adj_pred_v(const Static<R>* __restrict__ S, const Dyn<R>& d, const Site& st,
-                                           const StepRec<R>& r, R lx, R ly, R lz, const R (&lu)[3],
-                                           const R (&gw)[3], R (&n1)[3], R (&pt)[3]) {
-  const int i = st.i;
-  const R a = d.alpha ? d.alpha[i] : S->alpha_sub[st.s];
-  if (!LEAN && d.pinned && d.pinned[i]) {
+__device__ __forceinline__ void adj_pred_v(const Dyn<R>& d, const SiteK<R>& k, const SiteRec<R>& v, R lx, R ly,
+                                           R lz, const R (&lu)[3], const R (&gw)[3], R (&n1)[3], R (&pt)[3]) {
+  const R a = k.a;
+  if (k.pin) {
     // ep = e' = e: dL/de gathers lambda (in lu) and dL/dep; the spin feels no field (nu1 = 0)
     n1[0] = n1[1] = n1[2] = R(0);
     pt[0] = lu[0] + lx;
@@ -436,11 +466,11 @@ __device__ __forceinline__ void adj_pred_v(const Static<R>* __restrict__ S, cons
   } else if (!LEAN && d.integ == MX_DEPONDT) {
     // ep = Rot(dt w1) e, w1 = e x f(e, B1): dL/de = lu + Rot^T dL/dep + f1 x gw1 + rhs part,
     // gw1 = dt dL/dv1 + the corrector's share (gw)
-    const R ex = r.e[3 * i], ey = r.e[3 * i + 1], ez = r.e[3 * i + 2];
-    const R bx = r.B1[3 * i], by = r.B1[3 * i + 1], bz = r.B1[3 * i + 2];
+    const R ex = v.e[0], ey = v.e[1], ez = v.e[2];
+    const R bx = v.b1[0], by = v.b1[1], bz = v.b1[2];
     R fx, fy, fz;
     llg_rhs(ex, ey, ez, bx, by, bz, a, d.prec, fx, fy, fz);
-    const R vx = d.dt * r.w[3 * i], vy = d.dt * r.w[3 * i + 1], vz = d.dt * r.w[3 * i + 2];
+    const R vx = d.dt * v.w[0], vy = d.dt * v.w[1], vz = d.dt * v.w[2];
     R tx = lx, ty = ly, tz = lz;
     rotate(tx, ty, tz, -vx, -vy, -vz);
     R gvx, gvy, gvz;
@@ -449,7 +479,7 @@ __device__ __forceinline__ void adj_pred_v(const Static<R>* __restrict__ S, cons
     const R lfx = gwy * ez - gwz * ey, lfy = gwz * ex - gwx * ez, lfz = gwx * ey - gwy * ex;  // gw1 x e
     R gx, gy, gz, qx, qy, qz;
     rhs_vjp(ex, ey, ez, bx, by, bz, a, d.prec, lfx, lfy, lfz, gx, gy, gz, qx, qy, qz);
-    const R im2 = R(2) * (d.inv_mu ? d.inv_mu[i] : S->inv_mu_sub[st.s]);
+    const R im2 = k.im2;
     n1[0] = im2 * qx;
     n1[1] = im2 * qy;
     n1[2] = im2 * qz;
@@ -457,8 +487,8 @@ __device__ __forceinline__ void adj_pred_v(const Static<R>* __restrict__ S, cons
     pt[1] = lu[1] + ty + (fz * gwx - fx * gwz) + gy;
     pt[2] = lu[2] + tz + (fx * gwy - fy * gwx) + gz;
   } else {
-    const R ex = r.e[3 * i], ey = r.e[3 * i + 1], ez = r.e[3 * i + 2];
-    const R bx = r.B1[3 * i], by = r.B1[3 * i + 1], bz = r.B1[3 * i + 2];
+    const R ex = v.e[0], ey = v.e[1], ez = v.e[2];
+    const R bx = v.b1[0], by = v.b1[1], bz = v.b1[2];
     R fx, fy, fz;
     llg_rhs(ex, ey, ez, bx, by, bz, a, d.prec, fx, fy, fz);
     const R vx = ex + d.dt * fx, vy = ey + d.dt * fy, vz = ez + d.dt * fz;
@@ -469,7 +499,7 @@ __device__ __forceinline__ void adj_pred_v(const Static<R>* __restrict__ S, cons
     const R kx = d.dt * lx + hdt * ux, ky = d.dt * ly + hdt * uy, kz = d.dt * lz + hdt * uz;  // dL/dk1
     R gx, gy, gz, qx, qy, qz;
     rhs_vjp(ex, ey, ez, bx, by, bz, a, d.prec, kx, ky, kz, gx, gy, gz, qx, qy, qz);
-    const R im2 = R(2) * (d.inv_mu ? d.inv_mu[i] : S->inv_mu_sub[st.s]);
+    const R im2 = k.im2;
     n1[0] = im2 * qx;
     n1[1] = im2 * qy;
     n1[2] = im2 * qz;
@@ -565,14 +595,17 @@ __global__ __launch_bounds__(BLOCK) void k_adj_b(const Static<R>* __restrict__ S
     R hx, hy, hz, ox, oy, oz;
     adj_gather<R, GEO, NP, CM>(S, A, d, NuArray<R>{nu2}, r.ep, st, vx2, vy2, vz2, px, py, pz, hx, hy, hz, gp);
     onsite_T<R, LEAN>(S, d, st, px, py, pz, vx2, vy2, vz2, ox, oy, oz);
-    // dL/dep; no early return below: every thread reaches the workgroup reduction
+    // dL/dep; no early return below: every lane reaches the wavefront reduction
     const R lx = gep[3 * i] + hx - ox, ly = gep[3 * i + 1] + hy - oy, lz = gep[3 * i + 2] + hz - oz;
     const R lu[3] = {lamu[3 * i], lamu[3 * i + 1], lamu[3 * i + 2]};
+    const SiteK<R> sk = site_k<R, LEAN>(S, d, st);
     R gw[3] = {R(0), R(0), R(0)};
-    if (!LEAN && d.integ == MX_DEPONDT && !(d.pinned && d.pinned[i]))
+    if (!LEAN && d.integ == MX_DEPONDT && !sk.pin)
       for (int q = 0; q < 3; ++q) gw[q] = gwb[3 * i + q];
+    SiteRec<R> v;
+    load_rec(r, i, v);
     R n1[3], pt[3];
-    adj_pred_v<R, LEAN>(S, d, st, r, lx, ly, lz, lu, gw, n1, pt);
+    adj_pred_v<R, LEAN>(d, sk, v, lx, ly, lz, lu, gw, n1, pt);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       nu1[3 * i + q] = n1[q];
@@ -659,45 +692,52 @@ __global__ __launch_bounds__(BLOCK) void k_adj_seg(const Static<R>* __restrict__
   const uint32_t* abort = sy.ctl + 3;
   const size_t frows = size_t(gridDim.x) * WAVES * 3;
   double run = partial_row<NP>(gpart);
+  auto rec_of = [&](int n) {
+    return StepRec<R>{seg + size_t(n) * N3, seg + (size_t(C) + n) * N3, seg + (2 * size_t(C) + n) * N3,
+                      seg + (3 * size_t(C) + n) * N3, seg + (4 * size_t(C) + n) * N3};
+  };
+  // the site's constants, lambda and the first step's record; each later record is loaded a step ahead
   R lx = R(0), ly = R(0), lz = R(0);
+  SiteK<R> sk{R(0), R(0), false};
+  SiteRec<R> v = {};
+  double hmu = 0.0;
   if (live) {
+    sk = site_k<R, LEAN>(S, d, st);
+    if (gf) hmu = 0.5 / double(d.inv_mu ? d.inv_mu[i] : S->inv_mu_sub[st.s]);
     lx = lam[3 * i];
     ly = lam[3 * i + 1];
     lz = lam[3 * i + 2];
+    load_rec(rec_of(n_hi - 1), i, v);
   }
   bool ok = true;
   uint32_t k = tag0;
   for (int n = n_hi - 1; n >= n_lo; --n, k += 2) {
-    const StepRec<R> r{seg + size_t(n) * N3, seg + (size_t(C) + n) * N3, seg + (2 * size_t(C) + n) * N3,
-                       seg + (3 * size_t(C) + n) * N3, seg + (4 * size_t(C) + n) * N3};
+    const StepRec<R> r = rec_of(n);
     double gp[NP];
 #pragma unroll
     for (int p = 0; p < NP; ++p) gp[p] = 0.0;
     double fv[3] = {0.0, 0.0, 0.0};
     R pt[3] = {R(0), R(0), R(0)}, n1[3] = {R(0), R(0), R(0)};
-    double hmu = 0.0;
     // ---- local part, then the nu2 gather and the predictor's adjoint (k_adj_a / k_adj_c, k_adj_b)
     if (live) {
       R lu[3], ge[3], n2[3], gw[3];
-      adj_local_v<R, LEAN>(S, d, st, r, lx, ly, lz, lu, ge, n2, gw);
+      adj_local_v<R, LEAN>(d, sk, v, lx, ly, lz, lu, ge, n2, gw);
       publish_tagged<R>(recA + size_t(i) * NW, n2[0], n2[1], n2[2], k);
       if (nuh)
 #pragma unroll
         for (int q = 0; q < 3; ++q) nuh[(size_t(C) + n) * N3 + 3 * i + q] = n2[q];
       if (gf) {
-        hmu = 0.5 / double(d.inv_mu ? d.inv_mu[i] : S->inv_mu_sub[st.s]);
         fv[0] = hmu * double(n2[0]);
         fv[1] = hmu * double(n2[1]);
         fv[2] = hmu * double(n2[2]);
       }
-      const R px = r.ep[3 * i], py = r.ep[3 * i + 1], pz = r.ep[3 * i + 2];
       R hx, hy, hz, ox, oy, oz;
       const bool g = adj_gather<R, GEO, NP, CM>(S, A, d, TaggedFrom{recA, k, sy.spin, abort}, r.ep, st, n2[0], n2[1],
-                                                n2[2], px, py, pz, hx, hy, hz, gp);
+                                                n2[2], v.ep[0], v.ep[1], v.ep[2], hx, hy, hz, gp);
       ok = ok && g;
-      onsite_T<R, LEAN>(S, d, st, px, py, pz, n2[0], n2[1], n2[2], ox, oy, oz);
+      onsite_T<R, LEAN>(S, d, st, v.ep[0], v.ep[1], v.ep[2], n2[0], n2[1], n2[2], ox, oy, oz);
       const R mx = ge[0] + hx - ox, my = ge[1] + hy - oy, mz = ge[2] + hz - oz;
-      adj_pred_v<R, LEAN>(S, d, st, r, mx, my, mz, lu, gw, n1, pt);
+      adj_pred_v<R, LEAN>(d, sk, v, mx, my, mz, lu, gw, n1, pt);
       publish_tagged<R>(recB + size_t(i) * NW, n1[0], n1[1], n1[2], k + 1);
       if (nuh)
 #pragma unroll
@@ -705,28 +745,30 @@ __global__ __launch_bounds__(BLOCK) void k_adj_seg(const Static<R>* __restrict__
     }
     add_partials<NP>(gp, run);
     store_field_partials(fv, gf ? gf + (size_t(n) * 2 + 1) * frows : nullptr);
-    // ---- the nu1 gather: lambda of the step's input (k_adj_c)
+    // ---- the nu1 gather: lambda of the step's input (k_adj_c); the next step's record is in flight
 #pragma unroll
     for (int p = 0; p < NP; ++p) gp[p] = 0.0;
     fv[0] = fv[1] = fv[2] = 0.0;
+    SiteRec<R> vn = v;
     if (live) {
+      if (n > n_lo) load_rec(rec_of(n - 1), i, vn);
       if (gf) {
         fv[0] = hmu * double(n1[0]);
         fv[1] = hmu * double(n1[1]);
         fv[2] = hmu * double(n1[2]);
       }
-      const R ex = r.e[3 * i], ey = r.e[3 * i + 1], ez = r.e[3 * i + 2];
       R hx, hy, hz, ox, oy, oz;
       const bool g = adj_gather<R, GEO, NP, CM>(S, A, d, TaggedFrom{recB, k + 1, sy.spin, abort}, r.e, st, n1[0],
-                                                n1[1], n1[2], ex, ey, ez, hx, hy, hz, gp);
+                                                n1[1], n1[2], v.e[0], v.e[1], v.e[2], hx, hy, hz, gp);
       ok = ok && g;
-      onsite_T<R, LEAN>(S, d, st, ex, ey, ez, n1[0], n1[1], n1[2], ox, oy, oz);
+      onsite_T<R, LEAN>(S, d, st, v.e[0], v.e[1], v.e[2], n1[0], n1[1], n1[2], ox, oy, oz);
       lx = pt[0] + hx - ox;
       ly = pt[1] + hy - oy;
       lz = pt[2] + hz - oz;
     }
     add_partials<NP>(gp, run);
     store_field_partials(fv, gf ? gf + size_t(n) * 2 * frows : nullptr);
+    v = vn;
   }
   if (live) {
     lam[3 * i] = lx;
