@@ -374,7 +374,9 @@ def adjoint_rates(steps=500, checkpoint_every=50, reps=3):
     (4, -0.4, 1.0, 0.2) meV, 500 damped Heun steps (alpha = 0.3, dt = 1 fs, T = 0) from a seeded random
     start, MSE to the texture of (5, -0.5, 1.5, 0.3) meV, checkpoints every 50 steps, FP32.  forward =
     the recorded forward (mx_adjoint_forward), backward = segment replays + adjoint sweep
-    (mx_adjoint_backward); both timed with HIP events on the stream they run on, best of reps."""
+    (mx_adjoint_backward; each replay runs beside the previous segment's sweep on a second stream, and
+    the backward's events on the caller's stream enclose both); both timed with HIP events on the stream
+    they run on, best of reps."""
     import numpy as np
     import torch
     from magex_amd import configs

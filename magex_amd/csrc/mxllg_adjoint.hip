@@ -17,7 +17,8 @@
 //                   dL/dT_b += nu_i x_j^T + x_i nu_j^T;  per site dL/dA_i += -nu_i x_i^T.
 // The forward run keeps the spins every C steps; the backward replays each segment from its
 // checkpoint (saving e, ep, w = e + dt/2 k1, B(e), B(ep) per step, noise regenerated from the same counters) and
-// runs three kernels per step: one local, two gathers.
+// sweeps it: three kernels per step (one local, two gathers), or on grids the GPU holds at once one
+// persistent launch per segment, with the next segment's replay running beside it on a second stream.
 #include <cmath>
 #include <cstring>
 #include <algorithm>
@@ -670,7 +671,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_c(const Static<R>* __restrict__ S
 // nu1 of step n of all its neighbours, each of which publishes that only after reading the site's nu2
 // of step n (the stencil is symmetric), and likewise for nu1.  Every wait is for an earlier phase of the
 // sweep and every workgroup is resident, so the launch always completes; polls are bounded as the
-// forward's, and an aborted launch is counted the same way.
+// forward's, and an aborted launch is counted in the model's count of given-up waits (gave_up).
 // Same site map, arithmetic and reduction order as k_adj_a, k_adj_b and k_adj_c: the same result bits,
 // partial rows included.  seg: the records, C steps apart (e, ep, w, B1, B2); nuh (dense gradients, or
 // null): nu1 and nu2 of step n also to nuh[n] and nuh[C + n]; gf (or null): field-gradient rows of the
@@ -680,7 +681,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_seg(const Static<R>* __restrict__
                                                    const Dyn<R> d, const R* __restrict__ seg, int C, int n_hi,
                                                    int n_lo, R* __restrict__ lam, R* __restrict__ nuh,
                                                    double* __restrict__ gpart, double* __restrict__ gf,
-                                                   uint64_t* __restrict__ rec, uint32_t tag0, const StepSync sy) {
+                                                   uint64_t* __restrict__ rec, uint32_t tag0, const StepSync sy,
+                                                   uint32_t* __restrict__ gave_up) {
   __shared__ int sh_flag;
   constexpr int NW = TAG_WORDS<R>;
   Site st;
@@ -788,7 +790,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_seg(const Static<R>* __restrict__
   if (!sh_flag || threadIdx.x != 0) return;
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   if (__hip_atomic_load(sy.ctl + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-    __hip_atomic_fetch_add(sy.ctl + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(gave_up, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __hip_atomic_store(sy.ctl + 3, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __hip_atomic_store(sy.ctl, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -971,6 +973,19 @@ struct mx_adjoint {
   int64_t gf_rows = 0;
   bool want_field = false;
   int64_t sweep_step0 = 0;  // global step of the first record of the segment being swept
+  // overlapped backward (small grids): a second segment buffer, replayed on a stream of its own while the
+  // other is swept, that stream and its events
+  void* seg2_d = nullptr;
+  int seg2_cap = 0;
+  int64_t seg2_k = -1;  // segment whose records seg2_d holds (-1: none)
+  hipStream_t s2 = nullptr;
+  hipEvent_t ev_start = nullptr, ev_rep[2] = {nullptr, nullptr}, ev_sweep[2] = {nullptr, nullptr};
+  // the persistent sweep's own tagged records (2 x N x TAG_WORDS words) and control words, apart from
+  // the model's, which the replays use at the same time
+  uint64_t* srec_d = nullptr;
+  uint32_t* sctl_d = nullptr;
+  uint32_t srec_epoch = 0;
+  bool side_by_side = false;  // replays and sweeps now run beside each other: ordinary launches
 };
 
 namespace {
@@ -1087,14 +1102,16 @@ void copy_dev(const void* src, void* dst, size_t nelem, hipStream_t s) {
 void ensure_segment(mx_adjoint& a, int n) {
   if (n <= a.seg_cap) return;
   afree(a.seg_d);
-  a.seg_cap = 0;
-  a.seg_k = -1;
+  afree(a.seg2_d);
+  a.seg_cap = a.seg2_cap = 0;
+  a.seg_k = a.seg2_k = -1;
   aok(hipMalloc(&a.seg_d, 5 * size_t(n) * nbytes3(*a.m)), "hipMalloc segment records");
   a.seg_cap = n;
 }
 
-char* seg_slot(mx_adjoint& a, int which, int n) {
-  return static_cast<char*>(a.seg_d) + (size_t(which) * a.seg_cap + n) * nbytes3(*a.m);
+// Record (which, step n) of the segment buffer buf (seg_d or seg2_d, both seg_cap steps long).
+char* seg_slot(mx_adjoint& a, void* buf, int which, int n) {
+  return static_cast<char*>(buf) + (size_t(which) * a.seg_cap + n) * nbytes3(*a.m);
 }
 
 void to_internal(mx_adjoint& a, const void* src, void* dst, hipStream_t s) {
@@ -1113,33 +1130,34 @@ void to_magex(mx_adjoint& a, const void* src, void* dst, hipStream_t s) {
   else copy_dev<float>(src, dst, nel, s);
 }
 
-// Runs segment k (n_seg steps) from the magex-order spins src with the single-pass kernels, saving
-// every step's record; the final spins stay in ework_d (internal order).
-void record_segment(mx_adjoint& a, const void* src, int64_t k, int n_seg, hipStream_t s) {
+// Runs segment k (n_seg steps) from the magex-order spins src, saving every step's record into the
+// segment buffer buf (seg_d or seg2_d) on stream s; the final spins stay in ework_d (internal order).
+void record_segment(mx_adjoint& a, const void* src, int64_t k, int n_seg, hipStream_t s, void* buf) {
   Model& m = *a.m;
   void* ew = a.ework_d;
+  (buf == a.seg_d ? a.seg_k : a.seg2_k) = -1;
   to_internal(a, src, ew, s);
   // undriven segments of two or more steps on a model that takes persistent launches: one launch writes
   // the same records, bit for bit, as the two single-pass kernels per step below
   if (a.drive.empty() && n_seg >= 2 && persistent_steps(m) && ensure_step_scratch(m) == MX_OK) {
-    const StepRecOut ro{seg_slot(a, 0, 0), seg_slot(a, 1, 0), seg_slot(a, 2, 0), seg_slot(a, 3, 0),
-                        seg_slot(a, 4, 0)};
-    aok_rc(launch_heun_steps(m, ew, a.seed, a.step0 + uint64_t(k * a.C), n_seg, nullptr, s, nullptr, &ro),
+    const StepRecOut ro{seg_slot(a, buf, 0, 0), seg_slot(a, buf, 1, 0), seg_slot(a, buf, 2, 0),
+                        seg_slot(a, buf, 3, 0), seg_slot(a, buf, 4, 0)};
+    aok_rc(launch_heun_steps(m, ew, a.seed, a.step0 + uint64_t(k * a.C), n_seg, nullptr, s, nullptr, &ro,
+                             a.side_by_side),
            "recorded segment");
-    a.seg_k = k;
-    return;
+  } else {
+    for (int n = 0; n < n_seg; ++n) {
+      const uint64_t step = a.step0 + uint64_t(k * a.C + n);
+      const double* B = a.drive.empty() ? nullptr : a.drive.data() + 6 * size_t(k * a.C + n);
+      aok_rc(launch_heun_pass(m, 1, ew, seg_slot(a, buf, 1, n), seg_slot(a, buf, 2, n), a.seed, step,
+                              seg_slot(a, buf, 0, n), seg_slot(a, buf, 3, n), s, B),
+             "segment pass 1");
+      aok_rc(launch_heun_pass(m, 2, ew, seg_slot(a, buf, 1, n), seg_slot(a, buf, 2, n), a.seed, step, nullptr,
+                              seg_slot(a, buf, 4, n), s, B ? B + 3 : nullptr),
+             "segment pass 2");
+    }
   }
-  for (int n = 0; n < n_seg; ++n) {
-    const uint64_t step = a.step0 + uint64_t(k * a.C + n);
-    const double* B = a.drive.empty() ? nullptr : a.drive.data() + 6 * size_t(k * a.C + n);
-    aok_rc(launch_heun_pass(m, 1, ew, seg_slot(a, 1, n), seg_slot(a, 2, n), a.seed, step, seg_slot(a, 0, n),
-                            seg_slot(a, 3, n), s, B),
-           "segment pass 1");
-    aok_rc(launch_heun_pass(m, 2, ew, seg_slot(a, 1, n), seg_slot(a, 2, n), a.seed, step, nullptr,
-                            seg_slot(a, 4, n), s, B ? B + 3 : nullptr),
-           "segment pass 2");
-  }
-  a.seg_k = k;
+  (buf == a.seg_d ? a.seg_k : a.seg2_k) = k;
 }
 
 void ensure_buffers(mx_adjoint& a) {
@@ -1154,35 +1172,55 @@ void ensure_buffers(mx_adjoint& a) {
 }
 
 
+// The persistent sweep's own tagged records and control words (zeroed when allocated).
+void ensure_sweep_sync(mx_adjoint& a) {
+  const Model& m = *a.m;
+  if (!a.srec_d) {
+    const size_t n = 2 * size_t(m.N) * (m.prec == MX_FP64 ? TAG_WORDS<double> : TAG_WORDS<float>) * sizeof(uint64_t);
+    aok(hipMalloc(reinterpret_cast<void**>(&a.srec_d), n), "hipMalloc sweep records");
+    aok(hipMemset(a.srec_d, 0, n), "hipMemset sweep records");
+  }
+  if (!a.sctl_d) {
+    aok(hipMalloc(reinterpret_cast<void**>(&a.sctl_d), CTL_WORDS * sizeof(uint32_t)), "hipMalloc sweep control");
+    aok(hipMemset(a.sctl_d, 0, CTL_WORDS * sizeof(uint32_t)), "hipMemset sweep control");
+  }
+}
+
 // The whole sweep of a segment of two or more steps in persistent launches of up to STEPS_PER_LAUNCH
-// steps (k_adj_seg), on models that take the persistent forward launch (whose tagged records and control
-// words it shares: consecutive launches on them take different epochs); false when the model does not.
+// steps (k_adj_seg), on models that take the persistent forward launch; false when the model does not.
+// Its tagged records and control words are its own (replays of the next segment may run beside it on
+// the model's), consecutive launches take different epochs, and an aborted launch is counted with the
+// model's given-up waits.
 template <typename R, int GEO, int NP, int CM, bool LEAN>
 bool sweep_persistent(mx_adjoint& a, const Dyn<R>& d, const R* seg, int C, int n_seg, R* nuh, hipStream_t s) {
   if constexpr (GEO != GEO_PERIODIC && GEO != GEO_OPEN) {
     return false;
   } else {
     Model& m = *a.m;
-    if (n_seg < 2 || !persistent_steps(m) || ensure_step_scratch(m) != MX_OK || !m.rec_d || !m.stepsync_d)
-      return false;
+    if (n_seg < 2 || !persistent_steps(m) || ensure_step_scratch(m) != MX_OK || !m.stepsync_d) return false;
+    ensure_sweep_sync(a);
     const Static<R>* S = static_cast<const Static<R>*>(m.static_d);
     const AStatic<R>* A = static_cast<const AStatic<R>*>(a.astatic_d);
     StepSync sy{};
-    sy.cnt = m.stepsync_d;
-    sy.ctl = m.stepsync_d + stepsync_ctl(m);
-    sy.gu = reinterpret_cast<int32_t*>(sy.ctl + CTL_WORDS);
+    sy.ctl = a.sctl_d;
     sy.spin = 1u << 16;
+    uint32_t* gave_up = m.stepsync_d + stepsync_ctl(m) + 2;
     R* lam = static_cast<R*>(a.lam_d);
     double* gpart = a.gpart_d;
     double* gf = a.want_field ? a.gf_d + size_t(a.sweep_step0) * 2 * a.nblk * WAVES * 3 : nullptr;
-    uint64_t* rec = m.rec_d;
+    uint64_t* rec = a.srec_d;
     static_assert(2 * STEPS_PER_LAUNCH + 2 < (1 << 12), "sweep tags overflow their 12 bits");
     for (int hi = n_seg; hi > 0; hi -= STEPS_PER_LAUNCH) {
       int lo = std::max(0, hi - STEPS_PER_LAUNCH);
-      m.rec_epoch = m.rec_epoch % 15u + 1u;
-      uint32_t tag0 = m.rec_epoch << 12;
-      void* args[] = {&S, &A, const_cast<Dyn<R>*>(&d), &seg, &C, &hi, &lo, &lam, &nuh, &gpart, &gf, &rec, &tag0, &sy};
-      aok(hipLaunchCooperativeKernel(k_adj_seg<R, GEO, NP, CM, LEAN>, dim3(a.nblk), dim3(BLOCK), args, 0, s),
+      a.srec_epoch = a.srec_epoch % 15u + 1u;
+      uint32_t tag0 = a.srec_epoch << 12;
+      void* args[] = {&S, &A, const_cast<Dyn<R>*>(&d), &seg, &C,  &hi,  &lo,  &lam,
+                      &nuh, &gpart, &gf, &rec, &tag0, &sy, &gave_up};
+      // beside a replay, an ordinary launch: the runtime runs cooperative launches one at a time, and
+      // both grids fit on the GPU together (overlap_backward)
+      const void* fn = reinterpret_cast<const void*>(k_adj_seg<R, GEO, NP, CM, LEAN>);
+      aok(a.side_by_side ? hipLaunchKernel(fn, dim3(a.nblk), dim3(BLOCK), args, 0, s)
+                         : hipLaunchCooperativeKernel(fn, dim3(a.nblk), dim3(BLOCK), args, 0, s),
           "persistent adjoint sweep");
     }
     return true;
@@ -1315,6 +1353,30 @@ void ensure_dense(mx_adjoint& a, hipStream_t s) {
   }
 }
 
+// Small grids (persistent launches, two of which fit on the GPU at once) replay the next segment on a
+// stream of their own while the current one is swept: both are latency-bound launches of a few dozen
+// workgroups, so running them side by side hides the replays.
+bool overlap_backward(const Model& m) {
+  if (!persistent_steps(m)) return false;
+  int dev_cu = 0;
+  if (hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, m.device) != hipSuccess) return false;
+  return 2 * (m.ns * ((m.ncell + BLOCK - 1) / BLOCK)) <= dev_cu;
+}
+
+// The second segment buffer, the replay stream (non-blocking, default priority) and the events.
+void ensure_overlap(mx_adjoint& a) {
+  if (!a.seg2_d || a.seg2_cap != a.seg_cap) {
+    afree(a.seg2_d);
+    a.seg2_cap = 0;
+    a.seg2_k = -1;
+    aok(hipMalloc(&a.seg2_d, 5 * size_t(a.seg_cap) * nbytes3(*a.m)), "hipMalloc second segment records");
+    a.seg2_cap = a.seg_cap;
+  }
+  if (!a.s2) aok(hipStreamCreateWithFlags(&a.s2, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+  for (hipEvent_t* e : {&a.ev_start, &a.ev_rep[0], &a.ev_rep[1], &a.ev_sweep[0], &a.ev_sweep[1]})
+    if (!*e) aok(hipEventCreateWithFlags(e, hipEventDisableTiming), "hipEventCreateWithFlags");
+}
+
 // The adjoint sweep over every segment, newest first; with grad_bonds / grad_onsite also the dense
 // gradients (either may be null).
 int backward_impl(mx_adjoint* a, const void* grad_e, void* grad_e0, double* grad_theta, double* grad_bonds,
@@ -1345,13 +1407,44 @@ int backward_impl(mx_adjoint* a, const void* grad_e, void* grad_e0, double* grad
                      size_t(MAXP) * nrow);
   char* ck = static_cast<char*>(a->ckpt_d);
   const int C = a->C;
-  for (int64_t k = (a->nsteps - 1) / C; k >= 0 && a->nsteps > 0; --k) {
-    const int n_seg = int(std::min<int64_t>(C, a->nsteps - k * C));
-    // replay the segment from its checkpoint unless its records are still held
-    if (a->seg_k != k) record_segment(*a, ck + size_t(k) * b, k, n_seg, s);
+  const int64_t K = a->nsteps > 0 ? (a->nsteps - 1) / C + 1 : 0;  // segments
+  auto n_of = [&](int64_t k) { return int(std::min<int64_t>(C, a->nsteps - k * C)); };
+  auto sweep_from = [&](int64_t k, void* buf) {
     a->sweep_step0 = k * C;
-    if (fp64) sweep<double>(*a, static_cast<char*>(a->seg_d), b, a->seg_cap, n_seg, dense, s);
-    else sweep<float>(*a, static_cast<char*>(a->seg_d), b, a->seg_cap, n_seg, dense, s);
+    if (fp64) sweep<double>(*a, static_cast<char*>(buf), b, a->seg_cap, n_of(k), dense, s);
+    else sweep<float>(*a, static_cast<char*>(buf), b, a->seg_cap, n_of(k), dense, s);
+  };
+  if (K >= 2 && overlap_backward(m)) {
+    // segment k is swept from buffer (K - 1 - k) % 2 on s while segment k - 1 is replayed into the other
+    // buffer on s2; a buffer is replayed into only once the sweep that read it (two segments later) has
+    // ended, and a sweep starts only once its replay has.  The replays share ework_d, in order on s2.
+    ensure_overlap(*a);
+    void* buf[2] = {a->seg_d, a->seg2_d};
+    if (a->seg_k != K - 1) record_segment(*a, ck + size_t(K - 1) * b, K - 1, n_of(K - 1), s, buf[0]);
+    aok(hipEventRecord(a->ev_start, s), "hipEventRecord");
+    aok(hipStreamWaitEvent(a->s2, a->ev_start, 0), "hipStreamWaitEvent");
+    struct SideBySide {  // set for the loop, cleared on every exit
+      mx_adjoint& a;
+      explicit SideBySide(mx_adjoint& x) : a(x) { a.side_by_side = true; }
+      ~SideBySide() { a.side_by_side = false; }
+    } side(*a);
+    for (int64_t k = K - 1; k >= 0; --k) {
+      const int cur = int((K - 1 - k) & 1), nxt = cur ^ 1;
+      if (k >= 1) {
+        if (k + 1 <= K - 1) aok(hipStreamWaitEvent(a->s2, a->ev_sweep[nxt], 0), "hipStreamWaitEvent");
+        record_segment(*a, ck + size_t(k - 1) * b, k - 1, n_of(k - 1), a->s2, buf[nxt]);
+        aok(hipEventRecord(a->ev_rep[nxt], a->s2), "hipEventRecord");
+      }
+      if (k < K - 1) aok(hipStreamWaitEvent(s, a->ev_rep[cur], 0), "hipStreamWaitEvent");
+      sweep_from(k, buf[cur]);
+      aok(hipEventRecord(a->ev_sweep[cur], s), "hipEventRecord");
+    }
+  } else {
+    for (int64_t k = K - 1; k >= 0; --k) {
+      // replay the segment from its checkpoint unless its records are still held
+      if (a->seg_k != k) record_segment(*a, ck + size_t(k) * b, k, n_of(k), s, a->seg_d);
+      sweep_from(k, a->seg_d);
+    }
   }
   hipLaunchKernelGGL((k_final<MAXP>), dim3(1), dim3(BLOCK), 0, s, a->gpart_d, nrow, MAXP, 1.0, a->gout_d);
   hipLaunchKernelGGL((k_copy<double>), dim3(1), dim3(64), 0, s, a->gout_d, grad_theta, size_t(a->P));
@@ -1454,8 +1547,12 @@ void mx_adjoint_destroy(mx_adjoint* a) {
                    &a->part_d, &a->ework_d, reinterpret_cast<void**>(&a->gpart_d),
                    reinterpret_cast<void**>(&a->gout_d), reinterpret_cast<void**>(&a->gb_d),
                    reinterpret_cast<void**>(&a->ga_d), &a->nuh_d, reinterpret_cast<void**>(&a->cmap_d),
-                   reinterpret_cast<void**>(&a->gf_d), &a->gw_d})
+                   reinterpret_cast<void**>(&a->gf_d), &a->gw_d, &a->seg2_d, reinterpret_cast<void**>(&a->srec_d),
+                   reinterpret_cast<void**>(&a->sctl_d)})
     afree(*p);
+  for (hipEvent_t e : {a->ev_start, a->ev_rep[0], a->ev_rep[1], a->ev_sweep[0], a->ev_sweep[1]})
+    if (e) (void)hipEventDestroy(e);
+  if (a->s2) (void)hipStreamDestroy(a->s2);
   delete a;
 }
 
@@ -1510,7 +1607,7 @@ int forward_impl(mx_adjoint* a, void* e, int64_t nsteps, const double* B, uint64
     if (n <= 0) continue;
     if (k == k_last) {
       aok_rc(sync_model_static(m), "model constants");
-      record_segment(*a, e, k, int(n), s);
+      record_segment(*a, e, k, int(n), s, a->seg_d);
       to_magex(*a, a->ework_d, e, s);
     } else if (B) {
       aok_rc(mx_llg_run_driven(a->m, e, n, B + 6 * size_t(k * a->C), seed, step0 + uint64_t(k * a->C), stream),

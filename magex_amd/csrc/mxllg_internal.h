@@ -220,12 +220,16 @@ bool persistent_steps(const Model& m);
 // the step's input spins e, its predictor spins ep, w = e + dt/2 k1 (Depondt: the rotation vector), and the
 // effective fields of both passes (zero on pinned sites, whose w is their spin): what the adjoint's replays
 // store with the single-pass kernels.
+// plain: an ordinary launch instead of a cooperative one, for a caller that runs it beside one other
+// small persistent launch (the runtime runs cooperative launches one at a time); both grids must then fit
+// on the GPU together.
 struct StepRecOut {
   void *e, *ep, *w, *b1, *b2;
 };
 constexpr int STEPS_PER_LAUNCH = 1024;
 int launch_heun_steps(const Model& m, void* e, uint64_t seed, uint64_t step0, int64_t nsteps, const void* noise,
-                      hipStream_t s, const StepBufs* bufs = nullptr, const StepRecOut* rec = nullptr);
+                      hipStream_t s, const StepBufs* bufs = nullptr, const StepRecOut* rec = nullptr,
+                      bool plain = false);
 // The model's own step scratch (predictor buffers, control words, tagged records), allocated if missing
 // (MX_OK or an error code with the message set).
 int ensure_step_scratch(Model& m);

@@ -161,26 +161,32 @@ bool persistent_steps(const Model& m) {
 
 template <typename R, int CM, int GEO, bool LEAN>
 static hipError_t steps_coop(dim3 g, hipStream_t s, const Static<R>* S, Dyn<R> d, R* E, uint64_t* rec,
-                             uint64_t step0, int n, const R* nz, uint32_t tag0, StepSync sy, RecPtrs<R> ro) {
+                             uint64_t step0, int n, const R* nz, uint32_t tag0, StepSync sy, RecPtrs<R> ro,
+                             bool plain) {
   void* args[] = {&S, &d, &E, &rec, &step0, &n, &nz, &tag0, &sy, &ro};
+  if (plain)
+    return hipLaunchKernel(reinterpret_cast<const void*>(k_steps<R, CM, GEO, LEAN>), g, dim3(BLOCK), args, 0, s);
   return hipLaunchCooperativeKernel(k_steps<R, CM, GEO, LEAN>, g, dim3(BLOCK), args, 0, s);
 }
 
 template <typename R, int GEO, bool LEAN>
 static hipError_t steps_dispatch(const Model& m, dim3 g, hipStream_t s, const Static<R>* S, const Dyn<R>& d, R* E,
                                  uint64_t* rec, uint64_t step0, int n, const R* nz, uint32_t tag0,
-                                 const StepSync& sy, const RecPtrs<R>& ro) {
+                                 const StepSync& sy, const RecPtrs<R>& ro, bool plain) {
   switch (m.coup) {
-    case COUP_TABLE1: return steps_coop<R, COUP_TABLE1, GEO, LEAN>(g, s, S, d, E, rec, step0, n, nz, tag0, sy, ro);
-    case COUP_TABLE4: return steps_coop<R, COUP_TABLE4, GEO, LEAN>(g, s, S, d, E, rec, step0, n, nz, tag0, sy, ro);
-    case COUP_TABLE9: return steps_coop<R, COUP_TABLE9, GEO, LEAN>(g, s, S, d, E, rec, step0, n, nz, tag0, sy, ro);
-    default: return steps_coop<R, COUP_BOND, GEO, LEAN>(g, s, S, d, E, rec, step0, n, nz, tag0, sy, ro);
+    case COUP_TABLE1:
+      return steps_coop<R, COUP_TABLE1, GEO, LEAN>(g, s, S, d, E, rec, step0, n, nz, tag0, sy, ro, plain);
+    case COUP_TABLE4:
+      return steps_coop<R, COUP_TABLE4, GEO, LEAN>(g, s, S, d, E, rec, step0, n, nz, tag0, sy, ro, plain);
+    case COUP_TABLE9:
+      return steps_coop<R, COUP_TABLE9, GEO, LEAN>(g, s, S, d, E, rec, step0, n, nz, tag0, sy, ro, plain);
+    default: return steps_coop<R, COUP_BOND, GEO, LEAN>(g, s, S, d, E, rec, step0, n, nz, tag0, sy, ro, plain);
   }
 }
 
 template <typename R>
 static int steps_t(const Model& m, void* e, uint64_t seed, uint64_t step0, int64_t nsteps, const void* noise,
-                   hipStream_t s, const StepBufs* bufs, const StepRecOut* rec) {
+                   hipStream_t s, const StepBufs* bufs, const StepRecOut* rec, bool plain) {
   uint32_t* sync = bufs ? bufs->sync : m.stepsync_d;
   uint64_t* recs = bufs ? bufs->rec : m.rec_d;
   if (!sync || !recs || !persistent_steps(m)) {
@@ -219,11 +225,11 @@ static int steps_t(const Model& m, void* e, uint64_t seed, uint64_t step0, int64
     const uint32_t tag0 = m.rec_epoch << 12;
     hipError_t err;
     if (m.geo == GEO_PERIODIC)
-      err = lean ? steps_dispatch<R, GEO_PERIODIC, true>(m, g, s, S, d, E, recs, st, k, nz, tag0, sy, ro)
-                 : steps_dispatch<R, GEO_PERIODIC, false>(m, g, s, S, d, E, recs, st, k, nz, tag0, sy, ro);
+      err = lean ? steps_dispatch<R, GEO_PERIODIC, true>(m, g, s, S, d, E, recs, st, k, nz, tag0, sy, ro, plain)
+                 : steps_dispatch<R, GEO_PERIODIC, false>(m, g, s, S, d, E, recs, st, k, nz, tag0, sy, ro, plain);
     else
-      err = lean ? steps_dispatch<R, GEO_OPEN, true>(m, g, s, S, d, E, recs, st, k, nz, tag0, sy, ro)
-                 : steps_dispatch<R, GEO_OPEN, false>(m, g, s, S, d, E, recs, st, k, nz, tag0, sy, ro);
+      err = lean ? steps_dispatch<R, GEO_OPEN, true>(m, g, s, S, d, E, recs, st, k, nz, tag0, sy, ro, plain)
+                 : steps_dispatch<R, GEO_OPEN, false>(m, g, s, S, d, E, recs, st, k, nz, tag0, sy, ro, plain);
     if (err != hipSuccess) {
       set_error(std::string("persistent heun steps: ") + hipGetErrorString(err));
       return MX_EHIP;
@@ -233,9 +239,9 @@ static int steps_t(const Model& m, void* e, uint64_t seed, uint64_t step0, int64
 }
 
 int launch_heun_steps(const Model& m, void* e, uint64_t seed, uint64_t step0, int64_t nsteps, const void* noise,
-                      hipStream_t s, const StepBufs* bufs, const StepRecOut* rec) {
-  return m.prec == MX_FP64 ? steps_t<double>(m, e, seed, step0, nsteps, noise, s, bufs, rec)
-                           : steps_t<float>(m, e, seed, step0, nsteps, noise, s, bufs, rec);
+                      hipStream_t s, const StepBufs* bufs, const StepRecOut* rec, bool plain) {
+  return m.prec == MX_FP64 ? steps_t<double>(m, e, seed, step0, nsteps, noise, s, bufs, rec, plain)
+                           : steps_t<float>(m, e, seed, step0, nsteps, noise, s, bufs, rec, plain);
 }
 
 }  // namespace mx
