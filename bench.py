@@ -29,6 +29,8 @@ sys.path.insert(0, ROOT)
 # the CPU baseline runs one OpenMP thread per core, pinned; set before any OpenMP runtime starts
 os.environ.setdefault("OMP_PROC_BIND", "close")
 os.environ.setdefault("OMP_PLACES", "cores")
+# the CPUs this process may run on, read before OpenMP pins the main thread to one of them
+CPUS_USABLE = len(os.sched_getaffinity(0))
 
 METRIC = "spin-steps/sec + % HBM roofline (FP32 vs FP64); batched speedup; grad cost/forward"
 HBM_PEAK_GBS = 8000.0
@@ -167,9 +169,9 @@ def cpu_model():
 
 def cpu_topology():
     """Model, sockets, physical cores and threads per core of this host (lscpu), plus the CPUs this
-    process may run on."""
+    process may run on (read at start-up)."""
     import subprocess
-    info = {"cpus_usable": len(os.sched_getaffinity(0))}
+    info = {"cpus_usable": CPUS_USABLE}
     try:
         txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
     except (OSError, subprocess.SubprocessError):
