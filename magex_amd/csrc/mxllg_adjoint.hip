@@ -185,7 +185,9 @@ struct NuArray {
 // end re-read the last one with weight zero), so a wavefront keeps 2 CH gathers in flight.  From tagged
 // records, each batch's live entries are re-read until every one carries its tag (x and the couplings
 // are in flight meanwhile); false when a lane gives up (after src.spin polls, or once *src.abort is raised).
-template <typename R, int GEO, int NP, int CM, typename Src>
+// PAR = false leaves the parameter terms out (x is then not read): the persistent sweep adds them later
+// (k_par_seg), off its chain of exchanges.
+template <typename R, int GEO, int NP, int CM, typename Src, bool PAR = true>
 __device__ __forceinline__ bool adj_gather(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
                                            const Dyn<R>& d, const Src& src, const R* __restrict__ x,
                                            const Site& st,
@@ -222,9 +224,11 @@ __device__ __forceinline__ bool adj_gather(const Static<R>* __restrict__ S, cons
         vy[u] = src.nu[3 * j + 1];
         vz[u] = src.nu[3 * j + 2];
       }
-      qx[u] = x[3 * j];
-      qy[u] = x[3 * j + 1];
-      qz[u] = x[3 * j + 2];
+      if constexpr (PAR) {
+        qx[u] = x[3 * j];
+        qy[u] = x[3 * j + 1];
+        qz[u] = x[3 * j + 2];
+      }
       if constexpr (CM == COUP_BOND) {
         const bool back = (en.tb >> 16) != 0;
         const R* T = d.bond + (size_t(en.slot) * ncell + (back ? cn : st.c)) * 9;
@@ -279,9 +283,11 @@ __device__ __forceinline__ bool adj_gather(const Static<R>* __restrict__ S, cons
       vx[u] *= wt[u];
       vy[u] *= wt[u];
       vz[u] *= wt[u];
-      qx[u] *= wt[u];
-      qy[u] *= wt[u];
-      qz[u] *= wt[u];
+      if constexpr (PAR) {
+        qx[u] *= wt[u];
+        qy[u] *= wt[u];
+        qz[u] *= wt[u];
+      }
     }
 #pragma unroll
     for (int u = 0; u < CH; ++u) {
@@ -291,6 +297,67 @@ __device__ __forceinline__ bool adj_gather(const Static<R>* __restrict__ S, cons
       hx += T[0] * vx[u] + T[1] * vy[u] + T[2] * vz[u];
       hy += T[3] * vx[u] + T[4] * vy[u] + T[5] * vz[u];
       hz += T[6] * vx[u] + T[7] * vy[u] + T[8] * vz[u];
+      if constexpr (PAR) {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+          const R* U = A->U[kk[u]][p];
+          t[p][0] += U[0] * qx[u] + U[1] * qy[u] + U[2] * qz[u];
+          t[p][1] += U[3] * qx[u] + U[4] * qy[u] + U[5] * qz[u];
+          t[p][2] += U[6] * qx[u] + U[7] * qy[u] + U[8] * qz[u];
+        }
+      }
+    }
+  }
+  if constexpr (PAR) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const R* a = A->Aun[st.s][p];
+      const R ux = t[p][0] - (a[0] * xix + a[1] * xiy + a[2] * xiz);
+      const R uy = t[p][1] - (a[3] * xix + a[4] * xiy + a[5] * xiz);
+      const R uz = t[p][2] - (a[6] * xix + a[7] * xiy + a[8] * xiz);
+      gp[p] += double(nix * ux + niy * uy + niz * uz);
+    }
+  }
+  return good;
+}
+
+// The parameter terms alone, as adj_gather sums them (the same batches, dead-entry weights, order and
+// expressions, hence the same bits): G_p = nu_i . (sum_k U[k][p] x_j - Aun[s][p] x_i) added to gp.
+template <typename R, int GEO, int NP>
+__device__ __forceinline__ void param_terms(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
+                                            const R* __restrict__ x, const Site& st, R nix, R niy, R niz, R xix,
+                                            R xiy, R xiz, double (&gp)[NP]) {
+  constexpr int CH = 8;
+  const int ncell = S->ncell;
+  R t[NP][3];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) t[p][0] = t[p][1] = t[p][2] = R(0);
+  const int kb = S->sptr[st.s], ke = S->sptr[st.s + 1];
+  for (int k0 = kb; k0 < ke; k0 += CH) {
+    R qx[CH], qy[CH], qz[CH], wt[CH];
+    int kk[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const bool live = k0 + u < ke;
+      kk[u] = live ? k0 + u : ke - 1;
+      const Ent en = S->ent[kk[u]];
+      bool ok;
+      const int cn = neighbour_cell<R, GEO>(S, st, en, ok);
+      ok = ok && live;
+      const int j = (en.tb & 0xffff) * ncell + cn;
+      qx[u] = x[3 * j];
+      qy[u] = x[3 * j + 1];
+      qz[u] = x[3 * j + 2];
+      wt[u] = ok ? R(1) : R(0);
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      qx[u] *= wt[u];
+      qy[u] *= wt[u];
+      qz[u] *= wt[u];
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
 #pragma unroll
       for (int p = 0; p < NP; ++p) {
         const R* U = A->U[kk[u]][p];
@@ -308,7 +375,6 @@ __device__ __forceinline__ bool adj_gather(const Static<R>* __restrict__ S, cons
     const R uz = t[p][2] - (a[6] * xix + a[7] * xiy + a[8] * xiz);
     gp[p] += double(nix * ux + niy * uy + niz * uz);
   }
-  return good;
 }
 
 // Saved state of one step: e, ep, w = e + dt/2 k1, B(e), B(ep), each N x 3 in internal site order.
@@ -662,8 +728,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_c(const Static<R>* __restrict__ S
 // Steps [n_lo, n_hi) of a replayed segment's adjoint sweep (newest first) in one cooperative launch
 // (every workgroup resident from the start), for models that take the persistent forward launch.  Each
 // workgroup keeps its sites' adjoint state (lambda, lambda_u, gep, the corrector's share of dL/dw1, the
-// partial dL/de) and each wavefront its row of parameter partials in registers for the whole launch (no
-// barrier between its exchanges); sites exchange
+// partial dL/de) in registers for the whole launch; sites exchange
 // nu2 and nu1 through tagged records (recA, recB: 2 x N x TAG_WORDS words) as the persistent forward
 // exchanges spins.  Per step, with k = tag0 + 2 (n_hi - 1 - n): the local part publishes nu2 (tag k),
 // the nu2 gather publishes nu1 (k + 1), the nu1 gather gives lambda of the step's input.  A record is
@@ -672,15 +737,16 @@ __global__ __launch_bounds__(BLOCK) void k_adj_c(const Static<R>* __restrict__ S
 // of step n (the stencil is symmetric), and likewise for nu1.  Every wait is for an earlier phase of the
 // sweep and every workgroup is resident, so the launch always completes; polls are bounded as the
 // forward's, and an aborted launch is counted in the model's count of given-up waits (gave_up).
-// Same site map, arithmetic and reduction order as k_adj_a, k_adj_b and k_adj_c: the same result bits,
-// partial rows included.  seg: the records, C steps apart (e, ep, w, B1, B2); nuh (dense gradients, or
-// null): nu1 and nu2 of step n also to nuh[n] and nuh[C + n]; gf (or null): field-gradient rows of the
-// segment's first step.  lambda comes from lam and goes back to it.
+// Same site map and arithmetic as k_adj_a, k_adj_b and k_adj_c: the same result bits.  The parameter
+// terms are left out of the exchanges' chain: nu1 and nu2 of step n go to nuh[n] and nuh[C + n], and
+// k_par_seg / k_par_fold add the terms afterwards as those kernels would have.  seg: the records, C steps
+// apart (e, ep, w, B1, B2); gf (or null): field-gradient rows of the segment's first step.  lambda comes
+// from lam and goes back to it.
 template <typename R, int GEO, int NP, int CM, bool LEAN>
 __global__ __launch_bounds__(BLOCK) void k_adj_seg(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
                                                    const Dyn<R> d, const R* __restrict__ seg, int C, int n_hi,
                                                    int n_lo, R* __restrict__ lam, R* __restrict__ nuh,
-                                                   double* __restrict__ gpart, double* __restrict__ gf,
+                                                   double* __restrict__ gf,
                                                    uint64_t* __restrict__ rec, uint32_t tag0, const StepSync sy,
                                                    uint32_t* __restrict__ gave_up) {
   __shared__ int sh_flag;
@@ -693,7 +759,6 @@ __global__ __launch_bounds__(BLOCK) void k_adj_seg(const Static<R>* __restrict__
   uint64_t* recB = rec + size_t(S->N) * NW;
   const uint32_t* abort = sy.ctl + 3;
   const size_t frows = size_t(gridDim.x) * WAVES * 3;
-  double run = partial_row<NP>(gpart);
   auto rec_of = [&](int n) {
     return StepRec<R>{seg + size_t(n) * N3, seg + (size_t(C) + n) * N3, seg + (2 * size_t(C) + n) * N3,
                       seg + (3 * size_t(C) + n) * N3, seg + (4 * size_t(C) + n) * N3};
@@ -715,9 +780,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_seg(const Static<R>* __restrict__
   uint32_t k = tag0;
   for (int n = n_hi - 1; n >= n_lo; --n, k += 2) {
     const StepRec<R> r = rec_of(n);
-    double gp[NP];
-#pragma unroll
-    for (int p = 0; p < NP; ++p) gp[p] = 0.0;
+    double gp[NP];  // unused: the parameter terms are added by k_par_seg
     double fv[3] = {0.0, 0.0, 0.0};
     R pt[3] = {R(0), R(0), R(0)}, n1[3] = {R(0), R(0), R(0)};
     // ---- local part, then the nu2 gather and the predictor's adjoint (k_adj_a / k_adj_c, k_adj_b)
@@ -734,8 +797,9 @@ __global__ __launch_bounds__(BLOCK) void k_adj_seg(const Static<R>* __restrict__
         fv[2] = hmu * double(n2[2]);
       }
       R hx, hy, hz, ox, oy, oz;
-      const bool g = adj_gather<R, GEO, NP, CM>(S, A, d, TaggedFrom{recA, k, sy.spin, abort}, r.ep, st, n2[0], n2[1],
-                                                n2[2], v.ep[0], v.ep[1], v.ep[2], hx, hy, hz, gp);
+      const bool g = adj_gather<R, GEO, NP, CM, TaggedFrom, false>(S, A, d, TaggedFrom{recA, k, sy.spin, abort}, r.ep,
+                                                                  st, n2[0], n2[1], n2[2], v.ep[0], v.ep[1],
+                                                                  v.ep[2], hx, hy, hz, gp);
       ok = ok && g;
       onsite_T<R, LEAN>(S, d, st, v.ep[0], v.ep[1], v.ep[2], n2[0], n2[1], n2[2], ox, oy, oz);
       const R mx = ge[0] + hx - ox, my = ge[1] + hy - oy, mz = ge[2] + hz - oz;
@@ -745,11 +809,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_seg(const Static<R>* __restrict__
 #pragma unroll
         for (int q = 0; q < 3; ++q) nuh[size_t(n) * N3 + 3 * i + q] = n1[q];
     }
-    add_partials<NP>(gp, run);
     store_field_partials(fv, gf ? gf + (size_t(n) * 2 + 1) * frows : nullptr);
     // ---- the nu1 gather: lambda of the step's input (k_adj_c); the next step's record is in flight
-#pragma unroll
-    for (int p = 0; p < NP; ++p) gp[p] = 0.0;
     fv[0] = fv[1] = fv[2] = 0.0;
     SiteRec<R> vn = v;
     if (live) {
@@ -760,15 +821,15 @@ __global__ __launch_bounds__(BLOCK) void k_adj_seg(const Static<R>* __restrict__
         fv[2] = hmu * double(n1[2]);
       }
       R hx, hy, hz, ox, oy, oz;
-      const bool g = adj_gather<R, GEO, NP, CM>(S, A, d, TaggedFrom{recB, k + 1, sy.spin, abort}, r.e, st, n1[0],
-                                                n1[1], n1[2], v.e[0], v.e[1], v.e[2], hx, hy, hz, gp);
+      const bool g = adj_gather<R, GEO, NP, CM, TaggedFrom, false>(S, A, d, TaggedFrom{recB, k + 1, sy.spin, abort},
+                                                                  r.e, st, n1[0], n1[1], n1[2], v.e[0], v.e[1],
+                                                                  v.e[2], hx, hy, hz, gp);
       ok = ok && g;
       onsite_T<R, LEAN>(S, d, st, v.e[0], v.e[1], v.e[2], n1[0], n1[1], n1[2], ox, oy, oz);
       lx = pt[0] + hx - ox;
       ly = pt[1] + hy - oy;
       lz = pt[2] + hz - oz;
     }
-    add_partials<NP>(gp, run);
     store_field_partials(fv, gf ? gf + size_t(n) * 2 * frows : nullptr);
     v = vn;
   }
@@ -777,7 +838,6 @@ __global__ __launch_bounds__(BLOCK) void k_adj_seg(const Static<R>* __restrict__
     lam[3 * i + 1] = ly;
     lam[3 * i + 2] = lz;
   }
-  if ((threadIdx.x & 63) < NP) gpart[wave_row() * MAXP + (threadIdx.x & 63)] = run;
   if (!ok) __hip_atomic_store(const_cast<uint32_t*>(abort), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   // ---- the last workgroup to finish counts an aborted launch and clears the control words
   const int bad = __syncthreads_or(ok ? 0 : 1);
@@ -793,6 +853,45 @@ __global__ __launch_bounds__(BLOCK) void k_adj_seg(const Static<R>* __restrict__
     __hip_atomic_fetch_add(gave_up, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __hip_atomic_store(sy.ctl + 3, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __hip_atomic_store(sy.ctl, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Parameter terms of every pass of a segment swept by k_adj_seg, one grid row per pass q in sweep order
+// (q = 2 (n_seg - 1 - n): the nu2 pass of step n, at ep; q + 1: its nu1 pass, at e), from the nu1 / nu2
+// the sweep saved (nuh[n], nuh[C + n]) and the records (seg, C steps apart).  Each wavefront's sums go to
+// sums[q][row][MAXP]: what k_adj_b / k_adj_c add to their rows, computed alike (same site map, terms and
+// wavefront sums).
+template <typename R, int GEO, int NP>
+__global__ __launch_bounds__(BLOCK) void k_par_seg(const Static<R>* __restrict__ S, const AStatic<R>* __restrict__ A,
+                                                   const R* __restrict__ seg, const R* __restrict__ nuh, int C,
+                                                   int n_seg, int q0, double* __restrict__ sums) {
+  const int q = q0 + int(blockIdx.y), n = n_seg - 1 - (q >> 1);
+  const size_t N3 = size_t(S->N) * 3, slot = ((q & 1) ? 0 : size_t(C)) + size_t(n);
+  const R* x = seg + slot * N3;   // ep (record 1) for the nu2 pass, e (record 0) for the nu1 pass
+  const R* nu = nuh + slot * N3;  // nu2 at C + n, nu1 at n
+  double gp[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) gp[p] = 0.0;
+  Site st;
+  if (site_of<R, GEO>(S, st)) {
+    const int i = st.i;
+    param_terms<R, GEO, NP>(S, A, x, st, nu[3 * i], nu[3 * i + 1], nu[3 * i + 2], x[3 * i], x[3 * i + 1],
+                            x[3 * i + 2], gp);
+  }
+#pragma unroll
+  for (int p = 0; p < NP; ++p) gp[p] = wave_sum(gp[p]);
+  const int lane = threadIdx.x & 63;
+  if (lane < NP) sums[(size_t(q) * gridDim.x * WAVES + wave_row()) * MAXP + lane] = lane_pick<NP>(gp);
+}
+
+// Adds npass passes' sums to the rows of gpart in sweep order (thread = row x NP + parameter).
+template <int NP>
+__global__ void k_par_fold(const double* __restrict__ sums, int npass, int nrow, double* __restrict__ gpart) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrow * NP) return;
+  const int row = t / NP, p = t - row * NP;
+  double v = gpart[size_t(row) * MAXP + p];
+  for (int q = 0; q < npass; ++q) v = v + sums[(size_t(q) * nrow + row) * MAXP + p];
+  gpart[size_t(row) * MAXP + p] = v;
 }
 
 template <typename T>
@@ -985,6 +1084,10 @@ struct mx_adjoint {
   uint64_t* srec_d = nullptr;
   uint32_t* sctl_d = nullptr;
   uint32_t srec_epoch = 0;
+  // per-pass wavefront sums of the parameter terms of one persistently swept segment (2 x seg_cap passes x
+  // nblk x WAVES rows x MAXP)
+  double* psum_d = nullptr;
+  int psum_cap = 0;
   bool side_by_side = false;  // replays and sweeps now run beside each other: ordinary launches
 };
 
@@ -1186,19 +1289,41 @@ void ensure_sweep_sync(mx_adjoint& a) {
   }
 }
 
+// The per-step nu1 / nu2 slots of a segment (2 x seg_cap x N x 3), for the dense terms and for the
+// persistent sweep's parameter terms.
+void ensure_nuh(mx_adjoint& a) {
+  if (a.nuh_cap >= a.seg_cap) return;
+  afree(a.nuh_d);
+  a.nuh_cap = 0;
+  aok(hipMalloc(&a.nuh_d, 2 * size_t(a.seg_cap) * nbytes3(*a.m)), "hipMalloc adjoint vectors of a segment");
+  a.nuh_cap = a.seg_cap;
+}
+
 // The whole sweep of a segment of two or more steps in persistent launches of up to STEPS_PER_LAUNCH
 // steps (k_adj_seg), on models that take the persistent forward launch; false when the model does not.
 // Its tagged records and control words are its own (replays of the next segment may run beside it on
 // the model's), consecutive launches take different epochs, and an aborted launch is counted with the
-// model's given-up waits.
+// model's given-up waits.  Every step's nu1 and nu2 go to the segment's slots (nuh), and k_par_seg /
+// k_par_fold then add the parameter terms of all the segment's passes to the partial rows.
 template <typename R, int GEO, int NP, int CM, bool LEAN>
-bool sweep_persistent(mx_adjoint& a, const Dyn<R>& d, const R* seg, int C, int n_seg, R* nuh, hipStream_t s) {
+bool sweep_persistent(mx_adjoint& a, const Dyn<R>& d, const R* seg, int C, int n_seg, hipStream_t s) {
   if constexpr (GEO != GEO_PERIODIC && GEO != GEO_OPEN) {
     return false;
   } else {
     Model& m = *a.m;
     if (n_seg < 2 || !persistent_steps(m) || ensure_step_scratch(m) != MX_OK || !m.stepsync_d) return false;
     ensure_sweep_sync(a);
+    ensure_nuh(a);
+    const int nrow = a.nblk * WAVES;
+    if (a.psum_cap < 2 * a.seg_cap) {
+      if (a.psum_d) (void)hipFree(a.psum_d);
+      a.psum_d = nullptr;
+      a.psum_cap = 0;
+      aok(hipMalloc(reinterpret_cast<void**>(&a.psum_d), sizeof(double) * 2 * size_t(a.seg_cap) * nrow * MAXP),
+          "hipMalloc parameter sums");
+      a.psum_cap = 2 * a.seg_cap;
+    }
+    R* nuh = static_cast<R*>(a.nuh_d);
     const Static<R>* S = static_cast<const Static<R>*>(m.static_d);
     const AStatic<R>* A = static_cast<const AStatic<R>*>(a.astatic_d);
     StepSync sy{};
@@ -1206,7 +1331,6 @@ bool sweep_persistent(mx_adjoint& a, const Dyn<R>& d, const R* seg, int C, int n
     sy.spin = 1u << 16;
     uint32_t* gave_up = m.stepsync_d + stepsync_ctl(m) + 2;
     R* lam = static_cast<R*>(a.lam_d);
-    double* gpart = a.gpart_d;
     double* gf = a.want_field ? a.gf_d + size_t(a.sweep_step0) * 2 * a.nblk * WAVES * 3 : nullptr;
     uint64_t* rec = a.srec_d;
     static_assert(2 * STEPS_PER_LAUNCH + 2 < (1 << 12), "sweep tags overflow their 12 bits");
@@ -1214,8 +1338,8 @@ bool sweep_persistent(mx_adjoint& a, const Dyn<R>& d, const R* seg, int C, int n
       int lo = std::max(0, hi - STEPS_PER_LAUNCH);
       a.srec_epoch = a.srec_epoch % 15u + 1u;
       uint32_t tag0 = a.srec_epoch << 12;
-      void* args[] = {&S, &A, const_cast<Dyn<R>*>(&d), &seg, &C,  &hi,  &lo,  &lam,
-                      &nuh, &gpart, &gf, &rec, &tag0, &sy, &gave_up};
+      void* args[] = {&S, &A, const_cast<Dyn<R>*>(&d), &seg, &C, &hi, &lo, &lam, &nuh, &gf, &rec, &tag0, &sy,
+                      &gave_up};
       // beside a replay, an ordinary launch: the runtime runs cooperative launches one at a time, and
       // both grids fit on the GPU together (overlap_backward)
       const void* fn = reinterpret_cast<const void*>(k_adj_seg<R, GEO, NP, CM, LEAN>);
@@ -1223,6 +1347,11 @@ bool sweep_persistent(mx_adjoint& a, const Dyn<R>& d, const R* seg, int C, int n
                          : hipLaunchCooperativeKernel(fn, dim3(a.nblk), dim3(BLOCK), args, 0, s),
           "persistent adjoint sweep");
     }
+    for (int q0 = 0; q0 < 2 * n_seg; q0 += 32768)  // grid rows: at most 65535
+      hipLaunchKernelGGL((k_par_seg<R, GEO, NP>), dim3(a.nblk, std::min(32768, 2 * n_seg - q0)), dim3(BLOCK), 0, s, S,
+                         A, seg, nuh, C, n_seg, q0, a.psum_d);
+    hipLaunchKernelGGL((k_par_fold<NP>), dim3((nrow * NP + 255) / 256), dim3(256), 0, s, a.psum_d, 2 * n_seg, nrow,
+                       a.gpart_d);
     return true;
   }
 }
@@ -1269,7 +1398,7 @@ void sweep_t(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s, R* 
   auto frow = [&](int n, int pass) -> double* {
     return a.want_field ? a.gf_d + (size_t(a.sweep_step0 + n) * 2 + pass) * a.nblk * WAVES * 3 : nullptr;
   };
-  if (sweep_persistent<R, GEO, NP, CM, LEAN>(a, d, seg, C, n_seg, nuh, s)) {
+  if (sweep_persistent<R, GEO, NP, CM, LEAN>(a, d, seg, C, n_seg, s)) {
     if (nuh)
       hipLaunchKernelGGL((k_dense_seg<R, GEO, 4>), g, b, 0, s, S, A, seg, nuh, C, n_seg, a.gb_d, a.ga_d);
     return;
@@ -1334,12 +1463,7 @@ void sweep(mx_adjoint& a, char* seg, size_t b, int C, int n_seg, bool dense, hip
 void ensure_dense(mx_adjoint& a, hipStream_t s) {
   const Model& m = *a.m;
   const size_t nb = size_t(a.nfwd) * 9 * m.ncell, na = size_t(9) * m.N;
-  if (a.nuh_cap < a.seg_cap) {
-    afree(a.nuh_d);
-    a.nuh_cap = 0;
-    aok(hipMalloc(&a.nuh_d, 2 * size_t(a.seg_cap) * nbytes3(m)), "hipMalloc dense adjoint vectors");
-    a.nuh_cap = a.seg_cap;
-  }
+  ensure_nuh(a);
   if (!a.gb_d) aok(hipMalloc(reinterpret_cast<void**>(&a.gb_d), sizeof(double) * std::max<size_t>(nb, 1)), "hipMalloc dense bond gradient");
   if (!a.ga_d) aok(hipMalloc(reinterpret_cast<void**>(&a.ga_d), sizeof(double) * na), "hipMalloc dense on-site gradient");
   if (nb) hipLaunchKernelGGL((k_zero<double>), dim3((nb + 255) / 256), dim3(256), 0, s, a.gb_d, nb);
@@ -1548,7 +1672,7 @@ void mx_adjoint_destroy(mx_adjoint* a) {
                    reinterpret_cast<void**>(&a->gout_d), reinterpret_cast<void**>(&a->gb_d),
                    reinterpret_cast<void**>(&a->ga_d), &a->nuh_d, reinterpret_cast<void**>(&a->cmap_d),
                    reinterpret_cast<void**>(&a->gf_d), &a->gw_d, &a->seg2_d, reinterpret_cast<void**>(&a->srec_d),
-                   reinterpret_cast<void**>(&a->sctl_d)})
+                   reinterpret_cast<void**>(&a->sctl_d), reinterpret_cast<void**>(&a->psum_d)})
     afree(*p);
   for (hipEvent_t e : {a->ev_start, a->ev_rep[0], a->ev_rep[1], a->ev_sweep[0], a->ev_sweep[1]})
     if (e) (void)hipEventDestroy(e);
