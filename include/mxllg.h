@@ -330,7 +330,11 @@ int mx_adjoint_forward(mx_adjoint* a, void* e, int64_t nsteps, uint64_t seed, ui
 int mx_adjoint_forward_driven(mx_adjoint* a, void* e, int64_t nsteps, const double* B, uint64_t seed,
                               uint64_t step0, void* stream);
 /* Given grad_e = dL/de_final (device, N x 3), writes dL/de_0 into grad_e0 (device, N x 3, may be null)
- * and dL/dtheta into grad_theta (device, nparam doubles), for the last mx_adjoint_forward. */
+ * and dL/dtheta into grad_theta (device, nparam doubles), for the last mx_adjoint_forward.  On grids the
+ * GPU holds twice over at once (persistent launches), segment replays run on a second stream that the
+ * handle owns (non-blocking, default priority), ordered against `stream` with events: it waits for the
+ * work queued on `stream` before the call, and everything it runs is joined back into `stream` before
+ * the call's last work there.  Polls that give up count in mx_model_step_info's given-up waits. */
 int mx_adjoint_backward(mx_adjoint* a, const void* grad_e, void* grad_e0, double* grad_theta, void* stream);
 /* mx_adjoint_backward plus the dense gradients at the current couplings (either may be null):
  * grad_bonds (device, nbonds x ncell x 9 doubles, the layout of mx_model_set_bond_tensors) receives
