@@ -914,8 +914,10 @@ __global__ void k_copy(const T* __restrict__ a, T* __restrict__ b, size_t n) {
 // pass by pass in the order the sweep runs them (newest step first, its corrector pass -- nu2 at ep --
 // before its predictor pass -- nu1 at e), so the sums are those of adding every pass's terms to the
 // accumulators as it runs, and every accumulator is read and written once per segment instead of once
-// per pass.  One thread per site, up to FG forward entries at a time in FP64 registers; neighbours past
-// an open face count zero.  Accumulators: gb[(f * 9 + q) * ncell + cell], ga[q * N + site].
+// per pass.  One thread per site and group of FG forward entries (grid row blockIdx.y: group y of the
+// site's sublattice), the group's accumulators in FP64 registers; group 0 also holds the site's on-site
+// term.  Neighbours past an open face count zero.  Accumulators: gb[(f * 9 + q) * ncell + cell],
+// ga[q * N + site].
 template <typename R, int GEO, int FG>
 __global__ __launch_bounds__(BLOCK) void k_dense_seg(const Static<R>* __restrict__ S,
                                                      const AStatic<R>* __restrict__ A, const R* __restrict__ seg,
@@ -926,9 +928,13 @@ __global__ __launch_bounds__(BLOCK) void k_dense_seg(const Static<R>* __restrict
   const int i = st.i, ncell = S->ncell;
   const size_t N3 = size_t(S->N) * 3, N = size_t(S->N);
   const int kb = S->sptr[st.s], ke = S->sptr[st.s + 1];
+  const bool first = blockIdx.y == 0;
   int k = kb;
-  for (bool first = true;; first = false) {
-    // the next FG forward entries of the sublattice (wavefront-uniform)
+  // skip the forward entries of the earlier groups (wavefront-uniform)
+  for (int skip = int(blockIdx.y) * FG; k < ke && (skip > 0 || A->fslot[k] < 0); ++k)
+    if (A->fslot[k] >= 0) --skip;
+  {
+    // this group's FG forward entries of the sublattice (wavefront-uniform)
     int kk[FG], nf = 0;
 #pragma unroll
     for (int u = 0; u < FG; ++u) {
@@ -936,7 +942,7 @@ __global__ __launch_bounds__(BLOCK) void k_dense_seg(const Static<R>* __restrict
       kk[u] = k < ke ? k++ : -1;
       nf += kk[u] >= 0 ? 1 : 0;
     }
-    if (nf == 0 && !first) break;
+    if (nf == 0 && !first) return;
     int jj[FG];
     R w[FG];
     double acc[FG][9];
@@ -999,7 +1005,6 @@ __global__ __launch_bounds__(BLOCK) void k_dense_seg(const Static<R>* __restrict
 #pragma unroll
       for (int q = 0; q < 9; ++q) ga[q * N + i] = aa[q];
     }
-    if (k >= ke) break;
   }
 }
 
@@ -1356,6 +1361,20 @@ bool sweep_persistent(mx_adjoint& a, const Dyn<R>& d, const R* seg, int C, int n
   }
 }
 
+// Forward entries per k_dense_seg thread, and its grid rows: groups of that many forward entries of the
+// largest sublattice stencil (at least one row, which also holds the on-site terms).
+constexpr int DENSE_FG = 2;
+int dense_rows(const mx_adjoint& a) {
+  const Model& m = *a.m;
+  int mx = 0;
+  for (int sl = 0; sl < m.ns; ++sl) {
+    int c = 0;
+    for (int k = m.sptr[sl]; k < m.sptr[sl + 1]; ++k) c += a.fslot[k] >= 0 ? 1 : 0;
+    mx = std::max(mx, c);
+  }
+  return std::max(1, (mx + DENSE_FG - 1) / DENSE_FG);
+}
+
 // Adjoint sweep over one replayed segment of n_seg steps (records rec(n), newest last): the local
 // part of the segment's last step from lam, then per step (newest first) k_adj_b and k_adj_c, the
 // latter fused with the next-older step's local part.  Leaves dL/de of the segment's first input in lam.
@@ -1400,7 +1419,8 @@ void sweep_t(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s, R* 
   };
   if (sweep_persistent<R, GEO, NP, CM, LEAN>(a, d, seg, C, n_seg, s)) {
     if (nuh)
-      hipLaunchKernelGGL((k_dense_seg<R, GEO, 4>), g, b, 0, s, S, A, seg, nuh, C, n_seg, a.gb_d, a.ga_d);
+      hipLaunchKernelGGL((k_dense_seg<R, GEO, DENSE_FG>), dim3(a.nblk, dense_rows(a)), b, 0, s, S, A, seg, nuh, C,
+                         n_seg, a.gb_d, a.ga_d);
     return;
   }
   hipLaunchKernelGGL((k_adj_a<R, GEO, LEAN>), g, b, 0, s, S, d, lam, rec[n_seg - 1], lamu, gep, nu2_of(n_seg - 1),
@@ -1416,7 +1436,8 @@ void sweep_t(mx_adjoint& a, const StepRec<R>* rec, int n_seg, hipStream_t s, R* 
                          rec[n], lamu, gep, nu2_of(n), a.gpart_d, frow(n, 0), gwb);
   }
   if (nuh)
-    hipLaunchKernelGGL((k_dense_seg<R, GEO, 4>), g, b, 0, s, S, A, seg, nuh, C, n_seg, a.gb_d, a.ga_d);
+    hipLaunchKernelGGL((k_dense_seg<R, GEO, DENSE_FG>), dim3(a.nblk, dense_rows(a)), b, 0, s, S, A, seg, nuh, C,
+                       n_seg, a.gb_d, a.ga_d);
 }
 
 template <typename R, int NP, int CM, bool LEAN>
