@@ -892,7 +892,16 @@ __global__ void k_par_fold(const double* __restrict__ sums, int npass, int nrow,
   if (t >= nrow * NP) return;
   const int row = t / NP, p = t - row * NP;
   double v = gpart[size_t(row) * MAXP + p];
-  for (int q = 0; q < npass; ++q) v = v + sums[(size_t(q) * nrow + row) * MAXP + p];
+  // 16 passes' sums in flight at a time, added in pass order
+  constexpr int U = 16;
+  for (int q0 = 0; q0 < npass; q0 += U) {
+    double x[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) x[u] = q0 + u < npass ? sums[(size_t(q0 + u) * nrow + row) * MAXP + p] : 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (q0 + u < npass) v = v + x[u];
+  }
   gpart[size_t(row) * MAXP + p] = v;
 }
 
@@ -1350,7 +1359,7 @@ bool sweep_persistent(mx_adjoint& a, const Dyn<R>& d, const R* seg, int C, int n
       // beside a replay, an ordinary launch: the runtime runs cooperative launches one at a time, and
       // both grids fit on the GPU together (overlap_backward)
       const void* fn = reinterpret_cast<const void*>(k_adj_seg<R, GEO, NP, CM, LEAN>);
-      aok(a.side_by_side ? hipLaunchKernel(fn, dim3(a.nblk), dim3(BLOCK), args, 0, s)
+      aok(a.side_by_side ? hipLaunchKernel(fn, dim3(a.nblk), dim3(BLOCK), args, SIDE_BY_SIDE_LDS, s)
                          : hipLaunchCooperativeKernel(fn, dim3(a.nblk), dim3(BLOCK), args, 0, s),
           "persistent adjoint sweep");
     }

@@ -227,6 +227,9 @@ struct StepRecOut {
   void *e, *ep, *w, *b1, *b2;
 };
 constexpr int STEPS_PER_LAUNCH = 1024;
+// Dynamic LDS (unused) of a plain launch that runs beside another persistent one: more than half of a
+// compute unit's 160 KB, so the two grids never share a compute unit.
+constexpr size_t SIDE_BY_SIDE_LDS = 96 * 1024;
 int launch_heun_steps(const Model& m, void* e, uint64_t seed, uint64_t step0, int64_t nsteps, const void* noise,
                       hipStream_t s, const StepBufs* bufs = nullptr, const StepRecOut* rec = nullptr,
                       bool plain = false);

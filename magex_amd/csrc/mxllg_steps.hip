@@ -164,8 +164,9 @@ static hipError_t steps_coop(dim3 g, hipStream_t s, const Static<R>* S, Dyn<R> d
                              uint64_t step0, int n, const R* nz, uint32_t tag0, StepSync sy, RecPtrs<R> ro,
                              bool plain) {
   void* args[] = {&S, &d, &E, &rec, &step0, &n, &nz, &tag0, &sy, &ro};
-  if (plain)
-    return hipLaunchKernel(reinterpret_cast<const void*>(k_steps<R, CM, GEO, LEAN>), g, dim3(BLOCK), args, 0, s);
+  if (plain)  // with more LDS than half a compute unit's, no two workgroups of the side-by-side launches share one
+    return hipLaunchKernel(reinterpret_cast<const void*>(k_steps<R, CM, GEO, LEAN>), g, dim3(BLOCK), args,
+                           SIDE_BY_SIDE_LDS, s);
   return hipLaunchCooperativeKernel(k_steps<R, CM, GEO, LEAN>, g, dim3(BLOCK), args, 0, s);
 }
 
