@@ -977,32 +977,54 @@ __global__ __launch_bounds__(BLOCK) void k_dense_seg(const Static<R>* __restrict
 #pragma unroll
       for (int q = 0; q < 9; ++q) aa[q] = ga[q * N + i];
     }
-    for (int n = n_seg - 1; n >= 0; --n) {
+    // the passes in sweep order, q = 0 .. 2 n_seg - 1: step n_seg - 1 - q / 2, its nu2 pass (at ep) before
+    // its nu1 pass (at e); each pass's values are loaded while the previous pass is added
+    struct Ld {
+      R v[3], x[3], vj[FG][3], xj[FG][3];
+    };
+    auto load = [&](int q, Ld& L) {
+      const int n = n_seg - 1 - (q >> 1), pass = 1 - (q & 1);
+      const R* x = seg + (size_t(pass) * C + n) * N3;
+      const R* v = nuh + (size_t(pass) * C + n) * N3;
 #pragma unroll
-      for (int pass = 1; pass >= 0; --pass) {
-        const R* x = seg + (size_t(pass) * C + n) * N3;
-        const R* v = nuh + (size_t(pass) * C + n) * N3;
-        const double ni[3] = {double(v[3 * i]), double(v[3 * i + 1]), double(v[3 * i + 2])};
-        const double xi[3] = {double(x[3 * i]), double(x[3 * i + 1]), double(x[3 * i + 2])};
+      for (int c = 0; c < 3; ++c) {
+        L.v[c] = v[3 * i + c];
+        L.x[c] = x[3 * i + c];
+      }
 #pragma unroll
-        for (int u = 0; u < FG; ++u) {
-          if (u < nf) {
-            const int j = jj[u];
-            const double nj[3] = {double(v[3 * j] * w[u]), double(v[3 * j + 1] * w[u]), double(v[3 * j + 2] * w[u])};
-            const double xj[3] = {double(x[3 * j] * w[u]), double(x[3 * j + 1] * w[u]), double(x[3 * j + 2] * w[u])};
+      for (int u = 0; u < FG; ++u)
+        if (u < nf)
 #pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-              for (int c = 0; c < 3; ++c) acc[u][3 * r + c] = acc[u][3 * r + c] + (ni[r] * xj[c] + xi[r] * nj[c]);
+          for (int c = 0; c < 3; ++c) {
+            L.vj[u][c] = v[3 * jj[u] + c];
+            L.xj[u][c] = x[3 * jj[u] + c];
           }
-        }
-        if (first) {
+    };
+    const int npass = 2 * n_seg;
+    Ld cur, nxt;
+    load(0, cur);
+    for (int q = 0; q < npass; ++q) {
+      if (q + 1 < npass) load(q + 1, nxt);
+      const double ni[3] = {double(cur.v[0]), double(cur.v[1]), double(cur.v[2])};
+      const double xi[3] = {double(cur.x[0]), double(cur.x[1]), double(cur.x[2])};
+#pragma unroll
+      for (int u = 0; u < FG; ++u) {
+        if (u < nf) {
+          const double nj[3] = {double(cur.vj[u][0] * w[u]), double(cur.vj[u][1] * w[u]), double(cur.vj[u][2] * w[u])};
+          const double xj[3] = {double(cur.xj[u][0] * w[u]), double(cur.xj[u][1] * w[u]), double(cur.xj[u][2] * w[u])};
 #pragma unroll
           for (int r = 0; r < 3; ++r)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) aa[3 * r + c] -= ni[r] * xi[c];
+            for (int c = 0; c < 3; ++c) acc[u][3 * r + c] = acc[u][3 * r + c] + (ni[r] * xj[c] + xi[r] * nj[c]);
         }
       }
+      if (first) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) aa[3 * r + c] -= ni[r] * xi[c];
+      }
+      cur = nxt;
     }
 #pragma unroll
     for (int u = 0; u < FG; ++u) {
@@ -1374,7 +1396,7 @@ bool sweep_persistent(mx_adjoint& a, const Dyn<R>& d, const R* seg, int C, int n
 
 // Forward entries per k_dense_seg thread, and its grid rows: groups of that many forward entries of the
 // largest sublattice stencil (at least one row, which also holds the on-site terms).
-constexpr int DENSE_FG = 2;
+constexpr int DENSE_FG = 4;
 int dense_rows(const mx_adjoint& a) {
   const Model& m = *a.m;
   int mx = 0;
