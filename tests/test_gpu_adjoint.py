@@ -247,6 +247,28 @@ def test_persistent_sweep_equals_per_pass_sweep(gpu, precision):
         assert np.array_equal(a, b), (k, float(np.abs(a - b).max()))
 
 
+def test_persistent_sweep_longer_than_one_launch(gpu):
+    """A segment longer than one persistent launch (1024 steps): the replay and the sweep each split into
+    two launches (lambda and the partial rows carried through memory, a fresh tag epoch each), and the
+    parameter terms of all 2200 passes are summed after the sweep.  The gradients equal step mode 1's
+    (one kernel per pass) bit for bit, with no poll given up; square 8^2, 1100 steps, one segment."""
+    print("case: persistent sweep over 1100 steps in one segment, square 8^2", flush=True)
+    L, steps = 8, 1100
+    _, ref_build = _builders(L)
+    o = ref_build(THETA0)
+    e0 = o.init_random(30)
+    tgt = torch.as_tensor(o.init_random(31)).cuda()
+    out = {}
+    for mode in (1, 2):
+        sol = _solver(L, C=steps, T=20.0, seed=3)
+        sol.dm.set_step_mode(mode)
+        e, gt, g0 = _grads(sol, THETA0, e0, tgt, steps)
+        assert sol.dm.step_info()[1] == 0
+        out[mode] = (e.cpu().numpy(), gt, g0)
+    for a, b in zip(out[1], out[2]):
+        assert np.array_equal(a, b), float(np.abs(a - b).max())
+
+
 def test_adjoint_forward_is_the_plain_trajectory(gpu):
     """mx_adjoint_forward at theta is the plain solver on the builder's system at theta."""
     print("case: adjoint forward vs LLGSolver, square 32^2", flush=True)
