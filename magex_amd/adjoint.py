@@ -206,7 +206,11 @@ class DifferentiableLLG:
         return out
 
     def backward(self, grad_e, want_e0=True):
-        """(dL/dtheta (P,) float64 device tensor, dL/de0 or None) for the last forward()."""
+        """(dL/dtheta (P,) float64 device tensor, dL/de0 or None) for the last forward().
+
+        Runs on torch's current stream; on grids the GPU holds twice over at once, each checkpoint segment
+        is replayed on a second, internal stream beside the previous segment's sweep, ordered against the
+        current stream with events (so the results are the current stream's, as for any other call)."""
         g = grad_e.detach().to(device="cuda", dtype=self.dtype).contiguous()
         gt = torch.empty(MAXP, dtype=torch.float64, device="cuda")
         g0 = torch.empty_like(g) if want_e0 else None
