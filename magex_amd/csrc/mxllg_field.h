@@ -327,7 +327,6 @@ struct Batch {
   __device__ __forceinline__ bool gather_tagged(const Static<R>* __restrict__ S, const Dyn<R>& d,
                                                 const uint64_t* __restrict__ rec, const Site& st, int k0, int ke,
                                                 uint32_t want, uint32_t spin, const uint32_t* abort, Mid&& mid) {
-    constexpr int NW = TAG_WORDS<R>;
     static_assert(CH <= 32, "pending entries are a 32-bit mask");
     Ent raw[CH];
     records(S, k0, raw);
@@ -362,6 +361,58 @@ struct Batch {
         for (int q = 0; q < NJ; ++q) J[u][q] = T[q];
       }
     }
+    return poll_tagged(rec, jj, pend, k0, ke, want, spin, abort, mid);
+  }
+
+  // The record indices jj of the batch's entries from k0 and the mask of those live and inside an open
+  // face, as gather_tagged computes them (held across a persistent launch's steps).
+  static __device__ __forceinline__ void tagged_at(const Static<R>* __restrict__ S, const Site& st, int k0, int ke,
+                                                   int (&jj)[CH], uint32_t& okm) {
+    Ent raw[CH];
+    records(S, k0, raw);
+    const int ncell = S->ncell;
+    okm = 0;
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const bool live = k0 + u < ke;
+      Ent en;
+      en.slot = live ? raw[u].slot : 0;
+      en.tb = live ? raw[u].tb : st.s;
+      en.d01 = live ? raw[u].d01 : 0;
+      en.d2 = live ? raw[u].d2 : 0;
+      bool ok;
+      const int cn = neighbour_cell<R, GEO>(S, st, en, ok);
+      if (ok && live) okm |= 1u << u;
+      jj[u] = (en.tb & 0xffff) * ncell + cn;
+    }
+  }
+
+  // As gather_tagged (table couplings), with the entries' record indices jj and live-and-inside mask okm
+  // already known (tagged_at): no neighbour arithmetic.
+  template <typename Mid>
+  __device__ __forceinline__ bool gather_tagged_at(const Static<R>* __restrict__ S, const uint64_t* __restrict__ rec,
+                                                   int k0, int ke, const int (&jj)[CH], uint32_t okm, uint32_t want,
+                                                   uint32_t spin, const uint32_t* abort, Mid&& mid) {
+    static_assert(CM != COUP_BOND, "per-bond tensors are read per step");
+    Ent raw[CH];
+    records(S, k0, raw);
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      ex[u] = ey[u] = ez[u] = R(0);
+      const R* T = S->tab + NJ * (k0 + u < ke ? raw[u].slot : 0);
+#pragma unroll
+      for (int q = 0; q < NJ; ++q) J[u][q] = T[q];
+    }
+    return poll_tagged(rec, jj, okm, k0, ke, want, spin, abort, mid);
+  }
+
+  // The polls of the tagged gathers: every live entry's words re-read until all carry the tag `want`,
+  // pend the entries still to take.
+  template <typename Mid>
+  __device__ __forceinline__ bool poll_tagged(const uint64_t* __restrict__ rec, const int (&jj)[CH], uint32_t pend,
+                                              int k0, int ke, uint32_t want, uint32_t spin, const uint32_t* abort,
+                                              Mid&& mid) {
+    constexpr int NW = TAG_WORDS<R>;
     for (uint32_t it = 0;; ++it) {
       // every live entry's words in flight at once (a branch per entry would wait on each in turn)
       uint64_t w[CH][NW];

@@ -58,6 +58,7 @@ __global__ __launch_bounds__(BLOCK) void k_steps(const Static<R>* __restrict__ S
   bool ok = true;
   if (live) {
     Held<R, CM, GEO, 0> h;
+    hold_indices<R, CM, GEO, 0>(S, st, kb, ke, h);  // table stencils of up to two batches: indices held
     h.a = (!LEAN && d.alpha) ? d.alpha[i] : S->alpha_sub[st.s];
     h.x = e[3 * i];
     h.y = e[3 * i + 1];
