@@ -323,6 +323,98 @@ __device__ __forceinline__ bool adj_gather(const Static<R>* __restrict__ S, cons
   return good;
 }
 
+// Neighbour indices of the persistent sweep's tagged gathers, held for a whole launch: FP32 table
+// couplings whose sublattice stencil is one batch (as in the persistent forward, FP64 would pay more in
+// registers than it saves).  j and the live-and-inside mask ok are computed as adj_gather computes them.
+template <typename R, int CM>
+struct AdjHeld {
+  static constexpr bool HOLD = sizeof(R) == 4 && CM != COUP_BOND;
+  int j[8];
+  uint32_t ok;
+  bool on;
+};
+
+template <typename R, int GEO, int CM>
+__device__ __forceinline__ void adj_hold(const Static<R>* __restrict__ S, const Site& st, AdjHeld<R, CM>& h) {
+  const int kb = S->sptr[st.s], ke = S->sptr[st.s + 1], ncell = S->ncell;
+  h.on = AdjHeld<R, CM>::HOLD && ke > kb && ke - kb <= 8;  // wavefront-uniform
+  h.ok = 0;
+  if constexpr (!AdjHeld<R, CM>::HOLD) return;
+  if (!h.on) return;
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const bool live = kb + u < ke;
+    const Ent en = S->ent[live ? kb + u : ke - 1];
+    bool ok;
+    const int cn = neighbour_cell<R, GEO>(S, st, en, ok);
+    if (ok && live) h.ok |= 1u << u;
+    h.j[u] = (en.tb & 0xffff) * ncell + cn;
+  }
+}
+
+// adj_gather's tagged path without parameter terms (PAR = false), with held indices: the same polls,
+// weights, order and sums.
+template <typename R, int CM>
+__device__ __forceinline__ bool adj_gather_held(const Static<R>* __restrict__ S, const TaggedFrom& src,
+                                                const Site& st, const AdjHeld<R, CM>& h, R& hx, R& hy, R& hz) {
+  constexpr int CH = 8;
+  constexpr int NW = TAG_WORDS<R>;
+  const int kb = S->sptr[st.s], ke = S->sptr[st.s + 1];
+  hx = hy = hz = R(0);
+  R vx[CH], vy[CH], vz[CH];
+  uint32_t pend = h.ok;
+#pragma unroll
+  for (int u = 0; u < CH; ++u) vx[u] = vy[u] = vz[u] = R(0);
+  bool good = true;
+  for (uint32_t it = 0;; ++it) {
+    uint64_t w[CH][NW];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      if (kb + u < ke) {  // wavefront-uniform
+        const uint64_t* r = src.rec + size_t(h.j[u]) * NW;
+#pragma unroll
+        for (int q = 0; q < NW; ++q)
+          w[u][q] = __hip_atomic_load(const_cast<uint64_t*>(r + q), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      if (kb + u < ke) {
+        bool g = true;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) g = g && uint32_t(w[u][q] >> 48) == src.want;
+        R a, b, c;
+        untag3<R>(w[u], a, b, c);
+        const bool take = g && (pend >> u & 1u);
+        vx[u] = take ? a : vx[u];
+        vy[u] = take ? b : vy[u];
+        vz[u] = take ? c : vz[u];
+        pend &= g ? ~(1u << u) : ~0u;
+      }
+    }
+    if (!pend) break;
+    if (it >= src.spin ||
+        ((it & 15) == 15 &&
+         __hip_atomic_load(const_cast<uint32_t*>(src.abort), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+      good = false;
+      break;
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+#pragma unroll
+  for (int u = 0; u < CH; ++u) {
+    const R wt = (h.ok >> u & 1u) ? R(1) : R(0);
+    vx[u] *= wt;
+    vy[u] *= wt;
+    vz[u] *= wt;
+    const R* T = S->tab + 9 * S->ent[kb + u < ke ? kb + u : ke - 1].slot;
+    hx += T[0] * vx[u] + T[1] * vy[u] + T[2] * vz[u];
+    hy += T[3] * vx[u] + T[4] * vy[u] + T[5] * vz[u];
+    hz += T[6] * vx[u] + T[7] * vy[u] + T[8] * vz[u];
+  }
+  return good;
+}
+
 // The parameter terms alone, as adj_gather sums them (the same batches, dead-entry weights, order and
 // expressions, hence the same bits): G_p = nu_i . (sum_k U[k][p] x_j - Aun[s][p] x_i) added to gp.
 template <typename R, int GEO, int NP>
@@ -770,7 +862,10 @@ __global__ __launch_bounds__(BLOCK) void k_adj_seg(const Static<R>* __restrict__
   SiteK<R> sk{R(0), R(0), false};
   SiteRec<R> v = {};
   double hmu = 0.0;
+  AdjHeld<R, CM> hj;
+  hj.on = false;
   if (live) {
+    adj_hold<R, GEO, CM>(S, st, hj);
     sk = site_k<R, LEAN>(S, d, st);
     if (gf) hmu = 0.5 / double(d.inv_mu ? d.inv_mu[i] : S->inv_mu_sub[st.s]);
     lx = lam[3 * i];
@@ -799,9 +894,12 @@ __global__ __launch_bounds__(BLOCK) void k_adj_seg(const Static<R>* __restrict__
         fv[2] = hmu * double(n2[2]);
       }
       R hx, hy, hz, ox, oy, oz;
-      const bool g = adj_gather<R, GEO, NP, CM, TaggedFrom, false>(S, A, d, TaggedFrom{recA, k, sy.spin, abort}, r.ep,
-                                                                  st, n2[0], n2[1], n2[2], v.ep[0], v.ep[1],
-                                                                  v.ep[2], hx, hy, hz, gp);
+      const TaggedFrom fromA{recA, k, sy.spin, abort};
+      const bool g = (AdjHeld<R, CM>::HOLD && hj.on)
+                         ? adj_gather_held<R, CM>(S, fromA, st, hj, hx, hy, hz)
+                         : adj_gather<R, GEO, NP, CM, TaggedFrom, false>(S, A, d, fromA, r.ep, st, n2[0], n2[1],
+                                                                         n2[2], v.ep[0], v.ep[1], v.ep[2], hx, hy,
+                                                                         hz, gp);
       ok = ok && g;
       onsite_T<R, LEAN>(S, d, st, v.ep[0], v.ep[1], v.ep[2], n2[0], n2[1], n2[2], ox, oy, oz);
       const R mx = ge[0] + hx - ox, my = ge[1] + hy - oy, mz = ge[2] + hz - oz;
@@ -823,9 +921,12 @@ __global__ __launch_bounds__(BLOCK) void k_adj_seg(const Static<R>* __restrict__
         fv[2] = hmu * double(n1[2]);
       }
       R hx, hy, hz, ox, oy, oz;
-      const bool g = adj_gather<R, GEO, NP, CM, TaggedFrom, false>(S, A, d, TaggedFrom{recB, k + 1, sy.spin, abort},
-                                                                  r.e, st, n1[0], n1[1], n1[2], v.e[0], v.e[1],
-                                                                  v.e[2], hx, hy, hz, gp);
+      const TaggedFrom fromB{recB, k + 1, sy.spin, abort};
+      const bool g = (AdjHeld<R, CM>::HOLD && hj.on)
+                         ? adj_gather_held<R, CM>(S, fromB, st, hj, hx, hy, hz)
+                         : adj_gather<R, GEO, NP, CM, TaggedFrom, false>(S, A, d, fromB, r.e, st, n1[0], n1[1],
+                                                                         n1[2], v.e[0], v.e[1], v.e[2], hx, hy, hz,
+                                                                         gp);
       ok = ok && g;
       onsite_T<R, LEAN>(S, d, st, v.e[0], v.e[1], v.e[2], n1[0], n1[1], n1[2], ox, oy, oz);
       lx = pt[0] + hx - ox;
