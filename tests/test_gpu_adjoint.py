@@ -269,6 +269,30 @@ def test_persistent_sweep_longer_than_one_launch(gpu):
         assert np.array_equal(a, b), float(np.abs(a - b).max())
 
 
+def test_overlapped_backward_on_a_side_stream(gpu):
+    """The overlapped backward (replays on the handle's own stream, ordered with events) issued on a torch
+    side stream gives the same bits as on the default stream, twice in a row (square 16^2, 120 steps in
+    segments of 10, T = 30 K)."""
+    print("case: overlapped backward on a torch side stream, square 16^2", flush=True)
+    L, steps = 16, 120
+    _, ref_build = _builders(L)
+    o = ref_build(THETA0)
+    e0 = o.init_random(40)
+    tgt = torch.as_tensor(o.init_random(41)).cuda()
+    sol = _solver(L, C=10, T=30.0, seed=4)
+    ref = _grads(sol, THETA0, e0, tgt, steps)
+    side = torch.cuda.Stream()          # default priority
+    for _ in range(2):
+        with torch.cuda.stream(side):
+            e = sol.forward(THETA0, e0, steps)
+            gt, g0 = sol.backward(_mse_grad(e, tgt))
+        side.synchronize()
+        assert torch.equal(e, ref[0])
+        assert np.array_equal(gt.cpu().numpy(), ref[1])
+        assert np.array_equal(g0.double().cpu().numpy(), ref[2])
+    assert sol.dm.step_info()[1] == 0
+
+
 def test_adjoint_forward_is_the_plain_trajectory(gpu):
     """mx_adjoint_forward at theta is the plain solver on the builder's system at theta."""
     print("case: adjoint forward vs LLGSolver, square 32^2", flush=True)
