@@ -819,10 +819,12 @@ __global__ __launch_bounds__(BLOCK) void k_adj_c(const Static<R>* __restrict__ S
   store_field_partials(fv, gf);
 }
 
-// Steps [n_lo, n_hi) of a replayed segment's adjoint sweep (newest first) in one cooperative launch
-// (every workgroup resident from the start), for models that take the persistent forward launch.  Each
-// workgroup keeps its sites' adjoint state (lambda, lambda_u, gep, the corrector's share of dL/dw1, the
-// partial dL/de) in registers for the whole launch; sites exchange
+// Steps [n_lo, n_hi) of a replayed segment's adjoint sweep (newest first) in one launch whose workgroups
+// are all resident from the start (cooperative, or an ordinary launch of a grid that fits beside the
+// replay running next to it), for models that take the persistent forward launch.  Each workgroup keeps
+// its sites' adjoint state (lambda, lambda_u, gep, the corrector's share of dL/dw1, the partial dL/de)
+// and, for one-batch FP32 table stencils, its neighbour indices (AdjHeld) in registers for the whole
+// launch; sites exchange
 // nu2 and nu1 through tagged records (recA, recB: 2 x N x TAG_WORDS words) as the persistent forward
 // exchanges spins.  Per step, with k = tag0 + 2 (n_hi - 1 - n): the local part publishes nu2 (tag k),
 // the nu2 gather publishes nu1 (k + 1), the nu1 gather gives lambda of the step's input.  A record is
