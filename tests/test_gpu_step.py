@@ -68,6 +68,21 @@ def test_triangular_three_shells_dmi(gpu, precision, graph):
     _modes_agree(gpu, s, precision, 40, 5e-4, 0.05, 20.0, graph=graph)
 
 
+@pytest.mark.parametrize("case", ["tri3_periodic", "tri3_open", "tri2_open", "sc2_jani"])
+def test_two_batch_tables_fp32(gpu, case):
+    """FP32 table stencils of 10 to 18 entries (two batches, both with held record indices in
+    persistent launches), width-4 and width-9 tables, periodic and open axes."""
+    print(f"case: two-batch table {case}", flush=True)
+    if case == "sc2_jani":
+        s = _system((6, 5, 4), (0.1, 0, 0.3), lattice="sc", J=[1.0, 0.3],
+                    Jani=[[0.1, 0.02, 0], [0.02, -0.05, 0], [0, 0, -0.05]], D=0.1, K=0.05)
+    else:
+        pbc = (True, True, True) if case == "tri3_periodic" else (False, True, False)
+        J = [5.0, -0.5, 0.2] if case.startswith("tri3") else [1.0, 0.2]
+        s = _system((23, 19, 1), (0, 0, 2.0), pbc, lattice="triangular", J=J, D=1.5, K=0.3)
+    _modes_agree(gpu, s, "fp32", 60, 5e-4, 0.05, 20.0)
+
+
 def test_bcc_per_bond_two_held_batches(gpu):
     """Two sublattices, 14 per-bond tensors per site (both held batches), per-site SIA; generated noise
     (the plain-model kernel) and injected noise."""
