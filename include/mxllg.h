@@ -226,6 +226,12 @@ int mx_model_set_step_mode(mx_model* m, int32_t mode);
  * instead, plus persistent launches that stopped early on a timed-out wait (0 in normal operation;
  * synchronises the device).  Either may be NULL. */
 int mx_model_step_info(mx_model* m, int32_t* one_launch, int64_t* given_up);
+/* Polls after which a wait of the one-launch steps, persistent runs and persistent adjoint sweeps gives
+ * up (default 65536, far beyond any wait of normal operation; 0: after the first unsuccessful poll).  A
+ * one-launch step's given-up waits are finished by the launch's last workgroup (same results); a
+ * persistent launch that gives up stops early and leaves its spins part-way, which given_up reports.
+ * For tests of those paths, which then run on every launch.  No counterpart in magex. */
+int mx_model_set_wait_limit(mx_model* m, int32_t polls);
 
 /* Grade of the thermal-noise generator (the counter-based stream that replaces magex's
  * PCG64 / ziggurat draws, magex/llg.py:119, fast.py:123-200): MX_NOISE_DOUBLE (default for MX_FP64) draws

@@ -44,6 +44,7 @@ SIGNATURES = {
     "mx_model_set_dynamics": (_I, [_P, _D, _I, _P, _I, _P, _I, _I]),
     "mx_model_set_graph_steps": (_I, [_P, ctypes.c_int32]),
     "mx_model_set_step_mode": (_I, [_P, ctypes.c_int32]),
+    "mx_model_set_wait_limit": (_I, [_P, ctypes.c_int32]),
     "mx_model_set_noise": (_I, [_P, ctypes.c_int32]),
     "mx_model_step_info": (_I, [_P, _P, _P]),
     "mx_llg_run_driven": (_I, [_P, _P, _I64, _P, _U64, _U64, _P]),

@@ -183,14 +183,20 @@ class DifferentiableLLG:
         (field_sequence builds it from a field_func), or None for the system's own field."""
         self.set_params(theta)
         e = self._spins(e0)
+        start = e.clone()
         L = self.dm.lib
-        if field is None:
-            check(L.mx_adjoint_forward(self.h, ctypes.c_void_p(e.data_ptr()), int(nsteps), self.seed, int(step0),
-                                       _stream()))
-        else:
-            B = np.ascontiguousarray(np.asarray(field, float).reshape(int(nsteps), 2, 3))
-            check(L.mx_adjoint_forward_driven(self.h, ctypes.c_void_p(e.data_ptr()), int(nsteps), hptr(B), self.seed,
-                                              int(step0), _stream()))
+        B = None if field is None else np.ascontiguousarray(np.asarray(field, float).reshape(int(nsteps), 2, 3))
+
+        def work():
+            if B is None:
+                check(L.mx_adjoint_forward(self.h, ctypes.c_void_p(e.data_ptr()), int(nsteps), self.seed,
+                                           int(step0), _stream()))
+            else:
+                check(L.mx_adjoint_forward_driven(self.h, ctypes.c_void_p(e.data_ptr()), int(nsteps), hptr(B),
+                                                  self.seed, int(step0), _stream()))
+
+        # a persistent launch that gave up a wait (its spins part-way) is run again one launch per step
+        self.dm.checked(work, lambda: e.copy_(start))
         self._nsteps = int(nsteps)
         self._token += 1
         return e
@@ -214,9 +220,9 @@ class DifferentiableLLG:
         g = grad_e.detach().to(device="cuda", dtype=self.dtype).contiguous()
         gt = torch.empty(MAXP, dtype=torch.float64, device="cuda")
         g0 = torch.empty_like(g) if want_e0 else None
-        check(self.dm.lib.mx_adjoint_backward(self.h, ctypes.c_void_p(g.data_ptr()),
-                                              ctypes.c_void_p(g0.data_ptr() if g0 is not None else 0),
-                                              ctypes.c_void_p(gt.data_ptr()), _stream()))
+        self.dm.checked(lambda: check(self.dm.lib.mx_adjoint_backward(
+            self.h, ctypes.c_void_p(g.data_ptr()), ctypes.c_void_p(g0.data_ptr() if g0 is not None else 0),
+            ctypes.c_void_p(gt.data_ptr()), _stream())), lambda: None)
         return gt[:self.P], g0
 
     @property
@@ -235,10 +241,10 @@ class DifferentiableLLG:
         g0 = torch.empty_like(g) if want_e0 else None
         gb = torch.empty((self.nbonds, self.system.ncell, 3, 3), dtype=torch.float64, device="cuda")
         ga = torch.empty((self.N, 3, 3), dtype=torch.float64, device="cuda")
-        check(self.dm.lib.mx_adjoint_backward_dense(self.h, ctypes.c_void_p(g.data_ptr()),
-                                                    ctypes.c_void_p(g0.data_ptr() if g0 is not None else 0),
-                                                    ctypes.c_void_p(gt.data_ptr()), ctypes.c_void_p(gb.data_ptr()),
-                                                    ctypes.c_void_p(ga.data_ptr()), _stream()))
+        self.dm.checked(lambda: check(self.dm.lib.mx_adjoint_backward_dense(
+            self.h, ctypes.c_void_p(g.data_ptr()), ctypes.c_void_p(g0.data_ptr() if g0 is not None else 0),
+            ctypes.c_void_p(gt.data_ptr()), ctypes.c_void_p(gb.data_ptr()), ctypes.c_void_p(ga.data_ptr()),
+            _stream())), lambda: None)
         return gt[:self.P], g0, gb, ga
 
     def backward_field(self, grad_e, want_e0=True):
@@ -248,10 +254,9 @@ class DifferentiableLLG:
         gt = torch.empty(MAXP, dtype=torch.float64, device="cuda")
         g0 = torch.empty_like(g) if want_e0 else None
         gB = torch.empty((self._nsteps, 2, 3), dtype=torch.float64, device="cuda")
-        check(self.dm.lib.mx_adjoint_backward_field(self.h, ctypes.c_void_p(g.data_ptr()),
-                                                    ctypes.c_void_p(g0.data_ptr() if g0 is not None else 0),
-                                                    ctypes.c_void_p(gt.data_ptr()), ctypes.c_void_p(gB.data_ptr()),
-                                                    _stream()))
+        self.dm.checked(lambda: check(self.dm.lib.mx_adjoint_backward_field(
+            self.h, ctypes.c_void_p(g.data_ptr()), ctypes.c_void_p(g0.data_ptr() if g0 is not None else 0),
+            ctypes.c_void_p(gt.data_ptr()), ctypes.c_void_p(gB.data_ptr()), _stream())), lambda: None)
         return gt[:self.P], g0, gB
 
     def trajectory(self, theta, e0, nsteps, step0=0, field=None):

@@ -1469,7 +1469,7 @@ bool sweep_persistent(mx_adjoint& a, const Dyn<R>& d, const R* seg, int C, int n
     const AStatic<R>* A = static_cast<const AStatic<R>*>(a.astatic_d);
     StepSync sy{};
     sy.ctl = a.sctl_d;
-    sy.spin = 1u << 16;
+    sy.spin = m.wait_polls;
     uint32_t* gave_up = m.stepsync_d + stepsync_ctl(m) + 2;
     R* lam = static_cast<R*>(a.lam_d);
     double* gf = a.want_field ? a.gf_d + size_t(a.sweep_step0) * 2 * a.nblk * WAVES * 3 : nullptr;

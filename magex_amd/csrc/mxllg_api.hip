@@ -756,6 +756,19 @@ int mx_model_set_noise(mx_model* m, int32_t grade) {
   });
 }
 
+int mx_model_set_wait_limit(mx_model* m, int32_t polls) {
+  return guarded([&]() -> int {
+    need_model(m);
+    if (polls < 0) fail(MX_EINVAL, "the wait limit is a number of polls, 0 or more");
+    if (uint32_t(polls) != m->wait_polls) {
+      use_device(*m);
+      drop_graph(*m);  // a captured step loop bakes the limit into its launches
+    }
+    m->wait_polls = uint32_t(polls);
+    return MX_OK;
+  });
+}
+
 int mx_model_set_step_mode(mx_model* m, int32_t mode) {
   return guarded([&]() -> int {
     need_model(m);

@@ -175,6 +175,9 @@ struct Model {
   // one-launch steps: mode (0: two launches per step; 1: one launch per step where the model allows it;
   // 2, the default: also runs of steps in persistent launches where they pay), control words
   int step_mode = 2;
+  // polls after which a wait of the one-launch, persistent and adjoint-sweep kernels gives up (each
+  // poll is a round of loads and a short sleep)
+  uint32_t wait_polls = 1u << 16;
   uint32_t* stepsync_d = nullptr;  // L0 plane counters, 4 control words, nblk give-up slots
   // persistent launches (k_steps): tagged spin records (2 x N x TAG_WORDS 64-bit words: spins, predictor
   // spins; zeroed when allocated) and the epoch of the last launch on the model (each launch takes the

@@ -672,7 +672,7 @@ static int heun_t(const Model& m, void* e, uint64_t seed, uint64_t step, const v
     sy.gu = reinterpret_cast<int32_t*>(sy.ctl + CTL_WORDS);
     sy.r0 = reach0(m);
     sy.PS = m.L[1] * m.L[2];
-    sy.spin = 1u << 16;
+    sy.spin = m.wait_polls;
     const bool lean = lean_model(m) && !noise;
     if (m.geo == GEO_PERIODIC) step_dispatch<R, GEO_PERIODIC>(m, nb, lean, g, s, S, d, E, ep, k1, step, nz, sy);
     else step_dispatch<R, GEO_OPEN>(m, nb, lean, g, s, S, d, E, ep, k1, step, nz, sy);
@@ -687,7 +687,7 @@ static int heun_t(const Model& m, void* e, uint64_t seed, uint64_t step, const v
       sy.gu = reinterpret_cast<int32_t*>(sy.ctl + CTL_WORDS);
       sy.r0 = reach0(m);
       sy.PS = m.L[1] * m.L[2];
-      sy.spin = 1u << 16;
+      sy.spin = m.wait_polls;
       uint32_t* tick = sync + stepsync_tick(m);
       const bool lean = lean_model(m) && !noise;
       const dim3 gw(G);

@@ -204,7 +204,7 @@ static int steps_t(const Model& m, void* e, uint64_t seed, uint64_t step0, int64
   sy.gu = reinterpret_cast<int32_t*>(sy.ctl + CTL_WORDS);
   sy.r0 = reach0(m);
   sy.PS = m.L[1] * m.L[2];
-  sy.spin = 1u << 16;
+  sy.spin = m.wait_polls;
   static_assert(2 * STEPS_PER_LAUNCH + 2 < (1 << 12), "step tags overflow their 12 bits");
   const bool lean = lean_model(m) && !noise;
   const dim3 g(nblk_of(m));

@@ -159,6 +159,8 @@ class DeviceModel:
             check(L.mx_model_set_pinned(h, hptr(np.ascontiguousarray(system.pinned, np.uint8))))
         self.set_field(system.B)
         self._dyn = None
+        self._mode = 2       # the library's default step mode
+        self._gu_seen = 0    # given-up waits accounted for by checked()
 
     @staticmethod
     def _broadcast_bonds(system):
@@ -220,6 +222,31 @@ class DeviceModel:
         (default): as 1, and runs of two or more steps in persistent launches where they pay; 3: as 2, and
         one walked launch per step for FP32 per-bond grids larger than the GPU holds at once."""
         check(self.lib.mx_model_set_step_mode(self.h, int(mode)))
+        self._mode = int(mode)
+
+    def set_wait_limit(self, polls):
+        """Polls after which a wait of the one-launch, persistent and adjoint-sweep kernels gives up
+        (default 65536; 0: after the first unsuccessful poll), for tests of the give-up paths."""
+        check(self.lib.mx_model_set_wait_limit(self.h, int(polls)))
+
+    def checked(self, work, undo):
+        """work() (device calls on the current stream), done again after undo() with step mode 1 if a
+        wait gave up meanwhile.  A persistent launch that gives up stops early and leaves its spins
+        part-way; one launch per step finishes a given-up step in the launch's last workgroup instead, and
+        every step form gives the same bits, so the repeat is the result work() should have given.
+        Synchronises the device once (twice on a repeat)."""
+        work()
+        gu = self.step_info()[1]
+        if gu != self._gu_seen and self._mode >= 2:  # modes 0 and 1 never stop part-way
+            undo()
+            mode = self._mode
+            self.set_step_mode(min(mode, 1))
+            try:
+                work()
+            finally:
+                self.set_step_mode(mode)
+            gu = self.step_info()[1]
+        self._gu_seen = gu
 
     def step_info(self):
         """(launch form: 0 two per step, 1 one per step, 2 persistent runs, 3 one walked launch per step;

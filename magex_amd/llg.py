@@ -281,7 +281,14 @@ class LLGSolver:
         n = 0
         for m in order:
             if m > n:
-                self.step_device(ed, m - n)
+                # a persistent launch that gave up a wait is run again one launch per step
+                k, s0, t_0, keep = m - n, self.step_count, self.time, ed.clone()
+
+                def undo():
+                    ed.copy_(keep)
+                    self.step_count, self.time = s0, t_0
+
+                dm.checked(lambda: self.step_device(ed, k), undo)
                 n = m
             if n % output_every == 0 or n == steps:
                 o = dm.observables(ed)
