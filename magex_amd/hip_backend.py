@@ -47,7 +47,8 @@ class HipLLG:
             n = steps - done
             if callback and every:
                 n = min(n, every - done % every)
-            self.step_device(e_dev, n)
+            self.sol.T = self.T
+            self.sol.steps_checked(e_dev, n)
             done += n
             if callback and every and done % every == 0:
                 callback(done, e_dev.cpu().numpy().astype(float))

@@ -198,6 +198,20 @@ class LLGSolver:
         self.time += nsteps * self.dt
         return e_dev
 
+    def steps_checked(self, e_dev, nsteps):
+        """step_device(e_dev, nsteps), then, if a persistent launch gave up a wait (its spins part-way),
+        the same steps again from the same spins one launch per step (DeviceModel.checked): the result
+        of an undisturbed run.  Synchronises."""
+        dm = self._sync()
+        s0, t0, keep = self.step_count, self.time, e_dev.clone()
+
+        def undo():
+            e_dev.copy_(keep)
+            self.step_count, self.time = s0, t0
+
+        dm.checked(lambda: self.step_device(e_dev, nsteps), undo)
+        return e_dev
+
     def field(self, e, t=None):
         """B_eff + spin-orbit torque fields + field_func(t) + extra_field(e) (T), zero on pinned sites
         (magex/llg.py:124-136); numpy in -> numpy out, device tensors stay on the device.  t defaults to
@@ -281,14 +295,7 @@ class LLGSolver:
         n = 0
         for m in order:
             if m > n:
-                # a persistent launch that gave up a wait is run again one launch per step
-                k, s0, t_0, keep = m - n, self.step_count, self.time, ed.clone()
-
-                def undo():
-                    ed.copy_(keep)
-                    self.step_count, self.time = s0, t_0
-
-                dm.checked(lambda: self.step_device(ed, k), undo)
+                self.steps_checked(ed, m - n)
                 n = m
             if n % output_every == 0 or n == steps:
                 o = dm.observables(ed)

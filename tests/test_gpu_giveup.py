@@ -61,6 +61,13 @@ def test_persistent_run_that_gives_up_is_run_again(gpu):
     assert res[0][2] > 0
     assert np.array_equal(res[0][0], res[1 << 16][0])
     assert np.array_equal(res[0][1], res[1 << 16][1])
+    # magex's GPU-class shape runs its steps the same way
+    from magex_amd.hip_backend import CudaLLG
+    ref = CudaLLG(_tri(48), 0.05, 5e-4, 20.0, 3, precision="fp32").run(e0, 150)
+    g = CudaLLG(_tri(48), 0.05, 5e-4, 20.0, 3, precision="fp32")
+    g.sol._sync().set_wait_limit(0)
+    assert np.array_equal(g.run(e0, 150), ref)
+    assert g.sol.dm.step_info()[1] > 0
     # the unchecked call reports the stopped launch
     s = _tri(48)
     dm = s.device_model("fp32")
