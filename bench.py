@@ -11,7 +11,11 @@ HBM-roofline case.  Prints one JSON line.
 roofline.achieved = algorithmic bytes per step / measured step time, with the algorithmic bytes of a
 Heun step: 6w (read e, write e) + 6w if the SIA is per site + 9w nnb/2 if bond tensors are per bond
 (each unordered bond stored once), w = 4 (FP32) or 8 (FP64) bytes.  The step time comes from HIP
-events recorded on the stream the kernels run on.
+events recorded on the stream the kernels run on, around one mx_llg_run call of K steps (the solver's
+set-up calls made before it).  Before the W warm-up steps, untimed steps run for SETTLE_S seconds: the
+GPU's clocks reach their sustained level only after some milliseconds of continuous work, and a small
+W (5 steps of bcc64 are 0.3 ms) would otherwise time part of that ramp (measured: 20 timed steps at
+61 us per step after 5 warm-up steps, 58 after 200; 56 in long runs).
 cpu_baseline: the C++/OpenMP restatement of magex's fused CPU Heun (oracle/cpu_llg.cpp), FP64, on a
 bounded sample of the same workload, on this host's cores.
 """
@@ -34,6 +38,7 @@ CPUS_USABLE = len(os.sched_getaffinity(0))
 
 METRIC = "spin-steps/sec + % HBM roofline (FP32 vs FP64); batched speedup; grad cost/forward"
 HBM_PEAK_GBS = 8000.0
+SETTLE_S = 0.05   # untimed work before the warm-up steps (clock settling)
 
 
 def alg_bytes(system, precision):
@@ -72,17 +77,29 @@ def gpu_rate(system, p, precision, steps, warmup):
     sol = LLGSolver(system, p["alpha"], p["dt"], p["temperature"], p["seed"], precision=precision)
     e = torch.as_tensor(system.init_spins("random", seed=0), device="cuda",
                         dtype=torch.float32 if precision == "fp32" else torch.float64).contiguous()
-    sol.step_device(e, warmup)
+    dm = sol._sync()                     # the solver's set-up calls (dynamics, field, noise grade)
+    seed, step, n = sol.noise_seed, 0, 10
+    t_settle = time.perf_counter()
+    while time.perf_counter() - t_settle < SETTLE_S:
+        t = time.perf_counter()
+        dm.run(e, n, seed, step)
+        step += n
+        torch.cuda.synchronize()
+        if time.perf_counter() - t < 2e-3:   # chunks of at least 2 ms of work
+            n *= 2
+    dm.run(e, warmup, seed, step)
+    step += warmup
     torch.cuda.synchronize()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     start.record()
-    sol.step_device(e, steps)
+    dm.run(e, steps, seed, step)
     stop.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ev_ms = start.elapsed_time(stop)
     assert torch.isfinite(e).all(), "non-finite spins"
+    assert dm.step_info()[1] == 0, "a wait of the step kernels gave up"
     return dict(wall_s=wall, event_ms=ev_ms, step_ms=ev_ms / steps)
 
 
@@ -393,19 +410,20 @@ def adjoint_rates(steps=500, checkpoint_every=50, reps=3):
     for r in range(reps + 1):
         torch.cuda.synchronize()
         ev[0].record()
-        e = sol.forward(theta0, e0, steps)
+        e = sol.forward(theta0, e0, steps, checked=False)
         ev[1].record()
         g = 2.0 * (e - tgt) / e.numel()
         torch.cuda.synchronize()
         ev[1].synchronize()
         t_f = ev[0].elapsed_time(ev[1])
         ev[1].record()
-        gt, _ = sol.backward(g)
+        gt, _ = sol.backward(g, checked=False)
         ev[2].record()
         torch.cuda.synchronize()
         if r:                                   # the first round warms up
             fwd.append(t_f)
             bwd.append(ev[1].elapsed_time(ev[2]))
+    assert sol.dm.step_info()[1] == 0, "a wait of the adjoint kernels gave up"
     f, b = min(fwd), min(bwd)
     return {"workload": "square 128^2 (16384 spins), J1 J2 D K learnable, 500 steps alpha=0.3 dt=1 fs T=0, "
                         "checkpoints every 50, MSE loss, FP32",
@@ -448,21 +466,22 @@ def adjoint_bcc64_rates(steps=100, checkpoint_every=50, reps=2):
     for r in range(reps + 1):
         torch.cuda.synchronize()
         ev[0].record()
-        e = sol.forward(theta0, e0, steps)
+        e = sol.forward(theta0, e0, steps, checked=False)
         ev[1].record()
         g = 2.0 * (e - tgt) / e.numel()
         torch.cuda.synchronize()
         t_f = ev[0].elapsed_time(ev[1])
         ev[1].record()
-        gt, _ = sol.backward(g, want_e0=False)
+        gt, _ = sol.backward(g, want_e0=False, checked=False)
         ev[2].record()
-        gt2, _, gb, ga = sol.backward_dense(g, want_e0=False)
+        gt2, _, gb, ga = sol.backward_dense(g, want_e0=False, checked=False)
         ev[3].record()
         torch.cuda.synchronize()
         if r:
             fwd.append(t_f)
             bwd.append(ev[1].elapsed_time(ev[2]))
             dns.append(ev[2].elapsed_time(ev[3]))
+    assert sol.dm.step_info()[1] == 0, "a wait of the adjoint kernels gave up"
     f, b, bd = min(fwd), min(bwd), min(dns)
     return {"workload": "bcc 64^3 (524288 spins), per-bond tensors and per-site SIA fixed, J1 J2 learnable, "
                         "100 steps alpha=0.1 dt=0.1 fs T=300 K, checkpoints every 50, MSE loss, FP32",

@@ -177,10 +177,19 @@ class DifferentiableLLG:
             raise ValueError(f"spins must be ({self.N}, 3)")
         return e.clone()
 
-    def forward(self, theta, e0, nsteps, step0=0, field=None):
+    def _run(self, work, undo, checked):
+        # checked: a persistent launch that gave up a wait is run again one launch per step (synchronises)
+        if checked:
+            self.dm.checked(work, undo)
+        else:
+            work()
+
+    def forward(self, theta, e0, nsteps, step0=0, field=None, checked=True):
         """Spins after nsteps steps from e0 (new device tensor); records checkpoints for backward().
         field: the uniform field (T) of the predictor and corrector pass of each step, (nsteps, 2, 3)
-        (field_sequence builds it from a field_func), or None for the system's own field."""
+        (field_sequence builds it from a field_func), or None for the system's own field.
+        checked=False leaves the work queued without the give-up check (and its synchronisation): the
+        caller reads self.dm.step_info()[1] itself."""
         self.set_params(theta)
         e = self._spins(e0)
         start = e.clone()
@@ -196,7 +205,7 @@ class DifferentiableLLG:
                                                   self.seed, int(step0), _stream()))
 
         # a persistent launch that gave up a wait (its spins part-way) is run again one launch per step
-        self.dm.checked(work, lambda: e.copy_(start))
+        self._run(work, lambda: e.copy_(start), checked)
         self._nsteps = int(nsteps)
         self._token += 1
         return e
@@ -211,7 +220,7 @@ class DifferentiableLLG:
                 out[n, k] = self.system.B + np.asarray(field_func(tk), float).reshape(3)
         return out
 
-    def backward(self, grad_e, want_e0=True):
+    def backward(self, grad_e, want_e0=True, checked=True):
         """(dL/dtheta (P,) float64 device tensor, dL/de0 or None) for the last forward().
 
         Runs on torch's current stream; on grids the GPU holds twice over at once, each checkpoint segment
@@ -220,9 +229,9 @@ class DifferentiableLLG:
         g = grad_e.detach().to(device="cuda", dtype=self.dtype).contiguous()
         gt = torch.empty(MAXP, dtype=torch.float64, device="cuda")
         g0 = torch.empty_like(g) if want_e0 else None
-        self.dm.checked(lambda: check(self.dm.lib.mx_adjoint_backward(
+        self._run(lambda: check(self.dm.lib.mx_adjoint_backward(
             self.h, ctypes.c_void_p(g.data_ptr()), ctypes.c_void_p(g0.data_ptr() if g0 is not None else 0),
-            ctypes.c_void_p(gt.data_ptr()), _stream())), lambda: None)
+            ctypes.c_void_p(gt.data_ptr()), _stream())), lambda: None, checked)
         return gt[:self.P], g0
 
     @property
@@ -230,7 +239,7 @@ class DifferentiableLLG:
         """Unordered bonds per cell: the forward stencil entries (SpinSystem.forward_entries)."""
         return int(self.dm.lib.mx_adjoint_num_bonds(self.h))
 
-    def backward_dense(self, grad_e, want_e0=True):
+    def backward_dense(self, grad_e, want_e0=True, checked=True):
         """(dL/dtheta, dL/de0 or None, dL/dT (nbonds, ncell, 3, 3), dL/dA (N, 3, 3)) for the last forward().
 
         T is each unordered bond's tensor in the layout of SpinSystem.set_bond_tensors (it acts as J_ij = T
@@ -241,22 +250,22 @@ class DifferentiableLLG:
         g0 = torch.empty_like(g) if want_e0 else None
         gb = torch.empty((self.nbonds, self.system.ncell, 3, 3), dtype=torch.float64, device="cuda")
         ga = torch.empty((self.N, 3, 3), dtype=torch.float64, device="cuda")
-        self.dm.checked(lambda: check(self.dm.lib.mx_adjoint_backward_dense(
+        self._run(lambda: check(self.dm.lib.mx_adjoint_backward_dense(
             self.h, ctypes.c_void_p(g.data_ptr()), ctypes.c_void_p(g0.data_ptr() if g0 is not None else 0),
             ctypes.c_void_p(gt.data_ptr()), ctypes.c_void_p(gb.data_ptr()), ctypes.c_void_p(ga.data_ptr()),
-            _stream())), lambda: None)
+            _stream())), lambda: None, checked)
         return gt[:self.P], g0, gb, ga
 
-    def backward_field(self, grad_e, want_e0=True):
+    def backward_field(self, grad_e, want_e0=True, checked=True):
         """(dL/dtheta, dL/de0 or None, dL/dB (nsteps, 2, 3)) for the last forward(): dL/dB is the gradient
         with respect to the uniform field of each pass of each step (driven or not), float64 device tensor."""
         g = grad_e.detach().to(device="cuda", dtype=self.dtype).contiguous()
         gt = torch.empty(MAXP, dtype=torch.float64, device="cuda")
         g0 = torch.empty_like(g) if want_e0 else None
         gB = torch.empty((self._nsteps, 2, 3), dtype=torch.float64, device="cuda")
-        self.dm.checked(lambda: check(self.dm.lib.mx_adjoint_backward_field(
+        self._run(lambda: check(self.dm.lib.mx_adjoint_backward_field(
             self.h, ctypes.c_void_p(g.data_ptr()), ctypes.c_void_p(g0.data_ptr() if g0 is not None else 0),
-            ctypes.c_void_p(gt.data_ptr()), ctypes.c_void_p(gB.data_ptr()), _stream())), lambda: None)
+            ctypes.c_void_p(gt.data_ptr()), ctypes.c_void_p(gB.data_ptr()), _stream())), lambda: None, checked)
         return gt[:self.P], g0, gB
 
     def trajectory(self, theta, e0, nsteps, step0=0, field=None):

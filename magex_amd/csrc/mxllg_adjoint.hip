@@ -1309,7 +1309,26 @@ void apply_theta(mx_adjoint& a) {
       for (int c = 0; c < m.ncell; ++c)
         for (int q = 0; q < 9; ++q) J[(size_t(f) * m.ncell + c) * 9 + q] += u[q];
     }
+    const std::vector<int> before = m.ent_order;
     aok_rc(mx_model_set_bond_tensors(a.m, J.data()), "per-bond tensors");
+    if (m.ent_order != before) {
+      // the per-bond form pairs each same-sublattice bond's two entries (on its first upload): the units
+      // and forward slots, held by entry position, move with their entries
+      std::vector<int> old_pos(m.nent);
+      for (int p = 0; p < m.nent; ++p) old_pos[before[p]] = p;
+      std::vector<double> U(a.U.size());
+      std::vector<int> fs(m.nent);
+      for (int p = 0; p < m.nent; ++p) {
+        const int o = old_pos[m.ent_order[p]];
+        fs[p] = a.fslot[o];
+        for (int r = 0; r < a.P; ++r)
+          for (int q = 0; q < 9; ++q) U[(size_t(r) * m.nent + p) * 9 + q] = a.U[(size_t(r) * m.nent + o) * 9 + q];
+      }
+      a.U.swap(U);
+      a.fslot.swap(fs);
+      if (m.prec == MX_FP64) upload_astatic<double>(a);
+      else upload_astatic<float>(a);
+    }
   }
   if (a.aoff.empty()) {
     m.amode = ONSITE_SUB;

@@ -369,7 +369,8 @@ int mx_model_set_stencil(mx_model* m, int32_t nent, const int32_t* ent, int32_t 
     // group entries by source sublattice; inside a group, bonds to other sublattices come first, ordered
     // by their pair (the entry and its reverse partner share a key), so the two workgroups that read a
     // per-bond tensor from either end reach it at the same point of their loops and the second read
-    // finds it in L2; same-sublattice bonds follow in the caller's order
+    // finds it in L2; same-sublattice bonds follow in the caller's order (mx_model_set_bond_tensors
+    // pairs them as well)
     std::vector<int> pair_key(nent);
     for (int k = 0; k < nent; ++k) {
       pair_key[k] = k;
@@ -452,6 +453,27 @@ int mx_model_set_bond_tensors(mx_model* m, const double* J) {
             raw(q, 4) == -raw(k, 4))
           partner[k] = q;
       if (partner[k] < 0) fail(MX_EINVAL, "stencil entry without a reverse partner: per-bond storage needs J_ji");
+    }
+    // per-bond tensors: a same-sublattice bond's two entries side by side (in the order of the pair's
+    // first entry), so that its owner and its partner read the tensor in the same batch -- FP64 bcc 64^3
+    // 815.9 -> 768.2 MB per Heun step.  Table couplings keep the caller's order there: paired, the
+    // persistent FP32 table steps measured slower (triangular 256^2 8.80 -> 9.11 us per step).
+    for (int s = 0; s < m->ns; ++s) {
+      std::vector<int> pos;
+      for (int p = m->sptr[s]; p < m->sptr[s + 1]; ++p)
+        if (raw(m->ent_order[p], 1) == s) pos.push_back(p);
+      std::vector<int> ks;
+      for (int p : pos) ks.push_back(m->ent_order[p]);
+      std::stable_sort(ks.begin(), ks.end(),
+                       [&](int a, int b) { return std::min(a, partner[a]) < std::min(b, partner[b]); });
+      std::vector<Ent> es;
+      for (int k : ks)
+        for (int p : pos)
+          if (m->ent_order[p] == k) es.push_back(m->ent[p]);
+      for (size_t q = 0; q < pos.size(); ++q) {
+        m->ent_order[pos[q]] = ks[q];
+        m->ent[pos[q]] = es[q];
+      }
     }
     // entries (internal order) point at a forward slot: their own, or the partner's (back)
     for (size_t pos = 0; pos < m->ent.size(); ++pos) {
